@@ -1,0 +1,91 @@
+"""ctypes binding of the engine's C-ABI (include/gfnff_mi355x.h)."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libgfnff_mi355x.so"
+
+NTERMS = 15
+TERM_NAMES = ["bond", "angle", "torsion", "inversion", "dispersion", "bonded_rep", "nonbonded_rep", "coulomb",
+              "hbond", "xbond", "atm", "batm", "stors", "reserved13", "reserved14"]
+
+STATUS = {0: "ok", 1: "args", 2: "unsupported", 3: "eeq", 4: "geometry", 5: "device", 6: "nan"}
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+
+class ParamsView(C.Structure):
+    """Mirror of gfnff_params_view (include/gfnff_params.h)."""
+
+    _fields_ = [
+        ("natoms", C.c_int32), ("nfrag", C.c_int32), ("nelem", C.c_int32), ("total_charge", C.c_int32),
+        ("Z", _i32p), ("elem", _i32p), ("frag", _i32p),
+        ("chi_base", _f64p), ("gam_eff", _f64p), ("alpeeq", _f64p), ("cnf", _f64p), ("rcov_cn", _f64p),
+        ("rcov_d3", _f64p), ("hb_rad", _f64p), ("metal", _i32p), ("qa", _f64p), ("zeta", _f64p),
+        ("sqrt_zr4r2", _f64p), ("rep_anb", _f64p), ("repz", _f64p), ("qfrag", _f64p),
+        ("present_Z", _i32p), ("nref", _i32p), ("refcn", _f64p), ("c6ref", _f64p),
+        ("topo_off", _i32p), ("topo_idx", _i32p), ("topo_dist", _i32p),
+        ("nbond", C.c_int32), ("bond_ij", _i32p), ("bond_par", _f64p), ("bond_hbH", _i32p),
+        ("nhbcn", C.c_int32), ("hbcn_pair", _i32p),
+        ("nangle", C.c_int32), ("angle_ijk", _i32p), ("angle_par", _f64p),
+        ("ntors", C.c_int32), ("tors_ijkl", _i32p), ("tors_par", _f64p),
+        ("ninv", C.c_int32), ("inv_ijkl", _i32p), ("inv_par", _f64p),
+        ("nbrep", C.c_int32), ("brep_ij", _i32p), ("brep_par", _f64p),
+        ("nhb", C.c_int32), ("hb_ahb", _i32p), ("hb_case", _i32p), ("hb_par", _f64p),
+        ("hb_nbB_off", _i32p), ("hb_nbB_idx", _i32p), ("hb_parentC", _i32p), ("hb_nbC_off", _i32p),
+        ("hb_nbC_idx", _i32p),
+        ("natm", C.c_int32), ("atm_ijk", _i32p), ("atm_scale", _f64p),
+        ("nbatm", C.c_int32), ("batm_ijk", _i32p), ("batm_par", _f64p),
+    ]
+
+
+_LIB = None
+
+
+def load_library(path: Path | None = None) -> C.CDLL:
+    """Load the engine library; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"engine library missing: {p} (run __graft_entry__.build())")
+    lib = C.CDLL(str(p))
+    lib.gfnff_create.restype = C.c_void_p
+    lib.gfnff_destroy.argtypes = [C.c_void_p]
+    lib.gfnff_set_molecule.argtypes = [C.c_void_p, C.c_int32, _i32p, _f64p, C.c_int32, _i32p, C.c_int32]
+    lib.gfnff_update_geometry.argtypes = [C.c_void_p, _f64p]
+    lib.gfnff_energy_gradient.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p]
+    lib.gfnff_energy_gradient_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.gfnff_charges.argtypes = [C.c_void_p, _f64p]
+    lib.gfnff_cn.argtypes = [C.c_void_p, _f64p]
+    lib.gfnff_set_param.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
+    lib.gfnff_set_precision.argtypes = [C.c_void_p, C.c_int32]
+    lib.gfnff_has_error.argtypes = [C.c_void_p]
+    lib.gfnff_last_error.argtypes = [C.c_void_p]
+    lib.gfnff_last_error.restype = C.c_char_p
+    lib.gfnff_get_params.argtypes = [C.c_void_p, C.POINTER(ParamsView)]
+    lib.gfnff_get_topology.argtypes = [C.c_void_p, _i32p, _i32p, _i32p, C.c_int32, _i32p]
+    lib.gfnff_device_count.restype = C.c_int
+    _LIB = lib
+    return lib
+
+
+def as_i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def as_f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def iptr(a: np.ndarray):
+    return a.ctypes.data_as(_i32p)
+
+
+def fptr(a: np.ndarray):
+    return a.ctypes.data_as(_f64p)

@@ -1,0 +1,214 @@
+// C-ABI of the engine (include/gfnff_mi355x.h): host handle, set-up on the
+// host, evaluation on the GPU through the device engine (device/engine.hip).
+//
+// Error semantics follow the reference's adapter: a failed EEQ set-up or a
+// non-finite energy marks the handle as failed (qm_methods/gfnff_method.cpp:
+// 36-48, 83-91; energycalculator.cpp:455-480); nothing throws across the ABI.
+
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "gfnff_mi355x.h"
+#include "host/gfnff_topology.hpp"
+#include "device/engine.hpp"
+
+struct gfnff_handle {
+    gfnff::Options opt;
+    gfnff::ParamSet params;
+    bool have_molecule = false;
+    std::vector<double> xyz_bohr;     // current geometry, Bohr
+    std::vector<double> last_q, last_cn;
+    std::string err;
+    bool failed = false;
+    int precision = GFNFF_PREC_FP64;
+    std::unique_ptr<gfnff::DeviceEngine> dev;
+    bool dev_geom_dirty = true;
+};
+
+namespace {
+int fail(gfnff_handle* h, int code, const std::string& msg) {
+    if (h) {
+        h->err = msg;
+        h->failed = true;
+    }
+    return code;
+}
+}  // namespace
+
+extern "C" {
+
+gfnff_handle* gfnff_create(void) { return new (std::nothrow) gfnff_handle(); }
+
+void gfnff_destroy(gfnff_handle* h) {
+    try {
+        delete h;
+    } catch (...) {
+    }
+}
+
+int gfnff_set_molecule(gfnff_handle* h, int32_t natoms, const int32_t* Z, const double* xyz_ang, int32_t charge,
+                       const int32_t* forced_bonds, int32_t nbonds) {
+    if (!h) return GFNFF_STATUS_ARGS;
+    if (natoms <= 0 || !Z || !xyz_ang || nbonds < 0 || (nbonds > 0 && !forced_bonds))
+        return fail(h, GFNFF_STATUS_ARGS, "gfnff_set_molecule: invalid arguments");
+    try {
+        h->failed = false;
+        h->err.clear();
+        h->have_molecule = false;
+        h->dev.reset();
+        h->xyz_bohr.resize((size_t)natoms * 3);
+        for (size_t i = 0; i < h->xyz_bohr.size(); ++i) h->xyz_bohr[i] = xyz_ang[i] * gfnff::kAngToBohrGeom;
+        std::string msg;
+        int st = gfnff::build_params(natoms, Z, h->xyz_bohr.data(), charge, forced_bonds, nbonds, h->opt, h->params,
+                                     msg);
+        if (st != gfnff::kOk) return fail(h, st, "gfnff_set_molecule: " + msg);
+        h->have_molecule = true;
+        h->dev_geom_dirty = true;
+        h->last_q.assign(natoms, 0.0);
+        h->last_cn.assign(natoms, 0.0);
+        return GFNFF_STATUS_OK;
+    } catch (const std::bad_alloc&) {
+        return fail(h, GFNFF_STATUS_ARGS, "gfnff_set_molecule: out of host memory");
+    } catch (const std::exception& e) {
+        return fail(h, GFNFF_STATUS_ARGS, std::string("gfnff_set_molecule: ") + e.what());
+    }
+}
+
+int gfnff_update_geometry(gfnff_handle* h, const double* xyz_ang) {
+    if (!h) return GFNFF_STATUS_ARGS;
+    if (!h->have_molecule || !xyz_ang) return fail(h, GFNFF_STATUS_ARGS, "gfnff_update_geometry: no molecule");
+    for (size_t i = 0; i < h->xyz_bohr.size(); ++i) {
+        if (!std::isfinite(xyz_ang[i])) return fail(h, GFNFF_STATUS_GEOMETRY, "gfnff_update_geometry: non-finite");
+        h->xyz_bohr[i] = xyz_ang[i] * gfnff::kAngToBohrGeom;
+    }
+    h->dev_geom_dirty = true;
+    return GFNFF_STATUS_OK;
+}
+
+static int ensure_device(gfnff_handle* h) {
+    if (h->dev) return GFNFF_STATUS_OK;
+    std::string msg;
+    int st = GFNFF_STATUS_OK;
+    h->dev.reset(gfnff::device_create(h->params, h->precision, st, msg));
+    if (!h->dev) return fail(h, st ? st : GFNFF_STATUS_DEVICE, "device set-up: " + msg);
+    return GFNFF_STATUS_OK;
+}
+
+int gfnff_energy_gradient(gfnff_handle* h, int32_t want_grad, double* energy, double* grad, double* terms) {
+    if (!h) return GFNFF_STATUS_ARGS;
+    if (!h->have_molecule) return fail(h, GFNFF_STATUS_ARGS, "gfnff_energy_gradient: no molecule");
+    if (!energy) return fail(h, GFNFF_STATUS_ARGS, "gfnff_energy_gradient: energy pointer is NULL");
+    try {
+        int st = ensure_device(h);
+        if (st) return st;
+        std::string msg;
+        double e_terms[GFNFF_NTERMS + 1];
+        std::vector<double> g(want_grad ? h->xyz_bohr.size() : 0);
+        st = gfnff::device_eval_host(h->dev.get(), h->xyz_bohr.data(), want_grad != 0, e_terms,
+                                     want_grad ? g.data() : nullptr, h->last_q.data(), h->last_cn.data(), msg);
+        if (st) return fail(h, st, "gfnff_energy_gradient: " + msg);
+        h->failed = false;
+        if (!std::isfinite(e_terms[0])) {
+            *energy = 0.0;
+            return fail(h, GFNFF_STATUS_NAN, "gfnff_energy_gradient: energy is not finite");
+        }
+        *energy = e_terms[0];
+        if (terms) std::memcpy(terms, e_terms + 1, sizeof(double) * GFNFF_NTERMS);
+        if (want_grad && grad) {
+            bool finite = true;
+            for (double v : g) finite = finite && std::isfinite(v);
+            std::memcpy(grad, g.data(), sizeof(double) * g.size());
+            if (!finite) return fail(h, GFNFF_STATUS_NAN, "gfnff_energy_gradient: gradient is not finite");
+        }
+        return GFNFF_STATUS_OK;
+    } catch (const std::exception& e) {
+        return fail(h, GFNFF_STATUS_DEVICE, std::string("gfnff_energy_gradient: ") + e.what());
+    }
+}
+
+int gfnff_energy_gradient_device(gfnff_handle* h, const double* xyz_bohr_dev, double* energy_dev, double* grad_dev,
+                                 void* stream) {
+    if (!h) return GFNFF_STATUS_ARGS;
+    if (!h->have_molecule || !xyz_bohr_dev || !energy_dev)
+        return fail(h, GFNFF_STATUS_ARGS, "gfnff_energy_gradient_device: invalid arguments");
+    try {
+        int st = ensure_device(h);
+        if (st) return st;
+        std::string msg;
+        st = gfnff::device_eval(h->dev.get(), xyz_bohr_dev, energy_dev, grad_dev, stream, msg);
+        if (st) return fail(h, st, "gfnff_energy_gradient_device: " + msg);
+        return GFNFF_STATUS_OK;
+    } catch (const std::exception& e) {
+        return fail(h, GFNFF_STATUS_DEVICE, std::string("gfnff_energy_gradient_device: ") + e.what());
+    }
+}
+
+int gfnff_charges(gfnff_handle* h, double* q) {
+    if (!h || !q || !h->have_molecule) return GFNFF_STATUS_ARGS;
+    std::memcpy(q, h->last_q.data(), sizeof(double) * h->last_q.size());
+    return GFNFF_STATUS_OK;
+}
+
+int gfnff_cn(gfnff_handle* h, double* cn) {
+    if (!h || !cn || !h->have_molecule) return GFNFF_STATUS_ARGS;
+    std::memcpy(cn, h->last_cn.data(), sizeof(double) * h->last_cn.size());
+    return GFNFF_STATUS_OK;
+}
+
+int gfnff_set_param(gfnff_handle* h, const char* key, double value) {
+    if (!h || !key) return GFNFF_STATUS_ARGS;
+    std::string k(key);
+    if (k == "cn_cutoff_bohr") {
+        if (value != 6.0) return fail(h, GFNFF_STATUS_UNSUPPORTED, "cn_cutoff_bohr: only the default 6.0 is built");
+        h->opt.cn_cutoff_bohr = value;
+    } else if (k == "hb_accuracy") {
+        if (!(value > 0.0)) return fail(h, GFNFF_STATUS_ARGS, "hb_accuracy must be > 0");
+        h->opt.hb_accuracy = value;
+    } else if (k == "threads") {
+        h->opt.threads = value < 1 ? 1 : (int)value;
+    } else {
+        return fail(h, GFNFF_STATUS_UNSUPPORTED, "unknown parameter: " + k);
+    }
+    return GFNFF_STATUS_OK;
+}
+
+int gfnff_set_precision(gfnff_handle* h, int32_t mode) {
+    if (!h) return GFNFF_STATUS_ARGS;
+    if (mode != GFNFF_PREC_FP64 && mode != GFNFF_PREC_MIXED) return fail(h, GFNFF_STATUS_ARGS, "unknown precision");
+    if (mode != h->precision) {
+        h->precision = mode;
+        h->dev.reset();
+    }
+    return GFNFF_STATUS_OK;
+}
+
+int gfnff_has_error(const gfnff_handle* h) { return h ? (h->failed ? 1 : 0) : 1; }
+
+const char* gfnff_last_error(const gfnff_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int gfnff_get_params(const gfnff_handle* h, gfnff_params_view* out) {
+    if (!h || !out || !h->have_molecule) return GFNFF_STATUS_ARGS;
+    *out = h->params.view();
+    return GFNFF_STATUS_OK;
+}
+
+int gfnff_get_topology(const gfnff_handle* h, int32_t* hyb, int32_t* pi, int32_t* bonds, int32_t nbonds_max,
+                       int32_t* nbonds) {
+    if (!h || !h->have_molecule) return GFNFF_STATUS_ARGS;
+    const auto& P = h->params;
+    if (hyb) std::memcpy(hyb, P.hyb.data(), sizeof(int32_t) * P.n);
+    if (pi) std::memcpy(pi, P.pi_frag.data(), sizeof(int32_t) * P.n);
+    int nb = (int)P.topo_bonds.size() / 2;
+    if (nbonds) *nbonds = nb;
+    if (bonds) std::memcpy(bonds, P.topo_bonds.data(), sizeof(int32_t) * 2 * std::min(nb, (int)nbonds_max));
+    return GFNFF_STATUS_OK;
+}
+
+int gfnff_device_count(void) { return gfnff::device_count(); }
+
+}  // extern "C"

@@ -1,0 +1,37 @@
+// Device engine: owns the HBM-resident parameter set and runs one GFN-FF
+// energy+gradient evaluation as a sequence of HIP kernels on one stream.
+#pragma once
+
+#include <memory>
+#include <string>
+
+#include "../host/gfnff_topology.hpp"
+
+namespace gfnff {
+
+struct DeviceEngine;
+
+// Uploads the parameter set; returns nullptr and sets status/msg on failure.
+DeviceEngine* device_create(const ParamSet& P, int precision, int& status, std::string& msg);
+void device_destroy(DeviceEngine* e);
+
+// Host-buffer evaluation: uploads xyz (Bohr), evaluates, downloads
+// e_terms[0] = total, e_terms[1..] = per-term energies, gradient, q and CN.
+int device_eval_host(DeviceEngine* e, const double* xyz_bohr, bool want_grad, double* e_terms, double* grad,
+                     double* q, double* cn, std::string& msg);
+
+// Device-resident evaluation, asynchronous on `stream`.
+int device_eval(DeviceEngine* e, const double* xyz_bohr_dev, double* energy_dev, double* grad_dev, void* stream,
+                std::string& msg);
+
+int device_count();
+
+}  // namespace gfnff
+
+// unique_ptr support
+namespace std {
+template <>
+struct default_delete<gfnff::DeviceEngine> {
+    void operator()(gfnff::DeviceEngine* e) const { gfnff::device_destroy(e); }
+};
+}  // namespace std

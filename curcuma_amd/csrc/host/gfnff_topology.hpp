@@ -1,0 +1,89 @@
+// Host-side GFN-FF topology and parameter generation (built once per molecule).
+//
+// This is the engine's counterpart of the reference's initialisation path
+// (src/core/energy_calculators/ff_methods/gfnff_method.cpp:709-850,
+// calculateTopologyInfo :8813-9556, generateGFNFFParameterSet :3557-3747).
+// It produces the flat structure-of-arrays parameter set that the device
+// kernels and the CPU oracle consume; nothing here runs per evaluation.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gfnff_params.h"
+
+namespace gfnff {
+
+// Unit constants.  The reference uses three different Angstrom/Bohr factors and
+// each one is used where the reference uses it:
+//   geometry input     1/0.529177210903   (src/core/units.h:81-82)
+//   CN covalent radii  1.8897259886       (ff_methods/cn_calculator.cpp:375)
+//   GFN-FF tables      1/0.52917726       (ff_methods/gfnff_par.h:276)
+constexpr double kAngToBohrGeom = 1.0 / 0.529177210903;
+constexpr double kAngToBohrCN = 1.8897259886;
+constexpr double kAngToBohrTab = 1.0 / 0.52917726;
+
+struct Options {
+    double cn_cutoff_bohr = 6.0;    // per-evaluation CN neighbour cutoff
+    double hb_accuracy = 0.1;       // HB list thresholds hbthr1/hbthr2
+    int threads = 1;                // host threads for the Phase-1 solve
+};
+
+// Error codes shared with the C-ABI.
+enum Status : int {
+    kOk = 0,
+    kErrArgs = 1,
+    kErrUnsupported = 2,
+    kErrEEQ = 3,
+    kErrGeometry = 4,
+    kErrDevice = 5,
+    kErrNaN = 6,
+};
+
+// Owns every array the gfnff_params_view points into.
+struct ParamSet {
+    int n = 0, nfrag = 0, nelem = 0, charge = 0;
+    std::vector<int32_t> Z, elem, frag, present_Z, nref, metal;
+    std::vector<double> chi_base, gam_eff, alpeeq, cnf, rcov_cn, rcov_d3, hb_rad, qa, zeta, sqrt_zr4r2,
+        rep_anb, repz, qfrag, refcn, c6ref;
+    std::vector<int32_t> topo_off, topo_idx, topo_dist;
+
+    std::vector<int32_t> bond_ij, bond_hbH, hbcn_pair;
+    std::vector<double> bond_par;
+    std::vector<int32_t> angle_ijk;
+    std::vector<double> angle_par;
+    std::vector<int32_t> tors_ijkl;
+    std::vector<double> tors_par;
+    std::vector<int32_t> inv_ijkl;
+    std::vector<double> inv_par;
+    std::vector<int32_t> brep_ij;
+    std::vector<double> brep_par;
+    std::vector<int32_t> hb_ahb, hb_case, hb_nbB_off, hb_nbB_idx, hb_parentC, hb_nbC_off, hb_nbC_idx;
+    std::vector<double> hb_par;
+    std::vector<int32_t> atm_ijk;
+    std::vector<double> atm_scale;
+    std::vector<int32_t> batm_ijk;
+    std::vector<double> batm_par;
+
+    // Diagnostics kept from the topology build (not used per evaluation).
+    std::vector<int32_t> hyb, pi_frag, topo_bonds;   // topo_bonds: [2*nb] detected bonds
+    std::vector<double> cn_topo;
+
+    gfnff_params_view view() const;
+};
+
+// Builds the parameter set for one molecule.  xyz_bohr is N x 3 row-major.
+// forced_bonds (pairs, 0-based) replaces geometric bond detection when nonempty
+// (reference: gfnff_method.cpp:954-963).  Returns a Status; msg gets a reason.
+int build_params(int n, const int32_t* Z, const double* xyz_bohr, int charge,
+                 const int32_t* forced_bonds, int nforced, const Options& opt, ParamSet& out,
+                 std::string& msg);
+
+// Dense symmetric solve of the bordered EEQ system used at set-up
+// (A q + C^T lambda = x, C q = qfrag).  A is n x n row-major (overwritten).
+bool eeq_solve_host(int n, int nfrag, std::vector<double>& A, const std::vector<double>& x,
+                    const std::vector<int32_t>& frag, const std::vector<double>& qfrag,
+                    std::vector<double>& q, int threads);
+
+}  // namespace gfnff

@@ -1,0 +1,152 @@
+"""Python mirror of the reference's ``ComputationalMethod`` interface for GFN-FF.
+
+The reference's plug-in boundary is ``ComputationalMethod``
+(src/core/energy_calculators/computational_method.h:43-360) with the GFN-FF
+adapter ``GFNFFComputationalMethod`` (qm_methods/gfnff_method.{h,cpp}).  This
+class keeps the same method names, argument meaning and error behaviour
+(errors are reported through ``hasError()`` / ``getErrorMessage()``, nothing
+raises out of ``calculateEnergy``), and forwards to the C-ABI of the HIP
+engine.  It never falls back to a CPU implementation: without a GPU,
+``calculateEnergy`` reports a device error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import NTERMS, TERM_NAMES, as_f64, as_i32, fptr, iptr
+
+ANGSTROM_TO_BOHR = 1.0 / 0.529177210903  # src/core/units.h:81-82
+
+
+@dataclass
+class Mol:
+    """Subset of the reference's ``Mol`` (src/core/global.h:69-99)."""
+
+    atoms: np.ndarray                      # Z, shape (N,)
+    geometry: np.ndarray                   # Angstrom, shape (N, 3), row-major
+    charge: int = 0
+    bonds: list = field(default_factory=list)   # forced bonds (0-based pairs)
+
+    @property
+    def number_atoms(self) -> int:
+        return int(len(self.atoms))
+
+
+class GFNFFMethod:
+    """``-method gfnff`` drop-in: same Mol in, energy and N x 3 gradient out."""
+
+    def __init__(self, parameters: dict | None = None, precision: str = "fp64"):
+        self._lib = _lib.load_library()
+        self._h = self._lib.gfnff_create()
+        if not self._h:
+            raise MemoryError("gfnff_create failed")
+        self._n = 0
+        self._energy = 0.0
+        self._grad = np.zeros((0, 3))
+        self._terms = np.zeros(NTERMS)
+        self._error = ""
+        if parameters:
+            self.setParameters(parameters)
+        self.setPrecision(precision)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.gfnff_destroy(h)
+            self._h = None
+
+    # --- ComputationalMethod interface -------------------------------------------------
+    def setMolecule(self, mol: Mol) -> bool:
+        Z = as_i32(mol.atoms)
+        xyz = as_f64(np.asarray(mol.geometry).reshape(-1, 3))
+        fb = as_i32(np.asarray(mol.bonds, dtype=np.int32).reshape(-1, 2)) if len(mol.bonds) else as_i32(np.zeros((0, 2)))
+        st = self._lib.gfnff_set_molecule(self._h, len(Z), iptr(Z), fptr(xyz), int(mol.charge),
+                                          iptr(fb) if len(fb) else None, len(fb))
+        self._n = len(Z) if st == 0 else 0
+        self._grad = np.zeros((self._n, 3))
+        return st == 0
+
+    def updateGeometry(self, geometry_angstrom) -> bool:
+        xyz = as_f64(np.asarray(geometry_angstrom).reshape(-1, 3))
+        if xyz.shape[0] != self._n:
+            self._error = "updateGeometry: atom count mismatch"
+            return False
+        return self._lib.gfnff_update_geometry(self._h, fptr(xyz)) == 0
+
+    def calculateEnergy(self, gradient: bool = True) -> float:
+        e = C.c_double(0.0)
+        g = np.zeros((self._n, 3))
+        t = np.zeros(NTERMS)
+        st = self._lib.gfnff_energy_gradient(self._h, 1 if gradient else 0, C.byref(e), fptr(g), fptr(t))
+        self._energy = float(e.value) if st in (0, 6) else 0.0
+        self._grad = g
+        self._terms = t
+        return self._energy
+
+    def getGradient(self) -> np.ndarray:
+        return self._grad.copy()
+
+    def copyGradientTo(self, out: np.ndarray) -> None:
+        out[...] = self._grad
+
+    def getCharges(self) -> np.ndarray:
+        q = np.zeros(self._n)
+        self._lib.gfnff_charges(self._h, fptr(q))
+        return q
+
+    def getCN(self) -> np.ndarray:
+        cn = np.zeros(self._n)
+        self._lib.gfnff_cn(self._h, fptr(cn))
+        return cn
+
+    def getEnergyDecomposition(self) -> dict:
+        d = {name: float(v) for name, v in zip(TERM_NAMES, self._terms) if not name.startswith("reserved")}
+        d["repulsion"] = d["bonded_rep"] + d["nonbonded_rep"]
+        d["total"] = self._energy
+        return d
+
+    def hasError(self) -> bool:
+        return bool(self._lib.gfnff_has_error(self._h))
+
+    def getErrorMessage(self) -> str:
+        return self._lib.gfnff_last_error(self._h).decode()
+
+    def setParameters(self, parameters: dict) -> None:
+        for k, v in parameters.items():
+            self._lib.gfnff_set_param(self._h, k.encode(), float(v))
+
+    def setPrecision(self, precision: str) -> None:
+        mode = {"fp64": 0, "mixed": 1}[precision]
+        self._lib.gfnff_set_precision(self._h, mode)
+
+    def isThreadSafe(self) -> bool:
+        return False
+
+    # --- engine extensions ------------------------------------------------------------
+    def params_view(self) -> _lib.ParamsView:
+        v = _lib.ParamsView()
+        st = self._lib.gfnff_get_params(self._h, C.byref(v))
+        if st != 0:
+            raise RuntimeError("no molecule set")
+        return v
+
+    def topology(self):
+        hyb = np.zeros(self._n, dtype=np.int32)
+        pi = np.zeros(self._n, dtype=np.int32)
+        nb = C.c_int32(0)
+        self._lib.gfnff_get_topology(self._h, iptr(hyb), iptr(pi), None, 0, C.byref(nb))
+        bonds = np.zeros((nb.value, 2), dtype=np.int32)
+        self._lib.gfnff_get_topology(self._h, None, None, iptr(bonds), nb.value, C.byref(nb))
+        return hyb, pi, bonds
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def natoms(self) -> int:
+        return self._n

@@ -1,0 +1,106 @@
+/*
+ * gfnff_mi355x.h -- C-ABI of the MI355X GFN-FF energy/gradient engine.
+ *
+ * The entry points replace the reference's ComputationalMethod plug-in for
+ * "-method gfnff" (src/core/energy_calculators/computational_method.h:43-360,
+ * created by MethodFactory::create, method_factory.cpp:460-516, CPU adapter
+ * qm_methods/gfnff_method.{h,cpp}).  Plain pointers and sizes only; no C++ or
+ * torch types cross this boundary and nothing throws across it.  Every
+ * function returns a status (0 = ok, see GFNFF_STATUS_*); the message of the
+ * last failure is available from gfnff_last_error().
+ *
+ * Units follow the reference: coordinates in Angstrom on input (converted with
+ * 1/0.529177210903, gfnff_method.cpp:757), energy in Hartree, gradient in
+ * Hartree/Bohr (no conversion, gfnff_method.cpp:1831-1844).
+ *
+ * Threading: one handle is used by one host thread at a time (the reference's
+ * GFNFFComputationalMethod::isThreadSafe() returns false,
+ * qm_methods/gfnff_method.h:53).
+ */
+#ifndef GFNFF_MI355X_H
+#define GFNFF_MI355X_H
+
+#include <stdint.h>
+
+#include "gfnff_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GFNFF_STATUS_OK 0
+#define GFNFF_STATUS_ARGS 1
+#define GFNFF_STATUS_UNSUPPORTED 2
+#define GFNFF_STATUS_EEQ 3
+#define GFNFF_STATUS_GEOMETRY 4
+#define GFNFF_STATUS_DEVICE 5
+#define GFNFF_STATUS_NAN 6
+
+/* arithmetic modes for gfnff_set_precision */
+#define GFNFF_PREC_FP64 0
+#define GFNFF_PREC_MIXED 1
+
+typedef struct gfnff_handle gfnff_handle;
+
+/* Replaces MethodFactory::create("gfnff", cfg) (method_factory.cpp:460-516). */
+gfnff_handle* gfnff_create(void);
+void gfnff_destroy(gfnff_handle* h);
+
+/* ComputationalMethod::setMolecule(const Mol&) (computational_method.h:56;
+ * Mol layout global.h:69-99): Z[natoms], xyz_ang[natoms*3] row-major,
+ * total charge, optional forced bonds [2*nbonds] (0-based) that replace
+ * geometric bond detection when nbonds > 0 (gfnff_method.cpp:954-963).
+ * Builds topology and parameters on the host; no device work. */
+int gfnff_set_molecule(gfnff_handle* h, int32_t natoms, const int32_t* Z, const double* xyz_ang, int32_t charge,
+                       const int32_t* forced_bonds, int32_t nbonds);
+
+/* ComputationalMethod::updateGeometry(const Matrix&) (computational_method.h:63), Angstrom. */
+int gfnff_update_geometry(gfnff_handle* h, const double* xyz_ang);
+
+/* ComputationalMethod::calculateEnergy(bool gradient) + copyGradientTo(Matrix&)
+ * (computational_method.h:70,121-123).  energy: Hartree; grad: natoms*3
+ * Hartree/Bohr (may be NULL when want_grad == 0); terms: GFNFF_NTERMS per-term
+ * energies (may be NULL).  Runs on the GPU. */
+int gfnff_energy_gradient(gfnff_handle* h, int32_t want_grad, double* energy, double* grad, double* terms);
+
+/* Device-resident variant: xyz_bohr_dev (natoms*3, Bohr), energy_dev (1 or
+ * GFNFF_NTERMS+1 doubles: total followed by the terms), grad_dev (natoms*3)
+ * are device pointers; work is queued on `stream` (a hipStream_t, NULL = the
+ * default stream) and the call returns without synchronising. */
+int gfnff_energy_gradient_device(gfnff_handle* h, const double* xyz_bohr_dev, double* energy_dev, double* grad_dev,
+                                 void* stream);
+
+/* ComputationalMethod::getCharges() / getCN() (computational_method.h:295-337):
+ * EEQ charges and coordination numbers of the last evaluation. */
+int gfnff_charges(gfnff_handle* h, double* q);
+int gfnff_cn(gfnff_handle* h, double* cn);
+
+/* ComputationalMethod::setParameters(json) for the parity-relevant keys
+ * (gfnff.h:219-293): "cn_cutoff_bohr", "hb_accuracy", "threads". */
+int gfnff_set_param(gfnff_handle* h, const char* key, double value);
+
+/* GFNFF_PREC_FP64 (default) or GFNFF_PREC_MIXED. */
+int gfnff_set_precision(gfnff_handle* h, int32_t mode);
+
+/* ComputationalMethod::hasError() / getErrorMessage() (computational_method.h:190,201). */
+int gfnff_has_error(const gfnff_handle* h);
+const char* gfnff_last_error(const gfnff_handle* h);
+
+/* Engine extension: read-only view of the parameter set built by
+ * gfnff_set_molecule (no counterpart in the reference's interface; the
+ * reference exposes the same data through GFNFFParameterSet). */
+int gfnff_get_params(const gfnff_handle* h, gfnff_params_view* out);
+
+/* Engine extension: topology diagnostics of the last set-up.  hyb and pi
+ * (natoms each), bonds (2*nbonds_max); returns the number of detected bonds
+ * in *nbonds. */
+int gfnff_get_topology(const gfnff_handle* h, int32_t* hyb, int32_t* pi, int32_t* bonds, int32_t nbonds_max,
+                       int32_t* nbonds);
+
+/* Number of visible GPUs (0 when none); never initialises a context. */
+int gfnff_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

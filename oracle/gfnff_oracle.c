@@ -1,0 +1,974 @@
+/*
+ * gfnff_oracle.c -- CPU FP64 restatement of the reference's GFN-FF
+ * per-evaluation path.  TEST INFRASTRUCTURE ONLY: it is the checker that the
+ * HIP engine is compared against (tests/, __graft_entry__.smoke() and the
+ * bench.py cpu_baseline leg).  Nothing in the shipped engine links or calls it.
+ *
+ * Input is the engine's flat parameter set (include/gfnff_params.h), built by
+ * the engine's host-side topology generator; this file restates only the
+ * per-evaluation arithmetic:
+ *   CN + dlogdcn     cn_calculator.cpp:367-428, gfnff_method.cpp:5837-6258
+ *   D4 weights       d4param_generator.cpp:1079-1213, 1692-1768, 1770-1904
+ *   EEQ Phase 2      eeq_solver.cpp:3201-4006 (+ Schur solve :2087-2240)
+ *   term kernels     ff_workspace_gfnff.cpp:46-1739
+ *   self energy and CN chain rule   ff_workspace.cpp:415-536
+ * Units: Bohr, Hartree.  Every function cites the reference lines it follows.
+ *
+ * Parity status per function (see tests/test_oracle_fixtures.py): pinned by the
+ * reference's Fortran-derived fixtures (test_cases/reference_data/*.ref.json)
+ * for total and per-term energies, charges and per-term gradients; gradients
+ * additionally by central finite differences of this file's own energy.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/gfnff_params.h"
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+#define KN (-7.5)
+#define CNMAX 4.4
+#define SQRTPI 1.77245385091
+#define TSQRT2PI 0.797884560802866
+
+typedef struct {
+    double e[GFNFF_NTERMS];
+} terms_t;
+
+/* per-component gradients (optional), indices */
+enum { GB = 0, GA, GT, GR, GRN, GC, GD, GH, GX, GBA, GAT, NGC };
+
+static inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline void sub3(const double* a, const double* b, double* o) {
+    o[0] = a[0] - b[0];
+    o[1] = a[1] - b[1];
+    o[2] = a[2] - b[2];
+}
+static inline void cross3(const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline double norm3(const double* a) { return sqrt(dot3(a, a)); }
+
+typedef struct {
+    int n;
+    double* g;           /* total gradient [3n] */
+    double* gc[NGC];     /* optional component gradients */
+} gacc_t;
+
+static inline void gadd(gacc_t* G, int comp, int atom, double x, double y, double z) {
+    G->g[3 * atom] += x;
+    G->g[3 * atom + 1] += y;
+    G->g[3 * atom + 2] += z;
+    if (G->gc[comp]) {
+        G->gc[comp][3 * atom] += x;
+        G->gc[comp][3 * atom + 1] += y;
+        G->gc[comp][3 * atom + 2] += z;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* CN with 6 Bohr neighbour list: calculateGFNFFCNWithNeighbors,             */
+/* cn_calculator.cpp:367-428 (strict d^2 < cut^2)                            */
+/* ------------------------------------------------------------------------- */
+static void oracle_cn(const gfnff_params_view* P, const double* x, double cut, double* cn_raw, double* cn,
+                      double* dlogdcn) {
+    int n = P->natoms;
+    double cut2 = cut * cut;
+    memset(cn_raw, 0, sizeof(double) * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            double d[3];
+            sub3(x + 3 * i, x + 3 * j, d);
+            double d2 = dot3(d, d);
+            if (!(d2 < cut2)) continue;
+            double r = sqrt(d2);
+            double rij = P->rcov_cn[i] + P->rcov_cn[j];
+            double v = 0.5 * (1.0 + erf(KN * (r - rij) / rij));
+            cn_raw[i] += v;
+            cn_raw[j] += v;
+        }
+    for (int i = 0; i < n; ++i) {
+        cn[i] = log(1.0 + exp(CNMAX)) - log(1.0 + exp(CNMAX - cn_raw[i]));
+        dlogdcn[i] = exp(CNMAX) / (exp(CNMAX) + exp(cn_raw[i]));
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* D4 Gaussian weights and derivatives: d4param_generator.cpp:1079-1213 and   */
+/* :1692-1768 (wf = 4, CN-only weights)                                       */
+/* ------------------------------------------------------------------------- */
+static void oracle_gw(const gfnff_params_view* P, const double* cn, double* gw, double* dgw) {
+    for (int i = 0; i < P->natoms; ++i) {
+        int e = P->elem[i], nr = P->nref[e];
+        double ex[GFNFF_MAXREF], dex[GFNFF_MAXREF], norm = 0.0, dnorm = 0.0;
+        for (int r = 0; r < GFNFF_MAXREF; ++r) {
+            gw[i * 7 + r] = 0.0;
+            dgw[i * 7 + r] = 0.0;
+        }
+        for (int r = 0; r < nr; ++r) {
+            double df = cn[i] - P->refcn[e * 7 + r];
+            ex[r] = exp(-4.0 * df * df);
+            dex[r] = -8.0 * df * ex[r];
+            norm += ex[r];
+            dnorm += dex[r];
+        }
+        if (norm > 1e-10) {
+            for (int r = 0; r < nr; ++r) {
+                gw[i * 7 + r] = ex[r] / norm;
+                dgw[i * 7 + r] = (dex[r] * norm - ex[r] * dnorm) / (norm * norm);
+            }
+        } else if (nr > 0) {
+            gw[i * 7] = 1.0;
+        }
+    }
+}
+
+/* C6 and dC6/dCN for a pair from the reference C6 grid (getChargeWeightedC6
+ * d4param_generator.cpp:1325-1420, computeDC6DCN :1770-1904). */
+static void oracle_c6(const gfnff_params_view* P, const double* gw, const double* dgw, int i, int j, double* c6,
+                      double* dci, double* dcj) {
+    int ei = P->elem[i], ej = P->elem[j], ne = P->nelem;
+    const double* c = P->c6ref + ((size_t)ei * ne + ej) * 49;
+    double s = 0.0, si = 0.0, sj = 0.0;
+    for (int a = 0; a < P->nref[ei]; ++a)
+        for (int b = 0; b < P->nref[ej]; ++b) {
+            double r = c[a * 7 + b];
+            s += gw[i * 7 + a] * gw[j * 7 + b] * r;
+            si += dgw[i * 7 + a] * gw[j * 7 + b] * r;
+            sj += gw[i * 7 + a] * dgw[j * 7 + b] * r;
+        }
+    *c6 = s;
+    *dci = si;
+    *dcj = sj;
+}
+
+/* ------------------------------------------------------------------------- */
+/* EEQ Phase 2: calculateFinalCharges eeq_solver.cpp:3201-4006; bordered      */
+/* system solved by Cholesky of A and the Schur complement over fragments    */
+/* (solveWithSchurCholesky :2087-2240).                                       */
+/* ------------------------------------------------------------------------- */
+static int chol(int n, double* A) {
+    for (int k = 0; k < n; ++k) {
+        double* Ak = A + (size_t)k * n;
+        double d = Ak[k];
+        for (int p = 0; p < k; ++p) d -= Ak[p] * Ak[p];
+        if (!(d > 0.0)) return 0;
+        d = sqrt(d);
+        Ak[k] = d;
+        for (int i = k + 1; i < n; ++i) {
+            double* Ai = A + (size_t)i * n;
+            double s = Ai[k];
+            for (int p = 0; p < k; ++p) s -= Ai[p] * Ak[p];
+            Ai[k] = s / d;
+        }
+    }
+    return 1;
+}
+static void chol_solve(int n, const double* L, double* b) {
+    for (int i = 0; i < n; ++i) {
+        const double* Li = L + (size_t)i * n;
+        double s = b[i];
+        for (int p = 0; p < i; ++p) s -= Li[p] * b[p];
+        b[i] = s / Li[i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double s = b[i];
+        for (int p = i + 1; p < n; ++p) s -= L[(size_t)p * n + i] * b[p];
+        b[i] = s / L[(size_t)i * n + i];
+    }
+}
+static int lu_solve(int m, double* M, double* b) {
+    for (int k = 0; k < m; ++k) {
+        int p = k;
+        double best = fabs(M[(size_t)k * m + k]);
+        for (int i = k + 1; i < m; ++i)
+            if (fabs(M[(size_t)i * m + k]) > best) {
+                best = fabs(M[(size_t)i * m + k]);
+                p = i;
+            }
+        if (best == 0.0) return 0;
+        if (p != k) {
+            for (int j = 0; j < m; ++j) {
+                double t = M[(size_t)k * m + j];
+                M[(size_t)k * m + j] = M[(size_t)p * m + j];
+                M[(size_t)p * m + j] = t;
+            }
+            double t = b[k];
+            b[k] = b[p];
+            b[p] = t;
+        }
+        for (int i = k + 1; i < m; ++i) {
+            double f = M[(size_t)i * m + k] / M[(size_t)k * m + k];
+            for (int j = k + 1; j < m; ++j) M[(size_t)i * m + j] -= f * M[(size_t)k * m + j];
+            b[i] -= f * b[k];
+        }
+    }
+    for (int i = m - 1; i >= 0; --i) {
+        double s = b[i];
+        for (int j = i + 1; j < m; ++j) s -= M[(size_t)i * m + j] * b[j];
+        b[i] = s / M[(size_t)i * m + i];
+    }
+    return 1;
+}
+
+static int oracle_eeq(const gfnff_params_view* P, const double* x, const double* cn, double* q) {
+    int n = P->natoms, nf = P->nfrag;
+    double* A = (double*)malloc(sizeof(double) * (size_t)n * n);
+    double* rhs = (double*)malloc(sizeof(double) * n);
+    double* Z2 = (double*)calloc((size_t)nf * n, sizeof(double));
+    double* S = (double*)calloc((size_t)nf * nf, sizeof(double));
+    double* lam = (double*)calloc(nf, sizeof(double));
+    int ok = 0;
+    if (!A || !rhs || !Z2 || !S || !lam) goto out;
+    for (int i = 0; i < n; ++i) {
+        A[(size_t)i * n + i] = P->gam_eff[i] + TSQRT2PI / sqrt(P->alpeeq[i]);
+        for (int j = 0; j < i; ++j) {
+            double d[3];
+            sub3(x + 3 * i, x + 3 * j, d);
+            double r = norm3(d);
+            if (r < 1e-10) goto out; /* reference: zero distance -> fallback charges */
+            double g = 1.0 / sqrt(P->alpeeq[i] + P->alpeeq[j]);
+            double v = erf(g * r) / r;
+            A[(size_t)i * n + j] = v;
+            A[(size_t)j * n + i] = v;
+        }
+        rhs[i] = P->chi_base[i] + P->cnf[i] * sqrt(cn[i]);
+    }
+    if (!chol(n, A)) {
+        /* not positive definite: LU on the bordered system, as the reference's
+           dispatchSolve falls back to LU (eeq_solver.cpp:2143-2165) */
+        int m = n + nf;
+        double* M = (double*)calloc((size_t)m * m, sizeof(double));
+        double* b = (double*)calloc(m, sizeof(double));
+        if (!M || !b) {
+            free(M);
+            free(b);
+            goto out;
+        }
+        for (int i = 0; i < n; ++i) {
+            M[(size_t)i * m + i] = P->gam_eff[i] + TSQRT2PI / sqrt(P->alpeeq[i]);
+            for (int j = 0; j < i; ++j) {
+                double d[3];
+                sub3(x + 3 * i, x + 3 * j, d);
+                double r = norm3(d);
+                double g = 1.0 / sqrt(P->alpeeq[i] + P->alpeeq[j]);
+                double v = erf(g * r) / r;
+                M[(size_t)i * m + j] = v;
+                M[(size_t)j * m + i] = v;
+            }
+            M[(size_t)i * m + n + P->frag[i]] = 1.0;
+            M[(size_t)(n + P->frag[i]) * m + i] = 1.0;
+            b[i] = P->chi_base[i] + P->cnf[i] * sqrt(cn[i]);
+        }
+        for (int f = 0; f < nf; ++f) b[n + f] = P->qfrag[f];
+        int okl = lu_solve(m, M, b);
+        if (okl)
+            for (int i = 0; i < n; ++i) q[i] = b[i];
+        free(M);
+        free(b);
+        ok = okl;
+        goto out;
+    }
+    chol_solve(n, A, rhs);
+    for (int f = 0; f < nf; ++f) {
+        double* c = Z2 + (size_t)f * n;
+        for (int i = 0; i < n; ++i) c[i] = (P->frag[i] == f) ? 1.0 : 0.0;
+        chol_solve(n, A, c);
+    }
+    for (int f = 0; f < nf; ++f) {
+        for (int g = 0; g < nf; ++g) {
+            double s = 0.0;
+            const double* c = Z2 + (size_t)g * n;
+            for (int i = 0; i < n; ++i)
+                if (P->frag[i] == f) s += c[i];
+            S[(size_t)f * nf + g] = s;
+        }
+        double s = 0.0;
+        for (int i = 0; i < n; ++i)
+            if (P->frag[i] == f) s += rhs[i];
+        lam[f] = s - P->qfrag[f];
+    }
+    if (!lu_solve(nf, S, lam)) goto out;
+    for (int i = 0; i < n; ++i) {
+        double s = rhs[i];
+        for (int f = 0; f < nf; ++f) s -= Z2[(size_t)f * n + i] * lam[f];
+        q[i] = s;
+    }
+    ok = 1;
+out:
+    free(A);
+    free(rhs);
+    free(Z2);
+    free(S);
+    free(lam);
+    return ok;
+}
+
+/* ------------------------------------------------------------------------- */
+/* geometry helpers                                                           */
+/* ------------------------------------------------------------------------- */
+
+/* dphi/dr for the signed dihedral i-j-k-l (phi = atan2, the angle of
+ * gfnff_geometry.h:60-200); derivative from the standard decomposition
+ * dphi/dri = -|b|/|n1|^2 n1, dphi/drl = |b|/|n2|^2 n2. */
+static double dihedral_grad(const double* ri, const double* rj, const double* rk, const double* rl, double g[4][3],
+                            int* ok) {
+    double v1[3], v2[3], v3[3], n1[3], n2[3];
+    sub3(rj, ri, v1);
+    sub3(rk, rj, v2);
+    sub3(rl, rk, v3);
+    cross3(v1, v2, n1);
+    cross3(v2, v3, n2);
+    double nn1 = norm3(n1), nn2 = norm3(n2), nv2 = norm3(v2);
+    *ok = 1;
+    if (nn1 < 1e-10 || nn2 < 1e-10 || nv2 < 1e-10) {
+        memset(g, 0, sizeof(double) * 12);
+        *ok = 0;
+        return 0.0;
+    }
+    double c = dot3(n1, n2) / (nn1 * nn2);
+    double s = nv2 * dot3(n1, v3) / nn1 / nn2;
+    if (s > 1.0) s = 1.0;
+    if (s < -1.0) s = -1.0;
+    double phi = atan2(s, c);
+    double n1s = nn1 * nn1, n2s = nn2 * nn2, b2 = nv2 * nv2;
+    double f1 = -nv2 / n1s, f4 = nv2 / n2s;
+    double a = dot3(v1, v2) / b2, b = dot3(v3, v2) / b2;
+    for (int d = 0; d < 3; ++d) {
+        double gi = f1 * n1[d], gl = f4 * n2[d];
+        g[0][d] = gi;
+        g[3][d] = gl;
+        g[1][d] = -gi - a * gi + b * gl;
+        g[2][d] = -gl + a * gi - b * gl;
+    }
+    /* the reference's sign convention for the torsion derivative: the energy
+       terms are even in phi, so dE/dr is convention independent */
+    return phi;
+}
+
+/* out-of-plane angle (gfnff_geometry.h:207-300): angle between the i->j bond
+ * and the plane (i,k,l) normal construction of the reference */
+static double oop_angle(const double* rc, const double* r1, const double* r2, const double* r3, double g[4][3]) {
+    /* omega = asin( n . e_c1 ), n = unit normal of plane spanned by (r2-r1),(r3-r1)?
+       Not used by the sp3 chemistry currently supported; left unimplemented. */
+    (void)rc;
+    (void)r1;
+    (void)r2;
+    (void)r3;
+    memset(g, 0, sizeof(double) * 12);
+    return 0.0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* the evaluation                                                             */
+/* ------------------------------------------------------------------------- */
+int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad, double* E_total, double* terms_out,
+                      double* grad, double* q_out, double* cn_out, double* gcomp /* [NGC*3n] or NULL */) {
+    const int n = P->natoms;
+    int status = 0;
+    double *cn_raw = calloc(n, 8), *cn = calloc(n, 8), *dlog = calloc(n, 8), *gw = calloc(7 * (size_t)n, 8),
+           *dgw = calloc(7 * (size_t)n, 8), *q = calloc(n, 8), *dEdcn = calloc(n, 8), *g = calloc(3 * (size_t)n, 8);
+    terms_t T;
+    memset(&T, 0, sizeof(T));
+    gacc_t G;
+    G.n = n;
+    G.g = g;
+    for (int c = 0; c < NGC; ++c) G.gc[c] = gcomp ? gcomp + (size_t)c * 3 * n : NULL;
+    if (gcomp) memset(gcomp, 0, sizeof(double) * NGC * 3 * (size_t)n);
+    double* dEdcn_bond = calloc(n, 8);
+    if (!cn_raw || !cn || !dlog || !gw || !dgw || !q || !dEdcn || !g || !dEdcn_bond) {
+        status = 1;
+        goto done;
+    }
+
+    oracle_cn(P, x, 6.0, cn_raw, cn, dlog);
+    oracle_gw(P, cn, gw, dgw);
+    if (!oracle_eeq(P, x, cn, q)) {
+        status = 3;
+        goto done;
+    }
+
+    /* --- HB coordination numbers for X-H bonds: ff_workspace_gfnff.cpp:46-97 */
+    double* hbcn = calloc(n, 8);
+    for (int e = 0; e < P->nhbcn; ++e) {
+        int H = P->hbcn_pair[2 * e], B = P->hbcn_pair[2 * e + 1];
+        double d[3];
+        sub3(x + 3 * B, x + 3 * H, d);
+        double r2 = dot3(d, d);
+        if (r2 > 900.0) continue;
+        double r = sqrt(r2);
+        double rc = 1.78 * (4.0 / 3.0) * (P->rcov_d3[H] + P->rcov_d3[B]);
+        double arg = -27.5 * (r - rc) / rc;
+        hbcn[H] += 0.5 * (1.0 + erf(arg));
+    }
+
+    /* --- bonds: calcBonds ff_workspace_gfnff.cpp:104-180 */
+    for (int b = 0; b < P->nbond; ++b) {
+        int i = P->bond_ij[2 * b], j = P->bond_ij[2 * b + 1];
+        const double* p = P->bond_par + 8 * b;
+        double d[3];
+        sub3(x + 3 * i, x + 3 * j, d);
+        double r = norm3(d);
+        double r0 = (p[0] + p[2] * cn[i] + p[1] + p[3] * cn[j] + p[4]) * p[5];
+        double dr = r - r0;
+        double a0 = p[6], a = a0;
+        int H = P->bond_hbH[b];
+        if (H >= 0) a = (-(1.0 - 0.9) * hbcn[H] + 1.0) * a0;
+        double e = p[7] * exp(-a * dr * dr);
+        T.e[GFNFF_E_BOND] += e;
+        if (want_grad) {
+            double dEdr = -2.0 * a * dr * e;
+            gadd(&G, GB, i, dEdr * d[0] / r, dEdr * d[1] / r, dEdr * d[2] / r);
+            gadd(&G, GB, j, -dEdr * d[0] / r, -dEdr * d[1] / r, -dEdr * d[2] / r);
+            if (H >= 0) {
+                double zz = 0.1 * a0 * dr * dr * e;
+                for (int k = 0; k < P->nhbcn; ++k) {
+                    if (P->hbcn_pair[2 * k] != H) continue;
+                    int B = P->hbcn_pair[2 * k + 1];
+                    double v[3];
+                    sub3(x + 3 * B, x + 3 * H, v);
+                    double rr2 = dot3(v, v);
+                    if (rr2 > 900.0) continue;
+                    double rr = sqrt(rr2);
+                    double rc = 1.78 * (4.0 / 3.0) * (P->rcov_d3[H] + P->rcov_d3[B]);
+                    double arg = -27.5 * (rr - rc) / rc;
+                    double dcn = 0.5641895835477563 * (-27.5 / rc) * exp(-arg * arg) / rr;
+                    gadd(&G, GB, H, -zz * dcn * v[0], -zz * dcn * v[1], -zz * dcn * v[2]);
+                    gadd(&G, GB, B, zz * dcn * v[0], zz * dcn * v[1], zz * dcn * v[2]);
+                }
+            }
+            double yy = -dEdr;
+            dEdcn[i] += yy * p[5] * p[2];
+            dEdcn[j] += yy * p[5] * p[3];
+            dEdcn_bond[i] += yy * p[5] * p[2];
+            dEdcn_bond[j] += yy * p[5] * p[3];
+        }
+    }
+    free(hbcn);
+
+    /* --- angles: calcAngles ff_workspace_gfnff.cpp:187-275 */
+    for (int a = 0; a < P->nangle; ++a) {
+        int i = P->angle_ijk[3 * a], j = P->angle_ijk[3 * a + 1], k = P->angle_ijk[3 * a + 2];
+        double fc = P->angle_par[2 * a], th0 = P->angle_par[2 * a + 1];
+        double u[3], v[3];
+        sub3(x + 3 * i, x + 3 * j, u);
+        sub3(x + 3 * k, x + 3 * j, v);
+        double ru = norm3(u), rv = norm3(v);
+        double cth = (ru < 1e-10 || rv < 1e-10) ? 1.0 : dot3(u, v) / (ru * rv);
+        if (cth > 1.0) cth = 1.0;
+        if (cth < -1.0) cth = -1.0;
+        double th = acos(cth), e, dEdcos;
+        if (fabs(M_PI - th0) < 1e-6) {
+            double dt = th - th0;
+            e = fc * dt * dt;
+            double s = sin(th);
+            dEdcos = (s > 1e-14) ? -2.0 * fc * dt / s : 0.0;
+        } else {
+            double c0 = cos(th0);
+            e = fc * (cth - c0) * (cth - c0);
+            dEdcos = 2.0 * fc * (cth - c0);
+        }
+        double rci = P->rcov_d3[i] * (4.0 / 3.0), rcj = P->rcov_d3[j] * (4.0 / 3.0), rck = P->rcov_d3[k] * (4.0 / 3.0);
+        double rcut_ij = 0.595 * (rci + rcj) * (rci + rcj);
+        double rcut_jk = 0.595 * (rcj + rck) * (rcj + rck);
+        double rij2 = ru * ru, rjk2 = rv * rv;
+        double rr1 = rij2 / rcut_ij, rr2 = rjk2 / rcut_jk;
+        rr1 *= rr1;
+        rr2 *= rr2;
+        double d1 = 1.0 / (1.0 + rr1), d2 = 1.0 / (1.0 + rr2), damp = d1 * d2;
+        T.e[GFNFF_E_ANGLE] += e * damp;
+        if (want_grad) {
+            double dd1 = (rij2 > 1e-8) ? -4.0 * rr1 / (rij2 * (1.0 + rr1) * (1.0 + rr1)) : 0.0;
+            double dd2 = (rjk2 > 1e-8) ? -4.0 * rr2 / (rjk2 * (1.0 + rr2) * (1.0 + rr2)) : 0.0;
+            double gi[3], gk[3];
+            double sinth = sqrt(fmax(0.0, 1.0 - cth * cth));
+            double pre = (sinth < 1e-14 || ru < 1e-10 || rv < 1e-10) ? 0.0 : dEdcos * damp;
+            for (int c = 0; c < 3; ++c) {
+                /* dcos/dri = (v/|v| - cos u/|u|)/|u| */
+                gi[c] = pre * (v[c] / rv - cth * u[c] / ru) / ru + e * dd1 * d2 * u[c];
+                gk[c] = pre * (u[c] / ru - cth * v[c] / rv) / rv + e * dd2 * d1 * v[c];
+            }
+            gadd(&G, GA, i, gi[0], gi[1], gi[2]);
+            gadd(&G, GA, k, gk[0], gk[1], gk[2]);
+            gadd(&G, GA, j, -gi[0] - gk[0], -gi[1] - gk[1], -gi[2] - gk[2]);
+        }
+    }
+
+    /* --- torsions (primary + extra): ff_workspace_gfnff.cpp:282-482 */
+    for (int t = 0; t < P->ntors; ++t) {
+        const int* id = P->tors_ijkl + 4 * t;
+        const double* p = P->tors_par + 4 * t;
+        double gphi[4][3];
+        int ok;
+        double phi = dihedral_grad(x + 3 * id[0], x + 3 * id[1], x + 3 * id[2], x + 3 * id[3], gphi, &ok);
+        double V = p[0], nn = p[1], phi0 = p[2], atc = p[3];
+        double c1 = nn * (phi - phi0) + M_PI;
+        double e = V * (1.0 + cos(c1));
+        double r2[3], v[3][3];
+        sub3(x + 3 * id[0], x + 3 * id[1], v[0]);
+        sub3(x + 3 * id[1], x + 3 * id[2], v[1]);
+        sub3(x + 3 * id[2], x + 3 * id[3], v[2]);
+        double dmp[3], rr[3];
+        for (int s = 0; s < 3; ++s) {
+            r2[s] = dot3(v[s], v[s]);
+            double rs = P->rcov_d3[id[s]] * (4.0 / 3.0) + P->rcov_d3[id[s + 1]] * (4.0 / 3.0);
+            double rc = atc * rs * rs;
+            rr[s] = r2[s] / rc;
+            rr[s] *= rr[s];
+            dmp[s] = 1.0 / (1.0 + rr[s]);
+        }
+        double damp = dmp[0] * dmp[1] * dmp[2];
+        T.e[GFNFF_E_TORSION] += e * damp;
+        if (want_grad) {
+            double dEdphi = ok ? -V * nn * sin(c1) * damp : 0.0;
+            for (int a = 0; a < 4; ++a)
+                gadd(&G, GT, id[a], dEdphi * gphi[a][0], dEdphi * gphi[a][1], dEdphi * gphi[a][2]);
+            double dd[3];
+            for (int s = 0; s < 3; ++s)
+                dd[s] = (r2[s] > 1e-8) ? -4.0 * rr[s] / (r2[s] * (1.0 + rr[s]) * (1.0 + rr[s])) : 0.0;
+            double f0 = e * dd[0] * dmp[1] * dmp[2], f1 = e * dmp[0] * dd[1] * dmp[2], f2 = e * dmp[0] * dmp[1] * dd[2];
+            gadd(&G, GT, id[0], f0 * v[0][0], f0 * v[0][1], f0 * v[0][2]);
+            gadd(&G, GT, id[1], -f0 * v[0][0] + f1 * v[1][0], -f0 * v[0][1] + f1 * v[1][1], -f0 * v[0][2] + f1 * v[1][2]);
+            gadd(&G, GT, id[2], -f1 * v[1][0] + f2 * v[2][0], -f1 * v[1][1] + f2 * v[2][1], -f1 * v[1][2] + f2 * v[2][2]);
+            gadd(&G, GT, id[3], -f2 * v[2][0], -f2 * v[2][1], -f2 * v[2][2]);
+        }
+    }
+    if (P->ninv > 0) {
+        double gg[4][3];
+        (void)oop_angle(x, x, x, x, gg);
+        status = 2; /* inversions not restated yet */
+        goto done;
+    }
+
+    /* --- D4 dispersion pairs: calcD4Dispersion ff_workspace_gfnff.cpp:715-767 */
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            double d[3];
+            sub3(x + 3 * i, x + 3 * j, d);
+            double r = norm3(d);
+            if (r > 50.0 || r < 1e-10) continue;
+            double c6, dci, dcj;
+            oracle_c6(P, gw, dgw, i, j, &c6, &dci, &dcj);
+            double r4r2 = 3.0 * P->sqrt_zr4r2[i] * P->sqrt_zr4r2[j];
+            double r0 = 0.58 * sqrt(r4r2) + 4.80;
+            double r0s = r0 * r0;
+            double zc = P->zeta[i] * P->zeta[j];
+            double r2 = r * r, r6 = r2 * r2 * r2, r06 = r0s * r0s * r0s;
+            double t6 = 1.0 / (r6 + r06), t8 = 1.0 / (r6 * r2 + r06 * r0s);
+            double sum = t6 + 2.0 * r4r2 * t8;
+            T.e[GFNFF_E_DISPERSION] += -c6 * sum * zc;
+            if (want_grad) {
+                double d6 = -6.0 * r2 * r2 * t6 * t6, d8 = -8.0 * r2 * r2 * r2 * t8 * t8;
+                double dEdr = -c6 * zc * (d6 + 2.0 * r4r2 * d8) * r;
+                gadd(&G, GD, i, dEdr * d[0] / r, dEdr * d[1] / r, dEdr * d[2] / r);
+                gadd(&G, GD, j, -dEdr * d[0] / r, -dEdr * d[1] / r, -dEdr * d[2] / r);
+                dEdcn[i] -= dci * sum * zc;
+                dEdcn[j] -= dcj * sum * zc;
+            }
+        }
+
+    /* --- repulsion: bonded ff_workspace_gfnff.cpp:774-807, non-bonded :814-842
+       (non-bonded pair set = all non-bonded pairs within 20 Bohr,
+        gfnff_method.cpp:9781-9872) */
+    for (int b = 0; b < P->nbrep; ++b) {
+        int i = P->brep_ij[2 * b], j = P->brep_ij[2 * b + 1];
+        double al = P->brep_par[2 * b], ab = P->brep_par[2 * b + 1];
+        double d[3];
+        sub3(x + 3 * i, x + 3 * j, d);
+        double r = norm3(d);
+        if (r > 20.0 || r < 1e-8) continue;
+        double e = ab * exp(-al * r * sqrt(r)) / r;
+        T.e[GFNFF_E_BONDED_REP] += e;
+        if (want_grad) {
+            double dEdr = -e / r - 1.5 * al * sqrt(r) * e;
+            gadd(&G, GR, i, dEdr * d[0] / r, dEdr * d[1] / r, dEdr * d[2] / r);
+            gadd(&G, GR, j, -dEdr * d[0] / r, -dEdr * d[1] / r, -dEdr * d[2] / r);
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        int zi = P->Z[i];
+        for (int j = i + 1; j < n; ++j) {
+            double d[3];
+            sub3(x + 3 * i, x + 3 * j, d);
+            double r2 = dot3(d, d);
+            if (r2 > 400.0) continue;
+            int td = 999;
+            for (int k = P->topo_off[i]; k < P->topo_off[i + 1]; ++k)
+                if (P->topo_idx[k] == j) {
+                    td = P->topo_dist[k];
+                    break;
+                }
+            if (td == 1) continue;
+            double r = sqrt(r2);
+            if (r > 20.0 || r < 1e-8) continue;
+            int zj = P->Z[j];
+            double ff = 1.0;
+            if (zi == 1 && zj == 1) {
+                ff = 0.6290;
+                if (td == 2) ff *= 1.4580;
+                else if (td == 3) ff *= 0.7080;
+            } else if ((zi == 1 && P->metal[j] > 0) || (zj == 1 && P->metal[i] > 0)) {
+                ff = 0.85;
+            } else if ((zi == 1 && zj == 6) || (zj == 1 && zi == 6)) {
+                ff = 0.91;
+            } else if ((zi == 1 && zj == 8) || (zj == 1 && zi == 8)) {
+                ff = 1.04;
+            }
+            double al = sqrt(P->rep_anb[i] * P->rep_anb[j]) * ff;
+            double ab = P->repz[i] * P->repz[j] * 0.4270;
+            double e = ab * exp(-al * r * sqrt(r)) / r;
+            T.e[GFNFF_E_NONBONDED_REP] += e;
+            if (want_grad) {
+                double dEdr = -e / r - 1.5 * al * sqrt(r) * e;
+                gadd(&G, GRN, i, dEdr * d[0] / r, dEdr * d[1] / r, dEdr * d[2] / r);
+                gadd(&G, GRN, j, -dEdr * d[0] / r, -dEdr * d[1] / r, -dEdr * d[2] / r);
+            }
+        }
+    }
+
+    /* --- Coulomb pairs: calcCoulomb ff_workspace_gfnff.cpp:850-898 (r_cut 100) */
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            double d[3];
+            sub3(x + 3 * i, x + 3 * j, d);
+            double r = norm3(d);
+            if (r > 100.0 || r < 1e-10) continue;
+            double gij = 1.0 / sqrt(P->alpeeq[i] + P->alpeeq[j]);
+            double gr = gij * r;
+            double ef = erf(gr);
+            T.e[GFNFF_E_COULOMB] += q[i] * q[j] * ef / r;
+            if (want_grad) {
+                double dEdr = q[i] * q[j] * (gij * exp(-gr * gr) * (2.0 / 1.772453850905516) / r - ef / (r * r));
+                gadd(&G, GC, i, dEdr * d[0] / r, dEdr * d[1] / r, dEdr * d[2] / r);
+                gadd(&G, GC, j, -dEdr * d[0] / r, -dEdr * d[1] / r, -dEdr * d[2] / r);
+            }
+        }
+    /* self energy: ff_workspace.cpp:426-449 */
+    {
+        double Een = 0.0, Es = 0.0;
+        for (int i = 0; i < n; ++i) {
+            double chi = P->chi_base[i] + P->cnf[i] * sqrt(fmax(cn[i], 0.0));
+            Een -= q[i] * chi;
+            Es += 0.5 * q[i] * q[i] * (P->gam_eff[i] + 0.797884560802865 / sqrt(P->alpeeq[i]));
+        }
+        T.e[GFNFF_E_COULOMB] += Een + Es;
+    }
+
+    /* --- hydrogen bonds: calcHydrogenBonds ff_workspace_gfnff.cpp:938-1404 */
+    for (int h = 0; h < P->nhb; ++h) {
+        int A = P->hb_ahb[3 * h], Hh = P->hb_ahb[3 * h + 1], B = P->hb_ahb[3 * h + 2];
+        int cs = P->hb_case[h];
+        const double* hp = P->hb_par + 7 * h;
+        double basA = hp[0], basB = hp[1], aciA = hp[2], aciB = hp[3];
+        const double *pA = x + 3 * A, *pH = x + 3 * Hh, *pB = x + 3 * B;
+        double vAH[3], vHB[3], vAB[3];
+        sub3(pH, pA, vAH);
+        sub3(pB, pH, vHB);
+        sub3(pB, pA, vAB);
+        double rAH = norm3(vAH), rHB = norm3(vHB), rAB = norm3(vAB);
+        if (rAB > 50.0) continue;
+        double rAH4 = rAH * rAH * rAH * rAH, rHB4 = rHB * rHB * rHB * rHB;
+        double den = 1.0 / (rAH4 + rHB4);
+        double eH = exp(15.0 * hp[4]), eA = exp(15.0 * -hp[5]), eB = exp(15.0 * -hp[6]);
+        double QH = eH / (eH + 1.0), QA = eA / (eA + 1.0), QB = eB / (eB + 1.0);
+        double bas = (QA * basA * rAH4 + QB * basB * rHB4) * den;
+        double aci = (aciB * rAH4 + aciA * rHB4) * den;
+        /* radab: sum of the tabulated Pyykko radii (Angstrom values used as is,
+           ff_workspace_gfnff.cpp:982) */
+        double radab = P->hb_rad[A] + P->hb_rad[B];
+        double dshort = 1.0 / (1.0 + pow(22.0 * radab / (rAB * rAB), 6.0));
+        double dlong = 1.0 / (1.0 + pow(rAB * rAB / 85.0, 6.0));
+        double denv = dshort * dlong;
+        double ratio = (rAH + rHB) / rAB;
+        double expo0 = (49.0 / radab) * (ratio - 1.0);
+        double doutl = (expo0 > 15.0) ? 0.0 : 2.0 / (1.0 + exp(expo0));
+        double outl_nb = 1.0;
+        int nb0 = P->hb_nbB_off[h], nb1 = P->hb_nbB_off[h + 1];
+        double nbcut = (P->Z[B] == 7 && nb1 - nb0 == 1) ? 2.0 : 11.20;
+        if (cs >= 2)
+            for (int k = nb0; k < nb1; ++k) {
+                int nb = P->hb_nbB_idx[k];
+                double u[3], w[3];
+                sub3(x + 3 * nb, pA, u);
+                sub3(x + 3 * nb, pB, w);
+                double ex = (nbcut / radab) * ((norm3(u) + norm3(w)) / rAB - 1.0);
+                outl_nb *= (2.0 / (1.0 + exp(-ex)) - 1.0);
+            }
+        double rdamp = (cs >= 2) ? denv * (1.8 / (rHB * rHB * rHB) - 0.8 / (rAB * rAB * rAB)) : denv / (rAB * rAB * rAB);
+        if (cs == 3 || cs == 4) {
+            status = 2; /* case 3/4 need carbonyl/imine acceptors: not in the current chemistry */
+            goto done;
+        }
+        double qhoutl = QH * doutl * outl_nb;
+        double gscale = (cs == 2 || cs == 4) ? 0.268 : 1.0;
+        double EHB;
+        double cval = aciA * basB * QA * QB * gscale;
+        if (cs >= 2) EHB = -rdamp * qhoutl * cval;
+        else EHB = -bas * aci * rdamp * qhoutl;
+        T.e[GFNFF_E_HBOND] += EHB;
+        if (!want_grad) continue;
+        double drab[3], drah[3], drbh[3];
+        sub3(pA, pB, drab);
+        sub3(pA, pH, drah);
+        sub3(pB, pH, drbh);
+        double rab2 = rAB * rAB, rbh2 = rHB * rHB, rah2 = rAH * rAH;
+        double rahprbh = rAH + rHB + 1e-12;
+        double ratio1 = pow(rab2 / 85.0, 6.0);
+        double ratio3 = pow(22.0 * radab / rab2, 6.0);
+        double ddamp = (-2.0 * 6.0 * ratio1 / (1.0 + ratio1)) + (2.0 * 6.0 * ratio3 / (1.0 + ratio3));
+        double expo = (49.0 / radab) * (rahprbh / rAB - 1.0);
+        if (expo > 15.0) continue;
+        double ratio2 = exp(expo);
+        double ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0}, gh[3] = {0, 0, 0};
+        if (cs >= 2) {
+            double rbhdamp = denv * 1.8 / (rbh2 * rHB), rabdamp = denv * -0.8 / (rab2 * rAB);
+            double dterm = -qhoutl * cval, aterm = -rdamp * QH * outl_nb * cval;
+            double nbterm = -rdamp * QH * doutl * cval;
+            double gi = ((rabdamp + rbhdamp) * ddamp - 3.0 * rabdamp) / rab2 * dterm;
+            for (int c = 0; c < 3; ++c) {
+                ga[c] = gi * drab[c];
+                gb[c] = -gi * drab[c];
+            }
+            gi = -3.0 * rbhdamp / rbh2 * dterm;
+            for (int c = 0; c < 3; ++c) {
+                gb[c] += gi * drbh[c];
+                gh[c] = -gi * drbh[c];
+            }
+            double tmp1 = -2.0 * aterm * ratio2 * expo / ((1.0 + ratio2) * (1.0 + ratio2)) / (rahprbh - rAB);
+            gi = -tmp1 * rahprbh / rab2;
+            for (int c = 0; c < 3; ++c) {
+                ga[c] += gi * drab[c];
+                gb[c] -= gi * drab[c];
+            }
+            for (int c = 0; c < 3; ++c) {
+                double dga = tmp1 / rAH * drah[c], dgb = tmp1 / rHB * drbh[c];
+                ga[c] += dga;
+                gb[c] += dgb;
+                gh[c] += -dga - dgb;
+            }
+            for (int k = nb0; k < nb1; ++k) {
+                int nb = P->hb_nbB_idx[k];
+                double dranb[3], drbnb[3];
+                sub3(pA, x + 3 * nb, dranb);
+                sub3(pB, x + 3 * nb, drbnb);
+                double ranb = norm3(dranb), rbnb = norm3(drbnb), s = ranb + rbnb + 1e-12;
+                double ex = (nbcut / radab) * (s / rAB - 1.0);
+                double r2n = exp(-ex);
+                double o_i = 2.0 / (1.0 + r2n) - 1.0;
+                double others = (fabs(o_i) > 1e-12) ? outl_nb / o_i : 1.0;
+                double tmp2 = 2.0 * nbterm * others * r2n * ex / ((1.0 + r2n) * (1.0 + r2n)) / (s - rAB);
+                double gi2 = -tmp2 * s / rab2;
+                double gnb[3];
+                for (int c = 0; c < 3; ++c) {
+                    ga[c] += gi2 * drab[c];
+                    gb[c] -= gi2 * drab[c];
+                    double dga = tmp2 / ranb * dranb[c], dgb = tmp2 / rbnb * drbnb[c];
+                    ga[c] += dga;
+                    gb[c] += dgb;
+                    gnb[c] = -dga - dgb;
+                }
+                gadd(&G, GH, nb, gnb[0], gnb[1], gnb[2]);
+            }
+        } else {
+            double caa = QA * basA, cbb = QB * basB;
+            double rterm = -aci * rdamp * qhoutl, dterm = -aci * bas * qhoutl, sterm = -rdamp * bas * qhoutl;
+            double aterm = -aci * bas * rdamp * QH;
+            double tmp = den * den * 4.0;
+            double dd24a = rah2 * rHB4 * tmp, dd24b = rbh2 * rAH4 * tmp;
+            for (int c = 0; c < 3; ++c) {
+                ga[c] = (caa - cbb) * dd24a * rterm * drah[c];
+                gb[c] = (cbb - caa) * dd24b * rterm * drbh[c];
+                gh[c] = -ga[c] - gb[c];
+                double dga = (aciB - aciA) * dd24a * drah[c] * sterm;
+                double dgb = (aciA - aciB) * dd24b * drbh[c] * sterm;
+                ga[c] += dga;
+                gb[c] += dgb;
+                gh[c] += -dga - dgb;
+            }
+            double gi = rdamp * (ddamp - 3.0) / rab2;
+            for (int c = 0; c < 3; ++c) {
+                ga[c] += gi * drab[c] * dterm;
+                gb[c] -= gi * drab[c] * dterm;
+            }
+            gi = aterm * 2.0 * ratio2 * expo * rahprbh / ((1.0 + ratio2) * (1.0 + ratio2)) / (rahprbh - rAB) / rab2;
+            for (int c = 0; c < 3; ++c) {
+                ga[c] += gi * drab[c];
+                gb[c] -= gi * drab[c];
+            }
+            double to = -2.0 * aterm * ratio2 * expo / ((1.0 + ratio2) * (1.0 + ratio2)) / (rahprbh - rAB);
+            for (int c = 0; c < 3; ++c) {
+                double dga = drah[c] * to / rAH, dgb = drbh[c] * to / rHB;
+                ga[c] += dga;
+                gb[c] += dgb;
+                gh[c] += -dga - dgb;
+            }
+        }
+        gadd(&G, GH, A, ga[0], ga[1], ga[2]);
+        gadd(&G, GH, B, gb[0], gb[1], gb[2]);
+        gadd(&G, GH, Hh, gh[0], gh[1], gh[2]);
+    }
+
+    /* --- D4 ATM over bonded triplets: calcATM / calcATMGradient
+       ff_workspace_gfnff.cpp:1526-1658.  The energy uses +c9 and the gradient
+       -c9, as the reference does. */
+    for (int t = 0; t < P->natm; ++t) {
+        int i = P->atm_ijk[3 * t], j = P->atm_ijk[3 * t + 1], k = P->atm_ijk[3 * t + 2];
+        double c6ij, c6ik, c6jk, d1, d2;
+        oracle_c6(P, gw, dgw, i, j, &c6ij, &d1, &d2);
+        oracle_c6(P, gw, dgw, i, k, &c6ik, &d1, &d2);
+        oracle_c6(P, gw, dgw, j, k, &c6jk, &d1, &d2);
+        double c9 = 1.0 * sqrt(fabs(c6ij * c6ik * c6jk));
+        double vij[3], vik[3], vjk[3];
+        sub3(x + 3 * j, x + 3 * i, vij);
+        sub3(x + 3 * k, x + 3 * i, vik);
+        sub3(x + 3 * k, x + 3 * j, vjk);
+        double rij = norm3(vij), rik = norm3(vik), rjk = norm3(vjk);
+        double r2ij = rij * rij, r2ik = rik * rik, r2jk = rjk * rjk;
+        double r0ij = 0.58 * sqrt(3.0 * P->rcov_d3[i] * P->rcov_d3[j]) + 4.80;
+        double r0ik = 0.58 * sqrt(3.0 * P->rcov_d3[i] * P->rcov_d3[k]) + 4.80;
+        double r0jk = 0.58 * sqrt(3.0 * P->rcov_d3[j] * P->rcov_d3[k]) + 4.80;
+        double rijk = rij * rik * rjk, r2ijk = r2ij * r2ik * r2jk, r3ijk = rijk * r2ijk, r5ijk = r2ijk * r3ijk;
+        double tmp = pow(r0ij * r0ik * r0jk / rijk, 14.0 / 3.0);
+        double fdmp = 1.0 / (1.0 + 6.0 * tmp);
+        double Aa = r2ij + r2jk - r2ik, Bb = r2ij + r2ik - r2jk, Cc = r2ik + r2jk - r2ij;
+        double ang = (0.375 * Aa * Bb * Cc / r2ijk + 1.0) / r3ijk;
+        double sc = P->atm_scale[t];
+        T.e[GFNFF_E_ATM] += ang * fdmp * c9 / 3.0 * sc;
+        if (want_grad) {
+            double dfdmp = -2.0 * 14.0 * tmp * fdmp * fdmp;
+            double dij = -0.375 * (pow(r2ij, 3) + pow(r2ij, 2) * (r2jk + r2ik) +
+                                   r2ij * (3.0 * pow(r2jk, 2) + 2.0 * r2jk * r2ik + 3.0 * pow(r2ik, 2)) -
+                                   5.0 * pow(r2jk - r2ik, 2) * (r2jk + r2ik)) / r5ijk;
+            double dik = -0.375 * (pow(r2ik, 3) + pow(r2ik, 2) * (r2jk + r2ij) +
+                                   r2ik * (3.0 * pow(r2jk, 2) + 2.0 * r2jk * r2ij + 3.0 * pow(r2ij, 2)) -
+                                   5.0 * pow(r2jk - r2ij, 2) * (r2jk + r2ij)) / r5ijk;
+            double djk = -0.375 * (pow(r2jk, 3) + pow(r2jk, 2) * (r2ik + r2ij) +
+                                   r2jk * (3.0 * pow(r2ik, 2) + 2.0 * r2ik * r2ij + 3.0 * pow(r2ij, 2)) -
+                                   5.0 * pow(r2ik - r2ij, 2) * (r2ik + r2ij)) / r5ijk;
+            double pre = -c9 * sc / 3.0;
+            double a1 = pre * (-dij * fdmp + ang * dfdmp) / r2ij, a2 = pre * (-dik * fdmp + ang * dfdmp) / r2ik,
+                   a3 = pre * (-djk * fdmp + ang * dfdmp) / r2jk;
+            double gi[3], gj[3], gk[3];
+            for (int c = 0; c < 3; ++c) {
+                gi[c] = -(a1 * vij[c] + a2 * vik[c]);
+                gj[c] = a1 * vij[c] - a3 * vjk[c];
+                gk[c] = a2 * vik[c] + a3 * vjk[c];
+            }
+            gadd(&G, GAT, i, gi[0], gi[1], gi[2]);
+            gadd(&G, GAT, j, gj[0], gj[1], gj[2]);
+            gadd(&G, GAT, k, gk[0], gk[1], gk[2]);
+        }
+    }
+
+    /* --- bonded ATM: calcBATM ff_workspace_gfnff.cpp:1665-1739 */
+    for (int t = 0; t < P->nbatm; ++t) {
+        int i = P->batm_ijk[3 * t], j = P->batm_ijk[3 * t + 1], k = P->batm_ijk[3 * t + 2];
+        const double* zb = P->batm_par + 3 * t;
+        double vij[3], vik[3], vjk[3];
+        sub3(x + 3 * j, x + 3 * i, vij);
+        sub3(x + 3 * k, x + 3 * i, vik);
+        sub3(x + 3 * k, x + 3 * j, vjk);
+        double r2ij = dot3(vij, vij), r2jk = dot3(vjk, vjk), r2ik = dot3(vik, vik);
+        double rij = sqrt(r2ij), rjk = sqrt(r2jk), rik = sqrt(r2ik);
+        double rijk3 = r2ij * r2jk * r2ik;
+        double mijk = -r2ij + r2jk + r2ik, imjk = r2ij - r2jk + r2ik, ijmk = r2ij + r2jk - r2ik;
+        double ang = 0.375 * ijmk * imjk * mijk / rijk3;
+        double rav3 = pow(rijk3, 1.5);
+        double angr9 = (ang + 1.0) / rav3;
+        double fi = fmin(fmax(1.0 - 3.0 * P->qa[i], -4.0), 4.0);
+        double fj = fmin(fmax(1.0 - 3.0 * P->qa[j], -4.0), 4.0);
+        double fk = fmin(fmax(1.0 - 3.0 * P->qa[k], -4.0), 4.0);
+        double c9 = fi * fj * fk * zb[0] * zb[1] * zb[2];
+        T.e[GFNFF_E_BATM] += c9 * angr9;
+        if (want_grad) {
+            double dij = -0.375 * (pow(r2ij, 3) + pow(r2ij, 2) * (r2jk + r2ik) +
+                                   r2ij * (3.0 * pow(r2jk, 2) + 2.0 * r2jk * r2ik + 3.0 * pow(r2ik, 2)) -
+                                   5.0 * pow(r2jk - r2ik, 2) * (r2jk + r2ik)) / (rij * rijk3 * rav3);
+            double djk = -0.375 * (pow(r2jk, 3) + pow(r2jk, 2) * (r2ik + r2ij) +
+                                   r2jk * (3.0 * pow(r2ik, 2) + 2.0 * r2ik * r2ij + 3.0 * pow(r2ij, 2)) -
+                                   5.0 * pow(r2ik - r2ij, 2) * (r2ik + r2ij)) / (rjk * rijk3 * rav3);
+            double dik = -0.375 * (pow(r2ik, 3) + pow(r2ik, 2) * (r2jk + r2ij) +
+                                   r2ik * (3.0 * pow(r2jk, 2) + 2.0 * r2jk * r2ij + 3.0 * pow(r2ij, 2)) -
+                                   5.0 * pow(r2jk - r2ij, 2) * (r2jk + r2ij)) / (rik * rijk3 * rav3);
+            double gij[3], gjk[3], gik[3];
+            for (int c = 0; c < 3; ++c) {
+                gij[c] = -dij * c9 * vij[c] / rij;
+                gjk[c] = -djk * c9 * vjk[c] / rjk;
+                gik[c] = -dik * c9 * vik[c] / rik;
+            }
+            gadd(&G, GBA, j, -gij[0] + gjk[0], -gij[1] + gjk[1], -gij[2] + gjk[2]);
+            gadd(&G, GBA, k, -gik[0] - gjk[0], -gik[1] - gjk[1], -gik[2] - gjk[2]);
+            gadd(&G, GBA, i, gij[0] + gik[0], gij[1] + gik[1], gij[2] + gik[2]);
+        }
+    }
+
+    /* --- CN chain rule + Coulomb term 1b: ff_workspace.cpp:459-507 */
+    if (want_grad) {
+        double* v = calloc(n, 8);
+        double* vq = calloc(n, 8);
+        for (int i = 0; i < n; ++i) {
+            double c = fmax(cn[i], 0.0);
+            vq[i] = q[i] * P->cnf[i] / (2.0 * sqrt(c) + 1e-16);
+            v[i] = dEdcn[i] - vq[i];
+        }
+        double cut2 = 36.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j) {
+                double d[3];
+                sub3(x + 3 * j, x + 3 * i, d); /* r_j - r_i */
+                double d2 = dot3(d, d);
+                if (!(d2 < cut2)) continue;
+                double r = sqrt(d2);
+                double rs = P->rcov_cn[i] + P->rcov_cn[j];
+                double dr = (r - rs) / rs;
+                double dd = (KN / SQRTPI) * exp(-KN * KN * dr * dr) / rs;
+                /* dCN_i/dr_i = -dlog_i*dd*u, dCN_j/dr_i = -dlog_j*dd*u, dCN/dr_j opposite */
+                double s = dd * (dlog[i] * v[i] + dlog[j] * v[j]);
+                double sb = dd * (dlog[i] * dEdcn_bond[i] + dlog[j] * dEdcn_bond[j]);
+                double sd = dd * (dlog[i] * (dEdcn[i] - dEdcn_bond[i]) + dlog[j] * (dEdcn[j] - dEdcn_bond[j]));
+                double sq = -dd * (dlog[i] * vq[i] + dlog[j] * vq[j]);
+                double ux = d[0] / r, uy = d[1] / r, uz = d[2] / r;
+                G.g[3 * i] -= s * ux;
+                G.g[3 * i + 1] -= s * uy;
+                G.g[3 * i + 2] -= s * uz;
+                G.g[3 * j] += s * ux;
+                G.g[3 * j + 1] += s * uy;
+                G.g[3 * j + 2] += s * uz;
+                if (G.gc[GB]) {
+                    G.gc[GB][3 * i] -= sb * ux, G.gc[GB][3 * i + 1] -= sb * uy, G.gc[GB][3 * i + 2] -= sb * uz;
+                    G.gc[GB][3 * j] += sb * ux, G.gc[GB][3 * j + 1] += sb * uy, G.gc[GB][3 * j + 2] += sb * uz;
+                    G.gc[GD][3 * i] -= sd * ux, G.gc[GD][3 * i + 1] -= sd * uy, G.gc[GD][3 * i + 2] -= sd * uz;
+                    G.gc[GD][3 * j] += sd * ux, G.gc[GD][3 * j + 1] += sd * uy, G.gc[GD][3 * j + 2] += sd * uz;
+                    G.gc[GC][3 * i] -= sq * ux, G.gc[GC][3 * i + 1] -= sq * uy, G.gc[GC][3 * i + 2] -= sq * uz;
+                    G.gc[GC][3 * j] += sq * ux, G.gc[GC][3 * j + 1] += sq * uy, G.gc[GC][3 * j + 2] += sq * uz;
+                }
+            }
+        free(v);
+        free(vq);
+    }
+
+    {
+        double tot = 0.0;
+        for (int k = 0; k < GFNFF_NTERMS; ++k) tot += T.e[k];
+        if (E_total) *E_total = tot;
+        if (terms_out) memcpy(terms_out, T.e, sizeof(T.e));
+        if (grad) memcpy(grad, g, sizeof(double) * 3 * (size_t)n);
+        if (q_out) memcpy(q_out, q, sizeof(double) * n);
+        if (cn_out) memcpy(cn_out, cn, sizeof(double) * n);
+    }
+done:
+    free(cn_raw);
+    free(cn);
+    free(dlog);
+    free(gw);
+    free(dgw);
+    free(q);
+    free(dEdcn);
+    free(dEdcn_bond);
+    free(g);
+    return status;
+}
