@@ -1,11 +1,1651 @@
-// placeholder, replaced by the kernel implementation
+// GFN-FF energy + gradient on one MI355X (gfx950), FP64 path.
+//
+// One evaluation is a fixed sequence of kernels on one stream:
+//   k_cn            CN (6 Bohr), log cap, dlogdcn, D4 weights, EEQ rhs, CN neighbour lists
+//   k_eeq_reduced   reduced EEQ matrix (fragment constraints eliminated)
+//   k_chol_*        blocked Cholesky (diag / panel / trailing update)
+//   k_trsv_*        forward/backward substitution
+//   k_charges       charges from the reduced solution, self energy, qtmp
+//   k_pairs         Coulomb + D4 dispersion + non-bonded repulsion, LDS-tiled,
+//                   one row atom per thread, full rows (no atomics)
+//   k_nbrep_fix     bonded / 1,3 / 1,4 H-H corrections of the repulsion pair sum
+//   k_bonds .. k_batm   bonded term lists writing per-atom gradient slots
+//   k_finish        slot gather + CN chain rule + gradient assembly
+//   k_reduce_terms  fixed-order energy reductions
+// Reference semantics: see oracle/gfnff_oracle.c (same formulas, same cut-offs).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
 #include "engine.hpp"
+
 namespace gfnff {
-struct DeviceEngine { int dummy; };
-DeviceEngine* device_create(const ParamSet&, int, int& status, std::string& msg) { status = 5; msg = "not built"; return nullptr; }
-void device_destroy(DeviceEngine* e) { delete e; }
-int device_eval_host(DeviceEngine*, const double*, bool, double*, double*, double*, double*, std::string& msg) { msg = "not built"; return 5; }
-int device_eval(DeviceEngine*, const double*, double*, double*, void*, std::string& msg) { msg = "not built"; return 5; }
-int device_count() { int n = 0; if (hipGetDeviceCount(&n) != hipSuccess) return 0; return n; }
+
+namespace {
+
+#define HIP_OK(x)                                                   \
+    do {                                                            \
+        hipError_t e_ = (x);                                        \
+        if (e_ != hipSuccess) {                                     \
+            msg = std::string(#x) + ": " + hipGetErrorString(e_);   \
+            return kErrDevice;                                      \
+        }                                                           \
+    } while (0)
+
+constexpr int NB = 64;          // Cholesky block size
+constexpr int MAXNB_CN = 128;   // CN neighbour list capacity per atom
+constexpr int TILE = 256;       // pair kernel tile / workgroup size
+
+constexpr double KN = -7.5;
+constexpr double CNMAX = 4.4;
+constexpr double SQRTPI_TRUNC = 1.77245385091;   // constant used by the reference's dCN
+constexpr double TSQRT2PI = 0.797884560802866;
+constexpr double TSQRT2PI_SELF = 0.797884560802865;
+
+enum Flag { FLAG_CHOL = 1, FLAG_CNOVF = 2 };
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__device__ inline double3 ld3(const double* x, int i) { return make_double3(x[3 * i], x[3 * i + 1], x[3 * i + 2]); }
+__device__ inline double3 operator-(double3 a, double3 b) { return make_double3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline double3 operator+(double3 a, double3 b) { return make_double3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ inline double3 operator*(double s, double3 a) { return make_double3(s * a.x, s * a.y, s * a.z); }
+__device__ inline double dot(double3 a, double3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline double3 cross(double3 a, double3 b) {
+    return make_double3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
+__device__ inline double nrm(double3 a) { return sqrt(dot(a, a)); }
+__device__ inline void st3(double* s, int slot, double3 v) {
+    s[3 * slot] = v.x;
+    s[3 * slot + 1] = v.y;
+    s[3 * slot + 2] = v.z;
+}
+
+// ---------------------------------------------------------------------------
+// K1: CN, log cap, dlogdcn, D4 weights, EEQ right-hand side, neighbour list
+// cn_calculator.cpp:367-428; gfnff_method.cpp:5880-5930; d4param_generator.cpp:1079-1213,1692-1768
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TILE) void k_cn(int n, const double* __restrict__ x, const double* __restrict__ rcov,
+                                             const int* __restrict__ elem, const int* __restrict__ nref,
+                                             const double* __restrict__ refcn, const double* __restrict__ chi_base,
+                                             const double* __restrict__ cnf, double* __restrict__ cn,
+                                             double* __restrict__ dlog, double* __restrict__ gw,
+                                             double* __restrict__ dgw, double* __restrict__ rhs,
+                                             int* __restrict__ nbn, int* __restrict__ nbl, int* __restrict__ flags) {
+    __shared__ double sx[TILE], sy[TILE], sz[TILE], sr[TILE];
+    int i = blockIdx.x * TILE + threadIdx.x;
+    double3 xi = i < n ? ld3(x, i) : make_double3(0, 0, 0);
+    double ri = i < n ? rcov[i] : 0.0;
+    double raw = 0.0;
+    int cnt = 0;
+    for (int t0 = 0; t0 < n; t0 += TILE) {
+        int j = t0 + threadIdx.x;
+        __syncthreads();
+        if (j < n) {
+            sx[threadIdx.x] = x[3 * j];
+            sy[threadIdx.x] = x[3 * j + 1];
+            sz[threadIdx.x] = x[3 * j + 2];
+            sr[threadIdx.x] = rcov[j];
+        }
+        __syncthreads();
+        int m = min(TILE, n - t0);
+        if (i < n) {
+            for (int jj = 0; jj < m; ++jj) {
+                int j2 = t0 + jj;
+                if (j2 == i) continue;
+                double dx = xi.x - sx[jj], dy = xi.y - sy[jj], dz = xi.z - sz[jj];
+                double d2 = dx * dx + dy * dy + dz * dz;
+                if (!(d2 < 36.0)) continue;
+                double r = sqrt(d2);
+                double rij = ri + sr[jj];
+                raw += 0.5 * (1.0 + erf(KN * (r - rij) / rij));
+                if (cnt < MAXNB_CN) nbl[(size_t)i * MAXNB_CN + cnt] = j2;
+                ++cnt;
+            }
+        }
+    }
+    if (i >= n) return;
+    if (cnt > MAXNB_CN) atomicOr(flags, FLAG_CNOVF);
+    nbn[i] = min(cnt, MAXNB_CN);
+    double c = log(1.0 + exp(CNMAX)) - log(1.0 + exp(CNMAX - raw));
+    cn[i] = c;
+    dlog[i] = exp(CNMAX) / (exp(CNMAX) + exp(raw));
+    int e = elem[i], nr = nref[e];
+    double ex[7], dex[7], norm = 0.0, dnorm = 0.0;
+#pragma unroll
+    for (int r = 0; r < 7; ++r) {
+        ex[r] = 0.0;
+        dex[r] = 0.0;
+        if (r < nr) {
+            double df = c - refcn[e * 7 + r];
+            ex[r] = exp(-4.0 * df * df);
+            dex[r] = -8.0 * df * ex[r];
+            norm += ex[r];
+            dnorm += dex[r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 7; ++r) {
+        double g = 0.0, dg = 0.0;
+        if (norm > 1e-10) {
+            if (r < nr) {
+                g = ex[r] / norm;
+                dg = (dex[r] * norm - ex[r] * dnorm) / (norm * norm);
+            }
+        } else if (r == 0 && nr > 0) {
+            g = 1.0;
+        }
+        gw[(size_t)i * 7 + r] = g;
+        dgw[(size_t)i * 7 + r] = dg;
+    }
+    rhs[i] = chi_base[i] + cnf[i] * sqrt(c);
+}
+
+// ---------------------------------------------------------------------------
+// K2: reduced EEQ matrix.  Per fragment f with atoms a_1 < ... < a_m, the
+// charges are q = qp + N p with qp = Q_f/m and columns e_a - e_last, so
+//   Ared(a,b) = A(ia,ib) - A(ia,lb) - A(la,ib) + A(la,lb)
+// with A the Coulomb matrix of eeq_solver.cpp:3571-3650.  Lower triangle of
+// the padded (np x np) matrix; the padding is the identity.
+// ---------------------------------------------------------------------------
+__device__ inline double eeq_a(int i, int j, const double* __restrict__ x, const double* __restrict__ alp,
+                               const double* __restrict__ diag) {
+    if (i == j) return diag[i];
+    double3 d = ld3(x, i) - ld3(x, j);
+    double r = nrm(d);
+    double g = 1.0 / sqrt(alp[i] + alp[j]);
+    return erf(g * r) / r;
+}
+
+__global__ __launch_bounds__(256) void k_eeq_diag(int n, const double* __restrict__ gam,
+                                                  const double* __restrict__ alp, double* __restrict__ diag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) diag[i] = gam[i] + TSQRT2PI / sqrt(alp[i]);
+}
+
+__global__ __launch_bounds__(256) void k_eeq_reduced(int nr, int np, const int* __restrict__ ra,
+                                                     const int* __restrict__ rl, const double* __restrict__ x,
+                                                     const double* __restrict__ alp,
+                                                     const double* __restrict__ diag, double* __restrict__ A) {
+    // one 64x64 lower tile per workgroup; grid enumerates tiles (bi >= bj)
+    int t = blockIdx.x;
+    int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+    while (bi * (bi + 1) / 2 > t) --bi;
+    int bj = t - bi * (bi + 1) / 2;
+    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
+        int r = bi * NB + e / NB, c = bj * NB + e % NB;
+        if (c > r) continue;
+        double v;
+        if (r >= nr || c >= nr) {
+            v = (r == c) ? 1.0 : 0.0;
+        } else {
+            int ia = ra[r], la = rl[r], ib = ra[c], lb = rl[c];
+            v = eeq_a(ia, ib, x, alp, diag) - eeq_a(ia, lb, x, alp, diag) - eeq_a(la, ib, x, alp, diag) +
+                eeq_a(la, lb, x, alp, diag);
+        }
+        A[(size_t)r * np + c] = v;
+    }
+}
+
+// reduced right-hand side: b(a) = (x - A qp)(ia) - (x - A qp)(la)
+__global__ __launch_bounds__(256) void k_eeq_rhs_qp(int n, const double* __restrict__ x,
+                                                    const double* __restrict__ alp, const double* __restrict__ diag,
+                                                    const double* __restrict__ qp, const double* __restrict__ rhs,
+                                                    double* __restrict__ r) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = 0; j < n; ++j)
+        if (qp[j] != 0.0) s += eeq_a(i, j, x, alp, diag) * qp[j];
+    r[i] = rhs[i] - s;
+}
+
+__global__ __launch_bounds__(256) void k_eeq_rhs_reduce(int nr, int np, const int* __restrict__ ra,
+                                                        const int* __restrict__ rl, const double* __restrict__ r,
+                                                        double* __restrict__ b) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= np) return;
+    b[a] = (a < nr) ? r[ra[a]] - r[rl[a]] : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// K3: blocked Cholesky, right-looking, lower, row-major with leading dim np.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_chol_diag(int np, int k0, double* __restrict__ A,
+                                                   double* __restrict__ Linv, int* __restrict__ flags) {
+    __shared__ double L[NB][NB + 1];
+    __shared__ double X[NB][NB + 1];
+    __shared__ int bad;
+    int tid = threadIdx.x;
+    if (tid == 0) bad = 0;
+    for (int e = tid; e < NB * NB; e += 256) {
+        int r = e / NB, c = e % NB;
+        L[r][c] = (c <= r) ? A[(size_t)(k0 + r) * np + k0 + c] : 0.0;
+    }
+    __syncthreads();
+    for (int k = 0; k < NB; ++k) {
+        if (tid == 0) {
+            double d = L[k][k];
+            if (!(d > 0.0)) {
+                bad = 1;
+                d = 1.0;
+            }
+            L[k][k] = sqrt(d);
+        }
+        __syncthreads();
+        double dk = L[k][k];
+        for (int r = k + 1 + tid; r < NB; r += 256) L[r][k] /= dk;
+        __syncthreads();
+        int m = NB - k - 1;
+        for (int e = tid; e < m * m; e += 256) {
+            int r = k + 1 + e / m, c = k + 1 + e % m;
+            if (c <= r) L[r][c] -= L[r][k] * L[c][k];
+        }
+        __syncthreads();
+    }
+    // inverse of the lower-triangular block, one column per thread
+    if (tid < NB) {
+        int c = tid;
+        for (int r = 0; r < NB; ++r) {
+            double s = (r == c) ? 1.0 : 0.0;
+            for (int p = c; p < r; ++p) s -= L[r][p] * X[p][c];
+            X[r][c] = (r < c) ? 0.0 : s / L[r][r];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < NB * NB; e += 256) {
+        int r = e / NB, c = e % NB;
+        if (c <= r) A[(size_t)(k0 + r) * np + k0 + c] = L[r][c];
+        Linv[(size_t)(k0 / NB) * NB * NB + e] = X[r][c];
+    }
+    if (tid == 0 && bad) atomicOr(flags, FLAG_CHOL);
+}
+
+// panel: rows below the diagonal block, L(i, k-block) = A(i, k-block) * Linv^T
+__global__ __launch_bounds__(256) void k_chol_panel(int np, int k0, double* __restrict__ A,
+                                                    const double* __restrict__ Linv) {
+    __shared__ double S[NB][NB + 1];
+    __shared__ double W[NB][NB + 1];
+    int tid = threadIdx.x;
+    int r0 = k0 + NB + blockIdx.x * NB;
+    const double* Li = Linv + (size_t)(k0 / NB) * NB * NB;
+    for (int e = tid; e < NB * NB; e += 256) {
+        int r = e / NB, c = e % NB;
+        S[r][c] = A[(size_t)(r0 + r) * np + k0 + c];
+        W[r][c] = Li[e];
+    }
+    __syncthreads();
+    // out(r, c) = sum_p S(r, p) * W(c, p), p <= c
+    int tr = tid / 16, tc = tid % 16;
+    double acc[4][4] = {};
+    for (int p = 0; p < NB; ++p) {
+        double a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a[u] = S[tr * 4 + u][p];
+            b[u] = W[tc * 4 + u][p];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) A[(size_t)(r0 + tr * 4 + u) * np + k0 + tc * 4 + v] = acc[u][v];
+}
+
+// trailing update of the lower tiles right of the panel:
+//   A(bi, bj) -= L(bi, k) L(bj, k)^T, bi >= bj > k
+__global__ __launch_bounds__(256) void k_chol_update(int np, int k0, double* __restrict__ A) {
+    __shared__ double P[NB][NB + 1];
+    __shared__ double Q[NB][NB + 1];
+    int t = blockIdx.x;
+    int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+    while (bi * (bi + 1) / 2 > t) --bi;
+    int bj = t - bi * (bi + 1) / 2;
+    int base = k0 + NB;
+    int ri = base + bi * NB, rj = base + bj * NB;
+    int tid = threadIdx.x;
+    for (int e = tid; e < NB * NB; e += 256) {
+        int r = e / NB, c = e % NB;
+        P[r][c] = A[(size_t)(ri + r) * np + k0 + c];
+        Q[r][c] = A[(size_t)(rj + r) * np + k0 + c];
+    }
+    __syncthreads();
+    int tr = tid / 16, tc = tid % 16;
+    double acc[4][4] = {};
+    for (int p = 0; p < NB; ++p) {
+        double a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a[u] = P[tr * 4 + u][p];
+            b[u] = Q[tc * 4 + u][p];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            int r = ri + tr * 4 + u, c = rj + tc * 4 + v;
+            if (c <= r) A[(size_t)r * np + c] -= acc[u][v];
+        }
+}
+
+// ---------------------------------------------------------------------------
+// K4: triangular solves with the blocked factor
+// ---------------------------------------------------------------------------
+// forward, block k: y_k = Linv_k * b_k, then b_i -= L(i, k) y_k for rows below
+__global__ __launch_bounds__(NB) void k_trsv_fwd_diag(int k0, const double* __restrict__ Linv,
+                                                      double* __restrict__ b) {
+    __shared__ double bk[NB];
+    int t = threadIdx.x;
+    bk[t] = b[k0 + t];
+    __syncthreads();
+    const double* Li = Linv + (size_t)(k0 / NB) * NB * NB;
+    double s = 0.0;
+    for (int p = 0; p <= t; ++p) s += Li[t * NB + p] * bk[p];
+    b[k0 + t] = s;
+}
+__global__ __launch_bounds__(256) void k_trsv_fwd_update(int np, int k0, const double* __restrict__ A,
+                                                         double* __restrict__ b) {
+    __shared__ double yk[NB];
+    if (threadIdx.x < NB) yk[threadIdx.x] = b[k0 + threadIdx.x];
+    __syncthreads();
+    int r = k0 + NB + blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= np) return;
+    const double* Ar = A + (size_t)r * np + k0;
+    double s = 0.0;
+    for (int p = 0; p < NB; ++p) s += Ar[p] * yk[p];
+    b[r] -= s;
+}
+// backward, block k (descending): x_k = Linv_k^T * c_k, then c_j -= L(k, j)^T x_k for j above
+__global__ __launch_bounds__(NB) void k_trsv_bwd_diag(int k0, const double* __restrict__ Linv,
+                                                      double* __restrict__ b) {
+    __shared__ double bk[NB];
+    int t = threadIdx.x;
+    bk[t] = b[k0 + t];
+    __syncthreads();
+    const double* Li = Linv + (size_t)(k0 / NB) * NB * NB;
+    double s = 0.0;
+    for (int p = t; p < NB; ++p) s += Li[p * NB + t] * bk[p];
+    b[k0 + t] = s;
+}
+__global__ __launch_bounds__(256) void k_trsv_bwd_update(int np, int k0, const double* __restrict__ A,
+                                                         double* __restrict__ b) {
+    // columns c < k0: b_c -= sum_{p in block} A(k0+p, c) x_p
+    __shared__ double xk[NB];
+    if (threadIdx.x < NB) xk[threadIdx.x] = b[k0 + threadIdx.x];
+    __syncthreads();
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= k0) return;
+    double s = 0.0;
+    for (int p = 0; p < NB; ++p) s += A[(size_t)(k0 + p) * np + c] * xk[p];
+    b[c] -= s;
+}
+
+// ---------------------------------------------------------------------------
+// K5: charges, self energy (ff_workspace.cpp:426-449) and qtmp (:459-507)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_charges(int n, const int* __restrict__ red_of,
+                                                 const int* __restrict__ last_of_frag_first,
+                                                 const int* __restrict__ frag_off,
+                                                 const int* __restrict__ frag_atoms, const int* __restrict__ frag,
+                                                 const double* __restrict__ qp, const double* __restrict__ p,
+                                                 const double* __restrict__ chi_base, const double* __restrict__ cnf,
+                                                 const double* __restrict__ gam, const double* __restrict__ alp,
+                                                 const double* __restrict__ cn, double* __restrict__ q,
+                                                 double* __restrict__ e_self, double* __restrict__ qtmp) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double qi;
+    int ri = red_of[i];
+    if (ri >= 0) {
+        qi = qp[i] + p[ri];
+    } else {  // last atom of its fragment
+        int f = frag[i];
+        double s = 0.0;
+        for (int k = frag_off[f]; k < frag_off[f + 1]; ++k) {
+            int a = frag_atoms[k];
+            int ra = red_of[a];
+            if (ra >= 0) s += p[ra];
+        }
+        qi = qp[i] - s;
+    }
+    (void)last_of_frag_first;
+    q[i] = qi;
+    double c = cn[i];
+    double chi = chi_base[i] + cnf[i] * sqrt(fmax(c, 0.0));
+    e_self[i] = -qi * chi + 0.5 * qi * qi * (gam[i] + TSQRT2PI_SELF / sqrt(alp[i]));
+    qtmp[i] = qi * cnf[i] / (2.0 * sqrt(fmax(c, 0.0)) + 1e-16);
+}
+
+// ---------------------------------------------------------------------------
+// K6: pair terms.  Row atom i per thread, j over all atoms in LDS tiles.
+//   Coulomb  (r <= 100)  ff_workspace_gfnff.cpp:850-898
+//   D4       (r <= 50)   :715-767 with C6 from the current Gaussian weights
+//   repulsion (r <= 20)  :814-842 with the element-pair factor; bonded and
+//                        1,3/1,4 H-H pairs are corrected by k_nbrep_fix
+// Each pair is evaluated from both sides; a row keeps half of each pair
+// energy and its own full gradient and dE/dCN, so no atomics are needed.
+// ---------------------------------------------------------------------------
+struct PairArgs {
+    int n, nelem;
+    const double* x;
+    const double* q;
+    const double* alp;
+    const double* gw;
+    const double* dgw;
+    const double* zeta;
+    const double* szr;
+    const double* anb;
+    const double* repz;
+    const int* Z;
+    const int* elem;
+    const int* nref;
+    const int* metal;
+    const double* c6ref;
+    double* g;        // [3n]
+    double* dEdcn;    // [n]
+    double* e_coul;   // [n]
+    double* e_disp;   // [n]
+    double* e_rep;    // [n]
+};
+
+__device__ inline double rep_ff(int zi, int zj, int mi, int mj) {
+    if (zi == 1 && zj == 1) return 0.6290;
+    if ((zi == 1 && mj > 0) || (zj == 1 && mi > 0)) return 0.85;
+    if ((zi == 1 && zj == 6) || (zj == 1 && zi == 6)) return 0.91;
+    if ((zi == 1 && zj == 8) || (zj == 1 && zi == 8)) return 1.04;
+    return 1.0;
+}
+
+__global__ __launch_bounds__(TILE) void k_pairs(PairArgs a) {
+    __shared__ double sx[TILE], sy[TILE], sz[TILE], sq[TILE], sal[TILE], szt[TILE], ssz[TILE], san[TILE],
+        srz[TILE];
+    __shared__ double sgw[TILE * 7], sdgw[TILE * 7];
+    __shared__ int sZ[TILE], sel[TILE], smt[TILE];
+    __shared__ double sc6[16 * 49];
+    const int n = a.n;
+    int i = blockIdx.x * TILE + threadIdx.x;
+    bool act = i < n;
+    int ne = a.nelem;
+    for (int e = threadIdx.x; e < ne * ne * 49 && e < 16 * 49; e += TILE) sc6[e] = a.c6ref[e];
+    double3 xi = act ? ld3(a.x, i) : make_double3(0, 0, 0);
+    double qi = act ? a.q[i] : 0.0, ali = act ? a.alp[i] : 1.0, zti = act ? a.zeta[i] : 0.0;
+    double szi = act ? a.szr[i] : 0.0, ani = act ? a.anb[i] : 0.0, rzi = act ? a.repz[i] : 0.0;
+    int zi = act ? a.Z[i] : 0, ei = act ? a.elem[i] : 0, mi = act ? a.metal[i] : 0;
+    int nri = act ? a.nref[ei] : 0;
+    double gwi[7], dgwi[7];
+#pragma unroll
+    for (int r = 0; r < 7; ++r) {
+        gwi[r] = act ? a.gw[(size_t)i * 7 + r] : 0.0;
+        dgwi[r] = act ? a.dgw[(size_t)i * 7 + r] : 0.0;
+    }
+    double gx = 0, gy = 0, gz = 0, dcn = 0, ec = 0, ed = 0, er = 0;
+    for (int t0 = 0; t0 < n; t0 += TILE) {
+        __syncthreads();
+        int j = t0 + threadIdx.x;
+        if (j < n) {
+            sx[threadIdx.x] = a.x[3 * j];
+            sy[threadIdx.x] = a.x[3 * j + 1];
+            sz[threadIdx.x] = a.x[3 * j + 2];
+            sq[threadIdx.x] = a.q[j];
+            sal[threadIdx.x] = a.alp[j];
+            szt[threadIdx.x] = a.zeta[j];
+            ssz[threadIdx.x] = a.szr[j];
+            san[threadIdx.x] = a.anb[j];
+            srz[threadIdx.x] = a.repz[j];
+            sZ[threadIdx.x] = a.Z[j];
+            sel[threadIdx.x] = a.elem[j];
+            smt[threadIdx.x] = a.metal[j];
+#pragma unroll
+            for (int r = 0; r < 7; ++r) {
+                sgw[threadIdx.x * 7 + r] = a.gw[(size_t)j * 7 + r];
+                sdgw[threadIdx.x * 7 + r] = a.dgw[(size_t)j * 7 + r];
+            }
+        }
+        __syncthreads();
+        if (!act) continue;
+        int m = min(TILE, n - t0);
+        for (int jj = 0; jj < m; ++jj) {
+            if (t0 + jj == i) continue;
+            double dx = xi.x - sx[jj], dy = xi.y - sy[jj], dz = xi.z - sz[jj];
+            double r2 = dx * dx + dy * dy + dz * dz;
+            double r = sqrt(r2);
+            double dEdr = 0.0;
+            // Coulomb
+            if (r <= 100.0 && r >= 1e-10) {
+                double gij = 1.0 / sqrt(ali + sal[jj]);
+                double gr = gij * r;
+                double ef = erf(gr);
+                double qq = qi * sq[jj];
+                ec += 0.5 * qq * ef / r;
+                dEdr += qq * (gij * exp(-gr * gr) * (2.0 / 1.772453850905516) / r - ef / r2);
+            }
+            // D4 dispersion
+            if (r <= 50.0 && r >= 1e-10) {
+                int ej = sel[jj];
+                const double* c = sc6 + (ei * ne + ej) * 49;
+                double c6 = 0.0, dci = 0.0;
+                double hj[7];
+#pragma unroll
+                for (int b = 0; b < 7; ++b) hj[b] = 0.0;
+                for (int aa = 0; aa < nri; ++aa) {
+#pragma unroll
+                    for (int b = 0; b < 7; ++b) {
+                        double cr = c[aa * 7 + b];
+                        hj[b] += gwi[aa] * cr;
+                        dci += dgwi[aa] * sgw[jj * 7 + b] * cr;
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < 7; ++b) c6 += hj[b] * sgw[jj * 7 + b];
+                double r4r2 = 3.0 * szi * ssz[jj];
+                double r0 = 0.58 * sqrt(r4r2) + 4.80;
+                double r0s = r0 * r0;
+                double zc = zti * szt[jj];
+                double r6 = r2 * r2 * r2, r06 = r0s * r0s * r0s;
+                double t6 = 1.0 / (r6 + r06), t8 = 1.0 / (r6 * r2 + r06 * r0s);
+                double sum = t6 + 2.0 * r4r2 * t8;
+                ed += 0.5 * (-c6 * sum * zc);
+                double d6 = -6.0 * r2 * r2 * t6 * t6, d8 = -8.0 * r2 * r2 * r2 * t8 * t8;
+                dEdr += -c6 * zc * (d6 + 2.0 * r4r2 * d8) * r;
+                dcn -= dci * sum * zc;
+            }
+            // non-bonded repulsion (generic pair factor)
+            if (r <= 20.0 && r >= 1e-8) {
+                double ff = rep_ff(zi, sZ[jj], mi, smt[jj]);
+                double al = sqrt(ani * san[jj]) * ff;
+                double ab = rzi * srz[jj] * 0.4270;
+                double e = ab * exp(-al * r * sqrt(r)) / r;
+                er += 0.5 * e;
+                dEdr += -e / r - 1.5 * al * sqrt(r) * e;
+            }
+            double s = dEdr / r;
+            gx += s * dx;
+            gy += s * dy;
+            gz += s * dz;
+        }
+    }
+    if (!act) return;
+    a.g[3 * i] = gx;
+    a.g[3 * i + 1] = gy;
+    a.g[3 * i + 2] = gz;
+    a.dEdcn[i] = dcn;
+    a.e_coul[i] = ec;
+    a.e_disp[i] = ed;
+    a.e_rep[i] = er;
+}
+
+// ---------------------------------------------------------------------------
+// K7: repulsion corrections over the topological neighbourhood
+// (gfnff_method.cpp:9781-9872: bonded pairs excluded, H-H 1,3 / 1,4 factors)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_nbrep_fix(int n, const double* __restrict__ x, const int* __restrict__ Z,
+                                                   const int* __restrict__ metal, const double* __restrict__ anb,
+                                                   const double* __restrict__ repz, const int* __restrict__ toff,
+                                                   const int* __restrict__ tidx, const int* __restrict__ tdist,
+                                                   double* __restrict__ g, double* __restrict__ e_rep) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double3 xi = ld3(x, i);
+    double gx = 0, gy = 0, gz = 0, er = 0;
+    for (int k = toff[i]; k < toff[i + 1]; ++k) {
+        int j = tidx[k], td = tdist[k];
+        int zi = Z[i], zj = Z[j];
+        double fgen = rep_ff(zi, zj, metal[i], metal[j]);
+        double fnew;
+        if (td == 1) fnew = 0.0;
+        else if (zi == 1 && zj == 1 && td == 2) fnew = 0.6290 * 1.4580;
+        else if (zi == 1 && zj == 1 && td == 3) fnew = 0.6290 * 0.7080;
+        else continue;
+        double3 d = xi - ld3(x, j);
+        double r = nrm(d);
+        if (r > 20.0 || r < 1e-8) continue;
+        double ab = repz[i] * repz[j] * 0.4270;
+        double base = sqrt(anb[i] * anb[j]);
+        double de = 0.0, dEdr = 0.0;
+        {
+            double al = base * fgen;
+            double e = ab * exp(-al * r * sqrt(r)) / r;
+            de -= e;
+            dEdr -= -e / r - 1.5 * al * sqrt(r) * e;
+        }
+        if (fnew != 0.0) {
+            double al = base * fnew;
+            double e = ab * exp(-al * r * sqrt(r)) / r;
+            de += e;
+            dEdr += -e / r - 1.5 * al * sqrt(r) * e;
+        }
+        er += 0.5 * de;
+        double s = dEdr / r;
+        gx += s * d.x;
+        gy += s * d.y;
+        gz += s * d.z;
+    }
+    g[3 * i] += gx;
+    g[3 * i + 1] += gy;
+    g[3 * i + 2] += gz;
+    e_rep[i] += er;
+}
+
+// ---------------------------------------------------------------------------
+// K8: bonded term lists.  Each entry writes its atom gradients to fixed slots
+// (gathered per atom in k_finish in a fixed order) and its energy to e_out.
+// ---------------------------------------------------------------------------
+// HB coordination of X-H hydrogens (ff_workspace_gfnff.cpp:46-97) and bonds (:104-180)
+__global__ __launch_bounds__(256) void k_bonds(int nb, const int* __restrict__ ij, const double* __restrict__ par,
+                                               const int* __restrict__ hb_off, const int* __restrict__ hb_B,
+                                               const int* __restrict__ slot0, const double* __restrict__ x,
+                                               const double* __restrict__ cn, const double* __restrict__ rcov_d3,
+                                               const int* __restrict__ bond_H, double* __restrict__ slots,
+                                               double* __restrict__ dcn_slot, double* __restrict__ e_out) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    int i = ij[2 * b], j = ij[2 * b + 1];
+    const double* p = par + 8 * b;
+    double3 d = ld3(x, i) - ld3(x, j);
+    double r = nrm(d);
+    double r0 = (p[0] + p[2] * cn[i] + p[1] + p[3] * cn[j] + p[4]) * p[5];
+    double dr = r - r0;
+    double a0 = p[6], al = a0;
+    int H = bond_H[b];
+    double hbcn = 0.0;
+    if (H >= 0) {
+        double3 xh = ld3(x, H);
+        for (int k = hb_off[b]; k < hb_off[b + 1]; ++k) {
+            int B = hb_B[k];
+            double3 v = ld3(x, B) - xh;
+            double rr2 = dot(v, v);
+            if (rr2 > 900.0) continue;
+            double rr = sqrt(rr2);
+            double rc = 1.78 * (4.0 / 3.0) * (rcov_d3[H] + rcov_d3[B]);
+            hbcn += 0.5 * (1.0 + erf(-27.5 * (rr - rc) / rc));
+        }
+        al = (-(1.0 - 0.9) * hbcn + 1.0) * a0;
+    }
+    double e = p[7] * exp(-al * dr * dr);
+    e_out[b] = e;
+    double dEdr = -2.0 * al * dr * e;
+    int s = slot0[b];
+    st3(slots, s, (dEdr / r) * d);
+    st3(slots, s + 1, (-dEdr / r) * d);
+    double yy = -dEdr;
+    dcn_slot[2 * b] = yy * p[5] * p[2];
+    dcn_slot[2 * b + 1] = yy * p[5] * p[3];
+    if (H >= 0) {
+        double zz = 0.1 * a0 * dr * dr * e;
+        double3 xh = ld3(x, H);
+        double3 gH = make_double3(0, 0, 0);
+        int q = s + 3;
+        for (int k = hb_off[b]; k < hb_off[b + 1]; ++k, ++q) {
+            int B = hb_B[k];
+            double3 v = ld3(x, B) - xh;
+            double rr2 = dot(v, v);
+            double3 gB = make_double3(0, 0, 0);
+            if (rr2 <= 900.0) {
+                double rr = sqrt(rr2);
+                double rc = 1.78 * (4.0 / 3.0) * (rcov_d3[H] + rcov_d3[B]);
+                double arg = -27.5 * (rr - rc) / rc;
+                double dc = 0.5641895835477563 * (-27.5 / rc) * exp(-arg * arg) / rr;
+                gH = gH + (-zz * dc) * v;
+                gB = (zz * dc) * v;
+            }
+            st3(slots, q, gB);
+        }
+        st3(slots, s + 2, gH);
+    }
+}
+
+// angles: ff_workspace_gfnff.cpp:187-275
+__global__ __launch_bounds__(256) void k_angles(int na, const int* __restrict__ ijk, const double* __restrict__ par,
+                                                const int* __restrict__ slot0, const double* __restrict__ x,
+                                                const double* __restrict__ rcov_d3, double* __restrict__ slots,
+                                                double* __restrict__ e_out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= na) return;
+    int i = ijk[3 * t], j = ijk[3 * t + 1], k = ijk[3 * t + 2];
+    double fc = par[2 * t], th0 = par[2 * t + 1];
+    double3 u = ld3(x, i) - ld3(x, j), v = ld3(x, k) - ld3(x, j);
+    double ru = nrm(u), rv = nrm(v);
+    double cth = (ru < 1e-10 || rv < 1e-10) ? 1.0 : dot(u, v) / (ru * rv);
+    cth = fmin(1.0, fmax(-1.0, cth));
+    double th = acos(cth), e, dEdcos;
+    if (fabs(M_PI - th0) < 1e-6) {
+        double dt = th - th0;
+        e = fc * dt * dt;
+        double sn = sin(th);
+        dEdcos = (sn > 1e-14) ? -2.0 * fc * dt / sn : 0.0;
+    } else {
+        double c0 = cos(th0);
+        e = fc * (cth - c0) * (cth - c0);
+        dEdcos = 2.0 * fc * (cth - c0);
+    }
+    double rci = rcov_d3[i] * (4.0 / 3.0), rcj = rcov_d3[j] * (4.0 / 3.0), rck = rcov_d3[k] * (4.0 / 3.0);
+    double rc1 = 0.595 * (rci + rcj) * (rci + rcj), rc2 = 0.595 * (rcj + rck) * (rcj + rck);
+    double r12 = ru * ru, r22 = rv * rv;
+    double rr1 = r12 / rc1, rr2 = r22 / rc2;
+    rr1 *= rr1;
+    rr2 *= rr2;
+    double d1 = 1.0 / (1.0 + rr1), d2 = 1.0 / (1.0 + rr2), damp = d1 * d2;
+    e_out[t] = e * damp;
+    double dd1 = (r12 > 1e-8) ? -4.0 * rr1 / (r12 * (1.0 + rr1) * (1.0 + rr1)) : 0.0;
+    double dd2 = (r22 > 1e-8) ? -4.0 * rr2 / (r22 * (1.0 + rr2) * (1.0 + rr2)) : 0.0;
+    double sinth = sqrt(fmax(0.0, 1.0 - cth * cth));
+    double pre = (sinth < 1e-14 || ru < 1e-10 || rv < 1e-10) ? 0.0 : dEdcos * damp;
+    double3 gi = (pre / ru) * ((1.0 / rv) * v - (cth / ru) * u) + (e * dd1 * d2) * u;
+    double3 gk = (pre / rv) * ((1.0 / ru) * u - (cth / rv) * v) + (e * dd2 * d1) * v;
+    int s = slot0[t];
+    st3(slots, s, gi);
+    st3(slots, s + 1, (-1.0) * (gi + gk));
+    st3(slots, s + 2, gk);
+}
+
+// signed dihedral and its gradient (phi as in gfnff_geometry.h:60-110)
+__device__ inline double dihed(double3 ri, double3 rj, double3 rk, double3 rl, double3 g[4], bool& ok) {
+    double3 v1 = rj - ri, v2 = rk - rj, v3 = rl - rk;
+    double3 n1 = cross(v1, v2), n2 = cross(v2, v3);
+    double nn1 = nrm(n1), nn2 = nrm(n2), nv2 = nrm(v2);
+    ok = !(nn1 < 1e-10 || nn2 < 1e-10 || nv2 < 1e-10);
+    if (!ok) {
+        for (int a = 0; a < 4; ++a) g[a] = make_double3(0, 0, 0);
+        return 0.0;
+    }
+    double c = dot(n1, n2) / (nn1 * nn2);
+    double s = nv2 * dot(n1, v3) / nn1 / nn2;
+    s = fmin(1.0, fmax(-1.0, s));
+    double phi = atan2(s, c);
+    double n1s = nn1 * nn1, n2s = nn2 * nn2, b2 = nv2 * nv2;
+    double3 gi = (-nv2 / n1s) * n1, gl = (nv2 / n2s) * n2;
+    double aa = dot(v1, v2) / b2, bb = dot(v3, v2) / b2;
+    g[0] = gi;
+    g[3] = gl;
+    g[1] = (-1.0 - aa) * gi + bb * gl;
+    g[2] = aa * gi + (-1.0 - bb) * gl;
+    return phi;
+}
+
+// torsions (primary and extra): ff_workspace_gfnff.cpp:282-482
+__global__ __launch_bounds__(256) void k_tors(int nt, const int* __restrict__ ijkl, const double* __restrict__ par,
+                                              const int* __restrict__ slot0, const double* __restrict__ x,
+                                              const double* __restrict__ rcov_d3, double* __restrict__ slots,
+                                              double* __restrict__ e_out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    int id[4] = {ijkl[4 * t], ijkl[4 * t + 1], ijkl[4 * t + 2], ijkl[4 * t + 3]};
+    const double* p = par + 4 * t;
+    double3 r[4];
+    for (int a = 0; a < 4; ++a) r[a] = ld3(x, id[a]);
+    double3 gp[4];
+    bool ok;
+    double phi = dihed(r[0], r[1], r[2], r[3], gp, ok);
+    double V = p[0], nn = p[1], phi0 = p[2], atc = p[3];
+    double c1 = nn * (phi - phi0) + M_PI;
+    double e = V * (1.0 + cos(c1));
+    double3 v[3] = {r[0] - r[1], r[1] - r[2], r[2] - r[3]};
+    double dmp[3], rr[3], r2[3];
+    for (int s = 0; s < 3; ++s) {
+        r2[s] = dot(v[s], v[s]);
+        double rs = rcov_d3[id[s]] * (4.0 / 3.0) + rcov_d3[id[s + 1]] * (4.0 / 3.0);
+        double rc = atc * rs * rs;
+        rr[s] = r2[s] / rc;
+        rr[s] *= rr[s];
+        dmp[s] = 1.0 / (1.0 + rr[s]);
+    }
+    double damp = dmp[0] * dmp[1] * dmp[2];
+    e_out[t] = e * damp;
+    double dEdphi = ok ? -V * nn * sin(c1) * damp : 0.0;
+    double dd[3];
+    for (int s = 0; s < 3; ++s) dd[s] = (r2[s] > 1e-8) ? -4.0 * rr[s] / (r2[s] * (1.0 + rr[s]) * (1.0 + rr[s])) : 0.0;
+    double f0 = e * dd[0] * dmp[1] * dmp[2], f1 = e * dmp[0] * dd[1] * dmp[2], f2 = e * dmp[0] * dmp[1] * dd[2];
+    int s = slot0[t];
+    st3(slots, s, dEdphi * gp[0] + f0 * v[0]);
+    st3(slots, s + 1, dEdphi * gp[1] + (-f0) * v[0] + f1 * v[1]);
+    st3(slots, s + 2, dEdphi * gp[2] + (-f1) * v[1] + f2 * v[2]);
+    st3(slots, s + 3, dEdphi * gp[3] + (-f2) * v[2]);
+}
+
+// bonded repulsion: ff_workspace_gfnff.cpp:774-807
+__global__ __launch_bounds__(256) void k_brep(int nb, const int* __restrict__ ij, const double* __restrict__ par,
+                                              const int* __restrict__ slot0, const double* __restrict__ x,
+                                              double* __restrict__ slots, double* __restrict__ e_out) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    int i = ij[2 * b], j = ij[2 * b + 1];
+    double3 d = ld3(x, i) - ld3(x, j);
+    double r = nrm(d);
+    double e = 0.0, dEdr = 0.0;
+    if (!(r > 20.0 || r < 1e-8)) {
+        double al = par[2 * b], ab = par[2 * b + 1];
+        e = ab * exp(-al * r * sqrt(r)) / r;
+        dEdr = -e / r - 1.5 * al * sqrt(r) * e;
+    }
+    e_out[b] = e;
+    int s = slot0[b];
+    st3(slots, s, (dEdr / r) * d);
+    st3(slots, s + 1, (-dEdr / r) * d);
+}
+
+// hydrogen bonds, cases 1 and 2: ff_workspace_gfnff.cpp:938-1404
+__global__ __launch_bounds__(256) void k_hbonds(int nh, const int* __restrict__ ahb, const int* __restrict__ hcase,
+                                                const double* __restrict__ hp, const int* __restrict__ nb_off,
+                                                const int* __restrict__ nb_idx, const int* __restrict__ slot0,
+                                                const double* __restrict__ x, const double* __restrict__ hb_rad,
+                                                const int* __restrict__ Z, double* __restrict__ slots,
+                                                double* __restrict__ e_out) {
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    int A = ahb[3 * h], H = ahb[3 * h + 1], B = ahb[3 * h + 2];
+    int cs = hcase[h];
+    const double* q = hp + 7 * h;
+    double basA = q[0], basB = q[1], aciA = q[2], aciB = q[3];
+    double3 pA = ld3(x, A), pH = ld3(x, H), pB = ld3(x, B);
+    double rAH = nrm(pH - pA), rHB = nrm(pB - pH), rAB = nrm(pB - pA);
+    int s0 = slot0[h];
+    int nb0 = nb_off[h], nb1 = nb_off[h + 1];
+    double3 z3 = make_double3(0, 0, 0);
+    double3 ga = z3, gb = z3, gh = z3;
+    double E = 0.0;
+    if (rAB <= 50.0) {
+        double rAH4 = rAH * rAH * rAH * rAH, rHB4 = rHB * rHB * rHB * rHB;
+        double den = 1.0 / (rAH4 + rHB4);
+        double eH = exp(15.0 * q[4]), eA = exp(15.0 * -q[5]), eB = exp(15.0 * -q[6]);
+        double QH = eH / (eH + 1.0), QA = eA / (eA + 1.0), QB = eB / (eB + 1.0);
+        double bas = (QA * basA * rAH4 + QB * basB * rHB4) * den;
+        double aci = (aciB * rAH4 + aciA * rHB4) * den;
+        double radab = hb_rad[A] + hb_rad[B];
+        double dsh = 1.0 / (1.0 + pow(22.0 * radab / (rAB * rAB), 6.0));
+        double dlg = 1.0 / (1.0 + pow(rAB * rAB / 85.0, 6.0));
+        double denv = dsh * dlg;
+        double expo0 = (49.0 / radab) * ((rAH + rHB) / rAB - 1.0);
+        double doutl = (expo0 > 15.0) ? 0.0 : 2.0 / (1.0 + exp(expo0));
+        double outl_nb = 1.0;
+        double nbcut = (Z[B] == 7 && nb1 - nb0 == 1) ? 2.0 : 11.20;
+        if (cs >= 2)
+            for (int k = nb0; k < nb1; ++k) {
+                double3 pn = ld3(x, nb_idx[k]);
+                double ex = (nbcut / radab) * ((nrm(pn - pA) + nrm(pn - pB)) / rAB - 1.0);
+                outl_nb *= (2.0 / (1.0 + exp(-ex)) - 1.0);
+            }
+        double rdamp = (cs >= 2) ? denv * (1.8 / (rHB * rHB * rHB) - 0.8 / (rAB * rAB * rAB)) : denv / (rAB * rAB * rAB);
+        double qhoutl = QH * doutl * outl_nb;
+        double gscale = (cs == 2 || cs == 4) ? 0.268 : 1.0;
+        double cval = aciA * basB * QA * QB * gscale;
+        E = (cs >= 2) ? -rdamp * qhoutl * cval : -bas * aci * rdamp * qhoutl;
+        double3 drab = pA - pB, drah = pA - pH, drbh = pB - pH;
+        double rab2 = rAB * rAB, rbh2 = rHB * rHB, rah2 = rAH * rAH;
+        double rahprbh = rAH + rHB + 1e-12;
+        double ratio1 = pow(rab2 / 85.0, 6.0), ratio3 = pow(22.0 * radab / rab2, 6.0);
+        double ddamp = (-2.0 * 6.0 * ratio1 / (1.0 + ratio1)) + (2.0 * 6.0 * ratio3 / (1.0 + ratio3));
+        double expo = (49.0 / radab) * (rahprbh / rAB - 1.0);
+        if (!(expo > 15.0)) {
+            double ratio2 = exp(expo);
+            if (cs >= 2) {
+                double rbhdamp = denv * 1.8 / (rbh2 * rHB), rabdamp = denv * -0.8 / (rab2 * rAB);
+                double dterm = -qhoutl * cval, aterm = -rdamp * QH * outl_nb * cval;
+                double nbterm = -rdamp * QH * doutl * cval;
+                double gi = ((rabdamp + rbhdamp) * ddamp - 3.0 * rabdamp) / rab2 * dterm;
+                ga = gi * drab;
+                gb = (-gi) * drab;
+                gi = -3.0 * rbhdamp / rbh2 * dterm;
+                gb = gb + gi * drbh;
+                gh = (-gi) * drbh;
+                double tmp1 = -2.0 * aterm * ratio2 * expo / ((1.0 + ratio2) * (1.0 + ratio2)) / (rahprbh - rAB);
+                gi = -tmp1 * rahprbh / rab2;
+                ga = ga + gi * drab;
+                gb = gb + (-gi) * drab;
+                double3 dga = (tmp1 / rAH) * drah, dgb = (tmp1 / rHB) * drbh;
+                ga = ga + dga;
+                gb = gb + dgb;
+                gh = gh + (-1.0) * (dga + dgb);
+                int q2 = s0 + 3;
+                for (int k = nb0; k < nb1; ++k, ++q2) {
+                    double3 pn = ld3(x, nb_idx[k]);
+                    double3 dranb = pA - pn, drbnb = pB - pn;
+                    double ranb = nrm(dranb), rbnb = nrm(drbnb), sm = ranb + rbnb + 1e-12;
+                    double ex = (nbcut / radab) * (sm / rAB - 1.0);
+                    double r2n = exp(-ex);
+                    double oi = 2.0 / (1.0 + r2n) - 1.0;
+                    double others = (fabs(oi) > 1e-12) ? outl_nb / oi : 1.0;
+                    double tmp2 = 2.0 * nbterm * others * r2n * ex / ((1.0 + r2n) * (1.0 + r2n)) / (sm - rAB);
+                    double gi2 = -tmp2 * sm / rab2;
+                    ga = ga + gi2 * drab;
+                    gb = gb + (-gi2) * drab;
+                    double3 da = (tmp2 / ranb) * dranb, db = (tmp2 / rbnb) * drbnb;
+                    ga = ga + da;
+                    gb = gb + db;
+                    st3(slots, q2, (-1.0) * (da + db));
+                }
+            } else {
+                double caa = QA * basA, cbb = QB * basB;
+                double rterm = -aci * rdamp * qhoutl, dterm = -aci * bas * qhoutl, sterm = -rdamp * bas * qhoutl;
+                double aterm = -aci * bas * rdamp * QH;
+                double tmp = den * den * 4.0;
+                double dd24a = rah2 * rHB4 * tmp, dd24b = rbh2 * rAH4 * tmp;
+                ga = ((caa - cbb) * dd24a * rterm) * drah;
+                gb = ((cbb - caa) * dd24b * rterm) * drbh;
+                gh = (-1.0) * (ga + gb);
+                double3 dga = ((aciB - aciA) * dd24a * sterm) * drah, dgb = ((aciA - aciB) * dd24b * sterm) * drbh;
+                ga = ga + dga;
+                gb = gb + dgb;
+                gh = gh + (-1.0) * (dga + dgb);
+                double gi = rdamp * (ddamp - 3.0) / rab2;
+                ga = ga + (gi * dterm) * drab;
+                gb = gb + (-gi * dterm) * drab;
+                gi = aterm * 2.0 * ratio2 * expo * rahprbh / ((1.0 + ratio2) * (1.0 + ratio2)) / (rahprbh - rAB) / rab2;
+                ga = ga + gi * drab;
+                gb = gb + (-gi) * drab;
+                double to = -2.0 * aterm * ratio2 * expo / ((1.0 + ratio2) * (1.0 + ratio2)) / (rahprbh - rAB);
+                double3 da = (to / rAH) * drah, db = (to / rHB) * drbh;
+                ga = ga + da;
+                gb = gb + db;
+                gh = gh + (-1.0) * (da + db);
+            }
+        } else if (cs >= 2) {
+            int q2 = s0 + 3;
+            for (int k = nb0; k < nb1; ++k, ++q2) st3(slots, q2, z3);
+        }
+    } else if (cs >= 2) {
+        int q2 = s0 + 3;
+        for (int k = nb0; k < nb1; ++k, ++q2) st3(slots, q2, z3);
+    }
+    e_out[h] = E;
+    st3(slots, s0, ga);
+    st3(slots, s0 + 1, gh);
+    st3(slots, s0 + 2, gb);
+}
+
+// D4 ATM over bonded triplets (energy +c9, gradient -c9 as in
+// ff_workspace_gfnff.cpp:1526-1658) and bonded ATM (:1665-1739)
+__device__ inline double c6_pair(int i, int j, int ne, const int* elem, const int* nref, const double* c6ref,
+                                 const double* gw) {
+    int ei = elem[i], ej = elem[j];
+    const double* c = c6ref + ((size_t)ei * ne + ej) * 49;
+    double s = 0.0;
+    for (int a = 0; a < nref[ei]; ++a)
+        for (int b = 0; b < nref[ej]; ++b) s += gw[(size_t)i * 7 + a] * gw[(size_t)j * 7 + b] * c[a * 7 + b];
+    return s;
+}
+
+__global__ __launch_bounds__(256) void k_atm(int nt, const int* __restrict__ ijk, const double* __restrict__ sc,
+                                             const int* __restrict__ slot0, const double* __restrict__ x,
+                                             const double* __restrict__ rcov_d3, int ne, const int* __restrict__ elem,
+                                             const int* __restrict__ nref, const double* __restrict__ c6ref,
+                                             const double* __restrict__ gw, double* __restrict__ slots,
+                                             double* __restrict__ e_out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    int i = ijk[3 * t], j = ijk[3 * t + 1], k = ijk[3 * t + 2];
+    double c9 = sqrt(fabs(c6_pair(i, j, ne, elem, nref, c6ref, gw) * c6_pair(i, k, ne, elem, nref, c6ref, gw) *
+                          c6_pair(j, k, ne, elem, nref, c6ref, gw)));
+    double3 vij = ld3(x, j) - ld3(x, i), vik = ld3(x, k) - ld3(x, i), vjk = ld3(x, k) - ld3(x, j);
+    double rij = nrm(vij), rik = nrm(vik), rjk = nrm(vjk);
+    double r2ij = rij * rij, r2ik = rik * rik, r2jk = rjk * rjk;
+    double r0ij = 0.58 * sqrt(3.0 * rcov_d3[i] * rcov_d3[j]) + 4.80;
+    double r0ik = 0.58 * sqrt(3.0 * rcov_d3[i] * rcov_d3[k]) + 4.80;
+    double r0jk = 0.58 * sqrt(3.0 * rcov_d3[j] * rcov_d3[k]) + 4.80;
+    double rijk = rij * rik * rjk, r2ijk = r2ij * r2ik * r2jk, r3ijk = rijk * r2ijk, r5ijk = r2ijk * r3ijk;
+    double tmp = pow(r0ij * r0ik * r0jk / rijk, 14.0 / 3.0);
+    double fdmp = 1.0 / (1.0 + 6.0 * tmp);
+    double Aa = r2ij + r2jk - r2ik, Bb = r2ij + r2ik - r2jk, Cc = r2ik + r2jk - r2ij;
+    double ang = (0.375 * Aa * Bb * Cc / r2ijk + 1.0) / r3ijk;
+    double s = sc[t];
+    e_out[t] = ang * fdmp * c9 / 3.0 * s;
+    double dfdmp = -2.0 * 14.0 * tmp * fdmp * fdmp;
+    double dij = -0.375 * (pow(r2ij, 3) + pow(r2ij, 2) * (r2jk + r2ik) +
+                           r2ij * (3.0 * pow(r2jk, 2) + 2.0 * r2jk * r2ik + 3.0 * pow(r2ik, 2)) -
+                           5.0 * pow(r2jk - r2ik, 2) * (r2jk + r2ik)) / r5ijk;
+    double dik = -0.375 * (pow(r2ik, 3) + pow(r2ik, 2) * (r2jk + r2ij) +
+                           r2ik * (3.0 * pow(r2jk, 2) + 2.0 * r2jk * r2ij + 3.0 * pow(r2ij, 2)) -
+                           5.0 * pow(r2jk - r2ij, 2) * (r2jk + r2ij)) / r5ijk;
+    double djk = -0.375 * (pow(r2jk, 3) + pow(r2jk, 2) * (r2ik + r2ij) +
+                           r2jk * (3.0 * pow(r2ik, 2) + 2.0 * r2ik * r2ij + 3.0 * pow(r2ij, 2)) -
+                           5.0 * pow(r2ik - r2ij, 2) * (r2ik + r2ij)) / r5ijk;
+    double pre = -c9 * s / 3.0;
+    double a1 = pre * (-dij * fdmp + ang * dfdmp) / r2ij, a2 = pre * (-dik * fdmp + ang * dfdmp) / r2ik,
+           a3 = pre * (-djk * fdmp + ang * dfdmp) / r2jk;
+    int q = slot0[t];
+    st3(slots, q, (-1.0) * (a1 * vij + a2 * vik));
+    st3(slots, q + 1, a1 * vij + (-a3) * vjk);
+    st3(slots, q + 2, a2 * vik + a3 * vjk);
+}
+
+__global__ __launch_bounds__(256) void k_batm(int nt, const int* __restrict__ ijk, const double* __restrict__ zb,
+                                              const int* __restrict__ slot0, const double* __restrict__ x,
+                                              const double* __restrict__ qa, double* __restrict__ slots,
+                                              double* __restrict__ e_out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    int i = ijk[3 * t], j = ijk[3 * t + 1], k = ijk[3 * t + 2];
+    double3 vij = ld3(x, j) - ld3(x, i), vik = ld3(x, k) - ld3(x, i), vjk = ld3(x, k) - ld3(x, j);
+    double r2ij = dot(vij, vij), r2jk = dot(vjk, vjk), r2ik = dot(vik, vik);
+    double rij = sqrt(r2ij), rjk = sqrt(r2jk), rik = sqrt(r2ik);
+    double rijk3 = r2ij * r2jk * r2ik;
+    double mijk = -r2ij + r2jk + r2ik, imjk = r2ij - r2jk + r2ik, ijmk = r2ij + r2jk - r2ik;
+    double ang = 0.375 * ijmk * imjk * mijk / rijk3;
+    double rav3 = pow(rijk3, 1.5);
+    double angr9 = (ang + 1.0) / rav3;
+    double fi = fmin(fmax(1.0 - 3.0 * qa[i], -4.0), 4.0);
+    double fj = fmin(fmax(1.0 - 3.0 * qa[j], -4.0), 4.0);
+    double fk = fmin(fmax(1.0 - 3.0 * qa[k], -4.0), 4.0);
+    double c9 = fi * fj * fk * zb[3 * t] * zb[3 * t + 1] * zb[3 * t + 2];
+    e_out[t] = c9 * angr9;
+    double dij = -0.375 * (pow(r2ij, 3) + pow(r2ij, 2) * (r2jk + r2ik) +
+                           r2ij * (3.0 * pow(r2jk, 2) + 2.0 * r2jk * r2ik + 3.0 * pow(r2ik, 2)) -
+                           5.0 * pow(r2jk - r2ik, 2) * (r2jk + r2ik)) / (rij * rijk3 * rav3);
+    double djk = -0.375 * (pow(r2jk, 3) + pow(r2jk, 2) * (r2ik + r2ij) +
+                           r2jk * (3.0 * pow(r2ik, 2) + 2.0 * r2ik * r2ij + 3.0 * pow(r2ij, 2)) -
+                           5.0 * pow(r2ik - r2ij, 2) * (r2ik + r2ij)) / (rjk * rijk3 * rav3);
+    double dik = -0.375 * (pow(r2ik, 3) + pow(r2ik, 2) * (r2jk + r2ij) +
+                           r2ik * (3.0 * pow(r2jk, 2) + 2.0 * r2jk * r2ij + 3.0 * pow(r2ij, 2)) -
+                           5.0 * pow(r2jk - r2ij, 2) * (r2jk + r2ij)) / (rik * rijk3 * rav3);
+    double3 gij = (-dij * c9 / rij) * vij, gjk = (-djk * c9 / rjk) * vjk, gik = (-dik * c9 / rik) * vik;
+    int q = slot0[t];
+    st3(slots, q, gij + gik);                 // i
+    st3(slots, q + 1, (-1.0) * gij + gjk);    // j
+    st3(slots, q + 2, (-1.0) * (gik + gjk));  // k
+}
+
+// ---------------------------------------------------------------------------
+// K9: gradient assembly: slot gather, bond dE/dCN gather, CN chain rule with
+// Coulomb term 1b (ff_workspace.cpp:459-507), fixed order per atom.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dcn_total(int n, const int* __restrict__ doff, const int* __restrict__ didx,
+                                                   const double* __restrict__ dcn_slot, double* __restrict__ dEdcn,
+                                                   const double* __restrict__ qtmp, double* __restrict__ v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = dEdcn[i];
+    for (int k = doff[i]; k < doff[i + 1]; ++k) s += dcn_slot[didx[k]];
+    dEdcn[i] = s;
+    v[i] = s - qtmp[i];
+}
+
+__global__ __launch_bounds__(256) void k_finish(int n, const double* __restrict__ x, const double* __restrict__ rcov,
+                                                const double* __restrict__ dlog, const double* __restrict__ v,
+                                                const int* __restrict__ nbn, const int* __restrict__ nbl,
+                                                const int* __restrict__ soff, const int* __restrict__ sidx,
+                                                const double* __restrict__ slots, const double* __restrict__ gpair,
+                                                double* __restrict__ grad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double3 g = ld3(gpair, i);
+    for (int k = soff[i]; k < soff[i + 1]; ++k) g = g + ld3(slots, sidx[k]);
+    double3 xi = ld3(x, i);
+    double ai = dlog[i] * v[i];
+    for (int k = 0; k < nbn[i]; ++k) {
+        int j = nbl[(size_t)i * MAXNB_CN + k];
+        double3 d = ld3(x, j) - xi;
+        double r = nrm(d);
+        double rs = rcov[i] + rcov[j];
+        double dr = (r - rs) / rs;
+        double dd = (KN / SQRTPI_TRUNC) * exp(-KN * KN * dr * dr) / rs;
+        double s = dd * (ai + dlog[j] * v[j]);
+        g = g + (-s / r) * d;
+    }
+    grad[3 * i] = g.x;
+    grad[3 * i + 1] = g.y;
+    grad[3 * i + 2] = g.z;
+}
+
+// fixed-shape reduction of up to 16 segments into out[seg]; one workgroup
+// per segment, strided partial sums then a tree in LDS
+struct Segs {
+    const double* p[16];
+    int n[16];
+    int dst[16];
+};
+__global__ __launch_bounds__(256) void k_reduce(Segs s, double* __restrict__ out) {
+    __shared__ double red[256];
+    int sg = blockIdx.x;
+    const double* p = s.p[sg];
+    int n = s.n[sg];
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < n; k += 256) acc += p[k];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[s.dst[sg]] = red[0];
+}
+
+// out[0] = total, out[1..15] = terms (term index + 1), fixed summation order
+__global__ void k_total(const double* __restrict__ parts, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double t[GFNFF_NTERMS];
+    for (int k = 0; k < GFNFF_NTERMS; ++k) t[k] = 0.0;
+    // parts: 0 bond,1 angle,2 tors,3 inv,4 disp(pairs),5 brep,6 nbrep(pairs incl fix),7 coul pairs,
+    // 8 self,9 hb,10 atm,11 batm
+    t[GFNFF_E_BOND] = parts[0];
+    t[GFNFF_E_ANGLE] = parts[1];
+    t[GFNFF_E_TORSION] = parts[2];
+    t[GFNFF_E_INVERSION] = parts[3];
+    t[GFNFF_E_DISPERSION] = parts[4];
+    t[GFNFF_E_BONDED_REP] = parts[5];
+    t[GFNFF_E_NONBONDED_REP] = parts[6];
+    t[GFNFF_E_COULOMB] = parts[7] + parts[8];
+    t[GFNFF_E_HBOND] = parts[9];
+    t[GFNFF_E_ATM] = parts[10];
+    t[GFNFF_E_BATM] = parts[11];
+    double tot = 0.0;
+    for (int k = 0; k < GFNFF_NTERMS; ++k) tot += t[k];
+    out[0] = tot;
+    for (int k = 0; k < GFNFF_NTERMS; ++k) out[1 + k] = t[k];
+}
+
+__global__ void k_copy(int n, const double* __restrict__ a, double* __restrict__ b) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+__global__ void k_fill_i(int n, int* __restrict__ a, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct DeviceEngine {
+    int n = 0, nelem = 0, nfrag = 0;
+    int nr = 0, np = 0;                 // reduced EEQ size, padded size
+    int nbond = 0, nangle = 0, ntors = 0, nbrep = 0, nhb = 0, natm = 0, nbatm = 0;
+    int nslots = 0;
+    bool charged = false;
+    hipStream_t own = nullptr;
+    std::vector<void*> bufs;
+    // parameters (device)
+    int *Z, *elem, *nref, *metal, *frag, *toff, *tidx, *tdist;
+    double *chi_base, *gam, *alp, *cnf, *rcov_cn, *rcov_d3, *hb_rad, *qa, *zeta, *szr, *anb, *repz, *refcn, *c6ref;
+    int *red_atom, *red_last, *red_of, *frag_off, *frag_atoms;
+    double* qp;
+    int *bond_ij, *bond_H, *bond_hboff, *bond_hbB, *bond_slot;
+    double* bond_par;
+    int *ang_ijk, *ang_slot;
+    double* ang_par;
+    int *tor_ijkl, *tor_slot;
+    double* tor_par;
+    int *brep_ij, *brep_slot;
+    double* brep_par;
+    int *hb_ahb, *hb_case, *hb_nboff, *hb_nbidx, *hb_slot;
+    double* hb_par;
+    int *atm_ijk, *atm_slot;
+    double* atm_sc;
+    int *batm_ijk, *batm_slot;
+    double* batm_zb;
+    int *slot_off, *slot_idx, *dcn_off, *dcn_idx;
+    // work
+    double *x, *cn, *dlog, *gw, *dgw, *rhs, *Ared, *Linv, *b, *rres, *q, *eself, *qtmp, *gpair, *dEdcn, *vchain,
+        *slots, *dcn_slot, *grad, *e_coul, *e_disp, *e_rep, *e_bond, *e_ang, *e_tor, *e_brep, *e_hb, *e_atm,
+        *e_batm, *parts, *out, *diag;
+    int *nbn, *nbl, *flags;
+
+    template <class T>
+    T* alloc(size_t count) {
+        void* p = nullptr;
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        if (hipMalloc(&p, bytes) != hipSuccess) throw std::bad_alloc();
+        bufs.push_back(p);
+        return (T*)p;
+    }
+    template <class T>
+    T* upload(const std::vector<T>& v) {
+        T* d = alloc<T>(v.size());
+        if (!v.empty() && hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            throw std::runtime_error("upload failed");
+        return d;
+    }
+    ~DeviceEngine() {
+        if (own) {
+            (void)hipStreamSynchronize(own);
+            (void)hipStreamDestroy(own);
+        }
+        for (void* p : bufs) (void)hipFree(p);
+    }
+};
+
+DeviceEngine* device_create(const ParamSet& P, int precision, int& status, std::string& msg) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        status = kErrDevice;
+        msg = "no HIP device available";
+        return nullptr;
+    }
+    if (precision != 0) {
+        status = kErrUnsupported;
+        msg = "mixed precision path not built yet";
+        return nullptr;
+    }
+    if (!P.inv_par.empty()) {
+        status = kErrUnsupported;
+        msg = "inversion terms are not built on the device yet";
+        return nullptr;
+    }
+    for (int c : P.hb_case)
+        if (c > 2) {
+            status = kErrUnsupported;
+            msg = "hydrogen-bond cases 3/4 are not built on the device yet";
+            return nullptr;
+        }
+    auto* E = new DeviceEngine();
+    try {
+        const int n = P.n;
+        E->n = n;
+        E->nelem = P.nelem;
+        E->nfrag = P.nfrag;
+        if (P.nelem * P.nelem * 49 > 16 * 49) {
+            status = kErrUnsupported;
+            msg = "more than 4 distinct elements are not supported by the pair kernel yet";
+            delete E;
+            return nullptr;
+        }
+        E->Z = E->upload(P.Z);
+        E->elem = E->upload(P.elem);
+        E->nref = E->upload(P.nref);
+        E->metal = E->upload(P.metal);
+        E->frag = E->upload(P.frag);
+        E->toff = E->upload(P.topo_off);
+        E->tidx = E->upload(P.topo_idx);
+        E->tdist = E->upload(P.topo_dist);
+        E->chi_base = E->upload(P.chi_base);
+        E->gam = E->upload(P.gam_eff);
+        E->alp = E->upload(P.alpeeq);
+        E->cnf = E->upload(P.cnf);
+        E->rcov_cn = E->upload(P.rcov_cn);
+        E->rcov_d3 = E->upload(P.rcov_d3);
+        E->hb_rad = E->upload(P.hb_rad);
+        E->qa = E->upload(P.qa);
+        E->zeta = E->upload(P.zeta);
+        E->szr = E->upload(P.sqrt_zr4r2);
+        E->anb = E->upload(P.rep_anb);
+        E->repz = E->upload(P.repz);
+        E->refcn = E->upload(P.refcn);
+        E->c6ref = E->upload(P.c6ref);
+
+        // fragment null-space reduction
+        std::vector<std::vector<int>> fa(P.nfrag);
+        for (int i = 0; i < n; ++i) fa[P.frag[i]].push_back(i);
+        std::vector<int> red_atom, red_last, red_of(n, -1), foff(P.nfrag + 1, 0), fatoms;
+        std::vector<double> qp(n, 0.0);
+        for (int f = 0; f < P.nfrag; ++f) {
+            int m = (int)fa[f].size();
+            int last = fa[f].back();
+            for (int k = 0; k + 1 < m; ++k) {
+                red_of[fa[f][k]] = (int)red_atom.size();
+                red_atom.push_back(fa[f][k]);
+                red_last.push_back(last);
+            }
+            for (int a : fa[f]) {
+                qp[a] = P.qfrag[f] / m;
+                fatoms.push_back(a);
+            }
+            foff[f + 1] = (int)fatoms.size();
+            if (P.qfrag[f] != 0.0) E->charged = true;
+        }
+        E->nr = (int)red_atom.size();
+        E->np = std::max(NB, cdiv(std::max(E->nr, 1), NB) * NB);
+        E->red_atom = E->upload(red_atom);
+        E->red_last = E->upload(red_last);
+        E->red_of = E->upload(red_of);
+        E->frag_off = E->upload(foff);
+        E->frag_atoms = E->upload(fatoms);
+        E->qp = E->upload(qp);
+
+        // term lists and gradient slots
+        int slot = 0;
+        std::vector<int> slot_atom;
+        auto add_slot = [&](int atom) {
+            slot_atom.push_back(atom);
+            return slot++;
+        };
+        // bonds: i, j, (H, B...) for HB-modulated bonds
+        E->nbond = (int)P.bond_hbH.size();
+        std::vector<int> bhoff(E->nbond + 1, 0), bhB, bslot(E->nbond);
+        {
+            std::vector<std::vector<int>> byH(n);
+            for (size_t k = 0; k + 1 < P.hbcn_pair.size(); k += 2) byH[P.hbcn_pair[k]].push_back(P.hbcn_pair[k + 1]);
+            for (int b = 0; b < E->nbond; ++b) {
+                int H = P.bond_hbH[b];
+                bslot[b] = add_slot(P.bond_ij[2 * b]);
+                add_slot(P.bond_ij[2 * b + 1]);
+                if (H >= 0) {
+                    add_slot(H);
+                    for (int B : byH[H]) {
+                        bhB.push_back(B);
+                        add_slot(B);
+                    }
+                }
+                bhoff[b + 1] = (int)bhB.size();
+            }
+        }
+        E->bond_ij = E->upload(P.bond_ij);
+        E->bond_par = E->upload(P.bond_par);
+        E->bond_H = E->upload(P.bond_hbH);
+        E->bond_hboff = E->upload(bhoff);
+        E->bond_hbB = E->upload(bhB);
+        E->bond_slot = E->upload(bslot);
+        E->nangle = (int)(P.angle_par.size() / 2);
+        std::vector<int> aslot(E->nangle);
+        for (int t = 0; t < E->nangle; ++t) {
+            aslot[t] = add_slot(P.angle_ijk[3 * t]);
+            add_slot(P.angle_ijk[3 * t + 1]);
+            add_slot(P.angle_ijk[3 * t + 2]);
+        }
+        E->ang_ijk = E->upload(P.angle_ijk);
+        E->ang_par = E->upload(P.angle_par);
+        E->ang_slot = E->upload(aslot);
+        E->ntors = (int)(P.tors_par.size() / 4);
+        std::vector<int> tslot(E->ntors);
+        for (int t = 0; t < E->ntors; ++t) {
+            tslot[t] = add_slot(P.tors_ijkl[4 * t]);
+            for (int a = 1; a < 4; ++a) add_slot(P.tors_ijkl[4 * t + a]);
+        }
+        E->tor_ijkl = E->upload(P.tors_ijkl);
+        E->tor_par = E->upload(P.tors_par);
+        E->tor_slot = E->upload(tslot);
+        E->nbrep = (int)(P.brep_par.size() / 2);
+        std::vector<int> rslot(E->nbrep);
+        for (int b = 0; b < E->nbrep; ++b) {
+            rslot[b] = add_slot(P.brep_ij[2 * b]);
+            add_slot(P.brep_ij[2 * b + 1]);
+        }
+        E->brep_ij = E->upload(P.brep_ij);
+        E->brep_par = E->upload(P.brep_par);
+        E->brep_slot = E->upload(rslot);
+        E->nhb = (int)P.hb_case.size();
+        std::vector<int> hslot(E->nhb);
+        for (int h = 0; h < E->nhb; ++h) {
+            hslot[h] = add_slot(P.hb_ahb[3 * h]);
+            add_slot(P.hb_ahb[3 * h + 1]);
+            add_slot(P.hb_ahb[3 * h + 2]);
+            for (int k = P.hb_nbB_off[h]; k < P.hb_nbB_off[h + 1]; ++k) add_slot(P.hb_nbB_idx[k]);
+        }
+        E->hb_ahb = E->upload(P.hb_ahb);
+        E->hb_case = E->upload(P.hb_case);
+        E->hb_par = E->upload(P.hb_par);
+        E->hb_nboff = E->upload(P.hb_nbB_off);
+        E->hb_nbidx = E->upload(P.hb_nbB_idx);
+        E->hb_slot = E->upload(hslot);
+        E->natm = (int)P.atm_scale.size();
+        std::vector<int> atslot(E->natm);
+        for (int t = 0; t < E->natm; ++t) {
+            atslot[t] = add_slot(P.atm_ijk[3 * t]);
+            add_slot(P.atm_ijk[3 * t + 1]);
+            add_slot(P.atm_ijk[3 * t + 2]);
+        }
+        E->atm_ijk = E->upload(P.atm_ijk);
+        E->atm_sc = E->upload(P.atm_scale);
+        E->atm_slot = E->upload(atslot);
+        E->nbatm = (int)(P.batm_par.size() / 3);
+        std::vector<int> btslot(E->nbatm);
+        for (int t = 0; t < E->nbatm; ++t) {
+            btslot[t] = add_slot(P.batm_ijk[3 * t]);
+            add_slot(P.batm_ijk[3 * t + 1]);
+            add_slot(P.batm_ijk[3 * t + 2]);
+        }
+        E->batm_ijk = E->upload(P.batm_ijk);
+        E->batm_zb = E->upload(P.batm_par);
+        E->batm_slot = E->upload(btslot);
+        E->nslots = slot;
+        // per-atom CSR over slots (ascending slot order)
+        std::vector<int> soff(n + 1, 0), sidx(slot);
+        for (int a : slot_atom) soff[a + 1]++;
+        for (int i = 0; i < n; ++i) soff[i + 1] += soff[i];
+        {
+            std::vector<int> fill(soff.begin(), soff.end() - 1);
+            for (int s = 0; s < slot; ++s) sidx[fill[slot_atom[s]]++] = s;
+        }
+        E->slot_off = E->upload(soff);
+        E->slot_idx = E->upload(sidx);
+        std::vector<int> doff(n + 1, 0), didx(2 * E->nbond);
+        for (int b = 0; b < E->nbond; ++b) {
+            doff[P.bond_ij[2 * b] + 1]++;
+            doff[P.bond_ij[2 * b + 1] + 1]++;
+        }
+        for (int i = 0; i < n; ++i) doff[i + 1] += doff[i];
+        {
+            std::vector<int> fill(doff.begin(), doff.end() - 1);
+            for (int b = 0; b < E->nbond; ++b) {
+                didx[fill[P.bond_ij[2 * b]]++] = 2 * b;
+                didx[fill[P.bond_ij[2 * b + 1]]++] = 2 * b + 1;
+            }
+        }
+        E->dcn_off = E->upload(doff);
+        E->dcn_idx = E->upload(didx);
+
+        // work buffers
+        E->x = E->alloc<double>(3 * (size_t)n);
+        E->cn = E->alloc<double>(n);
+        E->dlog = E->alloc<double>(n);
+        E->gw = E->alloc<double>(7 * (size_t)n);
+        E->dgw = E->alloc<double>(7 * (size_t)n);
+        E->rhs = E->alloc<double>(n);
+        E->diag = E->alloc<double>(n);
+        E->Ared = E->alloc<double>((size_t)E->np * E->np);
+        E->Linv = E->alloc<double>((size_t)E->np * NB);
+        E->b = E->alloc<double>(E->np);
+        E->rres = E->alloc<double>(n);
+        E->q = E->alloc<double>(n);
+        E->eself = E->alloc<double>(n);
+        E->qtmp = E->alloc<double>(n);
+        E->gpair = E->alloc<double>(3 * (size_t)n);
+        E->dEdcn = E->alloc<double>(n);
+        E->vchain = E->alloc<double>(n);
+        E->slots = E->alloc<double>(3 * (size_t)std::max(slot, 1));
+        E->dcn_slot = E->alloc<double>(2 * (size_t)std::max(E->nbond, 1));
+        E->grad = E->alloc<double>(3 * (size_t)n);
+        E->e_coul = E->alloc<double>(n);
+        E->e_disp = E->alloc<double>(n);
+        E->e_rep = E->alloc<double>(n);
+        E->e_bond = E->alloc<double>(std::max(E->nbond, 1));
+        E->e_ang = E->alloc<double>(std::max(E->nangle, 1));
+        E->e_tor = E->alloc<double>(std::max(E->ntors, 1));
+        E->e_brep = E->alloc<double>(std::max(E->nbrep, 1));
+        E->e_hb = E->alloc<double>(std::max(E->nhb, 1));
+        E->e_atm = E->alloc<double>(std::max(E->natm, 1));
+        E->e_batm = E->alloc<double>(std::max(E->nbatm, 1));
+        E->parts = E->alloc<double>(16);
+        E->out = E->alloc<double>(GFNFF_NTERMS + 1);
+        E->nbn = E->alloc<int>(n);
+        E->nbl = E->alloc<int>((size_t)n * MAXNB_CN);
+        E->flags = E->alloc<int>(1);
+        if (hipStreamCreate(&E->own) != hipSuccess) throw std::runtime_error("stream");
+        hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, E->own, 1, E->flags, 0);
+        hipLaunchKernelGGL(k_eeq_diag, dim3(cdiv(n, 256)), dim3(256), 0, E->own, n, E->gam, E->alp, E->diag);
+        if (hipStreamSynchronize(E->own) != hipSuccess) throw std::runtime_error("init sync");
+    } catch (const std::bad_alloc&) {
+        status = kErrDevice;
+        msg = "device allocation failed";
+        delete E;
+        return nullptr;
+    } catch (const std::exception& ex) {
+        status = kErrDevice;
+        msg = ex.what();
+        delete E;
+        return nullptr;
+    }
+    status = kOk;
+    return E;
+}
+
+void device_destroy(DeviceEngine* e) { delete e; }
+
+static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::string& msg) {
+    const int n = E->n;
+    hipLaunchKernelGGL(k_copy, dim3(cdiv(3 * n, 256)), dim3(256), 0, s, 3 * n, xin, E->x);
+    hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, s, 1, E->flags, 0);
+    hipLaunchKernelGGL(k_cn, dim3(cdiv(n, TILE)), dim3(TILE), 0, s, n, E->x, E->rcov_cn, E->elem, E->nref, E->refcn,
+                       E->chi_base, E->cnf, E->cn, E->dlog, E->gw, E->dgw, E->rhs, E->nbn, E->nbl, E->flags);
+    // EEQ
+    const int np = E->np, nbk = np / NB;
+    hipLaunchKernelGGL(k_eeq_reduced, dim3(nbk * (nbk + 1) / 2), dim3(256), 0, s, E->nr, np, E->red_atom,
+                       E->red_last, E->x, E->alp, E->diag, E->Ared);
+    const double* rsrc = E->rhs;
+    if (E->charged) {
+        hipLaunchKernelGGL(k_eeq_rhs_qp, dim3(cdiv(n, 256)), dim3(256), 0, s, n, E->x, E->alp, E->diag, E->qp,
+                           E->rhs, E->rres);
+        rsrc = E->rres;
+    }
+    hipLaunchKernelGGL(k_eeq_rhs_reduce, dim3(cdiv(np, 256)), dim3(256), 0, s, E->nr, np, E->red_atom, E->red_last,
+                       rsrc, E->b);
+    for (int k = 0; k < nbk; ++k) {
+        int k0 = k * NB;
+        hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, s, np, k0, E->Ared, E->Linv, E->flags);
+        int rem = nbk - k - 1;
+        if (rem > 0) {
+            hipLaunchKernelGGL(k_chol_panel, dim3(rem), dim3(256), 0, s, np, k0, E->Ared, E->Linv);
+            hipLaunchKernelGGL(k_chol_update, dim3(rem * (rem + 1) / 2), dim3(256), 0, s, np, k0, E->Ared);
+        }
+    }
+    for (int k = 0; k < nbk; ++k) {
+        int k0 = k * NB;
+        hipLaunchKernelGGL(k_trsv_fwd_diag, dim3(1), dim3(NB), 0, s, k0, E->Linv, E->b);
+        int rows = np - k0 - NB;
+        if (rows > 0)
+            hipLaunchKernelGGL(k_trsv_fwd_update, dim3(cdiv(rows, 256)), dim3(256), 0, s, np, k0, E->Ared, E->b);
+    }
+    for (int k = nbk - 1; k >= 0; --k) {
+        int k0 = k * NB;
+        hipLaunchKernelGGL(k_trsv_bwd_diag, dim3(1), dim3(NB), 0, s, k0, E->Linv, E->b);
+        if (k0 > 0)
+            hipLaunchKernelGGL(k_trsv_bwd_update, dim3(cdiv(k0, 256)), dim3(256), 0, s, np, k0, E->Ared, E->b);
+    }
+    hipLaunchKernelGGL(k_charges, dim3(cdiv(n, 256)), dim3(256), 0, s, n, E->red_of, (const int*)nullptr,
+                       E->frag_off, E->frag_atoms, E->frag, E->qp, E->b, E->chi_base, E->cnf, E->gam, E->alp, E->cn,
+                       E->q, E->eself, E->qtmp);
+    // pair terms
+    PairArgs pa;
+    pa.n = n;
+    pa.nelem = E->nelem;
+    pa.x = E->x;
+    pa.q = E->q;
+    pa.alp = E->alp;
+    pa.gw = E->gw;
+    pa.dgw = E->dgw;
+    pa.zeta = E->zeta;
+    pa.szr = E->szr;
+    pa.anb = E->anb;
+    pa.repz = E->repz;
+    pa.Z = E->Z;
+    pa.elem = E->elem;
+    pa.nref = E->nref;
+    pa.metal = E->metal;
+    pa.c6ref = E->c6ref;
+    pa.g = E->gpair;
+    pa.dEdcn = E->dEdcn;
+    pa.e_coul = E->e_coul;
+    pa.e_disp = E->e_disp;
+    pa.e_rep = E->e_rep;
+    hipLaunchKernelGGL(k_pairs, dim3(cdiv(n, TILE)), dim3(TILE), 0, s, pa);
+    hipLaunchKernelGGL(k_nbrep_fix, dim3(cdiv(n, 256)), dim3(256), 0, s, n, E->x, E->Z, E->metal, E->anb, E->repz,
+                       E->toff, E->tidx, E->tdist, E->gpair, E->e_rep);
+    // bonded lists
+    if (E->nbond)
+        hipLaunchKernelGGL(k_bonds, dim3(cdiv(E->nbond, 256)), dim3(256), 0, s, E->nbond, E->bond_ij, E->bond_par,
+                           E->bond_hboff, E->bond_hbB, E->bond_slot, E->x, E->cn, E->rcov_d3, E->bond_H, E->slots,
+                           E->dcn_slot, E->e_bond);
+    if (E->nangle)
+        hipLaunchKernelGGL(k_angles, dim3(cdiv(E->nangle, 256)), dim3(256), 0, s, E->nangle, E->ang_ijk, E->ang_par,
+                           E->ang_slot, E->x, E->rcov_d3, E->slots, E->e_ang);
+    if (E->ntors)
+        hipLaunchKernelGGL(k_tors, dim3(cdiv(E->ntors, 256)), dim3(256), 0, s, E->ntors, E->tor_ijkl, E->tor_par,
+                           E->tor_slot, E->x, E->rcov_d3, E->slots, E->e_tor);
+    if (E->nbrep)
+        hipLaunchKernelGGL(k_brep, dim3(cdiv(E->nbrep, 256)), dim3(256), 0, s, E->nbrep, E->brep_ij, E->brep_par,
+                           E->brep_slot, E->x, E->slots, E->e_brep);
+    if (E->nhb)
+        hipLaunchKernelGGL(k_hbonds, dim3(cdiv(E->nhb, 256)), dim3(256), 0, s, E->nhb, E->hb_ahb, E->hb_case,
+                           E->hb_par, E->hb_nboff, E->hb_nbidx, E->hb_slot, E->x, E->hb_rad, E->Z, E->slots, E->e_hb);
+    if (E->natm)
+        hipLaunchKernelGGL(k_atm, dim3(cdiv(E->natm, 256)), dim3(256), 0, s, E->natm, E->atm_ijk, E->atm_sc,
+                           E->atm_slot, E->x, E->rcov_d3, E->nelem, E->elem, E->nref, E->c6ref, E->gw, E->slots,
+                           E->e_atm);
+    if (E->nbatm)
+        hipLaunchKernelGGL(k_batm, dim3(cdiv(E->nbatm, 256)), dim3(256), 0, s, E->nbatm, E->batm_ijk, E->batm_zb,
+                           E->batm_slot, E->x, E->qa, E->slots, E->e_batm);
+    hipLaunchKernelGGL(k_dcn_total, dim3(cdiv(n, 256)), dim3(256), 0, s, n, E->dcn_off, E->dcn_idx, E->dcn_slot,
+                       E->dEdcn, E->qtmp, E->vchain);
+    hipLaunchKernelGGL(k_finish, dim3(cdiv(n, 256)), dim3(256), 0, s, n, E->x, E->rcov_cn, E->dlog, E->vchain,
+                       E->nbn, E->nbl, E->slot_off, E->slot_idx, E->slots, E->gpair, E->grad);
+    Segs sg;
+    const double* ptrs[12] = {E->e_bond, E->e_ang, E->e_tor, nullptr, E->e_disp, E->e_brep,
+                              E->e_rep,  E->e_coul, E->eself, E->e_hb, E->e_atm, E->e_batm};
+    int cnts[12] = {E->nbond, E->nangle, E->ntors, 0, n, E->nbrep, n, n, n, E->nhb, E->natm, E->nbatm};
+    for (int k = 0; k < 12; ++k) {
+        sg.p[k] = ptrs[k] ? ptrs[k] : E->e_disp;
+        sg.n[k] = ptrs[k] ? cnts[k] : 0;
+        sg.dst[k] = k;
+    }
+    hipLaunchKernelGGL(k_reduce, dim3(12), dim3(256), 0, s, sg, E->parts);
+    hipLaunchKernelGGL(k_total, dim3(1), dim3(64), 0, s, E->parts, E->out);
+    if (hipGetLastError() != hipSuccess) {
+        msg = "kernel launch failed";
+        return kErrDevice;
+    }
+    return kOk;
+}
+
+int device_eval_host(DeviceEngine* E, const double* xyz_bohr, bool want_grad, double* e_terms, double* grad,
+                     double* q, double* cn, std::string& msg) {
+    const int n = E->n;
+    HIP_OK(hipMemcpyAsync(E->gpair, xyz_bohr, 3 * sizeof(double) * n, hipMemcpyHostToDevice, E->own));
+    // gpair is overwritten by k_pairs after the copy kernel has consumed it
+    int st = launch_eval(E, E->gpair, E->own, msg);
+    if (st) return st;
+    int flags = 0;
+    HIP_OK(hipMemcpyAsync(e_terms, E->out, sizeof(double) * (GFNFF_NTERMS + 1), hipMemcpyDeviceToHost, E->own));
+    HIP_OK(hipMemcpyAsync(&flags, E->flags, sizeof(int), hipMemcpyDeviceToHost, E->own));
+    if (want_grad && grad) HIP_OK(hipMemcpyAsync(grad, E->grad, 3 * sizeof(double) * n, hipMemcpyDeviceToHost, E->own));
+    if (q) HIP_OK(hipMemcpyAsync(q, E->q, sizeof(double) * n, hipMemcpyDeviceToHost, E->own));
+    if (cn) HIP_OK(hipMemcpyAsync(cn, E->cn, sizeof(double) * n, hipMemcpyDeviceToHost, E->own));
+    HIP_OK(hipStreamSynchronize(E->own));
+    if (flags & FLAG_CHOL) {
+        msg = "EEQ matrix is not positive definite on the constraint space";
+        return kErrEEQ;
+    }
+    if (flags & FLAG_CNOVF) {
+        msg = "more than 128 atoms within the 6 Bohr CN cut-off of one atom";
+        return kErrGeometry;
+    }
+    return kOk;
+}
+
+int device_eval(DeviceEngine* E, const double* xyz_bohr_dev, double* energy_dev, double* grad_dev, void* stream,
+                std::string& msg) {
+    hipStream_t s = (hipStream_t)stream;
+    int st = launch_eval(E, xyz_bohr_dev, s, msg);
+    if (st) return st;
+    hipLaunchKernelGGL(k_copy, dim3(1), dim3(64), 0, s, GFNFF_NTERMS + 1, E->out, energy_dev);
+    if (grad_dev) hipLaunchKernelGGL(k_copy, dim3(cdiv(3 * E->n, 256)), dim3(256), 0, s, 3 * E->n, E->grad, grad_dev);
+    if (hipGetLastError() != hipSuccess) {
+        msg = "kernel launch failed";
+        return kErrDevice;
+    }
+    return kOk;
+}
+
+int device_count() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+}  // namespace gfnff

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <numeric>
 #include <queue>
 #include <set>
 #include <thread>
@@ -528,54 +529,97 @@ bool phase1_charges(const Mol& m, Topo& t, int threads, std::string& msg) {
     }
     std::vector<float> radf(n);
     for (int i = 0; i < n; ++i) radf[i] = (float)((m.Z[i] >= 1 && m.Z[i] <= 86) ? rad[m.Z[i] - 1] : 1.0);
-    std::vector<double> A((size_t)n * n, 0.0);
-    auto row_worker = [&](int t0, int T) {
-        std::vector<float> dist(n);
-        for (int src = t0; src < n; src += T) {
-            std::fill(dist.begin(), dist.end(), RABD_CUT);
-            dist[src] = 0.0f;
-            using E = std::pair<float, int>;
-            std::priority_queue<E, std::vector<E>, std::greater<E>> pq;
-            pq.push({0.0f, src});
-            while (!pq.empty()) {
-                auto [d, u] = pq.top();
-                pq.pop();
-                if (d > dist[u]) continue;
-                if (d > TDIST_THR) continue;
-                for (int v : t.adj[u]) {
-                    float w = radf[u] + radf[v];
-                    float nd = d + w;
-                    if (nd < dist[v]) {
-                        dist[v] = nd;
-                        pq.push({nd, v});
+    // Matrix rows over an atom subset `sub` (local indices); Dijkstra from each
+    // source over the bond graph, capped as in the reference.
+    auto build = [&](const std::vector<int>& sub, std::vector<double>& A) {
+        const int ns = (int)sub.size();
+        A.assign((size_t)ns * ns, 0.0);
+        std::vector<int> loc(n, -1);
+        for (int k = 0; k < ns; ++k) loc[sub[k]] = k;
+        auto row_worker = [&](int t0, int T) {
+            std::vector<float> dist(n, RABD_CUT);
+            std::vector<int> touched;
+            for (int ls = t0; ls < ns; ls += T) {
+                int src = sub[ls];
+                for (int v : touched) dist[v] = RABD_CUT;
+                touched.assign(1, src);
+                dist[src] = 0.0f;
+                using E = std::pair<float, int>;
+                std::priority_queue<E, std::vector<E>, std::greater<E>> pq;
+                pq.push({0.0f, src});
+                while (!pq.empty()) {
+                    auto [d, u] = pq.top();
+                    pq.pop();
+                    if (d > dist[u]) continue;
+                    if (d > TDIST_THR) continue;
+                    for (int v : t.adj[u]) {
+                        float w = radf[u] + radf[v];
+                        float nd = d + w;
+                        if (nd < dist[v]) {
+                            if (dist[v] == RABD_CUT) touched.push_back(v);
+                            dist[v] = nd;
+                            pq.push({nd, v});
+                        }
                     }
                 }
+                double* Ai = &A[(size_t)ls * ns];
+                Ai[ls] = gam_v[src] + TSQRT2PI / std::sqrt(alp_v[src]);
+                for (int lj = 0; lj < ls; ++lj) {
+                    int j = sub[lj];
+                    float rij = dist[j];
+                    double val = (rij > TDIST_THR) ? (double)RABD_CUT : (double)rij;
+                    double r = RFGOED1 * val / BOHR;
+                    double g = 1.0 / std::sqrt(alp_v[src] + alp_v[j]);
+                    Ai[lj] = std::erf(g * r) / r;
+                }
             }
-            double* Ai = &A[(size_t)src * n];
-            Ai[src] = gam_v[src] + TSQRT2PI / std::sqrt(alp_v[src]);
-            for (int j = 0; j < src; ++j) {
-                float rij = dist[j];
-                double val = (rij > TDIST_THR) ? (double)RABD_CUT : (double)rij;
-                double r = RFGOED1 * val / BOHR;
-                double g = 1.0 / std::sqrt(alp_v[src] + alp_v[j]);
-                Ai[j] = std::erf(g * r) / r;
-            }
-        }
-    };
-    int T = std::max(1, std::min(threads, n));
-    {
+        };
+        int T = std::max(1, std::min(threads, ns));
         std::vector<std::thread> th;
         for (int k = 1; k < T; ++k) th.emplace_back(row_worker, k, T);
         row_worker(0, T);
         for (auto& x : th) x.join();
-    }
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < i; ++j) A[(size_t)j * n + i] = A[(size_t)i * n + j];
-    std::vector<int32_t> fr(n);
-    for (int i = 0; i < n; ++i) fr[i] = t.frag[i] - 1;
-    if (!eeq_solve_host(n, t.nfrag, A, chi_v, fr, t.qfrag, t.qa, threads)) {
-        msg = "Phase-1 EEQ solve failed";
-        return false;
+        for (int i = 0; i < ns; ++i)
+            for (int j = 0; j < i; ++j) A[(size_t)j * ns + i] = A[(size_t)i * ns + j];
+    };
+    // Atoms of different fragments are at the capped topological distance
+    // r_cap = 1.175*13/0.52917726 Bohr.  When erf(gamma_ij r_cap) is exactly 1
+    // for every pair, the inter-fragment coupling is the constant 1/r_cap,
+    // which within a fragment is a constant shift absorbed by that fragment's
+    // Lagrange multiplier (every fragment's charge is fixed), so the bordered
+    // system decouples into independent per-fragment systems exactly.
+    const double rcap = RFGOED1 * (double)RABD_CUT / BOHR;
+    double amax = 0.0;
+    for (double a : alp_v) amax = std::max(amax, a);
+    bool decoupled = t.nfrag > 1 && std::erf(rcap / std::sqrt(2.0 * amax)) == 1.0;
+    if (decoupled) {
+        std::vector<std::vector<int>> fa(t.nfrag);
+        for (int i = 0; i < n; ++i) fa[t.frag[i] - 1].push_back(i);
+        t.qa.assign(n, 0.0);
+        for (int f = 0; f < t.nfrag; ++f) {
+            const auto& sub = fa[f];
+            std::vector<double> A, xs(sub.size()), qs;
+            for (size_t k = 0; k < sub.size(); ++k) xs[k] = chi_v[sub[k]];
+            build(sub, A);
+            std::vector<int32_t> fr(sub.size(), 0);
+            std::vector<double> qf{t.qfrag[f]};
+            if (!eeq_solve_host((int)sub.size(), 1, A, xs, fr, qf, qs, 1)) {
+                msg = "Phase-1 EEQ solve failed";
+                return false;
+            }
+            for (size_t k = 0; k < sub.size(); ++k) t.qa[sub[k]] = qs[k];
+        }
+    } else {
+        std::vector<int> all(n);
+        std::iota(all.begin(), all.end(), 0);
+        std::vector<double> A;
+        build(all, A);
+        std::vector<int32_t> fr(n);
+        for (int i = 0; i < n; ++i) fr[i] = t.frag[i] - 1;
+        if (!eeq_solve_host(n, t.nfrag, A, chi_v, fr, t.qfrag, t.qa, threads)) {
+            msg = "Phase-1 EEQ solve failed";
+            return false;
+        }
     }
     for (double q : t.qa)
         if (!std::isfinite(q)) {

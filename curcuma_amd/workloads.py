@@ -1,0 +1,110 @@
+"""Seeded synthetic molecules for the benchmark and the parity tests.
+
+Recipes (all coordinates in Angstrom):
+  * n-alkane CnH(2n+2), all-trans, ideal geometry (C-C 1.53, C-H 1.09,
+    tetrahedral angles), Gaussian coordinate noise.
+  * water cluster: the first ``nmol`` sites of a cubic lattice filled in
+    lexicographic (i, j, k) order, O-H 0.96, H-O-H 104.5 deg, uniform random
+    rigid orientations, uniform jitter of the O position.
+  * parallel alkane bundle: chains on a hexagonal lattice.
+Random numbers come from numpy's PCG64 generator seeded per workload.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+TETRA = math.acos(-1.0 / 3.0)  # 109.47 deg
+
+
+def alkane(nc: int, sigma: float = 0.0, seed: int = 0, cc: float = 1.53, ch: float = 1.09):
+    """All-trans n-alkane C_nH_(2n+2); returns (Z, xyz[A])."""
+    half = TETRA / 2.0
+    dx, dy = cc * math.sin(half), cc * math.cos(half)
+    # backbone with one virtual atom on each end for the terminal methyl hydrogens
+    back = np.array([[(k) * dx, (k % 2) * dy, 0.0] for k in range(-1, nc + 1)])
+    carb = back[1:-1]
+    Z = [6] * nc
+    H = []
+    zax = np.array([0.0, 0.0, 1.0])
+    ca, sa = math.cos(TETRA / 2.0), math.sin(TETRA / 2.0)
+    for k in range(nc):
+        c = back[k + 1]
+        prv, nxt = back[k], back[k + 2]
+        b = 2.0 * c - prv - nxt
+        b /= np.linalg.norm(b)
+        H.append(c + ch * (ca * b + sa * zax))
+        H.append(c + ch * (ca * b - sa * zax))
+        if k == 0:
+            u = prv - c
+            H.append(c + ch * u / np.linalg.norm(u))
+        if k == nc - 1:
+            u = nxt - c
+            H.append(c + ch * u / np.linalg.norm(u))
+    xyz = np.vstack([carb, np.array(H)])
+    Z = np.array(Z + [1] * len(H), dtype=np.int32)
+    if sigma > 0:
+        rng = np.random.default_rng(seed)
+        xyz = xyz + rng.normal(0.0, sigma, xyz.shape)
+    return Z, xyz
+
+
+def _random_rotation(rng) -> np.ndarray:
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    a, b, c, d = q
+    return np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                     [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+                     [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+
+
+def water_cluster(nmol: int = 3000, spacing: float = 3.1, jitter: float = 0.05, seed: int = 2, oh: float = 0.96,
+                  hoh_deg: float = 104.5):
+    """Non-periodic water cluster on a cubic lattice; returns (Z, xyz[A])."""
+    side = int(math.ceil(nmol ** (1.0 / 3.0) - 1e-9))
+    rng = np.random.default_rng(seed)
+    half = math.radians(hoh_deg) / 2.0
+    local = np.array([[0.0, 0.0, 0.0], [oh * math.sin(half), oh * math.cos(half), 0.0],
+                      [-oh * math.sin(half), oh * math.cos(half), 0.0]])
+    Z, xyz = [], []
+    count = 0
+    for i in range(side):
+        for j in range(side):
+            for k in range(side):
+                if count == nmol:
+                    break
+                site = np.array([i, j, k], dtype=float) * spacing + rng.uniform(-jitter, jitter, 3)
+                R = _random_rotation(rng)
+                for a, zz in zip(local, (8, 1, 1)):
+                    xyz.append(site + R @ a)
+                    Z.append(zz)
+                count += 1
+    return np.array(Z, dtype=np.int32), np.array(xyz)
+
+
+def alkane_bundle(nchains: int = 80, nc: int = 80, a: float = 4.5, sigma: float = 0.02, seed: int = 4):
+    """Parallel all-trans alkanes on the first nchains sites of a hexagonal lattice."""
+    Z0, x0 = alkane(nc)
+    rng = np.random.default_rng(seed)
+    side = int(math.ceil(math.sqrt(nchains)))
+    Z, xyz = [], []
+    count = 0
+    for r in range(side):
+        for c in range(side):
+            if count == nchains:
+                break
+            off = np.array([0.0, (c + 0.5 * (r % 2)) * a, r * a * math.sqrt(3.0) / 2.0])
+            xyz.append(x0 + off)
+            Z.append(Z0)
+            count += 1
+    xyz = np.vstack(xyz)
+    xyz = xyz + rng.normal(0.0, sigma, xyz.shape)
+    return np.concatenate(Z).astype(np.int32), xyz
+
+
+WORKLOADS = {
+    "alkane_c40h82": lambda: alkane(40, sigma=0.02, seed=1),
+    "water_3000": lambda: water_cluster(3000, seed=2),
+    "alkane_bundle_80xc80": lambda: alkane_bundle(),
+}

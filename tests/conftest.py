@@ -1,0 +1,70 @@
+import ctypes as C
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(ROOT))
+
+from curcuma_amd._lib import NTERMS, ParamsView  # noqa: E402
+
+FIXTURES = ROOT / "tests" / "golden" / "fixtures"
+ORACLE_LIB = ROOT / "oracle" / "_build" / "liboracle.so"
+BOHR = 0.529177210903
+
+# per-component gradient slots written by the oracle (oracle/gfnff_oracle.c)
+GCOMP = ["Bond", "Angle", "Torsion", "Repulsion", "NBRep", "ES", "Disp", "HB", "XB", "BATM", "ATM"]
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")
+
+
+def load_fixture(name: str) -> dict:
+    return json.loads((FIXTURES / f"{name}.json").read_text())
+
+
+class Oracle:
+    """ctypes wrapper of the CPU oracle (test infrastructure only)."""
+
+    def __init__(self):
+        if not ORACLE_LIB.exists():
+            raise RuntimeError(f"oracle not built: {ORACLE_LIB}")
+        self.lib = C.CDLL(str(ORACLE_LIB))
+        self.lib.gfnff_oracle_eval.argtypes = [C.POINTER(ParamsView), C.c_void_p, C.c_int, C.c_void_p,
+                                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+
+    def eval(self, view: ParamsView, xyz_bohr: np.ndarray, grad: bool = True, components: bool = False):
+        n = view.natoms
+        x = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1)
+        E = C.c_double()
+        terms = np.zeros(NTERMS)
+        g = np.zeros(3 * n)
+        q = np.zeros(n)
+        cn = np.zeros(n)
+        gc = np.zeros(len(GCOMP) * 3 * n) if components else None
+        st = self.lib.gfnff_oracle_eval(C.byref(view), x.ctypes.data, 1 if grad else 0, C.byref(E),
+                                        terms.ctypes.data, g.ctypes.data, q.ctypes.data, cn.ctypes.data,
+                                        gc.ctypes.data if components else None)
+        if st != 0:
+            raise RuntimeError(f"oracle status {st}")
+        out = {"E": E.value, "terms": terms, "grad": g.reshape(n, 3), "q": q, "cn": cn}
+        if components:
+            out["comp"] = {k: gc.reshape(len(GCOMP), n, 3)[i] for i, k in enumerate(GCOMP)}
+        return out
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    return Oracle()
+
+
+def gpu_available() -> bool:
+    try:
+        from curcuma_amd._lib import load_library
+        return load_library().gfnff_device_count() > 0
+    except Exception:
+        return False
