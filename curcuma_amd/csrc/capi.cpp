@@ -211,4 +211,14 @@ int gfnff_get_topology(const gfnff_handle* h, int32_t* hyb, int32_t* pi, int32_t
 
 int gfnff_device_count(void) { return gfnff::device_count(); }
 
+int gfnff_dense_cholesky(int32_t n, const double* a, double* l_out) {
+    if (n <= 0 || !a || !l_out) return GFNFF_STATUS_ARGS;
+    std::string msg;
+    try {
+        return gfnff::device_dense_cholesky(n, a, l_out, msg);
+    } catch (...) {
+        return GFNFF_STATUS_DEVICE;
+    }
+}
+
 }  // extern "C"

@@ -217,54 +217,69 @@ __global__ __launch_bounds__(256) void k_eeq_rhs_reduce(int nr, int np, const in
 // ---------------------------------------------------------------------------
 // K3: blocked Cholesky, right-looking, lower, row-major with leading dim np.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_chol_diag(int np, int k0, double* __restrict__ A,
-                                                   double* __restrict__ Linv, int* __restrict__ flags) {
-    __shared__ double L[NB][NB + 1];
-    __shared__ double X[NB][NB + 1];
-    __shared__ int bad;
-    int tid = threadIdx.x;
-    if (tid == 0) bad = 0;
-    for (int e = tid; e < NB * NB; e += 256) {
-        int r = e / NB, c = e % NB;
-        L[r][c] = (c <= r) ? A[(size_t)(k0 + r) * np + k0 + c] : 0.0;
-    }
-    __syncthreads();
+// broadcast of a lane's double to the whole wavefront through SGPRs
+__device__ __forceinline__ double bcast(double v, int lane) {
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// One wavefront factorises the 64x64 diagonal block: lane j keeps column j
+// in registers.  At step k lane k finalises its column; the column entries
+// reach the other lanes as SGPR broadcasts (readlane), so the rank-1 update
+// needs no LDS round trips.  The inverse of the factor is built the same way,
+// lane c computing column c by forward substitution with broadcast rows.
+__global__ __launch_bounds__(64) void k_chol_diag(int np, int k0, double* __restrict__ A,
+                                                  double* __restrict__ Linv, int* __restrict__ flags) {
+    const int j = threadIdx.x;
+    double c[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) c[i] = (i >= j) ? A[(size_t)(k0 + i) * np + k0 + j] : 0.0;
+    int bad = 0;
+#pragma unroll
     for (int k = 0; k < NB; ++k) {
-        if (tid == 0) {
-            double d = L[k][k];
-            if (!(d > 0.0)) {
-                bad = 1;
-                d = 1.0;
-            }
-            L[k][k] = sqrt(d);
+        double dk = bcast(c[k], k);
+        if (!(dk > 0.0)) {
+            bad = 1;
+            dk = 1.0;
         }
-        __syncthreads();
-        double dk = L[k][k];
-        for (int r = k + 1 + tid; r < NB; r += 256) L[r][k] /= dk;
-        __syncthreads();
-        int m = NB - k - 1;
-        for (int e = tid; e < m * m; e += 256) {
-            int r = k + 1 + e / m, c = k + 1 + e % m;
-            if (c <= r) L[r][c] -= L[r][k] * L[c][k];
+        dk = sqrt(dk);
+        double inv = 1.0 / dk;
+        // column k (owned by lane k): L(i,k) = c_k[i] / dk for i > k
+        double lk[NB];
+#pragma unroll
+        for (int i = k + 1; i < NB; ++i) lk[i] = bcast(c[i], k) * inv;
+        if (j == k) {
+            c[k] = dk;
+#pragma unroll
+            for (int i = k + 1; i < NB; ++i) c[i] = lk[i];
         }
-        __syncthreads();
-    }
-    // inverse of the lower-triangular block, one column per thread
-    if (tid < NB) {
-        int c = tid;
-        for (int r = 0; r < NB; ++r) {
-            double s = (r == c) ? 1.0 : 0.0;
-            for (int p = c; p < r; ++p) s -= L[r][p] * X[p][c];
-            X[r][c] = (r < c) ? 0.0 : s / L[r][r];
+        // lane j > k: c[i] -= L(i,k) L(j,k) for i >= j
+        double ljk = 0.0;
+#pragma unroll
+        for (int i = k + 1; i < NB; ++i) ljk = (i == j) ? lk[i] : ljk;
+        if (j > k) {
+#pragma unroll
+            for (int i = k + 1; i < NB; ++i)
+                if (i >= j) c[i] -= lk[i] * ljk;
         }
     }
-    __syncthreads();
-    for (int e = tid; e < NB * NB; e += 256) {
-        int r = e / NB, c = e % NB;
-        if (c <= r) A[(size_t)(k0 + r) * np + k0 + c] = L[r][c];
-        Linv[(size_t)(k0 / NB) * NB * NB + e] = X[r][c];
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+        if (i >= j) A[(size_t)(k0 + i) * np + k0 + j] = c[i];
+    // X = L^{-1}: lane j computes column j; L(r,p) is lane p's c[r]
+    double xv[NB];
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+        double sacc = (r == j) ? 1.0 : 0.0;
+#pragma unroll
+        for (int p = 0; p < r; ++p) sacc -= bcast(c[r], p) * xv[p];
+        xv[r] = (r < j) ? 0.0 : sacc / bcast(c[r], r);
     }
-    if (tid == 0 && bad) atomicOr(flags, FLAG_CHOL);
+    double* Li = Linv + (size_t)(k0 / NB) * NB * NB;
+#pragma unroll
+    for (int r = 0; r < NB; ++r) Li[r * NB + j] = xv[r];
+    if (bad && j == 0) atomicOr(flags, FLAG_CHOL);
 }
 
 // panel: rows below the diagonal block, L(i, k-block) = A(i, k-block) * Linv^T
@@ -304,6 +319,13 @@ __global__ __launch_bounds__(256) void k_chol_panel(int np, int k0, double* __re
 
 // trailing update of the lower tiles right of the panel:
 //   A(bi, bj) -= L(bi, k) L(bj, k)^T, bi >= bj > k
+// FP64 MFMA (v_mfma_f64_16x16x4f64): each of the 4 wavefronts owns a 32x32
+// quarter of the 64x64 tile as 2x2 blocks of 16x16; the two 64x64 panels are
+// staged in LDS.  Operand layout per lane: A(i = lane%16, k = lane/16),
+// B(k = lane/16, j = lane%16); result element v of a block sits at
+// (row 4*v + lane/16, column lane%16) (layout measured on gfx950).
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
 __global__ __launch_bounds__(256) void k_chol_update(int np, int k0, double* __restrict__ A) {
     __shared__ double P[NB][NB + 1];
     __shared__ double Q[NB][NB + 1];
@@ -321,79 +343,89 @@ __global__ __launch_bounds__(256) void k_chol_update(int np, int k0, double* __r
         Q[r][c] = A[(size_t)(rj + r) * np + k0 + c];
     }
     __syncthreads();
-    int tr = tid / 16, tc = tid % 16;
-    double acc[4][4] = {};
-    for (int p = 0; p < NB; ++p) {
-        double a[4], b[4];
+    int wave = tid / 64, lane = tid % 64;
+    int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    int li = lane % 16, lk = lane / 16;
+    dbl4 acc[2][2];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            a[u] = P[tr * 4 + u][p];
-            b[u] = Q[tc * 4 + u][p];
-        }
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+        for (int b = 0; b < 2; ++b) acc[a][b] = (dbl4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int k = 0; k < NB; k += 4) {
+        double fa0 = P[wr + li][k + lk], fa1 = P[wr + 16 + li][k + lk];
+        double fb0 = Q[wc + li][k + lk], fb1 = Q[wc + 16 + li][k + lk];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb1, acc[1][1], 0, 0, 0);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            int r = ri + tr * 4 + u, c = rj + tc * 4 + v;
-            if (c <= r) A[(size_t)r * np + c] -= acc[u][v];
-        }
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int r = ri + wr + 16 * a + 4 * v + lk;
+                int c = rj + wc + 16 * b + li;
+                if (c <= r) A[(size_t)r * np + c] -= acc[a][b][v];
+            }
 }
 
 // ---------------------------------------------------------------------------
 // K4: triangular solves with the blocked factor
 // ---------------------------------------------------------------------------
-// forward, block k: y_k = Linv_k * b_k, then b_i -= L(i, k) y_k for rows below
-__global__ __launch_bounds__(NB) void k_trsv_fwd_diag(int k0, const double* __restrict__ Linv,
-                                                      double* __restrict__ b) {
-    __shared__ double bk[NB];
-    int t = threadIdx.x;
-    bk[t] = b[k0 + t];
-    __syncthreads();
+// Blocked substitution with the stored inverses of the diagonal blocks.
+// Forward, block k: y_k = Linv_k b_k (every workgroup recomputes the 64x64
+// product from LDS; workgroup 0 stores it), then rows below the block get
+// b_r -= L(r, k) y_k.  Backward, block k (descending): x_k = Linv_k^T c_k,
+// then c_j -= L(k, j)^T x_k for j < k0.
+__global__ __launch_bounds__(256) void k_trsv_fwd(int np, int k0, const double* __restrict__ A,
+                                                  const double* __restrict__ Linv, double* __restrict__ b,
+                                                  double* __restrict__ y) {
+    __shared__ double W[NB][NB + 1];
+    __shared__ double bk[NB], yk[NB];
     const double* Li = Linv + (size_t)(k0 / NB) * NB * NB;
-    double s = 0.0;
-    for (int p = 0; p <= t; ++p) s += Li[t * NB + p] * bk[p];
-    b[k0 + t] = s;
-}
-__global__ __launch_bounds__(256) void k_trsv_fwd_update(int np, int k0, const double* __restrict__ A,
-                                                         double* __restrict__ b) {
-    __shared__ double yk[NB];
-    if (threadIdx.x < NB) yk[threadIdx.x] = b[k0 + threadIdx.x];
+    for (int e = threadIdx.x; e < NB * NB; e += 256) W[e / NB][e % NB] = Li[e];
+    if (threadIdx.x < NB) bk[threadIdx.x] = b[k0 + threadIdx.x];
     __syncthreads();
-    int r = k0 + NB + blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x < NB) {
+        int t = threadIdx.x;
+        double s = 0.0;
+        for (int p = 0; p <= t; ++p) s += W[t][p] * bk[p];
+        yk[t] = s;
+        if (blockIdx.x == 0) y[k0 + t] = s;
+    }
+    __syncthreads();
+    int r = k0 + NB + blockIdx.x * 256 + threadIdx.x;
     if (r >= np) return;
     const double* Ar = A + (size_t)r * np + k0;
     double s = 0.0;
     for (int p = 0; p < NB; ++p) s += Ar[p] * yk[p];
     b[r] -= s;
 }
-// backward, block k (descending): x_k = Linv_k^T * c_k, then c_j -= L(k, j)^T x_k for j above
-__global__ __launch_bounds__(NB) void k_trsv_bwd_diag(int k0, const double* __restrict__ Linv,
-                                                      double* __restrict__ b) {
-    __shared__ double bk[NB];
-    int t = threadIdx.x;
-    bk[t] = b[k0 + t];
-    __syncthreads();
+__global__ __launch_bounds__(256) void k_trsv_bwd(int np, int k0, const double* __restrict__ A,
+                                                  const double* __restrict__ Linv, double* __restrict__ c,
+                                                  double* __restrict__ xout) {
+    __shared__ double W[NB][NB + 1];
+    __shared__ double ck[NB], xk[NB];
     const double* Li = Linv + (size_t)(k0 / NB) * NB * NB;
-    double s = 0.0;
-    for (int p = t; p < NB; ++p) s += Li[p * NB + t] * bk[p];
-    b[k0 + t] = s;
-}
-__global__ __launch_bounds__(256) void k_trsv_bwd_update(int np, int k0, const double* __restrict__ A,
-                                                         double* __restrict__ b) {
-    // columns c < k0: b_c -= sum_{p in block} A(k0+p, c) x_p
-    __shared__ double xk[NB];
-    if (threadIdx.x < NB) xk[threadIdx.x] = b[k0 + threadIdx.x];
+    for (int e = threadIdx.x; e < NB * NB; e += 256) W[e / NB][e % NB] = Li[e];
+    if (threadIdx.x < NB) ck[threadIdx.x] = c[k0 + threadIdx.x];
     __syncthreads();
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= k0) return;
+    if (threadIdx.x < NB) {
+        int t = threadIdx.x;
+        double s = 0.0;
+        for (int p = t; p < NB; ++p) s += W[p][t] * ck[p];
+        xk[t] = s;
+        if (blockIdx.x == 0) xout[k0 + t] = s;
+    }
+    __syncthreads();
+    int col = blockIdx.x * 256 + threadIdx.x;
+    if (col >= k0) return;
     double s = 0.0;
-    for (int p = 0; p < NB; ++p) s += A[(size_t)(k0 + p) * np + c] * xk[p];
-    b[c] -= s;
+    for (int p = 0; p < NB; ++p) s += A[(size_t)(k0 + p) * np + col] * xk[p];
+    c[col] -= s;
 }
 
 // ---------------------------------------------------------------------------
@@ -840,24 +872,23 @@ __global__ __launch_bounds__(256) void k_brep(int nb, const int* __restrict__ ij
 }
 
 // hydrogen bonds, cases 1 and 2: ff_workspace_gfnff.cpp:938-1404
-__global__ __launch_bounds__(256) void k_hbonds(int nh, const int* __restrict__ ahb, const int* __restrict__ hcase,
-                                                const double* __restrict__ hp, const int* __restrict__ nb_off,
-                                                const int* __restrict__ nb_idx, const int* __restrict__ slot0,
-                                                const double* __restrict__ x, const double* __restrict__ hb_rad,
-                                                const int* __restrict__ Z, double* __restrict__ slots,
-                                                double* __restrict__ e_out) {
-    int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= nh) return;
+// Evaluates hydrogen bond h and returns its energy; gout receives the
+// gradient on the atom in `role` (0 = A, 1 = H, 2 = B, 3 + k = k-th
+// neighbour of B).
+__device__ double hb_eval(int h, int role, const int* __restrict__ ahb, const int* __restrict__ hcase,
+                          const double* __restrict__ hp, const int* __restrict__ nb_off,
+                          const int* __restrict__ nb_idx, const double* __restrict__ x,
+                          const double* __restrict__ hb_rad, const int* __restrict__ Z, double3& gout) {
     int A = ahb[3 * h], H = ahb[3 * h + 1], B = ahb[3 * h + 2];
     int cs = hcase[h];
     const double* q = hp + 7 * h;
     double basA = q[0], basB = q[1], aciA = q[2], aciB = q[3];
     double3 pA = ld3(x, A), pH = ld3(x, H), pB = ld3(x, B);
     double rAH = nrm(pH - pA), rHB = nrm(pB - pH), rAB = nrm(pB - pA);
-    int s0 = slot0[h];
     int nb0 = nb_off[h], nb1 = nb_off[h + 1];
     double3 z3 = make_double3(0, 0, 0);
     double3 ga = z3, gb = z3, gh = z3;
+    gout = z3;
     double E = 0.0;
     if (rAB <= 50.0) {
         double rAH4 = rAH * rAH * rAH * rAH, rHB4 = rHB * rHB * rHB * rHB;
@@ -911,8 +942,8 @@ __global__ __launch_bounds__(256) void k_hbonds(int nh, const int* __restrict__ 
                 ga = ga + dga;
                 gb = gb + dgb;
                 gh = gh + (-1.0) * (dga + dgb);
-                int q2 = s0 + 3;
-                for (int k = nb0; k < nb1; ++k, ++q2) {
+                int kk = 3;
+                for (int k = nb0; k < nb1; ++k, ++kk) {
                     double3 pn = ld3(x, nb_idx[k]);
                     double3 dranb = pA - pn, drbnb = pB - pn;
                     double ranb = nrm(dranb), rbnb = nrm(drbnb), sm = ranb + rbnb + 1e-12;
@@ -927,7 +958,7 @@ __global__ __launch_bounds__(256) void k_hbonds(int nh, const int* __restrict__ 
                     double3 da = (tmp2 / ranb) * dranb, db = (tmp2 / rbnb) * drbnb;
                     ga = ga + da;
                     gb = gb + db;
-                    st3(slots, q2, (-1.0) * (da + db));
+                    if (role == kk) gout = (-1.0) * (da + db);
                 }
             } else {
                 double caa = QA * basA, cbb = QB * basB;
@@ -954,18 +985,56 @@ __global__ __launch_bounds__(256) void k_hbonds(int nh, const int* __restrict__ 
                 gb = gb + db;
                 gh = gh + (-1.0) * (da + db);
             }
-        } else if (cs >= 2) {
-            int q2 = s0 + 3;
-            for (int k = nb0; k < nb1; ++k, ++q2) st3(slots, q2, z3);
         }
-    } else if (cs >= 2) {
-        int q2 = s0 + 3;
-        for (int k = nb0; k < nb1; ++k, ++q2) st3(slots, q2, z3);
     }
-    e_out[h] = E;
-    st3(slots, s0, ga);
-    st3(slots, s0 + 1, gh);
-    st3(slots, s0 + 2, gb);
+    if (role == 0) gout = ga;
+    else if (role == 1) gout = gh;
+    else if (role == 2) gout = gb;
+    return E;
+}
+
+__global__ __launch_bounds__(256) void k_hb_energy(int nh, const int* __restrict__ ahb, const int* __restrict__ hcase,
+                                                   const double* __restrict__ hp, const int* __restrict__ nb_off,
+                                                   const int* __restrict__ nb_idx, const double* __restrict__ x,
+                                                   const double* __restrict__ hb_rad, const int* __restrict__ Z,
+                                                   double* __restrict__ e_out) {
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    double3 g;
+    e_out[h] = hb_eval(h, -1, ahb, hcase, hp, nb_off, nb_idx, x, hb_rad, Z, g);
+}
+
+// owner-computes gradient: one wavefront per atom evaluates every hydrogen
+// bond the atom takes part in and keeps its own component; lanes stride the
+// atom's list and a fixed butterfly combines them (deterministic, no atomics)
+__global__ __launch_bounds__(256) void k_hb_owner(int n, const int* __restrict__ off, const int* __restrict__ ent,
+                                                  const int* __restrict__ ahb, const int* __restrict__ hcase,
+                                                  const double* __restrict__ hp, const int* __restrict__ nb_off,
+                                                  const int* __restrict__ nb_idx, const double* __restrict__ x,
+                                                  const double* __restrict__ hb_rad, const int* __restrict__ Z,
+                                                  double* __restrict__ ghb) {
+    int a = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    int lane = threadIdx.x % 64;
+    if (a >= n) return;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    for (int k = off[a] + lane; k < off[a + 1]; k += 64) {
+        int e = ent[k];
+        double3 g;
+        hb_eval(e >> 4, e & 15, ahb, hcase, hp, nb_off, nb_idx, x, hb_rad, Z, g);
+        gx += g.x;
+        gy += g.y;
+        gz += g.z;
+    }
+    for (int w = 32; w > 0; w >>= 1) {
+        gx += __shfl_xor(gx, w, 64);
+        gy += __shfl_xor(gy, w, 64);
+        gz += __shfl_xor(gz, w, 64);
+    }
+    if (lane == 0) {
+        ghb[3 * a] = gx;
+        ghb[3 * a + 1] = gy;
+        ghb[3 * a + 2] = gz;
+    }
 }
 
 // D4 ATM over bonded triplets (energy +c9, gradient -c9 as in
@@ -1079,10 +1148,10 @@ __global__ __launch_bounds__(256) void k_finish(int n, const double* __restrict_
                                                 const int* __restrict__ nbn, const int* __restrict__ nbl,
                                                 const int* __restrict__ soff, const int* __restrict__ sidx,
                                                 const double* __restrict__ slots, const double* __restrict__ gpair,
-                                                double* __restrict__ grad) {
+                                                const double* __restrict__ ghb, double* __restrict__ grad) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double3 g = ld3(gpair, i);
+    double3 g = ld3(gpair, i) + ld3(ghb, i);
     for (int k = soff[i]; k < soff[i + 1]; ++k) g = g + ld3(slots, sidx[k]);
     double3 xi = ld3(x, i);
     double ai = dlog[i] * v[i];
@@ -1108,20 +1177,31 @@ struct Segs {
     int n[16];
     int dst[16];
 };
-__global__ __launch_bounds__(256) void k_reduce(Segs s, double* __restrict__ out) {
+constexpr int RED_BLOCKS = 64;
+// stage 1: segment sg, block b reduces the fixed chunk b of the segment
+__global__ __launch_bounds__(256) void k_reduce1(Segs s, double* __restrict__ part) {
     __shared__ double red[256];
-    int sg = blockIdx.x;
+    int sg = blockIdx.y, b = blockIdx.x;
     const double* p = s.p[sg];
     int n = s.n[sg];
+    int chunk = (n + RED_BLOCKS - 1) / RED_BLOCKS;
+    int lo = b * chunk, hi = min(n, lo + chunk);
     double acc = 0.0;
-    for (int k = threadIdx.x; k < n; k += 256) acc += p[k];
+    for (int k = lo + threadIdx.x; k < hi; k += 256) acc += p[k];
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[s.dst[sg]] = red[0];
+    if (threadIdx.x == 0) part[sg * RED_BLOCKS + b] = red[0];
+}
+// stage 2: one wavefront per segment, fixed tree over the block partials
+__global__ __launch_bounds__(64) void k_reduce2(Segs s, const double* __restrict__ part, double* __restrict__ out) {
+    int sg = blockIdx.x;
+    double v = part[sg * RED_BLOCKS + threadIdx.x];
+    for (int w = 32; w > 0; w >>= 1) v += __shfl_xor(v, w, 64);
+    if (threadIdx.x == 0) out[s.dst[sg]] = v;
 }
 
 // out[0] = total, out[1..15] = terms (term index + 1), fixed summation order
@@ -1185,7 +1265,7 @@ struct DeviceEngine {
     double* tor_par;
     int *brep_ij, *brep_slot;
     double* brep_par;
-    int *hb_ahb, *hb_case, *hb_nboff, *hb_nbidx, *hb_slot;
+    int *hb_ahb, *hb_case, *hb_nboff, *hb_nbidx, *hbo_off, *hbo_ent;
     double* hb_par;
     int *atm_ijk, *atm_slot;
     double* atm_sc;
@@ -1195,7 +1275,7 @@ struct DeviceEngine {
     // work
     double *x, *cn, *dlog, *gw, *dgw, *rhs, *Ared, *Linv, *b, *rres, *q, *eself, *qtmp, *gpair, *dEdcn, *vchain,
         *slots, *dcn_slot, *grad, *e_coul, *e_disp, *e_rep, *e_bond, *e_ang, *e_tor, *e_brep, *e_hb, *e_atm,
-        *e_batm, *parts, *out, *diag;
+        *e_batm, *parts, *rpart, *out, *diag, *ghb, *ybuf, *xbuf;
     int *nbn, *nbl, *flags;
 
     template <class T>
@@ -1371,19 +1451,31 @@ DeviceEngine* device_create(const ParamSet& P, int precision, int& status, std::
         E->brep_par = E->upload(P.brep_par);
         E->brep_slot = E->upload(rslot);
         E->nhb = (int)P.hb_case.size();
-        std::vector<int> hslot(E->nhb);
-        for (int h = 0; h < E->nhb; ++h) {
-            hslot[h] = add_slot(P.hb_ahb[3 * h]);
-            add_slot(P.hb_ahb[3 * h + 1]);
-            add_slot(P.hb_ahb[3 * h + 2]);
-            for (int k = P.hb_nbB_off[h]; k < P.hb_nbB_off[h + 1]; ++k) add_slot(P.hb_nbB_idx[k]);
+        {
+            // per-atom list of (hb << 4 | role) for the owner-computes gradient
+            std::vector<int> cnt(n + 1, 0);
+            for (int h = 0; h < E->nhb; ++h) {
+                int nbb = P.hb_nbB_off[h + 1] - P.hb_nbB_off[h];
+                if (nbb > 12 || h >= (1 << 27)) throw std::runtime_error("hydrogen-bond list exceeds the owner encoding");
+                for (int r = 0; r < 3; ++r) cnt[P.hb_ahb[3 * h + r] + 1]++;
+                for (int k = P.hb_nbB_off[h]; k < P.hb_nbB_off[h + 1]; ++k) cnt[P.hb_nbB_idx[k] + 1]++;
+            }
+            for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+            std::vector<int> ent(cnt[n]), fill(cnt.begin(), cnt.end() - 1);
+            for (int h = 0; h < E->nhb; ++h) {
+                for (int r = 0; r < 3; ++r) ent[fill[P.hb_ahb[3 * h + r]]++] = (h << 4) | r;
+                int kk = 3;
+                for (int k = P.hb_nbB_off[h]; k < P.hb_nbB_off[h + 1]; ++k, ++kk)
+                    ent[fill[P.hb_nbB_idx[k]]++] = (h << 4) | kk;
+            }
+            E->hbo_off = E->upload(cnt);
+            E->hbo_ent = E->upload(ent);
         }
         E->hb_ahb = E->upload(P.hb_ahb);
         E->hb_case = E->upload(P.hb_case);
         E->hb_par = E->upload(P.hb_par);
         E->hb_nboff = E->upload(P.hb_nbB_off);
         E->hb_nbidx = E->upload(P.hb_nbB_idx);
-        E->hb_slot = E->upload(hslot);
         E->natm = (int)P.atm_scale.size();
         std::vector<int> atslot(E->natm);
         for (int t = 0; t < E->natm; ++t) {
@@ -1442,6 +1534,8 @@ DeviceEngine* device_create(const ParamSet& P, int precision, int& status, std::
         E->Ared = E->alloc<double>((size_t)E->np * E->np);
         E->Linv = E->alloc<double>((size_t)E->np * NB);
         E->b = E->alloc<double>(E->np);
+        E->ybuf = E->alloc<double>(E->np);
+        E->xbuf = E->alloc<double>(E->np);
         E->rres = E->alloc<double>(n);
         E->q = E->alloc<double>(n);
         E->eself = E->alloc<double>(n);
@@ -1463,6 +1557,8 @@ DeviceEngine* device_create(const ParamSet& P, int precision, int& status, std::
         E->e_atm = E->alloc<double>(std::max(E->natm, 1));
         E->e_batm = E->alloc<double>(std::max(E->nbatm, 1));
         E->parts = E->alloc<double>(16);
+        E->rpart = E->alloc<double>(16 * RED_BLOCKS);
+        E->ghb = E->alloc<double>(3 * (size_t)n);
         E->out = E->alloc<double>(GFNFF_NTERMS + 1);
         E->nbn = E->alloc<int>(n);
         E->nbl = E->alloc<int>((size_t)n * MAXNB_CN);
@@ -1508,7 +1604,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
                        rsrc, E->b);
     for (int k = 0; k < nbk; ++k) {
         int k0 = k * NB;
-        hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, s, np, k0, E->Ared, E->Linv, E->flags);
+        hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(NB), 0, s, np, k0, E->Ared, E->Linv, E->flags);
         int rem = nbk - k - 1;
         if (rem > 0) {
             hipLaunchKernelGGL(k_chol_panel, dim3(rem), dim3(256), 0, s, np, k0, E->Ared, E->Linv);
@@ -1517,19 +1613,17 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     }
     for (int k = 0; k < nbk; ++k) {
         int k0 = k * NB;
-        hipLaunchKernelGGL(k_trsv_fwd_diag, dim3(1), dim3(NB), 0, s, k0, E->Linv, E->b);
         int rows = np - k0 - NB;
-        if (rows > 0)
-            hipLaunchKernelGGL(k_trsv_fwd_update, dim3(cdiv(rows, 256)), dim3(256), 0, s, np, k0, E->Ared, E->b);
+        hipLaunchKernelGGL(k_trsv_fwd, dim3(std::max(1, cdiv(rows, 256))), dim3(256), 0, s, np, k0, E->Ared, E->Linv,
+                           E->b, E->ybuf);
     }
     for (int k = nbk - 1; k >= 0; --k) {
         int k0 = k * NB;
-        hipLaunchKernelGGL(k_trsv_bwd_diag, dim3(1), dim3(NB), 0, s, k0, E->Linv, E->b);
-        if (k0 > 0)
-            hipLaunchKernelGGL(k_trsv_bwd_update, dim3(cdiv(k0, 256)), dim3(256), 0, s, np, k0, E->Ared, E->b);
+        hipLaunchKernelGGL(k_trsv_bwd, dim3(std::max(1, cdiv(k0, 256))), dim3(256), 0, s, np, k0, E->Ared, E->Linv,
+                           E->ybuf, E->xbuf);
     }
     hipLaunchKernelGGL(k_charges, dim3(cdiv(n, 256)), dim3(256), 0, s, n, E->red_of, (const int*)nullptr,
-                       E->frag_off, E->frag_atoms, E->frag, E->qp, E->b, E->chi_base, E->cnf, E->gam, E->alp, E->cn,
+                       E->frag_off, E->frag_atoms, E->frag, E->qp, E->xbuf, E->chi_base, E->cnf, E->gam, E->alp, E->cn,
                        E->q, E->eself, E->qtmp);
     // pair terms
     PairArgs pa;
@@ -1572,8 +1666,10 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         hipLaunchKernelGGL(k_brep, dim3(cdiv(E->nbrep, 256)), dim3(256), 0, s, E->nbrep, E->brep_ij, E->brep_par,
                            E->brep_slot, E->x, E->slots, E->e_brep);
     if (E->nhb)
-        hipLaunchKernelGGL(k_hbonds, dim3(cdiv(E->nhb, 256)), dim3(256), 0, s, E->nhb, E->hb_ahb, E->hb_case,
-                           E->hb_par, E->hb_nboff, E->hb_nbidx, E->hb_slot, E->x, E->hb_rad, E->Z, E->slots, E->e_hb);
+        hipLaunchKernelGGL(k_hb_energy, dim3(cdiv(E->nhb, 256)), dim3(256), 0, s, E->nhb, E->hb_ahb, E->hb_case,
+                           E->hb_par, E->hb_nboff, E->hb_nbidx, E->x, E->hb_rad, E->Z, E->e_hb);
+    hipLaunchKernelGGL(k_hb_owner, dim3(cdiv(n, 4)), dim3(256), 0, s, n, E->hbo_off, E->hbo_ent, E->hb_ahb,
+                       E->hb_case, E->hb_par, E->hb_nboff, E->hb_nbidx, E->x, E->hb_rad, E->Z, E->ghb);
     if (E->natm)
         hipLaunchKernelGGL(k_atm, dim3(cdiv(E->natm, 256)), dim3(256), 0, s, E->natm, E->atm_ijk, E->atm_sc,
                            E->atm_slot, E->x, E->rcov_d3, E->nelem, E->elem, E->nref, E->c6ref, E->gw, E->slots,
@@ -1584,7 +1680,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     hipLaunchKernelGGL(k_dcn_total, dim3(cdiv(n, 256)), dim3(256), 0, s, n, E->dcn_off, E->dcn_idx, E->dcn_slot,
                        E->dEdcn, E->qtmp, E->vchain);
     hipLaunchKernelGGL(k_finish, dim3(cdiv(n, 256)), dim3(256), 0, s, n, E->x, E->rcov_cn, E->dlog, E->vchain,
-                       E->nbn, E->nbl, E->slot_off, E->slot_idx, E->slots, E->gpair, E->grad);
+                       E->nbn, E->nbl, E->slot_off, E->slot_idx, E->slots, E->gpair, E->ghb, E->grad);
     Segs sg;
     const double* ptrs[12] = {E->e_bond, E->e_ang, E->e_tor, nullptr, E->e_disp, E->e_brep,
                               E->e_rep,  E->e_coul, E->eself, E->e_hb, E->e_atm, E->e_batm};
@@ -1594,7 +1690,8 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         sg.n[k] = ptrs[k] ? cnts[k] : 0;
         sg.dst[k] = k;
     }
-    hipLaunchKernelGGL(k_reduce, dim3(12), dim3(256), 0, s, sg, E->parts);
+    hipLaunchKernelGGL(k_reduce1, dim3(RED_BLOCKS, 12), dim3(256), 0, s, sg, E->rpart);
+    hipLaunchKernelGGL(k_reduce2, dim3(12), dim3(64), 0, s, sg, E->rpart, E->parts);
     hipLaunchKernelGGL(k_total, dim3(1), dim3(64), 0, s, E->parts, E->out);
     if (hipGetLastError() != hipSuccess) {
         msg = "kernel launch failed";
@@ -1646,6 +1743,60 @@ int device_count() {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+// Blocked Cholesky of a dense SPD matrix (the EEQ factorisation kernels on an
+// arbitrary input).  a: n x n row-major; l_out receives the lower factor.
+int device_dense_cholesky(int n, const double* a, double* l_out, std::string& msg) {
+    if (n <= 0) {
+        msg = "n must be positive";
+        return kErrArgs;
+    }
+    const int np = cdiv(n, NB) * NB, nbk = np / NB;
+    std::vector<double> h((size_t)np * np, 0.0);
+    for (int i = 0; i < np; ++i)
+        for (int j = 0; j < np; ++j) h[(size_t)i * np + j] = (i < n && j < n) ? a[(size_t)i * n + j] : (i == j ? 1.0 : 0.0);
+    double *dA = nullptr, *dL = nullptr;
+    int* dflag = nullptr;
+    hipStream_t s = nullptr;
+    int st = kOk;
+    auto cleanup = [&]() {
+        if (s) (void)hipStreamDestroy(s);
+        if (dA) (void)hipFree(dA);
+        if (dL) (void)hipFree(dL);
+        if (dflag) (void)hipFree(dflag);
+    };
+    if (hipStreamCreate(&s) != hipSuccess || hipMalloc(&dA, h.size() * 8) != hipSuccess ||
+        hipMalloc(&dL, (size_t)np * NB * 8) != hipSuccess || hipMalloc(&dflag, sizeof(int)) != hipSuccess) {
+        cleanup();
+        msg = "device allocation failed";
+        return kErrDevice;
+    }
+    int zero = 0, flag = 0;
+    (void)hipMemcpyAsync(dA, h.data(), h.size() * 8, hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(dflag, &zero, sizeof(int), hipMemcpyHostToDevice, s);
+    for (int k = 0; k < nbk; ++k) {
+        int k0 = k * NB;
+        hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(NB), 0, s, np, k0, dA, dL, dflag);
+        int rem = nbk - k - 1;
+        if (rem > 0) {
+            hipLaunchKernelGGL(k_chol_panel, dim3(rem), dim3(256), 0, s, np, k0, dA, dL);
+            hipLaunchKernelGGL(k_chol_update, dim3(rem * (rem + 1) / 2), dim3(256), 0, s, np, k0, dA);
+        }
+    }
+    (void)hipMemcpyAsync(h.data(), dA, h.size() * 8, hipMemcpyDeviceToHost, s);
+    (void)hipMemcpyAsync(&flag, dflag, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        msg = "kernel failure";
+        st = kErrDevice;
+    } else if (flag & FLAG_CHOL) {
+        msg = "matrix is not positive definite";
+        st = kErrEEQ;
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) l_out[(size_t)i * n + j] = (j <= i) ? h[(size_t)i * np + j] : 0.0;
+    cleanup();
+    return st;
 }
 
 }  // namespace gfnff

@@ -26,6 +26,9 @@ int device_eval(DeviceEngine* e, const double* xyz_bohr_dev, double* energy_dev,
 
 int device_count();
 
+// Dense SPD factorisation with the EEQ Cholesky kernels (host buffers).
+int device_dense_cholesky(int n, const double* a, double* l_out, std::string& msg);
+
 }  // namespace gfnff
 
 // unique_ptr support

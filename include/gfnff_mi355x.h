@@ -97,8 +97,14 @@ int gfnff_get_params(const gfnff_handle* h, gfnff_params_view* out);
 int gfnff_get_topology(const gfnff_handle* h, int32_t* hyb, int32_t* pi, int32_t* bonds, int32_t nbonds_max,
                        int32_t* nbonds);
 
-/* Number of visible GPUs (0 when none); never initialises a context. */
+/* Number of visible GPUs (0 when none). */
 int gfnff_device_count(void);
+
+/* Engine extension (no counterpart in the reference): dense Cholesky of an
+ * n x n SPD matrix (row-major, host buffers) with the engine's blocked FP64
+ * factorisation kernels; l_out receives the lower factor.  Used to test the
+ * EEQ factorisation in isolation. */
+int gfnff_dense_cholesky(int32_t n, const double* a, double* l_out);
 
 #ifdef __cplusplus
 }
