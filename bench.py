@@ -26,8 +26,6 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "GFN-FF E+grad atom·evals/s, mixed vs FP64 on 1 GPU at |ΔE|≤1e-6 Eh/atom"
-PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix), FLOP/s
-PEAK_HBM = 8.0e12    # bytes/s
 
 
 def parse():
@@ -36,9 +34,13 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="water_3000")
-    p.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    p.add_argument("--precision", default="mixed", choices=["fp64", "mixed"],
+                   help="path of the headline number; the FP64 path is always timed beside it")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample", default="water_400")
+    p.add_argument("--roofline-kernel", default="auto", help="kernel class timed for the roofline object")
+    p.add_argument("--copy-calibration-mib", type=int, default=0,
+                   help="run one device copy of this size first (HBM counter calibration under rocprofv3 --pmc)")
     return p.parse_args()
 
 
@@ -63,13 +65,26 @@ def cpu_baseline(sample: str):
     while True:
         orc.eval(v, x, grad=True)
         reps += 1
-        if time.perf_counter() - t0 > 10.0 or reps >= 3:
+        if time.perf_counter() - t0 > 10.0 and reps >= 3:
             break
     dt = (time.perf_counter() - t0) / reps
     return {"value": len(Z) / dt, "unit": "atom*evals/s", "cores": 1, "kind": "port",
             "sample": f"water cluster of {nmol} H2O ({len(Z)} atoms) from the same generator, "
                       f"{reps} FP64 energy+gradient evaluation(s) of the single-threaded CPU oracle, "
                       f"{dt:.3f} s each"}
+
+
+def pmc_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (profiles/pmc_<kernel>_<workload>.md, line 'traffic_bytes_per_launch: X'),
+    or None when no such summary exists."""
+    f = ROOT / "profiles" / f"pmc_{kernel}_{workload}.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def main():
@@ -83,47 +98,92 @@ def main():
     from curcuma_amd._lib import NTERMS
     from curcuma_amd.workloads import WORKLOADS
 
+    if args.copy_calibration_mib > 0:
+        nel = args.copy_calibration_mib * (1 << 20) // 8
+        a = torch.ones(nel, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
     Z, xyz = WORKLOADS[args.workload]()
     n = len(Z)
-    m = GFNFFMethod(precision=args.precision)
-    t0 = time.perf_counter()
-    if not m.setMolecule(Mol(Z, xyz)):
-        print("setMolecule failed: " + m.getErrorMessage(), file=sys.stderr)
-        sys.exit(1)
-    t_setup = time.perf_counter() - t0
-    lib = m._lib
     dev = torch.device("cuda:0")
     x_dev = torch.tensor(xyz * (1.0 / 0.529177210903), dtype=torch.float64, device=dev).contiguous()
-    e_dev = torch.zeros(NTERMS + 1, dtype=torch.float64, device=dev)
-    g_dev = torch.zeros((n, 3), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream()
     sp = C.c_void_p(stream.cuda_stream)
 
-    def step():
-        st = lib.gfnff_energy_gradient_device(m.handle, C.c_void_p(x_dev.data_ptr()), C.c_void_p(e_dev.data_ptr()),
-                                              C.c_void_p(g_dev.data_ptr()), sp)
-        if st != 0:
-            raise RuntimeError(lib.gfnff_last_error(m.handle).decode())
+    def make(precision):
+        m = GFNFFMethod(precision=precision)
+        t0 = time.perf_counter()
+        if not m.setMolecule(Mol(Z, xyz)):
+            print("setMolecule failed: " + m.getErrorMessage(), file=sys.stderr)
+            sys.exit(1)
+        e_dev = torch.zeros(NTERMS + 1, dtype=torch.float64, device=dev)
+        g_dev = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+        lib = m._lib
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+        def step():
+            st = lib.gfnff_energy_gradient_device(m.handle, C.c_void_p(x_dev.data_ptr()),
+                                                  C.c_void_p(e_dev.data_ptr()), C.c_void_p(g_dev.data_ptr()), sp)
+            if st != 0:
+                raise RuntimeError(lib.gfnff_last_error(m.handle).decode())
+        return m, step, e_dev, g_dev, time.perf_counter() - t0
+
+    from curcuma_amd.profiling import KernelTimer, roofline, survey
+
+    def timed(m, step, kernel):
+        """Warm-up, per-class survey, then the timed region with HIP events on
+        the evaluation stream around every launch of `kernel`."""
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        timer = KernelTimer(m.handle, m._lib)
+        per_class = survey(timer, step)
+        k = kernel if kernel != "auto" else max(per_class, key=lambda c: per_class[c][0])
+        timer.select(k)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        k_ms, k_launches = timer.read()
+        timer.select(None)
+        return dt, k, k_ms, k_launches, per_class
+
+    # FP64 path (the reference's arithmetic) and the mixed-precision path on the same input
+    m64, step64, e64_dev, g64_dev, setup64 = make("fp64")
+    dt64, _, _, _, classes64 = timed(m64, step64, "pairs")
+    headline = args.precision
+    if headline == "mixed":
+        mh, steph, eh_dev, gh_dev, setup_h = make("mixed")
+        dt, kernel, k_ms, k_launches, per_class = timed(mh, steph, args.roofline_kernel)
+    else:
+        mh, steph, eh_dev, gh_dev, setup_h = m64, step64, e64_dev, g64_dev, setup64
+        dt, kernel, k_ms, k_launches, per_class = timed(mh, steph, args.roofline_kernel)
     ms = dt / args.steps * 1e3
     value = n * args.steps / dt
+    value64 = n * args.steps / dt64
 
-    # accuracy against the host API path of the same engine (sanity) and timing
-    e_last = float(e_dev[0].item())
-    roof = None
-    try:
-        from curcuma_amd.profiling import kernel_roofline
-        roof = kernel_roofline(m, x_dev, e_dev, g_dev, stream, args.steps)
-    except Exception as ex:  # profiling is optional for the headline number
-        print(f"roofline measurement failed: {ex}", file=sys.stderr)
+    e64 = float(e64_dev[0].item())
+    eh = float(eh_dev[0].item())
+    g64 = g64_dev.cpu().numpy()
+    gh = gh_dev.cpu().numpy()
+    accuracy = {"vs": "FP64 GPU path (parity-pinned to the FP64 oracle)",
+                "dE_per_atom_Eh": abs(eh - e64) / n,
+                "grad_rms_Eh_per_bohr": float(np.sqrt(np.mean((gh - g64) ** 2))),
+                "bar": "|dE| <= 1e-6 Eh/atom, grad RMS <= 1e-5 Eh/Bohr"}
+
+    pv = mh.params_view()
+    nr = pv.natoms - pv.nfrag
+    info = {"xyz_bohr": xyz * (1.0 / 0.529177210903), "np_pad": max(64, -(-nr // 64) * 64), "nhb": pv.nhb,
+            "nbond": pv.nbond}
+    roof = roofline(kernel, k_ms, k_launches, args.steps, info, traffic=pmc_traffic(kernel, args.workload),
+                    precision=headline)
+    roof["kernel_ms_per_eval"] = {c: round(v[0], 4) for c, v in sorted(per_class.items(), key=lambda t: -t[1][0])}
+    t_setup = setup_h
+    e_last = eh
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -141,12 +201,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f64" if headline == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: non-periodic water cluster, 3000 H2O (9000 atoms), 3.1 A cubic "
-                               "lattice, seeded random orientations, +-0.05 A jitter; FP64 drop-in",
-                   "atoms": n, "precision": args.precision, "setup_s": round(t_setup, 3),
-                   "energy_Eh": e_last},
+                               "lattice, seeded random orientations, +-0.05 A jitter",
+                   "atoms": n, "precision": headline, "setup_s": round(t_setup, 3), "energy_Eh": e_last},
+        "fp64": {"value": value64, "ms_per_step": dt64 / args.steps * 1e3, "energy_Eh": e64,
+                 "kernel_ms_per_eval": {c: round(v[0], 4) for c, v in
+                                        sorted(classes64.items(), key=lambda t: -t[1][0])}},
+        "mixed_over_fp64": value / value64,
+        "accuracy": accuracy,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

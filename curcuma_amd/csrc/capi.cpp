@@ -211,6 +211,31 @@ int gfnff_get_topology(const gfnff_handle* h, int32_t* hyb, int32_t* pi, int32_t
 
 int gfnff_device_count(void) { return gfnff::device_count(); }
 
+int gfnff_cn_neighbours(gfnff_handle* h, int32_t* counts, int32_t* idx, int32_t maxnb) {
+    if (!h || !counts || !idx || maxnb <= 0 || !h->dev) return GFNFF_STATUS_ARGS;
+    std::string msg;
+    int st = gfnff::device_cn_neighbours(h->dev.get(), counts, idx, maxnb, msg);
+    return st ? fail(h, st, "gfnff_cn_neighbours: " + msg) : GFNFF_STATUS_OK;
+}
+
+int gfnff_profile_kernel(gfnff_handle* h, const char* name) {
+    if (!h || !h->have_molecule) return GFNFF_STATUS_ARGS;
+    int st = ensure_device(h);
+    if (st) return st;
+    std::string msg;
+    st = gfnff::device_profile_select(h->dev.get(), name, msg);
+    return st ? fail(h, st, "gfnff_profile_kernel: " + msg) : GFNFF_STATUS_OK;
+}
+
+int gfnff_profile_read(gfnff_handle* h, double* total_ms, int32_t* launches) {
+    if (!h || !total_ms || !launches || !h->dev) return GFNFF_STATUS_ARGS;
+    std::string msg;
+    int nl = 0;
+    int st = gfnff::device_profile_read(h->dev.get(), total_ms, &nl, msg);
+    *launches = nl;
+    return st ? fail(h, st, "gfnff_profile_read: " + msg) : GFNFF_STATUS_OK;
+}
+
 int gfnff_dense_cholesky(int32_t n, const double* a, double* l_out) {
     if (n <= 0 || !a || !l_out) return GFNFF_STATUS_ARGS;
     std::string msg;

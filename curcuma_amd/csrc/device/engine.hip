@@ -6,8 +6,10 @@
 //   k_chol_*        blocked Cholesky (diag / panel / trailing update)
 //   k_trsv_*        forward/backward substitution
 //   k_charges       charges from the reduced solution, self energy, qtmp
-//   k_pairs         Coulomb + D4 dispersion + non-bonded repulsion, LDS-tiled,
-//                   one row atom per thread, full rows (no atomics)
+//   k_d4_half       per-atom D4 half contraction of C6 and dC6/dCN
+//   k_pairs_part    Coulomb + D4 dispersion + non-bonded repulsion, LDS-tiled,
+//                   one row atom per thread, full rows split in j chunks
+//   k_pairs_reduce  fixed-order sum of the chunk partials (no atomics)
 //   k_nbrep_fix     bonded / 1,3 / 1,4 H-H corrections of the repulsion pair sum
 //   k_bonds .. k_batm   bonded term lists writing per-atom gradient slots
 //   k_finish        slot gather + CN chain rule + gradient assembly
@@ -37,6 +39,7 @@ namespace {
     } while (0)
 
 constexpr int NB = 64;          // Cholesky block size
+#define MFMA_F32_ROW_FROM_REG false  // FP32 16x16x4 result layout (see MfmaT)
 constexpr int MAXNB_CN = 128;   // CN neighbour list capacity per atom
 constexpr int TILE = 256;       // pair kernel tile / workgroup size
 
@@ -70,46 +73,65 @@ __device__ inline void st3(double* s, int slot, double3 v) {
 // K1: CN, log cap, dlogdcn, D4 weights, EEQ right-hand side, neighbour list
 // cn_calculator.cpp:367-428; gfnff_method.cpp:5880-5930; d4param_generator.cpp:1079-1213,1692-1768
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(TILE) void k_cn(int n, const double* __restrict__ x, const double* __restrict__ rcov,
-                                             const int* __restrict__ elem, const int* __restrict__ nref,
-                                             const double* __restrict__ refcn, const double* __restrict__ chi_base,
-                                             const double* __restrict__ cnf, double* __restrict__ cn,
-                                             double* __restrict__ dlog, double* __restrict__ gw,
-                                             double* __restrict__ dgw, double* __restrict__ rhs,
-                                             int* __restrict__ nbn, int* __restrict__ nbl, int* __restrict__ flags) {
-    __shared__ double sx[TILE], sy[TILE], sz[TILE], sr[TILE];
-    int i = blockIdx.x * TILE + threadIdx.x;
-    double3 xi = i < n ? ld3(x, i) : make_double3(0, 0, 0);
-    double ri = i < n ? rcov[i] : 0.0;
+// One wavefront per atom i (four atoms per workgroup share the LDS j tiles).
+// Lanes test 64 partners j at a time; __ballot and a prefix popcount give the
+// position of every hit, so each neighbour list is ascending in j with the
+// strict d^2 < 36 compare of the reference's builder (cn_calculator.cpp:
+// 367-428).  The raw CN is summed per lane and combined by a fixed butterfly.
+__global__ __launch_bounds__(256) void k_cn(int n, const double* __restrict__ x, const double* __restrict__ rcov,
+                                            const int* __restrict__ elem, const int* __restrict__ nref,
+                                            const double* __restrict__ refcn, const double* __restrict__ chi_base,
+                                            const double* __restrict__ cnf, double* __restrict__ cn,
+                                            double* __restrict__ dlog, double* __restrict__ gw,
+                                            double* __restrict__ dgw, double* __restrict__ rhs,
+                                            int* __restrict__ nbn, int* __restrict__ nbl, int* __restrict__ flags) {
+    __shared__ double sx[256], sy[256], sz[256], sr[256];
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const bool act = i < n;
+    double3 xi = act ? ld3(x, i) : make_double3(0, 0, 0);
+    double ri = act ? rcov[i] : 0.0;
     double raw = 0.0;
     int cnt = 0;
-    for (int t0 = 0; t0 < n; t0 += TILE) {
-        int j = t0 + threadIdx.x;
+    for (int t0 = 0; t0 < n; t0 += 256) {
         __syncthreads();
-        if (j < n) {
-            sx[threadIdx.x] = x[3 * j];
-            sy[threadIdx.x] = x[3 * j + 1];
-            sz[threadIdx.x] = x[3 * j + 2];
-            sr[threadIdx.x] = rcov[j];
+        int jl = t0 + threadIdx.x;
+        if (jl < n) {
+            sx[threadIdx.x] = x[3 * jl];
+            sy[threadIdx.x] = x[3 * jl + 1];
+            sz[threadIdx.x] = x[3 * jl + 2];
+            sr[threadIdx.x] = rcov[jl];
         }
         __syncthreads();
-        int m = min(TILE, n - t0);
-        if (i < n) {
-            for (int jj = 0; jj < m; ++jj) {
-                int j2 = t0 + jj;
-                if (j2 == i) continue;
+        const int m = min(256, n - t0);
+        for (int s0 = 0; s0 < m; s0 += 64) {
+            const int jj = s0 + lane, j = t0 + jj;
+            bool hit = false;
+            double contrib = 0.0;
+            if (act && jj < m && j != i) {
                 double dx = xi.x - sx[jj], dy = xi.y - sy[jj], dz = xi.z - sz[jj];
-                double d2 = dx * dx + dy * dy + dz * dz;
-                if (!(d2 < 36.0)) continue;
-                double r = sqrt(d2);
-                double rij = ri + sr[jj];
-                raw += 0.5 * (1.0 + erf(KN * (r - rij) / rij));
-                if (cnt < MAXNB_CN) nbl[(size_t)i * MAXNB_CN + cnt] = j2;
-                ++cnt;
+                // unfused products and sums: the list membership test matches
+                // the reference's ((dx*dx + dy*dy) + dz*dz) < 36 bit for bit
+                double d2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+                if (d2 < 36.0) {
+                    hit = true;
+                    double r = sqrt(d2);
+                    double rij = ri + sr[jj];
+                    contrib = 0.5 * (1.0 + erf(KN * (r - rij) / rij));
+                }
             }
+            const unsigned long long mask = __ballot(hit);
+            if (hit) {
+                int pos = cnt + __popcll(mask & ((1ull << lane) - 1ull));
+                if (pos < MAXNB_CN) nbl[(size_t)i * MAXNB_CN + pos] = j;
+                raw += contrib;
+            }
+            cnt += __popcll(mask);
         }
     }
-    if (i >= n) return;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) raw += __shfl_xor(raw, off);
+    if (!act || lane != 0) return;
     if (cnt > MAXNB_CN) atomicOr(flags, FLAG_CNOVF);
     nbn[i] = min(cnt, MAXNB_CN);
     double c = log(1.0 + exp(CNMAX)) - log(1.0 + exp(CNMAX - raw));
@@ -217,79 +239,85 @@ __global__ __launch_bounds__(256) void k_eeq_rhs_reduce(int nr, int np, const in
 // ---------------------------------------------------------------------------
 // K3: blocked Cholesky, right-looking, lower, row-major with leading dim np.
 // ---------------------------------------------------------------------------
-// broadcast of a lane's double to the whole wavefront through SGPRs
-__device__ __forceinline__ double bcast(double v, int lane) {
-    int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-    int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-    return __hiloint2double(hi, lo);
-}
-
-// One wavefront factorises the 64x64 diagonal block: lane j keeps column j
-// in registers.  At step k lane k finalises its column; the column entries
-// reach the other lanes as SGPR broadcasts (readlane), so the rank-1 update
-// needs no LDS round trips.  The inverse of the factor is built the same way,
-// lane c computing column c by forward substitution with broadcast rows.
-__global__ __launch_bounds__(64) void k_chol_diag(int np, int k0, double* __restrict__ A,
-                                                  double* __restrict__ Linv, int* __restrict__ flags) {
-    const int j = threadIdx.x;
-    double c[NB];
+// Factorisation L L^T of one NB x NB diagonal block and the inverse of L by
+// one workgroup of 256 threads.  Thread (t, q) = (tid / 4, tid % 4) keeps the
+// entries j = 4u + q of row t in registers.  At step k the owners of column k
+// publish it in LDS (double-buffered, one barrier per step) and every thread
+// updates A(t, j) -= A(t, k) A(j, k) / A(k, k) for j > k from broadcast reads.
+// The inverse X = L^{-1} is formed row by row: thread (c, q) keeps
+// X(4u + q, c) in registers, sums its share of L(r, :) X(:, c), and the four
+// shares are combined with quad shuffles (no barrier).  Both step loops are
+// unrolled so that every register index is static and the LDS reads of a step
+// issue back to back.
+template <typename T>
+__global__ __launch_bounds__(256) void k_chol_diag(int np, int k0, T* __restrict__ A, T* __restrict__ Linv,
+                                                   int* __restrict__ flags) {
+    __shared__ T sL[NB][NB + 1];
+    __shared__ T col[2][NB];
+    __shared__ T rdiag[NB];
+    const int tid = threadIdx.x;
+    const int t = tid >> 2, q = tid & 3;
+    for (int e = tid; e < NB * NB; e += 256) sL[e / NB][e % NB] = A[(size_t)(k0 + e / NB) * np + k0 + e % NB];
+    __syncthreads();
+    T a[NB / 4];
 #pragma unroll
-    for (int i = 0; i < NB; ++i) c[i] = (i >= j) ? A[(size_t)(k0 + i) * np + k0 + j] : 0.0;
+    for (int u = 0; u < NB / 4; ++u) a[u] = sL[t][4 * u + q];
     int bad = 0;
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        double dk = bcast(c[k], k);
-        if (!(dk > 0.0)) {
+        const int buf = k & 1;
+        if (q == (k & 3)) col[buf][t] = a[k >> 2];
+        __syncthreads();
+        T d = col[buf][k];
+        if (!(d > T(0))) {
             bad = 1;
-            dk = 1.0;
+            d = T(1);
         }
-        dk = sqrt(dk);
-        double inv = 1.0 / dk;
-        // column k (owned by lane k): L(i,k) = c_k[i] / dk for i > k
-        double lk[NB];
+        const T sd = sqrt(d);
+        const T ct = col[buf][t];
+        const T f = ct / d;
 #pragma unroll
-        for (int i = k + 1; i < NB; ++i) lk[i] = bcast(c[i], k) * inv;
-        if (j == k) {
-            c[k] = dk;
-#pragma unroll
-            for (int i = k + 1; i < NB; ++i) c[i] = lk[i];
-        }
-        // lane j > k: c[i] -= L(i,k) L(j,k) for i >= j
-        double ljk = 0.0;
-#pragma unroll
-        for (int i = k + 1; i < NB; ++i) ljk = (i == j) ? lk[i] : ljk;
-        if (j > k) {
-#pragma unroll
-            for (int i = k + 1; i < NB; ++i)
-                if (i >= j) c[i] -= lk[i] * ljk;
-        }
+        for (int u = k >> 2; u < NB / 4; ++u)
+            if (4 * u + q > k) a[u] -= f * col[buf][4 * u + q];
+        if (q == (k & 3)) a[k >> 2] = (t == k) ? sd : (t > k ? ct / sd : a[k >> 2]);
     }
 #pragma unroll
-    for (int i = 0; i < NB; ++i)
-        if (i >= j) A[(size_t)(k0 + i) * np + k0 + j] = c[i];
-    // X = L^{-1}: lane j computes column j; L(r,p) is lane p's c[r]
-    double xv[NB];
+    for (int u = 0; u < NB / 4; ++u) sL[t][4 * u + q] = (4 * u + q <= t) ? a[u] : T(0);
+    __syncthreads();
+    if (q == 0) rdiag[t] = T(1) / sL[t][t];
+    for (int e = tid; e < NB * NB; e += 256) {
+        int r = e / NB, c = e % NB;
+        if (c <= r) A[(size_t)(k0 + r) * np + k0 + c] = sL[r][c];
+    }
+    __syncthreads();
+    const int c = t;
+    T xr[NB / 4];
+#pragma unroll
+    for (int u = 0; u < NB / 4; ++u) xr[u] = T(0);
 #pragma unroll
     for (int r = 0; r < NB; ++r) {
-        double sacc = (r == j) ? 1.0 : 0.0;
+        T part = 0.0;
 #pragma unroll
-        for (int p = 0; p < r; ++p) sacc -= bcast(c[r], p) * xv[p];
-        xv[r] = (r < j) ? 0.0 : sacc / bcast(c[r], r);
+        for (int u = 0; u <= (r - 1) / 4 && r > 0; ++u) part += sL[r][4 * u + q] * xr[u];
+        part += __shfl_xor(part, 1);
+        part += __shfl_xor(part, 2);
+        const T x = (((r == c) ? T(1) : T(0)) - part) * rdiag[r];
+        if (q == (r & 3)) xr[r >> 2] = x;
     }
-    double* Li = Linv + (size_t)(k0 / NB) * NB * NB;
+    T* Li = Linv + (size_t)(k0 / NB) * NB * NB;
 #pragma unroll
-    for (int r = 0; r < NB; ++r) Li[r * NB + j] = xv[r];
-    if (bad && j == 0) atomicOr(flags, FLAG_CHOL);
+    for (int u = 0; u < NB / 4; ++u) Li[(4 * u + q) * NB + c] = xr[u];
+    if (bad && tid == 0) atomicOr(flags, FLAG_CHOL);
 }
 
 // panel: rows below the diagonal block, L(i, k-block) = A(i, k-block) * Linv^T
-__global__ __launch_bounds__(256) void k_chol_panel(int np, int k0, double* __restrict__ A,
-                                                    const double* __restrict__ Linv) {
-    __shared__ double S[NB][NB + 1];
-    __shared__ double W[NB][NB + 1];
+template <typename T>
+__global__ __launch_bounds__(256) void k_chol_panel(int np, int k0, T* __restrict__ A, const T* __restrict__ Linv) {
+    __shared__ T S[NB][NB + 1];
+    __shared__ T W[NB][NB + 1];
     int tid = threadIdx.x;
     int r0 = k0 + NB + blockIdx.x * NB;
-    const double* Li = Linv + (size_t)(k0 / NB) * NB * NB;
+    const T* Li = Linv + (size_t)(k0 / NB) * NB * NB;
     for (int e = tid; e < NB * NB; e += 256) {
         int r = e / NB, c = e % NB;
         S[r][c] = A[(size_t)(r0 + r) * np + k0 + c];
@@ -298,9 +326,9 @@ __global__ __launch_bounds__(256) void k_chol_panel(int np, int k0, double* __re
     __syncthreads();
     // out(r, c) = sum_p S(r, p) * W(c, p), p <= c
     int tr = tid / 16, tc = tid % 16;
-    double acc[4][4] = {};
+    T acc[4][4] = {};
     for (int p = 0; p < NB; ++p) {
-        double a[4], b[4];
+        T a[4], b[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             a[u] = S[tr * 4 + u][p];
@@ -317,60 +345,117 @@ __global__ __launch_bounds__(256) void k_chol_panel(int np, int k0, double* __re
         for (int v = 0; v < 4; ++v) A[(size_t)(r0 + tr * 4 + u) * np + k0 + tc * 4 + v] = acc[u][v];
 }
 
-// trailing update of the lower tiles right of the panel:
-//   A(bi, bj) -= L(bi, k) L(bj, k)^T, bi >= bj > k
-// FP64 MFMA (v_mfma_f64_16x16x4f64): each of the 4 wavefronts owns a 32x32
-// quarter of the 64x64 tile as 2x2 blocks of 16x16; the two 64x64 panels are
-// staged in LDS.  Operand layout per lane: A(i = lane%16, k = lane/16),
-// B(k = lane/16, j = lane%16); result element v of a block sits at
-// (row 4*v + lane/16, column lane%16) (layout measured on gfx950).
+// Update of a set of lower 64 x 64 tiles by a block of factored columns:
+//   A(bi, bj) -= sum_{c in [kc0, kc0 + kw)} L(bi, c) L(bj, c)^T
+// over the trapezoid of tiles with column blocks bj in [bj0, bj0 + w) and rows
+// bi in [bj, nbk).  Tiles are numbered row by row (the first w rows form a
+// triangle, every later row holds w tiles) and handed out XCD-contiguously:
+// workgroups are dispatched round-robin over the 8 XCDs, so consecutive tile
+// numbers (same block row, same L panel rows) are given to workgroups of one
+// XCD and share its L2.  The rank kw is streamed through LDS in 64-column
+// chunks.  FP64 MFMA (v_mfma_f64_16x16x4f64): each of the 4 wavefronts owns a
+// 32x32 quarter of the tile as 2x2 blocks of 16x16.  Operand layout per lane:
+// A(i = lane%16, k = lane/16), B(k = lane/16, j = lane%16); result element v
+// of a block sits at (row 4*v + lane/16, column lane%16) (measured on gfx950).
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(256) void k_chol_update(int np, int k0, double* __restrict__ A) {
-    __shared__ double P[NB][NB + 1];
-    __shared__ double Q[NB][NB + 1];
-    int t = blockIdx.x;
-    int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
-    while (bi * (bi + 1) / 2 > t) --bi;
-    int bj = t - bi * (bi + 1) / 2;
-    int base = k0 + NB;
-    int ri = base + bi * NB, rj = base + bj * NB;
-    int tid = threadIdx.x;
-    for (int e = tid; e < NB * NB; e += 256) {
-        int r = e / NB, c = e % NB;
-        P[r][c] = A[(size_t)(ri + r) * np + k0 + c];
-        Q[r][c] = A[(size_t)(rj + r) * np + k0 + c];
+// Per-type 16x16x4 MFMA and its result layout: element v of a lane is at row
+// 4*v + lane/16 (kRowFromReg) or 4*(lane/16) + v, column lane%16.
+template <typename T>
+struct MfmaT;
+template <>
+struct MfmaT<double> {
+    typedef dbl4 V;
+    static constexpr bool kRowFromReg = true;
+    static __device__ __forceinline__ V op(double a, double b, V c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
-    __syncthreads();
-    int wave = tid / 64, lane = tid % 64;
-    int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    int li = lane % 16, lk = lane / 16;
-    dbl4 acc[2][2];
+};
+template <>
+struct MfmaT<float> {
+    typedef flt4 V;
+    static constexpr bool kRowFromReg = MFMA_F32_ROW_FROM_REG;
+    static __device__ __forceinline__ V op(float a, float b, V c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+__device__ inline int xcd_tile(int b, int nb) {
+    const int x = b & 7, l = b >> 3, per = nb >> 3, rem = nb & 7;
+    return x * per + min(x, rem) + l;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_chol_update(int np, int kc0, int kw, int bj0, int w, int nbk,
+                                                     T* __restrict__ A) {
+    using M = MfmaT<T>;
+    __shared__ T P[NB][NB + 1];
+    __shared__ T Q[NB][NB + 1];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int head = w * (w + 1) / 2;
+    int bi, bj;
+    if (t < head) {
+        int r = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while ((r + 1) * (r + 2) / 2 <= t) ++r;
+        while (r * (r + 1) / 2 > t) --r;
+        bi = bj0 + r;
+        bj = bj0 + (t - r * (r + 1) / 2);
+    } else {
+        const int t2 = t - head;
+        bi = bj0 + w + t2 / w;
+        bj = bj0 + t2 % w;
+    }
+    if (bi >= nbk) return;
+    const int ri = bi * NB, rj = bj * NB;
+    const int tid = threadIdx.x;
+    const int wave = tid / 64, lane = tid % 64;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int li = lane % 16, lk = lane / 16;
+    typename M::V acc[2][2];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = (dbl4){0.0, 0.0, 0.0, 0.0};
+        for (int y = 0; y < 2; ++y) acc[x][y] = typename M::V{0, 0, 0, 0};
+    for (int kc = kc0; kc < kc0 + kw; kc += NB) {
+        __syncthreads();
+        for (int e = tid; e < NB * NB; e += 256) {
+            const int r = e / NB, c = e % NB;
+            P[r][c] = A[(size_t)(ri + r) * np + kc + c];
+            Q[r][c] = A[(size_t)(rj + r) * np + kc + c];
+        }
+        __syncthreads();
 #pragma unroll 4
-    for (int k = 0; k < NB; k += 4) {
-        double fa0 = P[wr + li][k + lk], fa1 = P[wr + 16 + li][k + lk];
-        double fb0 = Q[wc + li][k + lk], fb1 = Q[wc + 16 + li][k + lk];
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0, fb1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1, fb1, acc[1][1], 0, 0, 0);
+        for (int k = 0; k < NB; k += 4) {
+            const T fa0 = P[wr + li][k + lk], fa1 = P[wr + 16 + li][k + lk];
+            const T fb0 = Q[wc + li][k + lk], fb1 = Q[wc + 16 + li][k + lk];
+            acc[0][0] = M::op(fa0, fb0, acc[0][0]);
+            acc[0][1] = M::op(fa0, fb1, acc[0][1]);
+            acc[1][0] = M::op(fa1, fb0, acc[1][0]);
+            acc[1][1] = M::op(fa1, fb1, acc[1][1]);
+        }
     }
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int r = ri + wr + 16 * a + 4 * v + lk;
-                int c = rj + wc + 16 * b + li;
-                if (c <= r) A[(size_t)r * np + c] -= acc[a][b][v];
+                const int r = ri + wr + 16 * x + (M::kRowFromReg ? 4 * v + lk : 4 * lk + v);
+                const int c = rj + wc + 16 * y + li;
+                if (c <= r) A[(size_t)r * np + c] -= acc[x][y][v];
             }
 }
+
+// Blocked right-looking Cholesky of the padded np x np matrix in two levels:
+// column blocks of NB = 64 are factored (diag + panel) and applied at once
+// only to the remaining columns of their super-panel of SP blocks; the rest of
+// the matrix receives one rank-(SP NB) update per super-panel.  Every element
+// of the trailing matrix is then read and written once per 256 columns
+// instead of once per 64, which lifts the update above the HBM roofline.
+constexpr int SP = 4;
+constexpr int SYMV_SPLIT = 8;   // tile split of the refinement matvec
+constexpr int REFINE_STEPS = 3; // FP32 solves per mixed-precision EEQ solve
 
 // ---------------------------------------------------------------------------
 // K4: triangular solves with the blocked factor
@@ -380,52 +465,161 @@ __global__ __launch_bounds__(256) void k_chol_update(int np, int k0, double* __r
 // product from LDS; workgroup 0 stores it), then rows below the block get
 // b_r -= L(r, k) y_k.  Backward, block k (descending): x_k = Linv_k^T c_k,
 // then c_j -= L(k, j)^T x_k for j < k0.
-__global__ __launch_bounds__(256) void k_trsv_fwd(int np, int k0, const double* __restrict__ A,
-                                                  const double* __restrict__ Linv, double* __restrict__ b,
-                                                  double* __restrict__ y) {
-    __shared__ double W[NB][NB + 1];
-    __shared__ double bk[NB], yk[NB];
-    const double* Li = Linv + (size_t)(k0 / NB) * NB * NB;
-    for (int e = threadIdx.x; e < NB * NB; e += 256) W[e / NB][e % NB] = Li[e];
-    if (threadIdx.x < NB) bk[threadIdx.x] = b[k0 + threadIdx.x];
+// Forward, block k: y_k = Linv_k b_k, then b_r -= L(r, k-block) y_k for the
+// 64-row tile of this workgroup (rows r0 = k0 + NB + 64 blockIdx.x ...).  The
+// tile is staged in LDS with coalesced row reads; thread (row, q) sums the
+// products p = 4u + q and quad shuffles combine the four shares.  Every
+// workgroup forms y_k itself (64 x 64 product from LDS); workgroup 0 stores it.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv_fwd(int np, int k0, const T* __restrict__ A,
+                                                  const T* __restrict__ Linv, T* __restrict__ b,
+                                                  T* __restrict__ y) {
+    __shared__ T W[NB][NB + 1];
+    __shared__ T Tt[NB][NB + 1];
+    __shared__ T bk[NB], yk[NB];
+    const int tid = threadIdx.x, t = tid >> 2, q = tid & 3;
+    const int r0 = k0 + NB + blockIdx.x * NB;
+    const T* Li = Linv + (size_t)(k0 / NB) * NB * NB;
+    for (int e = tid; e < NB * NB; e += 256) {
+        const int rr = e / NB, p = e % NB;
+        W[rr][p] = Li[e];
+        Tt[rr][p] = (r0 + rr < np) ? A[(size_t)(r0 + rr) * np + k0 + p] : T(0);
+    }
+    if (tid < NB) bk[tid] = b[k0 + tid];
     __syncthreads();
-    if (threadIdx.x < NB) {
-        int t = threadIdx.x;
-        double s = 0.0;
-        for (int p = 0; p <= t; ++p) s += W[t][p] * bk[p];
+    T s = T(0);
+#pragma unroll
+    for (int u = 0; u < NB / 4; ++u) s += W[t][4 * u + q] * bk[4 * u + q];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if (q == 0) {
         yk[t] = s;
         if (blockIdx.x == 0) y[k0 + t] = s;
     }
     __syncthreads();
-    int r = k0 + NB + blockIdx.x * 256 + threadIdx.x;
-    if (r >= np) return;
-    const double* Ar = A + (size_t)r * np + k0;
-    double s = 0.0;
-    for (int p = 0; p < NB; ++p) s += Ar[p] * yk[p];
-    b[r] -= s;
+    s = T(0);
+#pragma unroll
+    for (int u = 0; u < NB / 4; ++u) s += Tt[t][4 * u + q] * yk[4 * u + q];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if (q == 0 && r0 + t < np) b[r0 + t] -= s;
 }
-__global__ __launch_bounds__(256) void k_trsv_bwd(int np, int k0, const double* __restrict__ A,
-                                                  const double* __restrict__ Linv, double* __restrict__ c,
-                                                  double* __restrict__ xout) {
-    __shared__ double W[NB][NB + 1];
-    __shared__ double ck[NB], xk[NB];
-    const double* Li = Linv + (size_t)(k0 / NB) * NB * NB;
-    for (int e = threadIdx.x; e < NB * NB; e += 256) W[e / NB][e % NB] = Li[e];
-    if (threadIdx.x < NB) ck[threadIdx.x] = c[k0 + threadIdx.x];
+// Backward, block k (descending): x_k = Linv_k^T c_k, then c_j -= L(k-block, j)^T x_k
+// for the 64 columns j of this workgroup (j < k0).  Wave g sums rows p in
+// [16 g, 16 g + 16) with coalesced row reads; the four partial sums are
+// combined in LDS in a fixed order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv_bwd(int np, int k0, const T* __restrict__ A,
+                                                  const T* __restrict__ Linv, T* __restrict__ c,
+                                                  T* __restrict__ xout) {
+    __shared__ T W[NB][NB + 1];
+    __shared__ T ck[NB], xk[NB];
+    __shared__ T part[4][NB];
+    const int tid = threadIdx.x, t = tid >> 2, q = tid & 3;
+    const T* Li = Linv + (size_t)(k0 / NB) * NB * NB;
+    for (int e = tid; e < NB * NB; e += 256) W[e / NB][e % NB] = Li[e];
+    if (tid < NB) ck[tid] = c[k0 + tid];
     __syncthreads();
-    if (threadIdx.x < NB) {
-        int t = threadIdx.x;
-        double s = 0.0;
-        for (int p = t; p < NB; ++p) s += W[p][t] * ck[p];
+    T s = T(0);
+#pragma unroll
+    for (int u = 0; u < NB / 4; ++u) s += W[4 * u + q][t] * ck[4 * u + q];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if (q == 0) {
         xk[t] = s;
         if (blockIdx.x == 0) xout[k0 + t] = s;
     }
     __syncthreads();
-    int col = blockIdx.x * 256 + threadIdx.x;
-    if (col >= k0) return;
-    double s = 0.0;
-    for (int p = 0; p < NB; ++p) s += A[(size_t)(k0 + p) * np + col] * xk[p];
-    c[col] -= s;
+    const int jj = tid & 63, g = tid >> 6;
+    const int col = blockIdx.x * NB + jj;
+    T acc = T(0);
+    if (col < k0) {
+#pragma unroll
+        for (int p = 16 * g; p < 16 * g + 16; ++p) acc += A[(size_t)(k0 + p) * np + col] * xk[p];
+    }
+    part[g][jj] = acc;
+    __syncthreads();
+    if (g == 0 && col < k0) c[col] -= ((part[0][jj] + part[1][jj]) + part[2][jj]) + part[3][jj];
+}
+
+// ---------------------------------------------------------------------------
+// K4b: mixed-precision EEQ.  The reduced matrix R is factored in FP32 and the
+// FP64 solution is won back by iterative refinement against the FP64 R:
+//   x = 0, r = b;  repeat: x += L32^-T L32^-1 r;  r = b - R x.
+// ---------------------------------------------------------------------------
+// FP32 copy of the lower tiles (bi >= bj) of R; tile t numbered row by row.
+__global__ __launch_bounds__(256) void k_lower_to_f32(int np, const double* __restrict__ A, float* __restrict__ A32) {
+    const int t = blockIdx.x;
+    int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+    while (bi * (bi + 1) / 2 > t) --bi;
+    const int bj = t - bi * (bi + 1) / 2;
+    for (int e = threadIdx.x; e < NB * NB; e += 256) {
+        const size_t o = (size_t)(bi * NB + e / NB) * np + bj * NB + e % NB;
+        A32[o] = (float)A[o];
+    }
+}
+
+// Partial products of the symmetric matvec y = R x from the lower tiles of R:
+// workgroup (I, sp) sums, for the 64 rows of block row I, the tiles J = sp,
+// sp + nsplit, ...: (I, J) by rows when J < I, the diagonal tile completed
+// symmetrically from its lower triangle, and (J, I)^T by columns when J > I.
+// part[sp][row]; summed in a fixed order by k_eeq_resid.
+__global__ __launch_bounds__(256) void k_symv_part(int np, int nsplit, const double* __restrict__ R,
+                                                   const double* __restrict__ x, double* __restrict__ part) {
+    __shared__ double Tt[NB][NB + 1];
+    __shared__ double xs[NB];
+    const int I = blockIdx.x, sp = blockIdx.y, nbk = np / NB;
+    const int tid = threadIdx.x, t = tid >> 2, q = tid & 3;
+    double acc = 0.0;
+    for (int J = sp; J < nbk; J += nsplit) {
+        __syncthreads();
+        const bool below = J > I;
+        const int r0 = below ? J * NB : I * NB, c0 = below ? I * NB : J * NB;
+        for (int e = tid; e < NB * NB; e += 256) {
+            const int r = e / NB, c = e % NB;
+            // diagonal tile: symmetric completion from its lower triangle
+            Tt[r][c] = (J == I && c > r) ? R[(size_t)(r0 + c) * np + c0 + r] : R[(size_t)(r0 + r) * np + c0 + c];
+        }
+        if (tid < NB) xs[tid] = x[J * NB + tid];
+        __syncthreads();
+        if (!below) {
+#pragma unroll
+            for (int u = 0; u < NB / 4; ++u) acc += Tt[t][4 * u + q] * xs[4 * u + q];
+        } else {
+#pragma unroll
+            for (int u = 0; u < NB / 4; ++u) acc += Tt[4 * u + q][t] * xs[4 * u + q];
+        }
+    }
+    acc += __shfl_xor(acc, 1);
+    acc += __shfl_xor(acc, 2);
+    if (q == 0) part[(size_t)sp * np + I * NB + t] = acc;
+}
+
+// r = b - sum_sp part[sp] (fixed order); rf = (float) r for the next FP32 solve.
+__global__ __launch_bounds__(256) void k_eeq_resid(int np, int nsplit, const double* __restrict__ part,
+                                                   const double* __restrict__ b, double* __restrict__ r,
+                                                   float* __restrict__ rf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    double y = 0.0;
+    for (int sp = 0; sp < nsplit; ++sp) y += part[(size_t)sp * np + i];
+    const double v = b[i] - y;
+    r[i] = v;
+    rf[i] = (float)v;
+}
+
+// x (+)= d for the FP32 correction d; first == 1 starts from x = d.
+__global__ __launch_bounds__(256) void k_refine_add(int np, int first, const float* __restrict__ d,
+                                                    double* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    x[i] = first ? (double)d[i] : x[i] + (double)d[i];
+}
+
+__global__ __launch_bounds__(256) void k_to_f32(int n, const double* __restrict__ a, float* __restrict__ b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -465,7 +659,7 @@ __global__ __launch_bounds__(256) void k_charges(int n, const int* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// K6: pair terms.  Row atom i per thread, j over all atoms in LDS tiles.
+// K6: pair terms.  Row atom i per thread, partners j staged in LDS tiles.
 //   Coulomb  (r <= 100)  ff_workspace_gfnff.cpp:850-898
 //   D4       (r <= 50)   :715-767 with C6 from the current Gaussian weights
 //   repulsion (r <= 20)  :814-842 with the element-pair factor; bonded and
@@ -489,6 +683,8 @@ struct PairArgs {
     const int* nref;
     const int* metal;
     const double* c6ref;
+    const double* h;  // [n][nelem][7] D4 half contraction (k_d4_half)
+    const double* dh;
     double* g;        // [3n]
     double* dEdcn;    // [n]
     double* e_coul;   // [n]
@@ -504,33 +700,73 @@ __device__ inline double rep_ff(int zi, int zj, int mi, int mj) {
     return 1.0;
 }
 
-__global__ __launch_bounds__(TILE) void k_pairs(PairArgs a) {
+// D4 half contraction per atom i and partner element e:
+//   h[i][e][b]  = sum_a gw_i[a]  c6ref[ei, e, a, b]
+//   dh[i][e][b] = sum_a dgw_i[a] c6ref[ei, e, a, b]
+// so that C6_ij = h[i][ej] . gw_j and dC6_ij/dCN_i = dh[i][ej] . gw_j
+// (the reference's half-contraction, d4param_generator.cpp:1235).
+__global__ __launch_bounds__(256) void k_d4_half(int n, int ne, const int* __restrict__ elem,
+                                                 const int* __restrict__ nref, const double* __restrict__ c6ref,
+                                                 const double* __restrict__ gw, const double* __restrict__ dgw,
+                                                 double* __restrict__ h, double* __restrict__ dh) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * ne) return;
+    int i = t / ne, e = t % ne;
+    int ei = elem[i], nri = nref[ei];
+    const double* c = c6ref + (size_t)(ei * ne + e) * 49;
+    double hb[7] = {}, dhb[7] = {};
+    for (int aa = 0; aa < nri; ++aa) {
+        double g = gw[(size_t)i * 7 + aa], dg = dgw[(size_t)i * 7 + aa];
+#pragma unroll
+        for (int bb = 0; bb < 7; ++bb) {
+            hb[bb] += g * c[aa * 7 + bb];
+            dhb[bb] += dg * c[aa * 7 + bb];
+        }
+    }
+#pragma unroll
+    for (int bb = 0; bb < 7; ++bb) {
+        h[(size_t)t * 7 + bb] = hb[bb];
+        dh[(size_t)t * 7 + bb] = dhb[bb];
+    }
+}
+
+// Pair terms over full rows, split into j chunks so that the launch fills the
+// chip: block (bx, by) owns rows i in [256 bx, 256 bx + 256) and partners j in
+// [by cj, by cj + cj).  Each thread accumulates its row's gradient, dE/dCN and
+// three energies over the chunk and writes them to part[by][k][i]; the chunks
+// are summed in a fixed order by k_pairs_reduce (no atomics, deterministic).
+// NE > 0: the D4 half contraction of row i for every partner element sits in
+// registers and is selected by the partner's element, which is uniform across
+// the wavefront because all lanes visit the same j; NE == 0: read from memory.
+constexpr int PART_K = 7;
+template <int NE>
+__global__ __launch_bounds__(TILE) void k_pairs_part(PairArgs a, int cj, double* __restrict__ part) {
     __shared__ double sx[TILE], sy[TILE], sz[TILE], sq[TILE], sal[TILE], szt[TILE], ssz[TILE], san[TILE],
         srz[TILE];
-    __shared__ double sgw[TILE * 7], sdgw[TILE * 7];
+    __shared__ double sgw[TILE * 7];
     __shared__ int sZ[TILE], sel[TILE], smt[TILE];
-    __shared__ double sc6[16 * 49];
-    const int n = a.n;
-    int i = blockIdx.x * TILE + threadIdx.x;
-    bool act = i < n;
-    int ne = a.nelem;
-    for (int e = threadIdx.x; e < ne * ne * 49 && e < 16 * 49; e += TILE) sc6[e] = a.c6ref[e];
+    const int n = a.n, ne = a.nelem;
+    const int i = blockIdx.x * TILE + threadIdx.x;
+    const bool act = i < n;
+    const int j0 = blockIdx.y * cj, j1 = min(n, j0 + cj);
     double3 xi = act ? ld3(a.x, i) : make_double3(0, 0, 0);
     double qi = act ? a.q[i] : 0.0, ali = act ? a.alp[i] : 1.0, zti = act ? a.zeta[i] : 0.0;
     double szi = act ? a.szr[i] : 0.0, ani = act ? a.anb[i] : 0.0, rzi = act ? a.repz[i] : 0.0;
-    int zi = act ? a.Z[i] : 0, ei = act ? a.elem[i] : 0, mi = act ? a.metal[i] : 0;
-    int nri = act ? a.nref[ei] : 0;
-    double gwi[7], dgwi[7];
+    int zi = act ? a.Z[i] : 0, mi = act ? a.metal[i] : 0;
+    constexpr int NH = NE > 0 ? NE * 7 : 1;
+    double hr[NH], dhr[NH];
+    if (NE > 0) {
 #pragma unroll
-    for (int r = 0; r < 7; ++r) {
-        gwi[r] = act ? a.gw[(size_t)i * 7 + r] : 0.0;
-        dgwi[r] = act ? a.dgw[(size_t)i * 7 + r] : 0.0;
+        for (int k = 0; k < NH; ++k) {
+            hr[k] = act ? a.h[(size_t)i * NH + k] : 0.0;
+            dhr[k] = act ? a.dh[(size_t)i * NH + k] : 0.0;
+        }
     }
     double gx = 0, gy = 0, gz = 0, dcn = 0, ec = 0, ed = 0, er = 0;
-    for (int t0 = 0; t0 < n; t0 += TILE) {
+    for (int t0 = j0; t0 < j1; t0 += TILE) {
         __syncthreads();
         int j = t0 + threadIdx.x;
-        if (j < n) {
+        if (j < j1) {
             sx[threadIdx.x] = a.x[3 * j];
             sy[threadIdx.x] = a.x[3 * j + 1];
             sz[threadIdx.x] = a.x[3 * j + 2];
@@ -544,18 +780,16 @@ __global__ __launch_bounds__(TILE) void k_pairs(PairArgs a) {
             sel[threadIdx.x] = a.elem[j];
             smt[threadIdx.x] = a.metal[j];
 #pragma unroll
-            for (int r = 0; r < 7; ++r) {
-                sgw[threadIdx.x * 7 + r] = a.gw[(size_t)j * 7 + r];
-                sdgw[threadIdx.x * 7 + r] = a.dgw[(size_t)j * 7 + r];
-            }
+            for (int r = 0; r < 7; ++r) sgw[threadIdx.x * 7 + r] = a.gw[(size_t)j * 7 + r];
         }
         __syncthreads();
         if (!act) continue;
-        int m = min(TILE, n - t0);
+        const int m = min(TILE, j1 - t0);
         for (int jj = 0; jj < m; ++jj) {
             if (t0 + jj == i) continue;
             double dx = xi.x - sx[jj], dy = xi.y - sy[jj], dz = xi.z - sz[jj];
             double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 > 10000.0) continue;  // beyond every pair cut-off (Coulomb 100 Bohr)
             double r = sqrt(r2);
             double dEdr = 0.0;
             // Coulomb
@@ -569,22 +803,29 @@ __global__ __launch_bounds__(TILE) void k_pairs(PairArgs a) {
             }
             // D4 dispersion
             if (r <= 50.0 && r >= 1e-10) {
-                int ej = sel[jj];
-                const double* c = sc6 + (ei * ne + ej) * 49;
+                const int ej = sel[jj];
                 double c6 = 0.0, dci = 0.0;
-                double hj[7];
+                if (NE > 0) {
 #pragma unroll
-                for (int b = 0; b < 7; ++b) hj[b] = 0.0;
-                for (int aa = 0; aa < nri; ++aa) {
+                    for (int e = 0; e < (NE > 0 ? NE : 1); ++e)
+                        if (e == ej) {
 #pragma unroll
-                    for (int b = 0; b < 7; ++b) {
-                        double cr = c[aa * 7 + b];
-                        hj[b] += gwi[aa] * cr;
-                        dci += dgwi[aa] * sgw[jj * 7 + b] * cr;
+                            for (int bb = 0; bb < 7; ++bb) {
+                                double gj = sgw[jj * 7 + bb];
+                                c6 += hr[e * 7 + bb] * gj;
+                                dci += dhr[e * 7 + bb] * gj;
+                            }
+                        }
+                } else {
+                    const double* hp = a.h + ((size_t)i * ne + ej) * 7;
+                    const double* dp = a.dh + ((size_t)i * ne + ej) * 7;
+#pragma unroll
+                    for (int bb = 0; bb < 7; ++bb) {
+                        double gj = sgw[jj * 7 + bb];
+                        c6 += hp[bb] * gj;
+                        dci += dp[bb] * gj;
                     }
                 }
-#pragma unroll
-                for (int b = 0; b < 7; ++b) c6 += hj[b] * sgw[jj * 7 + b];
                 double r4r2 = 3.0 * szi * ssz[jj];
                 double r0 = 0.58 * sqrt(r4r2) + 4.80;
                 double r0s = r0 * r0;
@@ -597,7 +838,8 @@ __global__ __launch_bounds__(TILE) void k_pairs(PairArgs a) {
                 dEdr += -c6 * zc * (d6 + 2.0 * r4r2 * d8) * r;
                 dcn -= dci * sum * zc;
             }
-            // non-bonded repulsion (generic pair factor)
+            // non-bonded repulsion (generic pair factor; bonded / 1,3 / 1,4
+            // corrections follow in k_nbrep_fix)
             if (r <= 20.0 && r >= 1e-8) {
                 double ff = rep_ff(zi, sZ[jj], mi, smt[jj]);
                 double al = sqrt(ani * san[jj]) * ff;
@@ -613,13 +855,169 @@ __global__ __launch_bounds__(TILE) void k_pairs(PairArgs a) {
         }
     }
     if (!act) return;
-    a.g[3 * i] = gx;
-    a.g[3 * i + 1] = gy;
-    a.g[3 * i + 2] = gz;
-    a.dEdcn[i] = dcn;
-    a.e_coul[i] = ec;
-    a.e_disp[i] = ed;
-    a.e_rep[i] = er;
+    double* p = part + (size_t)blockIdx.y * PART_K * n + i;
+    p[0] = gx;
+    p[(size_t)n] = gy;
+    p[(size_t)2 * n] = gz;
+    p[(size_t)3 * n] = dcn;
+    p[(size_t)4 * n] = ec;
+    p[(size_t)5 * n] = ed;
+    p[(size_t)6 * n] = er;
+}
+
+// Mixed-precision pair terms: the same rows, chunks and partial-sum layout as
+// k_pairs_part.  Coordinate differences are formed in FP64 (no loss of the
+// absolute positions), every term is evaluated in FP32 (hardware rsqrt, rcp,
+// exp, erf paths) and the row sums are accumulated in FP64.
+template <int NE>
+__global__ __launch_bounds__(TILE) void k_pairs_part32(PairArgs a, int cj, double* __restrict__ part) {
+    __shared__ double sx[TILE], sy[TILE], sz[TILE];
+    __shared__ float sq[TILE], sal[TILE], szt[TILE], ssz[TILE], san[TILE], srz[TILE];
+    __shared__ float sgw[TILE * 7];
+    __shared__ int sZ[TILE], sel[TILE], smt[TILE];
+    const int n = a.n, ne = a.nelem;
+    const int i = blockIdx.x * TILE + threadIdx.x;
+    const bool act = i < n;
+    const int j0 = blockIdx.y * cj, j1 = min(n, j0 + cj);
+    const double3 xi = act ? ld3(a.x, i) : make_double3(0, 0, 0);
+    const float qi = act ? (float)a.q[i] : 0.f, ali = act ? (float)a.alp[i] : 1.f;
+    const float zti = act ? (float)a.zeta[i] : 0.f, szi = act ? (float)a.szr[i] : 0.f;
+    const float ani = act ? (float)a.anb[i] : 0.f, rzi = act ? (float)a.repz[i] : 0.f;
+    const int zi = act ? a.Z[i] : 0, mi = act ? a.metal[i] : 0;
+    constexpr int NH = NE > 0 ? NE * 7 : 1;
+    float hr[NH], dhr[NH];
+    if (NE > 0) {
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            hr[k] = act ? (float)a.h[(size_t)i * NH + k] : 0.f;
+            dhr[k] = act ? (float)a.dh[(size_t)i * NH + k] : 0.f;
+        }
+    }
+    double gx = 0, gy = 0, gz = 0, dcn = 0, ec = 0, ed = 0, er = 0;
+    for (int t0 = j0; t0 < j1; t0 += TILE) {
+        __syncthreads();
+        const int j = t0 + threadIdx.x;
+        if (j < j1) {
+            sx[threadIdx.x] = a.x[3 * j];
+            sy[threadIdx.x] = a.x[3 * j + 1];
+            sz[threadIdx.x] = a.x[3 * j + 2];
+            sq[threadIdx.x] = (float)a.q[j];
+            sal[threadIdx.x] = (float)a.alp[j];
+            szt[threadIdx.x] = (float)a.zeta[j];
+            ssz[threadIdx.x] = (float)a.szr[j];
+            san[threadIdx.x] = (float)a.anb[j];
+            srz[threadIdx.x] = (float)a.repz[j];
+            sZ[threadIdx.x] = a.Z[j];
+            sel[threadIdx.x] = a.elem[j];
+            smt[threadIdx.x] = a.metal[j];
+#pragma unroll
+            for (int r = 0; r < 7; ++r) sgw[threadIdx.x * 7 + r] = (float)a.gw[(size_t)j * 7 + r];
+        }
+        __syncthreads();
+        if (!act) continue;
+        const int m = min(TILE, j1 - t0);
+        for (int jj = 0; jj < m; ++jj) {
+            if (t0 + jj == i) continue;
+            const double dxd = xi.x - sx[jj], dyd = xi.y - sy[jj], dzd = xi.z - sz[jj];
+            const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
+            if (r2d > 10000.0) continue;  // beyond every pair cut-off (Coulomb 100 Bohr)
+            const float r2 = (float)r2d;
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            const float r = r2 * rinv;
+            float dEdr = 0.f;
+            if (r2d >= 1e-20) {
+                // Coulomb (r <= 100)
+                const float gij = __builtin_amdgcn_rsqf(ali + sal[jj]);
+                const float gr = gij * r;
+                const float ef = erff(gr);
+                const float qq = qi * sq[jj];
+                ec += (double)(0.5f * qq * ef * rinv);
+                dEdr += qq * (gij * __expf(-gr * gr) * 1.1283791671f * rinv - ef * rinv * rinv);
+            }
+            if (r2d <= 2500.0 && r2d >= 1e-20) {
+                // D4 dispersion (r <= 50)
+                const int ej = sel[jj];
+                float c6 = 0.f, dci = 0.f;
+                if (NE > 0) {
+#pragma unroll
+                    for (int e = 0; e < (NE > 0 ? NE : 1); ++e)
+                        if (e == ej) {
+#pragma unroll
+                            for (int bb = 0; bb < 7; ++bb) {
+                                const float gj = sgw[jj * 7 + bb];
+                                c6 += hr[e * 7 + bb] * gj;
+                                dci += dhr[e * 7 + bb] * gj;
+                            }
+                        }
+                } else {
+                    const double* hp = a.h + ((size_t)i * ne + ej) * 7;
+                    const double* dp = a.dh + ((size_t)i * ne + ej) * 7;
+#pragma unroll
+                    for (int bb = 0; bb < 7; ++bb) {
+                        const float gj = sgw[jj * 7 + bb];
+                        c6 += (float)hp[bb] * gj;
+                        dci += (float)dp[bb] * gj;
+                    }
+                }
+                const float r4r2 = 3.f * szi * ssz[jj];
+                const float r0 = 0.58f * __builtin_sqrtf(r4r2) + 4.80f;
+                const float r0s = r0 * r0;
+                const float zc = zti * szt[jj];
+                const float r6 = r2 * r2 * r2, r06 = r0s * r0s * r0s;
+                const float t6 = __builtin_amdgcn_rcpf(r6 + r06), t8 = __builtin_amdgcn_rcpf(r6 * r2 + r06 * r0s);
+                const float sum = t6 + 2.f * r4r2 * t8;
+                ed += (double)(0.5f * (-c6 * sum * zc));
+                const float d6 = -6.f * r2 * r2 * t6 * t6, d8 = -8.f * r2 * r2 * r2 * t8 * t8;
+                dEdr += -c6 * zc * (d6 + 2.f * r4r2 * d8) * r;
+                dcn -= (double)(dci * sum * zc);
+            }
+            if (r2d <= 400.0 && r2d >= 1e-16) {
+                // non-bonded repulsion (r <= 20), generic pair factor
+                const float ff = (float)rep_ff(zi, sZ[jj], mi, smt[jj]);
+                const float al = __builtin_sqrtf(ani * san[jj]) * ff;
+                const float ab = rzi * srz[jj] * 0.4270f;
+                const float sr = __builtin_sqrtf(r);
+                const float e = ab * __expf(-al * r * sr) * rinv;
+                er += (double)(0.5f * e);
+                dEdr += -e * rinv - 1.5f * al * sr * e;
+            }
+            const double sgrad = (double)(dEdr * rinv);
+            gx += sgrad * dxd;
+            gy += sgrad * dyd;
+            gz += sgrad * dzd;
+        }
+    }
+    if (!act) return;
+    double* p = part + (size_t)blockIdx.y * PART_K * n + i;
+    p[0] = gx;
+    p[(size_t)n] = gy;
+    p[(size_t)2 * n] = gz;
+    p[(size_t)3 * n] = dcn;
+    p[(size_t)4 * n] = ec;
+    p[(size_t)5 * n] = ed;
+    p[(size_t)6 * n] = er;
+}
+
+// Sums the j-chunk partials of k_pairs_part in chunk order.
+__global__ __launch_bounds__(256) void k_pairs_reduce(int n, int nchunk, const double* __restrict__ part,
+                                                      double* __restrict__ g, double* __restrict__ dEdcn,
+                                                      double* __restrict__ e_coul, double* __restrict__ e_disp,
+                                                      double* __restrict__ e_rep) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc[PART_K] = {};
+    for (int c = 0; c < nchunk; ++c) {
+        const double* p = part + (size_t)c * PART_K * n + i;
+#pragma unroll
+        for (int k = 0; k < PART_K; ++k) acc[k] += p[(size_t)k * n];
+    }
+    g[3 * i] = acc[0];
+    g[3 * i + 1] = acc[1];
+    g[3 * i + 2] = acc[2];
+    dEdcn[i] = acc[3];
+    e_coul[i] = acc[4];
+    e_disp[i] = acc[5];
+    e_rep[i] = acc[6];
 }
 
 // ---------------------------------------------------------------------------
@@ -1244,11 +1642,55 @@ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+// Kernel classes that can be timed with HIP events (one class at a time).
+enum KernelClass {
+    KC_CN, KC_EEQ_BUILD, KC_CHOL_DIAG, KC_CHOL_PANEL, KC_CHOL_UPDATE, KC_TRSV, KC_CHARGES, KC_PAIRS, KC_PAIRS_AUX,
+    KC_NBREP, KC_REFINE, KC_BONDS, KC_ANGLES, KC_TORS, KC_BREP, KC_HB, KC_ATM, KC_BATM, KC_CHAIN, KC_REDUCE, KC_COUNT
+};
+const char* const kKernelClassNames[KC_COUNT] = {
+    "cn",    "eeq_build", "chol_diag", "chol_panel", "chol_update", "trsv", "charges", "pairs", "pairs_aux", "nbrep", "refine",
+    "bonds", "angles",    "torsions",  "brep",       "hb",          "atm",  "batm",    "chain", "reduce"};
+
+// Start/stop event pairs around every launch of the selected class, recorded
+// on the stream the kernels run on.
+struct KernelTimer {
+    int cls = -1;
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    void mark(int c, hipStream_t s) {
+        if (c != cls) return;
+        if (used == ev.size()) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) {
+                cls = -1;  // timing silently off when no event can be created; read() reports it
+                return;
+            }
+            ev.push_back(e);
+        }
+        (void)hipEventRecord(ev[used++], s);
+    }
+    ~KernelTimer() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+#define LAUNCH(cls, kern, grid, block, stream, ...)                           \
+    do {                                                                      \
+        E->timer.mark(cls, stream);                                           \
+        hipLaunchKernelGGL(kern, grid, block, 0, stream, __VA_ARGS__);        \
+        E->timer.mark(cls, stream);                                           \
+    } while (0)
+
 struct DeviceEngine {
+    KernelTimer timer;
     int n = 0, nelem = 0, nfrag = 0;
     int nr = 0, np = 0;                 // reduced EEQ size, padded size
     int nbond = 0, nangle = 0, ntors = 0, nbrep = 0, nhb = 0, natm = 0, nbatm = 0;
     int nslots = 0;
+    int pair_cj = 0, pair_nchunk = 0;   // j-chunk size and count of the pair kernel
+    bool mixed = false;                 // FP32 pair terms + FP32 factorisation with FP64 refinement
+    float *Ared32 = nullptr, *Linv32 = nullptr, *b32 = nullptr, *y32 = nullptr, *d32 = nullptr;
+    double *rvec = nullptr, *symv = nullptr;
     bool charged = false;
     hipStream_t own = nullptr;
     std::vector<void*> bufs;
@@ -1275,7 +1717,7 @@ struct DeviceEngine {
     // work
     double *x, *cn, *dlog, *gw, *dgw, *rhs, *Ared, *Linv, *b, *rres, *q, *eself, *qtmp, *gpair, *dEdcn, *vchain,
         *slots, *dcn_slot, *grad, *e_coul, *e_disp, *e_rep, *e_bond, *e_ang, *e_tor, *e_brep, *e_hb, *e_atm,
-        *e_batm, *parts, *rpart, *out, *diag, *ghb, *ybuf, *xbuf;
+        *e_batm, *parts, *rpart, *out, *diag, *ghb, *ybuf, *xbuf, *h6, *dh6, *ppart;
     int *nbn, *nbl, *flags;
 
     template <class T>
@@ -1309,9 +1751,9 @@ DeviceEngine* device_create(const ParamSet& P, int precision, int& status, std::
         msg = "no HIP device available";
         return nullptr;
     }
-    if (precision != 0) {
-        status = kErrUnsupported;
-        msg = "mixed precision path not built yet";
+    if (precision != 0 && precision != 1) {
+        status = kErrArgs;
+        msg = "unknown precision mode";
         return nullptr;
     }
     if (!P.inv_par.empty()) {
@@ -1326,6 +1768,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, int& status, std::
             return nullptr;
         }
     auto* E = new DeviceEngine();
+    E->mixed = precision == 1;
     try {
         const int n = P.n;
         E->n = n;
@@ -1536,6 +1979,15 @@ DeviceEngine* device_create(const ParamSet& P, int precision, int& status, std::
         E->b = E->alloc<double>(E->np);
         E->ybuf = E->alloc<double>(E->np);
         E->xbuf = E->alloc<double>(E->np);
+        if (E->mixed) {
+            E->Ared32 = E->alloc<float>((size_t)E->np * E->np);
+            E->Linv32 = E->alloc<float>((size_t)E->np * NB);
+            E->b32 = E->alloc<float>(E->np);
+            E->y32 = E->alloc<float>(E->np);
+            E->d32 = E->alloc<float>(E->np);
+            E->rvec = E->alloc<double>(E->np);
+            E->symv = E->alloc<double>((size_t)SYMV_SPLIT * E->np);
+        }
         E->rres = E->alloc<double>(n);
         E->q = E->alloc<double>(n);
         E->eself = E->alloc<double>(n);
@@ -1549,6 +2001,16 @@ DeviceEngine* device_create(const ParamSet& P, int precision, int& status, std::
         E->e_coul = E->alloc<double>(n);
         E->e_disp = E->alloc<double>(n);
         E->e_rep = E->alloc<double>(n);
+        E->h6 = E->alloc<double>((size_t)n * E->nelem * 7);
+        E->dh6 = E->alloc<double>((size_t)n * E->nelem * 7);
+        {
+            // j chunks: enough blocks to fill 256 CUs several times over
+            const int rowblk = cdiv(n, TILE);
+            int nch = std::min(cdiv(n, TILE), std::max(1, cdiv(4096, rowblk)));
+            E->pair_cj = cdiv(cdiv(n, nch), TILE) * TILE;
+            E->pair_nchunk = cdiv(n, E->pair_cj);
+        }
+        E->ppart = E->alloc<double>((size_t)E->pair_nchunk * PART_K * n);
         E->e_bond = E->alloc<double>(std::max(E->nbond, 1));
         E->e_ang = E->alloc<double>(std::max(E->nangle, 1));
         E->e_tor = E->alloc<double>(std::max(E->ntors, 1));
@@ -1584,45 +2046,104 @@ DeviceEngine* device_create(const ParamSet& P, int precision, int& status, std::
 
 void device_destroy(DeviceEngine* e) { delete e; }
 
+// Queues the two-level blocked factorisation of the lower triangle of A
+// (np x np, np a multiple of NB) on stream s; Linv receives the inverses of the
+// diagonal blocks, flags FLAG_CHOL on a non-positive pivot.  tm (optional)
+// times the kernel classes.
+template <typename T>
+static void queue_cholesky(KernelTimer* tm, hipStream_t s, int np, T* A, T* Linv, int* flags) {
+    auto mark = [&](int c) {
+        if (tm) tm->mark(c, s);
+    };
+    const int nbk = np / NB;
+    for (int K = 0; K < nbk; K += SP) {
+        const int kend = std::min(K + SP, nbk);
+        for (int k = K; k < kend; ++k) {
+            const int k0 = k * NB;
+            mark(KC_CHOL_DIAG);
+            hipLaunchKernelGGL(k_chol_diag<T>, dim3(1), dim3(256), 0, s, np, k0, A, Linv, flags);
+            mark(KC_CHOL_DIAG);
+            const int rem = nbk - k - 1;
+            if (rem == 0) continue;
+            mark(KC_CHOL_PANEL);
+            hipLaunchKernelGGL(k_chol_panel<T>, dim3(rem), dim3(256), 0, s, np, k0, A, Linv);
+            mark(KC_CHOL_PANEL);
+            const int w = kend - k - 1;
+            if (w > 0) {
+                const int ntiles = w * (w + 1) / 2 + (rem - w) * w;
+                mark(KC_CHOL_UPDATE);
+                hipLaunchKernelGGL(k_chol_update<T>, dim3(ntiles), dim3(256), 0, s, np, k0, NB, k + 1, w, nbk, A);
+                mark(KC_CHOL_UPDATE);
+            }
+        }
+        if (kend < nbk) {
+            const int M = nbk - kend;
+            mark(KC_CHOL_UPDATE);
+            hipLaunchKernelGGL(k_chol_update<T>, dim3(M * (M + 1) / 2), dim3(256), 0, s, np, K * NB, (kend - K) * NB,
+                               kend, M, nbk, A);
+            mark(KC_CHOL_UPDATE);
+        }
+    }
+}
+
 static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::string& msg) {
     const int n = E->n;
     hipLaunchKernelGGL(k_copy, dim3(cdiv(3 * n, 256)), dim3(256), 0, s, 3 * n, xin, E->x);
     hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, s, 1, E->flags, 0);
-    hipLaunchKernelGGL(k_cn, dim3(cdiv(n, TILE)), dim3(TILE), 0, s, n, E->x, E->rcov_cn, E->elem, E->nref, E->refcn,
+    LAUNCH(KC_CN, k_cn, dim3(cdiv(n, 4)), dim3(256), s, n, E->x, E->rcov_cn, E->elem, E->nref, E->refcn,
                        E->chi_base, E->cnf, E->cn, E->dlog, E->gw, E->dgw, E->rhs, E->nbn, E->nbl, E->flags);
     // EEQ
     const int np = E->np, nbk = np / NB;
-    hipLaunchKernelGGL(k_eeq_reduced, dim3(nbk * (nbk + 1) / 2), dim3(256), 0, s, E->nr, np, E->red_atom,
+    LAUNCH(KC_EEQ_BUILD, k_eeq_reduced, dim3(nbk * (nbk + 1) / 2), dim3(256), s, E->nr, np, E->red_atom,
                        E->red_last, E->x, E->alp, E->diag, E->Ared);
     const double* rsrc = E->rhs;
     if (E->charged) {
-        hipLaunchKernelGGL(k_eeq_rhs_qp, dim3(cdiv(n, 256)), dim3(256), 0, s, n, E->x, E->alp, E->diag, E->qp,
+        LAUNCH(KC_EEQ_BUILD, k_eeq_rhs_qp, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->alp, E->diag, E->qp,
                            E->rhs, E->rres);
         rsrc = E->rres;
     }
-    hipLaunchKernelGGL(k_eeq_rhs_reduce, dim3(cdiv(np, 256)), dim3(256), 0, s, E->nr, np, E->red_atom, E->red_last,
+    LAUNCH(KC_EEQ_BUILD, k_eeq_rhs_reduce, dim3(cdiv(np, 256)), dim3(256), s, E->nr, np, E->red_atom, E->red_last,
                        rsrc, E->b);
-    for (int k = 0; k < nbk; ++k) {
-        int k0 = k * NB;
-        hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(NB), 0, s, np, k0, E->Ared, E->Linv, E->flags);
-        int rem = nbk - k - 1;
-        if (rem > 0) {
-            hipLaunchKernelGGL(k_chol_panel, dim3(rem), dim3(256), 0, s, np, k0, E->Ared, E->Linv);
-            hipLaunchKernelGGL(k_chol_update, dim3(rem * (rem + 1) / 2), dim3(256), 0, s, np, k0, E->Ared);
+    if (!E->mixed) {
+        queue_cholesky(&E->timer, s, np, E->Ared, E->Linv, E->flags);
+        for (int k = 0; k < nbk; ++k) {
+            const int k0 = k * NB;
+            const int rows = np - k0 - NB;
+            LAUNCH(KC_TRSV, k_trsv_fwd<double>, dim3(std::max(1, cdiv(rows, NB))), dim3(256), s, np, k0, E->Ared,
+                   E->Linv, E->b, E->ybuf);
+        }
+        for (int k = nbk - 1; k >= 0; --k) {
+            const int k0 = k * NB;
+            LAUNCH(KC_TRSV, k_trsv_bwd<double>, dim3(std::max(1, cdiv(k0, NB))), dim3(256), s, np, k0, E->Ared,
+                   E->Linv, E->ybuf, E->xbuf);
+        }
+    } else {
+        LAUNCH(KC_EEQ_BUILD, k_lower_to_f32, dim3(nbk * (nbk + 1) / 2), dim3(256), s, np, E->Ared, E->Ared32);
+        queue_cholesky(&E->timer, s, np, E->Ared32, E->Linv32, E->flags);
+        LAUNCH(KC_REFINE, k_to_f32, dim3(cdiv(np, 256)), dim3(256), s, np, E->b, E->b32);
+        for (int it = 0; it < REFINE_STEPS; ++it) {
+            for (int k = 0; k < nbk; ++k) {
+                const int k0 = k * NB;
+                const int rows = np - k0 - NB;
+                LAUNCH(KC_TRSV, k_trsv_fwd<float>, dim3(std::max(1, cdiv(rows, NB))), dim3(256), s, np, k0,
+                       E->Ared32, E->Linv32, E->b32, E->y32);
+            }
+            for (int k = nbk - 1; k >= 0; --k) {
+                const int k0 = k * NB;
+                LAUNCH(KC_TRSV, k_trsv_bwd<float>, dim3(std::max(1, cdiv(k0, NB))), dim3(256), s, np, k0,
+                       E->Ared32, E->Linv32, E->y32, E->d32);
+            }
+            LAUNCH(KC_REFINE, k_refine_add, dim3(cdiv(np, 256)), dim3(256), s, np, it == 0 ? 1 : 0, E->d32,
+                   E->xbuf);
+            if (it + 1 < REFINE_STEPS) {
+                LAUNCH(KC_REFINE, k_symv_part, dim3(nbk, SYMV_SPLIT), dim3(256), s, np, SYMV_SPLIT, E->Ared,
+                       E->xbuf, E->symv);
+                LAUNCH(KC_REFINE, k_eeq_resid, dim3(cdiv(np, 256)), dim3(256), s, np, SYMV_SPLIT, E->symv, E->b,
+                       E->rvec, E->b32);
+            }
         }
     }
-    for (int k = 0; k < nbk; ++k) {
-        int k0 = k * NB;
-        int rows = np - k0 - NB;
-        hipLaunchKernelGGL(k_trsv_fwd, dim3(std::max(1, cdiv(rows, 256))), dim3(256), 0, s, np, k0, E->Ared, E->Linv,
-                           E->b, E->ybuf);
-    }
-    for (int k = nbk - 1; k >= 0; --k) {
-        int k0 = k * NB;
-        hipLaunchKernelGGL(k_trsv_bwd, dim3(std::max(1, cdiv(k0, 256))), dim3(256), 0, s, np, k0, E->Ared, E->Linv,
-                           E->ybuf, E->xbuf);
-    }
-    hipLaunchKernelGGL(k_charges, dim3(cdiv(n, 256)), dim3(256), 0, s, n, E->red_of, (const int*)nullptr,
+    LAUNCH(KC_CHARGES, k_charges, dim3(cdiv(n, 256)), dim3(256), s, n, E->red_of, (const int*)nullptr,
                        E->frag_off, E->frag_atoms, E->frag, E->qp, E->xbuf, E->chi_base, E->cnf, E->gam, E->alp, E->cn,
                        E->q, E->eself, E->qtmp);
     // pair terms
@@ -1643,43 +2164,68 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     pa.nref = E->nref;
     pa.metal = E->metal;
     pa.c6ref = E->c6ref;
+    pa.h = E->h6;
+    pa.dh = E->dh6;
     pa.g = E->gpair;
     pa.dEdcn = E->dEdcn;
     pa.e_coul = E->e_coul;
     pa.e_disp = E->e_disp;
     pa.e_rep = E->e_rep;
-    hipLaunchKernelGGL(k_pairs, dim3(cdiv(n, TILE)), dim3(TILE), 0, s, pa);
-    hipLaunchKernelGGL(k_nbrep_fix, dim3(cdiv(n, 256)), dim3(256), 0, s, n, E->x, E->Z, E->metal, E->anb, E->repz,
+    LAUNCH(KC_PAIRS_AUX, k_d4_half, dim3(cdiv(n * E->nelem, 256)), dim3(256), s, n, E->nelem, E->elem, E->nref,
+           E->c6ref, E->gw, E->dgw, E->h6, E->dh6);
+    {
+        const dim3 grid(cdiv(n, TILE), E->pair_nchunk);
+        if (!E->mixed) {
+            switch (E->nelem) {
+                case 1: LAUNCH(KC_PAIRS, k_pairs_part<1>, grid, dim3(TILE), s, pa, E->pair_cj, E->ppart); break;
+                case 2: LAUNCH(KC_PAIRS, k_pairs_part<2>, grid, dim3(TILE), s, pa, E->pair_cj, E->ppart); break;
+                case 3: LAUNCH(KC_PAIRS, k_pairs_part<3>, grid, dim3(TILE), s, pa, E->pair_cj, E->ppart); break;
+                case 4: LAUNCH(KC_PAIRS, k_pairs_part<4>, grid, dim3(TILE), s, pa, E->pair_cj, E->ppart); break;
+                default: LAUNCH(KC_PAIRS, k_pairs_part<0>, grid, dim3(TILE), s, pa, E->pair_cj, E->ppart); break;
+            }
+        } else {
+            switch (E->nelem) {
+                case 1: LAUNCH(KC_PAIRS, k_pairs_part32<1>, grid, dim3(TILE), s, pa, E->pair_cj, E->ppart); break;
+                case 2: LAUNCH(KC_PAIRS, k_pairs_part32<2>, grid, dim3(TILE), s, pa, E->pair_cj, E->ppart); break;
+                case 3: LAUNCH(KC_PAIRS, k_pairs_part32<3>, grid, dim3(TILE), s, pa, E->pair_cj, E->ppart); break;
+                case 4: LAUNCH(KC_PAIRS, k_pairs_part32<4>, grid, dim3(TILE), s, pa, E->pair_cj, E->ppart); break;
+                default: LAUNCH(KC_PAIRS, k_pairs_part32<0>, grid, dim3(TILE), s, pa, E->pair_cj, E->ppart); break;
+            }
+        }
+    }
+    LAUNCH(KC_PAIRS_AUX, k_pairs_reduce, dim3(cdiv(n, 256)), dim3(256), s, n, E->pair_nchunk, E->ppart, E->gpair,
+           E->dEdcn, E->e_coul, E->e_disp, E->e_rep);
+    LAUNCH(KC_NBREP, k_nbrep_fix, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->Z, E->metal, E->anb, E->repz,
                        E->toff, E->tidx, E->tdist, E->gpair, E->e_rep);
     // bonded lists
     if (E->nbond)
-        hipLaunchKernelGGL(k_bonds, dim3(cdiv(E->nbond, 256)), dim3(256), 0, s, E->nbond, E->bond_ij, E->bond_par,
+        LAUNCH(KC_BONDS, k_bonds, dim3(cdiv(E->nbond, 256)), dim3(256), s, E->nbond, E->bond_ij, E->bond_par,
                            E->bond_hboff, E->bond_hbB, E->bond_slot, E->x, E->cn, E->rcov_d3, E->bond_H, E->slots,
                            E->dcn_slot, E->e_bond);
     if (E->nangle)
-        hipLaunchKernelGGL(k_angles, dim3(cdiv(E->nangle, 256)), dim3(256), 0, s, E->nangle, E->ang_ijk, E->ang_par,
+        LAUNCH(KC_ANGLES, k_angles, dim3(cdiv(E->nangle, 256)), dim3(256), s, E->nangle, E->ang_ijk, E->ang_par,
                            E->ang_slot, E->x, E->rcov_d3, E->slots, E->e_ang);
     if (E->ntors)
-        hipLaunchKernelGGL(k_tors, dim3(cdiv(E->ntors, 256)), dim3(256), 0, s, E->ntors, E->tor_ijkl, E->tor_par,
+        LAUNCH(KC_TORS, k_tors, dim3(cdiv(E->ntors, 256)), dim3(256), s, E->ntors, E->tor_ijkl, E->tor_par,
                            E->tor_slot, E->x, E->rcov_d3, E->slots, E->e_tor);
     if (E->nbrep)
-        hipLaunchKernelGGL(k_brep, dim3(cdiv(E->nbrep, 256)), dim3(256), 0, s, E->nbrep, E->brep_ij, E->brep_par,
+        LAUNCH(KC_BREP, k_brep, dim3(cdiv(E->nbrep, 256)), dim3(256), s, E->nbrep, E->brep_ij, E->brep_par,
                            E->brep_slot, E->x, E->slots, E->e_brep);
     if (E->nhb)
-        hipLaunchKernelGGL(k_hb_energy, dim3(cdiv(E->nhb, 256)), dim3(256), 0, s, E->nhb, E->hb_ahb, E->hb_case,
+        LAUNCH(KC_HB, k_hb_energy, dim3(cdiv(E->nhb, 256)), dim3(256), s, E->nhb, E->hb_ahb, E->hb_case,
                            E->hb_par, E->hb_nboff, E->hb_nbidx, E->x, E->hb_rad, E->Z, E->e_hb);
-    hipLaunchKernelGGL(k_hb_owner, dim3(cdiv(n, 4)), dim3(256), 0, s, n, E->hbo_off, E->hbo_ent, E->hb_ahb,
+    LAUNCH(KC_HB, k_hb_owner, dim3(cdiv(n, 4)), dim3(256), s, n, E->hbo_off, E->hbo_ent, E->hb_ahb,
                        E->hb_case, E->hb_par, E->hb_nboff, E->hb_nbidx, E->x, E->hb_rad, E->Z, E->ghb);
     if (E->natm)
-        hipLaunchKernelGGL(k_atm, dim3(cdiv(E->natm, 256)), dim3(256), 0, s, E->natm, E->atm_ijk, E->atm_sc,
+        LAUNCH(KC_ATM, k_atm, dim3(cdiv(E->natm, 256)), dim3(256), s, E->natm, E->atm_ijk, E->atm_sc,
                            E->atm_slot, E->x, E->rcov_d3, E->nelem, E->elem, E->nref, E->c6ref, E->gw, E->slots,
                            E->e_atm);
     if (E->nbatm)
-        hipLaunchKernelGGL(k_batm, dim3(cdiv(E->nbatm, 256)), dim3(256), 0, s, E->nbatm, E->batm_ijk, E->batm_zb,
+        LAUNCH(KC_BATM, k_batm, dim3(cdiv(E->nbatm, 256)), dim3(256), s, E->nbatm, E->batm_ijk, E->batm_zb,
                            E->batm_slot, E->x, E->qa, E->slots, E->e_batm);
-    hipLaunchKernelGGL(k_dcn_total, dim3(cdiv(n, 256)), dim3(256), 0, s, n, E->dcn_off, E->dcn_idx, E->dcn_slot,
+    LAUNCH(KC_CHAIN, k_dcn_total, dim3(cdiv(n, 256)), dim3(256), s, n, E->dcn_off, E->dcn_idx, E->dcn_slot,
                        E->dEdcn, E->qtmp, E->vchain);
-    hipLaunchKernelGGL(k_finish, dim3(cdiv(n, 256)), dim3(256), 0, s, n, E->x, E->rcov_cn, E->dlog, E->vchain,
+    LAUNCH(KC_CHAIN, k_finish, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->rcov_cn, E->dlog, E->vchain,
                        E->nbn, E->nbl, E->slot_off, E->slot_idx, E->slots, E->gpair, E->ghb, E->grad);
     Segs sg;
     const double* ptrs[12] = {E->e_bond, E->e_ang, E->e_tor, nullptr, E->e_disp, E->e_brep,
@@ -1690,9 +2236,9 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         sg.n[k] = ptrs[k] ? cnts[k] : 0;
         sg.dst[k] = k;
     }
-    hipLaunchKernelGGL(k_reduce1, dim3(RED_BLOCKS, 12), dim3(256), 0, s, sg, E->rpart);
-    hipLaunchKernelGGL(k_reduce2, dim3(12), dim3(64), 0, s, sg, E->rpart, E->parts);
-    hipLaunchKernelGGL(k_total, dim3(1), dim3(64), 0, s, E->parts, E->out);
+    LAUNCH(KC_REDUCE, k_reduce1, dim3(RED_BLOCKS, 12), dim3(256), s, sg, E->rpart);
+    LAUNCH(KC_REDUCE, k_reduce2, dim3(12), dim3(64), s, sg, E->rpart, E->parts);
+    LAUNCH(KC_REDUCE, k_total, dim3(1), dim3(64), s, E->parts, E->out);
     if (hipGetLastError() != hipSuccess) {
         msg = "kernel launch failed";
         return kErrDevice;
@@ -1704,7 +2250,7 @@ int device_eval_host(DeviceEngine* E, const double* xyz_bohr, bool want_grad, do
                      double* q, double* cn, std::string& msg) {
     const int n = E->n;
     HIP_OK(hipMemcpyAsync(E->gpair, xyz_bohr, 3 * sizeof(double) * n, hipMemcpyHostToDevice, E->own));
-    // gpair is overwritten by k_pairs after the copy kernel has consumed it
+    // gpair is overwritten by k_pairs_reduce after the copy kernel has consumed it
     int st = launch_eval(E, E->gpair, E->own, msg);
     if (st) return st;
     int flags = 0;
@@ -1739,6 +2285,46 @@ int device_eval(DeviceEngine* E, const double* xyz_bohr_dev, double* energy_dev,
     return kOk;
 }
 
+int device_cn_neighbours(DeviceEngine* E, int* counts, int* idx, int maxnb, std::string& msg) {
+    const int n = E->n;
+    std::vector<int> nbl((size_t)n * MAXNB_CN);
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(counts, E->nbn, sizeof(int) * n, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(nbl.data(), E->nbl, sizeof(int) * nbl.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < maxnb; ++k)
+            idx[(size_t)i * maxnb + k] = (k < counts[i] && k < MAXNB_CN) ? nbl[(size_t)i * MAXNB_CN + k] : -1;
+    return kOk;
+}
+
+int device_profile_select(DeviceEngine* E, const char* name, std::string& msg) {
+    E->timer.used = 0;
+    E->timer.cls = -1;
+    if (!name || !*name) return kOk;
+    for (int c = 0; c < KC_COUNT; ++c)
+        if (std::strcmp(name, kKernelClassNames[c]) == 0) {
+            E->timer.cls = c;
+            return kOk;
+        }
+    msg = std::string("unknown kernel class: ") + name;
+    return kErrArgs;
+}
+
+int device_profile_read(DeviceEngine* E, double* total_ms, int* launches, std::string& msg) {
+    KernelTimer& T = E->timer;
+    *total_ms = 0.0;
+    *launches = 0;
+    if (T.used >= 2) HIP_OK(hipEventSynchronize(T.ev[T.used - 1]));
+    for (size_t i = 0; i + 1 < T.used; i += 2) {
+        float ms = 0.0f;
+        HIP_OK(hipEventElapsedTime(&ms, T.ev[i], T.ev[i + 1]));
+        *total_ms += ms;
+        ++*launches;
+    }
+    T.used = 0;
+    return kOk;
+}
+
 int device_count() {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -1752,7 +2338,7 @@ int device_dense_cholesky(int n, const double* a, double* l_out, std::string& ms
         msg = "n must be positive";
         return kErrArgs;
     }
-    const int np = cdiv(n, NB) * NB, nbk = np / NB;
+    const int np = cdiv(n, NB) * NB;
     std::vector<double> h((size_t)np * np, 0.0);
     for (int i = 0; i < np; ++i)
         for (int j = 0; j < np; ++j) h[(size_t)i * np + j] = (i < n && j < n) ? a[(size_t)i * n + j] : (i == j ? 1.0 : 0.0);
@@ -1775,15 +2361,7 @@ int device_dense_cholesky(int n, const double* a, double* l_out, std::string& ms
     int zero = 0, flag = 0;
     (void)hipMemcpyAsync(dA, h.data(), h.size() * 8, hipMemcpyHostToDevice, s);
     (void)hipMemcpyAsync(dflag, &zero, sizeof(int), hipMemcpyHostToDevice, s);
-    for (int k = 0; k < nbk; ++k) {
-        int k0 = k * NB;
-        hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(NB), 0, s, np, k0, dA, dL, dflag);
-        int rem = nbk - k - 1;
-        if (rem > 0) {
-            hipLaunchKernelGGL(k_chol_panel, dim3(rem), dim3(256), 0, s, np, k0, dA, dL);
-            hipLaunchKernelGGL(k_chol_update, dim3(rem * (rem + 1) / 2), dim3(256), 0, s, np, k0, dA);
-        }
-    }
+    queue_cholesky(nullptr, s, np, dA, dL, dflag);
     (void)hipMemcpyAsync(h.data(), dA, h.size() * 8, hipMemcpyDeviceToHost, s);
     (void)hipMemcpyAsync(&flag, dflag, sizeof(int), hipMemcpyDeviceToHost, s);
     if (hipStreamSynchronize(s) != hipSuccess) {

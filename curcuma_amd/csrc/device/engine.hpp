@@ -26,6 +26,14 @@ int device_eval(DeviceEngine* e, const double* xyz_bohr_dev, double* energy_dev,
 
 int device_count();
 
+// CN neighbour lists of the last evaluation (ascending j, strict d^2 < 36 Bohr^2):
+// counts[n], idx[n * maxnb] (-1 past the end of a list).
+int device_cn_neighbours(DeviceEngine* e, int* counts, int* idx, int maxnb, std::string& msg);
+
+// HIP-event timing of one kernel class on the evaluation stream.
+int device_profile_select(DeviceEngine* e, const char* name, std::string& msg);
+int device_profile_read(DeviceEngine* e, double* total_ms, int* launches, std::string& msg);
+
 // Dense SPD factorisation with the EEQ Cholesky kernels (host buffers).
 int device_dense_cholesky(int n, const double* a, double* l_out, std::string& msg);
 

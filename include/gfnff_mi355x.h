@@ -79,7 +79,10 @@ int gfnff_cn(gfnff_handle* h, double* cn);
  * (gfnff.h:219-293): "cn_cutoff_bohr", "hb_accuracy", "threads". */
 int gfnff_set_param(gfnff_handle* h, const char* key, double value);
 
-/* GFNFF_PREC_FP64 (default) or GFNFF_PREC_MIXED. */
+/* GFNFF_PREC_FP64 (default, the reference's arithmetic) or GFNFF_PREC_MIXED:
+ * pair terms in FP32 with FP64 coordinate differences and FP64 row sums; the
+ * reduced EEQ matrix factored in FP32 (FP32 MFMA) with FP64 iterative
+ * refinement against the FP64 matrix; everything else FP64. */
 int gfnff_set_precision(gfnff_handle* h, int32_t mode);
 
 /* ComputationalMethod::hasError() / getErrorMessage() (computational_method.h:190,201). */
@@ -99,6 +102,26 @@ int gfnff_get_topology(const gfnff_handle* h, int32_t* hyb, int32_t* pi, int32_t
 
 /* Number of visible GPUs (0 when none). */
 int gfnff_device_count(void);
+
+/* Engine extension: the CN neighbour lists of the last evaluation, as the
+ * reference's pair builder #1 defines them (CNCalculator::
+ * calculateGFNFFCNWithNeighbors, cn_calculator.cpp:367-428): every pair with
+ * d^2 < 36 Bohr^2 (strict), each list ascending in j.  counts[natoms],
+ * idx[natoms * maxnb], entries past a list's end are -1.  Lists are capped at
+ * 128 entries (the evaluation fails with GFNFF_STATUS_GEOMETRY beyond that). */
+int gfnff_cn_neighbours(gfnff_handle* h, int32_t* counts, int32_t* idx, int32_t maxnb);
+
+/* Engine extension (no counterpart in the reference): HIP-event timing of one
+ * kernel class on the evaluation stream, for roofline measurement.  Classes:
+ * cn eeq_build chol_diag chol_panel chol_update trsv charges pairs pairs_aux nbrep
+ * refine bonds angles torsions brep hb atm batm chain reduce; NULL or "" switches timing
+ * off.  Every launch of the class in later evaluations is bracketed by a
+ * start/stop event pair. */
+int gfnff_profile_kernel(gfnff_handle* h, const char* name);
+
+/* Waits for the recorded events and returns the summed launch time (ms) and
+ * the number of launches since the last read or selection; restarts the count. */
+int gfnff_profile_read(gfnff_handle* h, double* total_ms, int32_t* launches);
 
 /* Engine extension (no counterpart in the reference): dense Cholesky of an
  * n x n SPD matrix (row-major, host buffers) with the engine's blocked FP64

@@ -98,3 +98,43 @@ def test_fd_gradient_gpu():
             em = m.calculateEnergy(False)
             fd = (ep - em) / (2 * 1e-5)
             assert abs(fd - g[a, c]) < 1e-7, (a, c, fd, g[a, c])
+
+
+def brute_cn_lists(xb):
+    """Pair builder #1 of the reference (cn_calculator.cpp:392-406): i < j scan,
+    d2 = dx*dx + dy*dy + dz*dz (unfused, left to right), strict d2 < 36, j
+    appended to i and i to j, so every list is ascending."""
+    n = len(xb)
+    lists = [[] for _ in range(n)]
+    for i in range(n):
+        d = xb[i] - xb[i + 1:]
+        d2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+        for j in (np.nonzero(d2 < 36.0)[0] + i + 1):
+            lists[i].append(int(j))
+            lists[int(j)].append(i)
+    return lists
+
+
+@pytest.mark.parametrize("case", ["water64", "alkane40"])
+def test_cn_neighbour_lists_bit_exact(case):
+    from curcuma_amd._lib import as_i32, iptr, load_library
+    if case == "water64":
+        Z, xyz = water_cluster(64, seed=11)
+    else:
+        Z, xyz = alkane(40, sigma=0.02, seed=1)
+    m = GFNFFMethod()
+    assert m.setMolecule(Mol(Z, xyz)), m.getErrorMessage()
+    m.calculateEnergy(True)
+    assert not m.hasError(), m.getErrorMessage()
+    n = len(Z)
+    maxnb = 128
+    counts = as_i32(np.zeros(n))
+    idx = as_i32(np.zeros(n * maxnb))
+    assert load_library().gfnff_cn_neighbours(m.handle, iptr(counts), iptr(idx), maxnb) == 0
+    ref = brute_cn_lists(np.asarray(xyz) * (1.0 / BOHR))
+    total = 0
+    for i in range(n):
+        got = list(idx[i * maxnb:i * maxnb + counts[i]])
+        assert got == ref[i], (i, got, ref[i])
+        total += len(got)
+    print(f"{case}: {n} atoms, {total // 2} pairs, lists identical")
