@@ -38,6 +38,7 @@ def parse():
                    help="path of the headline number; the FP64 path is always timed beside it")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample", default="water_400")
+    p.add_argument("--lookahead", type=int, default=0, help="engine option cholesky_lookahead (0/1)")
     p.add_argument("--roofline-kernel", default="auto", help="kernel class timed for the roofline object")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="run one device copy of this size first (HBM counter calibration under rocprofv3 --pmc)")
@@ -114,7 +115,7 @@ def main():
     sp = C.c_void_p(stream.cuda_stream)
 
     def make(precision):
-        m = GFNFFMethod(precision=precision)
+        m = GFNFFMethod(precision=precision, parameters={"cholesky_lookahead": args.lookahead})
         t0 = time.perf_counter()
         if not m.setMolecule(Mol(Z, xyz)):
             print("setMolecule failed: " + m.getErrorMessage(), file=sys.stderr)

@@ -72,6 +72,10 @@ def load_library(path: Path | None = None) -> C.CDLL:
     lib.gfnff_get_topology.argtypes = [C.c_void_p, _i32p, _i32p, _i32p, C.c_int32, _i32p]
     lib.gfnff_device_count.restype = C.c_int
     lib.gfnff_dense_cholesky.argtypes = [C.c_int32, _f64p, _f64p]
+    lib.gfnff_dense_cholesky_f32.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
+    lib.gfnff_param_count.argtypes = [C.c_void_p]
+    lib.gfnff_param_scales.argtypes = [C.c_void_p, _f64p, C.c_int32]
+    lib.gfnff_param_adjoint.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int32]
     lib.gfnff_cn_neighbours.argtypes = [C.c_void_p, _i32p, _i32p, C.c_int32]
     lib.gfnff_profile_kernel.argtypes = [C.c_void_p, C.c_char_p]
     lib.gfnff_profile_read.argtypes = [C.c_void_p, _f64p, _i32p]

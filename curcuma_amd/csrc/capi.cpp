@@ -28,6 +28,9 @@ struct gfnff_handle {
     int precision = GFNFF_PREC_FP64;
     std::unique_ptr<gfnff::DeviceEngine> dev;
     bool dev_geom_dirty = true;
+    // per-element parameter scales (repz, repulsion exponent, sqrt(Z r4r2)) and
+    // the unscaled parameters they multiply
+    std::vector<double> theta, base_repz, base_anb, base_szr, base_brep;
 };
 
 namespace {
@@ -69,6 +72,11 @@ int gfnff_set_molecule(gfnff_handle* h, int32_t natoms, const int32_t* Z, const 
         if (st != gfnff::kOk) return fail(h, st, "gfnff_set_molecule: " + msg);
         h->have_molecule = true;
         h->dev_geom_dirty = true;
+        h->theta.assign(3 * h->params.nelem, 1.0);
+        h->base_repz = h->params.repz;
+        h->base_anb = h->params.rep_anb;
+        h->base_szr = h->params.sqrt_zr4r2;
+        h->base_brep = h->params.brep_par;
         h->last_q.assign(natoms, 0.0);
         h->last_cn.assign(natoms, 0.0);
         return GFNFF_STATUS_OK;
@@ -94,7 +102,7 @@ static int ensure_device(gfnff_handle* h) {
     if (h->dev) return GFNFF_STATUS_OK;
     std::string msg;
     int st = GFNFF_STATUS_OK;
-    h->dev.reset(gfnff::device_create(h->params, h->precision, st, msg));
+    h->dev.reset(gfnff::device_create(h->params, h->precision, h->opt.cholesky_lookahead, st, msg));
     if (!h->dev) return fail(h, st ? st : GFNFF_STATUS_DEVICE, "device set-up: " + msg);
     return GFNFF_STATUS_OK;
 }
@@ -171,6 +179,9 @@ int gfnff_set_param(gfnff_handle* h, const char* key, double value) {
         h->opt.hb_accuracy = value;
     } else if (k == "threads") {
         h->opt.threads = value < 1 ? 1 : (int)value;
+    } else if (k == "cholesky_lookahead") {
+        h->opt.cholesky_lookahead = value != 0.0;
+        h->dev.reset();
     } else {
         return fail(h, GFNFF_STATUS_UNSUPPORTED, "unknown parameter: " + k);
     }
@@ -236,11 +247,73 @@ int gfnff_profile_read(gfnff_handle* h, double* total_ms, int32_t* launches) {
     return st ? fail(h, st, "gfnff_profile_read: " + msg) : GFNFF_STATUS_OK;
 }
 
+int gfnff_param_count(const gfnff_handle* h) {
+    if (!h || !h->have_molecule) return -1;
+    return (int)h->theta.size();
+}
+
+int gfnff_param_scales(gfnff_handle* h, const double* theta, int32_t ntheta) {
+    if (!h || !theta || !h->have_molecule) return GFNFF_STATUS_ARGS;
+    if (ntheta != (int32_t)h->theta.size())
+        return fail(h, GFNFF_STATUS_ARGS, "gfnff_param_scales: expected 3 * nelem scales");
+    for (int k = 0; k < ntheta; ++k)
+        if (!(theta[k] > 0.0) || !std::isfinite(theta[k]))
+            return fail(h, GFNFF_STATUS_ARGS, "gfnff_param_scales: scales must be positive and finite");
+    auto& P = h->params;
+    const int ne = P.nelem;
+    h->theta.assign(theta, theta + ntheta);
+    for (int i = 0; i < P.n; ++i) {
+        const int e = P.elem[i];
+        P.repz[i] = h->base_repz[i] * theta[e];
+        P.rep_anb[i] = h->base_anb[i] * theta[ne + e];
+        P.sqrt_zr4r2[i] = h->base_szr[i] * theta[2 * ne + e];
+    }
+    const int nb = (int)P.brep_par.size() / GFNFF_REP_NPAR;
+    for (int b = 0; b < nb; ++b) {
+        const int ei = P.elem[P.brep_ij[2 * b]], ej = P.elem[P.brep_ij[2 * b + 1]];
+        P.brep_par[2 * b] = h->base_brep[2 * b] * std::sqrt(theta[ne + ei] * theta[ne + ej]);
+        P.brep_par[2 * b + 1] = h->base_brep[2 * b + 1] * theta[ei] * theta[ej];
+    }
+    if (h->dev) {
+        std::string msg;
+        int st = gfnff::device_update_params(h->dev.get(), P, msg);
+        if (st) return fail(h, st, "gfnff_param_scales: " + msg);
+    }
+    return GFNFF_STATUS_OK;
+}
+
+int gfnff_param_adjoint(gfnff_handle* h, const double* w, double* dtheta, int32_t ntheta) {
+    if (!h || !w || !dtheta || !h->have_molecule) return GFNFF_STATUS_ARGS;
+    if (!h->dev) return fail(h, GFNFF_STATUS_ARGS, "gfnff_param_adjoint: no evaluation yet");
+    if (ntheta != (int32_t)h->theta.size())
+        return fail(h, GFNFF_STATUS_ARGS, "gfnff_param_adjoint: expected 3 * nelem outputs");
+    try {
+        std::string msg;
+        std::vector<double> dln(ntheta);
+        int st = gfnff::device_param_adjoint(h->dev.get(), w, dln.data(), msg);
+        if (st) return fail(h, st, "gfnff_param_adjoint: " + msg);
+        for (int k = 0; k < ntheta; ++k) dtheta[k] = dln[k] / h->theta[k];
+        return GFNFF_STATUS_OK;
+    } catch (const std::exception& e) {
+        return fail(h, GFNFF_STATUS_DEVICE, std::string("gfnff_param_adjoint: ") + e.what());
+    }
+}
+
 int gfnff_dense_cholesky(int32_t n, const double* a, double* l_out) {
     if (n <= 0 || !a || !l_out) return GFNFF_STATUS_ARGS;
     std::string msg;
     try {
         return gfnff::device_dense_cholesky(n, a, l_out, msg);
+    } catch (...) {
+        return GFNFF_STATUS_DEVICE;
+    }
+}
+
+int gfnff_dense_cholesky_f32(int32_t n, const float* a, float* l_out) {
+    if (n <= 0 || !a || !l_out) return GFNFF_STATUS_ARGS;
+    std::string msg;
+    try {
+        return gfnff::device_dense_cholesky_f32(n, a, l_out, msg);
     } catch (...) {
         return GFNFF_STATUS_DEVICE;
     }

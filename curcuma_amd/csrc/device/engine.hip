@@ -49,7 +49,7 @@ constexpr double SQRTPI_TRUNC = 1.77245385091;   // constant used by the referen
 constexpr double TSQRT2PI = 0.797884560802866;
 constexpr double TSQRT2PI_SELF = 0.797884560802865;
 
-enum Flag { FLAG_CHOL = 1, FLAG_CNOVF = 2 };
+enum Flag { FLAG_CHOL = 1, FLAG_CNOVF = 2, FLAG_SOLVE = 4 };
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(256) void k_chol_diag(int np, int k0, T* __restrict
     __shared__ T rdiag[NB];
     const int tid = threadIdx.x;
     const int t = tid >> 2, q = tid & 3;
-    for (int e = tid; e < NB * NB; e += 256) sL[e / NB][e % NB] = A[(size_t)(k0 + e / NB) * np + k0 + e % NB];
+    _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) sL[e / NB][e % NB] = A[(size_t)(k0 + e / NB) * np + k0 + e % NB];
     __syncthreads();
     T a[NB / 4];
 #pragma unroll
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(256) void k_chol_diag(int np, int k0, T* __restrict
     for (int u = 0; u < NB / 4; ++u) sL[t][4 * u + q] = (4 * u + q <= t) ? a[u] : T(0);
     __syncthreads();
     if (q == 0) rdiag[t] = T(1) / sL[t][t];
-    for (int e = tid; e < NB * NB; e += 256) {
+    _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) {
         int r = e / NB, c = e % NB;
         if (c <= r) A[(size_t)(k0 + r) * np + k0 + c] = sL[r][c];
     }
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(256) void k_chol_panel(int np, int k0, T* __restric
     int tid = threadIdx.x;
     int r0 = k0 + NB + blockIdx.x * NB;
     const T* Li = Linv + (size_t)(k0 / NB) * NB * NB;
-    for (int e = tid; e < NB * NB; e += 256) {
+    _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) {
         int r = e / NB, c = e % NB;
         S[r][c] = A[(size_t)(r0 + r) * np + k0 + c];
         W[r][c] = Li[e];
@@ -417,14 +417,29 @@ __global__ __launch_bounds__(256) void k_chol_update(int np, int kc0, int kw, in
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) acc[x][y] = typename M::V{0, 0, 0, 0};
+    // chunk kc of both panels: element u of this thread is row tid/64 + 4u,
+    // column tid%64; the next chunk is fetched into registers while the
+    // current one feeds the MFMAs
+    T pr[NB * NB / 256], qr[NB * NB / 256];
+    auto fetch = [&](int kc) {
+#pragma unroll
+        for (int u = 0; u < NB * NB / 256; ++u) {
+            const int r = (tid >> 6) + 4 * u, c = tid & 63;
+            pr[u] = A[(size_t)(ri + r) * np + kc + c];
+            qr[u] = A[(size_t)(rj + r) * np + kc + c];
+        }
+    };
+    fetch(kc0);
     for (int kc = kc0; kc < kc0 + kw; kc += NB) {
         __syncthreads();
-        for (int e = tid; e < NB * NB; e += 256) {
-            const int r = e / NB, c = e % NB;
-            P[r][c] = A[(size_t)(ri + r) * np + kc + c];
-            Q[r][c] = A[(size_t)(rj + r) * np + kc + c];
+#pragma unroll
+        for (int u = 0; u < NB * NB / 256; ++u) {
+            const int r = (tid >> 6) + 4 * u, c = tid & 63;
+            P[r][c] = pr[u];
+            Q[r][c] = qr[u];
         }
         __syncthreads();
+        if (kc + NB < kc0 + kw) fetch(kc + NB);
 #pragma unroll 4
         for (int k = 0; k < NB; k += 4) {
             const T fa0 = P[wr + li][k + lk], fa1 = P[wr + 16 + li][k + lk];
@@ -435,6 +450,9 @@ __global__ __launch_bounds__(256) void k_chol_update(int np, int kc0, int kw, in
             acc[1][1] = M::op(fa1, fb1, acc[1][1]);
         }
     }
+    // off-diagonal tiles: all 16 read-modify-writes issued unconditionally;
+    // diagonal tiles keep the lower triangle only
+    T old[2][2][4];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -443,7 +461,17 @@ __global__ __launch_bounds__(256) void k_chol_update(int np, int kc0, int kw, in
             for (int v = 0; v < 4; ++v) {
                 const int r = ri + wr + 16 * x + (M::kRowFromReg ? 4 * v + lk : 4 * lk + v);
                 const int c = rj + wc + 16 * y + li;
-                if (c <= r) A[(size_t)r * np + c] -= acc[x][y][v];
+                old[x][y][v] = A[(size_t)r * np + c];
+            }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = ri + wr + 16 * x + (M::kRowFromReg ? 4 * v + lk : 4 * lk + v);
+                const int c = rj + wc + 16 * y + li;
+                if (bi != bj || c <= r) A[(size_t)r * np + c] = old[x][y][v] - acc[x][y][v];
             }
 }
 
@@ -480,10 +508,11 @@ __global__ __launch_bounds__(256) void k_trsv_fwd(int np, int k0, const T* __res
     const int tid = threadIdx.x, t = tid >> 2, q = tid & 3;
     const int r0 = k0 + NB + blockIdx.x * NB;
     const T* Li = Linv + (size_t)(k0 / NB) * NB * NB;
-    for (int e = tid; e < NB * NB; e += 256) {
+    _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) {
         const int rr = e / NB, p = e % NB;
         W[rr][p] = Li[e];
-        Tt[rr][p] = (r0 + rr < np) ? A[(size_t)(r0 + rr) * np + k0 + p] : T(0);
+        const T v = A[(size_t)min(r0 + rr, np - 1) * np + k0 + p];  // clamped: unconditional load
+        Tt[rr][p] = (r0 + rr < np) ? v : T(0);
     }
     if (tid < NB) bk[tid] = b[k0 + tid];
     __syncthreads();
@@ -517,7 +546,7 @@ __global__ __launch_bounds__(256) void k_trsv_bwd(int np, int k0, const T* __res
     __shared__ T part[4][NB];
     const int tid = threadIdx.x, t = tid >> 2, q = tid & 3;
     const T* Li = Linv + (size_t)(k0 / NB) * NB * NB;
-    for (int e = tid; e < NB * NB; e += 256) W[e / NB][e % NB] = Li[e];
+    _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) W[e / NB][e % NB] = Li[e];
     if (tid < NB) ck[tid] = c[k0 + tid];
     __syncthreads();
     T s = T(0);
@@ -542,6 +571,119 @@ __global__ __launch_bounds__(256) void k_trsv_bwd(int np, int k0, const T* __res
     if (g == 0 && col < k0) c[col] -= ((part[0][jj] + part[1][jj]) + part[2][jj]) + part[3][jj];
 }
 
+// Single-launch sweeps.  One workgroup per 64-row block; block w needs the
+// solution blocks k < w (forward) or k > w (backward).  Each workgroup stages
+// the L tile of the next dependency in LDS before waiting for its ready flag,
+// folds L(w,k) y_k into its right-hand side as soon as y_k is published, then
+// applies the stored inverse of its diagonal block and publishes y_w with an
+// agent-scope release.  Workgroups wait only for blocks published by
+// lower-numbered workgroups (the backward sweep numbers blocks in descending
+// order), so progress never depends on a workgroup that has not started.
+// Every wait is bounded: after SPIN_MAX polls the workgroup sets FLAG_SOLVE
+// and continues, so the launch always terminates.
+constexpr int SPIN_MAX = 1 << 18;
+
+// Waits for ready flag f; gives up at once when any workgroup has already
+// flagged a failed wait, so a broken sweep ends after one bounded wait.
+__device__ inline bool wait_ready(const int* f, int* flags) {
+    for (int it = 0; it < SPIN_MAX; ++it) {
+        if (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0) return true;
+        if (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_SOLVE) return false;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    atomicOr(flags, FLAG_SOLVE);
+    return false;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv_fwd_sweep(int np, const T* __restrict__ A, const T* __restrict__ Linv,
+                                                        const T* __restrict__ b, T* __restrict__ y,
+                                                        int* __restrict__ ready, int* __restrict__ flags) {
+    __shared__ T Tt[NB][NB + 1];
+    __shared__ T yk[NB];
+    __shared__ T bw[NB];
+    const int w = blockIdx.x, tid = threadIdx.x, t = tid >> 2, q = tid & 3;
+    T acc = (q == 0) ? b[w * NB + t] : T(0);  // thread (t, q): share of row t
+    bool ok = true;
+    for (int k = 0; k < w; ++k) {
+        __syncthreads();
+        _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256)
+            Tt[e / NB][e % NB] = A[(size_t)(w * NB + e / NB) * np + k * NB + e % NB];
+        if (tid < NB) {
+            ok = wait_ready(ready + k, flags) && ok;
+            yk[tid] = __hip_atomic_load(y + k * NB + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < NB / 4; ++u) acc -= Tt[t][4 * u + q] * yk[4 * u + q];
+    }
+    acc += __shfl_xor(acc, 1);
+    acc += __shfl_xor(acc, 2);
+    __syncthreads();
+    if (q == 0) bw[t] = acc;
+    const T* Li = Linv + (size_t)w * NB * NB;
+    _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) Tt[e / NB][e % NB] = Li[e];
+    __syncthreads();
+    T s = T(0);
+#pragma unroll
+    for (int u = 0; u < NB / 4; ++u) s += Tt[t][4 * u + q] * bw[4 * u + q];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if (q == 0) {
+        __hip_atomic_store(y + w * NB + t, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();  // the block is visible device-wide before the flag
+    }
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(ready + w, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (!ok && tid < NB) atomicOr(flags, FLAG_SOLVE);
+}
+
+// Backward: x_w = Linv_w^T (c_w - sum_{k > w} L(k, w)^T x_k), workgroup g
+// handles block w = nbk - 1 - g.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv_bwd_sweep(int np, const T* __restrict__ A, const T* __restrict__ Linv,
+                                                        const T* __restrict__ c, T* __restrict__ x,
+                                                        int* __restrict__ ready, int* __restrict__ flags) {
+    __shared__ T Tt[NB][NB + 1];
+    __shared__ T xk[NB];
+    __shared__ T cw[NB];
+    const int nbk = np / NB;
+    const int w = nbk - 1 - (int)blockIdx.x, tid = threadIdx.x, t = tid >> 2, q = tid & 3;
+    T acc = (q == 0) ? c[w * NB + t] : T(0);  // thread (t, q): share of column t
+    bool ok = true;
+    for (int k = nbk - 1; k > w; --k) {
+        __syncthreads();
+        _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256)
+            Tt[e / NB][e % NB] = A[(size_t)(k * NB + e / NB) * np + w * NB + e % NB];
+        if (tid < NB) {
+            ok = wait_ready(ready + k, flags) && ok;
+            xk[tid] = __hip_atomic_load(x + k * NB + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < NB / 4; ++u) acc -= Tt[4 * u + q][t] * xk[4 * u + q];
+    }
+    acc += __shfl_xor(acc, 1);
+    acc += __shfl_xor(acc, 2);
+    __syncthreads();
+    if (q == 0) cw[t] = acc;
+    const T* Li = Linv + (size_t)w * NB * NB;
+    _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) Tt[e / NB][e % NB] = Li[e];
+    __syncthreads();
+    T s = T(0);
+#pragma unroll
+    for (int u = 0; u < NB / 4; ++u) s += Tt[4 * u + q][t] * cw[4 * u + q];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if (q == 0) {
+        __hip_atomic_store(x + w * NB + t, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();  // the block is visible device-wide before the flag
+    }
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(ready + w, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (!ok && tid < NB) atomicOr(flags, FLAG_SOLVE);
+}
+
 // ---------------------------------------------------------------------------
 // K4b: mixed-precision EEQ.  The reduced matrix R is factored in FP32 and the
 // FP64 solution is won back by iterative refinement against the FP64 R:
@@ -554,7 +696,7 @@ __global__ __launch_bounds__(256) void k_lower_to_f32(int np, const double* __re
     while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
     while (bi * (bi + 1) / 2 > t) --bi;
     const int bj = t - bi * (bi + 1) / 2;
-    for (int e = threadIdx.x; e < NB * NB; e += 256) {
+    _Pragma("unroll") for (int u_ = 0, e = (int)threadIdx.x; u_ < NB * NB / 256; ++u_, e += 256) {
         const size_t o = (size_t)(bi * NB + e / NB) * np + bj * NB + e % NB;
         A32[o] = (float)A[o];
     }
@@ -576,7 +718,7 @@ __global__ __launch_bounds__(256) void k_symv_part(int np, int nsplit, const dou
         __syncthreads();
         const bool below = J > I;
         const int r0 = below ? J * NB : I * NB, c0 = below ? I * NB : J * NB;
-        for (int e = tid; e < NB * NB; e += 256) {
+        _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) {
             const int r = e / NB, c = e % NB;
             // diagonal tile: symmetric completion from its lower triangle
             Tt[r][c] = (J == I && c > r) ? R[(size_t)(r0 + c) * np + c0 + r] : R[(size_t)(r0 + r) * np + c0 + c];
@@ -1270,13 +1412,14 @@ __global__ __launch_bounds__(256) void k_brep(int nb, const int* __restrict__ ij
 }
 
 // hydrogen bonds, cases 1 and 2: ff_workspace_gfnff.cpp:938-1404
-// Evaluates hydrogen bond h and returns its energy; gout receives the
-// gradient on the atom in `role` (0 = A, 1 = H, 2 = B, 3 + k = k-th
-// neighbour of B).
-__device__ double hb_eval(int h, int role, const int* __restrict__ ahb, const int* __restrict__ hcase,
+// Evaluates hydrogen bond h once: returns its energy and writes the gradient
+// of every participating atom to the bond's slots s0 + role (0 = A, 1 = H,
+// 2 = B, 3 + k = k-th neighbour of B).
+__device__ double hb_eval(int h, int s0, const int* __restrict__ ahb, const int* __restrict__ hcase,
                           const double* __restrict__ hp, const int* __restrict__ nb_off,
                           const int* __restrict__ nb_idx, const double* __restrict__ x,
-                          const double* __restrict__ hb_rad, const int* __restrict__ Z, double3& gout) {
+                          const double* __restrict__ hb_rad, const int* __restrict__ Z,
+                          double* __restrict__ slots) {
     int A = ahb[3 * h], H = ahb[3 * h + 1], B = ahb[3 * h + 2];
     int cs = hcase[h];
     const double* q = hp + 7 * h;
@@ -1286,8 +1429,8 @@ __device__ double hb_eval(int h, int role, const int* __restrict__ ahb, const in
     int nb0 = nb_off[h], nb1 = nb_off[h + 1];
     double3 z3 = make_double3(0, 0, 0);
     double3 ga = z3, gb = z3, gh = z3;
-    gout = z3;
     double E = 0.0;
+    bool nb_written = false;
     if (rAB <= 50.0) {
         double rAH4 = rAH * rAH * rAH * rAH, rHB4 = rHB * rHB * rHB * rHB;
         double den = 1.0 / (rAH4 + rHB4);
@@ -1340,6 +1483,7 @@ __device__ double hb_eval(int h, int role, const int* __restrict__ ahb, const in
                 ga = ga + dga;
                 gb = gb + dgb;
                 gh = gh + (-1.0) * (dga + dgb);
+                nb_written = true;
                 int kk = 3;
                 for (int k = nb0; k < nb1; ++k, ++kk) {
                     double3 pn = ld3(x, nb_idx[k]);
@@ -1356,7 +1500,7 @@ __device__ double hb_eval(int h, int role, const int* __restrict__ ahb, const in
                     double3 da = (tmp2 / ranb) * dranb, db = (tmp2 / rbnb) * drbnb;
                     ga = ga + da;
                     gb = gb + db;
-                    if (role == kk) gout = (-1.0) * (da + db);
+                    st3(slots, s0 + kk, (-1.0) * (da + db));
                 }
             } else {
                 double caa = QA * basA, cbb = QB * basB;
@@ -1385,55 +1529,54 @@ __device__ double hb_eval(int h, int role, const int* __restrict__ ahb, const in
             }
         }
     }
-    if (role == 0) gout = ga;
-    else if (role == 1) gout = gh;
-    else if (role == 2) gout = gb;
+    st3(slots, s0, ga);
+    st3(slots, s0 + 1, gh);
+    st3(slots, s0 + 2, gb);
+    if (!nb_written)
+        for (int kk = 3; kk < 3 + nb1 - nb0; ++kk) st3(slots, s0 + kk, z3);
     return E;
 }
 
-__global__ __launch_bounds__(256) void k_hb_energy(int nh, const int* __restrict__ ahb, const int* __restrict__ hcase,
-                                                   const double* __restrict__ hp, const int* __restrict__ nb_off,
-                                                   const int* __restrict__ nb_idx, const double* __restrict__ x,
-                                                   const double* __restrict__ hb_rad, const int* __restrict__ Z,
-                                                   double* __restrict__ e_out) {
-    int h = blockIdx.x * blockDim.x + threadIdx.x;
+// one thread per hydrogen bond: energy and the gradient slots of its atoms
+__global__ __launch_bounds__(256) void k_hb_full(int nh, const int* __restrict__ ahb, const int* __restrict__ hcase,
+                                                 const double* __restrict__ hp, const int* __restrict__ nb_off,
+                                                 const int* __restrict__ nb_idx, const int* __restrict__ slot0,
+                                                 const double* __restrict__ x, const double* __restrict__ hb_rad,
+                                                 const int* __restrict__ Z, double* __restrict__ slots,
+                                                 double* __restrict__ e_out) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= nh) return;
-    double3 g;
-    e_out[h] = hb_eval(h, -1, ahb, hcase, hp, nb_off, nb_idx, x, hb_rad, Z, g);
+    e_out[h] = hb_eval(h, slot0[h], ahb, hcase, hp, nb_off, nb_idx, x, hb_rad, Z, slots);
 }
 
-// owner-computes gradient: one wavefront per atom evaluates every hydrogen
-// bond the atom takes part in and keeps its own component; lanes stride the
-// atom's list and a fixed butterfly combines them (deterministic, no atomics)
-__global__ __launch_bounds__(256) void k_hb_owner(int n, const int* __restrict__ off, const int* __restrict__ ent,
-                                                  const int* __restrict__ ahb, const int* __restrict__ hcase,
-                                                  const double* __restrict__ hp, const int* __restrict__ nb_off,
-                                                  const int* __restrict__ nb_idx, const double* __restrict__ x,
-                                                  const double* __restrict__ hb_rad, const int* __restrict__ Z,
-                                                  double* __restrict__ ghb) {
-    int a = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-    int lane = threadIdx.x % 64;
-    if (a >= n) return;
-    double gx = 0.0, gy = 0.0, gz = 0.0;
-    for (int k = off[a] + lane; k < off[a + 1]; k += 64) {
-        int e = ent[k];
-        double3 g;
-        hb_eval(e >> 4, e & 15, ahb, hcase, hp, nb_off, nb_idx, x, hb_rad, Z, g);
-        gx += g.x;
-        gy += g.y;
-        gz += g.z;
+// per-atom sum of the hydrogen-bond gradient slots: one wavefront per atom,
+// lanes stride the atom's slot list (CSR order), fixed butterfly
+__global__ __launch_bounds__(256) void k_hb_gather(int n, const int* __restrict__ off, const int* __restrict__ idx,
+                                                   const double* __restrict__ slots, double* __restrict__ g) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    double gx = 0, gy = 0, gz = 0;
+    for (int k = off[i] + lane; k < off[i + 1]; k += 64) {
+        const int s = idx[k];
+        gx += slots[3 * (size_t)s];
+        gy += slots[3 * (size_t)s + 1];
+        gz += slots[3 * (size_t)s + 2];
     }
-    for (int w = 32; w > 0; w >>= 1) {
-        gx += __shfl_xor(gx, w, 64);
-        gy += __shfl_xor(gy, w, 64);
-        gz += __shfl_xor(gz, w, 64);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        gx += __shfl_xor(gx, o);
+        gy += __shfl_xor(gy, o);
+        gz += __shfl_xor(gz, o);
     }
     if (lane == 0) {
-        ghb[3 * a] = gx;
-        ghb[3 * a + 1] = gy;
-        ghb[3 * a + 2] = gz;
+        g[3 * i] = gx;
+        g[3 * i + 1] = gy;
+        g[3 * i + 2] = gz;
     }
 }
+
+
 
 // D4 ATM over bonded triplets (energy +c9, gradient -c9 as in
 // ff_workspace_gfnff.cpp:1526-1658) and bonded ATM (:1665-1739)
@@ -1524,6 +1667,280 @@ __global__ __launch_bounds__(256) void k_batm(int nt, const int* __restrict__ ij
     st3(slots, q, gij + gik);                 // i
     st3(slots, q + 1, (-1.0) * gij + gjk);    // j
     st3(slots, q + 2, (-1.0) * (gik + gjk));  // k
+}
+
+// ---------------------------------------------------------------------------
+// K10: parameter adjoint.  For per-element scale factors theta of the
+// repulsion charge (repz), the repulsion exponent (rep_anb / repa) and the D4
+// sqrt(Z r4r2), and a weight field w (w_i = dLoss/dg_i of a loss on the
+// gradient g, e.g. 2 (g - g_ref) for force matching), the kernels accumulate
+//   dLoss/dln(theta_p) = sum_i w_i . d g_i / d ln(theta_p).
+// Only the D4 pair term (energy gradient and its dE/dCN chain) and the bonded
+// and non-bonded repulsion depend on these parameters; ATM uses rcov_d3, the
+// EEQ and CN are parameter-free here.  Output layout: [p * ne + e] with
+// p = 0 repz, 1 exponent, 2 sqrt(Z r4r2).
+// ---------------------------------------------------------------------------
+constexpr int ADJ_P = 3;
+
+// u_a = dlog_a sum_{b in nb(a)} dd(r_ab) (w_a - w_b) . (x_a - x_b) / r_ab:
+// the contraction of w with d CN_a / dx (same arithmetic as k_finish).
+__global__ __launch_bounds__(256) void k_adj_u(int n, const double* __restrict__ x, const double* __restrict__ rcov,
+                                               const double* __restrict__ dlog, const int* __restrict__ nbn,
+                                               const int* __restrict__ nbl, const double* __restrict__ w,
+                                               double* __restrict__ u) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const double3 xa = ld3(x, a), wa = ld3(w, a);
+    double acc = 0.0;
+    for (int k = 0; k < nbn[a]; ++k) {
+        const int b = nbl[(size_t)a * MAXNB_CN + k];
+        const double3 d = xa - ld3(x, b);
+        const double r = nrm(d);
+        const double rs = rcov[a] + rcov[b];
+        const double dr = (r - rs) / rs;
+        const double dd = (KN / SQRTPI_TRUNC) * exp(-KN * KN * dr * dr) / rs;
+        acc += dd * dot(wa - ld3(w, b), d) / r;
+    }
+    u[a] = dlog[a] * acc;
+}
+
+// derivatives of the repulsion pair term e = ab exp(-al r^1.5) / r:
+//   dE'/d ln ab = E',  dE'/d al = -0.5 sqrt(r) e + 1.5 al r^2 e
+__device__ inline void rep_adj(double ab, double al, double r, double& dp_lnab, double& dp_al) {
+    const double sr = sqrt(r);
+    const double e = ab * exp(-al * r * sr) / r;
+    dp_lnab = -e / r - 1.5 * al * sr * e;
+    dp_al = -0.5 * sr * e + 1.5 * al * r * r * e;
+}
+
+// D4 pair, derivatives with respect to q = r4r2 of E' (=dE/dr) and of the row
+// atom's dE/dCN share -dci sum zeta (ff_workspace_gfnff.cpp:715-767):
+__device__ inline void d4_adj(double c6, double dci, double zc, double q, double r2, double r, double& dEp_dq,
+                              double& ddcn_dq) {
+    const double r0 = 0.58 * sqrt(q) + 4.80;
+    const double dr0 = 0.29 / sqrt(q);
+    const double r0s = r0 * r0;
+    const double r6 = r2 * r2 * r2, r06 = r0s * r0s * r0s;
+    const double t6 = 1.0 / (r6 + r06), t8 = 1.0 / (r6 * r2 + r06 * r0s);
+    const double dt6 = -t6 * t6 * 6.0 * r06 / r0 * dr0;        // d t6 / dq
+    const double dt8 = -t8 * t8 * 8.0 * r06 * r0 * dr0;        // d t8 / dq
+    const double d8 = -8.0 * r2 * r2 * r2 * t8 * t8;
+    const double dd6 = -6.0 * r2 * r2 * 2.0 * t6 * dt6;
+    const double dd8 = -8.0 * r2 * r2 * r2 * 2.0 * t8 * dt8;
+    dEp_dq = -c6 * zc * (dd6 + 2.0 * d8 + 2.0 * q * dd8) * r;
+    ddcn_dq = -dci * zc * (dt6 + 2.0 * t8 + 2.0 * q * dt8);
+}
+
+// Pair adjoint over full rows in j chunks (the pair kernel's decomposition).
+// Row i collects sum_j (w_i . r_hat_ij) dE'_ij/dln(theta) + u_i d(dEdcn_i)/dln(theta)
+// for its own element and the partner's; part[chunk][k * n + i], k = p * ne + e.
+template <int NE>
+__global__ __launch_bounds__(TILE) void k_adj_pairs(PairArgs a, const double* __restrict__ w,
+                                                    const double* __restrict__ u, int cj,
+                                                    double* __restrict__ part) {
+    __shared__ double sx[TILE], sy[TILE], sz[TILE], ssz[TILE], san[TILE], srz[TILE], szt[TILE];
+    __shared__ double sgw[TILE * 7];
+    __shared__ int sZ[TILE], sel[TILE], smt[TILE];
+    const int n = a.n;
+    const int i = blockIdx.x * TILE + threadIdx.x;
+    const bool act = i < n;
+    const int j0 = blockIdx.y * cj, j1 = min(n, j0 + cj);
+    const double3 xi = act ? ld3(a.x, i) : make_double3(0, 0, 0);
+    const double3 wi = act ? ld3(w, i) : make_double3(0, 0, 0);
+    const double ui = act ? u[i] : 0.0, zti = act ? a.zeta[i] : 0.0, szi = act ? a.szr[i] : 0.0;
+    const double ani = act ? a.anb[i] : 0.0, rzi = act ? a.repz[i] : 0.0;
+    const int zi = act ? a.Z[i] : 0, mi = act ? a.metal[i] : 0, ei = act ? a.elem[i] : 0;
+    double hr[NE * 7], dhr[NE * 7];
+#pragma unroll
+    for (int k = 0; k < NE * 7; ++k) {
+        hr[k] = act ? a.h[(size_t)i * NE * 7 + k] : 0.0;
+        dhr[k] = act ? a.dh[(size_t)i * NE * 7 + k] : 0.0;
+    }
+    double own[ADJ_P] = {0.0, 0.0, 0.0};    // contributions through the row atom's element
+    double oth[ADJ_P][NE];                  // through the partner's element
+#pragma unroll
+    for (int p = 0; p < ADJ_P; ++p)
+#pragma unroll
+        for (int e = 0; e < NE; ++e) oth[p][e] = 0.0;
+    for (int t0 = j0; t0 < j1; t0 += TILE) {
+        __syncthreads();
+        const int j = t0 + threadIdx.x;
+        if (j < j1) {
+            sx[threadIdx.x] = a.x[3 * j];
+            sy[threadIdx.x] = a.x[3 * j + 1];
+            sz[threadIdx.x] = a.x[3 * j + 2];
+            ssz[threadIdx.x] = a.szr[j];
+            san[threadIdx.x] = a.anb[j];
+            srz[threadIdx.x] = a.repz[j];
+            szt[threadIdx.x] = a.zeta[j];
+            sZ[threadIdx.x] = a.Z[j];
+            sel[threadIdx.x] = a.elem[j];
+            smt[threadIdx.x] = a.metal[j];
+#pragma unroll
+            for (int r = 0; r < 7; ++r) sgw[threadIdx.x * 7 + r] = a.gw[(size_t)j * 7 + r];
+        }
+        __syncthreads();
+        if (!act) continue;
+        const int m = min(TILE, j1 - t0);
+        for (int jj = 0; jj < m; ++jj) {
+            if (t0 + jj == i) continue;
+            const double dx = xi.x - sx[jj], dy = xi.y - sy[jj], dz = xi.z - sz[jj];
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 > 2500.0) continue;  // beyond the D4 and repulsion cut-offs
+            const double r = sqrt(r2);
+            if (r < 1e-10) continue;
+            const double sw = (wi.x * dx + wi.y * dy + wi.z * dz) / r;
+            const int ej = sel[jj];
+            double dd4 = 0.0, dz_ = 0.0, da_ = 0.0;
+            {   // D4 (r <= 50)
+                double c6 = 0.0, dci = 0.0;
+#pragma unroll
+                for (int e = 0; e < NE; ++e)
+                    if (e == ej) {
+#pragma unroll
+                        for (int bb = 0; bb < 7; ++bb) {
+                            const double gj = sgw[jj * 7 + bb];
+                            c6 += hr[e * 7 + bb] * gj;
+                            dci += dhr[e * 7 + bb] * gj;
+                        }
+                    }
+                const double q = 3.0 * szi * ssz[jj];
+                double dEp_dq, ddcn_dq;
+                d4_adj(c6, dci, zti * szt[jj], q, r2, r, dEp_dq, ddcn_dq);
+                dd4 = (sw * dEp_dq + ui * ddcn_dq) * q;  // d / d ln(theta) of one factor of q
+            }
+            if (r <= 20.0 && r >= 1e-8) {   // non-bonded repulsion, generic pair factor
+                const double ff = rep_ff(zi, sZ[jj], mi, smt[jj]);
+                const double al = sqrt(ani * san[jj]) * ff;
+                const double ab = rzi * srz[jj] * 0.4270;
+                double dp_lnab, dp_al;
+                rep_adj(ab, al, r, dp_lnab, dp_al);
+                dz_ = sw * dp_lnab;
+                da_ = sw * dp_al * 0.5 * al;
+            }
+            own[0] += dz_;
+            own[1] += da_;
+            own[2] += dd4;
+#pragma unroll
+            for (int e = 0; e < NE; ++e)
+                if (e == ej) {
+                    oth[0][e] += dz_;
+                    oth[1][e] += da_;
+                    oth[2][e] += dd4;
+                }
+        }
+    }
+    if (!act) return;
+    double* pp = part + (size_t)blockIdx.y * ADJ_P * NE * n + i;
+#pragma unroll
+    for (int p = 0; p < ADJ_P; ++p)
+#pragma unroll
+        for (int e = 0; e < NE; ++e) pp[(size_t)(p * NE + e) * n] = oth[p][e] + (e == ei ? own[p] : 0.0);
+}
+
+// Topological corrections of the non-bonded repulsion (bonded pairs removed,
+// H-H 1,3 / 1,4 factors), mirroring k_nbrep_fix; row-wise, appended as one
+// more chunk of partials.
+template <int NE>
+__global__ __launch_bounds__(256) void k_adj_nbrep_fix(int n, const double* __restrict__ x,
+                                                       const int* __restrict__ Z, const int* __restrict__ metal,
+                                                       const int* __restrict__ elem, const double* __restrict__ anb,
+                                                       const double* __restrict__ repz, const int* __restrict__ toff,
+                                                       const int* __restrict__ tidx, const int* __restrict__ tdist,
+                                                       const double* __restrict__ w, double* __restrict__ part) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double3 xi = ld3(x, i), wi = ld3(w, i);
+    double acc[ADJ_P - 1][NE];
+#pragma unroll
+    for (int p = 0; p < ADJ_P - 1; ++p)
+#pragma unroll
+        for (int e = 0; e < NE; ++e) acc[p][e] = 0.0;
+    const int ei = elem[i];
+    for (int k = toff[i]; k < toff[i + 1]; ++k) {
+        const int j = tidx[k], td = tdist[k];
+        const int zi = Z[i], zj = Z[j];
+        const double fgen = rep_ff(zi, zj, metal[i], metal[j]);
+        double fnew;
+        if (td == 1) fnew = 0.0;
+        else if (zi == 1 && zj == 1 && td == 2) fnew = 0.6290 * 1.4580;
+        else if (zi == 1 && zj == 1 && td == 3) fnew = 0.6290 * 0.7080;
+        else continue;
+        const double3 d = xi - ld3(x, j);
+        const double r = nrm(d);
+        if (r > 20.0 || r < 1e-8) continue;
+        const double sw = dot(wi, d) / r;
+        const double ab = repz[i] * repz[j] * 0.4270;
+        const double base = sqrt(anb[i] * anb[j]);
+        double dz_ = 0.0, da_ = 0.0, dp_lnab, dp_al;
+        rep_adj(ab, base * fgen, r, dp_lnab, dp_al);
+        dz_ -= sw * dp_lnab;
+        da_ -= sw * dp_al * 0.5 * base * fgen;
+        if (fnew != 0.0) {
+            rep_adj(ab, base * fnew, r, dp_lnab, dp_al);
+            dz_ += sw * dp_lnab;
+            da_ += sw * dp_al * 0.5 * base * fnew;
+        }
+        const int ej = elem[j];
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const double c = (e == ei ? 1.0 : 0.0) + (e == ej ? 1.0 : 0.0);
+            acc[0][e] += c * dz_;
+            acc[1][e] += c * da_;
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        part[(size_t)(0 * NE + e) * n + i] = acc[0][e];
+        part[(size_t)(1 * NE + e) * n + i] = acc[1][e];
+        part[(size_t)(2 * NE + e) * n + i] = 0.0;
+    }
+}
+
+// Bonded repulsion list: entry b couples atoms i, j with al = sqrt(repa_i repa_j),
+// ab = repz_i repz_j 1.7583; its gradient contributes (w_i - w_j) . r_hat dE'/dtheta.
+// One row of partials per entry: bpart[k * nb + b].
+template <int NE>
+__global__ __launch_bounds__(256) void k_adj_brep(int nb, const int* __restrict__ ij, const double* __restrict__ par,
+                                                  const int* __restrict__ elem, const double* __restrict__ x,
+                                                  const double* __restrict__ w, double* __restrict__ bpart) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int i = ij[2 * b], j = ij[2 * b + 1];
+    const double3 d = ld3(x, i) - ld3(x, j);
+    const double r = nrm(d);
+    double dz_ = 0.0, da_ = 0.0;
+    if (!(r > 20.0 || r < 1e-8)) {
+        const double al = par[2 * b], ab = par[2 * b + 1];
+        const double sw = dot(ld3(w, i) - ld3(w, j), d) / r;
+        double dp_lnab, dp_al;
+        rep_adj(ab, al, r, dp_lnab, dp_al);
+        dz_ = sw * dp_lnab;
+        da_ = sw * dp_al * 0.5 * al;
+    }
+    const int ei = elem[i], ej = elem[j];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        const double c = (e == ei ? 1.0 : 0.0) + (e == ej ? 1.0 : 0.0);
+        bpart[(size_t)(0 * NE + e) * nb + b] = c * dz_;
+        bpart[(size_t)(1 * NE + e) * nb + b] = c * da_;
+        bpart[(size_t)(2 * NE + e) * nb + b] = 0.0;
+    }
+}
+
+// Fixed-order reduction of the adjoint partials: one wavefront per output
+// component k; lanes stride the rows, then a fixed butterfly.
+__global__ __launch_bounds__(64) void k_adj_reduce(int n, int nchunk, const double* __restrict__ part, int nb,
+                                                   const double* __restrict__ bpart, int nk,
+                                                   double* __restrict__ out) {
+    const int k = blockIdx.x, lane = threadIdx.x;
+    double acc = 0.0;
+    for (int c = 0; c < nchunk; ++c)
+        for (int i = lane; i < n; i += 64) acc += part[((size_t)c * nk + k) * n + i];
+    for (int b = lane; b < nb; b += 64) acc += bpart[(size_t)k * nb + b];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) out[k] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -1674,6 +2091,12 @@ struct KernelTimer {
     }
 };
 
+// Side stream and fork/join events for the factorisation lookahead.
+struct Lookahead {
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+
 #define LAUNCH(cls, kern, grid, block, stream, ...)                           \
     do {                                                                      \
         E->timer.mark(cls, stream);                                           \
@@ -1689,6 +2112,10 @@ struct DeviceEngine {
     int nslots = 0;
     int pair_cj = 0, pair_nchunk = 0;   // j-chunk size and count of the pair kernel
     bool mixed = false;                 // FP32 pair terms + FP32 factorisation with FP64 refinement
+    Lookahead la;                       // side stream of the factorisation
+    bool use_lookahead = false;
+    int* ready = nullptr;               // per-block ready flags of the solve sweeps
+    double *adj_w = nullptr, *adj_u = nullptr, *adj_part = nullptr, *adj_bpart = nullptr, *adj_out = nullptr;
     float *Ared32 = nullptr, *Linv32 = nullptr, *b32 = nullptr, *y32 = nullptr, *d32 = nullptr;
     double *rvec = nullptr, *symv = nullptr;
     bool charged = false;
@@ -1707,7 +2134,8 @@ struct DeviceEngine {
     double* tor_par;
     int *brep_ij, *brep_slot;
     double* brep_par;
-    int *hb_ahb, *hb_case, *hb_nboff, *hb_nbidx, *hbo_off, *hbo_ent;
+    int *hb_ahb, *hb_case, *hb_nboff, *hb_nbidx, *hbs_off, *hbs_idx, *hb_slot0;
+    double* hb_slots;
     double* hb_par;
     int *atm_ijk, *atm_slot;
     double* atm_sc;
@@ -1720,6 +2148,33 @@ struct DeviceEngine {
         *e_batm, *parts, *rpart, *out, *diag, *ghb, *ybuf, *xbuf, *h6, *dh6, *ppart;
     int *nbn, *nbl, *flags;
 
+    PairArgs pair_args() const {
+        PairArgs a;
+        a.n = n;
+        a.nelem = nelem;
+        a.x = x;
+        a.q = q;
+        a.alp = alp;
+        a.gw = gw;
+        a.dgw = dgw;
+        a.zeta = zeta;
+        a.szr = szr;
+        a.anb = anb;
+        a.repz = repz;
+        a.Z = Z;
+        a.elem = elem;
+        a.nref = nref;
+        a.metal = metal;
+        a.c6ref = c6ref;
+        a.h = h6;
+        a.dh = dh6;
+        a.g = gpair;
+        a.dEdcn = dEdcn;
+        a.e_coul = e_coul;
+        a.e_disp = e_disp;
+        a.e_rep = e_rep;
+        return a;
+    }
     template <class T>
     T* alloc(size_t count) {
         void* p = nullptr;
@@ -1736,6 +2191,12 @@ struct DeviceEngine {
         return d;
     }
     ~DeviceEngine() {
+        if (la.side) {
+            (void)hipStreamSynchronize(la.side);
+            (void)hipStreamDestroy(la.side);
+        }
+        if (la.fork) (void)hipEventDestroy(la.fork);
+        if (la.join) (void)hipEventDestroy(la.join);
         if (own) {
             (void)hipStreamSynchronize(own);
             (void)hipStreamDestroy(own);
@@ -1744,7 +2205,7 @@ struct DeviceEngine {
     }
 };
 
-DeviceEngine* device_create(const ParamSet& P, int precision, int& status, std::string& msg) {
+DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, int& status, std::string& msg) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         status = kErrDevice;
@@ -1769,6 +2230,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, int& status, std::
         }
     auto* E = new DeviceEngine();
     E->mixed = precision == 1;
+    E->use_lookahead = lookahead;
     try {
         const int n = P.n;
         E->n = n;
@@ -1895,24 +2357,26 @@ DeviceEngine* device_create(const ParamSet& P, int precision, int& status, std::
         E->brep_slot = E->upload(rslot);
         E->nhb = (int)P.hb_case.size();
         {
-            // per-atom list of (hb << 4 | role) for the owner-computes gradient
-            std::vector<int> cnt(n + 1, 0);
+            // gradient slots of the hydrogen bonds, bond-major (3 + neighbours of
+            // B per bond), and the per-atom CSR over them in ascending slot order
+            std::vector<int> s0(E->nhb), atom_of;
+            atom_of.reserve((size_t)E->nhb * 5);
             for (int h = 0; h < E->nhb; ++h) {
-                int nbb = P.hb_nbB_off[h + 1] - P.hb_nbB_off[h];
-                if (nbb > 12 || h >= (1 << 27)) throw std::runtime_error("hydrogen-bond list exceeds the owner encoding");
-                for (int r = 0; r < 3; ++r) cnt[P.hb_ahb[3 * h + r] + 1]++;
-                for (int k = P.hb_nbB_off[h]; k < P.hb_nbB_off[h + 1]; ++k) cnt[P.hb_nbB_idx[k] + 1]++;
+                s0[h] = (int)atom_of.size();
+                for (int r = 0; r < 3; ++r) atom_of.push_back(P.hb_ahb[3 * h + r]);
+                for (int k = P.hb_nbB_off[h]; k < P.hb_nbB_off[h + 1]; ++k) atom_of.push_back(P.hb_nbB_idx[k]);
+                if (atom_of.size() > (size_t)INT32_MAX / 4)
+                    throw std::runtime_error("hydrogen-bond slot count exceeds the index range");
             }
+            std::vector<int> cnt(n + 1, 0), ent(atom_of.size());
+            for (int at : atom_of) cnt[at + 1]++;
             for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
-            std::vector<int> ent(cnt[n]), fill(cnt.begin(), cnt.end() - 1);
-            for (int h = 0; h < E->nhb; ++h) {
-                for (int r = 0; r < 3; ++r) ent[fill[P.hb_ahb[3 * h + r]]++] = (h << 4) | r;
-                int kk = 3;
-                for (int k = P.hb_nbB_off[h]; k < P.hb_nbB_off[h + 1]; ++k, ++kk)
-                    ent[fill[P.hb_nbB_idx[k]]++] = (h << 4) | kk;
-            }
-            E->hbo_off = E->upload(cnt);
-            E->hbo_ent = E->upload(ent);
+            std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+            for (size_t k = 0; k < atom_of.size(); ++k) ent[fill[atom_of[k]]++] = (int)k;
+            E->hbs_off = E->upload(cnt);
+            E->hbs_idx = E->upload(ent);
+            E->hb_slot0 = E->upload(s0);
+            E->hb_slots = E->alloc<double>(3 * std::max<size_t>(atom_of.size(), 1));
         }
         E->hb_ahb = E->upload(P.hb_ahb);
         E->hb_case = E->upload(P.hb_case);
@@ -1979,6 +2443,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, int& status, std::
         E->b = E->alloc<double>(E->np);
         E->ybuf = E->alloc<double>(E->np);
         E->xbuf = E->alloc<double>(E->np);
+        E->ready = E->alloc<int>(E->np / NB);
         if (E->mixed) {
             E->Ared32 = E->alloc<float>((size_t)E->np * E->np);
             E->Linv32 = E->alloc<float>((size_t)E->np * NB);
@@ -2025,7 +2490,10 @@ DeviceEngine* device_create(const ParamSet& P, int precision, int& status, std::
         E->nbn = E->alloc<int>(n);
         E->nbl = E->alloc<int>((size_t)n * MAXNB_CN);
         E->flags = E->alloc<int>(1);
-        if (hipStreamCreate(&E->own) != hipSuccess) throw std::runtime_error("stream");
+        if (hipStreamCreate(&E->own) != hipSuccess || hipStreamCreate(&E->la.side) != hipSuccess ||
+            hipEventCreateWithFlags(&E->la.fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&E->la.join, hipEventDisableTiming) != hipSuccess)
+            throw std::runtime_error("stream");
         hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, E->own, 1, E->flags, 0);
         hipLaunchKernelGGL(k_eeq_diag, dim3(cdiv(n, 256)), dim3(256), 0, E->own, n, E->gam, E->alp, E->diag);
         if (hipStreamSynchronize(E->own) != hipSuccess) throw std::runtime_error("init sync");
@@ -2050,40 +2518,84 @@ void device_destroy(DeviceEngine* e) { delete e; }
 // (np x np, np a multiple of NB) on stream s; Linv receives the inverses of the
 // diagonal blocks, flags FLAG_CHOL on a non-positive pivot.  tm (optional)
 // times the kernel classes.
+//
+// Lookahead (la != nullptr): once super-panel K is factored, the columns of the
+// next super-panel receive their rank-(SP NB) update first on s; the rest of
+// the trailing matrix is updated on the side stream while s factors the next
+// super-panel, which touches only its own columns.  The side stream is joined
+// before the next trailing update and at the end.
 template <typename T>
-static void queue_cholesky(KernelTimer* tm, hipStream_t s, int np, T* A, T* Linv, int* flags) {
-    auto mark = [&](int c) {
-        if (tm) tm->mark(c, s);
+static void queue_cholesky(KernelTimer* tm, hipStream_t s, int np, T* A, T* Linv, int* flags,
+                           const Lookahead* la = nullptr) {
+    auto mark = [&](int c, hipStream_t st) {
+        if (tm) tm->mark(c, st);
     };
     const int nbk = np / NB;
+    bool pending = false;  // side-stream update not yet joined
     for (int K = 0; K < nbk; K += SP) {
         const int kend = std::min(K + SP, nbk);
         for (int k = K; k < kend; ++k) {
             const int k0 = k * NB;
-            mark(KC_CHOL_DIAG);
+            mark(KC_CHOL_DIAG, s);
             hipLaunchKernelGGL(k_chol_diag<T>, dim3(1), dim3(256), 0, s, np, k0, A, Linv, flags);
-            mark(KC_CHOL_DIAG);
+            mark(KC_CHOL_DIAG, s);
             const int rem = nbk - k - 1;
             if (rem == 0) continue;
-            mark(KC_CHOL_PANEL);
+            mark(KC_CHOL_PANEL, s);
             hipLaunchKernelGGL(k_chol_panel<T>, dim3(rem), dim3(256), 0, s, np, k0, A, Linv);
-            mark(KC_CHOL_PANEL);
+            mark(KC_CHOL_PANEL, s);
             const int w = kend - k - 1;
             if (w > 0) {
                 const int ntiles = w * (w + 1) / 2 + (rem - w) * w;
-                mark(KC_CHOL_UPDATE);
+                mark(KC_CHOL_UPDATE, s);
                 hipLaunchKernelGGL(k_chol_update<T>, dim3(ntiles), dim3(256), 0, s, np, k0, NB, k + 1, w, nbk, A);
-                mark(KC_CHOL_UPDATE);
+                mark(KC_CHOL_UPDATE, s);
             }
         }
-        if (kend < nbk) {
-            const int M = nbk - kend;
-            mark(KC_CHOL_UPDATE);
-            hipLaunchKernelGGL(k_chol_update<T>, dim3(M * (M + 1) / 2), dim3(256), 0, s, np, K * NB, (kend - K) * NB,
-                               kend, M, nbk, A);
-            mark(KC_CHOL_UPDATE);
+        if (kend >= nbk) break;
+        if (pending) {  // the previous side update wrote the tiles updated next
+            (void)hipStreamWaitEvent(s, la->join, 0);
+            pending = false;
         }
+        const int M = nbk - kend;
+        const int kend2 = std::min(kend + SP, nbk);
+        if (!la || kend2 >= nbk) {
+            mark(KC_CHOL_UPDATE, s);
+            hipLaunchKernelGGL(k_chol_update<T>, dim3(M * (M + 1) / 2), dim3(256), 0, s, np, K * NB,
+                               (kend - K) * NB, kend, M, nbk, A);
+            mark(KC_CHOL_UPDATE, s);
+            continue;
+        }
+        // rest of the trailing matrix (column blocks >= kend2) on the side stream
+        const int M2 = nbk - kend2;
+        (void)hipEventRecord(la->fork, s);
+        (void)hipStreamWaitEvent(la->side, la->fork, 0);
+        mark(KC_CHOL_UPDATE, la->side);
+        hipLaunchKernelGGL(k_chol_update<T>, dim3(M2 * (M2 + 1) / 2), dim3(256), 0, la->side, np, K * NB,
+                           (kend - K) * NB, kend2, M2, nbk, A);
+        mark(KC_CHOL_UPDATE, la->side);
+        (void)hipEventRecord(la->join, la->side);
+        pending = true;
+        // columns of the next super-panel (column blocks [kend, kend2), all rows) on s
+        const int w = kend2 - kend;
+        mark(KC_CHOL_UPDATE, s);
+        hipLaunchKernelGGL(k_chol_update<T>, dim3(w * (w + 1) / 2 + (M - w) * w), dim3(256), 0, s, np, K * NB,
+                           (kend - K) * NB, kend, w, nbk, A);
+        mark(KC_CHOL_UPDATE, s);
     }
+    if (pending) (void)hipStreamWaitEvent(s, la->join, 0);
+}
+
+// Forward and backward substitution with the blocked factor, one launch per
+// sweep (k_trsv_*_sweep); the ready flags are cleared by a fill kernel, not a
+// memset, so that the sequence can be captured in a graph.
+template <typename T>
+static void queue_solve(DeviceEngine* E, hipStream_t s, const T* A, const T* Linv, const T* b, T* y, T* x) {
+    const int np = E->np, nbk = np / NB;
+    LAUNCH(KC_TRSV, k_fill_i, dim3(cdiv(nbk, 256)), dim3(256), s, nbk, E->ready, 0);
+    LAUNCH(KC_TRSV, k_trsv_fwd_sweep<T>, dim3(nbk), dim3(256), s, np, A, Linv, b, y, E->ready, E->flags);
+    LAUNCH(KC_TRSV, k_fill_i, dim3(cdiv(nbk, 256)), dim3(256), s, nbk, E->ready, 0);
+    LAUNCH(KC_TRSV, k_trsv_bwd_sweep<T>, dim3(nbk), dim3(256), s, np, A, Linv, y, x, E->ready, E->flags);
 }
 
 static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::string& msg) {
@@ -2105,34 +2617,14 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     LAUNCH(KC_EEQ_BUILD, k_eeq_rhs_reduce, dim3(cdiv(np, 256)), dim3(256), s, E->nr, np, E->red_atom, E->red_last,
                        rsrc, E->b);
     if (!E->mixed) {
-        queue_cholesky(&E->timer, s, np, E->Ared, E->Linv, E->flags);
-        for (int k = 0; k < nbk; ++k) {
-            const int k0 = k * NB;
-            const int rows = np - k0 - NB;
-            LAUNCH(KC_TRSV, k_trsv_fwd<double>, dim3(std::max(1, cdiv(rows, NB))), dim3(256), s, np, k0, E->Ared,
-                   E->Linv, E->b, E->ybuf);
-        }
-        for (int k = nbk - 1; k >= 0; --k) {
-            const int k0 = k * NB;
-            LAUNCH(KC_TRSV, k_trsv_bwd<double>, dim3(std::max(1, cdiv(k0, NB))), dim3(256), s, np, k0, E->Ared,
-                   E->Linv, E->ybuf, E->xbuf);
-        }
+        queue_cholesky(&E->timer, s, np, E->Ared, E->Linv, E->flags, E->use_lookahead ? &E->la : nullptr);
+        queue_solve<double>(E, s, E->Ared, E->Linv, E->b, E->ybuf, E->xbuf);
     } else {
         LAUNCH(KC_EEQ_BUILD, k_lower_to_f32, dim3(nbk * (nbk + 1) / 2), dim3(256), s, np, E->Ared, E->Ared32);
-        queue_cholesky(&E->timer, s, np, E->Ared32, E->Linv32, E->flags);
+        queue_cholesky(&E->timer, s, np, E->Ared32, E->Linv32, E->flags, E->use_lookahead ? &E->la : nullptr);
         LAUNCH(KC_REFINE, k_to_f32, dim3(cdiv(np, 256)), dim3(256), s, np, E->b, E->b32);
         for (int it = 0; it < REFINE_STEPS; ++it) {
-            for (int k = 0; k < nbk; ++k) {
-                const int k0 = k * NB;
-                const int rows = np - k0 - NB;
-                LAUNCH(KC_TRSV, k_trsv_fwd<float>, dim3(std::max(1, cdiv(rows, NB))), dim3(256), s, np, k0,
-                       E->Ared32, E->Linv32, E->b32, E->y32);
-            }
-            for (int k = nbk - 1; k >= 0; --k) {
-                const int k0 = k * NB;
-                LAUNCH(KC_TRSV, k_trsv_bwd<float>, dim3(std::max(1, cdiv(k0, NB))), dim3(256), s, np, k0,
-                       E->Ared32, E->Linv32, E->y32, E->d32);
-            }
+            queue_solve<float>(E, s, E->Ared32, E->Linv32, E->b32, E->y32, E->d32);
             LAUNCH(KC_REFINE, k_refine_add, dim3(cdiv(np, 256)), dim3(256), s, np, it == 0 ? 1 : 0, E->d32,
                    E->xbuf);
             if (it + 1 < REFINE_STEPS) {
@@ -2147,30 +2639,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
                        E->frag_off, E->frag_atoms, E->frag, E->qp, E->xbuf, E->chi_base, E->cnf, E->gam, E->alp, E->cn,
                        E->q, E->eself, E->qtmp);
     // pair terms
-    PairArgs pa;
-    pa.n = n;
-    pa.nelem = E->nelem;
-    pa.x = E->x;
-    pa.q = E->q;
-    pa.alp = E->alp;
-    pa.gw = E->gw;
-    pa.dgw = E->dgw;
-    pa.zeta = E->zeta;
-    pa.szr = E->szr;
-    pa.anb = E->anb;
-    pa.repz = E->repz;
-    pa.Z = E->Z;
-    pa.elem = E->elem;
-    pa.nref = E->nref;
-    pa.metal = E->metal;
-    pa.c6ref = E->c6ref;
-    pa.h = E->h6;
-    pa.dh = E->dh6;
-    pa.g = E->gpair;
-    pa.dEdcn = E->dEdcn;
-    pa.e_coul = E->e_coul;
-    pa.e_disp = E->e_disp;
-    pa.e_rep = E->e_rep;
+    PairArgs pa = E->pair_args();
     LAUNCH(KC_PAIRS_AUX, k_d4_half, dim3(cdiv(n * E->nelem, 256)), dim3(256), s, n, E->nelem, E->elem, E->nref,
            E->c6ref, E->gw, E->dgw, E->h6, E->dh6);
     {
@@ -2212,10 +2681,9 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         LAUNCH(KC_BREP, k_brep, dim3(cdiv(E->nbrep, 256)), dim3(256), s, E->nbrep, E->brep_ij, E->brep_par,
                            E->brep_slot, E->x, E->slots, E->e_brep);
     if (E->nhb)
-        LAUNCH(KC_HB, k_hb_energy, dim3(cdiv(E->nhb, 256)), dim3(256), s, E->nhb, E->hb_ahb, E->hb_case,
-                           E->hb_par, E->hb_nboff, E->hb_nbidx, E->x, E->hb_rad, E->Z, E->e_hb);
-    LAUNCH(KC_HB, k_hb_owner, dim3(cdiv(n, 4)), dim3(256), s, n, E->hbo_off, E->hbo_ent, E->hb_ahb,
-                       E->hb_case, E->hb_par, E->hb_nboff, E->hb_nbidx, E->x, E->hb_rad, E->Z, E->ghb);
+        LAUNCH(KC_HB, k_hb_full, dim3(cdiv(E->nhb, 256)), dim3(256), s, E->nhb, E->hb_ahb, E->hb_case, E->hb_par,
+               E->hb_nboff, E->hb_nbidx, E->hb_slot0, E->x, E->hb_rad, E->Z, E->hb_slots, E->e_hb);
+    LAUNCH(KC_HB, k_hb_gather, dim3(cdiv(n, 4)), dim3(256), s, n, E->hbs_off, E->hbs_idx, E->hb_slots, E->ghb);
     if (E->natm)
         LAUNCH(KC_ATM, k_atm, dim3(cdiv(E->natm, 256)), dim3(256), s, E->natm, E->atm_ijk, E->atm_sc,
                            E->atm_slot, E->x, E->rcov_d3, E->nelem, E->elem, E->nref, E->c6ref, E->gw, E->slots,
@@ -2268,6 +2736,10 @@ int device_eval_host(DeviceEngine* E, const double* xyz_bohr, bool want_grad, do
         msg = "more than 128 atoms within the 6 Bohr CN cut-off of one atom";
         return kErrGeometry;
     }
+    if (flags & FLAG_SOLVE) {
+        msg = "EEQ triangular solve did not complete (a block waited beyond its bound)";
+        return kErrDevice;
+    }
     return kOk;
 }
 
@@ -2294,6 +2766,70 @@ int device_cn_neighbours(DeviceEngine* E, int* counts, int* idx, int maxnb, std:
     for (int i = 0; i < n; ++i)
         for (int k = 0; k < maxnb; ++k)
             idx[(size_t)i * maxnb + k] = (k < counts[i] && k < MAXNB_CN) ? nbl[(size_t)i * MAXNB_CN + k] : -1;
+    return kOk;
+}
+
+int device_update_params(DeviceEngine* E, const ParamSet& P, std::string& msg) {
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(E->szr, P.sqrt_zr4r2.data(), sizeof(double) * P.n, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(E->anb, P.rep_anb.data(), sizeof(double) * P.n, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(E->repz, P.repz.data(), sizeof(double) * P.n, hipMemcpyHostToDevice));
+    if (!P.brep_par.empty())
+        HIP_OK(hipMemcpy(E->brep_par, P.brep_par.data(), sizeof(double) * P.brep_par.size(),
+                         hipMemcpyHostToDevice));
+    return kOk;
+}
+
+template <int NE>
+static void queue_adjoint(DeviceEngine* E, hipStream_t s, const PairArgs& pa, double* part, double* bpart) {
+    const int n = E->n;
+    const dim3 grid(cdiv(n, TILE), E->pair_nchunk);
+    hipLaunchKernelGGL(k_adj_pairs<NE>, grid, dim3(TILE), 0, s, pa, E->adj_w, E->adj_u, E->pair_cj, part);
+    hipLaunchKernelGGL(k_adj_nbrep_fix<NE>, dim3(cdiv(n, 256)), dim3(256), 0, s, n, E->x, E->Z, E->metal, E->elem,
+                       E->anb, E->repz, E->toff, E->tidx, E->tdist, E->adj_w,
+                       part + (size_t)E->pair_nchunk * ADJ_P * NE * n);
+    if (E->nbrep)
+        hipLaunchKernelGGL(k_adj_brep<NE>, dim3(cdiv(E->nbrep, 256)), dim3(256), 0, s, E->nbrep, E->brep_ij,
+                           E->brep_par, E->elem, E->x, E->adj_w, bpart);
+    hipLaunchKernelGGL(k_adj_reduce, dim3(ADJ_P * NE), dim3(64), 0, s, n, E->pair_nchunk + 1, part, E->nbrep, bpart,
+                       ADJ_P * NE, E->adj_out);
+}
+
+// dLoss/dln(theta) = sum_i w_i . dg_i/dln(theta) at the state of the last
+// evaluation (coordinates, charges, CN and D4 weights on the device).
+int device_param_adjoint(DeviceEngine* E, const double* w, double* out, std::string& msg) {
+    const int n = E->n, ne = E->nelem;
+    if (ne < 1 || ne > 4) {
+        msg = "the parameter adjoint is built for up to 4 elements";
+        return kErrUnsupported;
+    }
+    hipStream_t s = E->own;
+    HIP_OK(hipDeviceSynchronize());
+    if (!E->adj_w) {
+        try {
+            E->adj_w = E->alloc<double>(3 * (size_t)n);
+            E->adj_u = E->alloc<double>(n);
+            E->adj_part = E->alloc<double>((size_t)(E->pair_nchunk + 1) * ADJ_P * ne * n);
+            E->adj_bpart = E->alloc<double>((size_t)ADJ_P * ne * std::max(E->nbrep, 1));
+            E->adj_out = E->alloc<double>(ADJ_P * ne);
+        } catch (const std::bad_alloc&) {
+            msg = "device allocation failed";
+            return kErrDevice;
+        }
+    }
+    HIP_OK(hipMemcpyAsync(E->adj_w, w, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_adj_u, dim3(cdiv(n, 256)), dim3(256), 0, s, n, E->x, E->rcov_cn, E->dlog, E->nbn, E->nbl,
+                       E->adj_w, E->adj_u);
+    PairArgs pa = E->pair_args();
+    switch (ne) {
+        case 1: queue_adjoint<1>(E, s, pa, E->adj_part, E->adj_bpart); break;
+        case 2: queue_adjoint<2>(E, s, pa, E->adj_part, E->adj_bpart); break;
+        case 3: queue_adjoint<3>(E, s, pa, E->adj_part, E->adj_bpart); break;
+        default: queue_adjoint<4>(E, s, pa, E->adj_part, E->adj_bpart); break;
+    }
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(out, E->adj_out, sizeof(double) * ADJ_P * ne, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
     return kOk;
 }
 
@@ -2333,16 +2869,17 @@ int device_count() {
 
 // Blocked Cholesky of a dense SPD matrix (the EEQ factorisation kernels on an
 // arbitrary input).  a: n x n row-major; l_out receives the lower factor.
-int device_dense_cholesky(int n, const double* a, double* l_out, std::string& msg) {
+template <typename T>
+static int dense_cholesky(int n, const T* a, T* l_out, std::string& msg) {
     if (n <= 0) {
         msg = "n must be positive";
         return kErrArgs;
     }
     const int np = cdiv(n, NB) * NB;
-    std::vector<double> h((size_t)np * np, 0.0);
+    std::vector<T> h((size_t)np * np, 0.0);
     for (int i = 0; i < np; ++i)
-        for (int j = 0; j < np; ++j) h[(size_t)i * np + j] = (i < n && j < n) ? a[(size_t)i * n + j] : (i == j ? 1.0 : 0.0);
-    double *dA = nullptr, *dL = nullptr;
+        for (int j = 0; j < np; ++j) h[(size_t)i * np + j] = (i < n && j < n) ? a[(size_t)i * n + j] : (i == j ? T(1) : T(0));
+    T *dA = nullptr, *dL = nullptr;
     int* dflag = nullptr;
     hipStream_t s = nullptr;
     int st = kOk;
@@ -2352,17 +2889,17 @@ int device_dense_cholesky(int n, const double* a, double* l_out, std::string& ms
         if (dL) (void)hipFree(dL);
         if (dflag) (void)hipFree(dflag);
     };
-    if (hipStreamCreate(&s) != hipSuccess || hipMalloc(&dA, h.size() * 8) != hipSuccess ||
-        hipMalloc(&dL, (size_t)np * NB * 8) != hipSuccess || hipMalloc(&dflag, sizeof(int)) != hipSuccess) {
+    if (hipStreamCreate(&s) != hipSuccess || hipMalloc(&dA, h.size() * sizeof(T)) != hipSuccess ||
+        hipMalloc(&dL, (size_t)np * NB * sizeof(T)) != hipSuccess || hipMalloc(&dflag, sizeof(int)) != hipSuccess) {
         cleanup();
         msg = "device allocation failed";
         return kErrDevice;
     }
     int zero = 0, flag = 0;
-    (void)hipMemcpyAsync(dA, h.data(), h.size() * 8, hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(dA, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s);
     (void)hipMemcpyAsync(dflag, &zero, sizeof(int), hipMemcpyHostToDevice, s);
     queue_cholesky(nullptr, s, np, dA, dL, dflag);
-    (void)hipMemcpyAsync(h.data(), dA, h.size() * 8, hipMemcpyDeviceToHost, s);
+    (void)hipMemcpyAsync(h.data(), dA, h.size() * sizeof(T), hipMemcpyDeviceToHost, s);
     (void)hipMemcpyAsync(&flag, dflag, sizeof(int), hipMemcpyDeviceToHost, s);
     if (hipStreamSynchronize(s) != hipSuccess) {
         msg = "kernel failure";
@@ -2372,9 +2909,16 @@ int device_dense_cholesky(int n, const double* a, double* l_out, std::string& ms
         st = kErrEEQ;
     }
     for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) l_out[(size_t)i * n + j] = (j <= i) ? h[(size_t)i * np + j] : 0.0;
+        for (int j = 0; j < n; ++j) l_out[(size_t)i * n + j] = (j <= i) ? h[(size_t)i * np + j] : T(0);
     cleanup();
     return st;
+}
+
+int device_dense_cholesky(int n, const double* a, double* l_out, std::string& msg) {
+    return dense_cholesky<double>(n, a, l_out, msg);
+}
+int device_dense_cholesky_f32(int n, const float* a, float* l_out, std::string& msg) {
+    return dense_cholesky<float>(n, a, l_out, msg);
 }
 
 }  // namespace gfnff

@@ -12,7 +12,7 @@ namespace gfnff {
 struct DeviceEngine;
 
 // Uploads the parameter set; returns nullptr and sets status/msg on failure.
-DeviceEngine* device_create(const ParamSet& P, int precision, int& status, std::string& msg);
+DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, int& status, std::string& msg);
 void device_destroy(DeviceEngine* e);
 
 // Host-buffer evaluation: uploads xyz (Bohr), evaluates, downloads
@@ -30,12 +30,19 @@ int device_count();
 // counts[n], idx[n * maxnb] (-1 past the end of a list).
 int device_cn_neighbours(DeviceEngine* e, int* counts, int* idx, int maxnb, std::string& msg);
 
+// Re-uploads the scaled repulsion / dispersion parameters of P.
+int device_update_params(DeviceEngine* e, const ParamSet& P, std::string& msg);
+// Parameter adjoint at the last evaluation: out[p * nelem + e] = sum_i w_i .
+// dg_i / dln(theta_pe) for p = repz, repulsion exponent, sqrt(Z r4r2).
+int device_param_adjoint(DeviceEngine* e, const double* w, double* out, std::string& msg);
+
 // HIP-event timing of one kernel class on the evaluation stream.
 int device_profile_select(DeviceEngine* e, const char* name, std::string& msg);
 int device_profile_read(DeviceEngine* e, double* total_ms, int* launches, std::string& msg);
 
 // Dense SPD factorisation with the EEQ Cholesky kernels (host buffers).
 int device_dense_cholesky(int n, const double* a, double* l_out, std::string& msg);
+int device_dense_cholesky_f32(int n, const float* a, float* l_out, std::string& msg);
 
 }  // namespace gfnff
 

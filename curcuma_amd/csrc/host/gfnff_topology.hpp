@@ -28,6 +28,7 @@ struct Options {
     double cn_cutoff_bohr = 6.0;    // per-evaluation CN neighbour cutoff
     double hb_accuracy = 0.1;       // HB list thresholds hbthr1/hbthr2
     int threads = 1;                // host threads for the Phase-1 solve
+    bool cholesky_lookahead = false;  // overlap trailing updates with the next super-panel (side stream)
 };
 
 // Error codes shared with the C-ABI.

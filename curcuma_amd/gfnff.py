@@ -143,6 +143,29 @@ class GFNFFMethod:
         self._lib.gfnff_get_topology(self._h, None, None, iptr(bonds), nb.value, C.byref(nb))
         return hyb, pi, bonds
 
+    def paramCount(self) -> int:
+        """Number of fitted per-element parameter scales (3 * number of elements)."""
+        return int(self._lib.gfnff_param_count(self._h))
+
+    def setParamScales(self, theta) -> None:
+        """Per-element scales of repz, the repulsion exponent and sqrt(Z r4r2)
+        (see include/gfnff_mi355x.h); all 1 after setMolecule."""
+        t = np.ascontiguousarray(theta, dtype=np.float64)
+        st = self._lib.gfnff_param_scales(self._h, fptr(t), len(t))
+        if st != 0:
+            raise RuntimeError(self.getErrorMessage())
+
+    def paramAdjoint(self, w) -> np.ndarray:
+        """sum_i w_i . d g_i / d theta at the last evaluation (w: natoms x 3)."""
+        wv = np.ascontiguousarray(w, dtype=np.float64).reshape(-1)
+        if wv.size != 3 * self._n:
+            raise ValueError("w must have natoms x 3 entries")
+        out = np.zeros(self.paramCount())
+        st = self._lib.gfnff_param_adjoint(self._h, fptr(wv), fptr(out), len(out))
+        if st != 0:
+            raise RuntimeError(self.getErrorMessage())
+        return out
+
     @property
     def handle(self):
         return self._h

@@ -76,7 +76,9 @@ int gfnff_charges(gfnff_handle* h, double* q);
 int gfnff_cn(gfnff_handle* h, double* cn);
 
 /* ComputationalMethod::setParameters(json) for the parity-relevant keys
- * (gfnff.h:219-293): "cn_cutoff_bohr", "hb_accuracy", "threads". */
+ * (gfnff.h:219-293): "cn_cutoff_bohr", "hb_accuracy", "threads"; engine
+ * tuning: "cholesky_lookahead" (0/1, overlap the trailing update with the
+ * next super-panel on a second stream; off by default). */
 int gfnff_set_param(gfnff_handle* h, const char* key, double value);
 
 /* GFNFF_PREC_FP64 (default, the reference's arithmetic) or GFNFF_PREC_MIXED:
@@ -103,6 +105,24 @@ int gfnff_get_topology(const gfnff_handle* h, int32_t* hyb, int32_t* pi, int32_t
 /* Number of visible GPUs (0 when none). */
 int gfnff_device_count(void);
 
+/* Parameter adjoint for force-matching fits (no counterpart in the
+ * reference's interface; used by a fitting harness).  The fitted parameters
+ * are per-element scale factors theta (3 * nelem, elements in the order of
+ * gfnff_params_view.present_Z):
+ *   theta[e]             repulsion charge repz (bonded and non-bonded,
+ *                        ff_workspace_gfnff.cpp:774-842)
+ *   theta[nelem + e]     repulsion exponent (non-bonded rep_anb, bonded repa)
+ *   theta[2 nelem + e]   D4 sqrt(Z r4r2) (ff_workspace_gfnff.cpp:715-767)
+ * gfnff_param_count returns 3 * nelem (-1 without a molecule);
+ * gfnff_param_scales sets theta (all 1 after gfnff_set_molecule);
+ * gfnff_param_adjoint returns dtheta[k] = sum_i w_i . d g_i / d theta_k at the
+ * last evaluation, for a weight field w (natoms * 3), e.g. w = 2 (g - g_ref)
+ * for the loss sum |g - g_ref|^2 = sum |F - F_ref|^2.  Built for up to 4
+ * elements (GFNFF_STATUS_UNSUPPORTED beyond). */
+int gfnff_param_count(const gfnff_handle* h);
+int gfnff_param_scales(gfnff_handle* h, const double* theta, int32_t ntheta);
+int gfnff_param_adjoint(gfnff_handle* h, const double* w, double* dtheta, int32_t ntheta);
+
 /* Engine extension: the CN neighbour lists of the last evaluation, as the
  * reference's pair builder #1 defines them (CNCalculator::
  * calculateGFNFFCNWithNeighbors, cn_calculator.cpp:367-428): every pair with
@@ -128,6 +148,9 @@ int gfnff_profile_read(gfnff_handle* h, double* total_ms, int32_t* launches);
  * factorisation kernels; l_out receives the lower factor.  Used to test the
  * EEQ factorisation in isolation. */
 int gfnff_dense_cholesky(int32_t n, const double* a, double* l_out);
+
+/* The same with the FP32 kernels of the mixed-precision path (FP32 MFMA). */
+int gfnff_dense_cholesky_f32(int32_t n, const float* a, float* l_out);
 
 #ifdef __cplusplus
 }

@@ -138,3 +138,19 @@ def test_cn_neighbour_lists_bit_exact(case):
         assert got == ref[i], (i, got, ref[i])
         total += len(got)
     print(f"{case}: {n} atoms, {total // 2} pairs, lists identical")
+
+
+def test_cholesky_lookahead_is_bitwise_identical():
+    """The lookahead option only reorders independent tile updates across two
+    streams; every tile sees the same operands, so results are bitwise equal."""
+    Z, xyz = water_cluster(512, seed=3)
+    out = []
+    for la in (0, 1):
+        m = GFNFFMethod(parameters={"cholesky_lookahead": la})
+        assert m.setMolecule(Mol(Z, xyz)), m.getErrorMessage()
+        e = m.calculateEnergy(True)
+        assert not m.hasError(), m.getErrorMessage()
+        out.append((e, m.getGradient().copy(), m.getCharges().copy()))
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
