@@ -75,6 +75,14 @@ def cpu_baseline(sample: str):
                       f"{dt:.3f} s each"}
 
 
+WORKLOAD_TEXT = {
+    "water_3000": "water_3000: non-periodic water cluster, 3000 H2O (9000 atoms), 3.1 A cubic lattice, "
+                  "seeded random orientations, +-0.05 A jitter",
+    "alkane_batch_4096": "alkane_batch_4096: 4096 independent all-trans n-alkanes, atom counts uniform in "
+                         "[20, 400] rounded to CnH(2n+2), sigma 0.03 A, seed 3, one launch per kernel",
+}
+
+
 def pmc_traffic(kernel: str, workload: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary (profiles/pmc_<kernel>_<workload>.md, line 'traffic_bytes_per_launch: X'),
@@ -95,9 +103,11 @@ def main():
               "the engine has no multi-GPU path in this build", file=sys.stderr)
         sys.exit(2)
     import torch
-    from curcuma_amd import GFNFFMethod, Mol
+    from curcuma_amd import GFNFFBatch, GFNFFMethod, Mol
     from curcuma_amd._lib import NTERMS
-    from curcuma_amd.workloads import WORKLOADS
+    from curcuma_amd.workloads import WORKLOADS, alkane_batch
+
+    BATCH_WORKLOADS = {"alkane_batch_4096": lambda: alkane_batch(4096, 20, 400, sigma=0.03, seed=3)}
 
     if args.copy_calibration_mib > 0:
         nel = args.copy_calibration_mib * (1 << 20) // 8
@@ -107,7 +117,12 @@ def main():
         torch.cuda.synchronize()
         del a, b
 
-    Z, xyz = WORKLOADS[args.workload]()
+    batch = args.workload in BATCH_WORKLOADS
+    if batch:
+        atom_off, Z, xyz = BATCH_WORKLOADS[args.workload]()
+    else:
+        Z, xyz = WORKLOADS[args.workload]()
+        atom_off = None
     n = len(Z)
     dev = torch.device("cuda:0")
     x_dev = torch.tensor(xyz * (1.0 / 0.529177210903), dtype=torch.float64, device=dev).contiguous()
@@ -115,10 +130,12 @@ def main():
     sp = C.c_void_p(stream.cuda_stream)
 
     def make(precision):
-        m = GFNFFMethod(precision=precision, parameters={"cholesky_lookahead": args.lookahead})
+        cls = GFNFFBatch if batch else GFNFFMethod
+        m = cls(precision=precision, parameters={"cholesky_lookahead": args.lookahead})
         t0 = time.perf_counter()
-        if not m.setMolecule(Mol(Z, xyz)):
-            print("setMolecule failed: " + m.getErrorMessage(), file=sys.stderr)
+        ok = m.setBatch(atom_off, Z, xyz) if batch else m.setMolecule(Mol(Z, xyz))
+        if not ok:
+            print("set-up failed: " + m.getErrorMessage(), file=sys.stderr)
             sys.exit(1)
         e_dev = torch.zeros(NTERMS + 1, dtype=torch.float64, device=dev)
         g_dev = torch.zeros((n, 3), dtype=torch.float64, device=dev)
@@ -180,6 +197,11 @@ def main():
     nr = pv.natoms - pv.nfrag
     info = {"xyz_bohr": xyz * (1.0 / 0.529177210903), "np_pad": max(64, -(-nr // 64) * 64), "nhb": pv.nhb,
             "nbond": pv.nbond}
+    if batch:
+        frag = np.ctypeslib.as_array(pv.frag, shape=(n,))
+        info["mol_off"] = atom_off
+        info["mol_nr"] = [int(atom_off[k + 1] - atom_off[k] - len(np.unique(frag[atom_off[k]:atom_off[k + 1]])))
+                          for k in range(len(atom_off) - 1)]
     roof = roofline(kernel, k_ms, k_launches, args.steps, info, traffic=pmc_traffic(kernel, args.workload),
                     precision=headline)
     roof["kernel_ms_per_eval"] = {c: round(v[0], 4) for c, v in sorted(per_class.items(), key=lambda t: -t[1][0])}
@@ -204,8 +226,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if headline == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: non-periodic water cluster, 3000 H2O (9000 atoms), 3.1 A cubic "
-                               "lattice, seeded random orientations, +-0.05 A jitter",
+        "config": {"workload": WORKLOAD_TEXT.get(args.workload, args.workload),
                    "atoms": n, "precision": headline, "setup_s": round(t_setup, 3), "energy_Eh": e_last},
         "fp64": {"value": value64, "ms_per_step": dt64 / args.steps * 1e3, "energy_Eh": e64,
                  "kernel_ms_per_eval": {c: round(v[0], 4) for c, v in

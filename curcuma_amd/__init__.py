@@ -1,4 +1,4 @@
 """MI355X-native GFN-FF energy and gradient engine (drop-in for ``-method gfnff``)."""
-from .gfnff import GFNFFMethod, Mol  # noqa: F401
+from .gfnff import GFNFFBatch, GFNFFMethod, Mol  # noqa: F401
 
-__all__ = ["GFNFFMethod", "Mol"]
+__all__ = ["GFNFFBatch", "GFNFFMethod", "Mol"]

@@ -59,6 +59,9 @@ def load_library(path: Path | None = None) -> C.CDLL:
     lib.gfnff_destroy.argtypes = [C.c_void_p]
     lib.gfnff_set_molecule.argtypes = [C.c_void_p, C.c_int32, _i32p, _f64p, C.c_int32, _i32p, C.c_int32]
     lib.gfnff_update_geometry.argtypes = [C.c_void_p, _f64p]
+    lib.gfnff_set_batch.argtypes = [C.c_void_p, C.c_int32, _i32p, _i32p, _f64p, _i32p]
+    lib.gfnff_batch_energies.argtypes = [C.c_void_p, _f64p]
+    lib.gfnff_batch_energies_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     lib.gfnff_energy_gradient.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p]
     lib.gfnff_energy_gradient_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.gfnff_charges.argtypes = [C.c_void_p, _f64p]

@@ -11,6 +11,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gfnff_mi355x.h"
@@ -85,6 +86,72 @@ int gfnff_set_molecule(gfnff_handle* h, int32_t natoms, const int32_t* Z, const 
     } catch (const std::exception& e) {
         return fail(h, GFNFF_STATUS_ARGS, std::string("gfnff_set_molecule: ") + e.what());
     }
+}
+
+int gfnff_set_batch(gfnff_handle* h, int32_t nmol, const int32_t* atom_off, const int32_t* Z, const double* xyz_ang,
+                    const int32_t* charges) {
+    if (!h) return GFNFF_STATUS_ARGS;
+    if (nmol < 1 || !atom_off || !Z || !xyz_ang || atom_off[0] != 0)
+        return fail(h, GFNFF_STATUS_ARGS, "gfnff_set_batch: invalid arguments");
+    for (int m = 0; m < nmol; ++m)
+        if (atom_off[m + 1] <= atom_off[m]) return fail(h, GFNFF_STATUS_ARGS, "gfnff_set_batch: empty molecule");
+    try {
+        h->failed = false;
+        h->err.clear();
+        h->have_molecule = false;
+        h->dev.reset();
+        const int n = atom_off[nmol];
+        h->xyz_bohr.resize((size_t)n * 3);
+        for (size_t i = 0; i < h->xyz_bohr.size(); ++i) h->xyz_bohr[i] = xyz_ang[i] * gfnff::kAngToBohrGeom;
+        std::vector<gfnff::ParamSet> mols(nmol);
+        std::vector<int> status(nmol, gfnff::kOk);
+        std::vector<std::string> msgs(nmol);
+        const int nthr = std::max(1, std::min<int>(nmol, (int)std::max(1u, std::thread::hardware_concurrency())));
+        std::vector<std::thread> pool;
+        for (int w = 0; w < nthr; ++w)
+            pool.emplace_back([&, w]() {
+                for (int m = w; m < nmol; m += nthr) {
+                    const int a0 = atom_off[m], na = atom_off[m + 1] - a0;
+                    status[m] = gfnff::build_params(na, Z + a0, h->xyz_bohr.data() + 3 * (size_t)a0,
+                                                    charges ? charges[m] : 0, nullptr, 0, h->opt, mols[m], msgs[m]);
+                }
+            });
+        for (auto& t : pool) t.join();
+        for (int m = 0; m < nmol; ++m)
+            if (status[m] != gfnff::kOk)
+                return fail(h, status[m], "gfnff_set_batch: molecule " + std::to_string(m) + ": " + msgs[m]);
+        std::string msg;
+        int st = gfnff::merge_batch(mols, h->params, msg);
+        if (st != gfnff::kOk) return fail(h, st, "gfnff_set_batch: " + msg);
+        h->have_molecule = true;
+        h->dev_geom_dirty = true;
+        h->last_q.assign(n, 0.0);
+        h->last_cn.assign(n, 0.0);
+        h->theta.assign(3 * h->params.nelem, 1.0);
+        h->base_repz = h->params.repz;
+        h->base_anb = h->params.rep_anb;
+        h->base_szr = h->params.sqrt_zr4r2;
+        h->base_brep = h->params.brep_par;
+        return GFNFF_STATUS_OK;
+    } catch (const std::bad_alloc&) {
+        return fail(h, GFNFF_STATUS_ARGS, "gfnff_set_batch: out of host memory");
+    } catch (const std::exception& e) {
+        return fail(h, GFNFF_STATUS_DEVICE, std::string("gfnff_set_batch: ") + e.what());
+    }
+}
+
+int gfnff_batch_energies(gfnff_handle* h, double* e_mol) {
+    if (!h || !e_mol || !h->have_molecule || !h->dev) return GFNFF_STATUS_ARGS;
+    std::string msg;
+    int st = gfnff::device_batch_energies(h->dev.get(), e_mol, false, nullptr, msg);
+    return st ? fail(h, st, "gfnff_batch_energies: " + msg) : GFNFF_STATUS_OK;
+}
+
+int gfnff_batch_energies_device(gfnff_handle* h, double* e_mol_dev, void* stream) {
+    if (!h || !e_mol_dev || !h->have_molecule || !h->dev) return GFNFF_STATUS_ARGS;
+    std::string msg;
+    int st = gfnff::device_batch_energies(h->dev.get(), e_mol_dev, true, stream, msg);
+    return st ? fail(h, st, "gfnff_batch_energies_device: " + msg) : GFNFF_STATUS_OK;
 }
 
 int gfnff_update_geometry(gfnff_handle* h, const double* xyz_ang) {

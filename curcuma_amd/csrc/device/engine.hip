@@ -84,7 +84,8 @@ __global__ __launch_bounds__(256) void k_cn(int n, const double* __restrict__ x,
                                             const double* __restrict__ cnf, double* __restrict__ cn,
                                             double* __restrict__ dlog, double* __restrict__ gw,
                                             double* __restrict__ dgw, double* __restrict__ rhs,
-                                            int* __restrict__ nbn, int* __restrict__ nbl, int* __restrict__ flags) {
+                                            int* __restrict__ nbn, int* __restrict__ nbl, int* __restrict__ flags,
+                                            const int* __restrict__ mol_of, const int* __restrict__ mol_off) {
     __shared__ double sx[256], sy[256], sz[256], sr[256];
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -93,22 +94,33 @@ __global__ __launch_bounds__(256) void k_cn(int n, const double* __restrict__ x,
     double ri = act ? rcov[i] : 0.0;
     double raw = 0.0;
     int cnt = 0;
-    for (int t0 = 0; t0 < n; t0 += 256) {
+    // partner range: all atoms, or in a batch the molecules of this block's atoms
+    int jlo = 0, jhi = n, mlo = 0, mhi = n;
+    if (mol_of) {
+        const int first = blockIdx.x * 4, last = min(n, first + 4) - 1;
+        jlo = mol_off[mol_of[first]];
+        jhi = mol_off[mol_of[last] + 1];
+        if (act) {
+            mlo = mol_off[mol_of[i]];
+            mhi = mol_off[mol_of[i] + 1];
+        }
+    }
+    for (int t0 = jlo; t0 < jhi; t0 += 256) {
         __syncthreads();
         int jl = t0 + threadIdx.x;
-        if (jl < n) {
+        if (jl < jhi) {
             sx[threadIdx.x] = x[3 * jl];
             sy[threadIdx.x] = x[3 * jl + 1];
             sz[threadIdx.x] = x[3 * jl + 2];
             sr[threadIdx.x] = rcov[jl];
         }
         __syncthreads();
-        const int m = min(256, n - t0);
+        const int m = min(256, jhi - t0);
         for (int s0 = 0; s0 < m; s0 += 64) {
             const int jj = s0 + lane, j = t0 + jj;
             bool hit = false;
             double contrib = 0.0;
-            if (act && jj < m && j != i) {
+            if (act && jj < m && j != i && j >= mlo && j < mhi) {
                 double dx = xi.x - sx[jj], dy = xi.y - sy[jj], dz = xi.z - sz[jj];
                 // unfused products and sums: the list membership test matches
                 // the reference's ((dx*dx + dy*dy) + dz*dz) < 36 bit for bit
@@ -219,11 +231,13 @@ __global__ __launch_bounds__(256) void k_eeq_reduced(int nr, int np, const int* 
 __global__ __launch_bounds__(256) void k_eeq_rhs_qp(int n, const double* __restrict__ x,
                                                     const double* __restrict__ alp, const double* __restrict__ diag,
                                                     const double* __restrict__ qp, const double* __restrict__ rhs,
-                                                    double* __restrict__ r) {
+                                                    double* __restrict__ r, const int* __restrict__ mol_of,
+                                                    const int* __restrict__ mol_off) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double s = 0.0;
-    for (int j = 0; j < n; ++j)
+    const int j0 = mol_of ? mol_off[mol_of[i]] : 0, j1 = mol_of ? mol_off[mol_of[i] + 1] : n;
+    for (int j = j0; j < j1; ++j)
         if (qp[j] != 0.0) s += eeq_a(i, j, x, alp, diag) * qp[j];
     r[i] = rhs[i] - s;
 }
@@ -233,12 +247,24 @@ __global__ __launch_bounds__(256) void k_eeq_rhs_reduce(int nr, int np, const in
                                                         double* __restrict__ b) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= np) return;
-    b[a] = (a < nr) ? r[ra[a]] - r[rl[a]] : 0.0;
+    b[a] = (a < nr && ra[a] >= 0) ? r[ra[a]] - r[rl[a]] : 0.0;  // ra < 0: padding of a batched block
 }
 
 // ---------------------------------------------------------------------------
 // K3: blocked Cholesky, right-looking, lower, row-major with leading dim np.
 // ---------------------------------------------------------------------------
+// 1 / sqrt(d) from the hardware estimate refined by Newton steps
+// (v_rsq_f64 ~23 bits: two steps reach double precision; f32: one step)
+__device__ __forceinline__ double rsqrt_nr(double d) {
+    double y = __builtin_amdgcn_rsq(d);
+    y = y * (1.5 - 0.5 * d * y * y);
+    return y * (1.5 - 0.5 * d * y * y);
+}
+__device__ __forceinline__ float rsqrt_nr(float d) {
+    const float y = __builtin_amdgcn_rsqf(d);
+    return y * (1.5f - 0.5f * d * y * y);
+}
+
 // Factorisation L L^T of one NB x NB diagonal block and the inverse of L by
 // one workgroup of 256 threads.  Thread (t, q) = (tid / 4, tid % 4) keeps the
 // entries j = 4u + q of row t in registers.  At step k the owners of column k
@@ -249,12 +275,11 @@ __global__ __launch_bounds__(256) void k_eeq_rhs_reduce(int nr, int np, const in
 // shares are combined with quad shuffles (no barrier).  Both step loops are
 // unrolled so that every register index is static and the LDS reads of a step
 // issue back to back.
+// The block routine operates on caller-provided LDS so that one workgroup can
+// also factor a whole small matrix (batched path).
 template <typename T>
-__global__ __launch_bounds__(256) void k_chol_diag(int np, int k0, T* __restrict__ A, T* __restrict__ Linv,
-                                                   int* __restrict__ flags) {
-    __shared__ T sL[NB][NB + 1];
-    __shared__ T col[2][NB];
-    __shared__ T rdiag[NB];
+__device__ __forceinline__ void chol_diag_block(int np, int k0, T* __restrict__ A, T* __restrict__ Linv, int* __restrict__ flags,
+                                T (*sL)[NB + 1], T (*col)[NB], T* rdiag) {
     const int tid = threadIdx.x;
     const int t = tid >> 2, q = tid & 3;
     _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) sL[e / NB][e % NB] = A[(size_t)(k0 + e / NB) * np + k0 + e % NB];
@@ -273,13 +298,14 @@ __global__ __launch_bounds__(256) void k_chol_diag(int np, int k0, T* __restrict
             bad = 1;
             d = T(1);
         }
-        const T sd = sqrt(d);
+        const T isd = rsqrt_nr(d);  // 1 / sqrt(d) without a divide on the step's critical path
+        const T sd = d * isd;
         const T ct = col[buf][t];
-        const T f = ct / d;
+        const T f = ct * (isd * isd);
 #pragma unroll
         for (int u = k >> 2; u < NB / 4; ++u)
             if (4 * u + q > k) a[u] -= f * col[buf][4 * u + q];
-        if (q == (k & 3)) a[k >> 2] = (t == k) ? sd : (t > k ? ct / sd : a[k >> 2]);
+        if (q == (k & 3)) a[k >> 2] = (t == k) ? sd : (t > k ? ct * isd : a[k >> 2]);
     }
 #pragma unroll
     for (int u = 0; u < NB / 4; ++u) sL[t][4 * u + q] = (4 * u + q <= t) ? a[u] : T(0);
@@ -310,13 +336,21 @@ __global__ __launch_bounds__(256) void k_chol_diag(int np, int k0, T* __restrict
     if (bad && tid == 0) atomicOr(flags, FLAG_CHOL);
 }
 
-// panel: rows below the diagonal block, L(i, k-block) = A(i, k-block) * Linv^T
 template <typename T>
-__global__ __launch_bounds__(256) void k_chol_panel(int np, int k0, T* __restrict__ A, const T* __restrict__ Linv) {
-    __shared__ T S[NB][NB + 1];
-    __shared__ T W[NB][NB + 1];
+__global__ __launch_bounds__(256) void k_chol_diag(int np, int k0, T* __restrict__ A, T* __restrict__ Linv,
+                                                   int* __restrict__ flags) {
+    __shared__ T sL[NB][NB + 1];
+    __shared__ T col[2][NB];
+    __shared__ T rdiag[NB];
+    chol_diag_block<T>(np, k0, A, Linv, flags, sL, col, rdiag);
+}
+
+// panel: rows below the diagonal block, L(i, k-block) = A(i, k-block) * Linv^T
+// panel tile: rows r0 .. r0 + 63 of the column block k0, L = A Linv_k^T
+template <typename T>
+__device__ __forceinline__ void chol_panel_tile(int np, int k0, int r0, T* __restrict__ A, const T* __restrict__ Linv,
+                                T (*S)[NB + 1], T (*W)[NB + 1]) {
     int tid = threadIdx.x;
-    int r0 = k0 + NB + blockIdx.x * NB;
     const T* Li = Linv + (size_t)(k0 / NB) * NB * NB;
     _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) {
         int r = e / NB, c = e % NB;
@@ -343,6 +377,13 @@ __global__ __launch_bounds__(256) void k_chol_panel(int np, int k0, T* __restric
     for (int u = 0; u < 4; ++u)
 #pragma unroll
         for (int v = 0; v < 4; ++v) A[(size_t)(r0 + tr * 4 + u) * np + k0 + tc * 4 + v] = acc[u][v];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_chol_panel(int np, int k0, T* __restrict__ A, const T* __restrict__ Linv) {
+    __shared__ T S[NB][NB + 1];
+    __shared__ T W[NB][NB + 1];
+    chol_panel_tile<T>(np, k0, k0 + NB + blockIdx.x * NB, A, Linv, S, W);
 }
 
 // Update of a set of lower 64 x 64 tiles by a block of factored columns:
@@ -386,28 +427,13 @@ __device__ inline int xcd_tile(int b, int nb) {
     return x * per + min(x, rem) + l;
 }
 
+// A(ri.., rj..) -= sum_{c in [kc0, kc0 + kw)} L(ri.., c) L(rj.., c)^T for one
+// 64 x 64 tile (lower triangle only when ri == rj), FP64/FP32 MFMA
 template <typename T>
-__global__ __launch_bounds__(256) void k_chol_update(int np, int kc0, int kw, int bj0, int w, int nbk,
-                                                     T* __restrict__ A) {
+__device__ __forceinline__ void chol_update_tile(int np, int ri, int rj, int kc0, int kw, T* __restrict__ A, T (*P)[NB + 1],
+                                 T (*Q)[NB + 1]) {
     using M = MfmaT<T>;
-    __shared__ T P[NB][NB + 1];
-    __shared__ T Q[NB][NB + 1];
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int head = w * (w + 1) / 2;
-    int bi, bj;
-    if (t < head) {
-        int r = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-        while ((r + 1) * (r + 2) / 2 <= t) ++r;
-        while (r * (r + 1) / 2 > t) --r;
-        bi = bj0 + r;
-        bj = bj0 + (t - r * (r + 1) / 2);
-    } else {
-        const int t2 = t - head;
-        bi = bj0 + w + t2 / w;
-        bj = bj0 + t2 % w;
-    }
-    if (bi >= nbk) return;
-    const int ri = bi * NB, rj = bj * NB;
+    const int bi = ri, bj = rj;  // only compared for equality (diagonal tile)
     const int tid = threadIdx.x;
     const int wave = tid / 64, lane = tid % 64;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
@@ -473,6 +499,29 @@ __global__ __launch_bounds__(256) void k_chol_update(int np, int kc0, int kw, in
                 const int c = rj + wc + 16 * y + li;
                 if (bi != bj || c <= r) A[(size_t)r * np + c] = old[x][y][v] - acc[x][y][v];
             }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_chol_update(int np, int kc0, int kw, int bj0, int w, int nbk,
+                                                     T* __restrict__ A) {
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int head = w * (w + 1) / 2;
+    int bi, bj;
+    if (t < head) {
+        int r = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while ((r + 1) * (r + 2) / 2 <= t) ++r;
+        while (r * (r + 1) / 2 > t) --r;
+        bi = bj0 + r;
+        bj = bj0 + (t - r * (r + 1) / 2);
+    } else {
+        const int t2 = t - head;
+        bi = bj0 + w + t2 / w;
+        bj = bj0 + t2 % w;
+    }
+    if (bi >= nbk) return;
+    __shared__ T P[NB][NB + 1];
+    __shared__ T Q[NB][NB + 1];
+    chol_update_tile<T>(np, bi * NB, bj * NB, kc0, kw, A, P, Q);
 }
 
 // Blocked right-looking Cholesky of the padded np x np matrix in two levels:
@@ -589,7 +638,7 @@ __device__ inline bool wait_ready(const int* f, int* flags) {
     for (int it = 0; it < SPIN_MAX; ++it) {
         if (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0) return true;
         if (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_SOLVE) return false;
-        __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_s_sleep(1);
     }
     atomicOr(flags, FLAG_SOLVE);
     return false;
@@ -600,10 +649,15 @@ __global__ __launch_bounds__(256) void k_trsv_fwd_sweep(int np, const T* __restr
                                                         const T* __restrict__ b, T* __restrict__ y,
                                                         int* __restrict__ ready, int* __restrict__ flags) {
     __shared__ T Tt[NB][NB + 1];
+    __shared__ T Wi[NB][NB + 1];  // inverse of the diagonal block, fetched first
     __shared__ T yk[NB];
     __shared__ T bw[NB];
     const int w = blockIdx.x, tid = threadIdx.x, t = tid >> 2, q = tid & 3;
     T acc = (q == 0) ? b[w * NB + t] : T(0);  // thread (t, q): share of row t
+    {
+        const T* Li = Linv + (size_t)w * NB * NB;
+        _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) Wi[e / NB][e % NB] = Li[e];
+    }
     bool ok = true;
     for (int k = 0; k < w; ++k) {
         __syncthreads();
@@ -621,12 +675,10 @@ __global__ __launch_bounds__(256) void k_trsv_fwd_sweep(int np, const T* __restr
     acc += __shfl_xor(acc, 2);
     __syncthreads();
     if (q == 0) bw[t] = acc;
-    const T* Li = Linv + (size_t)w * NB * NB;
-    _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) Tt[e / NB][e % NB] = Li[e];
     __syncthreads();
     T s = T(0);
 #pragma unroll
-    for (int u = 0; u < NB / 4; ++u) s += Tt[t][4 * u + q] * bw[4 * u + q];
+    for (int u = 0; u < NB / 4; ++u) s += Wi[t][4 * u + q] * bw[4 * u + q];
     s += __shfl_xor(s, 1);
     s += __shfl_xor(s, 2);
     if (q == 0) {
@@ -645,11 +697,16 @@ __global__ __launch_bounds__(256) void k_trsv_bwd_sweep(int np, const T* __restr
                                                         const T* __restrict__ c, T* __restrict__ x,
                                                         int* __restrict__ ready, int* __restrict__ flags) {
     __shared__ T Tt[NB][NB + 1];
+    __shared__ T Wi[NB][NB + 1];  // inverse of the diagonal block, fetched first
     __shared__ T xk[NB];
     __shared__ T cw[NB];
     const int nbk = np / NB;
     const int w = nbk - 1 - (int)blockIdx.x, tid = threadIdx.x, t = tid >> 2, q = tid & 3;
     T acc = (q == 0) ? c[w * NB + t] : T(0);  // thread (t, q): share of column t
+    {
+        const T* Li = Linv + (size_t)w * NB * NB;
+        _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) Wi[e / NB][e % NB] = Li[e];
+    }
     bool ok = true;
     for (int k = nbk - 1; k > w; --k) {
         __syncthreads();
@@ -667,12 +724,10 @@ __global__ __launch_bounds__(256) void k_trsv_bwd_sweep(int np, const T* __restr
     acc += __shfl_xor(acc, 2);
     __syncthreads();
     if (q == 0) cw[t] = acc;
-    const T* Li = Linv + (size_t)w * NB * NB;
-    _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) Tt[e / NB][e % NB] = Li[e];
     __syncthreads();
     T s = T(0);
 #pragma unroll
-    for (int u = 0; u < NB / 4; ++u) s += Tt[4 * u + q][t] * cw[4 * u + q];
+    for (int u = 0; u < NB / 4; ++u) s += Wi[4 * u + q][t] * cw[4 * u + q];
     s += __shfl_xor(s, 1);
     s += __shfl_xor(s, 2);
     if (q == 0) {
@@ -827,6 +882,8 @@ struct PairArgs {
     const double* c6ref;
     const double* h;  // [n][nelem][7] D4 half contraction (k_d4_half)
     const double* dh;
+    const int* mol_of;   // batches: molecule of each atom (nullptr: one molecule)
+    const int* mol_off;  // batches: atom offsets of the molecules
     double* g;        // [3n]
     double* dEdcn;    // [n]
     double* e_coul;   // [n]
@@ -840,6 +897,20 @@ __device__ inline double rep_ff(int zi, int zj, int mi, int mj) {
     if ((zi == 1 && zj == 6) || (zj == 1 && zi == 6)) return 0.91;
     if ((zi == 1 && zj == 8) || (zj == 1 && zi == 8)) return 1.04;
     return 1.0;
+}
+
+// Partner range of a pair-kernel workgroup: the j chunk of a single molecule,
+// or, in a batch, the union of the molecules of the workgroup's rows (each
+// row then keeps only partners of its own molecule).
+__device__ inline void pair_range(const PairArgs& a, int cj, int& j0, int& j1) {
+    if (!a.mol_of) {
+        j0 = blockIdx.y * cj;
+        j1 = min(a.n, j0 + cj);
+    } else {
+        const int first = blockIdx.x * TILE, last = min(a.n, first + TILE) - 1;
+        j0 = a.mol_off[a.mol_of[first]];
+        j1 = a.mol_off[a.mol_of[last] + 1];
+    }
 }
 
 // D4 half contraction per atom i and partner element e:
@@ -890,7 +961,10 @@ __global__ __launch_bounds__(TILE) void k_pairs_part(PairArgs a, int cj, double*
     const int n = a.n, ne = a.nelem;
     const int i = blockIdx.x * TILE + threadIdx.x;
     const bool act = i < n;
-    const int j0 = blockIdx.y * cj, j1 = min(n, j0 + cj);
+    int j0, j1;
+    pair_range(a, cj, j0, j1);
+    const int mlo = (a.mol_of && act) ? a.mol_off[a.mol_of[i]] : 0;
+    const int mhi = (a.mol_of && act) ? a.mol_off[a.mol_of[i] + 1] : n;
     double3 xi = act ? ld3(a.x, i) : make_double3(0, 0, 0);
     double qi = act ? a.q[i] : 0.0, ali = act ? a.alp[i] : 1.0, zti = act ? a.zeta[i] : 0.0;
     double szi = act ? a.szr[i] : 0.0, ani = act ? a.anb[i] : 0.0, rzi = act ? a.repz[i] : 0.0;
@@ -928,7 +1002,7 @@ __global__ __launch_bounds__(TILE) void k_pairs_part(PairArgs a, int cj, double*
         if (!act) continue;
         const int m = min(TILE, j1 - t0);
         for (int jj = 0; jj < m; ++jj) {
-            if (t0 + jj == i) continue;
+            if (t0 + jj == i || t0 + jj < mlo || t0 + jj >= mhi) continue;
             double dx = xi.x - sx[jj], dy = xi.y - sy[jj], dz = xi.z - sz[jj];
             double r2 = dx * dx + dy * dy + dz * dz;
             if (r2 > 10000.0) continue;  // beyond every pair cut-off (Coulomb 100 Bohr)
@@ -1020,7 +1094,10 @@ __global__ __launch_bounds__(TILE) void k_pairs_part32(PairArgs a, int cj, doubl
     const int n = a.n, ne = a.nelem;
     const int i = blockIdx.x * TILE + threadIdx.x;
     const bool act = i < n;
-    const int j0 = blockIdx.y * cj, j1 = min(n, j0 + cj);
+    int j0, j1;
+    pair_range(a, cj, j0, j1);
+    const int mlo = (a.mol_of && act) ? a.mol_off[a.mol_of[i]] : 0;
+    const int mhi = (a.mol_of && act) ? a.mol_off[a.mol_of[i] + 1] : n;
     const double3 xi = act ? ld3(a.x, i) : make_double3(0, 0, 0);
     const float qi = act ? (float)a.q[i] : 0.f, ali = act ? (float)a.alp[i] : 1.f;
     const float zti = act ? (float)a.zeta[i] : 0.f, szi = act ? (float)a.szr[i] : 0.f;
@@ -1059,7 +1136,7 @@ __global__ __launch_bounds__(TILE) void k_pairs_part32(PairArgs a, int cj, doubl
         if (!act) continue;
         const int m = min(TILE, j1 - t0);
         for (int jj = 0; jj < m; ++jj) {
-            if (t0 + jj == i) continue;
+            if (t0 + jj == i || t0 + jj < mlo || t0 + jj >= mhi) continue;
             const double dxd = xi.x - sx[jj], dyd = xi.y - sy[jj], dzd = xi.z - sz[jj];
             const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
             if (r2d > 10000.0) continue;  // beyond every pair cut-off (Coulomb 100 Bohr)
@@ -1670,6 +1747,146 @@ __global__ __launch_bounds__(256) void k_batm(int nt, const int* __restrict__ ij
 }
 
 // ---------------------------------------------------------------------------
+// K11: batches of independent molecules (one launch per kernel for the whole
+// batch).  Molecule m owns atoms [mol_off[m], mol_off[m+1]); every all-pairs
+// loop is restricted to the molecule, the bonded lists are concatenated.  Its
+// reduced EEQ system occupies the padded block [vo_m, vo_m + np_m) of the
+// concatenated reduced index space, its matrix lies at A + mo_m (np_m x np_m,
+// row-major) and the inverses of its diagonal blocks at Linv + vo_m NB.
+// ---------------------------------------------------------------------------
+constexpr int BATCH_MAX_NP = 512;  // largest padded reduced system of a batched molecule
+
+// one 64 x 64 lower tile (m, bi, bj) of a molecule's reduced matrix per workgroup
+__global__ __launch_bounds__(256) void k_eeq_reduced_batch(const int4* __restrict__ tiles, const int* __restrict__ mnr,
+                                                           const int* __restrict__ mnp,
+                                                           const long long* __restrict__ mo,
+                                                           const int* __restrict__ vo, const int* __restrict__ ra,
+                                                           const int* __restrict__ rl, const double* __restrict__ x,
+                                                           const double* __restrict__ alp,
+                                                           const double* __restrict__ diag, double* __restrict__ A) {
+    const int4 tt = tiles[blockIdx.x];
+    const int m = tt.x, bi = tt.y, bj = tt.z;
+    const int nr = mnr[m], np = mnp[m], base = vo[m];
+    double* Am = A + mo[m];
+    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
+        const int r = bi * NB + e / NB, c = bj * NB + e % NB;
+        if (c > r) continue;
+        double v;
+        if (r >= nr || c >= nr) {
+            v = (r == c) ? 1.0 : 0.0;
+        } else {
+            const int ia = ra[base + r], la = rl[base + r], ib = ra[base + c], lb = rl[base + c];
+            v = eeq_a(ia, ib, x, alp, diag) - eeq_a(ia, lb, x, alp, diag) - eeq_a(la, ib, x, alp, diag) +
+                eeq_a(la, lb, x, alp, diag);
+        }
+        Am[(size_t)r * np + c] = v;
+    }
+}
+
+// Factorisation and solve of one molecule's reduced system per workgroup:
+// the blocked right-looking Cholesky (diag / panel / MFMA tile update, the
+// routines of the single-molecule path) and both substitutions, all inside
+// the workgroup.  b, x: concatenated padded vectors.
+__global__ __launch_bounds__(256) void k_eeq_batch_solve(const int* __restrict__ mnp, const long long* __restrict__ mo,
+                                                         const int* __restrict__ vo, double* __restrict__ A,
+                                                         double* __restrict__ Linv, const double* __restrict__ b,
+                                                         double* __restrict__ x, int* __restrict__ flags) {
+    __shared__ double S0[NB][NB + 1];
+    __shared__ double S1[NB][NB + 1];
+    __shared__ double col[2][NB];
+    __shared__ double rdiag[NB];
+    __shared__ double vec[BATCH_MAX_NP];
+    __shared__ double tmp[NB];
+    const int m = blockIdx.x, np = mnp[m], nb = np / NB, base = vo[m];
+    double* Am = A + mo[m];
+    double* Lm = Linv + (size_t)base * NB;
+    const int tid = threadIdx.x, t = tid >> 2, q = tid & 3;
+    for (int kb = 0; kb < nb; ++kb) {
+        chol_diag_block<double>(np, kb * NB, Am, Lm, flags, S0, col, rdiag);
+        __syncthreads();
+        for (int bi = kb + 1; bi < nb; ++bi) {
+            chol_panel_tile<double>(np, kb * NB, bi * NB, Am, Lm, S0, S1);
+            __syncthreads();
+        }
+        for (int bi = kb + 1; bi < nb; ++bi)
+            for (int bj = kb + 1; bj <= bi; ++bj) {
+                chol_update_tile<double>(np, bi * NB, bj * NB, kb * NB, NB, Am, S0, S1);
+                __syncthreads();
+            }
+    }
+    for (int r = tid; r < np; r += 256) vec[r] = b[base + r];
+    __syncthreads();
+    // forward: y_kb = Linv_kb (b_kb - sum_{k < kb} L(kb, k) y_k), y kept in vec
+    for (int kb = 0; kb < nb; ++kb) {
+        double acc = (q == 0) ? vec[kb * NB + t] : 0.0;
+        for (int k = 0; k < kb; ++k)
+#pragma unroll
+            for (int u = 0; u < NB / 4; ++u)
+                acc -= Am[(size_t)(kb * NB + t) * np + k * NB + 4 * u + q] * vec[k * NB + 4 * u + q];
+        acc += __shfl_xor(acc, 1);
+        acc += __shfl_xor(acc, 2);
+        if (q == 0) tmp[t] = acc;
+        __syncthreads();
+        const double* Li = Lm + (size_t)kb * NB * NB;
+        double s = 0.0;
+#pragma unroll
+        for (int u = 0; u < NB / 4; ++u) s += Li[t * NB + 4 * u + q] * tmp[4 * u + q];
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        if (q == 0) vec[kb * NB + t] = s;
+        __syncthreads();
+    }
+    // backward: x_kb = Linv_kb^T (y_kb - sum_{k > kb} L(k, kb)^T x_k)
+    for (int kb = nb - 1; kb >= 0; --kb) {
+        double acc = (q == 0) ? vec[kb * NB + t] : 0.0;
+        for (int k = nb - 1; k > kb; --k)
+#pragma unroll
+            for (int u = 0; u < NB / 4; ++u)
+                acc -= Am[(size_t)(k * NB + 4 * u + q) * np + kb * NB + t] * vec[k * NB + 4 * u + q];
+        acc += __shfl_xor(acc, 1);
+        acc += __shfl_xor(acc, 2);
+        if (q == 0) tmp[t] = acc;
+        __syncthreads();
+        const double* Li = Lm + (size_t)kb * NB * NB;
+        double s = 0.0;
+#pragma unroll
+        for (int u = 0; u < NB / 4; ++u) s += Li[(4 * u + q) * NB + t] * tmp[4 * u + q];
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        if (q == 0) vec[kb * NB + t] = s;
+        __syncthreads();
+    }
+    for (int r = tid; r < np; r += 256) x[base + r] = vec[r];
+}
+
+// Per-molecule total energy: one wavefront per molecule sums the per-atom
+// terms over its atoms and the per-entry terms over its list segments
+// (seg[t * (nmol + 1) + m]) in a fixed order.
+struct MolEnergyArgs {
+    int nmol;
+    const int* mol_off;
+    const int* seg;        // [7][nmol + 1]: bond, angle, torsion, brep, hb, atm, batm
+    const double* atom_terms[4];   // coul, disp, rep, self
+    const double* list_terms[7];
+    double* e_mol;
+};
+__global__ __launch_bounds__(256) void k_mol_energy(MolEnergyArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int m = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (m >= a.nmol) return;
+    double acc = 0.0;
+    for (int k = 0; k < 4; ++k)
+        for (int i = a.mol_off[m] + lane; i < a.mol_off[m + 1]; i += 64) acc += a.atom_terms[k][i];
+    for (int k = 0; k < 7; ++k) {
+        const int* sg = a.seg + k * (a.nmol + 1);
+        for (int i = sg[m] + lane; i < sg[m + 1]; i += 64) acc += a.list_terms[k][i];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) a.e_mol[m] = acc;
+}
+
+// ---------------------------------------------------------------------------
 // K10: parameter adjoint.  For per-element scale factors theta of the
 // repulsion charge (repz), the repulsion exponent (rep_anb / repa) and the D4
 // sqrt(Z r4r2), and a weight field w (w_i = dLoss/dg_i of a loss on the
@@ -1744,7 +1961,10 @@ __global__ __launch_bounds__(TILE) void k_adj_pairs(PairArgs a, const double* __
     const int n = a.n;
     const int i = blockIdx.x * TILE + threadIdx.x;
     const bool act = i < n;
-    const int j0 = blockIdx.y * cj, j1 = min(n, j0 + cj);
+    int j0, j1;
+    pair_range(a, cj, j0, j1);
+    const int mlo = (a.mol_of && act) ? a.mol_off[a.mol_of[i]] : 0;
+    const int mhi = (a.mol_of && act) ? a.mol_off[a.mol_of[i] + 1] : n;
     const double3 xi = act ? ld3(a.x, i) : make_double3(0, 0, 0);
     const double3 wi = act ? ld3(w, i) : make_double3(0, 0, 0);
     const double ui = act ? u[i] : 0.0, zti = act ? a.zeta[i] : 0.0, szi = act ? a.szr[i] : 0.0;
@@ -1783,7 +2003,7 @@ __global__ __launch_bounds__(TILE) void k_adj_pairs(PairArgs a, const double* __
         if (!act) continue;
         const int m = min(TILE, j1 - t0);
         for (int jj = 0; jj < m; ++jj) {
-            if (t0 + jj == i) continue;
+            if (t0 + jj == i || t0 + jj < mlo || t0 + jj >= mhi) continue;
             const double dx = xi.x - sx[jj], dy = xi.y - sy[jj], dz = xi.z - sz[jj];
             const double r2 = dx * dx + dy * dy + dz * dz;
             if (r2 > 2500.0) continue;  // beyond the D4 and repulsion cut-offs
@@ -2115,6 +2335,13 @@ struct DeviceEngine {
     Lookahead la;                       // side stream of the factorisation
     bool use_lookahead = false;
     int* ready = nullptr;               // per-block ready flags of the solve sweeps
+    // batches of molecules (nmol > 1)
+    int nmol = 1, nbtiles = 0;
+    size_t mat_size = 0;                // doubles of the reduced EEQ matrix (or matrices)
+    int *mol_of = nullptr, *mol_off = nullptr, *mol_seg = nullptr, *mnr = nullptr, *mnp = nullptr, *mvo = nullptr;
+    long long* mmo = nullptr;
+    int4* btiles = nullptr;
+    double* e_mol = nullptr;
     double *adj_w = nullptr, *adj_u = nullptr, *adj_part = nullptr, *adj_bpart = nullptr, *adj_out = nullptr;
     float *Ared32 = nullptr, *Linv32 = nullptr, *b32 = nullptr, *y32 = nullptr, *d32 = nullptr;
     double *rvec = nullptr, *symv = nullptr;
@@ -2168,6 +2395,8 @@ struct DeviceEngine {
         a.c6ref = c6ref;
         a.h = h6;
         a.dh = dh6;
+        a.mol_of = mol_of;
+        a.mol_off = mol_off;
         a.g = gpair;
         a.dEdcn = dEdcn;
         a.e_coul = e_coul;
@@ -2287,6 +2516,61 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         }
         E->nr = (int)red_atom.size();
         E->np = std::max(NB, cdiv(std::max(E->nr, 1), NB) * NB);
+        E->mat_size = (size_t)E->np * E->np;
+        if (P.nmol > 1) {
+            // batch: each molecule's fragments form their own padded block
+            E->nmol = P.nmol;
+            std::vector<int> ra2, rl2, mnr(P.nmol), mnp(P.nmol), vo(P.nmol);
+            std::vector<long long> mo(P.nmol);
+            std::vector<int> fr_first(P.nmol + 1, 0);
+            for (int f = 0; f < P.nfrag; ++f) fr_first[P.mol_of_atom(fa[f][0]) + 1] = f + 1;
+            for (int m = 0; m < P.nmol; ++m) fr_first[m + 1] = std::max(fr_first[m + 1], fr_first[m]);
+            long long off = 0;
+            for (int m = 0; m < P.nmol; ++m) {
+                vo[m] = (int)ra2.size();
+                int loc = 0;
+                for (int f = fr_first[m]; f < fr_first[m + 1]; ++f) {
+                    const int cnt = (int)fa[f].size(), last = fa[f].back();
+                    for (int k = 0; k + 1 < cnt; ++k) {
+                        red_of[fa[f][k]] = (int)ra2.size();
+                        ra2.push_back(fa[f][k]);
+                        rl2.push_back(last);
+                        ++loc;
+                    }
+                }
+                mnr[m] = loc;
+                mnp[m] = std::max(NB, cdiv(std::max(loc, 1), NB) * NB);
+                if (mnp[m] > BATCH_MAX_NP) throw std::runtime_error("batched molecule too large");
+                while ((int)ra2.size() < vo[m] + mnp[m]) {
+                    ra2.push_back(-1);
+                    rl2.push_back(-1);
+                }
+                mo[m] = off;
+                off += (long long)mnp[m] * mnp[m];
+            }
+            red_atom.swap(ra2);
+            red_last.swap(rl2);
+            E->nr = (int)red_atom.size();
+            E->np = E->nr;
+            E->mat_size = (size_t)off;
+            std::vector<int4> tiles;
+            for (int m = 0; m < P.nmol; ++m)
+                for (int bi = 0; bi < mnp[m] / NB; ++bi)
+                    for (int bj = 0; bj <= bi; ++bj) tiles.push_back(make_int4(m, bi, bj, 0));
+            E->nbtiles = (int)tiles.size();
+            E->btiles = E->upload(tiles);
+            E->mnr = E->upload(mnr);
+            E->mnp = E->upload(mnp);
+            E->mvo = E->upload(vo);
+            E->mmo = E->upload(mo);
+            E->mol_off = E->upload(P.mol_off);
+            std::vector<int> mof(n);
+            for (int m = 0; m < P.nmol; ++m)
+                for (int i = P.mol_off[m]; i < P.mol_off[m + 1]; ++i) mof[i] = m;
+            E->mol_of = E->upload(mof);
+            E->mol_seg = E->upload(P.mol_seg);
+            E->e_mol = E->alloc<double>(P.nmol);
+        }
         E->red_atom = E->upload(red_atom);
         E->red_last = E->upload(red_last);
         E->red_of = E->upload(red_of);
@@ -2438,13 +2722,13 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->dgw = E->alloc<double>(7 * (size_t)n);
         E->rhs = E->alloc<double>(n);
         E->diag = E->alloc<double>(n);
-        E->Ared = E->alloc<double>((size_t)E->np * E->np);
+        E->Ared = E->alloc<double>(E->mat_size);
         E->Linv = E->alloc<double>((size_t)E->np * NB);
         E->b = E->alloc<double>(E->np);
         E->ybuf = E->alloc<double>(E->np);
         E->xbuf = E->alloc<double>(E->np);
         E->ready = E->alloc<int>(E->np / NB);
-        if (E->mixed) {
+        if (E->mixed && E->nmol == 1) {
             E->Ared32 = E->alloc<float>((size_t)E->np * E->np);
             E->Linv32 = E->alloc<float>((size_t)E->np * NB);
             E->b32 = E->alloc<float>(E->np);
@@ -2474,6 +2758,10 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
             int nch = std::min(cdiv(n, TILE), std::max(1, cdiv(4096, rowblk)));
             E->pair_cj = cdiv(cdiv(n, nch), TILE) * TILE;
             E->pair_nchunk = cdiv(n, E->pair_cj);
+            if (E->nmol > 1) {  // rows see only their molecule: one chunk
+                E->pair_cj = n;
+                E->pair_nchunk = 1;
+            }
         }
         E->ppart = E->alloc<double>((size_t)E->pair_nchunk * PART_K * n);
         E->e_bond = E->alloc<double>(std::max(E->nbond, 1));
@@ -2603,20 +2891,29 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     hipLaunchKernelGGL(k_copy, dim3(cdiv(3 * n, 256)), dim3(256), 0, s, 3 * n, xin, E->x);
     hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, s, 1, E->flags, 0);
     LAUNCH(KC_CN, k_cn, dim3(cdiv(n, 4)), dim3(256), s, n, E->x, E->rcov_cn, E->elem, E->nref, E->refcn,
-                       E->chi_base, E->cnf, E->cn, E->dlog, E->gw, E->dgw, E->rhs, E->nbn, E->nbl, E->flags);
+                       E->chi_base, E->cnf, E->cn, E->dlog, E->gw, E->dgw, E->rhs, E->nbn, E->nbl, E->flags, E->mol_of,
+           E->mol_off);
     // EEQ
     const int np = E->np, nbk = np / NB;
-    LAUNCH(KC_EEQ_BUILD, k_eeq_reduced, dim3(nbk * (nbk + 1) / 2), dim3(256), s, E->nr, np, E->red_atom,
-                       E->red_last, E->x, E->alp, E->diag, E->Ared);
+    if (E->nmol == 1)
+        LAUNCH(KC_EEQ_BUILD, k_eeq_reduced, dim3(nbk * (nbk + 1) / 2), dim3(256), s, E->nr, np, E->red_atom,
+               E->red_last, E->x, E->alp, E->diag, E->Ared);
+    else
+        LAUNCH(KC_EEQ_BUILD, k_eeq_reduced_batch, dim3(E->nbtiles), dim3(256), s, E->btiles, E->mnr, E->mnp, E->mmo,
+               E->mvo, E->red_atom, E->red_last, E->x, E->alp, E->diag, E->Ared);
     const double* rsrc = E->rhs;
     if (E->charged) {
         LAUNCH(KC_EEQ_BUILD, k_eeq_rhs_qp, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->alp, E->diag, E->qp,
-                           E->rhs, E->rres);
+                           E->rhs, E->rres, E->mol_of, E->mol_off);
         rsrc = E->rres;
     }
     LAUNCH(KC_EEQ_BUILD, k_eeq_rhs_reduce, dim3(cdiv(np, 256)), dim3(256), s, E->nr, np, E->red_atom, E->red_last,
                        rsrc, E->b);
-    if (!E->mixed) {
+    if (E->nmol > 1) {
+        // batch: every molecule's system factored and solved by one workgroup (FP64)
+        LAUNCH(KC_CHOL_DIAG, k_eeq_batch_solve, dim3(E->nmol), dim3(256), s, E->mnp, E->mmo, E->mvo, E->Ared,
+               E->Linv, E->b, E->xbuf, E->flags);
+    } else if (!E->mixed) {
         queue_cholesky(&E->timer, s, np, E->Ared, E->Linv, E->flags, E->use_lookahead ? &E->la : nullptr);
         queue_solve<double>(E, s, E->Ared, E->Linv, E->b, E->ybuf, E->xbuf);
     } else {
@@ -2707,6 +3004,20 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     LAUNCH(KC_REDUCE, k_reduce1, dim3(RED_BLOCKS, 12), dim3(256), s, sg, E->rpart);
     LAUNCH(KC_REDUCE, k_reduce2, dim3(12), dim3(64), s, sg, E->rpart, E->parts);
     LAUNCH(KC_REDUCE, k_total, dim3(1), dim3(64), s, E->parts, E->out);
+    if (E->nmol > 1) {
+        MolEnergyArgs ma;
+        ma.nmol = E->nmol;
+        ma.mol_off = E->mol_off;
+        ma.seg = E->mol_seg;
+        ma.atom_terms[0] = E->e_coul;
+        ma.atom_terms[1] = E->e_disp;
+        ma.atom_terms[2] = E->e_rep;
+        ma.atom_terms[3] = E->eself;
+        const double* lists[7] = {E->e_bond, E->e_ang, E->e_tor, E->e_brep, E->e_hb, E->e_atm, E->e_batm};
+        for (int k = 0; k < 7; ++k) ma.list_terms[k] = lists[k];
+        ma.e_mol = E->e_mol;
+        LAUNCH(KC_REDUCE, k_mol_energy, dim3(cdiv(E->nmol, 4)), dim3(256), s, ma);
+    }
     if (hipGetLastError() != hipSuccess) {
         msg = "kernel launch failed";
         return kErrDevice;
@@ -2766,6 +3077,22 @@ int device_cn_neighbours(DeviceEngine* E, int* counts, int* idx, int maxnb, std:
     for (int i = 0; i < n; ++i)
         for (int k = 0; k < maxnb; ++k)
             idx[(size_t)i * maxnb + k] = (k < counts[i] && k < MAXNB_CN) ? nbl[(size_t)i * MAXNB_CN + k] : -1;
+    return kOk;
+}
+
+int device_batch_energies(DeviceEngine* E, double* e_mol, bool device_dst, void* stream, std::string& msg) {
+    if (E->nmol < 2) {
+        msg = "not a batch";
+        return kErrArgs;
+    }
+    if (device_dst) {
+        hipLaunchKernelGGL(k_copy, dim3(cdiv(E->nmol, 256)), dim3(256), 0, (hipStream_t)stream, E->nmol, E->e_mol,
+                           e_mol);
+        HIP_OK(hipGetLastError());
+    } else {
+        HIP_OK(hipDeviceSynchronize());
+        HIP_OK(hipMemcpy(e_mol, E->e_mol, sizeof(double) * E->nmol, hipMemcpyDeviceToHost));
+    }
     return kOk;
 }
 

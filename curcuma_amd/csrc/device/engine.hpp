@@ -30,6 +30,9 @@ int device_count();
 // counts[n], idx[n * maxnb] (-1 past the end of a list).
 int device_cn_neighbours(DeviceEngine* e, int* counts, int* idx, int maxnb, std::string& msg);
 
+// Per-molecule energies of the last evaluation of a batch (host or device destination).
+int device_batch_energies(DeviceEngine* e, double* e_mol, bool device_dst, void* stream, std::string& msg);
+
 // Re-uploads the scaled repulsion / dispersion parameters of P.
 int device_update_params(DeviceEngine* e, const ParamSet& P, std::string& msg);
 // Parameter adjoint at the last evaluation: out[p * nelem + e] = sum_i w_i .

@@ -1533,6 +1533,150 @@ bool eeq_solve_host(int n, int nfrag, std::vector<double>& A, const std::vector<
 // ---------------------------------------------------------------------------
 // Parameter-set assembly: generateGFNFFParameterSet, gfnff_method.cpp:3557-3747
 // ---------------------------------------------------------------------------
+// D4 element tables for the elements in P.present_Z: reference counts, the
+// reference coordination numbers and the C6 reference grid (element-pair data
+// of the D4 model; d4param_generator.cpp:1079-1213).
+void set_element_tables(ParamSet& P) {
+    P.nelem = (int)P.present_Z.size();
+    const int ne = P.nelem;
+    P.nref.assign(ne, 0);
+    P.refcn.assign((size_t)ne * 7, 0.0);
+    P.c6ref.assign((size_t)ne * ne * 49, 0.0);
+    for (int e = 0; e < ne; ++e) {
+        const int z = P.present_Z[e];
+        P.nref[e] = std::min(d4_refn[z - 1], 7);
+        for (int r = 0; r < 7; ++r) P.refcn[e * 7 + r] = d4_refcn[(z - 1) * 7 + r];
+    }
+    for (int a = 0; a < ne; ++a)
+        for (int b = 0; b < ne; ++b)
+            for (int ra = 0; ra < P.nref[a]; ++ra)
+                for (int rb = 0; rb < P.nref[b]; ++rb) {
+                    const int za = P.present_Z[a] - 1, zb = P.present_Z[b] - 1;
+                    // the reference stores the lower triangle and mirrors it
+                    const double c6 = (za > zb || (za == zb && ra >= rb)) ? c6_reference(za, ra, zb, rb)
+                                                                          : c6_reference(zb, rb, za, ra);
+                    P.c6ref[(((size_t)a * ne + b) * 7 + ra) * 7 + rb] = c6;
+                }
+}
+
+int ParamSet::mol_of_atom(int i) const {
+    if (nmol <= 1) return 0;
+    return (int)(std::upper_bound(mol_off.begin(), mol_off.end(), i) - mol_off.begin()) - 1;
+}
+
+namespace {
+template <class T>
+void append(std::vector<T>& dst, const std::vector<T>& src) {
+    dst.insert(dst.end(), src.begin(), src.end());
+}
+void append_shift(std::vector<int32_t>& dst, const std::vector<int32_t>& src, int shift, bool keep_negative = false) {
+    for (int32_t v : src) dst.push_back((keep_negative && v < 0) ? v : v + shift);
+}
+void append_csr(std::vector<int32_t>& off, const std::vector<int32_t>& src_off, int base) {
+    for (size_t k = 1; k < src_off.size(); ++k) off.push_back(src_off[k] + base);
+}
+}  // namespace
+
+int merge_batch(const std::vector<ParamSet>& mols, ParamSet& B, std::string& msg) {
+    B = ParamSet();
+    const int nm = (int)mols.size();
+    if (nm < 1) {
+        msg = "empty batch";
+        return kErrArgs;
+    }
+    B.nmol = nm;
+    B.mol_off.assign(nm + 1, 0);
+    B.mol_seg.assign(7 * (size_t)(nm + 1), 0);
+    std::vector<int> seen(87, -1);
+    for (const auto& P : mols)
+        for (int z : P.present_Z)
+            if (seen[z] < 0) {
+                seen[z] = (int)B.present_Z.size();
+                B.present_Z.push_back(z);
+            }
+    set_element_tables(B);
+    B.topo_off.push_back(0);
+    B.hb_nbB_off.push_back(0);
+    B.hb_nbC_off.push_back(0);
+    int aoff = 0, foff = 0;
+    auto seg = [&](int k, int m) -> int32_t& { return B.mol_seg[(size_t)k * (nm + 1) + m]; };
+    for (int m = 0; m < nm; ++m) {
+        const ParamSet& P = mols[m];
+        if (!P.inv_par.empty()) {
+            msg = "batched molecules with inversion terms are not supported";
+            return kErrUnsupported;
+        }
+        B.mol_off[m] = aoff;
+        seg(0, m) = (int32_t)B.bond_hbH.size();
+        seg(1, m) = (int32_t)(B.angle_par.size() / GFNFF_ANGLE_NPAR);
+        seg(2, m) = (int32_t)(B.tors_par.size() / GFNFF_TORS_NPAR);
+        seg(3, m) = (int32_t)(B.brep_par.size() / GFNFF_REP_NPAR);
+        seg(4, m) = (int32_t)B.hb_case.size();
+        seg(5, m) = (int32_t)B.atm_scale.size();
+        seg(6, m) = (int32_t)(B.batm_par.size() / GFNFF_BATM_NPAR);
+        append(B.Z, P.Z);
+        for (int i = 0; i < P.n; ++i) B.elem.push_back(seen[P.Z[i]]);
+        append_shift(B.frag, P.frag, foff);
+        append(B.chi_base, P.chi_base);
+        append(B.gam_eff, P.gam_eff);
+        append(B.alpeeq, P.alpeeq);
+        append(B.cnf, P.cnf);
+        append(B.rcov_cn, P.rcov_cn);
+        append(B.rcov_d3, P.rcov_d3);
+        append(B.hb_rad, P.hb_rad);
+        append(B.metal, P.metal);
+        append(B.qa, P.qa);
+        append(B.zeta, P.zeta);
+        append(B.sqrt_zr4r2, P.sqrt_zr4r2);
+        append(B.rep_anb, P.rep_anb);
+        append(B.repz, P.repz);
+        append(B.qfrag, P.qfrag);
+        append(B.hyb, P.hyb);
+        append(B.pi_frag, P.pi_frag);
+        append(B.cn_topo, P.cn_topo);
+        append_shift(B.topo_bonds, P.topo_bonds, aoff);
+        append_csr(B.topo_off, P.topo_off, (int)B.topo_idx.size());
+        append_shift(B.topo_idx, P.topo_idx, aoff);
+        append(B.topo_dist, P.topo_dist);
+        append_shift(B.bond_ij, P.bond_ij, aoff);
+        append(B.bond_par, P.bond_par);
+        append_shift(B.bond_hbH, P.bond_hbH, aoff, true);
+        append_shift(B.hbcn_pair, P.hbcn_pair, aoff);
+        append_shift(B.angle_ijk, P.angle_ijk, aoff);
+        append(B.angle_par, P.angle_par);
+        append_shift(B.tors_ijkl, P.tors_ijkl, aoff);
+        append(B.tors_par, P.tors_par);
+        append_shift(B.brep_ij, P.brep_ij, aoff);
+        append(B.brep_par, P.brep_par);
+        append_shift(B.hb_ahb, P.hb_ahb, aoff);
+        append(B.hb_case, P.hb_case);
+        append(B.hb_par, P.hb_par);
+        append_csr(B.hb_nbB_off, P.hb_nbB_off, (int)B.hb_nbB_idx.size());
+        append_shift(B.hb_nbB_idx, P.hb_nbB_idx, aoff);
+        append_shift(B.hb_parentC, P.hb_parentC, aoff, true);
+        append_csr(B.hb_nbC_off, P.hb_nbC_off, (int)B.hb_nbC_idx.size());
+        append_shift(B.hb_nbC_idx, P.hb_nbC_idx, aoff);
+        append_shift(B.atm_ijk, P.atm_ijk, aoff);
+        append(B.atm_scale, P.atm_scale);
+        append_shift(B.batm_ijk, P.batm_ijk, aoff);
+        append(B.batm_par, P.batm_par);
+        aoff += P.n;
+        foff += P.nfrag;
+        B.charge += P.charge;
+    }
+    B.mol_off[nm] = aoff;
+    seg(0, nm) = (int32_t)B.bond_hbH.size();
+    seg(1, nm) = (int32_t)(B.angle_par.size() / GFNFF_ANGLE_NPAR);
+    seg(2, nm) = (int32_t)(B.tors_par.size() / GFNFF_TORS_NPAR);
+    seg(3, nm) = (int32_t)(B.brep_par.size() / GFNFF_REP_NPAR);
+    seg(4, nm) = (int32_t)B.hb_case.size();
+    seg(5, nm) = (int32_t)B.atm_scale.size();
+    seg(6, nm) = (int32_t)(B.batm_par.size() / GFNFF_BATM_NPAR);
+    B.n = aoff;
+    B.nfrag = foff;
+    return kOk;
+}
+
 int build_params(int n, const int32_t* Z, const double* xyz, int charge, const int32_t* forced, int nforced,
                  const Options& opt, ParamSet& P, std::string& msg) {
     if (n <= 0 || !Z || !xyz) {
@@ -1616,26 +1760,7 @@ int build_params(int n, const int32_t* Z, const double* xyz, int charge, const i
             }
             P.elem[i] = seen[Z[i]];
         }
-        P.nelem = (int)P.present_Z.size();
-        int ne = P.nelem;
-        P.nref.resize(ne);
-        P.refcn.assign((size_t)ne * 7, 0.0);
-        P.c6ref.assign((size_t)ne * ne * 49, 0.0);
-        for (int e = 0; e < ne; ++e) {
-            int z = P.present_Z[e];
-            P.nref[e] = std::min(d4_refn[z - 1], 7);
-            for (int r = 0; r < 7; ++r) P.refcn[e * 7 + r] = d4_refcn[(z - 1) * 7 + r];
-        }
-        for (int a = 0; a < ne; ++a)
-            for (int b = 0; b < ne; ++b)
-                for (int ra = 0; ra < P.nref[a]; ++ra)
-                    for (int rb = 0; rb < P.nref[b]; ++rb) {
-                        int za = P.present_Z[a] - 1, zb = P.present_Z[b] - 1;
-                        // the reference stores the lower triangle and mirrors it
-                        double c6 = (za > zb || (za == zb && ra >= rb)) ? c6_reference(za, ra, zb, rb)
-                                                                        : c6_reference(zb, rb, za, ra);
-                        P.c6ref[(((size_t)a * ne + b) * 7 + ra) * 7 + rb] = c6;
-                    }
+        set_element_tables(P);
     }
     P.chi_base.resize(n);
     P.gam_eff.resize(n);

@@ -71,8 +71,22 @@ struct ParamSet {
     std::vector<int32_t> hyb, pi_frag, topo_bonds;   // topo_bonds: [2*nb] detected bonds
     std::vector<double> cn_topo;
 
+    // batches of independent molecules (nmol > 1): atom offsets and, per term
+    // list (bond, angle, torsion, bonded repulsion, HB, ATM, BATM), the entry
+    // offsets of every molecule ([7][nmol + 1])
+    int nmol = 1;
+    std::vector<int32_t> mol_off, mol_seg;
+    int mol_of_atom(int i) const;
+
     gfnff_params_view view() const;
 };
+
+// D4 element tables (reference counts, reference CNs, C6 grid) for P.present_Z.
+void set_element_tables(ParamSet& P);
+
+// Concatenates independently generated molecules into one parameter set with
+// molecule offsets (the batched evaluation treats molecules as independent).
+int merge_batch(const std::vector<ParamSet>& mols, ParamSet& out, std::string& msg);
 
 // Builds the parameter set for one molecule.  xyz_bohr is N x 3 row-major.
 // forced_bonds (pairs, 0-based) replaces geometric bond detection when nonempty

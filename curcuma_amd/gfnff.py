@@ -173,3 +173,34 @@ class GFNFFMethod:
     @property
     def natoms(self) -> int:
         return self._n
+
+
+class GFNFFBatch(GFNFFMethod):
+    """Batch of independent molecules evaluated together (one launch per kernel
+    for the whole batch).  ``setBatch`` replaces ``setMolecule``;
+    ``calculateEnergy`` returns the sum over the batch and ``getEnergies`` the
+    per-molecule energies; ``getGradient`` covers all atoms."""
+
+    def setBatch(self, atom_off, Z, geometry_angstrom, charges=None) -> bool:
+        off = as_i32(atom_off)
+        z = as_i32(Z)
+        x = as_f64(geometry_angstrom).reshape(-1, 3)
+        ch = as_i32(charges) if charges is not None else None
+        self._nmol = len(off) - 1
+        self._off = off
+        st = self._lib.gfnff_set_batch(self._h, self._nmol, iptr(off), iptr(z), fptr(x),
+                                       iptr(ch) if ch is not None else None)
+        self._n = len(z) if st == 0 else 0
+        self._grad = np.zeros((self._n, 3))
+        return st == 0
+
+    def getEnergies(self) -> np.ndarray:
+        e = np.zeros(self._nmol)
+        st = self._lib.gfnff_batch_energies(self._h, fptr(e))
+        if st != 0:
+            raise RuntimeError(self.getErrorMessage())
+        return e
+
+    @property
+    def nmol(self) -> int:
+        return self._nmol

@@ -104,6 +104,17 @@ def work_per_eval(cls: str, info: dict, rc: dict) -> tuple[float, str]:
     info: {"xyz_bohr", "np_pad", "n_reduced", "nfrag", "nhb", "nbond", ...}.
     Returns (work, "flop" | "byte")."""
     n = len(info["xyz_bohr"])
+    if "mol_off" in info:  # batch of independent molecules
+        off = info["mol_off"]
+        if cls == "pairs":
+            tot = [0, 0, 0]
+            for k in range(len(off) - 1):
+                c = pair_counts(info["xyz_bohr"][off[k]:off[k + 1]])
+                tot = [a + b for a, b in zip(tot, c)]
+            nc, nd, nr = tot
+            return (nc * rc["COULOMB_PAIR_FLOPS"] + nd * rc["DISP_PAIR_FLOPS"] + nr * rc["REP_PAIR_FLOPS"]), "flop"
+        if cls == "chol_diag":  # k_eeq_batch_solve: factorisation + both substitutions per molecule
+            return sum(m ** 3 / rc["CHOL_DIV"] + rc["TRSV_FACTOR"] * m * m for m in info["mol_nr"]), "flop"
     if cls == "pairs":
         nc, nd, nr = info.get("pairs") or pair_counts(info["xyz_bohr"])
         return (nc * rc["COULOMB_PAIR_FLOPS"] + nd * rc["DISP_PAIR_FLOPS"] + nr * rc["REP_PAIR_FLOPS"]), "flop"

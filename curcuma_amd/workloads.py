@@ -103,6 +103,22 @@ def alkane_bundle(nchains: int = 80, nc: int = 80, a: float = 4.5, sigma: float 
     return np.concatenate(Z).astype(np.int32), xyz
 
 
+def alkane_batch(nmol: int = 4096, nmin: int = 20, nmax: int = 400, sigma: float = 0.03, seed: int = 3):
+    """Batch of independent all-trans n-alkanes: atom counts drawn uniformly in
+    [nmin, nmax] and rounded down to the nearest CnH(2n+2) (3n + 2 atoms),
+    ideal geometry plus Gaussian noise.  Returns (atom_off, Z, xyz[A])."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(nmin, nmax + 1, size=nmol)
+    Zs, xs, off = [], [], [0]
+    for k, na in enumerate(sizes):
+        nc = max(1, (int(na) - 2) // 3)
+        Z, xyz = alkane(nc, sigma=sigma, seed=int(seed * 1_000_003 + k))
+        Zs.append(Z)
+        xs.append(xyz)
+        off.append(off[-1] + len(Z))
+    return np.asarray(off, dtype=np.int32), np.concatenate(Zs).astype(np.int32), np.vstack(xs)
+
+
 WORKLOADS = {
     "alkane_c40h82": lambda: alkane(40, sigma=0.02, seed=1),
     "water_3000": lambda: water_cluster(3000, seed=2),

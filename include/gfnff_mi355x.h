@@ -54,6 +54,23 @@ void gfnff_destroy(gfnff_handle* h);
 int gfnff_set_molecule(gfnff_handle* h, int32_t natoms, const int32_t* Z, const double* xyz_ang, int32_t charge,
                        const int32_t* forced_bonds, int32_t nbonds);
 
+/* Batch of independent molecules (the batched workload; no counterpart in the
+ * reference's single-molecule interface, which a caller would loop over):
+ * molecule m is atoms [atom_off[m], atom_off[m+1]) of Z / xyz_ang (Angstrom),
+ * total charge charges[m] (NULL: all neutral).  Each molecule's topology and
+ * parameters are generated as by gfnff_set_molecule; every later evaluation
+ * (gfnff_energy_gradient, gfnff_energy_gradient_device) treats the molecules
+ * as independent systems in one launch per kernel: *energy is the sum over
+ * the batch, the gradient covers all atoms.  Molecules must have at most 512
+ * reduced EEQ dimensions (atoms minus fragments, rounded up to 64). */
+int gfnff_set_batch(gfnff_handle* h, int32_t nmol, const int32_t* atom_off, const int32_t* Z, const double* xyz_ang,
+                    const int32_t* charges);
+
+/* Per-molecule energies (nmol) of the last evaluation of a batch: host copy
+ * (synchronises) or device copy queued on `stream`. */
+int gfnff_batch_energies(gfnff_handle* h, double* e_mol);
+int gfnff_batch_energies_device(gfnff_handle* h, double* e_mol_dev, void* stream);
+
 /* ComputationalMethod::updateGeometry(const Matrix&) (computational_method.h:63), Angstrom. */
 int gfnff_update_geometry(gfnff_handle* h, const double* xyz_ang);
 
