@@ -80,6 +80,10 @@ WORKLOAD_TEXT = {
                   "seeded random orientations, +-0.05 A jitter",
     "alkane_batch_4096": "alkane_batch_4096: 4096 independent all-trans n-alkanes, atom counts uniform in "
                          "[20, 400] rounded to CnH(2n+2), sigma 0.03 A, seed 3, one launch per kernel",
+    "mixed_batch_4096": "mixed_batch_4096: 4096 independent molecules, each a seeded random choice of an "
+                        "all-trans n-alkane or an extended glycine oligopeptide H-(Gly)n-OH, atom counts "
+                        "uniform in [20, 400] rounded down to the family size (3n+2 / 7n+3), sigma 0.03 A, "
+                        "seed 3, one launch per kernel",
 }
 
 
@@ -105,9 +109,10 @@ def main():
     import torch
     from curcuma_amd import GFNFFBatch, GFNFFMethod, Mol
     from curcuma_amd._lib import NTERMS
-    from curcuma_amd.workloads import WORKLOADS, alkane_batch
+    from curcuma_amd.workloads import WORKLOADS, alkane_batch, mixed_batch
 
-    BATCH_WORKLOADS = {"alkane_batch_4096": lambda: alkane_batch(4096, 20, 400, sigma=0.03, seed=3)}
+    BATCH_WORKLOADS = {"alkane_batch_4096": lambda: alkane_batch(4096, 20, 400, sigma=0.03, seed=3),
+                       "mixed_batch_4096": lambda: mixed_batch(4096, 20, 400, sigma=0.03, seed=3)}
 
     if args.copy_calibration_mib > 0:
         nel = args.copy_calibration_mib * (1 << 20) // 8

@@ -34,6 +34,7 @@ class ParamsView(C.Structure):
         ("nangle", C.c_int32), ("angle_ijk", _i32p), ("angle_par", _f64p),
         ("ntors", C.c_int32), ("tors_ijkl", _i32p), ("tors_par", _f64p),
         ("ninv", C.c_int32), ("inv_ijkl", _i32p), ("inv_par", _f64p),
+        ("nstors", C.c_int32), ("stors_ijkl", _i32p), ("stors_par", _f64p),
         ("nbrep", C.c_int32), ("brep_ij", _i32p), ("brep_par", _f64p),
         ("nhb", C.c_int32), ("hb_ahb", _i32p), ("hb_case", _i32p), ("hb_par", _f64p),
         ("hb_nbB_off", _i32p), ("hb_nbB_idx", _i32p), ("hb_parentC", _i32p), ("hb_nbC_off", _i32p),

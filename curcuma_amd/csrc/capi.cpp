@@ -107,13 +107,15 @@ int gfnff_set_batch(gfnff_handle* h, int32_t nmol, const int32_t* atom_off, cons
         std::vector<int> status(nmol, gfnff::kOk);
         std::vector<std::string> msgs(nmol);
         const int nthr = std::max(1, std::min<int>(nmol, (int)std::max(1u, std::thread::hardware_concurrency())));
+        gfnff::Options one = h->opt;  // molecules in parallel, each built single-threaded
+        one.threads = 1;
         std::vector<std::thread> pool;
         for (int w = 0; w < nthr; ++w)
             pool.emplace_back([&, w]() {
                 for (int m = w; m < nmol; m += nthr) {
                     const int a0 = atom_off[m], na = atom_off[m + 1] - a0;
                     status[m] = gfnff::build_params(na, Z + a0, h->xyz_bohr.data() + 3 * (size_t)a0,
-                                                    charges ? charges[m] : 0, nullptr, 0, h->opt, mols[m], msgs[m]);
+                                                    charges ? charges[m] : 0, nullptr, 0, one, mols[m], msgs[m]);
                 }
             });
         for (auto& t : pool) t.join();

@@ -1404,7 +1404,7 @@ __global__ __launch_bounds__(256) void k_angles(int na, const int* __restrict__ 
 }
 
 // signed dihedral and its gradient (phi as in gfnff_geometry.h:60-110)
-__device__ inline double dihed(double3 ri, double3 rj, double3 rk, double3 rl, double3 g[4], bool& ok) {
+__device__ __forceinline__ double dihed(double3 ri, double3 rj, double3 rk, double3 rl, double3 g[4], bool& ok) {
     double3 v1 = rj - ri, v2 = rk - rj, v3 = rl - rk;
     double3 n1 = cross(v1, v2), n2 = cross(v2, v3);
     double nn1 = nrm(n1), nn2 = nrm(n2), nv2 = nrm(v2);
@@ -1467,6 +1467,101 @@ __global__ __launch_bounds__(256) void k_tors(int nt, const int* __restrict__ ij
     st3(slots, s + 3, dEdphi * gp[3] + (-f2) * v[2]);
 }
 
+// out-of-plane angle of centre ri against the plane through its first
+// neighbour rj (gfnff_geometry.h:207-300): n = (ri - rj) x (rk - rj),
+// v = rl - ri, omega = asin(n.v / (|n||v|)), with its gradient from
+// gn = v/(|n||v|) - s n/|n|^2, gv = n/(|n||v|) - s v/|v|^2.
+__device__ __forceinline__ double oop(double3 ri, double3 rj, double3 rk, double3 rl, double3 g[4]) {
+    double3 re = ri - rj, rd = rk - rj, rv = rl - ri;
+    double3 nn = cross(re, rd);
+    double N = nrm(nn), V = nrm(rv);
+    for (int a = 0; a < 4; ++a) g[a] = make_double3(0, 0, 0);
+    if (N < 1e-10 || V < 1e-10) return 0.0;
+    double s = fmin(1.0, fmax(-1.0, dot(nn, rv) / (N * V)));
+    double om = asin(s), co = cos(om);
+    if (fabs(N * V * co) < 1e-14) return om;
+    double3 gn = (1.0 / (N * V)) * rv + (-s / (N * N)) * nn;
+    double3 gv = (1.0 / (N * V)) * nn + (-s / (V * V)) * rv;
+    double3 dre = cross(rd, gn), drd = cross(gn, re);
+    double ic = 1.0 / co;
+    g[0] = ic * (dre + (-1.0) * gv);
+    g[1] = (-ic) * (dre + drd);
+    g[2] = ic * drd;
+    g[3] = ic * gv;
+    return om;
+}
+
+// inversions: ff_workspace_gfnff.cpp:489-604.  par: fc, omega0, type
+// (0: V (1 - cos w), -1: V (cos w - cos w0)^2); damping over the distances of
+// the first neighbour j to the centre and the other two neighbours.
+__global__ __launch_bounds__(256) void k_inv(int nv, const int* __restrict__ ijkl, const double* __restrict__ par,
+                                             const int* __restrict__ slot0, const double* __restrict__ x,
+                                             const double* __restrict__ rcov_d3, double* __restrict__ slots,
+                                             double* __restrict__ e_out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv) return;
+    int id[4] = {ijkl[4 * t], ijkl[4 * t + 1], ijkl[4 * t + 2], ijkl[4 * t + 3]};
+    const double* p = par + 3 * t;
+    double3 r[4];
+    double rc[4];
+    for (int a = 0; a < 4; ++a) {
+        r[a] = ld3(x, id[a]);
+        rc[a] = rcov_d3[id[a]] * (4.0 / 3.0);
+    }
+    double3 go[4];
+    double om = oop(r[0], r[1], r[2], r[3], go);
+    const int other[3] = {0, 2, 3};
+    double3 dv[3];
+    double r2[3], rr[3], dmp[3];
+    for (int s = 0; s < 3; ++s) {
+        int o = other[s];
+        dv[s] = r[1] - r[o];
+        r2[s] = dot(dv[s], dv[s]);
+        double rcut = 0.505 * (rc[1] + rc[o]) * (rc[1] + rc[o]);
+        rr[s] = (r2[s] / rcut) * (r2[s] / rcut);
+        dmp[s] = 1.0 / (1.0 + rr[s]);
+    }
+    double damp = dmp[0] * dmp[1] * dmp[2];
+    double V = p[0], et, dEdom;
+    if (p[2] == 0.0) {
+        et = V * (1.0 - cos(om));
+        dEdom = V * sin(om) * damp;
+    } else {
+        double df = cos(om) - cos(p[1]);
+        et = V * df * df;
+        dEdom = -2.0 * V * sin(om) * df * damp;
+    }
+    e_out[t] = et * damp;
+    double f[3];
+    for (int s = 0; s < 3; ++s) {
+        double dd = (r2[s] < 1e-8) ? 0.0 : -4.0 * rr[s] / (r2[s] * (1.0 + rr[s]) * (1.0 + rr[s]));
+        f[s] = et * dd * dmp[(s + 1) % 3] * dmp[(s + 2) % 3];
+    }
+    int s = slot0[t];
+    st3(slots, s, dEdom * go[0] + (-f[0]) * dv[0]);
+    st3(slots, s + 1, dEdom * go[1] + f[0] * dv[0] + f[1] * dv[1] + f[2] * dv[2]);
+    st3(slots, s + 2, dEdom * go[2] + (-f[1]) * dv[1]);
+    st3(slots, s + 3, dEdom * go[3] + (-f[2]) * dv[2]);
+}
+
+// triple-bond torsions: ff_workspace_gfnff.cpp:611-648 (undamped),
+// E = erefhalf (1 - cos 2 phi)
+__global__ __launch_bounds__(256) void k_stors(int nt, const int* __restrict__ ijkl, const double* __restrict__ eh,
+                                               const int* __restrict__ slot0, const double* __restrict__ x,
+                                               double* __restrict__ slots, double* __restrict__ e_out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    double3 gp[4];
+    bool ok;
+    double phi = dihed(ld3(x, ijkl[4 * t]), ld3(x, ijkl[4 * t + 1]), ld3(x, ijkl[4 * t + 2]), ld3(x, ijkl[4 * t + 3]),
+                       gp, ok);
+    double e = eh[t];
+    e_out[t] = e - e * cos(2.0 * phi);
+    double d = ok ? 2.0 * e * sin(2.0 * phi) : 0.0;
+    int s = slot0[t];
+    for (int a = 0; a < 4; ++a) st3(slots, s + a, d * gp[a]);
+}
+
 // bonded repulsion: ff_workspace_gfnff.cpp:774-807
 __global__ __launch_bounds__(256) void k_brep(int nb, const int* __restrict__ ij, const double* __restrict__ par,
                                               const int* __restrict__ slot0, const double* __restrict__ x,
@@ -1488,22 +1583,38 @@ __global__ __launch_bounds__(256) void k_brep(int nb, const int* __restrict__ ij
     st3(slots, s + 1, (-dEdr / r) * d);
 }
 
-// hydrogen bonds, cases 1 and 2: ff_workspace_gfnff.cpp:938-1404
+// hydrogen bonds, cases 1-4: ff_workspace_gfnff.cpp:938-1404
 // Evaluates hydrogen bond h once: returns its energy and writes the gradient
 // of every participating atom to the bond's slots s0 + role (0 = A, 1 = H,
-// 2 = B, 3 + k = k-th neighbour of B).
-__device__ double hb_eval(int h, int s0, const int* __restrict__ ahb, const int* __restrict__ hcase,
-                          const double* __restrict__ hp, const int* __restrict__ nb_off,
-                          const int* __restrict__ nb_idx, const double* __restrict__ x,
-                          const double* __restrict__ hb_rad, const int* __restrict__ Z,
+// 2 = B, 3 + k = k-th neighbour of B, then for case 3 the neighbours D of the
+// carbonyl carbon other than B).  Case 3: B is a carbonyl O whose only
+// neighbour (slot 3) is the carbon C; case 4: sp2 N acceptor with a virtual
+// lone pair opposite its neighbours.
+struct HBArgs {
+    const int* ahb;
+    const int* hcase;
+    const double* hp;
+    const int* nb_off;
+    const int* nb_idx;
+    const int* parentC;
+    const int* nbc_off;
+    const int* nbc_idx;
+    const double* hb_rad;
+    const double* repz;
+    const int* Z;
+};
+__device__ double hb_eval(int h, int s0, const HBArgs& a, const double* __restrict__ x,
                           double* __restrict__ slots) {
-    int A = ahb[3 * h], H = ahb[3 * h + 1], B = ahb[3 * h + 2];
-    int cs = hcase[h];
-    const double* q = hp + 7 * h;
+    int A = a.ahb[3 * h], H = a.ahb[3 * h + 1], B = a.ahb[3 * h + 2];
+    int cs = a.hcase[h];
+    const double* q = a.hp + 7 * h;
     double basA = q[0], basB = q[1], aciA = q[2], aciB = q[3];
     double3 pA = ld3(x, A), pH = ld3(x, H), pB = ld3(x, B);
     double rAH = nrm(pH - pA), rHB = nrm(pB - pH), rAB = nrm(pB - pA);
-    int nb0 = nb_off[h], nb1 = nb_off[h + 1];
+    const int nb0 = a.nb_off[h], nb1 = a.nb_off[h + 1];
+    const int c0 = (cs == 3) ? a.nbc_off[h] : 0, c1 = (cs == 3) ? a.nbc_off[h + 1] : 0;
+    const int C3 = (cs == 3) ? a.parentC[h] : -1;
+    const int sD = s0 + 3 + (nb1 - nb0);  // first slot of the D atoms
     double3 z3 = make_double3(0, 0, 0);
     double3 ga = z3, gb = z3, gh = z3;
     double E = 0.0;
@@ -1515,25 +1626,63 @@ __device__ double hb_eval(int h, int s0, const int* __restrict__ ahb, const int*
         double QH = eH / (eH + 1.0), QA = eA / (eA + 1.0), QB = eB / (eB + 1.0);
         double bas = (QA * basA * rAH4 + QB * basB * rHB4) * den;
         double aci = (aciB * rAH4 + aciA * rHB4) * den;
-        double radab = hb_rad[A] + hb_rad[B];
+        double radab = a.hb_rad[A] + a.hb_rad[B];
         double dsh = 1.0 / (1.0 + pow(22.0 * radab / (rAB * rAB), 6.0));
         double dlg = 1.0 / (1.0 + pow(rAB * rAB / 85.0, 6.0));
         double denv = dsh * dlg;
         double expo0 = (49.0 / radab) * ((rAH + rHB) / rAB - 1.0);
         double doutl = (expo0 > 15.0) ? 0.0 : 2.0 / (1.0 + exp(expo0));
         double outl_nb = 1.0;
-        double nbcut = (Z[B] == 7 && nb1 - nb0 == 1) ? 2.0 : 11.20;
+        double nbcut = (a.Z[B] == 7 && nb1 - nb0 == 1) ? 2.0 : 11.20;
         if (cs >= 2)
             for (int k = nb0; k < nb1; ++k) {
-                double3 pn = ld3(x, nb_idx[k]);
+                double3 pn = ld3(x, a.nb_idx[k]);
                 double ex = (nbcut / radab) * ((nrm(pn - pA) + nrm(pn - pB)) / rAB - 1.0);
                 outl_nb *= (2.0 / (1.0 + exp(-ex)) - 1.0);
             }
         double rdamp = (cs >= 2) ? denv * (1.8 / (rHB * rHB * rHB) - 0.8 / (rAB * rAB * rAB)) : denv / (rAB * rAB * rAB);
-        double qhoutl = QH * doutl * outl_nb;
-        double gscale = (cs == 2 || cs == 4) ? 0.268 : 1.0;
+        // case 4: virtual lone pair lp = B - lpd v/|v|, v = sum_nb (x_nb - x_B)
+        double outl_lp = 1.0, lpd = 0.0, xlp = 0.0;
+        double3 lpv = z3, lp = pB;
+        int nlp = 0;
+        if (cs == 4) {
+            lpd = 0.50 - 0.018 * a.repz[B];
+            for (int k = nb0; k < nb1; ++k) lpv = lpv + (ld3(x, a.nb_idx[k]) - pB);
+            double vn = nrm(lpv);
+            if (vn > 1e-10 && nb1 > nb0) {
+                nlp = nb1 - nb0;
+                lp = pB + (-lpd / vn) * lpv;
+                xlp = (56.0 / radab) * ((nrm(pA - lp) + lpd + 1e-12) / rAB - 1.0);
+                outl_lp = 2.0 / (1.0 + exp(xlp));
+            }
+        }
+        // case 3: H...O=C bend and D-C=O...H torsion factors
+        double eangl = 1.0, etors = 1.0;
+        double3 dB3 = z3, dC3 = z3, dH3 = z3;  // d eangl / d(B, C, H)
+        const double tfc = (1.0 - 0.94) / 2.0;
+        if (C3 >= 0) {
+            const double cc0 = cos(120.0 * M_PI / 180.0);
+            const double kb = (1.0 - 0.20) / ((1.0 - cc0) * (1.0 - cc0));
+            double3 u = ld3(x, C3) - pB, w = pH - pB;
+            double ru = nrm(u), rw = nrm(w);
+            double ca = fmin(1.0, fmax(-1.0, dot(u, w) / (ru * rw + 1e-14)));
+            eangl = 1.0 - kb * (ca - cc0) * (ca - cc0);
+            double dea = 2.0 * kb * (ca - cc0);
+            dC3 = (-dea / ru) * ((1.0 / rw) * w + (-ca / ru) * u);
+            dH3 = (-dea / rw) * ((1.0 / ru) * u + (-ca / rw) * w);
+            dB3 = (-1.0) * (dC3 + dH3);
+            double3 pC = ld3(x, C3);
+            for (int k = c0; k < c1; ++k) {
+                double3 gp[4];
+                bool ok;
+                double phi = dihed(ld3(x, a.nbc_idx[k]), pB, pC, pH, gp, ok);
+                etors *= (1.0 + cos(2.0 * (phi - M_PI / 2.0) + M_PI)) * tfc + 0.94;
+            }
+        }
+        double qhoutl = QH * doutl * outl_nb * outl_lp;
+        double gscale = (cs == 2 || cs == 4) ? 0.268 : (cs == 3 ? 0.350 : 1.0);
         double cval = aciA * basB * QA * QB * gscale;
-        E = (cs >= 2) ? -rdamp * qhoutl * cval : -bas * aci * rdamp * qhoutl;
+        E = (cs >= 2) ? -rdamp * qhoutl * cval * eangl * etors : -bas * aci * rdamp * qhoutl;
         double3 drab = pA - pB, drah = pA - pH, drbh = pB - pH;
         double rab2 = rAB * rAB, rbh2 = rHB * rHB, rah2 = rAH * rAH;
         double rahprbh = rAH + rHB + 1e-12;
@@ -1543,9 +1692,10 @@ __device__ double hb_eval(int h, int s0, const int* __restrict__ ahb, const int*
         if (!(expo > 15.0)) {
             double ratio2 = exp(expo);
             if (cs >= 2) {
+                double ct = cval * eangl * etors;
                 double rbhdamp = denv * 1.8 / (rbh2 * rHB), rabdamp = denv * -0.8 / (rab2 * rAB);
-                double dterm = -qhoutl * cval, aterm = -rdamp * QH * outl_nb * cval;
-                double nbterm = -rdamp * QH * doutl * cval;
+                double dterm = -qhoutl * ct, aterm = -rdamp * QH * outl_nb * outl_lp * ct;
+                double nbterm = -rdamp * QH * doutl * outl_lp * ct;
                 double gi = ((rabdamp + rbhdamp) * ddamp - 3.0 * rabdamp) / rab2 * dterm;
                 ga = gi * drab;
                 gb = (-gi) * drab;
@@ -1560,10 +1710,47 @@ __device__ double hb_eval(int h, int s0, const int* __restrict__ ahb, const int*
                 ga = ga + dga;
                 gb = gb + dgb;
                 gh = gh + (-1.0) * (dga + dgb);
+                // case 4: d outl_lp through s = |A - lp| + lpd and rAB
+                double3 pg = z3;
+                if (nlp > 0) {
+                    double lpterm = -rdamp * QH * doutl * outl_nb * cval;
+                    double3 dal = pA - lp;
+                    double ral = nrm(dal), sl = ral + lpd + 1e-12;
+                    double el = exp(xlp), fe = 2.0 * lpterm * el / ((1.0 + el) * (1.0 + el)) * (56.0 / radab);
+                    double dEds = -fe / rAB, dEdrab = fe * sl / rab2;
+                    double3 gA = (dEds / (ral + 1e-12)) * dal;
+                    ga = ga + (dEdrab / rAB) * drab + gA;
+                    gb = gb + (-dEdrab / rAB) * drab;
+                    double3 glp = (-1.0) * gA;
+                    double vn = nrm(lpv), vg = dot(lpv, glp);
+                    pg = (lpd / vn) * (glp + (-vg / (vn * vn)) * lpv);
+                    gb = gb + glp + (double)nlp * pg;
+                }
+                // case 3: bend and torsion factors
+                double3 gC = z3;
+                if (C3 >= 0) {
+                    double bterm = -rdamp * qhoutl * etors * cval, tterm = -rdamp * qhoutl * eangl * cval;
+                    gb = gb + bterm * dB3;
+                    gC = bterm * dC3;
+                    gh = gh + bterm * dH3;
+                    double3 pC = ld3(x, C3);
+                    for (int k = c0; k < c1; ++k) {
+                        double3 gp[4];
+                        bool ok;
+                        double phi = dihed(ld3(x, a.nbc_idx[k]), pB, pC, pH, gp, ok);
+                        double cc = 2.0 * (phi - M_PI / 2.0) + M_PI;
+                        double etk = (1.0 + cos(cc)) * tfc + 0.94;
+                        double f = ok ? (etors / etk) * tterm * (-2.0 * sin(cc) * tfc) : 0.0;
+                        st3(slots, sD + (k - c0), f * gp[0]);
+                        gb = gb + f * gp[1];
+                        gC = gC + f * gp[2];
+                        gh = gh + f * gp[3];
+                    }
+                }
                 nb_written = true;
                 int kk = 3;
                 for (int k = nb0; k < nb1; ++k, ++kk) {
-                    double3 pn = ld3(x, nb_idx[k]);
+                    double3 pn = ld3(x, a.nb_idx[k]);
                     double3 dranb = pA - pn, drbnb = pB - pn;
                     double ranb = nrm(dranb), rbnb = nrm(drbnb), sm = ranb + rbnb + 1e-12;
                     double ex = (nbcut / radab) * (sm / rAB - 1.0);
@@ -1577,7 +1764,9 @@ __device__ double hb_eval(int h, int s0, const int* __restrict__ ahb, const int*
                     double3 da = (tmp2 / ranb) * dranb, db = (tmp2 / rbnb) * drbnb;
                     ga = ga + da;
                     gb = gb + db;
-                    st3(slots, s0 + kk, (-1.0) * (da + db));
+                    double3 gn = (-1.0) * (da + db + pg);
+                    if (kk == 3 && C3 >= 0) gn = gn + gC;
+                    st3(slots, s0 + kk, gn);
                 }
             } else {
                 double caa = QA * basA, cbb = QB * basB;
@@ -1609,21 +1798,20 @@ __device__ double hb_eval(int h, int s0, const int* __restrict__ ahb, const int*
     st3(slots, s0, ga);
     st3(slots, s0 + 1, gh);
     st3(slots, s0 + 2, gb);
-    if (!nb_written)
+    if (!nb_written) {
         for (int kk = 3; kk < 3 + nb1 - nb0; ++kk) st3(slots, s0 + kk, z3);
+        for (int k = c0; k < c1; ++k) st3(slots, sD + (k - c0), z3);
+    }
     return E;
 }
 
 // one thread per hydrogen bond: energy and the gradient slots of its atoms
-__global__ __launch_bounds__(256) void k_hb_full(int nh, const int* __restrict__ ahb, const int* __restrict__ hcase,
-                                                 const double* __restrict__ hp, const int* __restrict__ nb_off,
-                                                 const int* __restrict__ nb_idx, const int* __restrict__ slot0,
-                                                 const double* __restrict__ x, const double* __restrict__ hb_rad,
-                                                 const int* __restrict__ Z, double* __restrict__ slots,
+__global__ __launch_bounds__(256) void k_hb_full(int nh, HBArgs a, const int* __restrict__ slot0,
+                                                 const double* __restrict__ x, double* __restrict__ slots,
                                                  double* __restrict__ e_out) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= nh) return;
-    e_out[h] = hb_eval(h, slot0[h], ahb, hcase, hp, nb_off, nb_idx, x, hb_rad, Z, slots);
+    e_out[h] = hb_eval(h, slot0[h], a, x, slots);
 }
 
 // per-atom sum of the hydrogen-bond gradient slots: one wavefront per atom,
@@ -1865,9 +2053,9 @@ __global__ __launch_bounds__(256) void k_eeq_batch_solve(const int* __restrict__
 struct MolEnergyArgs {
     int nmol;
     const int* mol_off;
-    const int* seg;        // [7][nmol + 1]: bond, angle, torsion, brep, hb, atm, batm
+    const int* seg;        // [9][nmol + 1]: bond, angle, torsion, brep, hb, atm, batm, inversion, stors
     const double* atom_terms[4];   // coul, disp, rep, self
-    const double* list_terms[7];
+    const double* list_terms[9];
     double* e_mol;
 };
 __global__ __launch_bounds__(256) void k_mol_energy(MolEnergyArgs a) {
@@ -1877,7 +2065,7 @@ __global__ __launch_bounds__(256) void k_mol_energy(MolEnergyArgs a) {
     double acc = 0.0;
     for (int k = 0; k < 4; ++k)
         for (int i = a.mol_off[m] + lane; i < a.mol_off[m + 1]; i += 64) acc += a.atom_terms[k][i];
-    for (int k = 0; k < 7; ++k) {
+    for (int k = 0; k < 9; ++k) {
         const int* sg = a.seg + k * (a.nmol + 1);
         for (int i = sg[m] + lane; i < sg[m + 1]; i += 64) acc += a.list_terms[k][i];
     }
@@ -2213,6 +2401,7 @@ struct Segs {
     int dst[16];
 };
 constexpr int RED_BLOCKS = 64;
+constexpr int NPARTS = 13;  // energy segments reduced per evaluation
 // stage 1: segment sg, block b reduces the fixed chunk b of the segment
 __global__ __launch_bounds__(256) void k_reduce1(Segs s, double* __restrict__ part) {
     __shared__ double red[256];
@@ -2245,7 +2434,7 @@ __global__ void k_total(const double* __restrict__ parts, double* __restrict__ o
     double t[GFNFF_NTERMS];
     for (int k = 0; k < GFNFF_NTERMS; ++k) t[k] = 0.0;
     // parts: 0 bond,1 angle,2 tors,3 inv,4 disp(pairs),5 brep,6 nbrep(pairs incl fix),7 coul pairs,
-    // 8 self,9 hb,10 atm,11 batm
+    // 8 self,9 hb,10 atm,11 batm,12 triple-bond torsions
     t[GFNFF_E_BOND] = parts[0];
     t[GFNFF_E_ANGLE] = parts[1];
     t[GFNFF_E_TORSION] = parts[2];
@@ -2257,6 +2446,7 @@ __global__ void k_total(const double* __restrict__ parts, double* __restrict__ o
     t[GFNFF_E_HBOND] = parts[9];
     t[GFNFF_E_ATM] = parts[10];
     t[GFNFF_E_BATM] = parts[11];
+    t[GFNFF_E_STORS] = parts[12];
     double tot = 0.0;
     for (int k = 0; k < GFNFF_NTERMS; ++k) tot += t[k];
     out[0] = tot;
@@ -2328,7 +2518,7 @@ struct DeviceEngine {
     KernelTimer timer;
     int n = 0, nelem = 0, nfrag = 0;
     int nr = 0, np = 0;                 // reduced EEQ size, padded size
-    int nbond = 0, nangle = 0, ntors = 0, nbrep = 0, nhb = 0, natm = 0, nbatm = 0;
+    int nbond = 0, nangle = 0, ntors = 0, nbrep = 0, nhb = 0, natm = 0, nbatm = 0, ninv = 0, nstors = 0;
     int nslots = 0;
     int pair_cj = 0, pair_nchunk = 0;   // j-chunk size and count of the pair kernel
     bool mixed = false;                 // FP32 pair terms + FP32 factorisation with FP64 refinement
@@ -2359,9 +2549,11 @@ struct DeviceEngine {
     double* ang_par;
     int *tor_ijkl, *tor_slot;
     double* tor_par;
+    int *inv_ijkl, *inv_slot, *st_ijkl, *st_slot;
+    double *inv_par, *st_par;
     int *brep_ij, *brep_slot;
     double* brep_par;
-    int *hb_ahb, *hb_case, *hb_nboff, *hb_nbidx, *hbs_off, *hbs_idx, *hb_slot0;
+    int *hb_ahb, *hb_case, *hb_nboff, *hb_nbidx, *hbs_off, *hbs_idx, *hb_slot0, *hb_parc, *hb_nbcoff, *hb_nbcidx;
     double* hb_slots;
     double* hb_par;
     int *atm_ijk, *atm_slot;
@@ -2372,7 +2564,7 @@ struct DeviceEngine {
     // work
     double *x, *cn, *dlog, *gw, *dgw, *rhs, *Ared, *Linv, *b, *rres, *q, *eself, *qtmp, *gpair, *dEdcn, *vchain,
         *slots, *dcn_slot, *grad, *e_coul, *e_disp, *e_rep, *e_bond, *e_ang, *e_tor, *e_brep, *e_hb, *e_atm,
-        *e_batm, *parts, *rpart, *out, *diag, *ghb, *ybuf, *xbuf, *h6, *dh6, *ppart;
+        *e_batm, *e_inv, *e_st, *parts, *rpart, *out, *diag, *ghb, *ybuf, *xbuf, *h6, *dh6, *ppart;
     int *nbn, *nbl, *flags;
 
     PairArgs pair_args() const {
@@ -2446,15 +2638,11 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         msg = "unknown precision mode";
         return nullptr;
     }
-    if (!P.inv_par.empty()) {
-        status = kErrUnsupported;
-        msg = "inversion terms are not built on the device yet";
-        return nullptr;
-    }
-    for (int c : P.hb_case)
-        if (c > 2) {
-            status = kErrUnsupported;
-            msg = "hydrogen-bond cases 3/4 are not built on the device yet";
+    for (size_t h = 0; h < P.hb_case.size(); ++h)
+        if (P.hb_case[h] == 3 && P.hb_parentC[h] >= 0 &&
+            (P.hb_nbB_off[h + 1] - P.hb_nbB_off[h] != 1 || P.hb_nbB_idx[P.hb_nbB_off[h]] != P.hb_parentC[h])) {
+            status = kErrArgs;
+            msg = "hydrogen-bond case 3 needs the carbonyl carbon as the acceptor's only neighbour";
             return nullptr;
         }
     auto* E = new DeviceEngine();
@@ -2630,6 +2818,24 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->tor_ijkl = E->upload(P.tors_ijkl);
         E->tor_par = E->upload(P.tors_par);
         E->tor_slot = E->upload(tslot);
+        E->ninv = (int)(P.inv_par.size() / 3);
+        std::vector<int> islot(E->ninv);
+        for (int t = 0; t < E->ninv; ++t) {
+            islot[t] = add_slot(P.inv_ijkl[4 * t]);
+            for (int a = 1; a < 4; ++a) add_slot(P.inv_ijkl[4 * t + a]);
+        }
+        E->inv_ijkl = E->upload(P.inv_ijkl);
+        E->inv_par = E->upload(P.inv_par);
+        E->inv_slot = E->upload(islot);
+        E->nstors = (int)P.stors_par.size();
+        std::vector<int> sslot(E->nstors);
+        for (int t = 0; t < E->nstors; ++t) {
+            sslot[t] = add_slot(P.stors_ijkl[4 * t]);
+            for (int a = 1; a < 4; ++a) add_slot(P.stors_ijkl[4 * t + a]);
+        }
+        E->st_ijkl = E->upload(P.stors_ijkl);
+        E->st_par = E->upload(P.stors_par);
+        E->st_slot = E->upload(sslot);
         E->nbrep = (int)(P.brep_par.size() / 2);
         std::vector<int> rslot(E->nbrep);
         for (int b = 0; b < E->nbrep; ++b) {
@@ -2642,13 +2848,16 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->nhb = (int)P.hb_case.size();
         {
             // gradient slots of the hydrogen bonds, bond-major (3 + neighbours of
-            // B per bond), and the per-atom CSR over them in ascending slot order
+            // B + for case 3 the neighbours of the carbonyl C other than B), and
+            // the per-atom CSR over them in ascending slot order
             std::vector<int> s0(E->nhb), atom_of;
             atom_of.reserve((size_t)E->nhb * 5);
             for (int h = 0; h < E->nhb; ++h) {
                 s0[h] = (int)atom_of.size();
                 for (int r = 0; r < 3; ++r) atom_of.push_back(P.hb_ahb[3 * h + r]);
                 for (int k = P.hb_nbB_off[h]; k < P.hb_nbB_off[h + 1]; ++k) atom_of.push_back(P.hb_nbB_idx[k]);
+                if (P.hb_case[h] == 3)
+                    for (int k = P.hb_nbC_off[h]; k < P.hb_nbC_off[h + 1]; ++k) atom_of.push_back(P.hb_nbC_idx[k]);
                 if (atom_of.size() > (size_t)INT32_MAX / 4)
                     throw std::runtime_error("hydrogen-bond slot count exceeds the index range");
             }
@@ -2667,6 +2876,9 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->hb_par = E->upload(P.hb_par);
         E->hb_nboff = E->upload(P.hb_nbB_off);
         E->hb_nbidx = E->upload(P.hb_nbB_idx);
+        E->hb_parc = E->upload(P.hb_parentC);
+        E->hb_nbcoff = E->upload(P.hb_nbC_off);
+        E->hb_nbcidx = E->upload(P.hb_nbC_idx);
         E->natm = (int)P.atm_scale.size();
         std::vector<int> atslot(E->natm);
         for (int t = 0; t < E->natm; ++t) {
@@ -2767,6 +2979,8 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->e_bond = E->alloc<double>(std::max(E->nbond, 1));
         E->e_ang = E->alloc<double>(std::max(E->nangle, 1));
         E->e_tor = E->alloc<double>(std::max(E->ntors, 1));
+        E->e_inv = E->alloc<double>(std::max(E->ninv, 1));
+        E->e_st = E->alloc<double>(std::max(E->nstors, 1));
         E->e_brep = E->alloc<double>(std::max(E->nbrep, 1));
         E->e_hb = E->alloc<double>(std::max(E->nhb, 1));
         E->e_atm = E->alloc<double>(std::max(E->natm, 1));
@@ -2974,12 +3188,21 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     if (E->ntors)
         LAUNCH(KC_TORS, k_tors, dim3(cdiv(E->ntors, 256)), dim3(256), s, E->ntors, E->tor_ijkl, E->tor_par,
                            E->tor_slot, E->x, E->rcov_d3, E->slots, E->e_tor);
+    if (E->ninv)
+        LAUNCH(KC_TORS, k_inv, dim3(cdiv(E->ninv, 256)), dim3(256), s, E->ninv, E->inv_ijkl, E->inv_par, E->inv_slot,
+               E->x, E->rcov_d3, E->slots, E->e_inv);
+    if (E->nstors)
+        LAUNCH(KC_TORS, k_stors, dim3(cdiv(E->nstors, 256)), dim3(256), s, E->nstors, E->st_ijkl, E->st_par,
+               E->st_slot, E->x, E->slots, E->e_st);
     if (E->nbrep)
         LAUNCH(KC_BREP, k_brep, dim3(cdiv(E->nbrep, 256)), dim3(256), s, E->nbrep, E->brep_ij, E->brep_par,
                            E->brep_slot, E->x, E->slots, E->e_brep);
-    if (E->nhb)
-        LAUNCH(KC_HB, k_hb_full, dim3(cdiv(E->nhb, 256)), dim3(256), s, E->nhb, E->hb_ahb, E->hb_case, E->hb_par,
-               E->hb_nboff, E->hb_nbidx, E->hb_slot0, E->x, E->hb_rad, E->Z, E->hb_slots, E->e_hb);
+    if (E->nhb) {
+        HBArgs ha{E->hb_ahb, E->hb_case, E->hb_par, E->hb_nboff, E->hb_nbidx, E->hb_parc,
+                  E->hb_nbcoff, E->hb_nbcidx, E->hb_rad, E->repz, E->Z};
+        LAUNCH(KC_HB, k_hb_full, dim3(cdiv(E->nhb, 256)), dim3(256), s, E->nhb, ha, E->hb_slot0, E->x, E->hb_slots,
+               E->e_hb);
+    }
     LAUNCH(KC_HB, k_hb_gather, dim3(cdiv(n, 4)), dim3(256), s, n, E->hbs_off, E->hbs_idx, E->hb_slots, E->ghb);
     if (E->natm)
         LAUNCH(KC_ATM, k_atm, dim3(cdiv(E->natm, 256)), dim3(256), s, E->natm, E->atm_ijk, E->atm_sc,
@@ -2993,16 +3216,17 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     LAUNCH(KC_CHAIN, k_finish, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->rcov_cn, E->dlog, E->vchain,
                        E->nbn, E->nbl, E->slot_off, E->slot_idx, E->slots, E->gpair, E->ghb, E->grad);
     Segs sg;
-    const double* ptrs[12] = {E->e_bond, E->e_ang, E->e_tor, nullptr, E->e_disp, E->e_brep,
-                              E->e_rep,  E->e_coul, E->eself, E->e_hb, E->e_atm, E->e_batm};
-    int cnts[12] = {E->nbond, E->nangle, E->ntors, 0, n, E->nbrep, n, n, n, E->nhb, E->natm, E->nbatm};
-    for (int k = 0; k < 12; ++k) {
-        sg.p[k] = ptrs[k] ? ptrs[k] : E->e_disp;
-        sg.n[k] = ptrs[k] ? cnts[k] : 0;
+    const double* ptrs[NPARTS] = {E->e_bond, E->e_ang,  E->e_tor, E->e_inv, E->e_disp, E->e_brep, E->e_rep,
+                                  E->e_coul, E->eself, E->e_hb,  E->e_atm, E->e_batm, E->e_st};
+    int cnts[NPARTS] = {E->nbond, E->nangle, E->ntors, E->ninv, n, E->nbrep, n, n, n, E->nhb, E->natm, E->nbatm,
+                        E->nstors};
+    for (int k = 0; k < NPARTS; ++k) {
+        sg.p[k] = ptrs[k];
+        sg.n[k] = cnts[k];
         sg.dst[k] = k;
     }
-    LAUNCH(KC_REDUCE, k_reduce1, dim3(RED_BLOCKS, 12), dim3(256), s, sg, E->rpart);
-    LAUNCH(KC_REDUCE, k_reduce2, dim3(12), dim3(64), s, sg, E->rpart, E->parts);
+    LAUNCH(KC_REDUCE, k_reduce1, dim3(RED_BLOCKS, NPARTS), dim3(256), s, sg, E->rpart);
+    LAUNCH(KC_REDUCE, k_reduce2, dim3(NPARTS), dim3(64), s, sg, E->rpart, E->parts);
     LAUNCH(KC_REDUCE, k_total, dim3(1), dim3(64), s, E->parts, E->out);
     if (E->nmol > 1) {
         MolEnergyArgs ma;
@@ -3013,8 +3237,9 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         ma.atom_terms[1] = E->e_disp;
         ma.atom_terms[2] = E->e_rep;
         ma.atom_terms[3] = E->eself;
-        const double* lists[7] = {E->e_bond, E->e_ang, E->e_tor, E->e_brep, E->e_hb, E->e_atm, E->e_batm};
-        for (int k = 0; k < 7; ++k) ma.list_terms[k] = lists[k];
+        const double* lists[9] = {E->e_bond, E->e_ang, E->e_tor, E->e_brep, E->e_hb,
+                                  E->e_atm,  E->e_batm, E->e_inv, E->e_st};
+        for (int k = 0; k < 9; ++k) ma.list_terms[k] = lists[k];
         ma.e_mol = E->e_mol;
         LAUNCH(KC_REDUCE, k_mol_energy, dim3(cdiv(E->nmol, 4)), dim3(256), s, ma);
     }

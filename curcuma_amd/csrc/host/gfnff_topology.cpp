@@ -629,6 +629,222 @@ bool phase1_charges(const Mol& m, Topo& t, int threads, std::string& msg) {
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Hueckel pi bond orders (HuckelSolver, huckel_solver.cpp:32-565)
+// ---------------------------------------------------------------------------
+
+// Symmetric eigenproblem by cyclic Jacobi rotations (Golub & Van Loan, Matrix
+// Computations, sec. 8.5, sym.schur2 + cyclic sweep).  a: n x n row-major,
+// destroyed.  Returns eigenvalues ascending in w and the matching
+// eigenvectors as the columns of v (row-major n x n).
+void sym_eig(int n, std::vector<double>& a, std::vector<double>& w, std::vector<double>& v) {
+    v.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) v[(size_t)i * n + i] = 1.0;
+    auto A = [&](int i, int j) -> double& { return a[(size_t)i * n + j]; };
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, dia = 0.0;
+        for (int i = 0; i < n; ++i) {
+            dia += A(i, i) * A(i, i);
+            for (int j = i + 1; j < n; ++j) off += A(i, j) * A(i, j);
+        }
+        if (off <= 1e-32 * (dia + 1e-300)) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                double apq = A(p, q);
+                if (apq == 0.0) continue;
+                double tau = (A(q, q) - A(p, p)) / (2.0 * apq);
+                double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::abs(tau) + std::sqrt(1.0 + tau * tau));
+                double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+                for (int k = 0; k < n; ++k) {  // A <- A J
+                    double akp = A(k, p), akq = A(k, q);
+                    A(k, p) = c * akp - s * akq;
+                    A(k, q) = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {  // A <- J^T A
+                    double apk = A(p, k), aqk = A(q, k);
+                    A(p, k) = c * apk - s * aqk;
+                    A(q, k) = s * apk + c * aqk;
+                }
+                A(p, q) = A(q, p) = 0.0;
+                for (int k = 0; k < n; ++k) {  // V <- V J
+                    double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
+                    v[(size_t)k * n + p] = c * vkp - s * vkq;
+                    v[(size_t)k * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    std::vector<int> ord(n);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return A(x, x) < A(y, y); });
+    w.resize(n);
+    std::vector<double> vs((size_t)n * n);
+    for (int c = 0; c < n; ++c) {
+        w[c] = A(ord[c], ord[c]);
+        for (int r = 0; r < n; ++r) vs[(size_t)r * n + c] = v[(size_t)r * n + ord[c]];
+    }
+    v.swap(vs);
+}
+
+// pi electrons an atom brings into its pi system (countPiElectrons,
+// huckel_solver.cpp:197-266; the reference passes itag = 0 for every atom,
+// gfnff_method.cpp:7397-7398)
+int pi_electrons(int z, int hyb) {
+    switch (z) {
+        case 5: return hyb == 1 ? 1 : 0;
+        case 6: return 1;
+        case 7: return hyb <= 2 ? 1 : (hyb == 3 ? 2 : 0);
+        case 8:
+        case 16: return (hyb == 1 || hyb == 2) ? 1 : (hyb == 3 ? 2 : 0);
+        case 9: return hyb != 1 ? 2 : 3;
+        case 17: return hyb == 0 ? 2 : (hyb == 1 ? 3 : 0);
+        default: return 0;
+    }
+}
+
+// Closed-shell occupations of one pi system: Fermi smearing at 4000 K per spin
+// channel, doubled, with the integer fallback for a degenerate HOMO/LUMO
+// (solveAndBuildDensity + fermiSmear, huckel_solver.cpp:374-514).  e in eV,
+// ascending.  Returns sum(occ * e).
+double pi_occupations(const std::vector<double>& e, int nel, std::vector<double>& occ) {
+    const int no = (int)e.size();
+    const int na = nel / 2;
+    const double bkt = 8.617333262e-5 * 4000.0;
+    occ.assign(no, 0.0);
+    if (na > 0) {
+        int ih = std::min(na, no) - 1, il = std::min(na, no - 1);
+        double ef = 0.5 * (e[ih] + e[il]);
+        for (int cyc = 0; cyc < 200; ++cyc) {
+            double tot = 0.0, der = 0.0;
+            for (int i = 0; i < no; ++i) {
+                double x = (e[i] - ef) / bkt, f = 0.0, d = 0.0;
+                if (x < 50.0) {
+                    double ex = std::exp(x);
+                    f = 1.0 / (ex + 1.0);
+                    d = ex / (bkt * (ex + 1.0) * (ex + 1.0));
+                }
+                occ[i] = f;
+                tot += f;
+                der += d;
+            }
+            ef += ((double)na - tot) / der;
+            if (std::abs((double)na - tot) < 1e-9) break;
+        }
+    }
+    for (double& o : occ) o *= 2.0;
+    if (na > 0 && na < no && std::abs(occ[na - 1] - occ[na]) < 1e-4) {
+        std::fill(occ.begin(), occ.end(), 0.0);
+        for (int i = 0; i < na; ++i) occ[i] = 2.0;
+    }
+    double el = 0.0;
+    for (int i = 0; i < no; ++i) el += occ[i] * e[i];
+    return el;
+}
+
+// Iterative Hueckel per pi system (calculatePiBondOrders, huckel_solver.cpp:32-196;
+// Hamiltonian :271-369).  Returns the pi bond order of every cached bond
+// (0 for bonds outside a pi system).  ipis[s] = charge localised on pi system s.
+std::vector<double> hueckel_pibo(const Mol& m, const Topo& t, const std::vector<int>& ipis) {
+    static const double hdiag[18] = {0, 0, 0, 0, 0, -0.5, 0, 0.14, -0.38, -0.29, 0, 0, 0, 0, 0, 0, -0.30, -0.30};
+    static const double hoff[18] = {0, 0, 0, 0, 0, 0.5, 1.0, 0.66, 1.1, 0.23, 0, 0, 0, 0, 0, 0, 0.6, 1.0};
+    const double hiter = 0.700, htriple = 1.45, hp3 = -0.24, pilpf = 0.530, ev = 0.1 * 27.2113957;
+    std::vector<double> pibo(t.bonds.size(), 0.0);
+    int maxid = 0;
+    for (int p : t.pi) maxid = std::max(maxid, p);
+    std::vector<int> loc(m.n, -1);
+    for (int s = 1; s <= maxid; ++s) {
+        std::vector<int> at;
+        int nel = 0;
+        for (int k = 0; k < m.n; ++k)
+            if (t.pi[k] == s) {
+                loc[k] = (int)at.size();
+                at.push_back(k);
+                nel += pi_electrons(m.Z[k], t.hyb[k]);
+            }
+        if (s < (int)ipis.size()) nel -= ipis[s];
+        const int np = (int)at.size();
+        std::vector<int> bl;  // cached bonds inside this pi system
+        for (size_t b = 0; b < t.bonds.size(); ++b)
+            if (loc[t.bonds[b].first] >= 0 && loc[t.bonds[b].second] >= 0) bl.push_back((int)b);
+        if (np >= 2 && nel >= 1) {
+            std::vector<double> Pm((size_t)np * np, 2.0 / 3.0), H, w, C, occ;
+            double eold = 0.0;
+            for (int it = 0; it < 5; ++it) {
+                H.assign((size_t)np * np, 0.0);
+                for (int a = 0; a < np; ++a) {
+                    int k = at[a], z = m.Z[k];
+                    int ne = std::min(pi_electrons(z, t.hyb[k]), 2);
+                    H[(size_t)a * np + a] = (z < 18 ? hdiag[z] : 0.0) + t.qa[k] * hp3 - (double)(ne - 1) * pilpf;
+                }
+                for (int b : bl) {
+                    int ii = t.bonds[b].first, jj = t.bonds[b].second;
+                    int ia = loc[ii], ja = loc[jj];
+                    int zi = m.Z[ii], zj = m.Z[jj];
+                    double beta = std::sqrt((zi < 18 ? hoff[zi] : 0.0) * (zj < 18 ? hoff[zj] : 0.0));
+                    beta -= 1.0e-9 * m.dist(ii, jj);
+                    double dmp = hiter;
+                    if (t.hyb[ii] == 1) dmp *= htriple;
+                    if (t.hyb[jj] == 1) dmp *= htriple;
+                    double h = -beta * (1.0 - dmp * (2.0 / 3.0 - Pm[(size_t)ja * np + ia]));
+                    H[(size_t)ja * np + ia] = h;
+                    H[(size_t)ia * np + ja] = h;
+                }
+                sym_eig(np, H, w, C);
+                for (double& x : w) x *= ev;
+                double enew = pi_occupations(w, nel, occ);
+                for (int r = 0; r < np; ++r)
+                    for (int c = 0; c < np; ++c) {
+                        double acc = 0.0;
+                        for (int o = 0; o < np; ++o) acc += C[(size_t)r * np + o] * C[(size_t)c * np + o] * occ[o];
+                        Pm[(size_t)r * np + c] = acc;
+                    }
+                if (std::abs(enew - eold) < 1e-4) break;
+                eold = enew;
+            }
+            for (int b : bl) pibo[b] = Pm[(size_t)loc[t.bonds[b].second] * np + loc[t.bonds[b].first]];
+        }
+        for (int k : at) loc[k] = -1;
+    }
+    return pibo;
+}
+
+// Charge localised on each pi system (gfnff_method.cpp:9311-9373): the change
+// of the H-condensed topology charges of its atoms when its fragment is made
+// neutral, times 1.1, rounded.  Zero for pi systems in neutral fragments.
+bool pi_system_charges(const Mol& m, const Topo& t, int threads, std::vector<int>& ipis, std::string& msg) {
+    int maxid = 0;
+    for (int p : t.pi) maxid = std::max(maxid, p);
+    ipis.assign(maxid + 1, 0);
+    auto qheavy = [&](const std::vector<double>& q) {
+        std::vector<double> qh = q;
+        for (int i = 0; i < m.n; ++i) {
+            if (m.Z[i] != 1 || t.adj[i].empty()) continue;
+            double share = q[i] / (double)t.adj[i].size();
+            for (int nb : t.adj[i]) qh[nb] += share;
+            qh[i] = 0.0;
+        }
+        return qh;
+    };
+    std::vector<double> qah;
+    for (int s = 1; s <= maxid; ++s) {
+        int first = -1;
+        for (int k = 0; k < m.n && first < 0; ++k)
+            if (t.pi[k] == s) first = k;
+        if (first < 0) continue;
+        int f = t.frag[first] - 1;
+        if (t.qfrag[f] == 0.0) continue;  // neutralising changes nothing
+        if (qah.empty()) qah = qheavy(t.qa);
+        Topo tn = t;
+        tn.qfrag[f] = 0.0;
+        if (!phase1_charges(m, tn, threads, msg)) return false;
+        std::vector<double> qnh = qheavy(tn.qa);
+        double dum = 0.0;
+        for (int k = 0; k < m.n; ++k)
+            if (t.pi[k] == s) dum += qah[k] - qnh[k];
+        ipis[s] = (int)std::lround(dum * 1.1);
+    }
+    return true;
+}
+
 // classifyBondType, gfnff_method.cpp:4021-4098, plus the sp re-classification
 // of calculateTopologyInfo :9430-9440 (pibo = 0 without a Hueckel pi system).
 void bond_types(const Mol& m, Topo& t, const std::vector<double>& pibo_of_bond) {
@@ -1586,7 +1802,7 @@ int merge_batch(const std::vector<ParamSet>& mols, ParamSet& B, std::string& msg
     }
     B.nmol = nm;
     B.mol_off.assign(nm + 1, 0);
-    B.mol_seg.assign(7 * (size_t)(nm + 1), 0);
+    B.mol_seg.assign(9 * (size_t)(nm + 1), 0);
     std::vector<int> seen(87, -1);
     for (const auto& P : mols)
         for (int z : P.present_Z)
@@ -1595,6 +1811,64 @@ int merge_batch(const std::vector<ParamSet>& mols, ParamSet& B, std::string& msg
                 B.present_Z.push_back(z);
             }
     set_element_tables(B);
+    // reserve every concatenated list once (avoids repeated regrowth of the
+    // multi-GB lists of large batches)
+    {
+        auto reserve = [&](auto member) {
+            size_t tot = 1;
+            for (const auto& P : mols) tot += (P.*member).size();
+            (B.*member).reserve(tot);
+        };
+        reserve(&ParamSet::Z);
+        reserve(&ParamSet::alpeeq);
+        reserve(&ParamSet::angle_ijk);
+        reserve(&ParamSet::angle_par);
+        reserve(&ParamSet::atm_ijk);
+        reserve(&ParamSet::atm_scale);
+        reserve(&ParamSet::batm_ijk);
+        reserve(&ParamSet::batm_par);
+        reserve(&ParamSet::bond_hbH);
+        reserve(&ParamSet::bond_ij);
+        reserve(&ParamSet::bond_par);
+        reserve(&ParamSet::brep_ij);
+        reserve(&ParamSet::brep_par);
+        reserve(&ParamSet::chi_base);
+        reserve(&ParamSet::cn_topo);
+        reserve(&ParamSet::cnf);
+        reserve(&ParamSet::frag);
+        reserve(&ParamSet::gam_eff);
+        reserve(&ParamSet::hb_ahb);
+        reserve(&ParamSet::hb_case);
+        reserve(&ParamSet::hb_nbB_idx);
+        reserve(&ParamSet::hb_nbB_off);
+        reserve(&ParamSet::hb_nbC_idx);
+        reserve(&ParamSet::hb_nbC_off);
+        reserve(&ParamSet::hb_par);
+        reserve(&ParamSet::hb_parentC);
+        reserve(&ParamSet::hb_rad);
+        reserve(&ParamSet::hbcn_pair);
+        reserve(&ParamSet::hyb);
+        reserve(&ParamSet::inv_ijkl);
+        reserve(&ParamSet::inv_par);
+        reserve(&ParamSet::metal);
+        reserve(&ParamSet::pi_frag);
+        reserve(&ParamSet::qa);
+        reserve(&ParamSet::qfrag);
+        reserve(&ParamSet::rcov_cn);
+        reserve(&ParamSet::rcov_d3);
+        reserve(&ParamSet::rep_anb);
+        reserve(&ParamSet::repz);
+        reserve(&ParamSet::sqrt_zr4r2);
+        reserve(&ParamSet::stors_ijkl);
+        reserve(&ParamSet::stors_par);
+        reserve(&ParamSet::topo_bonds);
+        reserve(&ParamSet::topo_dist);
+        reserve(&ParamSet::topo_idx);
+        reserve(&ParamSet::topo_off);
+        reserve(&ParamSet::tors_ijkl);
+        reserve(&ParamSet::tors_par);
+        reserve(&ParamSet::zeta);
+    }
     B.topo_off.push_back(0);
     B.hb_nbB_off.push_back(0);
     B.hb_nbC_off.push_back(0);
@@ -1602,10 +1876,6 @@ int merge_batch(const std::vector<ParamSet>& mols, ParamSet& B, std::string& msg
     auto seg = [&](int k, int m) -> int32_t& { return B.mol_seg[(size_t)k * (nm + 1) + m]; };
     for (int m = 0; m < nm; ++m) {
         const ParamSet& P = mols[m];
-        if (!P.inv_par.empty()) {
-            msg = "batched molecules with inversion terms are not supported";
-            return kErrUnsupported;
-        }
         B.mol_off[m] = aoff;
         seg(0, m) = (int32_t)B.bond_hbH.size();
         seg(1, m) = (int32_t)(B.angle_par.size() / GFNFF_ANGLE_NPAR);
@@ -1614,6 +1884,8 @@ int merge_batch(const std::vector<ParamSet>& mols, ParamSet& B, std::string& msg
         seg(4, m) = (int32_t)B.hb_case.size();
         seg(5, m) = (int32_t)B.atm_scale.size();
         seg(6, m) = (int32_t)(B.batm_par.size() / GFNFF_BATM_NPAR);
+        seg(7, m) = (int32_t)(B.inv_par.size() / GFNFF_INV_NPAR);
+        seg(8, m) = (int32_t)B.stors_par.size();
         append(B.Z, P.Z);
         for (int i = 0; i < P.n; ++i) B.elem.push_back(seen[P.Z[i]]);
         append_shift(B.frag, P.frag, foff);
@@ -1646,6 +1918,10 @@ int merge_batch(const std::vector<ParamSet>& mols, ParamSet& B, std::string& msg
         append(B.angle_par, P.angle_par);
         append_shift(B.tors_ijkl, P.tors_ijkl, aoff);
         append(B.tors_par, P.tors_par);
+        append_shift(B.inv_ijkl, P.inv_ijkl, aoff);
+        append(B.inv_par, P.inv_par);
+        append_shift(B.stors_ijkl, P.stors_ijkl, aoff);
+        append(B.stors_par, P.stors_par);
         append_shift(B.brep_ij, P.brep_ij, aoff);
         append(B.brep_par, P.brep_par);
         append_shift(B.hb_ahb, P.hb_ahb, aoff);
@@ -1672,6 +1948,8 @@ int merge_batch(const std::vector<ParamSet>& mols, ParamSet& B, std::string& msg
     seg(4, nm) = (int32_t)B.hb_case.size();
     seg(5, nm) = (int32_t)B.atm_scale.size();
     seg(6, nm) = (int32_t)(B.batm_par.size() / GFNFF_BATM_NPAR);
+    seg(7, nm) = (int32_t)(B.inv_par.size() / GFNFF_INV_NPAR);
+    seg(8, nm) = (int32_t)B.stors_par.size();
     B.n = aoff;
     B.nfrag = foff;
     return kOk;
@@ -1715,11 +1993,6 @@ int build_params(int n, const int32_t* Z, const double* xyz, int charge, const i
         int z = Z[i];
         t.is_metal[i] = metal_type[z - 1] > 0 || (z >= 21 && z <= 30) || (z >= 39 && z <= 48) || (z >= 57 && z <= 80);
     }
-    for (int i = 0; i < n; ++i)
-        if (t.pi[i] > 0) {
-            msg = "molecules with pi systems need the Hueckel bond orders, which are not built yet";
-            return kErrUnsupported;
-        }
     if (!phase1_charges(m, t, opt.threads, msg)) return kErrEEQ;
     t.dxi = calc_dxi(m, t);
     t.alpeeq = calc_alpeeq(m, t);
@@ -1728,8 +2001,12 @@ int build_params(int n, const int32_t* Z, const double* xyz, int charge, const i
         std::vector<bool> pia = eeq_pi_atoms(m, t);
         t.amide_h = eeq_amide_h(m, t, eeq_amide_n(m, t, pia));
     }
-    std::map<std::pair<int, int>, double> pibo;  // empty: no pi systems
-    std::vector<double> pibo_b(t.bonds.size(), 0.0);
+    std::vector<int> ipis;
+    if (!pi_system_charges(m, t, opt.threads, ipis, msg)) return kErrEEQ;
+    std::vector<double> pibo_b = hueckel_pibo(m, t, ipis);
+    std::map<std::pair<int, int>, double> pibo;  // bonds of the pi systems only
+    for (size_t b = 0; b < t.bonds.size(); ++b)
+        if (pibo_b[b] != 0.0) pibo[{std::min(t.bonds[b].first, t.bonds[b].second), std::max(t.bonds[b].first, t.bonds[b].second)}] = pibo_b[b];
     bond_types(m, t, pibo_b);
     topo_neighbourhood(n, t);
 
@@ -1807,7 +2084,8 @@ int build_params(int n, const int32_t* Z, const double* xyz, int charge, const i
         for (int j = i + 1; j < n; ++j)
             if (m.dist(i, j) < 1.3 * (rad_bohr(Z[i]) + rad_bohr(Z[j]))) pbonds.emplace_back(i, j);
     for (auto& b : pbonds) {
-        BondPar bp = bond_params(m, t, b.first, b.second, 0.0);
+        auto pit = pibo.find(b);
+        BondPar bp = bond_params(m, t, b.first, b.second, pit == pibo.end() ? 0.0 : pit->second);
         P.bond_ij.push_back(b.first);
         P.bond_ij.push_back(b.second);
         double v[8] = {bp.r0b_i, bp.r0b_j, bp.cnfak_i, bp.cnfak_j, bp.rabshift, bp.ff, bp.alpha, bp.fc};
@@ -1901,6 +2179,87 @@ int build_params(int n, const int32_t* Z, const double* xyz, int charge, const i
         }
         P.tors_ijkl.insert(P.tors_ijkl.end(), extra_ijkl.begin(), extra_ijkl.end());
         P.tors_par.insert(P.tors_par.end(), extra_par.begin(), extra_par.end());
+    }
+
+    // --- inversions (generateInversionsNative, gfnff_inversions.cpp:756-870): one
+    // term per three-coordinate pi atom or non-pi nitrogen.  The neighbours are
+    // ordered by the reference's partial selection sort on the centre distances
+    // (Fortran ssort): position p takes the last entry of the non-increasing run
+    // starting at p, and the pass stops at the first p whose run has length one.
+    for (int i = 0; i < n; ++i) {
+        if (t.adj[i].size() != 3) continue;
+        bool pi = t.pi[i] > 0, satn = !pi && Z[i] == 7;
+        if (!pi && !satn) continue;
+        int nb[3] = {t.adj[i][0], t.adj[i][1], t.adj[i][2]};
+        double sd[3] = {m.dist(i, nb[0]), m.dist(i, nb[1]), m.dist(i, nb[2])};
+        for (int p = 0; p < 2; ++p) {
+            int k = p;
+            double pp = sd[p];
+            for (int j = p + 1; j < 3 && !(sd[j] > pp); ++j) {
+                k = j;
+                pp = sd[j];
+            }
+            if (k == p) break;
+            sd[k] = sd[p];
+            sd[p] = pp;
+            std::swap(nb[p], nb[k]);
+        }
+        double fc, om0, type;
+        if (satn) {
+            fc = 0.0;
+            for (int q = 0; q < 3; ++q) fc += 0.60 * std::sqrt(repz[Z[nb[q]] - 1]);
+            om0 = 80.0 * M_PI / 180.0;
+            type = -1.0;
+        } else {
+            int ncarbo = 0, nf = 0;
+            for (int q = 0; q < 3; ++q) {
+                int zq = Z[nb[q]];
+                if (zq == 8 || zq == 16) ++ncarbo;
+                if (zq == 9 || zq == 17 || zq == 35 || zq == 53) ++nf;
+            }
+            double fqq = 1.0 + t.qa[i] * 5.0;
+            double sumppi = 0.0;
+            for (int q = 0; q < 3; ++q) {
+                auto it = pibo.find({std::min(i, nb[q]), std::max(i, nb[q])});
+                if (it != pibo.end()) sumppi += it->second;
+            }
+            double f2 = 1.0 - sumppi * 0.50;
+            fc = 1.05 * f2 * fqq;
+            if ((Z[i] == 5 || Z[i] == 6) && ncarbo > 0) fc *= 38.0;
+            if (Z[i] == 6 && nf > 0 && ncarbo == 0) fc *= 10.0;
+            if (Z[i] == 7 && ncarbo > 0) fc *= 10.0 / f2;
+            om0 = 0.0;
+            type = 0.0;
+        }
+        P.inv_ijkl.insert(P.inv_ijkl.end(), {i, nb[0], nb[1], nb[2]});
+        P.inv_par.insert(P.inv_par.end(), {fc, om0, type});
+    }
+
+    // --- triple-bond torsions (generateSTorsionsNative, gfnff_torsions.cpp:2070-2161):
+    // C(sp2)-C(sp2)-C#C-C(sp2)-C(sp2) chains, the outer sp2 carbons taken as the
+    // largest-index three-coordinate sp2 carbon neighbour
+    for (int i = 0; i < n; ++i) {
+        if (Z[i] != 6 || t.adj[i].size() != 2) continue;
+        for (int nbi : t.adj[i]) {
+            if (nbi <= i || Z[nbi] != 6 || t.adj[nbi].size() != 2) continue;
+            if (m.dist(i, nbi) > 2.37) continue;
+            int jj = -1, kk = -1;
+            for (int a : t.adj[i])
+                if (a != nbi && Z[a] == 6) jj = a;
+            for (int a : t.adj[nbi])
+                if (a != i && Z[a] == 6) kk = a;
+            if (jj < 0 || kk < 0 || t.hyb[jj] != 2 || t.hyb[kk] != 2) continue;
+            auto outer = [&](int c, int excl) {
+                int best = -1;
+                for (int a : t.adj[c])
+                    if (a != excl && t.hyb[a] == 2 && Z[a] == 6 && t.adj[a].size() == 3) best = std::max(best, a);
+                return best;
+            };
+            int ii = outer(jj, i), ll = outer(kk, nbi);
+            if (ii < 0 || ll < 0) continue;
+            P.stors_ijkl.insert(P.stors_ijkl.end(), {ii, jj, kk, ll});
+            P.stors_par.push_back(3.75e-4);
+        }
     }
 
     // --- bonded repulsion over the cached bond list (gfnff_method.cpp:9760-9780)
@@ -2199,6 +2558,9 @@ gfnff_params_view ParamSet::view() const {
     v.ninv = (int32_t)(inv_par.size() / GFNFF_INV_NPAR);
     v.inv_ijkl = inv_ijkl.data();
     v.inv_par = inv_par.data();
+    v.nstors = (int32_t)stors_par.size();
+    v.stors_ijkl = stors_ijkl.data();
+    v.stors_par = stors_par.data();
     v.nbrep = (int32_t)(brep_par.size() / GFNFF_REP_NPAR);
     v.brep_ij = brep_ij.data();
     v.brep_par = brep_par.data();

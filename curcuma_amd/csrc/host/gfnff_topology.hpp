@@ -58,6 +58,8 @@ struct ParamSet {
     std::vector<double> tors_par;
     std::vector<int32_t> inv_ijkl;
     std::vector<double> inv_par;
+    std::vector<int32_t> stors_ijkl;
+    std::vector<double> stors_par;
     std::vector<int32_t> brep_ij;
     std::vector<double> brep_par;
     std::vector<int32_t> hb_ahb, hb_case, hb_nbB_off, hb_nbB_idx, hb_parentC, hb_nbC_off, hb_nbC_idx;
@@ -72,8 +74,8 @@ struct ParamSet {
     std::vector<double> cn_topo;
 
     // batches of independent molecules (nmol > 1): atom offsets and, per term
-    // list (bond, angle, torsion, bonded repulsion, HB, ATM, BATM), the entry
-    // offsets of every molecule ([7][nmol + 1])
+    // list (bond, angle, torsion, bonded repulsion, HB, ATM, BATM, inversion,
+    // triple-bond torsion), the entry offsets of every molecule ([9][nmol + 1])
     int nmol = 1;
     std::vector<int32_t> mol_off, mol_seg;
     int mol_of_atom(int i) const;

@@ -88,6 +88,10 @@ typedef struct gfnff_params_view {
     const int32_t* inv_ijkl;    /* [4*ninv] i = centre */
     const double* inv_par;
 
+    int32_t nstors;             /* triple-bond torsions (undamped) */
+    const int32_t* stors_ijkl;  /* [4*nstors] */
+    const double* stors_par;    /* [nstors] erefhalf: E = erefhalf*(1 - cos 2 phi) */
+
     int32_t nbrep;
     const int32_t* brep_ij;
     const double* brep_par;

@@ -350,17 +350,39 @@ static double dihedral_grad(const double* ri, const double* rj, const double* rk
     return phi;
 }
 
-/* out-of-plane angle (gfnff_geometry.h:207-300): angle between the i->j bond
- * and the plane (i,k,l) normal construction of the reference */
-static double oop_angle(const double* rc, const double* r1, const double* r2, const double* r3, double g[4][3]) {
-    /* omega = asin( n . e_c1 ), n = unit normal of plane spanned by (r2-r1),(r3-r1)?
-       Not used by the sp3 chemistry currently supported; left unimplemented. */
-    (void)rc;
-    (void)r1;
-    (void)r2;
-    (void)r3;
+/* out-of-plane angle of the centre i against the plane through its first
+ * neighbour j (calculateOutOfPlaneAngle, gfnff_geometry.h:207-300):
+ * n = (ri - rj) x (rk - rj), v = rl - ri, omega = asin(n.v / (|n||v|)).
+ * Gradient from d(n.v/(|n||v|)) = gn.dn + gv.dv with gn = v/(|n||v|) - s n/|n|^2,
+ * gv = n/(|n||v|) - s v/|v|^2 and d(a.(e x d)) = (d x a).de + (a x e).dd. */
+static double oop_angle(const double* ri, const double* rj, const double* rk, const double* rl, double g[4][3]) {
+    double re[3], rd[3], rv[3], nn[3];
+    sub3(ri, rj, re);
+    sub3(rk, rj, rd);
+    sub3(rl, ri, rv);
+    cross3(re, rd, nn);
+    double N = norm3(nn), V = norm3(rv);
     memset(g, 0, sizeof(double) * 12);
-    return 0.0;
+    if (N < 1e-10 || V < 1e-10) return 0.0;
+    double s = dot3(nn, rv) / (N * V);
+    if (s > 1.0) s = 1.0;
+    if (s < -1.0) s = -1.0;
+    double om = asin(s), co = cos(om);
+    if (fabs(N * V * co) < 1e-14) return om;
+    double gn[3], gv[3], dre[3], drd[3];
+    for (int c = 0; c < 3; ++c) {
+        gn[c] = rv[c] / (N * V) - s * nn[c] / (N * N);
+        gv[c] = nn[c] / (N * V) - s * rv[c] / (V * V);
+    }
+    cross3(rd, gn, dre);
+    cross3(gn, re, drd);
+    for (int c = 0; c < 3; ++c) {
+        g[0][c] = (dre[c] - gv[c]) / co;
+        g[1][c] = (-dre[c] - drd[c]) / co;
+        g[2][c] = drd[c] / co;
+        g[3][c] = gv[c] / co;
+    }
+    return om;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -537,11 +559,66 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
             gadd(&G, GT, id[3], -f2 * v[2][0], -f2 * v[2][1], -f2 * v[2][2]);
         }
     }
-    if (P->ninv > 0) {
-        double gg[4][3];
-        (void)oop_angle(x, x, x, x, gg);
-        status = 2; /* inversions not restated yet */
-        goto done;
+    /* --- inversions: calcInversions ff_workspace_gfnff.cpp:489-604 (damping
+       over the three distances from the first neighbour j) */
+    for (int v = 0; v < P->ninv; ++v) {
+        const int* id = P->inv_ijkl + 4 * v;
+        const double* p = P->inv_par + 3 * v;
+        double gom[4][3];
+        double om = oop_angle(x + 3 * id[0], x + 3 * id[1], x + 3 * id[2], x + 3 * id[3], gom);
+        double rc[4];
+        for (int a = 0; a < 4; ++a) rc[a] = P->rcov_d3[id[a]] * (4.0 / 3.0);
+        double dv[3][3], r2[3], dmp[3], rr[3];
+        const int other[3] = {0, 2, 3};
+        for (int s2 = 0; s2 < 3; ++s2) {
+            int o = other[s2];
+            sub3(x + 3 * id[1], x + 3 * id[o], dv[s2]);
+            r2[s2] = dot3(dv[s2], dv[s2]);
+            double rcut = 0.505 * (rc[1] + rc[o]) * (rc[1] + rc[o]);
+            rr[s2] = (r2[s2] / rcut) * (r2[s2] / rcut);
+            dmp[s2] = 1.0 / (1.0 + rr[s2]);
+        }
+        double damp = dmp[0] * dmp[1] * dmp[2];
+        double V = p[0], et, dEdom;
+        if (p[2] == 0.0) {
+            et = V * (1.0 - cos(om));
+            dEdom = V * sin(om) * damp;
+        } else {
+            double df = cos(om) - cos(p[1]);
+            et = V * df * df;
+            dEdom = -2.0 * V * sin(om) * df * damp;
+        }
+        T.e[GFNFF_E_INVERSION] += et * damp;
+        if (want_grad) {
+            for (int a = 0; a < 4; ++a)
+                gadd(&G, GT, id[a], dEdom * gom[a][0], dEdom * gom[a][1], dEdom * gom[a][2]);
+            double f[3];
+            for (int s2 = 0; s2 < 3; ++s2) {
+                double dd = (r2[s2] < 1e-8) ? 0.0 : -4.0 * rr[s2] / (r2[s2] * (1.0 + rr[s2]) * (1.0 + rr[s2]));
+                f[s2] = et * dd * dmp[(s2 + 1) % 3] * dmp[(s2 + 2) % 3];
+            }
+            double gj[3] = {0, 0, 0};
+            for (int s2 = 0; s2 < 3; ++s2) {
+                int o = other[s2];
+                gadd(&G, GT, id[o], -f[s2] * dv[s2][0], -f[s2] * dv[s2][1], -f[s2] * dv[s2][2]);
+                for (int c = 0; c < 3; ++c) gj[c] += f[s2] * dv[s2][c];
+            }
+            gadd(&G, GT, id[1], gj[0], gj[1], gj[2]);
+        }
+    }
+
+    /* --- triple-bond torsions: calcSTorsions ff_workspace_gfnff.cpp:611-648 (undamped) */
+    for (int t = 0; t < P->nstors; ++t) {
+        const int* id = P->stors_ijkl + 4 * t;
+        double eh = P->stors_par[t];
+        double gphi[4][3];
+        int ok;
+        double phi = dihedral_grad(x + 3 * id[0], x + 3 * id[1], x + 3 * id[2], x + 3 * id[3], gphi, &ok);
+        T.e[GFNFF_E_STORS] += eh - eh * cos(2.0 * phi);
+        if (want_grad && ok) {
+            double d = 2.0 * eh * sin(2.0 * phi);
+            for (int a = 0; a < 4; ++a) gadd(&G, GT, id[a], d * gphi[a][0], d * gphi[a][1], d * gphi[a][2]);
+        }
     }
 
     /* --- D4 dispersion pairs: calcD4Dispersion ff_workspace_gfnff.cpp:715-767 */
@@ -699,15 +776,74 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
                 outl_nb *= (2.0 / (1.0 + exp(-ex)) - 1.0);
             }
         double rdamp = (cs >= 2) ? denv * (1.8 / (rHB * rHB * rHB) - 0.8 / (rAB * rAB * rAB)) : denv / (rAB * rAB * rAB);
-        if (cs == 3 || cs == 4) {
-            status = 2; /* case 3/4 need carbonyl/imine acceptors: not in the current chemistry */
-            goto done;
+        /* case 4 (sp2 N acceptor): virtual lone pair opposite the neighbour sum,
+           ff_workspace_gfnff.cpp:1002-1028 */
+        double outl_lp = 1.0, lpd = 0.0, lpv[3] = {0, 0, 0}, lp[3] = {pB[0], pB[1], pB[2]}, xlp = 0.0;
+        int nlp = 0;
+        if (cs == 4) {
+            lpd = 0.50 - 0.018 * P->repz[B];
+            for (int k = nb0; k < nb1; ++k)
+                for (int c = 0; c < 3; ++c) lpv[c] += x[3 * P->hb_nbB_idx[k] + c] - pB[c];
+            double vn = norm3(lpv);
+            nlp = nb1 - nb0;
+            if (vn > 1e-10 && nlp > 0) {
+                for (int c = 0; c < 3; ++c) lp[c] = pB[c] - lpd * lpv[c] / vn;
+                double d[3];
+                sub3(pA, lp, d);
+                xlp = (56.0 / radab) * ((norm3(d) + lpd + 1e-12) / rAB - 1.0);
+                outl_lp = 2.0 / (1.0 + exp(xlp));
+            } else {
+                nlp = 0;
+            }
         }
-        double qhoutl = QH * doutl * outl_nb;
-        double gscale = (cs == 2 || cs == 4) ? 0.268 : 1.0;
+        /* case 3 (carbonyl O acceptor): H...O=C bend and D-C=O...H torsion
+           factors, ff_workspace_gfnff.cpp:1046-1135 */
+        double eangl = 1.0, etors = 1.0, ga3[3][3];
+        int C3 = -1, c0i = 0, c1i = 0;
+        double et_k[16], gt_k[16][4][3];
+        if (cs == 3 && P->hb_parentC[h] >= 0) {
+            C3 = P->hb_parentC[h];
+            const double cc0 = cos(120.0 * M_PI / 180.0);
+            const double kb = (1.0 - 0.20) / ((1.0 - cc0) * (1.0 - cc0));
+            double u[3], w[3];
+            sub3(x + 3 * C3, pB, u);  /* C - B */
+            sub3(pH, pB, w);          /* H - B */
+            double ru = norm3(u), rw = norm3(w);
+            double ca = dot3(u, w) / (ru * rw + 1e-14);
+            if (ca > 1.0) ca = 1.0;
+            if (ca < -1.0) ca = -1.0;
+            double ea = kb * (ca - cc0) * (ca - cc0);
+            eangl = 1.0 - ea;
+            double dea = 2.0 * kb * (ca - cc0); /* d ea / d cos */
+            for (int c = 0; c < 3; ++c) {
+                double dcu = (w[c] / rw - ca * u[c] / ru) / ru, dcw = (u[c] / ru - ca * w[c] / rw) / rw;
+                ga3[1][c] = -dea * dcu;             /* d eangl / d C */
+                ga3[2][c] = -dea * dcw;             /* d eangl / d H */
+                ga3[0][c] = -ga3[1][c] - ga3[2][c]; /* d eangl / d B */
+            }
+            c0i = P->hb_nbC_off[h];
+            c1i = P->hb_nbC_off[h + 1];
+            if (c1i - c0i > 16) {
+                status = 1;
+                goto done;
+            }
+            for (int k = c0i; k < c1i; ++k) {
+                int D = P->hb_nbC_idx[k];
+                int ok;
+                double phi = dihedral_grad(x + 3 * D, pB, x + 3 * C3, pH, gt_k[k - c0i], &ok);
+                double c1 = 2.0 * (phi - M_PI / 2.0) + M_PI, fct = (1.0 - 0.94) / 2.0;
+                et_k[k - c0i] = (1.0 + cos(c1)) * fct + 0.94;
+                double dij = ok ? -2.0 * sin(c1) * fct : 0.0;
+                for (int a = 0; a < 4; ++a)
+                    for (int c = 0; c < 3; ++c) gt_k[k - c0i][a][c] *= dij;
+                etors *= et_k[k - c0i];
+            }
+        }
+        double qhoutl = QH * doutl * outl_nb * outl_lp;
+        double gscale = (cs == 2 || cs == 4) ? 0.268 : (cs == 3 ? 0.350 : 1.0);
         double EHB;
         double cval = aciA * basB * QA * QB * gscale;
-        if (cs >= 2) EHB = -rdamp * qhoutl * cval;
+        if (cs >= 2) EHB = -rdamp * qhoutl * cval * eangl * etors;
         else EHB = -bas * aci * rdamp * qhoutl;
         T.e[GFNFF_E_HBOND] += EHB;
         if (!want_grad) continue;
@@ -726,8 +862,9 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
         double ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0}, gh[3] = {0, 0, 0};
         if (cs >= 2) {
             double rbhdamp = denv * 1.8 / (rbh2 * rHB), rabdamp = denv * -0.8 / (rab2 * rAB);
-            double dterm = -qhoutl * cval, aterm = -rdamp * QH * outl_nb * cval;
-            double nbterm = -rdamp * QH * doutl * cval;
+            double ct = cval * eangl * etors;
+            double dterm = -qhoutl * ct, aterm = -rdamp * QH * outl_nb * outl_lp * ct;
+            double nbterm = -rdamp * QH * doutl * outl_lp * ct;
             double gi = ((rabdamp + rbhdamp) * ddamp - 3.0 * rabdamp) / rab2 * dterm;
             for (int c = 0; c < 3; ++c) {
                 ga[c] = gi * drab[c];
@@ -772,6 +909,47 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
                     gnb[c] = -dga - dgb;
                 }
                 gadd(&G, GH, nb, gnb[0], gnb[1], gnb[2]);
+            }
+            if (cs == 4 && nlp > 0) {
+                /* d outl_lp through |A - lp| + lpd over rAB; lp = B - lpd v/|v|,
+                   v = sum_nb (x_nb - x_B) */
+                double lpterm = -rdamp * QH * doutl * outl_nb * cval;
+                double dal[3];
+                sub3(pA, lp, dal);
+                double ral = norm3(dal), sl = ral + lpd + 1e-12;
+                double el = exp(xlp);
+                double dEds = -2.0 * lpterm * el / ((1.0 + el) * (1.0 + el)) * (56.0 / radab) / rAB;
+                double dEdrab = 2.0 * lpterm * el / ((1.0 + el) * (1.0 + el)) * (56.0 / radab) * sl / (rAB * rAB);
+                double glp[3];
+                for (int c = 0; c < 3; ++c) {
+                    ga[c] += dEdrab * drab[c] / rAB + dEds * dal[c] / (ral + 1e-12);
+                    gb[c] -= dEdrab * drab[c] / rAB;
+                    glp[c] = -dEds * dal[c] / (ral + 1e-12); /* dE/dlp */
+                }
+                /* lp = B - lpd v/|v|: dlp/dB = I + lpd nlp P/|v|, dlp/dx_nb = -lpd P/|v|,
+                   P = I - v v^T/|v|^2 */
+                double vn = norm3(lpv), vg = dot3(lpv, glp);
+                double pg[3];
+                for (int c = 0; c < 3; ++c) pg[c] = (glp[c] - lpv[c] * vg / (vn * vn)) * lpd / vn;
+                for (int c = 0; c < 3; ++c) gb[c] += glp[c] + nlp * pg[c];
+                for (int k = nb0; k < nb1; ++k) gadd(&G, GH, P->hb_nbB_idx[k], -pg[0], -pg[1], -pg[2]);
+            }
+            if (C3 >= 0) {
+                double bterm = -rdamp * qhoutl * etors * cval, tterm = -rdamp * qhoutl * eangl * cval;
+                for (int c = 0; c < 3; ++c) gb[c] += bterm * ga3[0][c];
+                gadd(&G, GH, C3, bterm * ga3[1][0], bterm * ga3[1][1], bterm * ga3[1][2]);
+                for (int c = 0; c < 3; ++c) gh[c] += bterm * ga3[2][c];
+                for (int k = c0i; k < c1i; ++k) {
+                    double f = 1.0;
+                    for (int k2 = c0i; k2 < c1i; ++k2)
+                        if (k2 != k) f *= et_k[k2 - c0i];
+                    f *= tterm;
+                    const double(*gk)[3] = gt_k[k - c0i];
+                    gadd(&G, GH, P->hb_nbC_idx[k], f * gk[0][0], f * gk[0][1], f * gk[0][2]);
+                    for (int c = 0; c < 3; ++c) gb[c] += f * gk[1][c];
+                    gadd(&G, GH, C3, f * gk[2][0], f * gk[2][1], f * gk[2][2]);
+                    for (int c = 0; c < 3; ++c) gh[c] += f * gk[3][c];
+                }
             }
         } else {
             double caa = QA * basA, cbb = QB * basB;

@@ -47,14 +47,28 @@ def test_nan_geometry_is_reported():
     assert "non-finite" in m.getErrorMessage()
 
 
-def test_pi_systems_are_refused_loudly():
-    """Molecules with pi systems need Hueckel bond orders, not built yet: the
-    engine refuses them instead of returning wrong numbers."""
+def test_unsupported_systems_are_refused_loudly():
+    """A charged system of exactly two bonded fragments (the reference
+    distributes the charge by an extra fragment EEQ that is not built here) is
+    refused with a message instead of returning wrong numbers."""
+    from curcuma_amd.workloads import water_cluster
+    Z, x = water_cluster(2, seed=1)
+    m = GFNFFMethod()
+    assert not m.setMolecule(Mol(Z, x, charge=1))
+    assert "two-fragment" in m.getErrorMessage()
+
+
+def test_pi_system_topology():
+    """Benzene: one pi system over the six carbons, hydrogens outside it."""
     from conftest import BOHR, load_fixture
     fx = load_fixture("C6H6")
+    Z = np.array(fx["Z"])
     m = GFNFFMethod()
-    assert not m.setMolecule(Mol(np.array(fx["Z"]), np.array(fx["xyz_bohr"]) * BOHR))
-    assert "pi" in m.getErrorMessage()
+    assert m.setMolecule(Mol(Z, np.array(fx["xyz_bohr"]) * BOHR)), m.getErrorMessage()
+    hyb, pi, bonds = m.topology()
+    assert set(pi[Z == 6]) == {1} and set(pi[Z == 1]) == {0}
+    assert set(hyb[Z == 6]) == {2}
+    assert m.params_view().ninv == 6
 
 
 def test_topology_of_water_and_alkane():

@@ -9,7 +9,7 @@ import pytest
 
 from conftest import BOHR, gpu_available
 from curcuma_amd import GFNFFBatch, GFNFFMethod, Mol
-from curcuma_amd.workloads import alkane_batch, water_cluster
+from curcuma_amd.workloads import alkane_batch, mixed_batch, water_cluster
 
 pytestmark = pytest.mark.gpu
 
@@ -47,6 +47,30 @@ def test_alkane_batch_matches_single_molecules(precision, oracle):
         if k == 0 and precision == "fp64":
             ref = oracle.eval(m1.params_view(), np.asarray(xyz[a0:a1]) * (1.0 / BOHR))
             assert abs(em[k] - ref["E"]) <= 1e-9 * max(1.0, abs(ref["E"]))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_alkane_peptide_batch(precision, oracle):
+    """Alkanes and glycine oligopeptides (pi systems, inversions, hydrogen-bond
+    cases 2/3) in one batch, each molecule against its single evaluation."""
+    off, Z, xyz = mixed_batch(nmol=10, nmin=20, nmax=200, seed=17)
+    b = GFNFFBatch(precision=precision)
+    assert b.setBatch(off, Z, xyz), b.getErrorMessage()
+    b.calculateEnergy(True)
+    assert not b.hasError(), b.getErrorMessage()
+    em, g = b.getEnergies(), b.getGradient()
+    tol = 1e-9 if precision == "fp64" else 1e-6
+    npep = 0
+    for k in range(len(off) - 1):
+        a0, a1 = off[k], off[k + 1]
+        e1, g1, m1 = single(Z[a0:a1], xyz[a0:a1], precision)
+        npep += int(m1.params_view().ninv > 0)
+        assert abs(em[k] - e1) <= tol * max(1.0, abs(e1) / 10), k
+        assert np.abs(g[a0:a1] - g1).max() <= tol, k
+        if precision == "fp64" and m1.params_view().ninv > 0:
+            ref = oracle.eval(m1.params_view(), np.asarray(xyz[a0:a1]) * (1.0 / BOHR))
+            assert abs(em[k] - ref["E"]) <= 1e-9 * max(1.0, abs(ref["E"]))
+    assert npep > 0
 
 
 def test_mixed_chemistry_batch():

@@ -11,7 +11,7 @@ import pytest
 
 from conftest import BOHR, gpu_available
 from curcuma_amd import GFNFFMethod, Mol
-from curcuma_amd.workloads import alkane, water_cluster
+from curcuma_amd.workloads import alkane, glycine_peptide, water_cluster
 
 pytestmark = pytest.mark.gpu
 
@@ -27,12 +27,14 @@ def evaluate(Z, xyz, precision):
     return m, e
 
 
-@pytest.mark.parametrize("case", ["water27", "alkane40", "water512"])
+@pytest.mark.parametrize("case", ["water27", "alkane40", "water512", "gly20"])
 def test_mixed_against_fp64_and_oracle(case, oracle):
     if case == "water27":
         Z, xyz = water_cluster(27, seed=5)
     elif case == "alkane40":
         Z, xyz = alkane(40, sigma=0.02, seed=1)
+    elif case == "gly20":
+        Z, xyz = glycine_peptide(20, sigma=0.03, seed=6)
     else:
         Z, xyz = water_cluster(512, seed=3)
     n = len(Z)

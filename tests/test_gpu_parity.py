@@ -11,7 +11,8 @@ import pytest
 
 from conftest import BOHR, gpu_available, load_fixture
 from curcuma_amd import GFNFFMethod, Mol
-from curcuma_amd.workloads import alkane, water_cluster
+from curcuma_amd.workloads import alkane, glycine_peptide, water_cluster
+from test_oracle_pi import hexadienyne
 
 pytestmark = pytest.mark.gpu
 
@@ -43,8 +44,11 @@ def check(m, e, ref, n, etol=1e-9, gtol=1e-9):
     assert np.abs(m.getCN() - ref["cn"]).max() <= 1e-12
 
 
-@pytest.mark.parametrize("name", ["H2O", "CH4", "CH3OH", "CH3OCH3", "HCl", "triose"])
+@pytest.mark.parametrize("name", ["H2O", "CH4", "CH3OH", "CH3OCH3", "HCl", "triose", "HCN", "O3", "C6H6",
+                                  "acetic_acid_dimer", "caffeine", "complex"])
 def test_fixture_molecules(name, oracle):
+    """Includes the pi fixtures: inversions (planar and saturated N) and
+    hydrogen bonds of cases 3 (carbonyl acceptor) and 4 (sp2 N lone pair)."""
     fx = load_fixture(name)
     Z = np.array(fx["Z"])
     xyz = np.array(fx["xyz_bohr"]) * BOHR
@@ -55,6 +59,21 @@ def test_fixture_molecules(name, oracle):
 def test_alkane_c40h82(oracle):
     Z, xyz = alkane(40, sigma=0.02, seed=1)
     m, e, ref = run_both(Z, xyz, oracle)
+    check(m, e, ref, len(Z))
+
+
+def test_glycine_peptide(oracle):
+    """Amide pi systems, inversions and intramolecular hydrogen bonds."""
+    Z, xyz = glycine_peptide(12, sigma=0.03, seed=2)
+    m, e, ref = run_both(Z, xyz, oracle)
+    assert m.params_view().ninv == 24
+    check(m, e, ref, len(Z))
+
+
+def test_triple_bond_torsion(oracle):
+    Z, xyz = hexadienyne(37.0)
+    m, e, ref = run_both(Z, xyz, oracle)
+    assert m.params_view().nstors == 1 and abs(m._terms[12]) > 1e-5
     check(m, e, ref, len(Z))
 
 
