@@ -39,6 +39,7 @@ class ParamsView(C.Structure):
         ("nhb", C.c_int32), ("hb_ahb", _i32p), ("hb_case", _i32p), ("hb_par", _f64p),
         ("hb_nbB_off", _i32p), ("hb_nbB_idx", _i32p), ("hb_parentC", _i32p), ("hb_nbC_off", _i32p),
         ("hb_nbC_idx", _i32p),
+        ("nxb", C.c_int32), ("xb_axb", _i32p), ("xb_par", _f64p),
         ("natm", C.c_int32), ("atm_ijk", _i32p), ("atm_scale", _f64p),
         ("nbatm", C.c_int32), ("batm_ijk", _i32p), ("batm_par", _f64p),
     ]

@@ -1612,8 +1612,8 @@ __device__ double hb_eval(int h, int s0, const HBArgs& a, const double* __restri
     double3 pA = ld3(x, A), pH = ld3(x, H), pB = ld3(x, B);
     double rAH = nrm(pH - pA), rHB = nrm(pB - pH), rAB = nrm(pB - pA);
     const int nb0 = a.nb_off[h], nb1 = a.nb_off[h + 1];
-    const int c0 = (cs == 3) ? a.nbc_off[h] : 0, c1 = (cs == 3) ? a.nbc_off[h + 1] : 0;
     const int C3 = (cs == 3) ? a.parentC[h] : -1;
+    const int c0 = (C3 >= 0) ? a.nbc_off[h] : 0, c1 = (C3 >= 0) ? a.nbc_off[h + 1] : 0;
     const int sD = s0 + 3 + (nb1 - nb0);  // first slot of the D atoms
     double3 z3 = make_double3(0, 0, 0);
     double3 ga = z3, gb = z3, gh = z3;
@@ -1803,6 +1803,54 @@ __device__ double hb_eval(int h, int s0, const HBArgs& a, const double* __restri
         for (int k = c0; k < c1; ++k) st3(slots, sD + (k - c0), z3);
     }
     return E;
+}
+
+// halogen bonds A-X...B: ff_workspace_gfnff.cpp:1411-1519.  The gradient is
+// the reference's: its short- and long-range damping derivatives
+// (-12 f_s t_s / (r (1 + t_s)) and -12 r f_l t_l / (70 (1 + t_l))) are kept as
+// they are, so along r_XB it is not the derivative of the energy.
+__global__ __launch_bounds__(256) void k_xb(int nx, const int* __restrict__ axb, const double* __restrict__ par,
+                                            const int* __restrict__ slot0, const double* __restrict__ x,
+                                            const double* __restrict__ hb_rad, double* __restrict__ slots,
+                                            double* __restrict__ e_out) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nx) return;
+    int A = axb[3 * b], X = axb[3 * b + 1], B = axb[3 * b + 2];
+    const double* p = par + 3 * b;
+    double3 pA = ld3(x, A), pX = ld3(x, X), pB = ld3(x, B);
+    double3 vax = pX - pA, vxb = pB - pX, vab = pB - pA;
+    double rax = nrm(vax), rxb = nrm(vxb), rab = nrm(vab);
+    double3 z3 = make_double3(0, 0, 0), gA = z3, gX = z3, gB = z3;
+    double E = 0.0;
+    double radab = hb_rad[A] + hb_rad[B];
+    double ex = 70.0 * ((rax + rxb) / rab - 1.0);
+    if (!(rxb > 20.0) && !(ex > 15.0)) {
+        double ts = pow(5.0 * radab / (rxb * rxb), 6.0), tl = pow(rxb * rxb / 70.0, 6.0);
+        double fs = 1.0 / (1.0 + ts), fl = 1.0 / (1.0 + tl);
+        double fo = 2.0 / (1.0 + exp(ex));
+        double eX = exp(15.0 * p[1]), eB = exp(15.0 * -p[2]);
+        double QX = eX / (eX + 0.03), QB = eB / (eB + 0.03);
+        double r3 = rxb * rxb * rxb;
+        double Rd = fs * fl * fo / r3;
+        double pre = -QB * p[0] * QX;
+        E = pre * Rd;
+        double dfs = -2.0 * 6.0 * fs * ts / (rxb * (1.0 + ts));
+        double dfl = -2.0 * 6.0 * rxb * fl * tl / (70.0 * (1.0 + tl));
+        double et = exp(ex), de = (1.0 + et) * (1.0 + et);
+        double dfo_ax = -2.0 * et * 70.0 / (rab * de);
+        double dfo_ab = 2.0 * et * 70.0 * (rax + rxb) / (rab * rab * de);
+        double dR_xb = (dfs * fl * fo + fs * dfl * fo + fs * fl * dfo_ax) / r3 - 3.0 * Rd / rxb;
+        double gxb = pre * dR_xb / (rxb + 1e-14), gax = pre * (fs * fl * dfo_ax / r3) / (rax + 1e-14);
+        double gab = pre * (fs * fl * dfo_ab / r3) / (rab + 1e-14);
+        gA = (-gax) * vax + (-gab) * vab;
+        gX = gax * vax + (-gxb) * vxb;
+        gB = gxb * vxb + gab * vab;
+    }
+    e_out[b] = E;
+    int s0 = slot0[b];
+    st3(slots, s0, gA);
+    st3(slots, s0 + 1, gX);
+    st3(slots, s0 + 2, gB);
 }
 
 // one thread per hydrogen bond: energy and the gradient slots of its atoms
@@ -2053,9 +2101,9 @@ __global__ __launch_bounds__(256) void k_eeq_batch_solve(const int* __restrict__
 struct MolEnergyArgs {
     int nmol;
     const int* mol_off;
-    const int* seg;        // [9][nmol + 1]: bond, angle, torsion, brep, hb, atm, batm, inversion, stors
+    const int* seg;        // [10][nmol + 1]: bond, angle, torsion, brep, hb, atm, batm, inversion, stors, xb
     const double* atom_terms[4];   // coul, disp, rep, self
-    const double* list_terms[9];
+    const double* list_terms[10];
     double* e_mol;
 };
 __global__ __launch_bounds__(256) void k_mol_energy(MolEnergyArgs a) {
@@ -2065,7 +2113,7 @@ __global__ __launch_bounds__(256) void k_mol_energy(MolEnergyArgs a) {
     double acc = 0.0;
     for (int k = 0; k < 4; ++k)
         for (int i = a.mol_off[m] + lane; i < a.mol_off[m + 1]; i += 64) acc += a.atom_terms[k][i];
-    for (int k = 0; k < 9; ++k) {
+    for (int k = 0; k < 10; ++k) {
         const int* sg = a.seg + k * (a.nmol + 1);
         for (int i = sg[m] + lane; i < sg[m + 1]; i += 64) acc += a.list_terms[k][i];
     }
@@ -2401,7 +2449,7 @@ struct Segs {
     int dst[16];
 };
 constexpr int RED_BLOCKS = 64;
-constexpr int NPARTS = 13;  // energy segments reduced per evaluation
+constexpr int NPARTS = 14;  // energy segments reduced per evaluation
 // stage 1: segment sg, block b reduces the fixed chunk b of the segment
 __global__ __launch_bounds__(256) void k_reduce1(Segs s, double* __restrict__ part) {
     __shared__ double red[256];
@@ -2434,7 +2482,7 @@ __global__ void k_total(const double* __restrict__ parts, double* __restrict__ o
     double t[GFNFF_NTERMS];
     for (int k = 0; k < GFNFF_NTERMS; ++k) t[k] = 0.0;
     // parts: 0 bond,1 angle,2 tors,3 inv,4 disp(pairs),5 brep,6 nbrep(pairs incl fix),7 coul pairs,
-    // 8 self,9 hb,10 atm,11 batm,12 triple-bond torsions
+    // 8 self,9 hb,10 atm,11 batm,12 triple-bond torsions,13 xb
     t[GFNFF_E_BOND] = parts[0];
     t[GFNFF_E_ANGLE] = parts[1];
     t[GFNFF_E_TORSION] = parts[2];
@@ -2447,6 +2495,7 @@ __global__ void k_total(const double* __restrict__ parts, double* __restrict__ o
     t[GFNFF_E_ATM] = parts[10];
     t[GFNFF_E_BATM] = parts[11];
     t[GFNFF_E_STORS] = parts[12];
+    t[GFNFF_E_XBOND] = parts[13];
     double tot = 0.0;
     for (int k = 0; k < GFNFF_NTERMS; ++k) tot += t[k];
     out[0] = tot;
@@ -2518,7 +2567,7 @@ struct DeviceEngine {
     KernelTimer timer;
     int n = 0, nelem = 0, nfrag = 0;
     int nr = 0, np = 0;                 // reduced EEQ size, padded size
-    int nbond = 0, nangle = 0, ntors = 0, nbrep = 0, nhb = 0, natm = 0, nbatm = 0, ninv = 0, nstors = 0;
+    int nbond = 0, nangle = 0, ntors = 0, nbrep = 0, nhb = 0, natm = 0, nbatm = 0, ninv = 0, nstors = 0, nxb = 0;
     int nslots = 0;
     int pair_cj = 0, pair_nchunk = 0;   // j-chunk size and count of the pair kernel
     bool mixed = false;                 // FP32 pair terms + FP32 factorisation with FP64 refinement
@@ -2549,8 +2598,8 @@ struct DeviceEngine {
     double* ang_par;
     int *tor_ijkl, *tor_slot;
     double* tor_par;
-    int *inv_ijkl, *inv_slot, *st_ijkl, *st_slot;
-    double *inv_par, *st_par;
+    int *inv_ijkl, *inv_slot, *st_ijkl, *st_slot, *xb_axb, *xb_slot;
+    double *inv_par, *st_par, *xb_par;
     int *brep_ij, *brep_slot;
     double* brep_par;
     int *hb_ahb, *hb_case, *hb_nboff, *hb_nbidx, *hbs_off, *hbs_idx, *hb_slot0, *hb_parc, *hb_nbcoff, *hb_nbcidx;
@@ -2564,7 +2613,7 @@ struct DeviceEngine {
     // work
     double *x, *cn, *dlog, *gw, *dgw, *rhs, *Ared, *Linv, *b, *rres, *q, *eself, *qtmp, *gpair, *dEdcn, *vchain,
         *slots, *dcn_slot, *grad, *e_coul, *e_disp, *e_rep, *e_bond, *e_ang, *e_tor, *e_brep, *e_hb, *e_atm,
-        *e_batm, *e_inv, *e_st, *parts, *rpart, *out, *diag, *ghb, *ybuf, *xbuf, *h6, *dh6, *ppart;
+        *e_batm, *e_inv, *e_st, *e_xb, *parts, *rpart, *out, *diag, *ghb, *ybuf, *xbuf, *h6, *dh6, *ppart;
     int *nbn, *nbl, *flags;
 
     PairArgs pair_args() const {
@@ -2836,6 +2885,16 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->st_ijkl = E->upload(P.stors_ijkl);
         E->st_par = E->upload(P.stors_par);
         E->st_slot = E->upload(sslot);
+        E->nxb = (int)(P.xb_par.size() / 3);
+        std::vector<int> xslot(E->nxb);
+        for (int t = 0; t < E->nxb; ++t) {
+            xslot[t] = add_slot(P.xb_axb[3 * t]);
+            add_slot(P.xb_axb[3 * t + 1]);
+            add_slot(P.xb_axb[3 * t + 2]);
+        }
+        E->xb_axb = E->upload(P.xb_axb);
+        E->xb_par = E->upload(P.xb_par);
+        E->xb_slot = E->upload(xslot);
         E->nbrep = (int)(P.brep_par.size() / 2);
         std::vector<int> rslot(E->nbrep);
         for (int b = 0; b < E->nbrep; ++b) {
@@ -2856,7 +2915,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
                 s0[h] = (int)atom_of.size();
                 for (int r = 0; r < 3; ++r) atom_of.push_back(P.hb_ahb[3 * h + r]);
                 for (int k = P.hb_nbB_off[h]; k < P.hb_nbB_off[h + 1]; ++k) atom_of.push_back(P.hb_nbB_idx[k]);
-                if (P.hb_case[h] == 3)
+                if (P.hb_case[h] == 3 && P.hb_parentC[h] >= 0)
                     for (int k = P.hb_nbC_off[h]; k < P.hb_nbC_off[h + 1]; ++k) atom_of.push_back(P.hb_nbC_idx[k]);
                 if (atom_of.size() > (size_t)INT32_MAX / 4)
                     throw std::runtime_error("hydrogen-bond slot count exceeds the index range");
@@ -2981,6 +3040,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->e_tor = E->alloc<double>(std::max(E->ntors, 1));
         E->e_inv = E->alloc<double>(std::max(E->ninv, 1));
         E->e_st = E->alloc<double>(std::max(E->nstors, 1));
+        E->e_xb = E->alloc<double>(std::max(E->nxb, 1));
         E->e_brep = E->alloc<double>(std::max(E->nbrep, 1));
         E->e_hb = E->alloc<double>(std::max(E->nhb, 1));
         E->e_atm = E->alloc<double>(std::max(E->natm, 1));
@@ -3194,6 +3254,9 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     if (E->nstors)
         LAUNCH(KC_TORS, k_stors, dim3(cdiv(E->nstors, 256)), dim3(256), s, E->nstors, E->st_ijkl, E->st_par,
                E->st_slot, E->x, E->slots, E->e_st);
+    if (E->nxb)
+        LAUNCH(KC_HB, k_xb, dim3(cdiv(E->nxb, 256)), dim3(256), s, E->nxb, E->xb_axb, E->xb_par, E->xb_slot, E->x,
+               E->hb_rad, E->slots, E->e_xb);
     if (E->nbrep)
         LAUNCH(KC_BREP, k_brep, dim3(cdiv(E->nbrep, 256)), dim3(256), s, E->nbrep, E->brep_ij, E->brep_par,
                            E->brep_slot, E->x, E->slots, E->e_brep);
@@ -3217,9 +3280,9 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
                        E->nbn, E->nbl, E->slot_off, E->slot_idx, E->slots, E->gpair, E->ghb, E->grad);
     Segs sg;
     const double* ptrs[NPARTS] = {E->e_bond, E->e_ang,  E->e_tor, E->e_inv, E->e_disp, E->e_brep, E->e_rep,
-                                  E->e_coul, E->eself, E->e_hb,  E->e_atm, E->e_batm, E->e_st};
+                                  E->e_coul, E->eself, E->e_hb,  E->e_atm, E->e_batm, E->e_st,  E->e_xb};
     int cnts[NPARTS] = {E->nbond, E->nangle, E->ntors, E->ninv, n, E->nbrep, n, n, n, E->nhb, E->natm, E->nbatm,
-                        E->nstors};
+                        E->nstors, E->nxb};
     for (int k = 0; k < NPARTS; ++k) {
         sg.p[k] = ptrs[k];
         sg.n[k] = cnts[k];
@@ -3237,9 +3300,9 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         ma.atom_terms[1] = E->e_disp;
         ma.atom_terms[2] = E->e_rep;
         ma.atom_terms[3] = E->eself;
-        const double* lists[9] = {E->e_bond, E->e_ang, E->e_tor, E->e_brep, E->e_hb,
-                                  E->e_atm,  E->e_batm, E->e_inv, E->e_st};
-        for (int k = 0; k < 9; ++k) ma.list_terms[k] = lists[k];
+        const double* lists[10] = {E->e_bond, E->e_ang,  E->e_tor, E->e_brep, E->e_hb,
+                                   E->e_atm,  E->e_batm, E->e_inv, E->e_st,   E->e_xb};
+        for (int k = 0; k < 10; ++k) ma.list_terms[k] = lists[k];
         ma.e_mol = E->e_mol;
         LAUNCH(KC_REDUCE, k_mol_energy, dim3(cdiv(E->nmol, 4)), dim3(256), s, ma);
     }

@@ -1802,7 +1802,7 @@ int merge_batch(const std::vector<ParamSet>& mols, ParamSet& B, std::string& msg
     }
     B.nmol = nm;
     B.mol_off.assign(nm + 1, 0);
-    B.mol_seg.assign(9 * (size_t)(nm + 1), 0);
+    B.mol_seg.assign(10 * (size_t)(nm + 1), 0);
     std::vector<int> seen(87, -1);
     for (const auto& P : mols)
         for (int z : P.present_Z)
@@ -1867,6 +1867,8 @@ int merge_batch(const std::vector<ParamSet>& mols, ParamSet& B, std::string& msg
         reserve(&ParamSet::topo_off);
         reserve(&ParamSet::tors_ijkl);
         reserve(&ParamSet::tors_par);
+        reserve(&ParamSet::xb_axb);
+        reserve(&ParamSet::xb_par);
         reserve(&ParamSet::zeta);
     }
     B.topo_off.push_back(0);
@@ -1886,6 +1888,7 @@ int merge_batch(const std::vector<ParamSet>& mols, ParamSet& B, std::string& msg
         seg(6, m) = (int32_t)(B.batm_par.size() / GFNFF_BATM_NPAR);
         seg(7, m) = (int32_t)(B.inv_par.size() / GFNFF_INV_NPAR);
         seg(8, m) = (int32_t)B.stors_par.size();
+        seg(9, m) = (int32_t)(B.xb_par.size() / GFNFF_XB_NPAR);
         append(B.Z, P.Z);
         for (int i = 0; i < P.n; ++i) B.elem.push_back(seen[P.Z[i]]);
         append_shift(B.frag, P.frag, foff);
@@ -1932,6 +1935,8 @@ int merge_batch(const std::vector<ParamSet>& mols, ParamSet& B, std::string& msg
         append_shift(B.hb_parentC, P.hb_parentC, aoff, true);
         append_csr(B.hb_nbC_off, P.hb_nbC_off, (int)B.hb_nbC_idx.size());
         append_shift(B.hb_nbC_idx, P.hb_nbC_idx, aoff);
+        append_shift(B.xb_axb, P.xb_axb, aoff);
+        append(B.xb_par, P.xb_par);
         append_shift(B.atm_ijk, P.atm_ijk, aoff);
         append(B.atm_scale, P.atm_scale);
         append_shift(B.batm_ijk, P.batm_ijk, aoff);
@@ -1950,6 +1955,7 @@ int merge_batch(const std::vector<ParamSet>& mols, ParamSet& B, std::string& msg
     seg(6, nm) = (int32_t)(B.batm_par.size() / GFNFF_BATM_NPAR);
     seg(7, nm) = (int32_t)(B.inv_par.size() / GFNFF_INV_NPAR);
     seg(8, nm) = (int32_t)B.stors_par.size();
+    seg(9, nm) = (int32_t)(B.xb_par.size() / GFNFF_XB_NPAR);
     B.n = aoff;
     B.nfrag = foff;
     return kOk;
@@ -2508,6 +2514,39 @@ int build_params(int n, const int32_t* Z, const double* xyz, int charge, const i
                 P.hbcn_pair.push_back(B);
             }
         }
+
+        // --- halogen (and chalcogen/pnictogen) bonds A-X...B
+        // (detectHalogenBondsNative, gfnff_method.cpp:8520-8662): X in
+        // {P, S (at most two neighbours), Cl, As, Se, Br, Sb, Te, I} bonded to A,
+        // acceptors B within 10 Bohr of X in ascending order with the HB
+        // basicities, carbon acceptors only in pi systems
+        auto xlike = [](int z) {
+            return z == 17 || z == 35 || z == 53 || z == 16 || z == 34 || z == 52 || z == 15 || z == 33 || z == 51;
+        };
+        for (auto& b : t.bonds) {
+            int X = -1, A = -1;
+            if (xlike(Z[b.first])) {
+                X = b.first;
+                A = b.second;
+            } else if (xlike(Z[b.second])) {
+                X = b.second;
+                A = b.first;
+            }
+            if (X < 0) continue;
+            if (Z[X] == 16 && t.adj[X].size() > 2) continue;
+            for (int B = 0; B < n; ++B) {
+                if (m.dist2(X, B) >= 100.0) continue;
+                if (B == A || B == X || cb[B] < 1e-6) continue;
+                if (Z[B] == 6) {
+                    if (t.pi[B] == 0 || q[B] >= 0.05) continue;
+                } else if (q[B] > 0.05) {
+                    continue;
+                }
+                if (t.bonded(X, B)) continue;
+                P.xb_axb.insert(P.xb_axb.end(), {A, X, B});
+                P.xb_par.insert(P.xb_par.end(), {xb_acidity[Z[X]], q[X], q[B]});
+            }
+        }
     }
     return kOk;
 }
@@ -2558,6 +2597,9 @@ gfnff_params_view ParamSet::view() const {
     v.ninv = (int32_t)(inv_par.size() / GFNFF_INV_NPAR);
     v.inv_ijkl = inv_ijkl.data();
     v.inv_par = inv_par.data();
+    v.nxb = (int32_t)(xb_par.size() / GFNFF_XB_NPAR);
+    v.xb_axb = xb_axb.data();
+    v.xb_par = xb_par.data();
     v.nstors = (int32_t)stors_par.size();
     v.stors_ijkl = stors_ijkl.data();
     v.stors_par = stors_par.data();

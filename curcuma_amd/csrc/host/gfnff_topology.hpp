@@ -64,6 +64,8 @@ struct ParamSet {
     std::vector<double> brep_par;
     std::vector<int32_t> hb_ahb, hb_case, hb_nbB_off, hb_nbB_idx, hb_parentC, hb_nbC_off, hb_nbC_idx;
     std::vector<double> hb_par;
+    std::vector<int32_t> xb_axb;
+    std::vector<double> xb_par;
     std::vector<int32_t> atm_ijk;
     std::vector<double> atm_scale;
     std::vector<int32_t> batm_ijk;
@@ -75,7 +77,7 @@ struct ParamSet {
 
     // batches of independent molecules (nmol > 1): atom offsets and, per term
     // list (bond, angle, torsion, bonded repulsion, HB, ATM, BATM, inversion,
-    // triple-bond torsion), the entry offsets of every molecule ([9][nmol + 1])
+    // triple-bond torsion, XB), the entry offsets of every molecule ([10][nmol + 1])
     int nmol = 1;
     std::vector<int32_t> mol_off, mol_seg;
     int mol_of_atom(int i) const;

@@ -30,6 +30,7 @@ extern "C" {
 #define GFNFF_REP_NPAR 2    /* alpha repab */
 #define GFNFF_HB_NPAR 7     /* bas_A bas_B aci_A aci_B q_H q_A q_B */
 #define GFNFF_BATM_NPAR 3   /* zb3atm_i zb3atm_j zb3atm_k */
+#define GFNFF_XB_NPAR 3     /* acidity_X q_X q_B */
 
 typedef struct gfnff_params_view {
     int32_t natoms;
@@ -105,6 +106,10 @@ typedef struct gfnff_params_view {
     const int32_t* hb_parentC;  /* [nhb] acceptor parent C (case 3) or -1 */
     const int32_t* hb_nbC_off;  /* [nhb+1] neighbours of C other than B (case 3) */
     const int32_t* hb_nbC_idx;
+
+    int32_t nxb;                /* halogen/chalcogen/pnictogen bonds A-X...B */
+    const int32_t* xb_axb;      /* [3*nxb] A, X, B */
+    const double* xb_par;       /* [3*nxb] */
 
     int32_t natm;
     const int32_t* atm_ijk;     /* [3*natm] */

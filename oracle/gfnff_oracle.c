@@ -990,6 +990,51 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
         gadd(&G, GH, Hh, gh[0], gh[1], gh[2]);
     }
 
+    /* --- halogen bonds A-X...B: calcHalogenBonds ff_workspace_gfnff.cpp:1411-1519.
+       E = -acid_X Q_X Q_B f_s(r_XB) f_l(r_XB) f_o / r_XB^3 with
+       f_s = 1/(1 + (5 radab / r^2)^6), f_l = 1/(1 + (r^2/70)^6),
+       f_o = 2/(1 + exp(70 ((r_AX + r_XB)/r_AB - 1))) (skipped beyond exponent 15),
+       Q = e^{15 q}/(e^{15 q} + 0.03).  The gradient is the reference's
+       (:1459-1509), whose short- and long-range damping derivatives are
+       -2*6 f_s t_s/(r (1 + t_s)) and -2*6 r f_l t_l/(70 (1 + t_l)) (t = the
+       powered ratios); these are not the derivatives of f_s and f_l, so this
+       gradient differs from finite differences of the energy along r_XB,
+       exactly as the reference's does.  The out-of-line part is exact. */
+    for (int b = 0; b < P->nxb; ++b) {
+        int A = P->xb_axb[3 * b], X = P->xb_axb[3 * b + 1], B = P->xb_axb[3 * b + 2];
+        const double* p = P->xb_par + 3 * b;
+        double vax[3], vxb[3], vab[3];
+        sub3(x + 3 * X, x + 3 * A, vax);
+        sub3(x + 3 * B, x + 3 * X, vxb);
+        sub3(x + 3 * B, x + 3 * A, vab);
+        double rax = norm3(vax), rxb = norm3(vxb), rab = norm3(vab);
+        if (rxb > 20.0) continue;
+        double radab = P->hb_rad[A] + P->hb_rad[B];
+        double ts = pow(5.0 * radab / (rxb * rxb), 6.0), tl = pow(rxb * rxb / 70.0, 6.0);
+        double fs = 1.0 / (1.0 + ts), fl = 1.0 / (1.0 + tl);
+        double ex = 70.0 * ((rax + rxb) / rab - 1.0);
+        if (ex > 15.0) continue;
+        double fo = 2.0 / (1.0 + exp(ex));
+        double eX = exp(15.0 * p[1]), eB = exp(15.0 * -p[2]);
+        double QX = eX / (eX + 0.03), QB = eB / (eB + 0.03);
+        double r3 = rxb * rxb * rxb;
+        double Rd = fs * fl * fo / r3;
+        double pre = -QB * p[0] * QX;
+        T.e[GFNFF_E_XBOND] += pre * Rd;
+        if (!want_grad) continue;
+        double dfs = -2.0 * 6.0 * fs * ts / (rxb * (1.0 + ts));
+        double dfl = -2.0 * 6.0 * rxb * fl * tl / (70.0 * (1.0 + tl));
+        double et = exp(ex), de = (1.0 + et) * (1.0 + et);
+        double dfo_ax = -2.0 * et * 70.0 / (rab * de);
+        double dfo_ab = 2.0 * et * 70.0 * (rax + rxb) / (rab * rab * de);
+        double dR_xb = (dfs * fl * fo + fs * dfl * fo + fs * fl * dfo_ax) / r3 - 3.0 * Rd / rxb;
+        double dR_ax = fs * fl * dfo_ax / r3, dR_ab = fs * fl * dfo_ab / r3;
+        double gxb = pre * dR_xb / (rxb + 1e-14), gax = pre * dR_ax / (rax + 1e-14), gab = pre * dR_ab / (rab + 1e-14);
+        gadd(&G, GX, A, -gax * vax[0] - gab * vab[0], -gax * vax[1] - gab * vab[1], -gax * vax[2] - gab * vab[2]);
+        gadd(&G, GX, X, gax * vax[0] - gxb * vxb[0], gax * vax[1] - gxb * vxb[1], gax * vax[2] - gxb * vxb[2]);
+        gadd(&G, GX, B, gxb * vxb[0] + gab * vab[0], gxb * vxb[1] + gab * vab[1], gxb * vxb[2] + gab * vab[2]);
+    }
+
     /* --- D4 ATM over bonded triplets: calcATM / calcATMGradient
        ff_workspace_gfnff.cpp:1526-1658.  The energy uses +c9 and the gradient
        -c9, as the reference does. */

@@ -12,7 +12,7 @@ import pytest
 from conftest import BOHR, gpu_available, load_fixture
 from curcuma_amd import GFNFFMethod, Mol
 from curcuma_amd.workloads import alkane, glycine_peptide, water_cluster
-from test_oracle_pi import hexadienyne
+from test_oracle_pi import bromomethane_ammonia, hexadienyne
 
 pytestmark = pytest.mark.gpu
 
@@ -74,6 +74,14 @@ def test_triple_bond_torsion(oracle):
     Z, xyz = hexadienyne(37.0)
     m, e, ref = run_both(Z, xyz, oracle)
     assert m.params_view().nstors == 1 and abs(m._terms[12]) > 1e-5
+    check(m, e, ref, len(Z))
+
+
+def test_halogen_bond(oracle):
+    Z, xyz = bromomethane_ammonia(3.0)
+    xyz = xyz + np.random.default_rng(5).normal(0.0, 0.05, xyz.shape)
+    m, e, ref = run_both(Z, xyz, oracle)
+    assert m.params_view().nxb >= 1 and m._terms[9] < 0.0
     check(m, e, ref, len(Z))
 
 
