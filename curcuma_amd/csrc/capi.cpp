@@ -32,6 +32,9 @@ struct gfnff_handle {
     // per-element parameter scales (repz, repulsion exponent, sqrt(Z r4r2)) and
     // the unscaled parameters they multiply
     std::vector<double> theta, base_repz, base_anb, base_szr, base_brep;
+    // geometry (Bohr) at which the hydrogen/halogen-bond lists were last
+    // detected during evaluation; empty until the first evaluation
+    std::vector<double> hb_ref;
 };
 
 namespace {
@@ -73,6 +76,7 @@ int gfnff_set_molecule(gfnff_handle* h, int32_t natoms, const int32_t* Z, const 
         if (st != gfnff::kOk) return fail(h, st, "gfnff_set_molecule: " + msg);
         h->have_molecule = true;
         h->dev_geom_dirty = true;
+        h->hb_ref.clear();
         h->theta.assign(3 * h->params.nelem, 1.0);
         h->base_repz = h->params.repz;
         h->base_anb = h->params.rep_anb;
@@ -156,6 +160,8 @@ int gfnff_batch_energies_device(gfnff_handle* h, double* e_mol_dev, void* stream
     return st ? fail(h, st, "gfnff_batch_energies_device: " + msg) : GFNFF_STATUS_OK;
 }
 
+static int refresh_pairs_if_needed(gfnff_handle* h);
+
 int gfnff_update_geometry(gfnff_handle* h, const double* xyz_ang) {
     if (!h) return GFNFF_STATUS_ARGS;
     if (!h->have_molecule || !xyz_ang) return fail(h, GFNFF_STATUS_ARGS, "gfnff_update_geometry: no molecule");
@@ -164,6 +170,32 @@ int gfnff_update_geometry(gfnff_handle* h, const double* xyz_ang) {
         h->xyz_bohr[i] = xyz_ang[i] * gfnff::kAngToBohrGeom;
     }
     h->dev_geom_dirty = true;
+    try {
+        return refresh_pairs_if_needed(h);
+    } catch (const std::exception& e) {
+        return fail(h, GFNFF_STATUS_ARGS, std::string("gfnff_update_geometry: ") + e.what());
+    }
+}
+
+// Re-detects the hydrogen/halogen-bond lists when the geometry has moved by
+// sqrt(sum dx^2)/N > hb_update_rmsd_bohr since the last detection, and on the
+// first evaluation (shouldUpdateHBXB + updateHBXBIfNeeded,
+// gfnff_method.cpp:2228-2259, 1597-1690).  A changed list rebuilds the device
+// engine's term lists.  Batches keep their set-up lists.
+static int refresh_pairs_if_needed(gfnff_handle* h) {
+    if (h->params.nmol != 1) return GFNFF_STATUS_OK;
+    const size_t n3 = h->xyz_bohr.size();
+    if (h->hb_ref.size() == n3) {
+        double s = 0.0;
+        for (size_t i = 0; i < n3; ++i) s += (h->xyz_bohr[i] - h->hb_ref[i]) * (h->xyz_bohr[i] - h->hb_ref[i]);
+        if (!(std::sqrt(s) / (double)h->params.n > h->opt.hb_update_rmsd_bohr)) return GFNFF_STATUS_OK;
+    }
+    bool changed = false;
+    std::string msg;
+    int st = gfnff::refresh_hbxb(h->params, h->xyz_bohr.data(), h->opt, changed, msg);
+    if (st) return fail(h, st, "hydrogen-bond list update: " + msg);
+    h->hb_ref = h->xyz_bohr;
+    if (changed) h->dev.reset();
     return GFNFF_STATUS_OK;
 }
 
@@ -181,7 +213,9 @@ int gfnff_energy_gradient(gfnff_handle* h, int32_t want_grad, double* energy, do
     if (!h->have_molecule) return fail(h, GFNFF_STATUS_ARGS, "gfnff_energy_gradient: no molecule");
     if (!energy) return fail(h, GFNFF_STATUS_ARGS, "gfnff_energy_gradient: energy pointer is NULL");
     try {
-        int st = ensure_device(h);
+        int st = refresh_pairs_if_needed(h);
+        if (st) return st;
+        st = ensure_device(h);
         if (st) return st;
         std::string msg;
         double e_terms[GFNFF_NTERMS + 1];
@@ -248,6 +282,9 @@ int gfnff_set_param(gfnff_handle* h, const char* key, double value) {
         h->opt.hb_accuracy = value;
     } else if (k == "threads") {
         h->opt.threads = value < 1 ? 1 : (int)value;
+    } else if (k == "hb_update_rmsd_bohr") {
+        if (!(value >= 0.0)) return fail(h, GFNFF_STATUS_ARGS, "hb_update_rmsd_bohr must be >= 0");
+        h->opt.hb_update_rmsd_bohr = value;
     } else if (k == "cholesky_lookahead") {
         h->opt.cholesky_lookahead = value != 0.0;
         h->dev.reset();

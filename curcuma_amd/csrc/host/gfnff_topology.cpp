@@ -1961,6 +1961,252 @@ int merge_batch(const std::vector<ParamSet>& mols, ParamSet& B, std::string& msg
     return kOk;
 }
 
+namespace {
+
+// Hydrogen-bond and halogen-bond lists of the current geometry
+// (detectHydrogenBondsNative, gfnff_method.cpp:7936-8494;
+// detectHalogenBondsNative :8520-8662).  Replaces P's hb_* and xb_* lists;
+// with cross_ref also derives the bond/HB cross reference (bond_hbH,
+// hbcn_pair) that the set-up keeps for the whole run (gfnff_method.cpp:3705-3731).
+void detect_hbxb(const Mol& m, const Topo& t, const Options& opt, ParamSet& P, bool cross_ref) {
+    const int n = m.n;
+    const int32_t* Z = m.Z;
+    const double* xyz = m.x;
+    P.hb_ahb.clear();
+    P.hb_case.clear();
+    P.hb_par.clear();
+    P.hb_nbB_off.clear();
+    P.hb_nbB_idx.clear();
+    P.hb_parentC.clear();
+    P.hb_nbC_off.clear();
+    P.hb_nbC_idx.clear();
+    P.xb_axb.clear();
+    P.xb_par.clear();
+    {
+        const auto& q = t.qa;
+        std::vector<double> cb(n), ca(n);
+        for (int i = 0; i < n; ++i) {
+            cb[i] = hb_basicity[Z[i]];
+            ca[i] = hb_acidity[Z[i]];
+            if (Z[i] == 8) {
+                bool carbonyl = t.adj[i].size() == 1 && t.pi[i] != 0 && Z[t.adj[i][0]] == 6 && t.pi[t.adj[i][0]] != 0;
+                bool nitro = t.adj[i].size() == 1 && Z[t.adj[i][0]] == 7;
+                if (carbonyl) cb[i] = 0.68;
+                else if (nitro) cb[i] = 0.47;
+            } else if (Z[i] == 6 && t.adj[i].size() == 2) {
+                double ph = bond_angle(m, t.adj[i][0], i, t.adj[i][1]) * 180.0 / M_PI;
+                if (ph < 150.0 && q[i] > -0.4) cb[i] = 1.46;
+            }
+        }
+        for (int i = 0; i < n; ++i) {  // amide H lowers the N acidity (FunctionalGroupDetector::isAmideHydrogen)
+            if (Z[i] != 1 || t.adj[i].size() != 1) continue;
+            int nn = t.adj[i][0];
+            if (!fg_amide_n(m, t, nn)) continue;
+            int c = 0;
+            for (int nb : t.adj[nn])
+                if (Z[nb] == 6 && t.hyb[nb] == 3) ++c;
+            if (c == 1) ca[nn] *= 0.80;
+        }
+        std::vector<int> hbh;
+        std::vector<char> is_hbh(n, 0);
+        for (int h = 0; h < n; ++h) {
+            if (Z[h] != 1 || t.hyb[h] == 1) continue;
+            int A = -1;
+            for (auto& b : t.bonds) {
+                if (b.first == h) A = b.second;
+                else if (b.second == h) A = b.first;
+                if (A != -1) break;
+            }
+            if (A == -1) continue;
+            double thr = 0.01;
+            if (Z[A] > 10) thr -= 0.20;
+            if (t.hyb[A] == 3 && Z[A] == 6) thr += 0.05;
+            if (q[h] > thr) {
+                hbh.push_back(h);
+                is_hbh[h] = 1;
+            }
+        }
+        const double hbthr1 = 200.0 - std::log10(opt.hb_accuracy) * 50.0;
+        const double hbthr2 = 400.0 - std::log10(opt.hb_accuracy) * 50.0;
+        auto ab_ok = [&](int a) {
+            if (Z[a] == 6 && t.pi[a] == 0) return false;
+            double thr = 0.10 + (Z[a] > 10 ? 0.2 : 0.0);
+            return !(q[a] > thr);
+        };
+        // uniform grid over candidate A/B atoms and HB hydrogens
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) {
+                lo[c] = std::min(lo[c], xyz[3 * i + c]);
+                hi[c] = std::max(hi[c], xyz[3 * i + c]);
+            }
+        const double cell = std::sqrt(hbthr2);
+        int nc[3];
+        for (int c = 0; c < 3; ++c) nc[c] = std::max(1, (int)std::ceil((hi[c] - lo[c]) / cell) + 1);
+        auto cid = [&](int i, int* o) {
+            for (int c = 0; c < 3; ++c) o[c] = std::min(nc[c] - 1, (int)((xyz[3 * i + c] - lo[c]) / cell));
+        };
+        std::map<long, std::vector<int>> hcell;
+        for (int h : hbh) {
+            int o[3];
+            cid(h, o);
+            hcell[((long)o[0] * nc[1] + o[1]) * nc[2] + o[2]].push_back(h);
+        }
+        std::vector<int> hb_A, hb_H, hb_B, hb_cs;
+        // A-B candidate pairs: (i, j) with j < i, legacy order
+        std::vector<int> cand;
+        for (int i = 0; i < n; ++i)
+            if (ab_ok(i)) cand.push_back(i);
+        std::map<long, std::vector<int>> acell;
+        for (int i : cand) {
+            int o[3];
+            cid(i, o);
+            acell[((long)o[0] * nc[1] + o[1]) * nc[2] + o[2]].push_back(i);
+        }
+        std::vector<std::vector<int>> hyd_on(n);
+        for (auto& b : t.bonds) {
+            if (is_hbh[b.first]) hyd_on[b.second].push_back(b.first);
+            if (is_hbh[b.second]) hyd_on[b.first].push_back(b.second);
+        }
+        for (int i : cand) {
+            int oi[3];
+            cid(i, oi);
+            std::vector<int> js;
+            for (int dx = -1; dx <= 1; ++dx)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dz = -1; dz <= 1; ++dz) {
+                        int c0 = oi[0] + dx, c1 = oi[1] + dy, c2 = oi[2] + dz;
+                        if (c0 < 0 || c1 < 0 || c2 < 0 || c0 >= nc[0] || c1 >= nc[1] || c2 >= nc[2]) continue;
+                        auto it = acell.find(((long)c0 * nc[1] + c1) * nc[2] + c2);
+                        if (it == acell.end()) continue;
+                        for (int j : it->second)
+                            if (j < i) js.push_back(j);
+                    }
+            std::sort(js.begin(), js.end());
+            for (int j : js) {
+                double s1 = cb[i] * ca[j], s2 = cb[j] * ca[i];
+                if (s1 < 1e-6 && s2 < 1e-6) continue;
+                double rab2 = m.dist2(i, j);
+                if (rab2 > hbthr1) continue;
+                bool nonb = !t.bonded(i, j);
+                std::vector<int> hs;
+                for (int dx = -1; dx <= 1; ++dx)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dz = -1; dz <= 1; ++dz) {
+                            int c0 = oi[0] + dx, c1 = oi[1] + dy, c2 = oi[2] + dz;
+                            if (c0 < 0 || c1 < 0 || c2 < 0 || c0 >= nc[0] || c1 >= nc[1] || c2 >= nc[2]) continue;
+                            auto it = hcell.find(((long)c0 * nc[1] + c1) * nc[2] + c2);
+                            if (it == hcell.end()) continue;
+                            hs.insert(hs.end(), it->second.begin(), it->second.end());
+                        }
+                for (int H : hyd_on[i])
+                    if (std::find(hs.begin(), hs.end(), H) == hs.end()) hs.push_back(H);
+                for (int H : hyd_on[j])
+                    if (std::find(hs.begin(), hs.end(), H) == hs.end()) hs.push_back(H);
+                std::sort(hs.begin(), hs.end());
+                for (int H : hs) {
+                    bool bi = t.bonded(i, H), bj = t.bonded(j, H);
+                    if (bi && nonb) {
+                        hb_A.push_back(i), hb_H.push_back(H), hb_B.push_back(j), hb_cs.push_back(2);
+                    } else if (bj && nonb) {
+                        hb_A.push_back(j), hb_H.push_back(H), hb_B.push_back(i), hb_cs.push_back(2);
+                    } else if (!bi && !bj) {
+                        if (rab2 + m.dist2(i, H) + m.dist2(j, H) < hbthr2)
+                            hb_A.push_back(i), hb_H.push_back(H), hb_B.push_back(j), hb_cs.push_back(1);
+                    }
+                }
+            }
+        }
+        P.hb_nbB_off.push_back(0);
+        P.hb_nbC_off.push_back(0);
+        std::map<std::pair<int, int>, std::vector<int>> ah_b;
+        for (size_t e = 0; e < hb_A.size(); ++e) {
+            int A = hb_A[e], H = hb_H[e], B = hb_B[e], cs = hb_cs[e];
+            int parent = -1;
+            if (cs == 2) {
+                if (Z[B] == 8 && t.adj[B].size() == 1) {
+                    int pa = t.adj[B][0];
+                    if (Z[pa] == 6 || Z[pa] == 7) {
+                        cs = 3;
+                        parent = pa;
+                    }
+                } else if (Z[B] == 7 && t.adj[B].size() == 2) {
+                    cs = 4;
+                }
+            }
+            P.hb_ahb.insert(P.hb_ahb.end(), {A, H, B});
+            P.hb_case.push_back(cs);
+            P.hb_par.insert(P.hb_par.end(), {cb[A], cb[B], ca[A], ca[B], q[H], q[A], q[B]});
+            if (cs >= 2)
+                for (int nb : t.adj[B]) P.hb_nbB_idx.push_back(nb);
+            P.hb_nbB_off.push_back((int)P.hb_nbB_idx.size());
+            P.hb_parentC.push_back(parent);
+            if (cs == 3)
+                for (int nb : t.adj[parent])
+                    if (nb != B) P.hb_nbC_idx.push_back(nb);
+            P.hb_nbC_off.push_back((int)P.hb_nbC_idx.size());
+            if (cs >= 2 && (Z[B] == 7 || Z[B] == 8)) ah_b[{A, H}].push_back(B);
+        }
+        // bond/HB cross reference (gfnff_method.cpp:3705-3731)
+        for (size_t b = 0; cross_ref && b < P.bond_hbH.size(); ++b) {
+            int i = P.bond_ij[2 * b], j = P.bond_ij[2 * b + 1];
+            int H = -1, A = -1;
+            if (Z[i] == 1) {
+                H = i;
+                A = j;
+            } else if (Z[j] == 1) {
+                H = j;
+                A = i;
+            } else {
+                continue;
+            }
+            if (Z[A] != 7 && Z[A] != 8) continue;
+            auto it = ah_b.find({A, H});
+            if (it == ah_b.end() || it->second.empty()) continue;
+            P.bond_hbH[b] = H;
+            for (int B : it->second) {
+                P.hbcn_pair.push_back(H);
+                P.hbcn_pair.push_back(B);
+            }
+        }
+
+        // --- halogen (and chalcogen/pnictogen) bonds A-X...B
+        // (detectHalogenBondsNative, gfnff_method.cpp:8520-8662): X in
+        // {P, S (at most two neighbours), Cl, As, Se, Br, Sb, Te, I} bonded to A,
+        // acceptors B within 10 Bohr of X in ascending order with the HB
+        // basicities, carbon acceptors only in pi systems
+        auto xlike = [](int z) {
+            return z == 17 || z == 35 || z == 53 || z == 16 || z == 34 || z == 52 || z == 15 || z == 33 || z == 51;
+        };
+        for (auto& b : t.bonds) {
+            int X = -1, A = -1;
+            if (xlike(Z[b.first])) {
+                X = b.first;
+                A = b.second;
+            } else if (xlike(Z[b.second])) {
+                X = b.second;
+                A = b.first;
+            }
+            if (X < 0) continue;
+            if (Z[X] == 16 && t.adj[X].size() > 2) continue;
+            for (int B = 0; B < n; ++B) {
+                if (m.dist2(X, B) >= 100.0) continue;
+                if (B == A || B == X || cb[B] < 1e-6) continue;
+                if (Z[B] == 6) {
+                    if (t.pi[B] == 0 || q[B] >= 0.05) continue;
+                } else if (q[B] > 0.05) {
+                    continue;
+                }
+                if (t.bonded(X, B)) continue;
+                P.xb_axb.insert(P.xb_axb.end(), {A, X, B});
+                P.xb_par.insert(P.xb_par.end(), {xb_acidity[Z[X]], q[X], q[B]});
+            }
+        }
+    }
+}
+
+}  // namespace
+
 int build_params(int n, const int32_t* Z, const double* xyz, int charge, const int32_t* forced, int nforced,
                  const Options& opt, ParamSet& P, std::string& msg) {
     if (n <= 0 || !Z || !xyz) {
@@ -2326,227 +2572,47 @@ int build_params(int n, const int32_t* Z, const double* xyz, int charge, const i
         }
     }
 
-    // --- hydrogen bonds (detectHydrogenBondsNative, gfnff_method.cpp:7936-8494)
-    {
-        const auto& q = t.qa;
-        std::vector<double> cb(n), ca(n);
-        for (int i = 0; i < n; ++i) {
-            cb[i] = hb_basicity[Z[i]];
-            ca[i] = hb_acidity[Z[i]];
-            if (Z[i] == 8) {
-                bool carbonyl = t.adj[i].size() == 1 && t.pi[i] != 0 && Z[t.adj[i][0]] == 6 && t.pi[t.adj[i][0]] != 0;
-                bool nitro = t.adj[i].size() == 1 && Z[t.adj[i][0]] == 7;
-                if (carbonyl) cb[i] = 0.68;
-                else if (nitro) cb[i] = 0.47;
-            } else if (Z[i] == 6 && t.adj[i].size() == 2) {
-                double ph = bond_angle(m, t.adj[i][0], i, t.adj[i][1]) * 180.0 / M_PI;
-                if (ph < 150.0 && q[i] > -0.4) cb[i] = 1.46;
-            }
-        }
-        for (int i = 0; i < n; ++i) {  // amide H lowers the N acidity (FunctionalGroupDetector::isAmideHydrogen)
-            if (Z[i] != 1 || t.adj[i].size() != 1) continue;
-            int nn = t.adj[i][0];
-            if (!fg_amide_n(m, t, nn)) continue;
-            int c = 0;
-            for (int nb : t.adj[nn])
-                if (Z[nb] == 6 && t.hyb[nb] == 3) ++c;
-            if (c == 1) ca[nn] *= 0.80;
-        }
-        std::vector<int> hbh;
-        std::vector<char> is_hbh(n, 0);
-        for (int h = 0; h < n; ++h) {
-            if (Z[h] != 1 || t.hyb[h] == 1) continue;
-            int A = -1;
-            for (auto& b : t.bonds) {
-                if (b.first == h) A = b.second;
-                else if (b.second == h) A = b.first;
-                if (A != -1) break;
-            }
-            if (A == -1) continue;
-            double thr = 0.01;
-            if (Z[A] > 10) thr -= 0.20;
-            if (t.hyb[A] == 3 && Z[A] == 6) thr += 0.05;
-            if (q[h] > thr) {
-                hbh.push_back(h);
-                is_hbh[h] = 1;
-            }
-        }
-        const double hbthr1 = 200.0 - std::log10(opt.hb_accuracy) * 50.0;
-        const double hbthr2 = 400.0 - std::log10(opt.hb_accuracy) * 50.0;
-        auto ab_ok = [&](int a) {
-            if (Z[a] == 6 && t.pi[a] == 0) return false;
-            double thr = 0.10 + (Z[a] > 10 ? 0.2 : 0.0);
-            return !(q[a] > thr);
-        };
-        // uniform grid over candidate A/B atoms and HB hydrogens
-        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-        for (int i = 0; i < n; ++i)
-            for (int c = 0; c < 3; ++c) {
-                lo[c] = std::min(lo[c], xyz[3 * i + c]);
-                hi[c] = std::max(hi[c], xyz[3 * i + c]);
-            }
-        const double cell = std::sqrt(hbthr2);
-        int nc[3];
-        for (int c = 0; c < 3; ++c) nc[c] = std::max(1, (int)std::ceil((hi[c] - lo[c]) / cell) + 1);
-        auto cid = [&](int i, int* o) {
-            for (int c = 0; c < 3; ++c) o[c] = std::min(nc[c] - 1, (int)((xyz[3 * i + c] - lo[c]) / cell));
-        };
-        std::map<long, std::vector<int>> hcell;
-        for (int h : hbh) {
-            int o[3];
-            cid(h, o);
-            hcell[((long)o[0] * nc[1] + o[1]) * nc[2] + o[2]].push_back(h);
-        }
-        std::vector<int> hb_A, hb_H, hb_B, hb_cs;
-        // A-B candidate pairs: (i, j) with j < i, legacy order
-        std::vector<int> cand;
-        for (int i = 0; i < n; ++i)
-            if (ab_ok(i)) cand.push_back(i);
-        std::map<long, std::vector<int>> acell;
-        for (int i : cand) {
-            int o[3];
-            cid(i, o);
-            acell[((long)o[0] * nc[1] + o[1]) * nc[2] + o[2]].push_back(i);
-        }
-        std::vector<std::vector<int>> hyd_on(n);
-        for (auto& b : t.bonds) {
-            if (is_hbh[b.first]) hyd_on[b.second].push_back(b.first);
-            if (is_hbh[b.second]) hyd_on[b.first].push_back(b.second);
-        }
-        for (int i : cand) {
-            int oi[3];
-            cid(i, oi);
-            std::vector<int> js;
-            for (int dx = -1; dx <= 1; ++dx)
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dz = -1; dz <= 1; ++dz) {
-                        int c0 = oi[0] + dx, c1 = oi[1] + dy, c2 = oi[2] + dz;
-                        if (c0 < 0 || c1 < 0 || c2 < 0 || c0 >= nc[0] || c1 >= nc[1] || c2 >= nc[2]) continue;
-                        auto it = acell.find(((long)c0 * nc[1] + c1) * nc[2] + c2);
-                        if (it == acell.end()) continue;
-                        for (int j : it->second)
-                            if (j < i) js.push_back(j);
-                    }
-            std::sort(js.begin(), js.end());
-            for (int j : js) {
-                double s1 = cb[i] * ca[j], s2 = cb[j] * ca[i];
-                if (s1 < 1e-6 && s2 < 1e-6) continue;
-                double rab2 = m.dist2(i, j);
-                if (rab2 > hbthr1) continue;
-                bool nonb = !t.bonded(i, j);
-                std::vector<int> hs;
-                for (int dx = -1; dx <= 1; ++dx)
-                    for (int dy = -1; dy <= 1; ++dy)
-                        for (int dz = -1; dz <= 1; ++dz) {
-                            int c0 = oi[0] + dx, c1 = oi[1] + dy, c2 = oi[2] + dz;
-                            if (c0 < 0 || c1 < 0 || c2 < 0 || c0 >= nc[0] || c1 >= nc[1] || c2 >= nc[2]) continue;
-                            auto it = hcell.find(((long)c0 * nc[1] + c1) * nc[2] + c2);
-                            if (it == hcell.end()) continue;
-                            hs.insert(hs.end(), it->second.begin(), it->second.end());
-                        }
-                for (int H : hyd_on[i])
-                    if (std::find(hs.begin(), hs.end(), H) == hs.end()) hs.push_back(H);
-                for (int H : hyd_on[j])
-                    if (std::find(hs.begin(), hs.end(), H) == hs.end()) hs.push_back(H);
-                std::sort(hs.begin(), hs.end());
-                for (int H : hs) {
-                    bool bi = t.bonded(i, H), bj = t.bonded(j, H);
-                    if (bi && nonb) {
-                        hb_A.push_back(i), hb_H.push_back(H), hb_B.push_back(j), hb_cs.push_back(2);
-                    } else if (bj && nonb) {
-                        hb_A.push_back(j), hb_H.push_back(H), hb_B.push_back(i), hb_cs.push_back(2);
-                    } else if (!bi && !bj) {
-                        if (rab2 + m.dist2(i, H) + m.dist2(j, H) < hbthr2)
-                            hb_A.push_back(i), hb_H.push_back(H), hb_B.push_back(j), hb_cs.push_back(1);
-                    }
-                }
-            }
-        }
-        P.hb_nbB_off.push_back(0);
-        P.hb_nbC_off.push_back(0);
-        std::map<std::pair<int, int>, std::vector<int>> ah_b;
-        for (size_t e = 0; e < hb_A.size(); ++e) {
-            int A = hb_A[e], H = hb_H[e], B = hb_B[e], cs = hb_cs[e];
-            int parent = -1;
-            if (cs == 2) {
-                if (Z[B] == 8 && t.adj[B].size() == 1) {
-                    int pa = t.adj[B][0];
-                    if (Z[pa] == 6 || Z[pa] == 7) {
-                        cs = 3;
-                        parent = pa;
-                    }
-                } else if (Z[B] == 7 && t.adj[B].size() == 2) {
-                    cs = 4;
-                }
-            }
-            P.hb_ahb.insert(P.hb_ahb.end(), {A, H, B});
-            P.hb_case.push_back(cs);
-            P.hb_par.insert(P.hb_par.end(), {cb[A], cb[B], ca[A], ca[B], q[H], q[A], q[B]});
-            if (cs >= 2)
-                for (int nb : t.adj[B]) P.hb_nbB_idx.push_back(nb);
-            P.hb_nbB_off.push_back((int)P.hb_nbB_idx.size());
-            P.hb_parentC.push_back(parent);
-            if (cs == 3)
-                for (int nb : t.adj[parent])
-                    if (nb != B) P.hb_nbC_idx.push_back(nb);
-            P.hb_nbC_off.push_back((int)P.hb_nbC_idx.size());
-            if (cs >= 2 && (Z[B] == 7 || Z[B] == 8)) ah_b[{A, H}].push_back(B);
-        }
-        // bond/HB cross reference (gfnff_method.cpp:3705-3731)
-        for (size_t b = 0; b < P.bond_hbH.size(); ++b) {
-            int i = P.bond_ij[2 * b], j = P.bond_ij[2 * b + 1];
-            int H = -1, A = -1;
-            if (Z[i] == 1) {
-                H = i;
-                A = j;
-            } else if (Z[j] == 1) {
-                H = j;
-                A = i;
-            } else {
-                continue;
-            }
-            if (Z[A] != 7 && Z[A] != 8) continue;
-            auto it = ah_b.find({A, H});
-            if (it == ah_b.end() || it->second.empty()) continue;
-            P.bond_hbH[b] = H;
-            for (int B : it->second) {
-                P.hbcn_pair.push_back(H);
-                P.hbcn_pair.push_back(B);
-            }
-        }
+    detect_hbxb(m, t, opt, P, true);
+    return kOk;
+}
 
-        // --- halogen (and chalcogen/pnictogen) bonds A-X...B
-        // (detectHalogenBondsNative, gfnff_method.cpp:8520-8662): X in
-        // {P, S (at most two neighbours), Cl, As, Se, Br, Sb, Te, I} bonded to A,
-        // acceptors B within 10 Bohr of X in ascending order with the HB
-        // basicities, carbon acceptors only in pi systems
-        auto xlike = [](int z) {
-            return z == 17 || z == 35 || z == 53 || z == 16 || z == 34 || z == 52 || z == 15 || z == 33 || z == 51;
-        };
-        for (auto& b : t.bonds) {
-            int X = -1, A = -1;
-            if (xlike(Z[b.first])) {
-                X = b.first;
-                A = b.second;
-            } else if (xlike(Z[b.second])) {
-                X = b.second;
-                A = b.first;
-            }
-            if (X < 0) continue;
-            if (Z[X] == 16 && t.adj[X].size() > 2) continue;
-            for (int B = 0; B < n; ++B) {
-                if (m.dist2(X, B) >= 100.0) continue;
-                if (B == A || B == X || cb[B] < 1e-6) continue;
-                if (Z[B] == 6) {
-                    if (t.pi[B] == 0 || q[B] >= 0.05) continue;
-                } else if (q[B] > 0.05) {
-                    continue;
-                }
-                if (t.bonded(X, B)) continue;
-                P.xb_axb.insert(P.xb_axb.end(), {A, X, B});
-                P.xb_par.insert(P.xb_par.end(), {xb_acidity[Z[X]], q[X], q[B]});
-            }
-        }
+int refresh_hbxb(ParamSet& P, const double* xyz_bohr, const Options& opt, bool& changed, std::string& msg) {
+    changed = false;
+    if (P.nmol != 1) {
+        msg = "hydrogen/halogen-bond lists of a batch are fixed at set-up";
+        return kErrArgs;
+    }
+    Mol m{P.n, P.Z.data(), xyz_bohr};
+    Topo t;
+    t.adj.assign(P.n, {});
+    for (size_t k = 0; k + 1 < P.topo_bonds.size(); k += 2) {
+        std::pair<int, int> b(P.topo_bonds[k], P.topo_bonds[k + 1]);
+        t.bonds.push_back(b);
+        t.adj[b.first].push_back(b.second);
+        t.adj[b.second].push_back(b.first);
+        t.bond_set.insert(b);
+    }
+    t.hyb.assign(P.hyb.begin(), P.hyb.end());
+    t.pi.assign(P.pi_frag.begin(), P.pi_frag.end());
+    t.qa = P.qa;
+    ParamSet Q;
+    Q.n = P.n;
+    detect_hbxb(m, t, opt, Q, false);
+    changed = Q.hb_ahb != P.hb_ahb || Q.hb_case != P.hb_case || Q.hb_par != P.hb_par ||
+              Q.hb_nbB_off != P.hb_nbB_off || Q.hb_nbB_idx != P.hb_nbB_idx || Q.hb_parentC != P.hb_parentC ||
+              Q.hb_nbC_off != P.hb_nbC_off || Q.hb_nbC_idx != P.hb_nbC_idx || Q.xb_axb != P.xb_axb ||
+              Q.xb_par != P.xb_par;
+    if (changed) {
+        P.hb_ahb.swap(Q.hb_ahb);
+        P.hb_case.swap(Q.hb_case);
+        P.hb_par.swap(Q.hb_par);
+        P.hb_nbB_off.swap(Q.hb_nbB_off);
+        P.hb_nbB_idx.swap(Q.hb_nbB_idx);
+        P.hb_parentC.swap(Q.hb_parentC);
+        P.hb_nbC_off.swap(Q.hb_nbC_off);
+        P.hb_nbC_idx.swap(Q.hb_nbC_idx);
+        P.xb_axb.swap(Q.xb_axb);
+        P.xb_par.swap(Q.xb_par);
     }
     return kOk;
 }

@@ -29,6 +29,7 @@ struct Options {
     double hb_accuracy = 0.1;       // HB list thresholds hbthr1/hbthr2
     int threads = 1;                // host threads for the Phase-1 solve
     bool cholesky_lookahead = false;  // overlap trailing updates with the next super-panel (side stream)
+    double hb_update_rmsd_bohr = 0.3;  // HB/XB re-detection threshold on sqrt(sum dx^2)/N
 };
 
 // Error codes shared with the C-ABI.
@@ -98,6 +99,12 @@ int merge_batch(const std::vector<ParamSet>& mols, ParamSet& out, std::string& m
 int build_params(int n, const int32_t* Z, const double* xyz_bohr, int charge,
                  const int32_t* forced_bonds, int nforced, const Options& opt, ParamSet& out,
                  std::string& msg);
+
+// Re-detects the hydrogen- and halogen-bond lists of a single molecule at the
+// geometry xyz_bohr with the set-up topology and topology charges
+// (updateHBXBIfNeeded, gfnff_method.cpp:1597-1690); the bond/HB cross
+// reference of the set-up is kept.  changed tells whether any list differs.
+int refresh_hbxb(ParamSet& P, const double* xyz_bohr, const Options& opt, bool& changed, std::string& msg);
 
 // Dense symmetric solve of the bordered EEQ system used at set-up
 // (A q + C^T lambda = x, C q = qfrag).  A is n x n row-major (overwritten).

@@ -71,7 +71,14 @@ int gfnff_set_batch(gfnff_handle* h, int32_t nmol, const int32_t* atom_off, cons
 int gfnff_batch_energies(gfnff_handle* h, double* e_mol);
 int gfnff_batch_energies_device(gfnff_handle* h, double* e_mol_dev, void* stream);
 
-/* ComputationalMethod::updateGeometry(const Matrix&) (computational_method.h:63), Angstrom. */
+/* ComputationalMethod::updateGeometry(const Matrix&) (computational_method.h:63), Angstrom.
+ * Like the reference's per-evaluation check (gfnff_method.cpp:1597-1690,
+ * 2228-2259), the hydrogen/halogen-bond lists of a single molecule are
+ * re-detected when sqrt(sum dx^2)/N exceeds hb_update_rmsd_bohr (0.3) since
+ * the last detection; the same check runs before every host-buffer
+ * evaluation (the first evaluation always re-detects).  The device-resident
+ * entry point does not read coordinates back: it uses the lists of the last
+ * host-side check. */
 int gfnff_update_geometry(gfnff_handle* h, const double* xyz_ang);
 
 /* ComputationalMethod::calculateEnergy(bool gradient) + copyGradientTo(Matrix&)
@@ -93,7 +100,8 @@ int gfnff_charges(gfnff_handle* h, double* q);
 int gfnff_cn(gfnff_handle* h, double* cn);
 
 /* ComputationalMethod::setParameters(json) for the parity-relevant keys
- * (gfnff.h:219-293): "cn_cutoff_bohr", "hb_accuracy", "threads"; engine
+ * (gfnff.h:219-293): "cn_cutoff_bohr", "hb_accuracy", "threads",
+ * "hb_update_rmsd_bohr" (gfnff_method.cpp:2250); engine
  * tuning: "cholesky_lookahead" (0/1, overlap the trailing update with the
  * next super-panel on a second stream; off by default). */
 int gfnff_set_param(gfnff_handle* h, const char* key, double value);

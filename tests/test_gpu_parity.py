@@ -181,3 +181,26 @@ def test_cholesky_lookahead_is_bitwise_identical():
     assert out[0][0] == out[1][0]
     assert np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][2], out[1][2])
+
+
+def test_hb_lists_refreshed_on_large_moves(oracle):
+    """After a geometry update beyond the re-detection threshold the device
+    engine evaluates the re-detected hydrogen-bond lists: same energy and
+    gradient as a fresh handle at that geometry, and as the oracle on the
+    refreshed parameters."""
+    Z, x0 = water_cluster(8, seed=12)
+    m = GFNFFMethod()
+    assert m.setMolecule(Mol(Z, x0))
+    m.calculateEnergy(True)
+    x1 = x0.copy()
+    x1[12:] += np.array([12.0, 0.0, 0.0])
+    assert m.updateGeometry(x1)
+    e = m.calculateEnergy(True)
+    assert not m.hasError(), m.getErrorMessage()
+    f = GFNFFMethod()
+    assert f.setMolecule(Mol(Z, x1))
+    ef = f.calculateEnergy(True)
+    assert abs(e - ef) <= 1e-9 * max(1.0, abs(ef))
+    assert np.abs(m.getGradient() - f.getGradient()).max() <= 1e-9
+    ref = oracle.eval(m.params_view(), x1 * (1.0 / BOHR))
+    assert abs(e - ref["E"]) <= 1e-9 * max(1.0, abs(ref["E"]))
