@@ -39,6 +39,7 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample", default="water_400")
     p.add_argument("--lookahead", type=int, default=0, help="engine option cholesky_lookahead (0/1)")
+    p.add_argument("--solver", type=int, default=0, help="engine option eeq_solver (0 PCG, 1 Cholesky)")
     p.add_argument("--roofline-kernel", default="auto", help="kernel class timed for the roofline object")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="run one device copy of this size first (HBM counter calibration under rocprofv3 --pmc)")
@@ -136,7 +137,7 @@ def main():
 
     def make(precision):
         cls = GFNFFBatch if batch else GFNFFMethod
-        m = cls(precision=precision, parameters={"cholesky_lookahead": args.lookahead})
+        m = cls(precision=precision, parameters={"cholesky_lookahead": args.lookahead, "eeq_solver": args.solver})
         t0 = time.perf_counter()
         ok = m.setBatch(atom_off, Z, xyz) if batch else m.setMolecule(Mol(Z, xyz))
         if not ok:
@@ -202,6 +203,8 @@ def main():
     nr = pv.natoms - pv.nfrag
     info = {"xyz_bohr": xyz * (1.0 / 0.529177210903), "np_pad": max(64, -(-nr // 64) * 64), "nhb": pv.nhb,
             "nbond": pv.nbond}
+    pcg_iters = mh.solverIterations() if not batch else 0
+    info["pcg_iters"] = pcg_iters
     if batch:
         frag = np.ctypeslib.as_array(pv.frag, shape=(n,))
         info["mol_off"] = atom_off
@@ -232,7 +235,10 @@ def main():
         "dtype": "f64" if headline == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": WORKLOAD_TEXT.get(args.workload, args.workload),
-                   "atoms": n, "precision": headline, "setup_s": round(t_setup, 3), "energy_Eh": e_last},
+                   "atoms": n, "precision": headline, "setup_s": round(t_setup, 3), "energy_Eh": e_last,
+                   "eeq_solver": ("per-molecule Cholesky" if batch else
+                                  ("PCG" if args.solver == 0 else "blocked Cholesky")),
+                   "pcg_iterations": pcg_iters},
         "fp64": {"value": value64, "ms_per_step": dt64 / args.steps * 1e3, "energy_Eh": e64,
                  "kernel_ms_per_eval": {c: round(v[0], 4) for c, v in
                                         sorted(classes64.items(), key=lambda t: -t[1][0])}},

@@ -76,6 +76,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
     lib.gfnff_get_params.argtypes = [C.c_void_p, C.POINTER(ParamsView)]
     lib.gfnff_get_topology.argtypes = [C.c_void_p, _i32p, _i32p, _i32p, C.c_int32, _i32p]
     lib.gfnff_device_count.restype = C.c_int
+    lib.gfnff_solver_iterations.argtypes = [C.c_void_p]
     lib.gfnff_dense_cholesky.argtypes = [C.c_int32, _f64p, _f64p]
     lib.gfnff_dense_cholesky_f32.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
     lib.gfnff_param_count.argtypes = [C.c_void_p]

@@ -203,7 +203,7 @@ static int ensure_device(gfnff_handle* h) {
     if (h->dev) return GFNFF_STATUS_OK;
     std::string msg;
     int st = GFNFF_STATUS_OK;
-    h->dev.reset(gfnff::device_create(h->params, h->precision, h->opt.cholesky_lookahead, st, msg));
+    h->dev.reset(gfnff::device_create(h->params, h->precision, h->opt.cholesky_lookahead, h->opt.eeq_solver, st, msg));
     if (!h->dev) return fail(h, st ? st : GFNFF_STATUS_DEVICE, "device set-up: " + msg);
     return GFNFF_STATUS_OK;
 }
@@ -285,6 +285,10 @@ int gfnff_set_param(gfnff_handle* h, const char* key, double value) {
     } else if (k == "hb_update_rmsd_bohr") {
         if (!(value >= 0.0)) return fail(h, GFNFF_STATUS_ARGS, "hb_update_rmsd_bohr must be >= 0");
         h->opt.hb_update_rmsd_bohr = value;
+    } else if (k == "eeq_solver") {
+        if (value != 0.0 && value != 1.0) return fail(h, GFNFF_STATUS_ARGS, "eeq_solver: 0 (PCG) or 1 (Cholesky)");
+        h->opt.eeq_solver = (int)value;
+        h->dev.reset();
     } else if (k == "cholesky_lookahead") {
         h->opt.cholesky_lookahead = value != 0.0;
         h->dev.reset();
@@ -327,6 +331,11 @@ int gfnff_get_topology(const gfnff_handle* h, int32_t* hyb, int32_t* pi, int32_t
 }
 
 int gfnff_device_count(void) { return gfnff::device_count(); }
+
+int gfnff_solver_iterations(gfnff_handle* h) {
+    if (!h || !h->dev) return -1;
+    return gfnff::device_solver_iterations(h->dev.get());
+}
 
 int gfnff_cn_neighbours(gfnff_handle* h, int32_t* counts, int32_t* idx, int32_t maxnb) {
     if (!h || !counts || !idx || maxnb <= 0 || !h->dev) return GFNFF_STATUS_ARGS;

@@ -205,8 +205,12 @@ __global__ __launch_bounds__(256) void k_eeq_diag(int n, const double* __restric
 __global__ __launch_bounds__(256) void k_eeq_reduced(int nr, int np, const int* __restrict__ ra,
                                                      const int* __restrict__ rl, const double* __restrict__ x,
                                                      const double* __restrict__ alp,
-                                                     const double* __restrict__ diag, double* __restrict__ A) {
-    // one 64x64 lower tile per workgroup; grid enumerates tiles (bi >= bj)
+                                                     const double* __restrict__ diag, double* __restrict__ A,
+                                                     int full) {
+    // one 64x64 lower tile per workgroup; grid enumerates tiles (bi >= bj).
+    // full != 0 also writes the mirrored upper tile (both triangles stored,
+    // for the row-major matrix-vector products of the iterative solver)
+    __shared__ double Tl[NB][NB + 1];
     int t = blockIdx.x;
     int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
     while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
@@ -224,6 +228,15 @@ __global__ __launch_bounds__(256) void k_eeq_reduced(int nr, int np, const int* 
                 eeq_a(la, lb, x, alp, diag);
         }
         A[(size_t)r * np + c] = v;
+        if (full) Tl[e / NB][e % NB] = v;
+    }
+    if (!full) return;
+    __syncthreads();
+    // upper counterpart: A(bj*64 + i, bi*64 + j) = T(j, i), row-contiguous stores
+    _Pragma("unroll") for (int u_ = 0, e = threadIdx.x; u_ < NB * NB / 256; ++u_, e += 256) {
+        const int i = e / NB, j = e % NB;
+        if (bi == bj && j <= i) continue;
+        A[(size_t)(bj * NB + i) * np + bi * NB + j] = Tl[j][i];
     }
 }
 
@@ -279,7 +292,7 @@ __device__ __forceinline__ float rsqrt_nr(float d) {
 // also factor a whole small matrix (batched path).
 template <typename T>
 __device__ __forceinline__ void chol_diag_block(int np, int k0, T* __restrict__ A, T* __restrict__ Linv, int* __restrict__ flags,
-                                T (*sL)[NB + 1], T (*col)[NB], T* rdiag) {
+                                T (*sL)[NB + 1], T (*col)[NB], T* rdiag, bool write_L = true) {
     const int tid = threadIdx.x;
     const int t = tid >> 2, q = tid & 3;
     _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) sL[e / NB][e % NB] = A[(size_t)(k0 + e / NB) * np + k0 + e % NB];
@@ -311,9 +324,11 @@ __device__ __forceinline__ void chol_diag_block(int np, int k0, T* __restrict__ 
     for (int u = 0; u < NB / 4; ++u) sL[t][4 * u + q] = (4 * u + q <= t) ? a[u] : T(0);
     __syncthreads();
     if (q == 0) rdiag[t] = T(1) / sL[t][t];
-    _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) {
-        int r = e / NB, c = e % NB;
-        if (c <= r) A[(size_t)(k0 + r) * np + k0 + c] = sL[r][c];
+    if (write_L) {
+        _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) {
+            int r = e / NB, c = e % NB;
+            if (c <= r) A[(size_t)(k0 + r) * np + k0 + c] = sL[r][c];
+        }
     }
     __syncthreads();
     const int c = t;
@@ -737,6 +752,241 @@ __global__ __launch_bounds__(256) void k_trsv_bwd_sweep(int np, const T* __restr
     __syncthreads();
     if (tid == 0) __hip_atomic_store(ready + w, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     if (!ok && tid < NB) atomicOr(flags, FLAG_SOLVE);
+}
+
+// ---------------------------------------------------------------------------
+// K4c: iterative EEQ solve.  The reduced EEQ matrices of non-bonded clusters
+// are well conditioned (water clusters: kappa ~ 6), so preconditioned CG
+// converges in ~15 matrix-vector products.  Each product streams the matrix
+// once from HBM (row-major, both triangles stored), which replaces the
+// latency-bound chain of the blocked factorisation.
+// ---------------------------------------------------------------------------
+// Block-Jacobi preconditioner: inverse factors of the 64 x 64 diagonal blocks
+// of R (one workgroup per block; R is left intact).
+__global__ __launch_bounds__(256) void k_bjacobi_setup(int np, double* __restrict__ R, double* __restrict__ Lj,
+                                                       int* __restrict__ flags) {
+    __shared__ double sL[NB][NB + 1];
+    __shared__ double col[2][NB];
+    __shared__ double rdiag[NB];
+    chol_diag_block<double>(np, blockIdx.x * NB, R, Lj, flags, sL, col, rdiag, false);
+}
+
+// Grid-wide barrier of a persistent kernel whose G workgroups are all
+// resident: arrival counter bar[0] and generation bar[1], bounded wait (a
+// timeout sets FLAG_SOLVE and every workgroup leaves).  The arrival is an
+// acquire-release atomic after a device fence, so every write made before the
+// barrier is visible to every workgroup after it.
+__device__ bool grid_barrier(int* bar, int G, int* flags) {
+    __shared__ int ok_s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int g0 = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const int arrived = __hip_atomic_fetch_add(bar, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        int ok = 1;
+        if (arrived == G - 1) {
+            __hip_atomic_store(bar, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(bar + 1, g0 + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            for (int it = 0; __hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g0; ++it) {
+                if (it > SPIN_MAX ||
+                    (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_SOLVE)) {
+                    ok = 0;
+                    atomicOr(flags, FLAG_SOLVE);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __threadfence();
+        ok_s = ok;
+    }
+    __syncthreads();
+    return ok_s != 0;
+}
+
+// Fixed-order sum of the G per-workgroup partials part[k * G + g] (the same
+// tree in every workgroup, so every workgroup takes the same decisions).
+__device__ double sum_partials(const double* part, int G, double* red) {
+    __syncthreads();
+    red[threadIdx.x] = (threadIdx.x < (unsigned)G)
+                           ? __hip_atomic_load(part + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           : 0.0;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < (unsigned)w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double v = red[0];
+    __syncthreads();
+    return v;
+}
+
+// z_b = Linv_b^T (Linv_b r_b) for the 64-block b (r_b and the result in LDS).
+__device__ void bjacobi_apply(const double* __restrict__ Li, const double* rb, double* wb, double* zb) {
+    const int tid = threadIdx.x, t = tid >> 2, q = tid & 3;
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < NB / 4; ++u) s += Li[t * NB + 4 * u + q] * rb[4 * u + q];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    __syncthreads();
+    if (q == 0) wb[t] = s;
+    __syncthreads();
+    s = 0.0;
+#pragma unroll
+    for (int u = 0; u < NB / 4; ++u) s += Li[(4 * u + q) * NB + t] * wb[4 * u + q];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if (q == 0) zb[t] = s;
+    __syncthreads();
+}
+
+// Persistent PCG on R x = b.  Workgroup g owns the 64-row blocks g, g + G, ...
+// (at most PCG_MAXB).  Per iteration: (1) p = z + beta p_prev (formed on the
+// fly by every reader; the owner stores its rows), q = R p for the owned rows,
+// partial p.q; barrier; (2) alpha = rz / pq, x += alpha p, r -= alpha q,
+// z = M^-1 r, partials r.z and r.r; barrier; beta = rz_new / rz.  Stops when
+// r.r <= tol2 * b.b (uniform decision from identical sums) or after maxit
+// iterations (FLAG_SOLVE).  Wave w handles rows 16w .. 16w + 15 of a block,
+// lanes stride the columns; cross-workgroup vectors (z, p) are read with
+// coherent loads.
+constexpr int PCG_MAXB = 8;
+constexpr double PCG_TOL2 = 1e-26;  // |r| <= 1e-13 |b|
+constexpr int PCG_MAXIT = 1000;
+__global__ __launch_bounds__(256) void k_eeq_pcg(int np, const double* __restrict__ R, const double* __restrict__ Lj,
+                                                 const double* __restrict__ b, double* __restrict__ x,
+                                                 double* __restrict__ r, double* __restrict__ z,
+                                                 double* __restrict__ pb, double* __restrict__ part,
+                                                 int* __restrict__ bar, int* __restrict__ iters, double tol2,
+                                                 int maxit, int* __restrict__ flags) {
+    __shared__ double red[256];
+    __shared__ double qown[PCG_MAXB][NB], pown[PCG_MAXB][NB];
+    __shared__ double rb[NB], wb[NB], zb[NB];
+    const int G = gridDim.x, g = blockIdx.x, nbk = np / NB;
+    const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
+    const int nown = (nbk - g + G - 1) / G;
+    double* const ppq = part;          // [G]
+    double* const prz = part + G;      // [G]
+    double* const prr = part + 2 * G;  // [G]
+    double* const pbb = part + 3 * G;  // [G]
+    // init: x = 0, r = b, z = M^-1 b; partials r.z and b.b
+    double lrz = 0.0, lbb = 0.0;
+    for (int o = 0; o < nown; ++o) {
+        const int kb = g + o * G;
+        if (tid < NB) {
+            const double bv = b[kb * NB + tid];
+            rb[tid] = bv;
+            x[kb * NB + tid] = 0.0;
+            r[kb * NB + tid] = bv;
+        }
+        __syncthreads();
+        bjacobi_apply(Lj + (size_t)kb * NB * NB, rb, wb, zb);
+        if (tid < NB) {
+            z[kb * NB + tid] = zb[tid];
+            lrz += rb[tid] * zb[tid];
+            lbb += rb[tid] * rb[tid];
+        }
+    }
+    // block-local sums of the thread contributions (fixed tree)
+    auto wg_sum = [&](double v) {
+        __syncthreads();
+        red[tid] = v;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (tid < w) red[tid] += red[tid + w];
+            __syncthreads();
+        }
+        const double s2 = red[0];
+        __syncthreads();
+        return s2;
+    };
+    double s_rz = wg_sum(lrz), s_bb = wg_sum(lbb);
+    if (tid == 0) {
+        prz[g] = s_rz;
+        pbb[g] = s_bb;
+    }
+    if (!grid_barrier(bar, G, flags)) return;
+    double rz = sum_partials(prz, G, red), bb = sum_partials(pbb, G, red);
+    double beta = 0.0;
+    int it = 0;
+    bool done = !(bb > 0.0);
+    for (; !done && it < maxit; ++it) {
+        const double* pprev = pb + (size_t)((it + 1) & 1) * np;
+        double* pcur = pb + (size_t)(it & 1) * np;
+        // (1) q = R p for the owned rows
+        double lpq = 0.0;
+        for (int o = 0; o < nown; ++o) {
+            const int kb = g + o * G;
+            double acc[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[i] = 0.0;
+            const double* Rrow = R + (size_t)(kb * NB + 16 * wv) * np;
+            for (int c = l; c < np; c += 64) {
+                const double zc = __hip_atomic_load(z + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const double pc = (it == 0) ? 0.0 : __hip_atomic_load(pprev + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const double pv = zc + beta * pc;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) acc[i] += Rrow[(size_t)i * np + c] * pv;
+            }
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+#pragma unroll
+                for (int o2 = 32; o2 > 0; o2 >>= 1) acc[i] += __shfl_xor(acc[i], o2);
+            if (l < 16) {
+                double qv = 0.0;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) qv = (l == i) ? acc[i] : qv;
+                qown[o][16 * wv + l] = qv;
+            }
+            if (tid < NB) {
+                const int row = kb * NB + tid;
+                const double zc = __hip_atomic_load(z + row, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const double pc = (it == 0) ? 0.0 : __hip_atomic_load(pprev + row, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const double pv = zc + beta * pc;
+                pown[o][tid] = pv;
+                pcur[row] = pv;
+            }
+            __syncthreads();
+            if (tid < NB) lpq += pown[o][tid] * qown[o][tid];
+        }
+        const double s_pq = wg_sum(lpq);
+        if (tid == 0) ppq[g] = s_pq;
+        if (!grid_barrier(bar, G, flags)) return;
+        const double pq = sum_partials(ppq, G, red);
+        const double alpha = rz / pq;
+        // (2) x, r, z and the partials r.z, r.r
+        double lrz2 = 0.0, lrr = 0.0;
+        for (int o = 0; o < nown; ++o) {
+            const int kb = g + o * G;
+            if (tid < NB) {
+                const int row = kb * NB + tid;
+                x[row] += alpha * pown[o][tid];
+                const double rv = r[row] - alpha * qown[o][tid];
+                r[row] = rv;
+                rb[tid] = rv;
+            }
+            __syncthreads();
+            bjacobi_apply(Lj + (size_t)kb * NB * NB, rb, wb, zb);
+            if (tid < NB) {
+                z[kb * NB + tid] = zb[tid];
+                lrz2 += rb[tid] * zb[tid];
+                lrr += rb[tid] * rb[tid];
+            }
+        }
+        const double s_rz2 = wg_sum(lrz2), s_rr = wg_sum(lrr);
+        if (tid == 0) {
+            prz[g] = s_rz2;
+            prr[g] = s_rr;
+        }
+        if (!grid_barrier(bar, G, flags)) return;
+        const double rz2 = sum_partials(prz, G, red), rr = sum_partials(prr, G, red);
+        beta = rz2 / rz;
+        rz = rz2;
+        done = rr <= tol2 * bb;
+    }
+    if (g == 0 && tid == 0) iters[0] = it;
+    if (!done && g == 0 && tid == 0) atomicOr(flags, FLAG_SOLVE);
 }
 
 // ---------------------------------------------------------------------------
@@ -2521,11 +2771,12 @@ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 // Kernel classes that can be timed with HIP events (one class at a time).
 enum KernelClass {
     KC_CN, KC_EEQ_BUILD, KC_CHOL_DIAG, KC_CHOL_PANEL, KC_CHOL_UPDATE, KC_TRSV, KC_CHARGES, KC_PAIRS, KC_PAIRS_AUX,
-    KC_NBREP, KC_REFINE, KC_BONDS, KC_ANGLES, KC_TORS, KC_BREP, KC_HB, KC_ATM, KC_BATM, KC_CHAIN, KC_REDUCE, KC_COUNT
+    KC_NBREP, KC_REFINE, KC_BONDS, KC_ANGLES, KC_TORS, KC_BREP, KC_HB, KC_ATM, KC_BATM, KC_CHAIN, KC_REDUCE, KC_PCG,
+    KC_COUNT
 };
 const char* const kKernelClassNames[KC_COUNT] = {
     "cn",    "eeq_build", "chol_diag", "chol_panel", "chol_update", "trsv", "charges", "pairs", "pairs_aux", "nbrep", "refine",
-    "bonds", "angles",    "torsions",  "brep",       "hb",          "atm",  "batm",    "chain", "reduce"};
+    "bonds", "angles",    "torsions",  "brep",       "hb",          "atm",  "batm",    "chain", "reduce", "pcg"};
 
 // Start/stop event pairs around every launch of the selected class, recorded
 // on the stream the kernels run on.
@@ -2573,6 +2824,10 @@ struct DeviceEngine {
     bool mixed = false;                 // FP32 pair terms + FP32 factorisation with FP64 refinement
     Lookahead la;                       // side stream of the factorisation
     bool use_lookahead = false;
+    bool use_cg = false;                // iterative EEQ solve (single molecules)
+    int cg_groups = 0;                  // workgroups of the persistent PCG kernel
+    double *cg_r = nullptr, *cg_z = nullptr, *cg_p = nullptr, *cg_part = nullptr;
+    int *cg_bar = nullptr, *cg_iters = nullptr;
     int* ready = nullptr;               // per-block ready flags of the solve sweeps
     // batches of molecules (nmol > 1)
     int nmol = 1, nbtiles = 0;
@@ -2675,7 +2930,8 @@ struct DeviceEngine {
     }
 };
 
-DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, int& status, std::string& msg) {
+DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, int solver, int& status,
+                            std::string& msg) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         status = kErrDevice;
@@ -2697,6 +2953,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
     auto* E = new DeviceEngine();
     E->mixed = precision == 1;
     E->use_lookahead = lookahead;
+    E->use_cg = solver == 0 && P.nmol == 1;
     try {
         const int n = P.n;
         E->n = n;
@@ -2999,7 +3256,19 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->ybuf = E->alloc<double>(E->np);
         E->xbuf = E->alloc<double>(E->np);
         E->ready = E->alloc<int>(E->np / NB);
-        if (E->mixed && E->nmol == 1) {
+        if (E->use_cg) {
+            const int nbk = E->np / NB;
+            E->cg_groups = std::min(nbk, 256);
+            if (cdiv(nbk, E->cg_groups) > PCG_MAXB) E->use_cg = false;
+        }
+        if (E->use_cg) {
+            E->cg_r = E->alloc<double>(E->np);
+            E->cg_z = E->alloc<double>(E->np);
+            E->cg_p = E->alloc<double>(2 * (size_t)E->np);
+            E->cg_part = E->alloc<double>(4 * (size_t)E->cg_groups);
+            E->cg_bar = E->alloc<int>(2);
+            E->cg_iters = E->alloc<int>(1);
+        } else if (E->mixed && E->nmol == 1) {
             E->Ared32 = E->alloc<float>((size_t)E->np * E->np);
             E->Linv32 = E->alloc<float>((size_t)E->np * NB);
             E->b32 = E->alloc<float>(E->np);
@@ -3171,7 +3440,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     const int np = E->np, nbk = np / NB;
     if (E->nmol == 1)
         LAUNCH(KC_EEQ_BUILD, k_eeq_reduced, dim3(nbk * (nbk + 1) / 2), dim3(256), s, E->nr, np, E->red_atom,
-               E->red_last, E->x, E->alp, E->diag, E->Ared);
+               E->red_last, E->x, E->alp, E->diag, E->Ared, E->use_cg ? 1 : 0);
     else
         LAUNCH(KC_EEQ_BUILD, k_eeq_reduced_batch, dim3(E->nbtiles), dim3(256), s, E->btiles, E->mnr, E->mnp, E->mmo,
                E->mvo, E->red_atom, E->red_last, E->x, E->alp, E->diag, E->Ared);
@@ -3187,6 +3456,12 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         // batch: every molecule's system factored and solved by one workgroup (FP64)
         LAUNCH(KC_CHOL_DIAG, k_eeq_batch_solve, dim3(E->nmol), dim3(256), s, E->mnp, E->mmo, E->mvo, E->Ared,
                E->Linv, E->b, E->xbuf, E->flags);
+    } else if (E->use_cg) {
+        // block-Jacobi factors of R's diagonal blocks, then the persistent PCG
+        LAUNCH(KC_CHOL_DIAG, k_bjacobi_setup, dim3(nbk), dim3(256), s, np, E->Ared, E->Linv, E->flags);
+        hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, s, 2, E->cg_bar, 0);
+        LAUNCH(KC_PCG, k_eeq_pcg, dim3(E->cg_groups), dim3(256), s, np, E->Ared, E->Linv, E->b, E->xbuf, E->cg_r,
+               E->cg_z, E->cg_p, E->cg_part, E->cg_bar, E->cg_iters, PCG_TOL2, PCG_MAXIT, E->flags);
     } else if (!E->mixed) {
         queue_cholesky(&E->timer, s, np, E->Ared, E->Linv, E->flags, E->use_lookahead ? &E->la : nullptr);
         queue_solve<double>(E, s, E->Ared, E->Linv, E->b, E->ybuf, E->xbuf);
@@ -3354,6 +3629,14 @@ int device_eval(DeviceEngine* E, const double* xyz_bohr_dev, double* energy_dev,
         return kErrDevice;
     }
     return kOk;
+}
+
+int device_solver_iterations(DeviceEngine* E) {
+    if (!E->use_cg) return 0;
+    int it = -1;
+    if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(&it, E->cg_iters, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return -1;
+    return it;
 }
 
 int device_cn_neighbours(DeviceEngine* E, int* counts, int* idx, int maxnb, std::string& msg) {

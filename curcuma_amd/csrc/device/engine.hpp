@@ -12,7 +12,10 @@ namespace gfnff {
 struct DeviceEngine;
 
 // Uploads the parameter set; returns nullptr and sets status/msg on failure.
-DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, int& status, std::string& msg);
+// solver: 0 = preconditioned CG on the reduced EEQ system (single molecules;
+// batches always use the per-molecule Cholesky), 1 = blocked Cholesky.
+DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, int solver, int& status,
+                            std::string& msg);
 void device_destroy(DeviceEngine* e);
 
 // Host-buffer evaluation: uploads xyz (Bohr), evaluates, downloads
@@ -25,6 +28,10 @@ int device_eval(DeviceEngine* e, const double* xyz_bohr_dev, double* energy_dev,
                 std::string& msg);
 
 int device_count();
+
+// Iterations of the last iterative EEQ solve (0 when the Cholesky path runs,
+// -1 on a device error); synchronises.
+int device_solver_iterations(DeviceEngine* e);
 
 // CN neighbour lists of the last evaluation (ascending j, strict d^2 < 36 Bohr^2):
 // counts[n], idx[n * maxnb] (-1 past the end of a list).

@@ -30,6 +30,7 @@ struct Options {
     int threads = 1;                // host threads for the Phase-1 solve
     bool cholesky_lookahead = false;  // overlap trailing updates with the next super-panel (side stream)
     double hb_update_rmsd_bohr = 0.3;  // HB/XB re-detection threshold on sqrt(sum dx^2)/N
+    int eeq_solver = 0;               // 0: preconditioned CG (single molecules), 1: blocked Cholesky
 };
 
 // Error codes shared with the C-ABI.

@@ -119,6 +119,10 @@ class GFNFFMethod:
         for k, v in parameters.items():
             self._lib.gfnff_set_param(self._h, k.encode(), float(v))
 
+    def solverIterations(self) -> int:
+        """Iterations of the last iterative EEQ solve (0 on the Cholesky path)."""
+        return int(self._lib.gfnff_solver_iterations(self._h))
+
     def setPrecision(self, precision: str) -> None:
         mode = {"fp64": 0, "mixed": 1}[precision]
         self._lib.gfnff_set_precision(self._h, mode)

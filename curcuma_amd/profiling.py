@@ -27,7 +27,7 @@ NB = 64                # Cholesky block size of the engine
 
 # every kernel class the engine can time
 CLASSES = ["cn", "eeq_build", "chol_diag", "chol_panel", "chol_update", "trsv", "charges", "pairs", "pairs_aux", "nbrep", "refine",
-           "bonds", "angles", "torsions", "brep", "hb", "atm", "batm", "chain", "reduce"]
+           "bonds", "angles", "torsions", "brep", "hb", "atm", "batm", "chain", "reduce", "pcg"]
 
 
 def read_counts(path: Path = HEADER) -> dict:
@@ -129,6 +129,9 @@ def work_per_eval(cls: str, info: dict, rc: dict) -> tuple[float, str]:
     if cls == "trsv":
         npd = info["np_pad"]
         return 2.0 * 8.0 * npd * npd / 2, "byte"  # the factor is read once per sweep
+    if cls == "pcg":  # the full FP64 matrix is streamed once per CG iteration
+        npd = info["np_pad"]
+        return info.get("pcg_iters", 0) * 8.0 * npd * npd, "byte"
     if cls == "eeq_build":
         npd = info["np_pad"]
         return 8.0 * npd * (npd + NB) / 2, "byte"

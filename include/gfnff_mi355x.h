@@ -101,9 +101,12 @@ int gfnff_cn(gfnff_handle* h, double* cn);
 
 /* ComputationalMethod::setParameters(json) for the parity-relevant keys
  * (gfnff.h:219-293): "cn_cutoff_bohr", "hb_accuracy", "threads",
- * "hb_update_rmsd_bohr" (gfnff_method.cpp:2250); engine
- * tuning: "cholesky_lookahead" (0/1, overlap the trailing update with the
- * next super-panel on a second stream; off by default). */
+ * "hb_update_rmsd_bohr" (gfnff_method.cpp:2250).  Engine keys:
+ * "eeq_solver" 0 = preconditioned CG on the reduced EEQ system (default for
+ * single molecules, stops at |r| <= 1e-13 |b|), 1 = blocked Cholesky, the
+ * direct solve of the reference (batches always factor per molecule);
+ * "cholesky_lookahead" 0/1 overlaps the trailing update with the next
+ * super-panel on a second stream (off by default). */
 int gfnff_set_param(gfnff_handle* h, const char* key, double value);
 
 /* GFNFF_PREC_FP64 (default, the reference's arithmetic) or GFNFF_PREC_MIXED:
@@ -129,6 +132,11 @@ int gfnff_get_topology(const gfnff_handle* h, int32_t* hyb, int32_t* pi, int32_t
 
 /* Number of visible GPUs (0 when none). */
 int gfnff_device_count(void);
+
+/* Iterations of the last iterative EEQ solve of the handle (0 when the
+ * Cholesky path ran, -1 without a device engine); synchronises. No
+ * counterpart in the reference (solver diagnostics). */
+int gfnff_solver_iterations(gfnff_handle* h);
 
 /* Parameter adjoint for force-matching fits (no counterpart in the
  * reference's interface; used by a fitting harness).  The fitted parameters

@@ -19,16 +19,18 @@ if not gpu_available():
     pytest.skip("no HIP device", allow_module_level=True)
 
 
-def evaluate(Z, xyz, precision):
+def evaluate(Z, xyz, precision, solver=0):
     m = GFNFFMethod(precision=precision)
+    m.setParameters({"eeq_solver": solver})
     assert m.setMolecule(Mol(Z, xyz)), m.getErrorMessage()
     e = m.calculateEnergy(True)
     assert not m.hasError(), m.getErrorMessage()
     return m, e
 
 
+@pytest.mark.parametrize("solver", [0, 1])
 @pytest.mark.parametrize("case", ["water27", "alkane40", "water512", "gly20"])
-def test_mixed_against_fp64_and_oracle(case, oracle):
+def test_mixed_against_fp64_and_oracle(case, solver, oracle):
     if case == "water27":
         Z, xyz = water_cluster(27, seed=5)
     elif case == "alkane40":
@@ -38,8 +40,8 @@ def test_mixed_against_fp64_and_oracle(case, oracle):
     else:
         Z, xyz = water_cluster(512, seed=3)
     n = len(Z)
-    m64, e64 = evaluate(Z, xyz, "fp64")
-    m32, e32 = evaluate(Z, xyz, "mixed")
+    m64, e64 = evaluate(Z, xyz, "fp64", solver)
+    m32, e32 = evaluate(Z, xyz, "mixed", solver)
     ref = oracle.eval(m64.params_view(), np.asarray(xyz) * (1.0 / BOHR))
     g64, g32 = m64.getGradient(), m32.getGradient()
     de_atom = abs(e32 - ref["E"]) / n

@@ -173,7 +173,7 @@ def test_cholesky_lookahead_is_bitwise_identical():
     Z, xyz = water_cluster(512, seed=3)
     out = []
     for la in (0, 1):
-        m = GFNFFMethod(parameters={"cholesky_lookahead": la})
+        m = GFNFFMethod(parameters={"cholesky_lookahead": la, "eeq_solver": 1})
         assert m.setMolecule(Mol(Z, xyz)), m.getErrorMessage()
         e = m.calculateEnergy(True)
         assert not m.hasError(), m.getErrorMessage()
@@ -204,3 +204,33 @@ def test_hb_lists_refreshed_on_large_moves(oracle):
     assert np.abs(m.getGradient() - f.getGradient()).max() <= 1e-9
     ref = oracle.eval(m.params_view(), x1 * (1.0 / BOHR))
     assert abs(e - ref["E"]) <= 1e-9 * max(1.0, abs(ref["E"]))
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_eeq_solvers_agree(solver, oracle):
+    """The iterative EEQ solve (default) and the blocked Cholesky give the
+    oracle's charges and energy; PCG converges in few iterations on a water
+    cluster (reduced matrix kappa ~ 6)."""
+    Z, xyz = water_cluster(216, seed=9)
+    m = GFNFFMethod()
+    m.setParameters({"eeq_solver": solver})
+    assert m.setMolecule(Mol(Z, xyz))
+    e = m.calculateEnergy(True)
+    assert not m.hasError(), m.getErrorMessage()
+    ref = oracle.eval(m.params_view(), np.asarray(xyz) * (1.0 / BOHR))
+    check(m, e, ref, len(Z))
+    its = m.solverIterations()
+    print(f"solver {solver}: iterations {its}")
+    if solver == 0:
+        assert 0 < its < 60
+    else:
+        assert its == 0
+
+
+def test_pcg_on_chain_molecule(oracle):
+    """Single-fragment chains are less well conditioned (kappa of the reduced
+    matrix ~1e3): PCG still converges to the oracle's charges."""
+    Z, xyz = alkane(120, sigma=0.02, seed=4)
+    m, e, ref = run_both(Z, xyz, oracle)
+    check(m, e, ref, len(Z))
+    assert 0 < m.solverIterations() < 400
