@@ -774,8 +774,10 @@ __global__ __launch_bounds__(256) void k_bjacobi_setup(int np, double* __restric
 // Grid-wide barrier of a persistent kernel whose G workgroups are all
 // resident: arrival counter bar[0] and generation bar[1], bounded wait (a
 // timeout sets FLAG_SOLVE and every workgroup leaves).  The arrival is an
-// acquire-release atomic after a device fence, so every write made before the
-// barrier is visible to every workgroup after it.
+// acquire-release atomic after a device fence and the waiter's generation
+// load an agent-scope acquire: every write made before the barrier is visible
+// to every workgroup after it, and the acquire invalidates the non-coherent
+// caches, so plain loads after the barrier see other workgroups' data.
 __device__ bool grid_barrier(int* bar, int G, int* flags) {
     __shared__ int ok_s;
     __syncthreads();
@@ -850,7 +852,7 @@ __device__ void bjacobi_apply(const double* __restrict__ Li, const double* rb, d
 // r.r <= tol2 * b.b (uniform decision from identical sums) or after maxit
 // iterations (FLAG_SOLVE).  Wave w handles rows 16w .. 16w + 15 of a block,
 // lanes stride the columns; cross-workgroup vectors (z, p) are read with
-// coherent loads.
+// plain loads after the barrier that ordered their writes.
 constexpr int PCG_MAXB = 8;
 constexpr double PCG_TOL2 = 1e-26;  // |r| <= 1e-13 |b|
 constexpr int PCG_MAXIT = 1000;
@@ -923,8 +925,8 @@ __global__ __launch_bounds__(256) void k_eeq_pcg(int np, const double* __restric
             for (int i = 0; i < 16; ++i) acc[i] = 0.0;
             const double* Rrow = R + (size_t)(kb * NB + 16 * wv) * np;
             for (int c = l; c < np; c += 64) {
-                const double zc = __hip_atomic_load(z + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const double pc = (it == 0) ? 0.0 : __hip_atomic_load(pprev + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const double zc = z[c];
+                const double pc = (it == 0) ? 0.0 : pprev[c];
                 const double pv = zc + beta * pc;
 #pragma unroll
                 for (int i = 0; i < 16; ++i) acc[i] += Rrow[(size_t)i * np + c] * pv;
@@ -2708,8 +2710,16 @@ __global__ __launch_bounds__(256) void k_reduce1(Segs s, double* __restrict__ pa
     int n = s.n[sg];
     int chunk = (n + RED_BLOCKS - 1) / RED_BLOCKS;
     int lo = b * chunk, hi = min(n, lo + chunk);
-    double acc = 0.0;
-    for (int k = lo + threadIdx.x; k < hi; k += 256) acc += p[k];
+    // eight independent partial sums per thread keep eight loads in flight
+    double a8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = lo + threadIdx.x; k < hi; k += 8 * 256) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const double v = p[min(k + u * 256, hi - 1)];  // unconditional load, masked add
+            a8[u] += (k + u * 256 < hi) ? v : 0.0;
+        }
+    }
+    double acc = ((a8[0] + a8[1]) + (a8[2] + a8[3])) + ((a8[4] + a8[5]) + (a8[6] + a8[7]));
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
