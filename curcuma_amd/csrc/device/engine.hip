@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -771,107 +772,207 @@ __global__ __launch_bounds__(256) void k_bjacobi_setup(int np, double* __restric
     chol_diag_block<double>(np, blockIdx.x * NB, R, Lj, flags, sL, col, rdiag, false);
 }
 
-// Grid-wide barrier of a persistent kernel whose G workgroups are all
-// resident: arrival counter bar[0] and generation bar[1], bounded wait (a
-// timeout sets FLAG_SOLVE and every workgroup leaves).  The arrival is an
-// acquire-release atomic after a device fence and the waiter's generation
-// load an agent-scope acquire: every write made before the barrier is visible
-// to every workgroup after it, and the acquire invalidates the non-coherent
-// caches, so plain loads after the barrier see other workgroups' data.
-__device__ bool grid_barrier(int* bar, int G, int* flags) {
-    __shared__ int ok_s;
+// The persistent PCG runs PCG_THREADS threads per workgroup: its waves split
+// a 64-row block into PCG_RG row groups of PCG_RW rows and the columns into
+// PCG_CS interleaved parts, so each CU keeps many waves of streaming loads of
+// R in flight.  Reductions inside a workgroup use the first 256 threads.
+constexpr int PCG_THREADS = 256;
+constexpr int PCG_RW = 16;                               // rows per wave
+constexpr int PCG_RG = NB / PCG_RW;                      // row groups per block
+constexpr int PCG_CS = PCG_THREADS / 64 / PCG_RG;        // column parts per block
+static_assert(PCG_RG * PCG_CS * 64 == PCG_THREADS && PCG_THREADS >= 256, "PCG layout");
+
+// Fixed-order sum of the first 256 threads' values (every thread returns the
+// total; threads >= 256 contribute nothing).
+__device__ double wg_sum256(double v, double* red) {
+    const int tid = threadIdx.x;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const int g0 = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const int arrived = __hip_atomic_fetch_add(bar, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        int ok = 1;
-        if (arrived == G - 1) {
-            __hip_atomic_store(bar, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(bar + 1, g0 + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            for (int it = 0; __hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g0; ++it) {
-                if (it > SPIN_MAX ||
-                    (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_SOLVE)) {
-                    ok = 0;
+    if (tid < 256) red[tid] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+    }
+    const double s = red[0];
+    __syncthreads();
+    return s;
+}
+
+// Device-coherent access for data shared between the workgroups of a
+// persistent kernel: relaxed agent-scope atomics, i.e. loads and stores that
+// bypass the non-coherent caches (no cache write-back or invalidation, whose
+// cost grows with the number of workgroups on a multi-die GPU).
+__device__ inline double ld_c(const double* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline void st_c(double* p, double v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline unsigned long long ld_cu(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_cu(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// s_waitcnt vmcnt(0): this wave's loads and stores have completed (device-
+// coherent stores are then visible to every device-coherent load).
+__device__ inline void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Grid-wide all-reduce of two values per workgroup for a persistent kernel
+// whose G <= 256 workgroups are all resident.  Mailboxes are 128-byte lines
+// of 64-bit words holding MB_EMPTY (a NaN pattern no arithmetic produces)
+// until written: workgroup g != 0 writes its values to line (g, j & 1) of
+// barrier j, workgroup 0's thread t polls line t, adds the values in a fixed
+// order and writes the sums to the broadcast line j & 1, which the other
+// workgroups poll.  A line is cleared by its writer one barrier later
+// (workgroup 0 has read it by then), and the clear completes before the
+// writer's next value, so a poller never sees a stale value.  Every wait is
+// bounded (FLAG_SOLVE on a timeout).  Ordering without cache maintenance:
+// the data a workgroup shares is written with device-coherent stores that
+// complete (vmcnt) before its mailbox value is written, and read with
+// device-coherent loads issued after the broadcast was seen, so every shared
+// write before the barrier is visible to every workgroup after it.
+// Mailbox layout: lines [0, 512) producers (2 per workgroup), 512 and 513
+// broadcast; MB_WORDS words in all.
+constexpr unsigned long long MB_EMPTY = 0x7ff0deadbeef0001ull;
+constexpr int MB_LINE = 16;                  // 64-bit words per 128-byte line
+constexpr int MB_WORDS = 514 * MB_LINE;
+
+__device__ bool grid_allreduce2(double v0, double v1, double* sums, unsigned long long* mb, int j, int* flags,
+                                double* red) {
+    __shared__ double sv[2];
+    __shared__ int bad_s;
+    const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, par = j & 1;
+    unsigned long long* const bc = mb + 512 * MB_LINE;
+    wait_vm();
+    __syncthreads();
+    if (tid == 0) bad_s = 0;
+    if (g != 0) {
+        if (tid == 0) {
+            unsigned long long* const own = mb + (2 * g + par) * MB_LINE;
+            unsigned long long* const old = mb + (2 * g + (par ^ 1)) * MB_LINE;
+            st_cu(old, MB_EMPTY);
+            st_cu(old + 1, MB_EMPTY);
+            wait_vm();
+            st_cu(own, __double_as_longlong(v0));
+            st_cu(own + 1, __double_as_longlong(v1));
+            const unsigned long long* const b = bc + par * MB_LINE;
+            unsigned long long a0 = MB_EMPTY, a1 = MB_EMPTY;
+            for (int it = 0;; ++it) {
+                a0 = ld_cu(b);
+                a1 = ld_cu(b + 1);
+                if (a0 != MB_EMPTY && a1 != MB_EMPTY) break;
+                if (it > SPIN_MAX || (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_SOLVE)) {
+                    bad_s = 1;
                     atomicOr(flags, FLAG_SOLVE);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
+            sv[0] = __longlong_as_double(a0);
+            sv[1] = __longlong_as_double(a1);
         }
-        __threadfence();
-        ok_s = ok;
-    }
-    __syncthreads();
-    return ok_s != 0;
-}
-
-// Fixed-order sum of the G per-workgroup partials part[k * G + g] (the same
-// tree in every workgroup, so every workgroup takes the same decisions).
-__device__ double sum_partials(const double* part, int G, double* red) {
-    __syncthreads();
-    red[threadIdx.x] = (threadIdx.x < (unsigned)G)
-                           ? __hip_atomic_load(part + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                           : 0.0;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < (unsigned)w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
+        sums[0] = sv[0];
+        sums[1] = sv[1];
+        return bad_s == 0;
     }
-    const double v = red[0];
+    // workgroup 0: gather, fixed-order sums, broadcast
     __syncthreads();
-    return v;
+    double u0 = 0.0, u1 = 0.0;
+    if (tid == 0) {
+        u0 = v0;
+        u1 = v1;
+    } else if (tid < G) {
+        const unsigned long long* const in = mb + (2 * tid + par) * MB_LINE;
+        unsigned long long a0 = MB_EMPTY, a1 = MB_EMPTY;
+        for (int it = 0;; ++it) {
+            a0 = ld_cu(in);
+            a1 = ld_cu(in + 1);
+            if (a0 != MB_EMPTY && a1 != MB_EMPTY) break;
+            if (it > SPIN_MAX || (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_SOLVE)) {
+                bad_s = 1;
+                atomicOr(flags, FLAG_SOLVE);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        u0 = __longlong_as_double(a0);
+        u1 = __longlong_as_double(a1);
+    }
+    const double s0 = wg_sum256(u0, red), s1 = wg_sum256(u1, red);
+    if (tid == 0 && bad_s == 0) {
+        unsigned long long* const old = bc + (par ^ 1) * MB_LINE;
+        st_cu(old, MB_EMPTY);
+        st_cu(old + 1, MB_EMPTY);
+        wait_vm();
+        unsigned long long* const out = bc + par * MB_LINE;
+        st_cu(out, __double_as_longlong(s0));
+        st_cu(out + 1, __double_as_longlong(s1));
+    }
+    sums[0] = s0;
+    sums[1] = s1;
+    __syncthreads();
+    return bad_s == 0;
 }
 
-// z_b = Linv_b^T (Linv_b r_b) for the 64-block b (r_b and the result in LDS).
+// z_b = Linv_b^T (Linv_b r_b) for the 64-block b (r_b and the result in LDS;
+// threads 0..255 compute, every thread synchronises).
 __device__ void bjacobi_apply(const double* __restrict__ Li, const double* rb, double* wb, double* zb) {
-    const int tid = threadIdx.x, t = tid >> 2, q = tid & 3;
+    const int tid = threadIdx.x, t = (tid >> 2) & 63, q = tid & 3;
+    const bool act = tid < 256;
     double s = 0.0;
+    if (act) {
 #pragma unroll
-    for (int u = 0; u < NB / 4; ++u) s += Li[t * NB + 4 * u + q] * rb[4 * u + q];
+        for (int u = 0; u < NB / 4; ++u) s += Li[t * NB + 4 * u + q] * rb[4 * u + q];
+    }
     s += __shfl_xor(s, 1);
     s += __shfl_xor(s, 2);
     __syncthreads();
-    if (q == 0) wb[t] = s;
+    if (act && q == 0) wb[t] = s;
     __syncthreads();
     s = 0.0;
+    if (act) {
 #pragma unroll
-    for (int u = 0; u < NB / 4; ++u) s += Li[(4 * u + q) * NB + t] * wb[4 * u + q];
+        for (int u = 0; u < NB / 4; ++u) s += Li[(4 * u + q) * NB + t] * wb[4 * u + q];
+    }
     s += __shfl_xor(s, 1);
     s += __shfl_xor(s, 2);
-    if (q == 0) zb[t] = s;
+    if (act && q == 0) zb[t] = s;
     __syncthreads();
 }
 
 // Persistent PCG on R x = b.  Workgroup g owns the 64-row blocks g, g + G, ...
-// (at most PCG_MAXB).  Per iteration: (1) p = z + beta p_prev (formed on the
-// fly by every reader; the owner stores its rows), q = R p for the owned rows,
-// partial p.q; barrier; (2) alpha = rz / pq, x += alpha p, r -= alpha q,
-// z = M^-1 r, partials r.z and r.r; barrier; beta = rz_new / rz.  Stops when
+// (at most PCG_MAXB) for the vector updates; the matrix-vector product is cut
+// into nbk x nchunk tiles (a 64-row block times a chunk of cwb 64-column
+// blocks), tile t processed by workgroup t mod G, so that every CU streams
+// its share of R.  Per iteration: (1) p = z + beta p_prev (formed on the fly
+// by every reader; owners store their rows); per tile the partial q rows go to
+// qpart[chunk] and the tile's p.q_partial to the workgroup's p.q sum (p.q is
+// linear in the partials); barrier; (2) alpha = rz / pq; owners add the
+// partial q rows in chunk order, x += alpha p, r -= alpha q, z = M^-1 r,
+// partials r.z and r.r; barrier; beta = rz_new / rz.  Stops when
 // r.r <= tol2 * b.b (uniform decision from identical sums) or after maxit
-// iterations (FLAG_SOLVE).  Wave w handles rows 16w .. 16w + 15 of a block,
-// lanes stride the columns; cross-workgroup vectors (z, p) are read with
-// plain loads after the barrier that ordered their writes.
+// iterations (FLAG_SOLVE).  In a tile, wave w handles rows
+// PCG_RW (w % PCG_RG) .. + PCG_RW - 1 and the columns
+// c_lo + 64 (w / PCG_RG) + lane + 64 PCG_CS k; the column parts are added in
+// LDS in a fixed order.  Cross-workgroup vectors (z, p, qpart) are written
+// and read with device-coherent accesses (ld_c, st_c).
 constexpr int PCG_MAXB = 8;
+constexpr int PCG_MAXCHUNK = 16;
 constexpr double PCG_TOL2 = 1e-26;  // |r| <= 1e-13 |b|
 constexpr int PCG_MAXIT = 1000;
-__global__ __launch_bounds__(256) void k_eeq_pcg(int np, const double* __restrict__ R, const double* __restrict__ Lj,
-                                                 const double* __restrict__ b, double* __restrict__ x,
-                                                 double* __restrict__ r, double* __restrict__ z,
-                                                 double* __restrict__ pb, double* __restrict__ part,
-                                                 int* __restrict__ bar, int* __restrict__ iters, double tol2,
-                                                 int maxit, int* __restrict__ flags) {
+__global__ __launch_bounds__(PCG_THREADS) void k_eeq_pcg(int np, int nchunk, int cwb, const double* __restrict__ R,
+                                                         const double* __restrict__ Lj, const double* __restrict__ b,
+                                                         double* __restrict__ x, double* __restrict__ r,
+                                                         double* __restrict__ z, double* __restrict__ pb,
+                                                         double* __restrict__ qpart, unsigned long long* __restrict__ mb,
+                                                         int* __restrict__ iters, double tol2, int maxit,
+                                                         int* __restrict__ flags) {
     __shared__ double red[256];
-    __shared__ double qown[PCG_MAXB][NB], pown[PCG_MAXB][NB];
-    __shared__ double rb[NB], wb[NB], zb[NB];
-    const int G = gridDim.x, g = blockIdx.x, nbk = np / NB;
+    __shared__ double pown[PCG_MAXB][NB];
+    __shared__ double qcs[PCG_CS][NB];
+    __shared__ double ptile[NB], rb[NB], wb[NB], zb[NB];
+    const int G = gridDim.x, g = blockIdx.x, nbk = np / NB, ntile = nbk * nchunk;
     const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
+    const int rg = wv % PCG_RG, cs = wv / PCG_RG;
     const int nown = (nbk - g + G - 1) / G;
-    double* const ppq = part;          // [G]
-    double* const prz = part + G;      // [G]
-    double* const prr = part + 2 * G;  // [G]
-    double* const pbb = part + 3 * G;  // [G]
     // init: x = 0, r = b, z = M^-1 b; partials r.z and b.b
     double lrz = 0.0, lbb = 0.0;
     for (int o = 0; o < nown; ++o) {
@@ -885,104 +986,106 @@ __global__ __launch_bounds__(256) void k_eeq_pcg(int np, const double* __restric
         __syncthreads();
         bjacobi_apply(Lj + (size_t)kb * NB * NB, rb, wb, zb);
         if (tid < NB) {
-            z[kb * NB + tid] = zb[tid];
+            st_c(z + kb * NB + tid, zb[tid]);
             lrz += rb[tid] * zb[tid];
             lbb += rb[tid] * rb[tid];
         }
     }
-    // block-local sums of the thread contributions (fixed tree)
-    auto wg_sum = [&](double v) {
-        __syncthreads();
-        red[tid] = v;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if (tid < w) red[tid] += red[tid + w];
-            __syncthreads();
-        }
-        const double s2 = red[0];
-        __syncthreads();
-        return s2;
-    };
-    double s_rz = wg_sum(lrz), s_bb = wg_sum(lbb);
-    if (tid == 0) {
-        prz[g] = s_rz;
-        pbb[g] = s_bb;
-    }
-    if (!grid_barrier(bar, G, flags)) return;
-    double rz = sum_partials(prz, G, red), bb = sum_partials(pbb, G, red);
+    int nbar = 0;  // barriers passed
+    double sums[2];
+    const double s_rz = wg_sum256(lrz, red), s_bb = wg_sum256(lbb, red);
+    if (!grid_allreduce2(s_rz, s_bb, sums, mb, nbar++, flags, red)) return;
+    double rz = sums[0];
+    const double bb = sums[1];
     double beta = 0.0;
     int it = 0;
     bool done = !(bb > 0.0);
     for (; !done && it < maxit; ++it) {
         const double* pprev = pb + (size_t)((it + 1) & 1) * np;
         double* pcur = pb + (size_t)(it & 1) * np;
-        // (1) q = R p for the owned rows
-        double lpq = 0.0;
+        // (1) owners: p rows; every tile: partial q rows and partial p.q
         for (int o = 0; o < nown; ++o) {
-            const int kb = g + o * G;
-            double acc[16];
+            if (tid < NB) {
+                const int row = (g + o * G) * NB + tid;
+                const double pc = (it == 0) ? 0.0 : ld_c(pprev + row);
+                const double pv = ld_c(z + row) + beta * pc;
+                pown[o][tid] = pv;
+                st_c(pcur + row, pv);
+            }
+        }
+        double lpq = 0.0;
+        for (int t = g; t < ntile; t += G) {
+            const int kb = t / nchunk, ch = t - kb * nchunk;
+            const int c_lo = ch * cwb * NB, c_hi = min(np, c_lo + cwb * NB);
+            double acc[PCG_RW];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) acc[i] = 0.0;
-            const double* Rrow = R + (size_t)(kb * NB + 16 * wv) * np;
-            for (int c = l; c < np; c += 64) {
-                const double zc = z[c];
-                const double pc = (it == 0) ? 0.0 : pprev[c];
-                const double pv = zc + beta * pc;
+            for (int i = 0; i < PCG_RW; ++i) acc[i] = 0.0;
+            const double* Rrow = R + (size_t)(kb * NB + PCG_RW * rg) * np;
+            for (int c = c_lo + NB * cs + l; c < c_hi; c += NB * PCG_CS) {
+                const double pc = (it == 0) ? 0.0 : ld_c(pprev + c);
+                const double pv = ld_c(z + c) + beta * pc;
 #pragma unroll
-                for (int i = 0; i < 16; ++i) acc[i] += Rrow[(size_t)i * np + c] * pv;
+                for (int i = 0; i < PCG_RW; ++i) acc[i] += Rrow[(size_t)i * np + c] * pv;
             }
 #pragma unroll
-            for (int i = 0; i < 16; ++i)
+            for (int i = 0; i < PCG_RW; ++i)
 #pragma unroll
                 for (int o2 = 32; o2 > 0; o2 >>= 1) acc[i] += __shfl_xor(acc[i], o2);
-            if (l < 16) {
+            if (l < PCG_RW) {
                 double qv = 0.0;
 #pragma unroll
-                for (int i = 0; i < 16; ++i) qv = (l == i) ? acc[i] : qv;
-                qown[o][16 * wv + l] = qv;
+                for (int i = 0; i < PCG_RW; ++i) qv = (l == i) ? acc[i] : qv;
+                qcs[cs][PCG_RW * rg + l] = qv;
             }
             if (tid < NB) {
                 const int row = kb * NB + tid;
-                const double zc = __hip_atomic_load(z + row, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const double pc = (it == 0) ? 0.0 : __hip_atomic_load(pprev + row, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const double pv = zc + beta * pc;
-                pown[o][tid] = pv;
-                pcur[row] = pv;
+                const double pc = (it == 0) ? 0.0 : ld_c(pprev + row);
+                ptile[tid] = ld_c(z + row) + beta * pc;
             }
             __syncthreads();
-            if (tid < NB) lpq += pown[o][tid] * qown[o][tid];
+            if (tid < NB) {
+                double qv = qcs[0][tid];
+#pragma unroll
+                for (int k = 1; k < PCG_CS; ++k) qv += qcs[k][tid];
+                st_c(qpart + (size_t)ch * np + kb * NB + tid, qv);
+                lpq += ptile[tid] * qv;
+            }
+            __syncthreads();
         }
-        const double s_pq = wg_sum(lpq);
-        if (tid == 0) ppq[g] = s_pq;
-        if (!grid_barrier(bar, G, flags)) return;
-        const double pq = sum_partials(ppq, G, red);
+        const double s_pq = wg_sum256(lpq, red);
+        if (!grid_allreduce2(s_pq, 0.0, sums, mb, nbar++, flags, red)) return;
+        const double pq = sums[0];
         const double alpha = rz / pq;
-        // (2) x, r, z and the partials r.z, r.r
+        // (2) owners: q rows from the chunk partials, x, r, z and the
+        // partials r.z, r.r
         double lrz2 = 0.0, lrr = 0.0;
         for (int o = 0; o < nown; ++o) {
             const int kb = g + o * G;
             if (tid < NB) {
                 const int row = kb * NB + tid;
+                // all chunk partials loaded at once (clamped index), added in chunk order
+                double qk[PCG_MAXCHUNK];
+#pragma unroll
+                for (int k = 0; k < PCG_MAXCHUNK; ++k) qk[k] = ld_c(qpart + (size_t)min(k, nchunk - 1) * np + row);
+                double qv = qk[0];
+#pragma unroll
+                for (int k = 1; k < PCG_MAXCHUNK; ++k) qv += (k < nchunk) ? qk[k] : 0.0;
                 x[row] += alpha * pown[o][tid];
-                const double rv = r[row] - alpha * qown[o][tid];
+                const double rv = r[row] - alpha * qv;
                 r[row] = rv;
                 rb[tid] = rv;
             }
             __syncthreads();
             bjacobi_apply(Lj + (size_t)kb * NB * NB, rb, wb, zb);
             if (tid < NB) {
-                z[kb * NB + tid] = zb[tid];
+                st_c(z + kb * NB + tid, zb[tid]);
                 lrz2 += rb[tid] * zb[tid];
                 lrr += rb[tid] * rb[tid];
             }
         }
-        const double s_rz2 = wg_sum(lrz2), s_rr = wg_sum(lrr);
-        if (tid == 0) {
-            prz[g] = s_rz2;
-            prr[g] = s_rr;
-        }
-        if (!grid_barrier(bar, G, flags)) return;
-        const double rz2 = sum_partials(prz, G, red), rr = sum_partials(prr, G, red);
+        const double s_rz2 = wg_sum256(lrz2, red), s_rr = wg_sum256(lrr, red);
+        if (!grid_allreduce2(s_rz2, s_rr, sums, mb, nbar++, flags, red)) return;
+        const double rz2 = sums[0], rr = sums[1];
         beta = rz2 / rz;
         rz = rz2;
         done = rr <= tol2 * bb;
@@ -2771,6 +2874,11 @@ __global__ void k_fill_i(int n, int* __restrict__ a, int v) {
     if (i < n) a[i] = v;
 }
 
+__global__ void k_fill_u64(int n, unsigned long long* __restrict__ a, unsigned long long v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
 }  // namespace
@@ -2836,8 +2944,10 @@ struct DeviceEngine {
     bool use_lookahead = false;
     bool use_cg = false;                // iterative EEQ solve (single molecules)
     int cg_groups = 0;                  // workgroups of the persistent PCG kernel
-    double *cg_r = nullptr, *cg_z = nullptr, *cg_p = nullptr, *cg_part = nullptr;
-    int *cg_bar = nullptr, *cg_iters = nullptr;
+    int cg_chunks = 1, cg_cwb = 0;      // column chunks of its matrix-vector tiles, width in 64-blocks
+    double *cg_r = nullptr, *cg_z = nullptr, *cg_p = nullptr, *cg_q = nullptr;
+    unsigned long long* cg_mb = nullptr;  // all-reduce mailboxes of the persistent PCG
+    int* cg_iters = nullptr;
     int* ready = nullptr;               // per-block ready flags of the solve sweeps
     // batches of molecules (nmol > 1)
     int nmol = 1, nbtiles = 0;
@@ -3267,16 +3377,40 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->xbuf = E->alloc<double>(E->np);
         E->ready = E->alloc<int>(E->np / NB);
         if (E->use_cg) {
+            // every workgroup of the persistent kernel must be resident at once
             const int nbk = E->np / NB;
-            E->cg_groups = std::min(nbk, 256);
-            if (cdiv(nbk, E->cg_groups) > PCG_MAXB) E->use_cg = false;
+            int dev = 0, ncu = 0, per_cu = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eeq_pcg, PCG_THREADS, 0) != hipSuccess) {
+                status = kErrDevice;
+                msg = "PCG occupancy query failed";
+                delete E;
+                return nullptr;
+            }
+            const int G = std::min(ncu * per_cu, 256);
+            // column chunks: least ceil(tiles / G) * (chunk width + 1), the
+            // critical path of tiles plus their epilogues
+            int best = 1, best_cost = INT_MAX;
+            for (int S = 1; S <= std::min(PCG_MAXCHUNK, nbk); ++S) {
+                const int w = cdiv(nbk, S), S2 = cdiv(nbk, w);
+                const int cost = cdiv(nbk * S2, std::max(G, 1)) * (w + 1);
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best = S2;
+                }
+            }
+            E->cg_chunks = best;
+            E->cg_cwb = cdiv(nbk, best);
+            E->cg_groups = std::min(G, nbk * best);
+            if (E->cg_groups < 1 || cdiv(nbk, E->cg_groups) > PCG_MAXB) E->use_cg = false;
         }
         if (E->use_cg) {
             E->cg_r = E->alloc<double>(E->np);
             E->cg_z = E->alloc<double>(E->np);
             E->cg_p = E->alloc<double>(2 * (size_t)E->np);
-            E->cg_part = E->alloc<double>(4 * (size_t)E->cg_groups);
-            E->cg_bar = E->alloc<int>(2);
+            E->cg_q = E->alloc<double>((size_t)E->cg_chunks * E->np);
+            E->cg_mb = E->alloc<unsigned long long>(MB_WORDS);
             E->cg_iters = E->alloc<int>(1);
         } else if (E->mixed && E->nmol == 1) {
             E->Ared32 = E->alloc<float>((size_t)E->np * E->np);
@@ -3469,9 +3603,10 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     } else if (E->use_cg) {
         // block-Jacobi factors of R's diagonal blocks, then the persistent PCG
         LAUNCH(KC_CHOL_DIAG, k_bjacobi_setup, dim3(nbk), dim3(256), s, np, E->Ared, E->Linv, E->flags);
-        hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, s, 2, E->cg_bar, 0);
-        LAUNCH(KC_PCG, k_eeq_pcg, dim3(E->cg_groups), dim3(256), s, np, E->Ared, E->Linv, E->b, E->xbuf, E->cg_r,
-               E->cg_z, E->cg_p, E->cg_part, E->cg_bar, E->cg_iters, PCG_TOL2, PCG_MAXIT, E->flags);
+        hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(MB_WORDS, 256)), dim3(256), 0, s, MB_WORDS, E->cg_mb, MB_EMPTY);
+        LAUNCH(KC_PCG, k_eeq_pcg, dim3(E->cg_groups), dim3(PCG_THREADS), s, np, E->cg_chunks, E->cg_cwb, E->Ared,
+               E->Linv, E->b, E->xbuf, E->cg_r, E->cg_z, E->cg_p, E->cg_q, E->cg_mb, E->cg_iters,
+               PCG_TOL2, PCG_MAXIT, E->flags);
     } else if (!E->mixed) {
         queue_cholesky(&E->timer, s, np, E->Ared, E->Linv, E->flags, E->use_lookahead ? &E->la : nullptr);
         queue_solve<double>(E, s, E->Ared, E->Linv, E->b, E->ybuf, E->xbuf);
