@@ -1939,37 +1939,44 @@ __global__ __launch_bounds__(256) void k_brep(int nb, const int* __restrict__ ij
 }
 
 // hydrogen bonds, cases 1-4: ff_workspace_gfnff.cpp:938-1404
-// Evaluates hydrogen bond h once: returns its energy and writes the gradient
-// of every participating atom to the bond's slots s0 + role (0 = A, 1 = H,
-// 2 = B, 3 + k = k-th neighbour of B, then for case 3 the neighbours D of the
-// carbonyl carbon other than B).  Case 3: B is a carbonyl O whose only
-// neighbour (slot 3) is the carbon C; case 4: sp2 N acceptor with a virtual
-// lone pair opposite its neighbours.
+// hb_core evaluates one triplet A-H...B: returns its energy and the gradient
+// on A, H and B; with XTRA it also writes the gradient of the acceptor's
+// neighbours to xs[k] (k-th neighbour of B) and, for case 3, of the
+// neighbours D of the carbonyl carbon other than B to xs[nnb + d].  Case 3: B
+// is a carbonyl O whose only neighbour is the carbon C; case 4: sp2 N
+// acceptor with a virtual lone pair opposite its neighbours.  The per-triplet
+// parameters of the reference (basicity, acidity, topology charges) are
+// per-atom values, and the neighbour lists depend on B only.
 struct HBArgs {
-    const int* ahb;
-    const int* hcase;
-    const double* hp;
-    const int* nb_off;
+    const double* cb;      // basicity per atom
+    const double* ca;      // acidity per atom
+    const double* Qh;      // per atom e^(15 q) / (e^(15 q) + 1), q the topology charge (role H)
+    const double* Qab;     // per atom e^(-15 q) / (e^(-15 q) + 1) (roles A, B)
+    const int* nb_off;     // per atom: neighbours of an acceptor of cases 2-4 (CSR)
     const int* nb_idx;
-    const int* parentC;
-    const int* nbc_off;
+    const int* parentC;    // per atom: carbonyl carbon of a case-3 acceptor, else -1
+    const int* nbc_off;    // per atom: neighbours of parentC other than the acceptor (CSR)
     const int* nbc_idx;
     const double* hb_rad;
     const double* repz;
     const int* Z;
 };
-__device__ double hb_eval(int h, int s0, const HBArgs& a, const double* __restrict__ x,
-                          double* __restrict__ slots) {
-    int A = a.ahb[3 * h], H = a.ahb[3 * h + 1], B = a.ahb[3 * h + 2];
-    int cs = a.hcase[h];
-    const double* q = a.hp + 7 * h;
-    double basA = q[0], basB = q[1], aciA = q[2], aciB = q[3];
+__device__ inline double pow6(double t) {
+    const double t2 = t * t;
+    return t2 * t2 * t2;
+}
+
+template <bool XTRA>
+__device__ double hb_core(int A, int H, int B, int cs, const HBArgs& a, const double* __restrict__ x,
+                          double3& ga_out, double3& gh_out, double3& gb_out, double* __restrict__ xs) {
+    const double basA = a.cb[A], basB = a.cb[B], aciA = a.ca[A], aciB = a.ca[B];
+
     double3 pA = ld3(x, A), pH = ld3(x, H), pB = ld3(x, B);
     double rAH = nrm(pH - pA), rHB = nrm(pB - pH), rAB = nrm(pB - pA);
-    const int nb0 = a.nb_off[h], nb1 = a.nb_off[h + 1];
-    const int C3 = (cs == 3) ? a.parentC[h] : -1;
-    const int c0 = (C3 >= 0) ? a.nbc_off[h] : 0, c1 = (C3 >= 0) ? a.nbc_off[h + 1] : 0;
-    const int sD = s0 + 3 + (nb1 - nb0);  // first slot of the D atoms
+    const int nb0 = (cs >= 2) ? a.nb_off[B] : 0, nb1 = (cs >= 2) ? a.nb_off[B + 1] : 0;
+    const int C3 = (cs == 3) ? a.parentC[B] : -1;
+    const int c0 = (C3 >= 0) ? a.nbc_off[B] : 0, c1 = (C3 >= 0) ? a.nbc_off[B + 1] : 0;
+    const int sD = nb1 - nb0;  // first extra slot of the D atoms
     double3 z3 = make_double3(0, 0, 0);
     double3 ga = z3, gb = z3, gh = z3;
     double E = 0.0;
@@ -1977,13 +1984,13 @@ __device__ double hb_eval(int h, int s0, const HBArgs& a, const double* __restri
     if (rAB <= 50.0) {
         double rAH4 = rAH * rAH * rAH * rAH, rHB4 = rHB * rHB * rHB * rHB;
         double den = 1.0 / (rAH4 + rHB4);
-        double eH = exp(15.0 * q[4]), eA = exp(15.0 * -q[5]), eB = exp(15.0 * -q[6]);
-        double QH = eH / (eH + 1.0), QA = eA / (eA + 1.0), QB = eB / (eB + 1.0);
+        const double QH = a.Qh[H], QA = a.Qab[A], QB = a.Qab[B];
         double bas = (QA * basA * rAH4 + QB * basB * rHB4) * den;
         double aci = (aciB * rAH4 + aciA * rHB4) * den;
         double radab = a.hb_rad[A] + a.hb_rad[B];
-        double dsh = 1.0 / (1.0 + pow(22.0 * radab / (rAB * rAB), 6.0));
-        double dlg = 1.0 / (1.0 + pow(rAB * rAB / 85.0, 6.0));
+        const double ratio3 = pow6(22.0 * radab / (rAB * rAB)), ratio1 = pow6(rAB * rAB / 85.0);
+        double dsh = 1.0 / (1.0 + ratio3);
+        double dlg = 1.0 / (1.0 + ratio1);
         double denv = dsh * dlg;
         double expo0 = (49.0 / radab) * ((rAH + rHB) / rAB - 1.0);
         double doutl = (expo0 > 15.0) ? 0.0 : 2.0 / (1.0 + exp(expo0));
@@ -2041,7 +2048,6 @@ __device__ double hb_eval(int h, int s0, const HBArgs& a, const double* __restri
         double3 drab = pA - pB, drah = pA - pH, drbh = pB - pH;
         double rab2 = rAB * rAB, rbh2 = rHB * rHB, rah2 = rAH * rAH;
         double rahprbh = rAH + rHB + 1e-12;
-        double ratio1 = pow(rab2 / 85.0, 6.0), ratio3 = pow(22.0 * radab / rab2, 6.0);
         double ddamp = (-2.0 * 6.0 * ratio1 / (1.0 + ratio1)) + (2.0 * 6.0 * ratio3 / (1.0 + ratio3));
         double expo = (49.0 / radab) * (rahprbh / rAB - 1.0);
         if (!(expo > 15.0)) {
@@ -2096,7 +2102,7 @@ __device__ double hb_eval(int h, int s0, const HBArgs& a, const double* __restri
                         double cc = 2.0 * (phi - M_PI / 2.0) + M_PI;
                         double etk = (1.0 + cos(cc)) * tfc + 0.94;
                         double f = ok ? (etors / etk) * tterm * (-2.0 * sin(cc) * tfc) : 0.0;
-                        st3(slots, sD + (k - c0), f * gp[0]);
+                        if (XTRA) st3(xs, sD + (k - c0), f * gp[0]);
                         gb = gb + f * gp[1];
                         gC = gC + f * gp[2];
                         gh = gh + f * gp[3];
@@ -2121,7 +2127,7 @@ __device__ double hb_eval(int h, int s0, const HBArgs& a, const double* __restri
                     gb = gb + db;
                     double3 gn = (-1.0) * (da + db + pg);
                     if (kk == 3 && C3 >= 0) gn = gn + gC;
-                    st3(slots, s0 + kk, gn);
+                    if (XTRA) st3(xs, kk - 3, gn);
                 }
             } else {
                 double caa = QA * basA, cbb = QB * basB;
@@ -2150,12 +2156,12 @@ __device__ double hb_eval(int h, int s0, const HBArgs& a, const double* __restri
             }
         }
     }
-    st3(slots, s0, ga);
-    st3(slots, s0 + 1, gh);
-    st3(slots, s0 + 2, gb);
-    if (!nb_written) {
-        for (int kk = 3; kk < 3 + nb1 - nb0; ++kk) st3(slots, s0 + kk, z3);
-        for (int k = c0; k < c1; ++k) st3(slots, sD + (k - c0), z3);
+    ga_out = ga;
+    gh_out = gh;
+    gb_out = gb;
+    if (XTRA && !nb_written) {
+        for (int kk = 0; kk < nb1 - nb0; ++kk) st3(xs, kk, z3);
+        for (int k = c0; k < c1; ++k) st3(xs, sD + (k - c0), z3);
     }
     return E;
 }
@@ -2208,43 +2214,161 @@ __global__ __launch_bounds__(256) void k_xb(int nx, const int* __restrict__ axb,
     st3(slots, s0 + 2, gB);
 }
 
-// one thread per hydrogen bond: energy and the gradient slots of its atoms
-__global__ __launch_bounds__(256) void k_hb_full(int nh, HBArgs a, const int* __restrict__ slot0,
-                                                 const double* __restrict__ x, double* __restrict__ slots,
-                                                 double* __restrict__ e_out) {
-    const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= nh) return;
-    e_out[h] = hb_eval(h, slot0[h], a, x, slots);
+// Case-1 triplet (H bonded to neither A nor B): the cs = 1 path of hb_core
+// without the neighbour, lone-pair and carbonyl factors (all 1).
+__device__ double hb_core1(int A, int H, int B, const HBArgs& a, const double* __restrict__ x, double3& ga,
+                           double3& gh, double3& gb) {
+    const double3 z3 = make_double3(0, 0, 0);
+    ga = z3;
+    gh = z3;
+    gb = z3;
+    const double3 pA = ld3(x, A), pH = ld3(x, H), pB = ld3(x, B);
+    const double rAH = nrm(pH - pA), rHB = nrm(pB - pH), rAB = nrm(pB - pA);
+    if (!(rAB <= 50.0)) return 0.0;
+    const double basA = a.cb[A], basB = a.cb[B], aciA = a.ca[A], aciB = a.ca[B];
+    const double QH = a.Qh[H], QA = a.Qab[A], QB = a.Qab[B];
+    const double rAH4 = rAH * rAH * rAH * rAH, rHB4 = rHB * rHB * rHB * rHB;
+    const double den = 1.0 / (rAH4 + rHB4);
+    const double bas = (QA * basA * rAH4 + QB * basB * rHB4) * den;
+    const double aci = (aciB * rAH4 + aciA * rHB4) * den;
+    const double radab = a.hb_rad[A] + a.hb_rad[B];
+    const double rab2 = rAB * rAB;
+    const double ratio3 = pow6(22.0 * radab / rab2), ratio1 = pow6(rab2 / 85.0);
+    const double denv = (1.0 / (1.0 + ratio3)) * (1.0 / (1.0 + ratio1));
+    const double expo0 = (49.0 / radab) * ((rAH + rHB) / rAB - 1.0);
+    const double doutl = (expo0 > 15.0) ? 0.0 : 2.0 / (1.0 + exp(expo0));
+    const double rdamp = denv / (rAB * rAB * rAB);
+    const double qhoutl = QH * doutl;
+    const double E = -bas * aci * rdamp * qhoutl;
+    const double3 drab = pA - pB, drah = pA - pH, drbh = pB - pH;
+    const double rbh2 = rHB * rHB, rah2 = rAH * rAH;
+    const double rahprbh = rAH + rHB + 1e-12;
+    const double ddamp = (-2.0 * 6.0 * ratio1 / (1.0 + ratio1)) + (2.0 * 6.0 * ratio3 / (1.0 + ratio3));
+    const double expo = (49.0 / radab) * (rahprbh / rAB - 1.0);
+    if (expo > 15.0) return E;
+    const double ratio2 = exp(expo);
+    const double caa = QA * basA, cbb = QB * basB;
+    const double rterm = -aci * rdamp * qhoutl, dterm = -aci * bas * qhoutl, sterm = -rdamp * bas * qhoutl;
+    const double aterm = -aci * bas * rdamp * QH;
+    const double tmp = den * den * 4.0;
+    const double dd24a = rah2 * rHB4 * tmp, dd24b = rbh2 * rAH4 * tmp;
+    ga = ((caa - cbb) * dd24a * rterm) * drah;
+    gb = ((cbb - caa) * dd24b * rterm) * drbh;
+    gh = (-1.0) * (ga + gb);
+    const double3 dga = ((aciB - aciA) * dd24a * sterm) * drah, dgb = ((aciA - aciB) * dd24b * sterm) * drbh;
+    ga = ga + dga;
+    gb = gb + dgb;
+    gh = gh + (-1.0) * (dga + dgb);
+    double gi = rdamp * (ddamp - 3.0) / rab2;
+    ga = ga + (gi * dterm) * drab;
+    gb = gb + (-gi * dterm) * drab;
+    const double e2 = (1.0 + ratio2) * (1.0 + ratio2);
+    gi = aterm * 2.0 * ratio2 * expo * rahprbh / e2 / (rahprbh - rAB) / rab2;
+    ga = ga + gi * drab;
+    gb = gb + (-gi) * drab;
+    const double to = -2.0 * aterm * ratio2 * expo / e2 / (rahprbh - rAB);
+    const double3 da = (to / rAH) * drah, db = (to / rHB) * drbh;
+    ga = ga + da;
+    gb = gb + db;
+    gh = gh + (-1.0) * (da + db);
+    return E;
 }
 
-// per-atom sum of the hydrogen-bond gradient slots: one wavefront per atom,
-// lanes stride the atom's slot list (CSR order), fixed butterfly
-__global__ __launch_bounds__(256) void k_hb_gather(int n, const int* __restrict__ off, const int* __restrict__ idx,
-                                                   const double* __restrict__ slots, double* __restrict__ g) {
+// Case-1 hydrogen bonds by role: the case-1 triplets are kept three times,
+// grouped by the atom in role A, H or B (entries: the other two atoms), each
+// group cut into chunks of at most HB_CHUNK triplets.  One wavefront per
+// chunk, one kernel per role, evaluates every triplet of its chunk (lanes
+// stride the chunk; a triplet is evaluated once per role, which costs less
+// than storing and gathering per-triplet gradients), adds the gradient on its
+// own atom (and, for role A, the energy) with a fixed butterfly and writes
+// one result per chunk.
+constexpr int HB_CHUNK = 256;
+struct HBRoles {
+    const int2* ent;
+    const int* goff;    // [ngt + 1]
+    const int* gatom;   // [ngt]
+    int ng0, ng1, ngt;  // role A chunks [0, ng0), H [ng0, ng1), B [ng1, ngt)
+};
+template <int ROLE>
+__global__ __launch_bounds__(256) void k_hb_role(HBRoles r, HBArgs a, const double* __restrict__ x,
+                                                 double* __restrict__ gout, double* __restrict__ egrp) {
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= n) return;
-    double gx = 0, gy = 0, gz = 0;
-    for (int k = off[i] + lane; k < off[i + 1]; k += 64) {
-        const int s = idx[k];
-        gx += slots[3 * (size_t)s];
-        gy += slots[3 * (size_t)s + 1];
-        gz += slots[3 * (size_t)s + 2];
+    const int g0 = (ROLE == 0) ? 0 : (ROLE == 1 ? r.ng0 : r.ng1), g1 = (ROLE == 0) ? r.ng0 : (ROLE == 1 ? r.ng1 : r.ngt);
+    const int w = g0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= g1) return;
+    const int at = r.gatom[w], e0 = r.goff[w], e1 = r.goff[w + 1];
+    double3 g = make_double3(0, 0, 0);
+    double E = 0.0;
+    for (int e = e0 + lane; e < e1; e += 64) {
+        const int2 en = r.ent[e];
+        double3 ga, gh, gb;
+        if (ROLE == 0) {
+            E += hb_core1(at, en.x, en.y, a, x, ga, gh, gb);
+            g = g + ga;
+        } else if (ROLE == 1) {
+            hb_core1(en.x, at, en.y, a, x, ga, gh, gb);
+            g = g + gh;
+        } else {
+            hb_core1(en.x, en.y, at, a, x, ga, gh, gb);
+            g = g + gb;
+        }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-        gx += __shfl_xor(gx, o);
-        gy += __shfl_xor(gy, o);
-        gz += __shfl_xor(gz, o);
+        g.x += __shfl_xor(g.x, o);
+        g.y += __shfl_xor(g.y, o);
+        g.z += __shfl_xor(g.z, o);
+        if (ROLE == 0) E += __shfl_xor(E, o);
     }
     if (lane == 0) {
-        g[3 * i] = gx;
-        g[3 * i + 1] = gy;
-        g[3 * i + 2] = gz;
+        st3(gout, w, g);
+        if (ROLE == 0) egrp[w] = E;
     }
 }
 
+// Hydrogen bonds of cases 2-4, one thread each: energy, and the gradient in
+// the triplet's slots s0 (A), s0 + 1 (H), s0 + 2 (B), s0 + 3 + k (k-th
+// neighbour of B), then the carbonyl's D atoms.
+__global__ __launch_bounds__(256) void k_hb_full(int nh, const int4* __restrict__ ahbc, const int* __restrict__ slot0,
+                                                 HBArgs a, const double* __restrict__ x, double* __restrict__ slots,
+                                                 double* __restrict__ e_out) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    const int4 t = ahbc[h];
+    const int s0 = slot0[h];
+    double3 ga, gh, gb;
+    e_out[h] = hb_core<true>(t.x, t.y, t.z, t.w, a, x, ga, gh, gb, slots + 3 * ((size_t)s0 + 3));
+    st3(slots, s0, ga);
+    st3(slots, s0 + 1, gh);
+    st3(slots, s0 + 2, gb);
+}
 
+// per-atom hydrogen-bond gradient, one wavefront per atom: the atom's
+// case-1 chunk results in roles A, H, B (chunks [crange[r (n + 1) + i],
+// crange[r (n + 1) + i + 1])) and its slots of cases 2-4; lanes stride the
+// lists, fixed butterfly
+__global__ __launch_bounds__(256) void k_hb_gather(int n, const int* __restrict__ crange,
+                                                   const double* __restrict__ gout, const int* __restrict__ xoff,
+                                                   const int* __restrict__ xidx, const double* __restrict__ xslots,
+                                                   double* __restrict__ g) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    double3 s = make_double3(0, 0, 0);
+#pragma unroll
+    for (int role = 0; role < 3; ++role) {
+        const int* cr = crange + role * (n + 1);
+        for (int k = cr[i] + lane; k < cr[i + 1]; k += 64) s = s + ld3(gout, k);
+    }
+    for (int k = xoff[i] + lane; k < xoff[i + 1]; k += 64) s = s + ld3(xslots, xidx[k]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s.x += __shfl_xor(s.x, o);
+        s.y += __shfl_xor(s.y, o);
+        s.z += __shfl_xor(s.z, o);
+    }
+    if (lane == 0) st3(g, i, s);
+}
 
 // D4 ATM over bonded triplets (energy +c9, gradient -c9 as in
 // ff_workspace_gfnff.cpp:1526-1658) and bonded ATM (:1665-1739)
@@ -2456,9 +2580,10 @@ __global__ __launch_bounds__(256) void k_eeq_batch_solve(const int* __restrict__
 struct MolEnergyArgs {
     int nmol;
     const int* mol_off;
-    const int* seg;        // [10][nmol + 1]: bond, angle, torsion, brep, hb, atm, batm, inversion, stors, xb
+    const int* seg;        // [11][nmol + 1]: bond, angle, torsion, brep, hb case 1 (role-A groups), atm, batm,
+                           // inversion, stors, xb, hb cases 2-4
     const double* atom_terms[4];   // coul, disp, rep, self
-    const double* list_terms[10];
+    const double* list_terms[11];
     double* e_mol;
 };
 __global__ __launch_bounds__(256) void k_mol_energy(MolEnergyArgs a) {
@@ -2468,7 +2593,7 @@ __global__ __launch_bounds__(256) void k_mol_energy(MolEnergyArgs a) {
     double acc = 0.0;
     for (int k = 0; k < 4; ++k)
         for (int i = a.mol_off[m] + lane; i < a.mol_off[m + 1]; i += 64) acc += a.atom_terms[k][i];
-    for (int k = 0; k < 10; ++k) {
+    for (int k = 0; k < 11; ++k) {
         const int* sg = a.seg + k * (a.nmol + 1);
         for (int i = sg[m] + lane; i < sg[m + 1]; i += 64) acc += a.list_terms[k][i];
     }
@@ -2804,7 +2929,7 @@ struct Segs {
     int dst[16];
 };
 constexpr int RED_BLOCKS = 64;
-constexpr int NPARTS = 14;  // energy segments reduced per evaluation
+constexpr int NPARTS = 15;  // energy segments reduced per evaluation
 // stage 1: segment sg, block b reduces the fixed chunk b of the segment
 __global__ __launch_bounds__(256) void k_reduce1(Segs s, double* __restrict__ part) {
     __shared__ double red[256];
@@ -2845,7 +2970,7 @@ __global__ void k_total(const double* __restrict__ parts, double* __restrict__ o
     double t[GFNFF_NTERMS];
     for (int k = 0; k < GFNFF_NTERMS; ++k) t[k] = 0.0;
     // parts: 0 bond,1 angle,2 tors,3 inv,4 disp(pairs),5 brep,6 nbrep(pairs incl fix),7 coul pairs,
-    // 8 self,9 hb,10 atm,11 batm,12 triple-bond torsions,13 xb
+    // 8 self,9 hb case 1,10 atm,11 batm,12 triple-bond torsions,13 xb,14 hb cases 2-4
     t[GFNFF_E_BOND] = parts[0];
     t[GFNFF_E_ANGLE] = parts[1];
     t[GFNFF_E_TORSION] = parts[2];
@@ -2854,7 +2979,7 @@ __global__ void k_total(const double* __restrict__ parts, double* __restrict__ o
     t[GFNFF_E_BONDED_REP] = parts[5];
     t[GFNFF_E_NONBONDED_REP] = parts[6];
     t[GFNFF_E_COULOMB] = parts[7] + parts[8];
-    t[GFNFF_E_HBOND] = parts[9];
+    t[GFNFF_E_HBOND] = parts[9] + parts[14];
     t[GFNFF_E_ATM] = parts[10];
     t[GFNFF_E_BATM] = parts[11];
     t[GFNFF_E_STORS] = parts[12];
@@ -2977,9 +3102,14 @@ struct DeviceEngine {
     double *inv_par, *st_par, *xb_par;
     int *brep_ij, *brep_slot;
     double* brep_par;
-    int *hb_ahb, *hb_case, *hb_nboff, *hb_nbidx, *hbs_off, *hbs_idx, *hb_slot0, *hb_parc, *hb_nbcoff, *hb_nbcidx;
-    double* hb_slots;
-    double* hb_par;
+    // hydrogen bonds by role (k_hb_roles): per-acceptor lists (CSR by atom),
+    // role-grouped entries, group results, extra slots and the gather maps
+    int *hb_nboff, *hb_nbidx, *hb_parc, *hb_nbcoff, *hb_nbcidx;
+    int *hb_goff, *hb_gatom, *hb_crange, *hb_s0, *hb_xoff, *hb_xidx;
+    int2* hb_ent;
+    int4* hb_ahbc;
+    int hb_ng0, hb_ng1, hb_ngt, nhb2;
+    double *hb_cb, *hb_ca, *hb_Qh, *hb_Qab, *hb_gout, *hb_xslots, *e_hb2;
     int *atm_ijk, *atm_slot;
     double* atm_sc;
     int *batm_ijk, *batm_slot;
@@ -3283,38 +3413,160 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->brep_slot = E->upload(rslot);
         E->nhb = (int)P.hb_case.size();
         {
-            // gradient slots of the hydrogen bonds, bond-major (3 + neighbours of
-            // B + for case 3 the neighbours of the carbonyl C other than B), and
-            // the per-atom CSR over them in ascending slot order
-            std::vector<int> s0(E->nhb), atom_of;
-            atom_of.reserve((size_t)E->nhb * 5);
-            for (int h = 0; h < E->nhb; ++h) {
-                s0[h] = (int)atom_of.size();
-                for (int r = 0; r < 3; ++r) atom_of.push_back(P.hb_ahb[3 * h + r]);
-                for (int k = P.hb_nbB_off[h]; k < P.hb_nbB_off[h + 1]; ++k) atom_of.push_back(P.hb_nbB_idx[k]);
-                if (P.hb_case[h] == 3 && P.hb_parentC[h] >= 0)
-                    for (int k = P.hb_nbC_off[h]; k < P.hb_nbC_off[h + 1]; ++k) atom_of.push_back(P.hb_nbC_idx[k]);
-                if (atom_of.size() > (size_t)INT32_MAX / 4)
+            // Hydrogen bonds.  The reference's per-triplet parameters are
+            // per-atom values and its neighbour lists depend on the acceptor
+            // only; both are checked while they are gathered.  Case-1 triplets
+            // go to the role lists, cases 2-4 to a compact list with slots.
+            const int nh = E->nhb;
+            if (nh && n >= (1 << 28)) throw std::runtime_error("hydrogen-bond lists need atom indices below 2^28");
+            std::vector<double> cb(n, 0.0), ca(n, 0.0), qt(n, 0.0);
+            std::vector<char> s_cb(n, 0), s_ca(n, 0), s_qt(n, 0);
+            auto put = [](std::vector<double>& v, std::vector<char>& set, int i, double val) {
+                if (set[i] && v[i] != val) throw std::runtime_error("hydrogen-bond parameter is not a per-atom value");
+                v[i] = val;
+                set[i] = 1;
+            };
+            std::vector<std::vector<int>> bnb(n), bD(n);
+            std::vector<int> bC(n, -1);
+            std::vector<char> s_nb(n, 0), s_c(n, 0);
+            for (int h = 0; h < nh; ++h) {
+                const int A = P.hb_ahb[3 * h], H = P.hb_ahb[3 * h + 1], B = P.hb_ahb[3 * h + 2], cs = P.hb_case[h];
+                const double* q = &P.hb_par[7 * (size_t)h];
+                put(cb, s_cb, A, q[0]);
+                put(cb, s_cb, B, q[1]);
+                put(ca, s_ca, A, q[2]);
+                put(ca, s_ca, B, q[3]);
+                put(qt, s_qt, H, q[4]);
+                put(qt, s_qt, A, q[5]);
+                put(qt, s_qt, B, q[6]);
+                if (cs < 1 || cs > 4) throw std::runtime_error("unknown hydrogen-bond case");
+                if (cs >= 2) {
+                    std::vector<int> l(P.hb_nbB_idx.begin() + P.hb_nbB_off[h], P.hb_nbB_idx.begin() + P.hb_nbB_off[h + 1]);
+                    if (s_nb[B] && bnb[B] != l) throw std::runtime_error("acceptor neighbour lists differ between hydrogen bonds");
+                    bnb[B] = l;
+                    s_nb[B] = 1;
+                }
+                if (cs == 3) {
+                    std::vector<int> l(P.hb_nbC_idx.begin() + P.hb_nbC_off[h], P.hb_nbC_idx.begin() + P.hb_nbC_off[h + 1]);
+                    if (s_c[B] && (bC[B] != P.hb_parentC[h] || bD[B] != l))
+                        throw std::runtime_error("carbonyl lists differ between hydrogen bonds");
+                    bC[B] = P.hb_parentC[h];
+                    bD[B] = l;
+                    s_c[B] = 1;
+                }
+            }
+            // charge switches Q(+q) for H and Q(-q) for A and B
+            std::vector<double> Qh(n), Qab(n);
+            for (int i = 0; i < n; ++i) {
+                const double eh = std::exp(15.0 * qt[i]), em = std::exp(15.0 * -qt[i]);
+                Qh[i] = eh / (eh + 1.0);
+                Qab[i] = em / (em + 1.0);
+            }
+            auto csr = [&](const std::vector<std::vector<int>>& L, std::vector<int>& off, std::vector<int>& idx) {
+                off.assign(n + 1, 0);
+                for (int i = 0; i < n; ++i) off[i + 1] = off[i] + (int)L[i].size();
+                idx.clear();
+                for (int i = 0; i < n; ++i) idx.insert(idx.end(), L[i].begin(), L[i].end());
+            };
+            std::vector<int> nboff, nbidx, nbcoff, nbcidx;
+            csr(bnb, nboff, nbidx);
+            csr(bD, nbcoff, nbcidx);
+            // case 1: role lists (counting sort by the role's atom, triplet order within a group)
+            std::vector<int> c1;
+            for (int h = 0; h < nh; ++h)
+                if (P.hb_case[h] == 1) c1.push_back(h);
+            std::vector<int2> ent((size_t)3 * c1.size());
+            std::vector<int> goff(1, 0), gatom, crange((size_t)3 * (n + 1), 0);
+            int ngr[3] = {0, 0, 0};
+            size_t eo = 0;
+            for (int role = 0; role < 3; ++role) {
+                std::vector<int> cnt(n + 1, 0);
+                for (int h : c1) cnt[P.hb_ahb[3 * h + role] + 1]++;
+                for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+                std::vector<int> order(c1.size()), fill(cnt.begin(), cnt.end() - 1);
+                for (int h : c1) order[fill[P.hb_ahb[3 * h + role]]++] = h;
+                for (int i = 0; i < n; ++i) {
+                    crange[(size_t)role * (n + 1) + i] = (int)gatom.size();
+                    for (int k = cnt[i]; k < cnt[i + 1]; ++k) {
+                        const int h = order[k];
+                        const int A = P.hb_ahb[3 * h], H = P.hb_ahb[3 * h + 1], B = P.hb_ahb[3 * h + 2];
+                        ent[eo++] = role == 0 ? make_int2(H, B) : (role == 1 ? make_int2(A, B) : make_int2(A, H));
+                        if ((k - cnt[i]) % HB_CHUNK == HB_CHUNK - 1 || k + 1 == cnt[i + 1]) {
+                            gatom.push_back(i);
+                            goff.push_back((int)eo);
+                        }
+                    }
+                }
+                crange[(size_t)role * (n + 1) + n] = (int)gatom.size();
+                ngr[role] = (int)gatom.size();
+            }
+            E->hb_ng0 = ngr[0];
+            E->hb_ng1 = ngr[1];
+            E->hb_ngt = ngr[2];
+            // cases 2-4: compact list, slots triplet-major (A, H, B, neighbours
+            // of B, carbonyl D atoms) and the per-atom CSR over them
+            std::vector<int4> ahbc;
+            std::vector<int> s0v, xatom, c2mol;
+            for (int h = 0; h < nh; ++h) {
+                const int cs = P.hb_case[h];
+                if (cs == 1) continue;
+                const int A = P.hb_ahb[3 * h], H = P.hb_ahb[3 * h + 1], B = P.hb_ahb[3 * h + 2];
+                ahbc.push_back(make_int4(A, H, B, cs));
+                s0v.push_back((int)xatom.size());
+                xatom.push_back(A);
+                xatom.push_back(H);
+                xatom.push_back(B);
+                for (int a2 : bnb[B]) xatom.push_back(a2);
+                if (cs == 3)
+                    for (int a2 : bD[B]) xatom.push_back(a2);
+                c2mol.push_back(h);
+                if (xatom.size() > (size_t)INT32_MAX / 4)
                     throw std::runtime_error("hydrogen-bond slot count exceeds the index range");
             }
-            std::vector<int> cnt(n + 1, 0), ent(atom_of.size());
-            for (int at : atom_of) cnt[at + 1]++;
-            for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
-            std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-            for (size_t k = 0; k < atom_of.size(); ++k) ent[fill[atom_of[k]]++] = (int)k;
-            E->hbs_off = E->upload(cnt);
-            E->hbs_idx = E->upload(ent);
-            E->hb_slot0 = E->upload(s0);
-            E->hb_slots = E->alloc<double>(3 * std::max<size_t>(atom_of.size(), 1));
+            E->nhb2 = (int)ahbc.size();
+            std::vector<int> xoff(n + 1, 0), xidx(xatom.size());
+            for (int at : xatom) xoff[at + 1]++;
+            for (int i = 0; i < n; ++i) xoff[i + 1] += xoff[i];
+            {
+                std::vector<int> fill(xoff.begin(), xoff.end() - 1);
+                for (size_t k = 0; k < xatom.size(); ++k) xidx[fill[xatom[k]]++] = (int)k;
+            }
+            E->hb_cb = E->upload(cb);
+            E->hb_ca = E->upload(ca);
+            E->hb_Qh = E->upload(Qh);
+            E->hb_Qab = E->upload(Qab);
+            E->hb_nboff = E->upload(nboff);
+            E->hb_nbidx = E->upload(nbidx);
+            E->hb_parc = E->upload(bC);
+            E->hb_nbcoff = E->upload(nbcoff);
+            E->hb_nbcidx = E->upload(nbcidx);
+            E->hb_ent = E->upload(ent);
+            E->hb_goff = E->upload(goff);
+            E->hb_gatom = E->upload(gatom);
+            E->hb_crange = E->upload(crange);
+            E->hb_ahbc = E->upload(ahbc);
+            E->hb_s0 = E->upload(s0v);
+            E->hb_xoff = E->upload(xoff);
+            E->hb_xidx = E->upload(xidx);
+            E->hb_gout = E->alloc<double>(3 * (size_t)std::max(E->hb_ngt, 1));
+            E->hb_xslots = E->alloc<double>(3 * std::max<size_t>(xatom.size(), 1));
+            E->e_hb2 = E->alloc<double>(std::max(E->nhb2, 1));
+            if (E->nmol > 1) {
+                // per-molecule segments: row 4 the case-1 role-A chunks of the
+                // molecule's atoms (chunks ascend in atom index), row 10 its
+                // triplets of cases 2-4 (in triplet order, molecule-major)
+                const int nm1 = E->nmol + 1;
+                std::vector<int> ms(P.mol_seg.begin(), P.mol_seg.begin() + 10 * nm1);
+                ms.resize(11 * nm1);
+                for (int m = 0; m <= E->nmol; ++m) {
+                    const int a0 = (m < E->nmol) ? P.mol_off[m] : n;
+                    ms[4 * nm1 + m] = (int)(std::lower_bound(gatom.begin(), gatom.begin() + ngr[0], a0) - gatom.begin());
+                    const int h0 = P.mol_seg[4 * nm1 + m];
+                    ms[10 * nm1 + m] = (int)(std::lower_bound(c2mol.begin(), c2mol.end(), h0) - c2mol.begin());
+                }
+                E->mol_seg = E->upload(ms);
+            }
         }
-        E->hb_ahb = E->upload(P.hb_ahb);
-        E->hb_case = E->upload(P.hb_case);
-        E->hb_par = E->upload(P.hb_par);
-        E->hb_nboff = E->upload(P.hb_nbB_off);
-        E->hb_nbidx = E->upload(P.hb_nbB_idx);
-        E->hb_parc = E->upload(P.hb_parentC);
-        E->hb_nbcoff = E->upload(P.hb_nbC_off);
-        E->hb_nbcidx = E->upload(P.hb_nbC_idx);
         E->natm = (int)P.atm_scale.size();
         std::vector<int> atslot(E->natm);
         for (int t = 0; t < E->natm; ++t) {
@@ -3455,7 +3707,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->e_st = E->alloc<double>(std::max(E->nstors, 1));
         E->e_xb = E->alloc<double>(std::max(E->nxb, 1));
         E->e_brep = E->alloc<double>(std::max(E->nbrep, 1));
-        E->e_hb = E->alloc<double>(std::max(E->nhb, 1));
+        E->e_hb = E->alloc<double>(std::max(E->hb_ng0, 1));
         E->e_atm = E->alloc<double>(std::max(E->natm, 1));
         E->e_batm = E->alloc<double>(std::max(E->nbatm, 1));
         E->parts = E->alloc<double>(16);
@@ -3681,12 +3933,23 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         LAUNCH(KC_BREP, k_brep, dim3(cdiv(E->nbrep, 256)), dim3(256), s, E->nbrep, E->brep_ij, E->brep_par,
                            E->brep_slot, E->x, E->slots, E->e_brep);
     if (E->nhb) {
-        HBArgs ha{E->hb_ahb, E->hb_case, E->hb_par, E->hb_nboff, E->hb_nbidx, E->hb_parc,
+        HBArgs ha{E->hb_cb, E->hb_ca, E->hb_Qh, E->hb_Qab, E->hb_nboff, E->hb_nbidx, E->hb_parc,
                   E->hb_nbcoff, E->hb_nbcidx, E->hb_rad, E->repz, E->Z};
-        LAUNCH(KC_HB, k_hb_full, dim3(cdiv(E->nhb, 256)), dim3(256), s, E->nhb, ha, E->hb_slot0, E->x, E->hb_slots,
-               E->e_hb);
+        HBRoles hr{E->hb_ent, E->hb_goff, E->hb_gatom, E->hb_ng0, E->hb_ng1, E->hb_ngt};
+        if (E->hb_ng0)
+            LAUNCH(KC_HB, k_hb_role<0>, dim3(cdiv(E->hb_ng0, 4)), dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+        if (E->hb_ng1 > E->hb_ng0)
+            LAUNCH(KC_HB, k_hb_role<1>, dim3(cdiv(E->hb_ng1 - E->hb_ng0, 4)), dim3(256), s, hr, ha, E->x, E->hb_gout,
+                   E->e_hb);
+        if (E->hb_ngt > E->hb_ng1)
+            LAUNCH(KC_HB, k_hb_role<2>, dim3(cdiv(E->hb_ngt - E->hb_ng1, 4)), dim3(256), s, hr, ha, E->x, E->hb_gout,
+                   E->e_hb);
+        if (E->nhb2)
+            LAUNCH(KC_HB, k_hb_full, dim3(cdiv(E->nhb2, 256)), dim3(256), s, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
+                   E->hb_xslots, E->e_hb2);
     }
-    LAUNCH(KC_HB, k_hb_gather, dim3(cdiv(n, 4)), dim3(256), s, n, E->hbs_off, E->hbs_idx, E->hb_slots, E->ghb);
+    LAUNCH(KC_HB, k_hb_gather, dim3(cdiv(n, 4)), dim3(256), s, n, E->hb_crange, E->hb_gout, E->hb_xoff, E->hb_xidx,
+           E->hb_xslots, E->ghb);
     if (E->natm)
         LAUNCH(KC_ATM, k_atm, dim3(cdiv(E->natm, 256)), dim3(256), s, E->natm, E->atm_ijk, E->atm_sc,
                            E->atm_slot, E->x, E->rcov_d3, E->nelem, E->elem, E->nref, E->c6ref, E->gw, E->slots,
@@ -3699,10 +3962,10 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     LAUNCH(KC_CHAIN, k_finish, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->rcov_cn, E->dlog, E->vchain,
                        E->nbn, E->nbl, E->slot_off, E->slot_idx, E->slots, E->gpair, E->ghb, E->grad);
     Segs sg;
-    const double* ptrs[NPARTS] = {E->e_bond, E->e_ang,  E->e_tor, E->e_inv, E->e_disp, E->e_brep, E->e_rep,
-                                  E->e_coul, E->eself, E->e_hb,  E->e_atm, E->e_batm, E->e_st,  E->e_xb};
-    int cnts[NPARTS] = {E->nbond, E->nangle, E->ntors, E->ninv, n, E->nbrep, n, n, n, E->nhb, E->natm, E->nbatm,
-                        E->nstors, E->nxb};
+    const double* ptrs[NPARTS] = {E->e_bond, E->e_ang,  E->e_tor, E->e_inv, E->e_disp, E->e_brep, E->e_rep, E->e_coul,
+                                  E->eself,  E->e_hb,   E->e_atm, E->e_batm, E->e_st,  E->e_xb,   E->e_hb2};
+    int cnts[NPARTS] = {E->nbond, E->nangle, E->ntors, E->ninv, n, E->nbrep, n, n, n, E->hb_ng0, E->natm, E->nbatm,
+                        E->nstors, E->nxb, E->nhb2};
     for (int k = 0; k < NPARTS; ++k) {
         sg.p[k] = ptrs[k];
         sg.n[k] = cnts[k];
@@ -3720,9 +3983,9 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         ma.atom_terms[1] = E->e_disp;
         ma.atom_terms[2] = E->e_rep;
         ma.atom_terms[3] = E->eself;
-        const double* lists[10] = {E->e_bond, E->e_ang,  E->e_tor, E->e_brep, E->e_hb,
-                                   E->e_atm,  E->e_batm, E->e_inv, E->e_st,   E->e_xb};
-        for (int k = 0; k < 10; ++k) ma.list_terms[k] = lists[k];
+        const double* lists[11] = {E->e_bond, E->e_ang,  E->e_tor, E->e_brep, E->e_hb, E->e_atm,
+                                   E->e_batm, E->e_inv, E->e_st,  E->e_xb,   E->e_hb2};
+        for (int k = 0; k < 11; ++k) ma.list_terms[k] = lists[k];
         ma.e_mol = E->e_mol;
         LAUNCH(KC_REDUCE, k_mol_energy, dim3(cdiv(E->nmol, 4)), dim3(256), s, ma);
     }
