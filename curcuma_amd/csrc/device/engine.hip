@@ -1651,13 +1651,23 @@ __global__ __launch_bounds__(256) void k_nbrep_fix(int n, const double* __restri
 // (gathered per atom in k_finish in a fixed order) and its energy to e_out.
 // ---------------------------------------------------------------------------
 // HB coordination of X-H hydrogens (ff_workspace_gfnff.cpp:46-97) and bonds (:104-180)
+// one wavefront per bond: lanes stride the bond's hydrogen-bond partner list
+// (HB coordination and its gradient), fixed butterflies; lane 0 writes the
+// bond's own slots
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
 __global__ __launch_bounds__(256) void k_bonds(int nb, const int* __restrict__ ij, const double* __restrict__ par,
                                                const int* __restrict__ hb_off, const int* __restrict__ hb_B,
                                                const int* __restrict__ slot0, const double* __restrict__ x,
                                                const double* __restrict__ cn, const double* __restrict__ rcov_d3,
                                                const int* __restrict__ bond_H, double* __restrict__ slots,
                                                double* __restrict__ dcn_slot, double* __restrict__ e_out) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (b >= nb) return;
     int i = ij[2 * b], j = ij[2 * b + 1];
     const double* p = par + 8 * b;
@@ -1666,11 +1676,12 @@ __global__ __launch_bounds__(256) void k_bonds(int nb, const int* __restrict__ i
     double r0 = (p[0] + p[2] * cn[i] + p[1] + p[3] * cn[j] + p[4]) * p[5];
     double dr = r - r0;
     double a0 = p[6], al = a0;
-    int H = bond_H[b];
-    double hbcn = 0.0;
+    const int H = bond_H[b];
+    const int k0 = (H >= 0) ? hb_off[b] : 0, k1 = (H >= 0) ? hb_off[b + 1] : 0;
     if (H >= 0) {
-        double3 xh = ld3(x, H);
-        for (int k = hb_off[b]; k < hb_off[b + 1]; ++k) {
+        const double3 xh = ld3(x, H);
+        double hbcn = 0.0;
+        for (int k = k0 + lane; k < k1; k += 64) {
             int B = hb_B[k];
             double3 v = ld3(x, B) - xh;
             double rr2 = dot(v, v);
@@ -1679,23 +1690,25 @@ __global__ __launch_bounds__(256) void k_bonds(int nb, const int* __restrict__ i
             double rc = 1.78 * (4.0 / 3.0) * (rcov_d3[H] + rcov_d3[B]);
             hbcn += 0.5 * (1.0 + erf(-27.5 * (rr - rc) / rc));
         }
+        hbcn = wave_sum(hbcn);
         al = (-(1.0 - 0.9) * hbcn + 1.0) * a0;
     }
     double e = p[7] * exp(-al * dr * dr);
-    e_out[b] = e;
     double dEdr = -2.0 * al * dr * e;
     int s = slot0[b];
-    st3(slots, s, (dEdr / r) * d);
-    st3(slots, s + 1, (-dEdr / r) * d);
-    double yy = -dEdr;
-    dcn_slot[2 * b] = yy * p[5] * p[2];
-    dcn_slot[2 * b + 1] = yy * p[5] * p[3];
+    if (lane == 0) {
+        e_out[b] = e;
+        st3(slots, s, (dEdr / r) * d);
+        st3(slots, s + 1, (-dEdr / r) * d);
+        double yy = -dEdr;
+        dcn_slot[2 * b] = yy * p[5] * p[2];
+        dcn_slot[2 * b + 1] = yy * p[5] * p[3];
+    }
     if (H >= 0) {
         double zz = 0.1 * a0 * dr * dr * e;
-        double3 xh = ld3(x, H);
+        const double3 xh = ld3(x, H);
         double3 gH = make_double3(0, 0, 0);
-        int q = s + 3;
-        for (int k = hb_off[b]; k < hb_off[b + 1]; ++k, ++q) {
+        for (int k = k0 + lane; k < k1; k += 64) {
             int B = hb_B[k];
             double3 v = ld3(x, B) - xh;
             double rr2 = dot(v, v);
@@ -1708,9 +1721,12 @@ __global__ __launch_bounds__(256) void k_bonds(int nb, const int* __restrict__ i
                 gH = gH + (-zz * dc) * v;
                 gB = (zz * dc) * v;
             }
-            st3(slots, q, gB);
+            st3(slots, s + 3 + (k - k0), gB);
         }
-        st3(slots, s + 2, gH);
+        gH.x = wave_sum(gH.x);
+        gH.y = wave_sum(gH.y);
+        gH.z = wave_sum(gH.z);
+        if (lane == 0) st3(slots, s + 2, gH);
     }
 }
 
@@ -2894,19 +2910,23 @@ __global__ __launch_bounds__(256) void k_dcn_total(int n, const int* __restrict_
     v[i] = s - qtmp[i];
 }
 
+// one wavefront per atom: lanes stride the atom's slot list and its CN
+// neighbour list (chain rule), fixed butterfly
 __global__ __launch_bounds__(256) void k_finish(int n, const double* __restrict__ x, const double* __restrict__ rcov,
                                                 const double* __restrict__ dlog, const double* __restrict__ v,
                                                 const int* __restrict__ nbn, const int* __restrict__ nbl,
                                                 const int* __restrict__ soff, const int* __restrict__ sidx,
                                                 const double* __restrict__ slots, const double* __restrict__ gpair,
                                                 const double* __restrict__ ghb, double* __restrict__ grad) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= n) return;
-    double3 g = ld3(gpair, i) + ld3(ghb, i);
-    for (int k = soff[i]; k < soff[i + 1]; ++k) g = g + ld3(slots, sidx[k]);
-    double3 xi = ld3(x, i);
-    double ai = dlog[i] * v[i];
-    for (int k = 0; k < nbn[i]; ++k) {
+    double3 g = make_double3(0, 0, 0);
+    for (int k = soff[i] + lane; k < soff[i + 1]; k += 64) g = g + ld3(slots, sidx[k]);
+    const double3 xi = ld3(x, i);
+    const double ai = dlog[i] * v[i];
+    const int nn = nbn[i];
+    for (int k = lane; k < nn; k += 64) {
         int j = nbl[(size_t)i * MAXNB_CN + k];
         double3 d = ld3(x, j) - xi;
         double r = nrm(d);
@@ -2916,9 +2936,13 @@ __global__ __launch_bounds__(256) void k_finish(int n, const double* __restrict_
         double s = dd * (ai + dlog[j] * v[j]);
         g = g + (-s / r) * d;
     }
-    grad[3 * i] = g.x;
-    grad[3 * i + 1] = g.y;
-    grad[3 * i + 2] = g.z;
+    g.x = wave_sum(g.x);
+    g.y = wave_sum(g.y);
+    g.z = wave_sum(g.z);
+    if (lane == 0) {
+        g = g + ld3(gpair, i) + ld3(ghb, i);
+        st3(grad, i, g);
+    }
 }
 
 // fixed-shape reduction of up to 16 segments into out[seg]; one workgroup
@@ -3911,7 +3935,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
                        E->toff, E->tidx, E->tdist, E->gpair, E->e_rep);
     // bonded lists
     if (E->nbond)
-        LAUNCH(KC_BONDS, k_bonds, dim3(cdiv(E->nbond, 256)), dim3(256), s, E->nbond, E->bond_ij, E->bond_par,
+        LAUNCH(KC_BONDS, k_bonds, dim3(cdiv(E->nbond, 4)), dim3(256), s, E->nbond, E->bond_ij, E->bond_par,
                            E->bond_hboff, E->bond_hbB, E->bond_slot, E->x, E->cn, E->rcov_d3, E->bond_H, E->slots,
                            E->dcn_slot, E->e_bond);
     if (E->nangle)
@@ -3959,7 +3983,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
                            E->batm_slot, E->x, E->qa, E->slots, E->e_batm);
     LAUNCH(KC_CHAIN, k_dcn_total, dim3(cdiv(n, 256)), dim3(256), s, n, E->dcn_off, E->dcn_idx, E->dcn_slot,
                        E->dEdcn, E->qtmp, E->vchain);
-    LAUNCH(KC_CHAIN, k_finish, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->rcov_cn, E->dlog, E->vchain,
+    LAUNCH(KC_CHAIN, k_finish, dim3(cdiv(n, 4)), dim3(256), s, n, E->x, E->rcov_cn, E->dlog, E->vchain,
                        E->nbn, E->nbl, E->slot_off, E->slot_idx, E->slots, E->gpair, E->ghb, E->grad);
     Segs sg;
     const double* ptrs[NPARTS] = {E->e_bond, E->e_ang,  E->e_tor, E->e_inv, E->e_disp, E->e_brep, E->e_rep, E->e_coul,
