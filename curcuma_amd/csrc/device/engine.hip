@@ -203,41 +203,132 @@ __global__ __launch_bounds__(256) void k_eeq_diag(int n, const double* __restric
     if (i < n) diag[i] = gam[i] + TSQRT2PI / sqrt(alp[i]);
 }
 
+// Reduced variables of one fragment are consecutive and share the last atom
+// la, so a 64-block of variables touches few distinct last atoms (32 for
+// water, 1 for one large molecule).  Per tile, the A entries that involve a
+// last atom, A(ia, lb), A(la, ib) and A(la, lb), are evaluated once per
+// distinct last atom into LDS (ul: the block's distinct last atoms, lidx: a
+// variable's index among them), and every element then needs one fresh
+// A(ia, ib).  The sum is formed in the same order as the direct form, so R is
+// bit-identical; tiles with more than RED_UMAX distinct last atoms use the
+// direct four-term form.
+constexpr int RED_UMAX = 32;
+// eeq_a for two distinct atoms from staged positions and alphas (the same
+// operations as eeq_a, so the same bits)
+__device__ inline double eeq_a_v(double3 pi, double ai, double3 pj, double aj) {
+    double3 d = pi - pj;
+    double r = nrm(d);
+    double g = 1.0 / sqrt(ai + aj);
+    return erf(g * r) / r;
+}
+struct RedSm {
+    union {
+        struct {
+            double Aiu[NB][RED_UMAX + 1], Aui[RED_UMAX][NB + 1], Auu[RED_UMAX][RED_UMAX + 1];
+        } c;
+        double Tl[NB][NB + 1];  // the transposed tile, after the caches are no longer read
+    };
+    double3 rp[NB], cp[NB], urp[RED_UMAX], ucp[RED_UMAX];
+    double ra_[NB], ca_[NB], ura[RED_UMAX], uca[RED_UMAX];
+    int sia[NB], sib[NB], slr[NB], slc[NB], sur[RED_UMAX], suc[RED_UMAX];
+};
 __global__ __launch_bounds__(256) void k_eeq_reduced(int nr, int np, const int* __restrict__ ra,
-                                                     const int* __restrict__ rl, const double* __restrict__ x,
-                                                     const double* __restrict__ alp,
+                                                     const int* __restrict__ rl, const int* __restrict__ lidx,
+                                                     const int* __restrict__ ul, const int* __restrict__ nul,
+                                                     const double* __restrict__ x, const double* __restrict__ alp,
                                                      const double* __restrict__ diag, double* __restrict__ A,
                                                      int full) {
-    // one 64x64 lower tile per workgroup; grid enumerates tiles (bi >= bj).
-    // full != 0 also writes the mirrored upper tile (both triangles stored,
-    // for the row-major matrix-vector products of the iterative solver)
-    __shared__ double Tl[NB][NB + 1];
+    // one 64x64 lower tile per workgroup of 256 threads; grid enumerates
+    // tiles (bi >= bj).  full != 0 also writes the mirrored upper tile (both
+    // triangles stored, for the row-major matrix-vector products of the
+    // iterative solver)
+    __shared__ RedSm sm;
     int t = blockIdx.x;
     int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
     while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
     while (bi * (bi + 1) / 2 > t) --bi;
     int bj = t - bi * (bi + 1) / 2;
-    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
-        int r = bi * NB + e / NB, c = bj * NB + e % NB;
-        if (c > r) continue;
-        double v;
-        if (r >= nr || c >= nr) {
-            v = (r == c) ? 1.0 : 0.0;
-        } else {
-            int ia = ra[r], la = rl[r], ib = ra[c], lb = rl[c];
-            v = eeq_a(ia, ib, x, alp, diag) - eeq_a(ia, lb, x, alp, diag) - eeq_a(la, ib, x, alp, diag) +
-                eeq_a(la, lb, x, alp, diag);
+    const int tid = threadIdx.x;
+    const int nur = nul[bi], nuc = nul[bj];
+    const bool ded = nur <= RED_UMAX && nuc <= RED_UMAX;
+    if (tid < NB) {
+        const int r = bi * NB + tid, c = bj * NB + tid;
+        const int ia = (r < nr) ? ra[r] : -1, ib = (c < nr) ? ra[c] : -1;
+        sm.sia[tid] = ia;
+        sm.slr[tid] = (r < nr) ? lidx[r] : 0;
+        sm.sib[tid] = ib;
+        sm.slc[tid] = (c < nr) ? lidx[c] : 0;
+        sm.rp[tid] = ld3(x, max(ia, 0));
+        sm.ra_[tid] = alp[max(ia, 0)];
+        sm.cp[tid] = ld3(x, max(ib, 0));
+        sm.ca_[tid] = alp[max(ib, 0)];
+    } else if (ded && tid < NB + RED_UMAX) {
+        const int u = tid - NB;
+        if (u < nur) {
+            const int a = ul[bi * NB + u];
+            sm.sur[u] = a;
+            sm.urp[u] = ld3(x, a);
+            sm.ura[u] = alp[a];
         }
-        A[(size_t)r * np + c] = v;
-        if (full) Tl[e / NB][e % NB] = v;
+        if (u < nuc) {
+            const int a = ul[bj * NB + u];
+            sm.suc[u] = a;
+            sm.ucp[u] = ld3(x, a);
+            sm.uca[u] = alp[a];
+        }
+    }
+    __syncthreads();
+    if (ded) {
+        for (int e = tid; e < NB * nuc; e += 256) {
+            const int r = e / nuc, u = e - r * nuc;
+            sm.c.Aiu[r][u] = (sm.sia[r] >= 0) ? eeq_a_v(sm.rp[r], sm.ra_[r], sm.ucp[u], sm.uca[u]) : 0.0;
+        }
+        for (int e = tid; e < nur * NB; e += 256) {
+            const int u = e / NB, c = e - u * NB;
+            sm.c.Aui[u][c] = (sm.sib[c] >= 0) ? eeq_a_v(sm.urp[u], sm.ura[u], sm.cp[c], sm.ca_[c]) : 0.0;
+        }
+        for (int e = tid; e < nur * nuc; e += 256) {
+            const int u = e / nuc, w = e - u * nuc;
+            sm.c.Auu[u][w] = (sm.sur[u] == sm.suc[w]) ? diag[sm.sur[u]]
+                                                      : eeq_a_v(sm.urp[u], sm.ura[u], sm.ucp[w], sm.uca[w]);
+        }
+        __syncthreads();
+    }
+    // thread tid owns column cc = tid % 64 of rows rr = tid / 64 + 4 u
+    constexpr int PER = NB * NB / 256;
+    const int cc = tid & 63, c = bj * NB + cc;
+    double vals[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int rr = (tid >> 6) + 4 * u, r = bi * NB + rr;
+        double v = 0.0;
+        if (c <= r) {
+            if (r >= nr || c >= nr) {
+                v = (r == c) ? 1.0 : 0.0;
+            } else if (ded) {
+                const int ia = sm.sia[rr], ib = sm.sib[cc];
+                const double a1 = (ia == ib) ? diag[ia] : eeq_a_v(sm.rp[rr], sm.ra_[rr], sm.cp[cc], sm.ca_[cc]);
+                v = a1 - sm.c.Aiu[rr][sm.slc[cc]] - sm.c.Aui[sm.slr[rr]][cc] + sm.c.Auu[sm.slr[rr]][sm.slc[cc]];
+            } else {
+                int ia = ra[r], la = rl[r], ib = ra[c], lb = rl[c];
+                v = eeq_a(ia, ib, x, alp, diag) - eeq_a(ia, lb, x, alp, diag) - eeq_a(la, ib, x, alp, diag) +
+                    eeq_a(la, lb, x, alp, diag);
+            }
+            A[(size_t)r * np + c] = v;
+        }
+        vals[u] = v;
     }
     if (!full) return;
+    __syncthreads();  // the caches are no longer read: Tl reuses their memory
+#pragma unroll
+    for (int u = 0; u < PER; ++u) sm.Tl[(tid >> 6) + 4 * u][cc] = vals[u];
     __syncthreads();
     // upper counterpart: A(bj*64 + i, bi*64 + j) = T(j, i), row-contiguous stores
-    _Pragma("unroll") for (int u_ = 0, e = threadIdx.x; u_ < NB * NB / 256; ++u_, e += 256) {
-        const int i = e / NB, j = e % NB;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int i = (tid >> 6) + 4 * u, j = cc;
         if (bi == bj && j <= i) continue;
-        A[(size_t)(bj * NB + i) * np + bi * NB + j] = Tl[j][i];
+        A[(size_t)(bj * NB + i) * np + bi * NB + j] = sm.Tl[j][i];
     }
 }
 
@@ -3115,6 +3206,7 @@ struct DeviceEngine {
     int *Z, *elem, *nref, *metal, *frag, *toff, *tidx, *tdist;
     double *chi_base, *gam, *alp, *cnf, *rcov_cn, *rcov_d3, *hb_rad, *qa, *zeta, *szr, *anb, *repz, *refcn, *c6ref;
     int *red_atom, *red_last, *red_of, *frag_off, *frag_atoms;
+    int *red_lidx, *red_ul, *red_nul;  // distinct last atoms per 64-block of reduced variables
     double* qp;
     int *bond_ij, *bond_H, *bond_hboff, *bond_hbB, *bond_slot;
     double* bond_par;
@@ -3341,6 +3433,26 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         }
         E->red_atom = E->upload(red_atom);
         E->red_last = E->upload(red_last);
+        {
+            // per 64-block of reduced variables: its distinct last atoms (in
+            // order of appearance) and each variable's index among them
+            const int nbk0 = E->np / NB;
+            std::vector<int> lidx(E->np, 0), ul((size_t)E->np, -1), nul(std::max(nbk0, 1), 0);
+            for (int bk = 0; bk < nbk0; ++bk) {
+                int cnt = 0;
+                for (int r = bk * NB; r < std::min((bk + 1) * NB, E->nr); ++r) {
+                    const int la = red_last[r];
+                    int u = 0;
+                    while (u < cnt && ul[(size_t)bk * NB + u] != la) ++u;
+                    if (u == cnt) ul[(size_t)bk * NB + cnt++] = la;
+                    lidx[r] = u;
+                }
+                nul[bk] = cnt;
+            }
+            E->red_lidx = E->upload(lidx);
+            E->red_ul = E->upload(ul);
+            E->red_nul = E->upload(nul);
+        }
         E->red_of = E->upload(red_of);
         E->frag_off = E->upload(foff);
         E->frag_atoms = E->upload(fatoms);
@@ -3860,7 +3972,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     const int np = E->np, nbk = np / NB;
     if (E->nmol == 1)
         LAUNCH(KC_EEQ_BUILD, k_eeq_reduced, dim3(nbk * (nbk + 1) / 2), dim3(256), s, E->nr, np, E->red_atom,
-               E->red_last, E->x, E->alp, E->diag, E->Ared, E->use_cg ? 1 : 0);
+               E->red_last, E->red_lidx, E->red_ul, E->red_nul, E->x, E->alp, E->diag, E->Ared, E->use_cg ? 1 : 0);
     else
         LAUNCH(KC_EEQ_BUILD, k_eeq_reduced_batch, dim3(E->nbtiles), dim3(256), s, E->btiles, E->mnr, E->mnp, E->mmo,
                E->mvo, E->red_atom, E->red_last, E->x, E->alp, E->diag, E->Ared);
