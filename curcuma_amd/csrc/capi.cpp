@@ -203,7 +203,8 @@ static int ensure_device(gfnff_handle* h) {
     if (h->dev) return GFNFF_STATUS_OK;
     std::string msg;
     int st = GFNFF_STATUS_OK;
-    h->dev.reset(gfnff::device_create(h->params, h->precision, h->opt.cholesky_lookahead, h->opt.eeq_solver, st, msg));
+    h->dev.reset(gfnff::device_create(h->params, h->precision, h->opt.cholesky_lookahead, h->opt.eeq_solver,
+                                      h->opt.pcg_tol, st, msg));
     if (!h->dev) return fail(h, st ? st : GFNFF_STATUS_DEVICE, "device set-up: " + msg);
     return GFNFF_STATUS_OK;
 }
@@ -288,6 +289,10 @@ int gfnff_set_param(gfnff_handle* h, const char* key, double value) {
     } else if (k == "eeq_solver") {
         if (value != 0.0 && value != 1.0) return fail(h, GFNFF_STATUS_ARGS, "eeq_solver: 0 (PCG) or 1 (Cholesky)");
         h->opt.eeq_solver = (int)value;
+        h->dev.reset();
+    } else if (k == "pcg_tol") {
+        if (!(value >= 0.0 && value <= 1e-6)) return fail(h, GFNFF_STATUS_ARGS, "pcg_tol: 0 (default) or in (0, 1e-6]");
+        h->opt.pcg_tol = value;
         h->dev.reset();
     } else if (k == "cholesky_lookahead") {
         h->opt.cholesky_lookahead = value != 0.0;

@@ -1047,7 +1047,8 @@ __device__ void bjacobi_apply(const double* __restrict__ Li, const double* rb, d
 // and read with device-coherent accesses (ld_c, st_c).
 constexpr int PCG_MAXB = 8;
 constexpr int PCG_MAXCHUNK = 16;
-constexpr double PCG_TOL2 = 1e-26;  // |r| <= 1e-13 |b|
+constexpr double PCG_TOL_FP64 = 1e-13;   // default |r| / |b| of the FP64 path
+constexpr double PCG_TOL_MIXED = 1e-10;  // default of the mixed path (EEQ energy is variational in q)
 constexpr int PCG_MAXIT = 1000;
 __global__ __launch_bounds__(PCG_THREADS) void k_eeq_pcg(int np, int nchunk, int cwb, const double* __restrict__ R,
                                                          const double* __restrict__ Lj, const double* __restrict__ b,
@@ -3184,6 +3185,7 @@ struct DeviceEngine {
     bool use_lookahead = false;
     bool use_cg = false;                // iterative EEQ solve (single molecules)
     int cg_groups = 0;                  // workgroups of the persistent PCG kernel
+    double cg_tol2 = 0.0;               // squared relative residual at which the PCG solve stops
     int cg_chunks = 1, cg_cwb = 0;      // column chunks of its matrix-vector tiles, width in 64-blocks
     double *cg_r = nullptr, *cg_z = nullptr, *cg_p = nullptr, *cg_q = nullptr;
     unsigned long long* cg_mb = nullptr;  // all-reduce mailboxes of the persistent PCG
@@ -3296,8 +3298,8 @@ struct DeviceEngine {
     }
 };
 
-DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, int solver, int& status,
-                            std::string& msg) {
+DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, int solver, double pcg_tol,
+                            int& status, std::string& msg) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         status = kErrDevice;
@@ -3320,6 +3322,10 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
     E->mixed = precision == 1;
     E->use_lookahead = lookahead;
     E->use_cg = solver == 0 && P.nmol == 1;
+    {
+        const double tol = pcg_tol > 0.0 ? pcg_tol : (E->mixed ? PCG_TOL_MIXED : PCG_TOL_FP64);
+        E->cg_tol2 = tol * tol;
+    }
     try {
         const int n = P.n;
         E->n = n;
@@ -3994,7 +4000,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(MB_WORDS, 256)), dim3(256), 0, s, MB_WORDS, E->cg_mb, MB_EMPTY);
         LAUNCH(KC_PCG, k_eeq_pcg, dim3(E->cg_groups), dim3(PCG_THREADS), s, np, E->cg_chunks, E->cg_cwb, E->Ared,
                E->Linv, E->b, E->xbuf, E->cg_r, E->cg_z, E->cg_p, E->cg_q, E->cg_mb, E->cg_iters,
-               PCG_TOL2, PCG_MAXIT, E->flags);
+               E->cg_tol2, PCG_MAXIT, E->flags);
     } else if (!E->mixed) {
         queue_cholesky(&E->timer, s, np, E->Ared, E->Linv, E->flags, E->use_lookahead ? &E->la : nullptr);
         queue_solve<double>(E, s, E->Ared, E->Linv, E->b, E->ybuf, E->xbuf);

@@ -14,8 +14,10 @@ struct DeviceEngine;
 // Uploads the parameter set; returns nullptr and sets status/msg on failure.
 // solver: 0 = preconditioned CG on the reduced EEQ system (single molecules;
 // batches always use the per-molecule Cholesky), 1 = blocked Cholesky.
-DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, int solver, int& status,
-                            std::string& msg);
+// pcg_tol: relative residual of the PCG solve (0: 1e-13 for FP64, 1e-10 for
+// the mixed path, whose accuracy bar is 1e-6 Eh/atom).
+DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, int solver, double pcg_tol,
+                            int& status, std::string& msg);
 void device_destroy(DeviceEngine* e);
 
 // Host-buffer evaluation: uploads xyz (Bohr), evaluates, downloads

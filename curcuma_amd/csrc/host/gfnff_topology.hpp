@@ -31,6 +31,7 @@ struct Options {
     bool cholesky_lookahead = false;  // overlap trailing updates with the next super-panel (side stream)
     double hb_update_rmsd_bohr = 0.3;  // HB/XB re-detection threshold on sqrt(sum dx^2)/N
     int eeq_solver = 0;               // 0: preconditioned CG (single molecules), 1: blocked Cholesky
+    double pcg_tol = 0.0;             // PCG relative residual |r| / |b|; 0: 1e-13 (FP64), 1e-10 (mixed)
 };
 
 // Error codes shared with the C-ABI.

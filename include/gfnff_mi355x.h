@@ -103,14 +103,17 @@ int gfnff_cn(gfnff_handle* h, double* cn);
  * (gfnff.h:219-293): "cn_cutoff_bohr", "hb_accuracy", "threads",
  * "hb_update_rmsd_bohr" (gfnff_method.cpp:2250).  Engine keys:
  * "eeq_solver" 0 = preconditioned CG on the reduced EEQ system (default for
- * single molecules, stops at |r| <= 1e-13 |b|), 1 = blocked Cholesky, the
- * direct solve of the reference (batches always factor per molecule);
- * "cholesky_lookahead" 0/1 overlaps the trailing update with the next
- * super-panel on a second stream (off by default). */
+ * single molecules), 1 = blocked Cholesky, the direct solve of the reference
+ * (batches always factor per molecule); "pcg_tol" the relative residual
+ * |r| / |b| at which the CG solve stops (0 = default: 1e-13 for FP64, 1e-10
+ * for the mixed path, at most 1e-6); "cholesky_lookahead" 0/1 overlaps the
+ * trailing update with the next super-panel on a second stream (off by
+ * default). */
 int gfnff_set_param(gfnff_handle* h, const char* key, double value);
 
 /* GFNFF_PREC_FP64 (default, the reference's arithmetic) or GFNFF_PREC_MIXED:
  * pair terms in FP32 with FP64 coordinate differences and FP64 row sums; the
+ * EEQ solve in FP64 to |r| <= 1e-10 |b| (CG), or with "eeq_solver" 1 the
  * reduced EEQ matrix factored in FP32 (FP32 MFMA) with FP64 iterative
  * refinement against the FP64 matrix; everything else FP64. */
 int gfnff_set_precision(gfnff_handle* h, int32_t mode);
