@@ -239,9 +239,10 @@ __global__ __launch_bounds__(256) void k_eeq_reduced(int nr, int np, const int* 
                                                      const double* __restrict__ diag, double* __restrict__ A,
                                                      int full) {
     // one 64x64 lower tile per workgroup of 256 threads; grid enumerates
-    // tiles (bi >= bj).  full != 0 also writes the mirrored upper tile (both
-    // triangles stored, for the row-major matrix-vector products of the
-    // iterative solver)
+    // tiles (bi >= bj).  full == 0: lower triangle of the row-major np x np
+    // matrix (Cholesky).  full == 1: also the mirrored upper tile.  full == 2:
+    // packed symmetric storage for the iterative solver, tile t at A + t 64^2
+    // (row-major), diagonal tiles completed from their lower triangle
     __shared__ RedSm sm;
     int t = blockIdx.x;
     int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
@@ -314,11 +315,14 @@ __global__ __launch_bounds__(256) void k_eeq_reduced(int nr, int np, const int* 
                 v = eeq_a(ia, ib, x, alp, diag) - eeq_a(ia, lb, x, alp, diag) - eeq_a(la, ib, x, alp, diag) +
                     eeq_a(la, lb, x, alp, diag);
             }
-            A[(size_t)r * np + c] = v;
+            if (full == 2)
+                A[(size_t)t * NB * NB + rr * NB + cc] = v;
+            else
+                A[(size_t)r * np + c] = v;
         }
         vals[u] = v;
     }
-    if (!full) return;
+    if (!full || (full == 2 && bi != bj)) return;
     __syncthreads();  // the caches are no longer read: Tl reuses their memory
 #pragma unroll
     for (int u = 0; u < PER; ++u) sm.Tl[(tid >> 6) + 4 * u][cc] = vals[u];
@@ -328,7 +332,10 @@ __global__ __launch_bounds__(256) void k_eeq_reduced(int nr, int np, const int* 
     for (int u = 0; u < PER; ++u) {
         const int i = (tid >> 6) + 4 * u, j = cc;
         if (bi == bj && j <= i) continue;
-        A[(size_t)(bj * NB + i) * np + bi * NB + j] = sm.Tl[j][i];
+        if (full == 2)
+            A[(size_t)t * NB * NB + i * NB + j] = sm.Tl[j][i];
+        else
+            A[(size_t)(bj * NB + i) * np + bi * NB + j] = sm.Tl[j][i];
     }
 }
 
@@ -855,23 +862,19 @@ __global__ __launch_bounds__(256) void k_trsv_bwd_sweep(int np, const T* __restr
 // ---------------------------------------------------------------------------
 // Block-Jacobi preconditioner: inverse factors of the 64 x 64 diagonal blocks
 // of R (one workgroup per block; R is left intact).
-__global__ __launch_bounds__(256) void k_bjacobi_setup(int np, double* __restrict__ R, double* __restrict__ Lj,
+// block-Jacobi factors: L^-1 of every diagonal block of the packed matrix
+// (block I at Rp + (I (I + 1) / 2 + I) 64^2)
+__global__ __launch_bounds__(256) void k_bjacobi_setup(double* __restrict__ Rp, double* __restrict__ Lj,
                                                        int* __restrict__ flags) {
     __shared__ double sL[NB][NB + 1];
     __shared__ double col[2][NB];
     __shared__ double rdiag[NB];
-    chol_diag_block<double>(np, blockIdx.x * NB, R, Lj, flags, sL, col, rdiag, false);
+    const size_t I = blockIdx.x;
+    chol_diag_block<double>(NB, 0, Rp + (I * (I + 1) / 2 + I) * NB * NB, Lj + I * NB * NB, flags, sL, col, rdiag,
+                            false);
 }
 
-// The persistent PCG runs PCG_THREADS threads per workgroup: its waves split
-// a 64-row block into PCG_RG row groups of PCG_RW rows and the columns into
-// PCG_CS interleaved parts, so each CU keeps many waves of streaming loads of
-// R in flight.  Reductions inside a workgroup use the first 256 threads.
 constexpr int PCG_THREADS = 256;
-constexpr int PCG_RW = 16;                               // rows per wave
-constexpr int PCG_RG = NB / PCG_RW;                      // row groups per block
-constexpr int PCG_CS = PCG_THREADS / 64 / PCG_RG;        // column parts per block
-static_assert(PCG_RG * PCG_CS * 64 == PCG_THREADS && PCG_THREADS >= 256, "PCG layout");
 
 // Fixed-order sum of the first 256 threads' values (every thread returns the
 // total; threads >= 256 contribute nothing).
@@ -1029,42 +1032,77 @@ __device__ void bjacobi_apply(const double* __restrict__ Li, const double* rb, d
     __syncthreads();
 }
 
-// Persistent PCG on R x = b.  Workgroup g owns the 64-row blocks g, g + G, ...
-// (at most PCG_MAXB) for the vector updates; the matrix-vector product is cut
-// into nbk x nchunk tiles (a 64-row block times a chunk of cwb 64-column
-// blocks), tile t processed by workgroup t mod G, so that every CU streams
-// its share of R.  Per iteration: (1) p = z + beta p_prev (formed on the fly
-// by every reader; owners store their rows); per tile the partial q rows go to
-// qpart[chunk] and the tile's p.q_partial to the workgroup's p.q sum (p.q is
-// linear in the partials); barrier; (2) alpha = rz / pq; owners add the
-// partial q rows in chunk order, x += alpha p, r -= alpha q, z = M^-1 r,
+// Persistent PCG on R x = b with R in packed symmetric storage: the 64x64
+// blocks (I, J), I >= J, at Rp + (I (I + 1) / 2 + J) 64^2, row-major, the
+// diagonal blocks full.  Workgroup g owns the 64-row blocks g, g + G, ... (at
+// most PCG_MAXB) for the vector updates.  The matrix-vector product runs over
+// the stored blocks, each block read once by one wavefront (contiguous,
+// balanced ranges of the packed order): lane l holds column l of the block,
+// the row sums R_IJ p_J come from a 16-row transpose-reduce of the lanes, the
+// column sums R_IJ^T p_I stay per lane.  Every (row block X, column block Y)
+// product goes to its own partial part[X][Y][64], which the owner of X adds
+// in Y order after the barrier; p.q is formed from the partials (it is
+// linear in them).  Per iteration: (1) p = z + beta p_prev (formed on the fly
+// by every reader; owners store their rows), block products; barrier;
+// (2) alpha = rz / pq; owners: q, x += alpha p, r -= alpha q, z = M^-1 r,
 // partials r.z and r.r; barrier; beta = rz_new / rz.  Stops when
 // r.r <= tol2 * b.b (uniform decision from identical sums) or after maxit
-// iterations (FLAG_SOLVE).  In a tile, wave w handles rows
-// PCG_RW (w % PCG_RG) .. + PCG_RW - 1 and the columns
-// c_lo + 64 (w / PCG_RG) + lane + 64 PCG_CS k; the column parts are added in
-// LDS in a fixed order.  Cross-workgroup vectors (z, p, qpart) are written
+// iterations (FLAG_SOLVE).  Cross-workgroup vectors (z, p, part) are written
 // and read with device-coherent accesses (ld_c, st_c).
 constexpr int PCG_MAXB = 8;
-constexpr int PCG_MAXCHUNK = 16;
 constexpr double PCG_TOL_FP64 = 1e-13;   // default |r| / |b| of the FP64 path
 constexpr double PCG_TOL_MIXED = 1e-10;  // default of the mixed path (EEQ energy is variational in q)
 constexpr int PCG_MAXIT = 1000;
-__global__ __launch_bounds__(PCG_THREADS) void k_eeq_pcg(int np, int nchunk, int cwb, const double* __restrict__ R,
-                                                         const double* __restrict__ Lj, const double* __restrict__ b,
-                                                         double* __restrict__ x, double* __restrict__ r,
-                                                         double* __restrict__ z, double* __restrict__ pb,
-                                                         double* __restrict__ qpart, unsigned long long* __restrict__ mb,
-                                                         int* __restrict__ iters, double tol2, int maxit,
-                                                         int* __restrict__ flags) {
+
+// Sum over the 64 lanes of 16 per-lane values v[k]: recursive halving, so
+// that lane l ends with the total of row k(l) = 8 b5 + 4 b4 + 2 b3 + b2 (b the
+// bits of l); the four lanes of a row hold the same bits.
+__device__ inline double rowsum16(const double (&v)[16], int l) {
+    double a8[8], a4[4], a2[2];
+    const bool h5 = (l & 32) != 0, h4 = (l & 16) != 0, h3 = (l & 8) != 0, h2 = (l & 4) != 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const double keep = h5 ? v[i + 8] : v[i], send = h5 ? v[i] : v[i + 8];
+        a8[i] = keep + __shfl_xor(send, 32);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double keep = h4 ? a8[i + 4] : a8[i], send = h4 ? a8[i] : a8[i + 4];
+        a4[i] = keep + __shfl_xor(send, 16);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const double keep = h3 ? a4[i + 2] : a4[i], send = h3 ? a4[i] : a4[i + 2];
+        a2[i] = keep + __shfl_xor(send, 8);
+    }
+    double a1 = (h2 ? a2[1] : a2[0]) + __shfl_xor(h2 ? a2[0] : a2[1], 4);
+    a1 += __shfl_xor(a1, 2);
+    a1 += __shfl_xor(a1, 1);
+    return a1;
+}
+
+__global__ __launch_bounds__(256) void k_eeq_pcg(int np, const double* __restrict__ Rp,
+                                                 const double* __restrict__ Lj, const double* __restrict__ b,
+                                                 double* __restrict__ x, double* __restrict__ r,
+                                                 double* __restrict__ z, double* __restrict__ pb,
+                                                 double* __restrict__ part, unsigned long long* __restrict__ mb,
+                                                 int* __restrict__ iters, double tol2, int maxit,
+                                                 int* __restrict__ flags) {
     __shared__ double red[256];
     __shared__ double pown[PCG_MAXB][NB];
-    __shared__ double qcs[PCG_CS][NB];
-    __shared__ double ptile[NB], rb[NB], wb[NB], zb[NB];
-    const int G = gridDim.x, g = blockIdx.x, nbk = np / NB, ntile = nbk * nchunk;
+    __shared__ double qs[4][NB];
+    __shared__ double rb[NB], wb[NB], zb[NB];
+    const int G = gridDim.x, g = blockIdx.x, nbk = np / NB;
     const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
-    const int rg = wv % PCG_RG, cs = wv / PCG_RG;
     const int nown = (nbk - g + G - 1) / G;
+    // this wavefront's blocks of the packed order: [t0, t1)
+    const long T = (long)nbk * (nbk + 1) / 2, W = 4L * G, gw = 4L * g + wv;
+    const int t0 = (int)(T * gw / W), t1 = (int)(T * (gw + 1) / W);
+    int I0 = (int)((sqrt(8.0 * t0 + 1.0) - 1.0) * 0.5);
+    while ((long)(I0 + 1) * (I0 + 2) / 2 <= t0) ++I0;
+    while ((long)I0 * (I0 + 1) / 2 > t0) --I0;
+    const int J0 = t0 - I0 * (I0 + 1) / 2;
+    const int kl = 8 * ((l >> 5) & 1) + 4 * ((l >> 4) & 1) + 2 * ((l >> 3) & 1) + ((l >> 2) & 1);
     // init: x = 0, r = b, z = M^-1 b; partials r.z and b.b
     double lrz = 0.0, lbb = 0.0;
     for (int o = 0; o < nown; ++o) {
@@ -1095,7 +1133,7 @@ __global__ __launch_bounds__(PCG_THREADS) void k_eeq_pcg(int np, int nchunk, int
     for (; !done && it < maxit; ++it) {
         const double* pprev = pb + (size_t)((it + 1) & 1) * np;
         double* pcur = pb + (size_t)(it & 1) * np;
-        // (1) owners: p rows; every tile: partial q rows and partial p.q
+        // (1) owners: p rows; every wavefront: its blocks' products
         for (int o = 0; o < nown; ++o) {
             if (tid < NB) {
                 const int row = (g + o * G) * NB + tid;
@@ -1106,62 +1144,69 @@ __global__ __launch_bounds__(PCG_THREADS) void k_eeq_pcg(int np, int nchunk, int
             }
         }
         double lpq = 0.0;
-        for (int t = g; t < ntile; t += G) {
-            const int kb = t / nchunk, ch = t - kb * nchunk;
-            const int c_lo = ch * cwb * NB, c_hi = min(np, c_lo + cwb * NB);
-            double acc[PCG_RW];
+        int I = I0, J = J0;
+        for (int t = t0; t < t1; ++t) {
+            const double* Rb = Rp + (size_t)t * NB * NB;
+            double rv[NB];
 #pragma unroll
-            for (int i = 0; i < PCG_RW; ++i) acc[i] = 0.0;
-            const double* Rrow = R + (size_t)(kb * NB + PCG_RW * rg) * np;
-            for (int c = c_lo + NB * cs + l; c < c_hi; c += NB * PCG_CS) {
-                const double pc = (it == 0) ? 0.0 : ld_c(pprev + c);
-                const double pv = ld_c(z + c) + beta * pc;
+            for (int k = 0; k < NB; ++k) rv[k] = Rb[k * NB + l];
+            const double pcj = (it == 0) ? 0.0 : ld_c(pprev + J * NB + l);
+            const double pj = ld_c(z + J * NB + l) + beta * pcj;
+            const double pci = (it == 0) ? 0.0 : ld_c(pprev + I * NB + l);
+            const double pil = ld_c(z + I * NB + l) + beta * pci;  // p of row I * 64 + l
+            double cs = 0.0;
 #pragma unroll
-                for (int i = 0; i < PCG_RW; ++i) acc[i] += Rrow[(size_t)i * np + c] * pv;
+            for (int h = 0; h < 4; ++h) {
+                double v[16];
+                double pk = 0.0;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const double pi = __shfl(pil, 16 * h + k);
+                    v[k] = rv[16 * h + k] * pj;
+                    cs += rv[16 * h + k] * pi;
+                    pk = (k == kl) ? pi : pk;
+                }
+                const double rs = rowsum16(v, l);
+                if ((l & 3) == 0) {
+                    st_c(part + ((size_t)I * nbk + J) * NB + 16 * h + kl, rs);
+                    lpq += pk * rs;
+                }
             }
-#pragma unroll
-            for (int i = 0; i < PCG_RW; ++i)
-#pragma unroll
-                for (int o2 = 32; o2 > 0; o2 >>= 1) acc[i] += __shfl_xor(acc[i], o2);
-            if (l < PCG_RW) {
-                double qv = 0.0;
-#pragma unroll
-                for (int i = 0; i < PCG_RW; ++i) qv = (l == i) ? acc[i] : qv;
-                qcs[cs][PCG_RW * rg + l] = qv;
+            if (I != J) {
+                st_c(part + ((size_t)J * nbk + I) * NB + l, cs);
+                lpq += pj * cs;
             }
-            if (tid < NB) {
-                const int row = kb * NB + tid;
-                const double pc = (it == 0) ? 0.0 : ld_c(pprev + row);
-                ptile[tid] = ld_c(z + row) + beta * pc;
+            if (++J > I) {
+                ++I;
+                J = 0;
             }
-            __syncthreads();
-            if (tid < NB) {
-                double qv = qcs[0][tid];
-#pragma unroll
-                for (int k = 1; k < PCG_CS; ++k) qv += qcs[k][tid];
-                st_c(qpart + (size_t)ch * np + kb * NB + tid, qv);
-                lpq += ptile[tid] * qv;
-            }
-            __syncthreads();
         }
         const double s_pq = wg_sum256(lpq, red);
         if (!grid_allreduce2(s_pq, 0.0, sums, mb, nbar++, flags, red)) return;
         const double pq = sums[0];
         const double alpha = rz / pq;
-        // (2) owners: q rows from the chunk partials, x, r, z and the
+        // (2) owners: q rows from the partials (Y order), x, r, z and the
         // partials r.z, r.r
         double lrz2 = 0.0, lrr = 0.0;
         for (int o = 0; o < nown; ++o) {
             const int kb = g + o * G;
+            {
+                // thread (row l, part wv) adds Y = wv, wv + 4, ... in batches of 8 loads
+                const double* pr = part + (size_t)kb * nbk * NB + l;
+                double acc = 0.0;
+                for (int y0 = wv; y0 < nbk; y0 += 32) {
+                    double t8[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) t8[u] = ld_c(pr + (size_t)min(y0 + 4 * u, nbk - 1) * NB);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) acc += (y0 + 4 * u < nbk) ? t8[u] : 0.0;
+                }
+                qs[wv][l] = acc;
+            }
+            __syncthreads();
             if (tid < NB) {
                 const int row = kb * NB + tid;
-                // all chunk partials loaded at once (clamped index), added in chunk order
-                double qk[PCG_MAXCHUNK];
-#pragma unroll
-                for (int k = 0; k < PCG_MAXCHUNK; ++k) qk[k] = ld_c(qpart + (size_t)min(k, nchunk - 1) * np + row);
-                double qv = qk[0];
-#pragma unroll
-                for (int k = 1; k < PCG_MAXCHUNK; ++k) qv += (k < nchunk) ? qk[k] : 0.0;
+                const double qv = (qs[0][tid] + qs[1][tid]) + (qs[2][tid] + qs[3][tid]);
                 x[row] += alpha * pown[o][tid];
                 const double rv = r[row] - alpha * qv;
                 r[row] = rv;
@@ -3186,8 +3231,8 @@ struct DeviceEngine {
     bool use_cg = false;                // iterative EEQ solve (single molecules)
     int cg_groups = 0;                  // workgroups of the persistent PCG kernel
     double cg_tol2 = 0.0;               // squared relative residual at which the PCG solve stops
-    int cg_chunks = 1, cg_cwb = 0;      // column chunks of its matrix-vector tiles, width in 64-blocks
-    double *cg_r = nullptr, *cg_z = nullptr, *cg_p = nullptr, *cg_q = nullptr;
+    double *cg_r = nullptr, *cg_z = nullptr, *cg_p = nullptr;
+    double* cg_part = nullptr;          // block-product partials [nbk][nbk][64]
     unsigned long long* cg_mb = nullptr;  // all-reduce mailboxes of the persistent PCG
     int* cg_iters = nullptr;
     int* ready = nullptr;               // per-block ready flags of the solve sweeps
@@ -3764,12 +3809,6 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->dgw = E->alloc<double>(7 * (size_t)n);
         E->rhs = E->alloc<double>(n);
         E->diag = E->alloc<double>(n);
-        E->Ared = E->alloc<double>(E->mat_size);
-        E->Linv = E->alloc<double>((size_t)E->np * NB);
-        E->b = E->alloc<double>(E->np);
-        E->ybuf = E->alloc<double>(E->np);
-        E->xbuf = E->alloc<double>(E->np);
-        E->ready = E->alloc<int>(E->np / NB);
         if (E->use_cg) {
             // every workgroup of the persistent kernel must be resident at once
             const int nbk = E->np / NB;
@@ -3782,28 +3821,22 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
                 delete E;
                 return nullptr;
             }
-            const int G = std::min(ncu * per_cu, 256);
-            // column chunks: least ceil(tiles / G) * (chunk width + 1), the
-            // critical path of tiles plus their epilogues
-            int best = 1, best_cost = INT_MAX;
-            for (int S = 1; S <= std::min(PCG_MAXCHUNK, nbk); ++S) {
-                const int w = cdiv(nbk, S), S2 = cdiv(nbk, w);
-                const int cost = cdiv(nbk * S2, std::max(G, 1)) * (w + 1);
-                if (cost < best_cost) {
-                    best_cost = cost;
-                    best = S2;
-                }
-            }
-            E->cg_chunks = best;
-            E->cg_cwb = cdiv(nbk, best);
-            E->cg_groups = std::min(G, nbk * best);
+            const long T = (long)nbk * (nbk + 1) / 2;
+            E->cg_groups = (int)std::min<long>({(long)ncu * per_cu, 256L, T});
             if (E->cg_groups < 1 || cdiv(nbk, E->cg_groups) > PCG_MAXB) E->use_cg = false;
+            if (E->use_cg) E->mat_size = (size_t)T * NB * NB;  // packed symmetric storage
         }
+        E->Ared = E->alloc<double>(E->mat_size);
+        E->Linv = E->alloc<double>((size_t)E->np * NB);
+        E->b = E->alloc<double>(E->np);
+        E->ybuf = E->alloc<double>(E->np);
+        E->xbuf = E->alloc<double>(E->np);
+        E->ready = E->alloc<int>(E->np / NB);
         if (E->use_cg) {
             E->cg_r = E->alloc<double>(E->np);
             E->cg_z = E->alloc<double>(E->np);
             E->cg_p = E->alloc<double>(2 * (size_t)E->np);
-            E->cg_q = E->alloc<double>((size_t)E->cg_chunks * E->np);
+            E->cg_part = E->alloc<double>((size_t)(E->np / NB) * E->np);
             E->cg_mb = E->alloc<unsigned long long>(MB_WORDS);
             E->cg_iters = E->alloc<int>(1);
         } else if (E->mixed && E->nmol == 1) {
@@ -3978,7 +4011,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     const int np = E->np, nbk = np / NB;
     if (E->nmol == 1)
         LAUNCH(KC_EEQ_BUILD, k_eeq_reduced, dim3(nbk * (nbk + 1) / 2), dim3(256), s, E->nr, np, E->red_atom,
-               E->red_last, E->red_lidx, E->red_ul, E->red_nul, E->x, E->alp, E->diag, E->Ared, E->use_cg ? 1 : 0);
+               E->red_last, E->red_lidx, E->red_ul, E->red_nul, E->x, E->alp, E->diag, E->Ared, E->use_cg ? 2 : 0);
     else
         LAUNCH(KC_EEQ_BUILD, k_eeq_reduced_batch, dim3(E->nbtiles), dim3(256), s, E->btiles, E->mnr, E->mnp, E->mmo,
                E->mvo, E->red_atom, E->red_last, E->x, E->alp, E->diag, E->Ared);
@@ -3996,11 +4029,10 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
                E->Linv, E->b, E->xbuf, E->flags);
     } else if (E->use_cg) {
         // block-Jacobi factors of R's diagonal blocks, then the persistent PCG
-        LAUNCH(KC_CHOL_DIAG, k_bjacobi_setup, dim3(nbk), dim3(256), s, np, E->Ared, E->Linv, E->flags);
+        LAUNCH(KC_CHOL_DIAG, k_bjacobi_setup, dim3(nbk), dim3(256), s, E->Ared, E->Linv, E->flags);
         hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(MB_WORDS, 256)), dim3(256), 0, s, MB_WORDS, E->cg_mb, MB_EMPTY);
-        LAUNCH(KC_PCG, k_eeq_pcg, dim3(E->cg_groups), dim3(PCG_THREADS), s, np, E->cg_chunks, E->cg_cwb, E->Ared,
-               E->Linv, E->b, E->xbuf, E->cg_r, E->cg_z, E->cg_p, E->cg_q, E->cg_mb, E->cg_iters,
-               E->cg_tol2, PCG_MAXIT, E->flags);
+        LAUNCH(KC_PCG, k_eeq_pcg, dim3(E->cg_groups), dim3(PCG_THREADS), s, np, E->Ared, E->Linv, E->b, E->xbuf,
+               E->cg_r, E->cg_z, E->cg_p, E->cg_part, E->cg_mb, E->cg_iters, E->cg_tol2, PCG_MAXIT, E->flags);
     } else if (!E->mixed) {
         queue_cholesky(&E->timer, s, np, E->Ared, E->Linv, E->flags, E->use_lookahead ? &E->la : nullptr);
         queue_solve<double>(E, s, E->Ared, E->Linv, E->b, E->ybuf, E->xbuf);

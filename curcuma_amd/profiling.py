@@ -129,9 +129,9 @@ def work_per_eval(cls: str, info: dict, rc: dict) -> tuple[float, str]:
     if cls == "trsv":
         npd = info["np_pad"]
         return 2.0 * 8.0 * npd * npd / 2, "byte"  # the factor is read once per sweep
-    if cls == "pcg":  # the full FP64 matrix is streamed once per CG iteration
-        npd = info["np_pad"]
-        return info.get("pcg_iters", 0) * 8.0 * npd * npd, "byte"
+    if cls == "pcg":  # the packed symmetric FP64 matrix (64x64 lower blocks) is streamed once per CG iteration
+        nbk = info["np_pad"] // NB
+        return info.get("pcg_iters", 0) * 8.0 * (nbk * (nbk + 1) // 2) * NB * NB, "byte"
     if cls == "eeq_build":
         npd = info["np_pad"]
         return 8.0 * npd * (npd + NB) / 2, "byte"

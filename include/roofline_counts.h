@@ -52,7 +52,9 @@
 
 /* EEQ linear algebra on an n x n system with nfrag constraints:
  *   Cholesky n^3/3; triangular solves 2 n^2 (1 + nfrag); Schur (2/3) nfrag^3;
- *   one FP64 refinement step 2 n^2 (residual) + 2 n^2 (solve), 8 n^2 bytes.
+ *   one FP64 refinement step 2 n^2 (residual) + 2 n^2 (solve), 8 n^2 bytes;
+ *   one conjugate-gradient iteration reads the packed symmetric matrix once:
+ *   8 (n/64)(n/64 + 1)/2 64^2 bytes (n padded to 64).
  * A rank-NB trailing update of the lower triangle (diagonal included) of an
  * m x m block counts m (m + 1) NB flops (one FMA per element and step). */
 #define GFNFF_RF_CHOL_DIV 3
