@@ -876,20 +876,21 @@ __global__ __launch_bounds__(256) void k_bjacobi_setup(double* __restrict__ Rp, 
 
 constexpr int PCG_THREADS = 256;
 
-// Fixed-order sum of the first 256 threads' values (every thread returns the
-// total; threads >= 256 contribute nothing).
+// Sum over the 64 lanes of a wavefront (fixed butterfly; every lane gets it).
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Fixed-order sum over a workgroup of 256 threads: wavefront butterflies,
+// then the four wavefront totals in order (every thread returns the total).
 __device__ double wg_sum256(double v, double* red) {
-    const int tid = threadIdx.x;
+    v = wave_sum(v);
     __syncthreads();
-    if (tid < 256) red[tid] = v;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (tid < w) red[tid] += red[tid + w];
-        __syncthreads();
-    }
-    const double s = red[0];
-    __syncthreads();
-    return s;
+    return (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // Device-coherent access for data shared between the workgroups of a
@@ -990,7 +991,15 @@ __device__ bool grid_allreduce2(double v0, double v1, double* sums, unsigned lon
         u0 = __longlong_as_double(a0);
         u1 = __longlong_as_double(a1);
     }
-    const double s0 = wg_sum256(u0, red), s1 = wg_sum256(u1, red);
+    u0 = wave_sum(u0);
+    u1 = wave_sum(u1);
+    __syncthreads();
+    if ((tid & 63) == 0) {
+        red[tid >> 6] = u0;
+        red[4 + (tid >> 6)] = u1;
+    }
+    __syncthreads();
+    const double s0 = (red[0] + red[1]) + (red[2] + red[3]), s1 = (red[4] + red[5]) + (red[6] + red[7]);
     if (tid == 0 && bad_s == 0) {
         unsigned long long* const old = bc + (par ^ 1) * MB_LINE;
         st_cu(old, MB_EMPTY);
@@ -1048,7 +1057,9 @@ __device__ void bjacobi_apply(const double* __restrict__ Li, const double* rb, d
 // partials r.z and r.r; barrier; beta = rz_new / rz.  Stops when
 // r.r <= tol2 * b.b (uniform decision from identical sums) or after maxit
 // iterations (FLAG_SOLVE).  Cross-workgroup vectors (z, p, part) are written
-// and read with device-coherent accesses (ld_c, st_c).
+// and read with device-coherent accesses (ld_c, st_c); an owner keeps its
+// rows of x and r, and the Linv of a single owned block, in LDS for the
+// whole solve and writes x at the end.
 constexpr int PCG_MAXB = 8;
 constexpr double PCG_TOL_FP64 = 1e-13;   // default |r| / |b| of the FP64 path
 constexpr double PCG_TOL_MIXED = 1e-10;  // default of the mixed path (EEQ energy is variational in q)
@@ -1083,8 +1094,8 @@ __device__ inline double rowsum16(const double (&v)[16], int l) {
 
 __global__ __launch_bounds__(256) void k_eeq_pcg(int np, const double* __restrict__ Rp,
                                                  const double* __restrict__ Lj, const double* __restrict__ b,
-                                                 double* __restrict__ x, double* __restrict__ r,
-                                                 double* __restrict__ z, double* __restrict__ pb,
+                                                 double* __restrict__ x, double* __restrict__ z,
+                                                 double* __restrict__ pb,
                                                  double* __restrict__ part, unsigned long long* __restrict__ mb,
                                                  int* __restrict__ iters, double tol2, int maxit,
                                                  int* __restrict__ flags) {
@@ -1092,6 +1103,8 @@ __global__ __launch_bounds__(256) void k_eeq_pcg(int np, const double* __restric
     __shared__ double pown[PCG_MAXB][NB];
     __shared__ double qs[4][NB];
     __shared__ double rb[NB], wb[NB], zb[NB];
+    __shared__ double xo[PCG_MAXB][NB], ro[PCG_MAXB][NB];  // owned rows of x and r, resident for the solve
+    __shared__ double Lc[NB * NB];                          // Linv of the owned block when there is one
     const int G = gridDim.x, g = blockIdx.x, nbk = np / NB;
     const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
     const int nown = (nbk - g + G - 1) / G;
@@ -1103,6 +1116,9 @@ __global__ __launch_bounds__(256) void k_eeq_pcg(int np, const double* __restric
     while ((long)I0 * (I0 + 1) / 2 > t0) --I0;
     const int J0 = t0 - I0 * (I0 + 1) / 2;
     const int kl = 8 * ((l >> 5) & 1) + 4 * ((l >> 4) & 1) + 2 * ((l >> 3) & 1) + ((l >> 2) & 1);
+    const bool lcache = nown == 1;
+    if (lcache)
+        for (int e = tid; e < NB * NB; e += 256) Lc[e] = Lj[(size_t)g * NB * NB + e];
     // init: x = 0, r = b, z = M^-1 b; partials r.z and b.b
     double lrz = 0.0, lbb = 0.0;
     for (int o = 0; o < nown; ++o) {
@@ -1110,11 +1126,11 @@ __global__ __launch_bounds__(256) void k_eeq_pcg(int np, const double* __restric
         if (tid < NB) {
             const double bv = b[kb * NB + tid];
             rb[tid] = bv;
-            x[kb * NB + tid] = 0.0;
-            r[kb * NB + tid] = bv;
+            xo[o][tid] = 0.0;
+            ro[o][tid] = bv;
         }
         __syncthreads();
-        bjacobi_apply(Lj + (size_t)kb * NB * NB, rb, wb, zb);
+        bjacobi_apply(lcache ? Lc : Lj + (size_t)kb * NB * NB, rb, wb, zb);
         if (tid < NB) {
             st_c(z + kb * NB + tid, zb[tid]);
             lrz += rb[tid] * zb[tid];
@@ -1191,29 +1207,28 @@ __global__ __launch_bounds__(256) void k_eeq_pcg(int np, const double* __restric
         for (int o = 0; o < nown; ++o) {
             const int kb = g + o * G;
             {
-                // thread (row l, part wv) adds Y = wv, wv + 4, ... in batches of 8 loads
+                // thread (row l, part wv) adds Y = wv, wv + 4, ... in batches of 32 loads
                 const double* pr = part + (size_t)kb * nbk * NB + l;
                 double acc = 0.0;
-                for (int y0 = wv; y0 < nbk; y0 += 32) {
-                    double t8[8];
+                for (int y0 = wv; y0 < nbk; y0 += 128) {
+                    double t32[32];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) t8[u] = ld_c(pr + (size_t)min(y0 + 4 * u, nbk - 1) * NB);
+                    for (int u = 0; u < 32; ++u) t32[u] = ld_c(pr + (size_t)min(y0 + 4 * u, nbk - 1) * NB);
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) acc += (y0 + 4 * u < nbk) ? t8[u] : 0.0;
+                    for (int u = 0; u < 32; ++u) acc += (y0 + 4 * u < nbk) ? t32[u] : 0.0;
                 }
                 qs[wv][l] = acc;
             }
             __syncthreads();
             if (tid < NB) {
-                const int row = kb * NB + tid;
                 const double qv = (qs[0][tid] + qs[1][tid]) + (qs[2][tid] + qs[3][tid]);
-                x[row] += alpha * pown[o][tid];
-                const double rv = r[row] - alpha * qv;
-                r[row] = rv;
+                xo[o][tid] += alpha * pown[o][tid];
+                const double rv = ro[o][tid] - alpha * qv;
+                ro[o][tid] = rv;
                 rb[tid] = rv;
             }
             __syncthreads();
-            bjacobi_apply(Lj + (size_t)kb * NB * NB, rb, wb, zb);
+            bjacobi_apply(lcache ? Lc : Lj + (size_t)kb * NB * NB, rb, wb, zb);
             if (tid < NB) {
                 st_c(z + kb * NB + tid, zb[tid]);
                 lrz2 += rb[tid] * zb[tid];
@@ -1227,6 +1242,8 @@ __global__ __launch_bounds__(256) void k_eeq_pcg(int np, const double* __restric
         rz = rz2;
         done = rr <= tol2 * bb;
     }
+    for (int o = 0; o < nown; ++o)
+        if (tid < NB) x[(g + o * G) * NB + tid] = xo[o][tid];
     if (g == 0 && tid == 0) iters[0] = it;
     if (!done && g == 0 && tid == 0) atomicOr(flags, FLAG_SOLVE);
 }
@@ -1791,11 +1808,6 @@ __global__ __launch_bounds__(256) void k_nbrep_fix(int n, const double* __restri
 // one wavefront per bond: lanes stride the bond's hydrogen-bond partner list
 // (HB coordination and its gradient), fixed butterflies; lane 0 writes the
 // bond's own slots
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
 
 __global__ __launch_bounds__(256) void k_bonds(int nb, const int* __restrict__ ij, const double* __restrict__ par,
                                                const int* __restrict__ hb_off, const int* __restrict__ hb_B,
@@ -3231,7 +3243,7 @@ struct DeviceEngine {
     bool use_cg = false;                // iterative EEQ solve (single molecules)
     int cg_groups = 0;                  // workgroups of the persistent PCG kernel
     double cg_tol2 = 0.0;               // squared relative residual at which the PCG solve stops
-    double *cg_r = nullptr, *cg_z = nullptr, *cg_p = nullptr;
+    double *cg_z = nullptr, *cg_p = nullptr;
     double* cg_part = nullptr;          // block-product partials [nbk][nbk][64]
     unsigned long long* cg_mb = nullptr;  // all-reduce mailboxes of the persistent PCG
     int* cg_iters = nullptr;
@@ -3833,7 +3845,6 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->xbuf = E->alloc<double>(E->np);
         E->ready = E->alloc<int>(E->np / NB);
         if (E->use_cg) {
-            E->cg_r = E->alloc<double>(E->np);
             E->cg_z = E->alloc<double>(E->np);
             E->cg_p = E->alloc<double>(2 * (size_t)E->np);
             E->cg_part = E->alloc<double>((size_t)(E->np / NB) * E->np);
@@ -4032,7 +4043,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         LAUNCH(KC_CHOL_DIAG, k_bjacobi_setup, dim3(nbk), dim3(256), s, E->Ared, E->Linv, E->flags);
         hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(MB_WORDS, 256)), dim3(256), 0, s, MB_WORDS, E->cg_mb, MB_EMPTY);
         LAUNCH(KC_PCG, k_eeq_pcg, dim3(E->cg_groups), dim3(PCG_THREADS), s, np, E->Ared, E->Linv, E->b, E->xbuf,
-               E->cg_r, E->cg_z, E->cg_p, E->cg_part, E->cg_mb, E->cg_iters, E->cg_tol2, PCG_MAXIT, E->flags);
+               E->cg_z, E->cg_p, E->cg_part, E->cg_mb, E->cg_iters, E->cg_tol2, PCG_MAXIT, E->flags);
     } else if (!E->mixed) {
         queue_cholesky(&E->timer, s, np, E->Ared, E->Linv, E->flags, E->use_lookahead ? &E->la : nullptr);
         queue_solve<double>(E, s, E->Ared, E->Linv, E->b, E->ybuf, E->xbuf);
