@@ -1046,9 +1046,9 @@ __device__ void bjacobi_apply(const double* __restrict__ Li, const double* rb, d
 // diagonal blocks full.  Workgroup g owns the 64-row blocks g, g + G, ... (at
 // most PCG_MAXB) for the vector updates.  The matrix-vector product runs over
 // the stored blocks, each block read once by one wavefront (contiguous,
-// balanced ranges of the packed order): lane l holds column l of the block,
-// the row sums R_IJ p_J come from a 16-row transpose-reduce of the lanes, the
-// column sums R_IJ^T p_I stay per lane.  Every (row block X, column block Y)
+// balanced ranges of the packed order): lane (rg, cg) holds 8 rows x 8
+// columns of the block, and the row sums R_IJ p_J and column sums
+// R_IJ^T p_I come from 8-lane transpose-reduces.  Every (row block X, column block Y)
 // product goes to its own partial part[X][Y][64], which the owner of X adds
 // in Y order after the barrier; p.q is formed from the partials (it is
 // linear in them).  Per iteration: (1) p = z + beta p_prev (formed on the fly
@@ -1065,31 +1065,25 @@ constexpr double PCG_TOL_FP64 = 1e-13;   // default |r| / |b| of the FP64 path
 constexpr double PCG_TOL_MIXED = 1e-10;  // default of the mixed path (EEQ energy is variational in q)
 constexpr int PCG_MAXIT = 1000;
 
-// Sum over the 64 lanes of 16 per-lane values v[k]: recursive halving, so
-// that lane l ends with the total of row k(l) = 8 b5 + 4 b4 + 2 b3 + b2 (b the
-// bits of l); the four lanes of a row hold the same bits.
-__device__ inline double rowsum16(const double (&v)[16], int l) {
-    double a8[8], a4[4], a2[2];
-    const bool h5 = (l & 32) != 0, h4 = (l & 16) != 0, h3 = (l & 8) != 0, h2 = (l & 4) != 0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const double keep = h5 ? v[i + 8] : v[i], send = h5 ? v[i] : v[i + 8];
-        a8[i] = keep + __shfl_xor(send, 32);
-    }
+// Sum of 8 per-lane values v[k] over the 8 lanes that differ in lane bits
+// s, s + 1, s + 2 (recursive halving): lane l ends with the total of
+// k(l) = 4 b(s+2) + 2 b(s+1) + b(s), b the bits of l; the eight lanes of a
+// group hold the same bits.
+template <int S>
+__device__ inline double tr_reduce8(const double (&v)[8], int l) {
+    double a4[4], a2[2];
+    const bool h2 = (l >> (S + 2)) & 1, h1 = (l >> (S + 1)) & 1, h0 = (l >> S) & 1;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const double keep = h4 ? a8[i + 4] : a8[i], send = h4 ? a8[i] : a8[i + 4];
-        a4[i] = keep + __shfl_xor(send, 16);
+        const double keep = h2 ? v[i + 4] : v[i], send = h2 ? v[i] : v[i + 4];
+        a4[i] = keep + __shfl_xor(send, 4 << S);
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-        const double keep = h3 ? a4[i + 2] : a4[i], send = h3 ? a4[i] : a4[i + 2];
-        a2[i] = keep + __shfl_xor(send, 8);
+        const double keep = h1 ? a4[i + 2] : a4[i], send = h1 ? a4[i] : a4[i + 2];
+        a2[i] = keep + __shfl_xor(send, 2 << S);
     }
-    double a1 = (h2 ? a2[1] : a2[0]) + __shfl_xor(h2 ? a2[0] : a2[1], 4);
-    a1 += __shfl_xor(a1, 2);
-    a1 += __shfl_xor(a1, 1);
-    return a1;
+    return (h0 ? a2[1] : a2[0]) + __shfl_xor(h0 ? a2[0] : a2[1], 1 << S);
 }
 
 __global__ __launch_bounds__(256) void k_eeq_pcg(int np, const double* __restrict__ Rp,
@@ -1105,6 +1099,7 @@ __global__ __launch_bounds__(256) void k_eeq_pcg(int np, const double* __restric
     __shared__ double rb[NB], wb[NB], zb[NB];
     __shared__ double xo[PCG_MAXB][NB], ro[PCG_MAXB][NB];  // owned rows of x and r, resident for the solve
     __shared__ double Lc[NB * NB];                          // Linv of the owned block when there is one
+    __shared__ __align__(16) double pw[4][2][NB];            // per wavefront: p of the block's rows and columns
     const int G = gridDim.x, g = blockIdx.x, nbk = np / NB;
     const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
     const int nown = (nbk - g + G - 1) / G;
@@ -1115,7 +1110,9 @@ __global__ __launch_bounds__(256) void k_eeq_pcg(int np, const double* __restric
     while ((long)(I0 + 1) * (I0 + 2) / 2 <= t0) ++I0;
     while ((long)I0 * (I0 + 1) / 2 > t0) --I0;
     const int J0 = t0 - I0 * (I0 + 1) / 2;
-    const int kl = 8 * ((l >> 5) & 1) + 4 * ((l >> 4) & 1) + 2 * ((l >> 3) & 1) + ((l >> 2) & 1);
+    // lane (rg, cg) holds rows 8 rg + i (i < 8) and columns 2 cg + (k & 1) + 16 (k >> 1) (k < 8) of a
+    // block: every 16-byte load instruction reads whole 128-byte row segments
+    const int rg = l >> 3, cg = l & 7;
     const bool lcache = nown == 1;
     if (lcache)
         for (int e = tid; e < NB * NB; e += 256) Lc[e] = Lj[(size_t)g * NB * NB + e];
@@ -1162,35 +1159,51 @@ __global__ __launch_bounds__(256) void k_eeq_pcg(int np, const double* __restric
         double lpq = 0.0;
         int I = I0, J = J0;
         for (int t = t0; t < t1; ++t) {
-            const double* Rb = Rp + (size_t)t * NB * NB;
-            double rv[NB];
+            const double2* Rb = reinterpret_cast<const double2*>(Rp + (size_t)t * NB * NB);
+            double rv[8][8];
 #pragma unroll
-            for (int k = 0; k < NB; ++k) rv[k] = Rb[k * NB + l];
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const double2 v2 = Rb[((8 * rg + i) * NB + 16 * j + 2 * cg) / 2];
+                    rv[i][2 * j] = v2.x;
+                    rv[i][2 * j + 1] = v2.y;
+                }
             const double pcj = (it == 0) ? 0.0 : ld_c(pprev + J * NB + l);
-            const double pj = ld_c(z + J * NB + l) + beta * pcj;
+            const double pjl = ld_c(z + J * NB + l) + beta * pcj;  // p of column J * 64 + l
             const double pci = (it == 0) ? 0.0 : ld_c(pprev + I * NB + l);
             const double pil = ld_c(z + I * NB + l) + beta * pci;  // p of row I * 64 + l
-            double cs = 0.0;
+            pw[wv][0][l] = pil;
+            pw[wv][1][l] = pjl;
+            __builtin_amdgcn_wave_barrier();
+            double pi[8], pj[8];
 #pragma unroll
-            for (int h = 0; h < 4; ++h) {
-                double v[16];
-                double pk = 0.0;
-#pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    const double pi = __shfl(pil, 16 * h + k);
-                    v[k] = rv[16 * h + k] * pj;
-                    cs += rv[16 * h + k] * pi;
-                    pk = (k == kl) ? pi : pk;
-                }
-                const double rs = rowsum16(v, l);
-                if ((l & 3) == 0) {
-                    st_c(part + ((size_t)I * nbk + J) * NB + 16 * h + kl, rs);
-                    lpq += pk * rs;
-                }
+            for (int k = 0; k < 8; ++k) {
+                pi[k] = pw[wv][0][8 * rg + k];
+                pj[k] = pw[wv][1][2 * cg + (k & 1) + 16 * (k >> 1)];
             }
+            const int cend = 2 * cg + (rg & 1) + 16 * (rg >> 1);  // the column this lane's column sum ends on
+            const double pjc = pw[wv][1][cend];
+            __builtin_amdgcn_wave_barrier();
+            double rs8[8], cs8[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                double a = 0.0, c = 0.0;
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    a += rv[k][m] * pj[m];
+                    c += rv[m][k] * pi[m];
+                }
+                rs8[k] = a;
+                cs8[k] = c;
+            }
+            const double rs = tr_reduce8<0>(rs8, l);  // row 8 rg + cg = l
+            st_c(part + ((size_t)I * nbk + J) * NB + l, rs);
+            lpq += pil * rs;
             if (I != J) {
-                st_c(part + ((size_t)J * nbk + I) * NB + l, cs);
-                lpq += pj * cs;
+                const double cs = tr_reduce8<3>(cs8, l);  // column cend
+                st_c(part + ((size_t)J * nbk + I) * NB + cend, cs);
+                lpq += pjc * cs;
             }
             if (++J > I) {
                 ++I;
