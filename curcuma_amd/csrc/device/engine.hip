@@ -2393,7 +2393,29 @@ __global__ __launch_bounds__(256) void k_xb(int nx, const int* __restrict__ axb,
 }
 
 // Case-1 triplet (H bonded to neither A nor B): the cs = 1 path of hb_core
-// without the neighbour, lone-pair and carbonyl factors (all 1).
+// without the neighbour, lone-pair and carbonyl factors (all 1).  T = double
+// is the reference arithmetic; T = float (mixed path) forms the coordinate
+// differences in FP64 and evaluates the term in FP32 with hardware rsqrt,
+// rcp and exp.
+__device__ inline double hb_sqrt(double v) { return sqrt(v); }
+__device__ inline float hb_sqrt(float v) { return __builtin_sqrtf(v); }
+__device__ inline double hb_rcp(double v) { return 1.0 / v; }
+__device__ inline float hb_rcp(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ inline double hb_exp(double v) { return exp(v); }
+__device__ inline float hb_exp(float v) { return __expf(v); }
+template <typename T>
+struct V3 {
+    T x, y, z;
+};
+template <typename T>
+__device__ inline V3<T> v3(double3 d) {
+    return {(T)d.x, (T)d.y, (T)d.z};
+}
+template <typename T>
+__device__ inline double3 d3(T s, V3<T> v) {
+    return make_double3((double)(s * v.x), (double)(s * v.y), (double)(s * v.z));
+}
+template <typename T>
 __device__ double hb_core1(int A, int H, int B, const HBArgs& a, const double* __restrict__ x, double3& ga,
                            double3& gh, double3& gb) {
     const double3 z3 = make_double3(0, 0, 0);
@@ -2401,54 +2423,56 @@ __device__ double hb_core1(int A, int H, int B, const HBArgs& a, const double* _
     gh = z3;
     gb = z3;
     const double3 pA = ld3(x, A), pH = ld3(x, H), pB = ld3(x, B);
-    const double rAH = nrm(pH - pA), rHB = nrm(pB - pH), rAB = nrm(pB - pA);
-    if (!(rAB <= 50.0)) return 0.0;
-    const double basA = a.cb[A], basB = a.cb[B], aciA = a.ca[A], aciB = a.ca[B];
-    const double QH = a.Qh[H], QA = a.Qab[A], QB = a.Qab[B];
-    const double rAH4 = rAH * rAH * rAH * rAH, rHB4 = rHB * rHB * rHB * rHB;
-    const double den = 1.0 / (rAH4 + rHB4);
-    const double bas = (QA * basA * rAH4 + QB * basB * rHB4) * den;
-    const double aci = (aciB * rAH4 + aciA * rHB4) * den;
-    const double radab = a.hb_rad[A] + a.hb_rad[B];
-    const double rab2 = rAB * rAB;
-    const double ratio3 = pow6(22.0 * radab / rab2), ratio1 = pow6(rab2 / 85.0);
-    const double denv = (1.0 / (1.0 + ratio3)) * (1.0 / (1.0 + ratio1));
-    const double expo0 = (49.0 / radab) * ((rAH + rHB) / rAB - 1.0);
-    const double doutl = (expo0 > 15.0) ? 0.0 : 2.0 / (1.0 + exp(expo0));
-    const double rdamp = denv / (rAB * rAB * rAB);
-    const double qhoutl = QH * doutl;
-    const double E = -bas * aci * rdamp * qhoutl;
-    const double3 drab = pA - pB, drah = pA - pH, drbh = pB - pH;
-    const double rbh2 = rHB * rHB, rah2 = rAH * rAH;
-    const double rahprbh = rAH + rHB + 1e-12;
-    const double ddamp = (-2.0 * 6.0 * ratio1 / (1.0 + ratio1)) + (2.0 * 6.0 * ratio3 / (1.0 + ratio3));
-    const double expo = (49.0 / radab) * (rahprbh / rAB - 1.0);
-    if (expo > 15.0) return E;
-    const double ratio2 = exp(expo);
-    const double caa = QA * basA, cbb = QB * basB;
-    const double rterm = -aci * rdamp * qhoutl, dterm = -aci * bas * qhoutl, sterm = -rdamp * bas * qhoutl;
-    const double aterm = -aci * bas * rdamp * QH;
-    const double tmp = den * den * 4.0;
-    const double dd24a = rah2 * rHB4 * tmp, dd24b = rbh2 * rAH4 * tmp;
-    ga = ((caa - cbb) * dd24a * rterm) * drah;
-    gb = ((cbb - caa) * dd24b * rterm) * drbh;
-    gh = (-1.0) * (ga + gb);
-    const double3 dga = ((aciB - aciA) * dd24a * sterm) * drah, dgb = ((aciA - aciB) * dd24b * sterm) * drbh;
-    ga = ga + dga;
-    gb = gb + dgb;
-    gh = gh + (-1.0) * (dga + dgb);
-    double gi = rdamp * (ddamp - 3.0) / rab2;
-    ga = ga + (gi * dterm) * drab;
-    gb = gb + (-gi * dterm) * drab;
-    const double e2 = (1.0 + ratio2) * (1.0 + ratio2);
-    gi = aterm * 2.0 * ratio2 * expo * rahprbh / e2 / (rahprbh - rAB) / rab2;
-    ga = ga + gi * drab;
-    gb = gb + (-gi) * drab;
-    const double to = -2.0 * aterm * ratio2 * expo / e2 / (rahprbh - rAB);
-    const double3 da = (to / rAH) * drah, db = (to / rHB) * drbh;
-    ga = ga + da;
-    gb = gb + db;
-    gh = gh + (-1.0) * (da + db);
+    // difference vectors drah = A - H, drbh = B - H, drab = A - B
+    const V3<T> drah = v3<T>(pA - pH), drbh = v3<T>(pB - pH), drab = v3<T>(pA - pB);
+    const T rah2 = drah.x * drah.x + drah.y * drah.y + drah.z * drah.z;
+    const T rbh2 = drbh.x * drbh.x + drbh.y * drbh.y + drbh.z * drbh.z;
+    const T rab2 = drab.x * drab.x + drab.y * drab.y + drab.z * drab.z;
+    const T rAH = hb_sqrt(rah2), rHB = hb_sqrt(rbh2), rAB = hb_sqrt(rab2);
+    if (!(rAB <= T(50))) return 0.0;
+    const T basA = (T)a.cb[A], basB = (T)a.cb[B], aciA = (T)a.ca[A], aciB = (T)a.ca[B];
+    const T QH = (T)a.Qh[H], QA = (T)a.Qab[A], QB = (T)a.Qab[B];
+    const T rAH4 = rah2 * rah2, rHB4 = rbh2 * rbh2;
+    const T den = hb_rcp(rAH4 + rHB4);
+    const T bas = (QA * basA * rAH4 + QB * basB * rHB4) * den;
+    const T aci = (aciB * rAH4 + aciA * rHB4) * den;
+    const T radab = (T)(a.hb_rad[A] + a.hb_rad[B]);
+    const T irab2 = hb_rcp(rab2), irAB = hb_rcp(rAB);
+    const T t3 = T(22) * radab * irab2, t1 = rab2 * T(1.0 / 85.0);
+    const T ratio3 = (t3 * t3) * (t3 * t3) * (t3 * t3), ratio1 = (t1 * t1) * (t1 * t1) * (t1 * t1);
+    const T denv = hb_rcp(T(1) + ratio3) * hb_rcp(T(1) + ratio1);
+    const T c49 = T(49) * hb_rcp(radab);
+    const T expo0 = c49 * ((rAH + rHB) * irAB - T(1));
+    const T doutl = (expo0 > T(15)) ? T(0) : T(2) * hb_rcp(T(1) + hb_exp(expo0));
+    const T rdamp = denv * irab2 * irAB;
+    const T qhoutl = QH * doutl;
+    const double E = (double)(-bas * aci * rdamp * qhoutl);
+    const T rahprbh = rAH + rHB + T(1e-12);
+    const T ddamp = (T(-12) * ratio1 * hb_rcp(T(1) + ratio1)) + (T(12) * ratio3 * hb_rcp(T(1) + ratio3));
+    const T expo = c49 * (rahprbh * irAB - T(1));
+    if (expo > T(15)) return E;
+    const T ratio2 = hb_exp(expo);
+    const T caa = QA * basA, cbb = QB * basB;
+    const T rterm = -aci * rdamp * qhoutl, dterm = -aci * bas * qhoutl, sterm = -rdamp * bas * qhoutl;
+    const T aterm = -aci * bas * rdamp * QH;
+    const T tmp = den * den * T(4);
+    const T dd24a = rah2 * rHB4 * tmp, dd24b = rbh2 * rAH4 * tmp;
+    // coefficients of drah, drbh, drab in the gradient on A and B (H: minus the sum)
+    // expo / (rAH + rHB - rAB) = 49 / (radab rAB): the reference divides the two
+    // (with a 1e-12 guard); the closed form stays finite for collinear A-H-B
+    const T ie2 = hb_rcp((T(1) + ratio2) * (T(1) + ratio2)), eir = c49 * irAB;
+    const T gi1 = rdamp * (ddamp - T(3)) * irab2 * dterm;
+    const T gi2 = aterm * T(2) * ratio2 * eir * rahprbh * ie2 * irab2;
+    const T to = T(-2) * aterm * ratio2 * ie2 * eir;
+    const T cah = (caa - cbb) * dd24a * rterm + (aciB - aciA) * dd24a * sterm + to * hb_rcp(rAH);
+    const T cbh = (cbb - caa) * dd24b * rterm + (aciA - aciB) * dd24b * sterm + to * hb_rcp(rHB);
+    const T cab = gi1 + gi2;
+    ga = make_double3((double)(cah * drah.x + cab * drab.x), (double)(cah * drah.y + cab * drab.y),
+                      (double)(cah * drah.z + cab * drab.z));
+    gb = make_double3((double)(cbh * drbh.x - cab * drab.x), (double)(cbh * drbh.y - cab * drab.y),
+                      (double)(cbh * drbh.z - cab * drab.z));
+    gh = make_double3(-(double)(cah * drah.x + cbh * drbh.x), -(double)(cah * drah.y + cbh * drbh.y),
+                      -(double)(cah * drah.z + cbh * drbh.z));
     return E;
 }
 
@@ -2467,7 +2491,7 @@ struct HBRoles {
     const int* gatom;   // [ngt]
     int ng0, ng1, ngt;  // role A chunks [0, ng0), H [ng0, ng1), B [ng1, ngt)
 };
-template <int ROLE>
+template <int ROLE, typename T>
 __global__ __launch_bounds__(256) void k_hb_role(HBRoles r, HBArgs a, const double* __restrict__ x,
                                                  double* __restrict__ gout, double* __restrict__ egrp) {
     const int lane = threadIdx.x & 63;
@@ -2481,13 +2505,13 @@ __global__ __launch_bounds__(256) void k_hb_role(HBRoles r, HBArgs a, const doub
         const int2 en = r.ent[e];
         double3 ga, gh, gb;
         if (ROLE == 0) {
-            E += hb_core1(at, en.x, en.y, a, x, ga, gh, gb);
+            E += hb_core1<T>(at, en.x, en.y, a, x, ga, gh, gb);
             g = g + ga;
         } else if (ROLE == 1) {
-            hb_core1(en.x, at, en.y, a, x, ga, gh, gb);
+            hb_core1<T>(en.x, at, en.y, a, x, ga, gh, gb);
             g = g + gh;
         } else {
-            hb_core1(en.x, en.y, at, a, x, ga, gh, gb);
+            hb_core1<T>(en.x, en.y, at, a, x, ga, gh, gb);
             g = g + gb;
         }
     }
@@ -3888,10 +3912,43 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->h6 = E->alloc<double>((size_t)n * E->nelem * 7);
         E->dh6 = E->alloc<double>((size_t)n * E->nelem * 7);
         {
-            // j chunks: enough blocks to fill 256 CUs several times over
+            // j chunks: the grid (row blocks x chunks) should fill whole
+            // rounds of the resident-workgroup capacity of the pair kernel in
+            // use (at most two rounds, at least 128 partners per chunk; ties
+            // to fewer chunks)
             const int rowblk = cdiv(n, TILE);
-            int nch = std::min(cdiv(n, TILE), std::max(1, cdiv(4096, rowblk)));
-            E->pair_cj = cdiv(cdiv(n, nch), TILE) * TILE;
+            const void* fn = nullptr;
+            switch (E->mixed ? std::min(P.nelem, 5) : 10 + std::min(P.nelem, 5)) {
+                case 1: fn = (const void*)k_pairs_part32<1>; break;
+                case 2: fn = (const void*)k_pairs_part32<2>; break;
+                case 3: fn = (const void*)k_pairs_part32<3>; break;
+                case 4: fn = (const void*)k_pairs_part32<4>; break;
+                case 5: fn = (const void*)k_pairs_part32<0>; break;
+                case 11: fn = (const void*)k_pairs_part<1>; break;
+                case 12: fn = (const void*)k_pairs_part<2>; break;
+                case 13: fn = (const void*)k_pairs_part<3>; break;
+                case 14: fn = (const void*)k_pairs_part<4>; break;
+                default: fn = (const void*)k_pairs_part<0>; break;
+            }
+            int dev = 0, ncu = 0, per_cu = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, TILE, 0) != hipSuccess || per_cu < 1)
+                throw std::runtime_error("pair kernel occupancy query failed");
+            const long cap = (long)ncu * per_cu;
+            int best = 1;
+            double best_eff = -1.0;
+            for (int nck = 1; nck <= std::min(n, (int)(2 * cap)); ++nck) {
+                const int cj = cdiv(n, nck), real = cdiv(n, cj);
+                const long wg = (long)rowblk * real, rounds = (wg + cap - 1) / cap;
+                if (rounds > 2 || cj < std::min(n, 128)) break;  // at least 128 partners per chunk
+                const double eff = (double)wg / (double)(rounds * cap);
+                if (eff > best_eff + 1e-9) {
+                    best_eff = eff;
+                    best = real;
+                }
+            }
+            E->pair_cj = cdiv(n, best);
             E->pair_nchunk = cdiv(n, E->pair_cj);
             if (E->nmol > 1) {  // rows see only their molecule: one chunk
                 E->pair_cj = n;
@@ -4134,14 +4191,21 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         HBArgs ha{E->hb_cb, E->hb_ca, E->hb_Qh, E->hb_Qab, E->hb_nboff, E->hb_nbidx, E->hb_parc,
                   E->hb_nbcoff, E->hb_nbcidx, E->hb_rad, E->repz, E->Z};
         HBRoles hr{E->hb_ent, E->hb_goff, E->hb_gatom, E->hb_ng0, E->hb_ng1, E->hb_ngt};
-        if (E->hb_ng0)
-            LAUNCH(KC_HB, k_hb_role<0>, dim3(cdiv(E->hb_ng0, 4)), dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
-        if (E->hb_ng1 > E->hb_ng0)
-            LAUNCH(KC_HB, k_hb_role<1>, dim3(cdiv(E->hb_ng1 - E->hb_ng0, 4)), dim3(256), s, hr, ha, E->x, E->hb_gout,
-                   E->e_hb);
-        if (E->hb_ngt > E->hb_ng1)
-            LAUNCH(KC_HB, k_hb_role<2>, dim3(cdiv(E->hb_ngt - E->hb_ng1, 4)), dim3(256), s, hr, ha, E->x, E->hb_gout,
-                   E->e_hb);
+        // case-1 role kernels: FP32 terms on the mixed path, FP64 otherwise
+        const int gr0[3] = {0, E->hb_ng0, E->hb_ng1}, gr1[3] = {E->hb_ng0, E->hb_ng1, E->hb_ngt};
+        for (int role = 0; role < 3; ++role) {
+            const dim3 grid(cdiv(gr1[role] - gr0[role], 4));
+            if (gr1[role] <= gr0[role]) continue;
+            if (E->mixed) {
+                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+            } else {
+                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+            }
+        }
         if (E->nhb2)
             LAUNCH(KC_HB, k_hb_full, dim3(cdiv(E->nhb2, 256)), dim3(256), s, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
                    E->hb_xslots, E->e_hb2);
