@@ -1739,6 +1739,247 @@ __global__ __launch_bounds__(TILE) void k_pairs_part32(PairArgs a, int cj, doubl
     p[(size_t)6 * n] = er;
 }
 
+// ---------------------------------------------------------------------------
+// Pair terms with each unordered pair evaluated once (single molecules).
+// Tiles of 64 rows x 64 columns of the lower triangle (row block R >= column
+// block C, tile t = R (R + 1) / 2 + C), one wavefront each.  The column
+// atoms' data sit in LDS; at step s lane l pairs its row atom with column
+// (l + s) mod 64, so every lane meets every column once and the 64 lanes of
+// one step touch 64 distinct columns: the column-side sums accumulate in LDS
+// without atomics, in a fixed order.  Diagonal tiles take s = 1 .. 31 and,
+// for l < 32, s = 32 (each unordered pair once).  Every tile writes its
+// row-side and column-side sums (PART_K quantities x 64 atoms) to partials of
+// its own, which k_pairs_sym_reduce adds per atom in a fixed order.  The pair
+// arithmetic is that of k_pairs_part (T = double) or k_pairs_part32
+// (T = float, FP64 coordinate differences and FP64 sums); the partner's
+// dC6/dCN uses sum_b h_i[e_j][b] dgw_j[b], the transposed half contraction.
+constexpr int PS_B = 64;
+__device__ inline float ps_rsqrt(float v) { return __builtin_amdgcn_rsqf(v); }
+__device__ inline double ps_rsqrt(double v) { return 1.0 / sqrt(v); }
+__device__ inline float ps_sqrt(float v) { return __builtin_sqrtf(v); }
+__device__ inline double ps_sqrt(double v) { return sqrt(v); }
+__device__ inline float ps_rcp(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ inline double ps_rcp(double v) { return 1.0 / v; }
+__device__ inline float ps_exp(float v) { return __expf(v); }
+__device__ inline double ps_exp(double v) { return exp(v); }
+__device__ inline float ps_erf(float v) { return erff(v); }
+__device__ inline double ps_erf(double v) { return erf(v); }
+
+template <typename T, int NE>
+__global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restrict__ part) {
+    __shared__ double cx[PS_B], cy[PS_B], cz[PS_B];
+    __shared__ T cq[PS_B], cal[PS_B], czt[PS_B], csz[PS_B], can[PS_B], crz[PS_B];
+    __shared__ T cgw[7][PS_B], cdgw[7][PS_B];
+    __shared__ int cZ[PS_B], cel[PS_B], cmt[PS_B];
+    __shared__ double acc[PART_K][PS_B];
+    const int t = blockIdx.x, l = threadIdx.x;
+    int R = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((R + 1) * (R + 2) / 2 <= t) ++R;
+    while (R * (R + 1) / 2 > t) --R;
+    const int C = t - R * (R + 1) / 2;
+    const int n = a.n, ne = a.nelem;
+    const int i = R * PS_B + l, jl = C * PS_B + l;
+    const bool acti = i < n;
+    {
+        const bool actj = jl < n;
+        const int js = actj ? jl : 0;
+        cx[l] = a.x[3 * js];
+        cy[l] = a.x[3 * js + 1];
+        cz[l] = a.x[3 * js + 2];
+        cq[l] = (T)a.q[js];
+        cal[l] = (T)a.alp[js];
+        czt[l] = (T)a.zeta[js];
+        csz[l] = (T)a.szr[js];
+        can[l] = (T)a.anb[js];
+        crz[l] = (T)a.repz[js];
+        cZ[l] = a.Z[js];
+        cel[l] = a.elem[js];
+        cmt[l] = a.metal[js];
+#pragma unroll
+        for (int r = 0; r < 7; ++r) {
+            cgw[r][l] = (T)a.gw[(size_t)js * 7 + r];
+            cdgw[r][l] = (T)a.dgw[(size_t)js * 7 + r];
+        }
+#pragma unroll
+        for (int k = 0; k < PART_K; ++k) acc[k][l] = 0.0;
+    }
+    __syncthreads();
+    const int is = acti ? i : 0;
+    const double3 xi = ld3(a.x, is);
+    const T qi = (T)a.q[is], ali = (T)a.alp[is], zti = (T)a.zeta[is], szi = (T)a.szr[is];
+    const T ani = (T)a.anb[is], rzi = (T)a.repz[is];
+    const int zi = a.Z[is], mi = a.metal[is];
+    constexpr int NH = NE > 0 ? NE * 7 : 1;
+    T hr[NH], dhr[NH];
+    if (NE > 0) {
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            hr[k] = (T)a.h[(size_t)is * NH + k];
+            dhr[k] = (T)a.dh[(size_t)is * NH + k];
+        }
+    }
+    const int jmax = min(PS_B, n - C * PS_B);  // columns present in this tile
+    const bool diag = R == C;
+    const int s0 = diag ? 1 : 0, s1 = diag ? (l < 32 ? 32 : 31) : PS_B - 1;
+    double gx = 0, gy = 0, gz = 0, dcn = 0, ec = 0, ed = 0, er = 0;
+    for (int s = s0; s <= s1; ++s) {
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);  // keep the LDS updates of consecutive steps in order
+        const int jj = (l + s) & (PS_B - 1);
+        if (!acti || jj >= jmax) continue;
+        const double dxd = xi.x - cx[jj], dyd = xi.y - cy[jj], dzd = xi.z - cz[jj];
+        const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
+        if (r2d > 10000.0) continue;  // beyond every pair cut-off (Coulomb 100 Bohr)
+        T dEdr = 0, hec = 0, hed = 0, her = 0, dci = 0, dcj = 0;
+        const T r2 = (T)r2d;
+        T r, rinv;
+        if (sizeof(T) == 4) {
+            rinv = ps_rsqrt(r2);
+            r = r2 * rinv;
+        } else {
+            r = ps_sqrt(r2);
+            rinv = T(1) / r;
+        }
+        const bool coul = (sizeof(T) == 4) ? (r2d >= 1e-20) : (r <= T(100) && r >= T(1e-10));
+        const bool disp = (sizeof(T) == 4) ? (r2d <= 2500.0 && r2d >= 1e-20) : (r <= T(50) && r >= T(1e-10));
+        const bool rep = (sizeof(T) == 4) ? (r2d <= 400.0 && r2d >= 1e-16) : (r <= T(20) && r >= T(1e-8));
+        if (coul) {
+            const T gij = ps_rsqrt(ali + cal[jj]);
+            const T gr = gij * r;
+            const T ef = ps_erf(gr);
+            const T qq = qi * cq[jj];
+            if (sizeof(T) == 4) {
+                hec = T(0.5) * qq * ef * rinv;
+                dEdr += qq * (gij * ps_exp(-gr * gr) * T(1.1283791671) * rinv - ef * rinv * rinv);
+            } else {
+                hec = T(0.5) * qq * ef / r;
+                dEdr += qq * (gij * ps_exp(-gr * gr) * T(2.0 / 1.772453850905516) / r - ef / r2);
+            }
+        }
+        if (disp) {
+            const int ej = cel[jj];
+            T c6 = 0;
+            if (NE > 0) {
+#pragma unroll
+                for (int e = 0; e < (NE > 0 ? NE : 1); ++e)
+                    if (e == ej) {
+#pragma unroll
+                        for (int bb = 0; bb < 7; ++bb) {
+                            const T gj = cgw[bb][jj], dgj = cdgw[bb][jj];
+                            c6 += hr[e * 7 + bb] * gj;
+                            dci += dhr[e * 7 + bb] * gj;
+                            dcj += hr[e * 7 + bb] * dgj;
+                        }
+                    }
+            } else {
+                const double* hp = a.h + ((size_t)is * ne + ej) * 7;
+                const double* dp = a.dh + ((size_t)is * ne + ej) * 7;
+#pragma unroll
+                for (int bb = 0; bb < 7; ++bb) {
+                    const T gj = cgw[bb][jj], dgj = cdgw[bb][jj];
+                    c6 += (T)hp[bb] * gj;
+                    dci += (T)dp[bb] * gj;
+                    dcj += (T)hp[bb] * dgj;
+                }
+            }
+            const T r4r2 = T(3) * szi * csz[jj];
+            const T r0 = T(0.58) * ps_sqrt(r4r2) + T(4.80);
+            const T r0s = r0 * r0;
+            const T zc = zti * czt[jj];
+            const T r6 = r2 * r2 * r2, r06 = r0s * r0s * r0s;
+            T t6, t8;
+            if (sizeof(T) == 4) {
+                t6 = ps_rcp(r6 + r06);
+                t8 = ps_rcp(r6 * r2 + r06 * r0s);
+            } else {
+                t6 = T(1) / (r6 + r06);
+                t8 = T(1) / (r6 * r2 + r06 * r0s);
+            }
+            const T sum = t6 + T(2) * r4r2 * t8;
+            hed = T(0.5) * (-c6 * sum * zc);
+            const T d6 = T(-6) * r2 * r2 * t6 * t6, d8 = T(-8) * r2 * r2 * r2 * t8 * t8;
+            dEdr += -c6 * zc * (d6 + T(2) * r4r2 * d8) * r;
+            dci = dci * sum * zc;
+            dcj = dcj * sum * zc;
+        }
+        if (rep) {
+            const T ff = (T)rep_ff(zi, cZ[jj], mi, cmt[jj]);
+            const T al = ps_sqrt(ani * can[jj]) * ff;
+            const T ab = rzi * crz[jj] * T(0.4270);
+            const T sr = ps_sqrt(r);
+            T e;
+            if (sizeof(T) == 4) {
+                e = ab * ps_exp(-al * r * sr) * rinv;
+                dEdr += -e * rinv - T(1.5) * al * sr * e;
+            } else {
+                e = ab * ps_exp(-al * r * sr) / r;
+                dEdr += -e / r - T(1.5) * al * sr * e;
+            }
+            her = T(0.5) * e;
+        }
+        const double sg = (sizeof(T) == 4) ? (double)(dEdr * rinv) : (double)dEdr / (double)r;
+        const double fx = sg * dxd, fy = sg * dyd, fz = sg * dzd;
+        gx += fx;
+        gy += fy;
+        gz += fz;
+        ec += (double)hec;
+        ed += (double)hed;
+        er += (double)her;
+        dcn -= (double)dci;
+        acc[0][jj] -= fx;
+        acc[1][jj] -= fy;
+        acc[2][jj] -= fz;
+        if (disp) acc[3][jj] -= (double)dcj;
+        acc[4][jj] += (double)hec;
+        if (disp) acc[5][jj] += (double)hed;
+        if (rep) acc[6][jj] += (double)her;
+    }
+    __syncthreads();
+    double* pr = part + (size_t)t * 2 * PART_K * PS_B;
+    const double iv[PART_K] = {gx, gy, gz, dcn, ec, ed, er};
+#pragma unroll
+    for (int k = 0; k < PART_K; ++k) {
+        pr[k * PS_B + l] = iv[k];
+        pr[(PART_K + k) * PS_B + l] = acc[k][l];
+    }
+}
+
+// Per atom a = A 64 + l: the row-side partials of tiles (A, C), C = 0 .. A,
+// then the column-side partials of tiles (R, A), R = A .. nb - 1, in that
+// order; one wavefront per block per quarter of the tiles, the four quarters
+// added in order.
+__global__ __launch_bounds__(256) void k_pairs_sym_reduce(int n, int nb, const double* __restrict__ part,
+                                                          double* __restrict__ g, double* __restrict__ dEdcn,
+                                                          double* __restrict__ e_coul, double* __restrict__ e_disp,
+                                                          double* __restrict__ e_rep) {
+    __shared__ double qs[4][PART_K][PS_B];
+    const int A = blockIdx.x, l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ntile = nb + 1;  // A + 1 row-side tiles and nb - A column-side tiles
+    double s[PART_K] = {0, 0, 0, 0, 0, 0, 0};
+    for (int u = w; u < ntile; u += 4) {
+        const bool rowside = u <= A;
+        const int Rr = u - 1;  // column side: tile (R, A), R = u - 1 = A .. nb - 1
+        const int t = rowside ? A * (A + 1) / 2 + u : Rr * (Rr + 1) / 2 + A;
+        const double* p = part + ((size_t)t * 2 + (rowside ? 0 : 1)) * PART_K * PS_B + l;
+#pragma unroll
+        for (int k = 0; k < PART_K; ++k) s[k] += p[k * PS_B];
+    }
+#pragma unroll
+    for (int k = 0; k < PART_K; ++k) qs[w][k][l] = s[k];
+    __syncthreads();
+    const int i = A * PS_B + threadIdx.x;
+    if (threadIdx.x >= PS_B || i >= n) return;
+    double v[PART_K];
+#pragma unroll
+    for (int k = 0; k < PART_K; ++k) v[k] = (qs[0][k][l] + qs[1][k][l]) + (qs[2][k][l] + qs[3][k][l]);
+    g[3 * i] = v[0];
+    g[3 * i + 1] = v[1];
+    g[3 * i + 2] = v[2];
+    dEdcn[i] = v[3];
+    e_coul[i] = v[4];
+    e_disp[i] = v[5];
+    e_rep[i] = v[6];
+}
+
 // Sums the j-chunk partials of k_pairs_part in chunk order.
 __global__ __launch_bounds__(256) void k_pairs_reduce(int n, int nchunk, const double* __restrict__ part,
                                                       double* __restrict__ g, double* __restrict__ dEdcn,
@@ -3274,6 +3515,8 @@ struct DeviceEngine {
     int nbond = 0, nangle = 0, ntors = 0, nbrep = 0, nhb = 0, natm = 0, nbatm = 0, ninv = 0, nstors = 0, nxb = 0;
     int nslots = 0;
     int pair_cj = 0, pair_nchunk = 0;   // j-chunk size and count of the pair kernel
+    int ps_nb = 0;                      // 64-atom blocks of the symmetric pair kernel (single molecules)
+    double* ps_part = nullptr;          // its per-tile row and column partials
     bool mixed = false;                 // FP32 pair terms + FP32 factorisation with FP64 refinement
     Lookahead la;                       // side stream of the factorisation
     bool use_lookahead = false;
@@ -3956,6 +4199,11 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
             }
         }
         E->ppart = E->alloc<double>((size_t)E->pair_nchunk * PART_K * n);
+        if (E->nmol == 1) {
+            // each unordered pair once: 64 x 64 tiles of the lower triangle
+            E->ps_nb = cdiv(n, PS_B);
+            E->ps_part = E->alloc<double>((size_t)E->ps_nb * (E->ps_nb + 1) / 2 * 2 * PART_K * PS_B);
+        }
         E->e_bond = E->alloc<double>(std::max(E->nbond, 1));
         E->e_ang = E->alloc<double>(std::max(E->nangle, 1));
         E->e_tor = E->alloc<double>(std::max(E->ntors, 1));
@@ -4140,7 +4388,28 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     PairArgs pa = E->pair_args();
     LAUNCH(KC_PAIRS_AUX, k_d4_half, dim3(cdiv(n * E->nelem, 256)), dim3(256), s, n, E->nelem, E->elem, E->nref,
            E->c6ref, E->gw, E->dgw, E->h6, E->dh6);
-    {
+    if (E->nmol == 1) {
+        const dim3 grid(E->ps_nb * (E->ps_nb + 1) / 2);
+        if (!E->mixed) {
+            switch (E->nelem) {
+                case 1: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 1>), grid, dim3(PS_B), s, pa, E->ps_part); break;
+                case 2: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 2>), grid, dim3(PS_B), s, pa, E->ps_part); break;
+                case 3: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 3>), grid, dim3(PS_B), s, pa, E->ps_part); break;
+                case 4: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 4>), grid, dim3(PS_B), s, pa, E->ps_part); break;
+                default: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 0>), grid, dim3(PS_B), s, pa, E->ps_part); break;
+            }
+        } else {
+            switch (E->nelem) {
+                case 1: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 1>), grid, dim3(PS_B), s, pa, E->ps_part); break;
+                case 2: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 2>), grid, dim3(PS_B), s, pa, E->ps_part); break;
+                case 3: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 3>), grid, dim3(PS_B), s, pa, E->ps_part); break;
+                case 4: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 4>), grid, dim3(PS_B), s, pa, E->ps_part); break;
+                default: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 0>), grid, dim3(PS_B), s, pa, E->ps_part); break;
+            }
+        }
+        LAUNCH(KC_PAIRS_AUX, k_pairs_sym_reduce, dim3(E->ps_nb), dim3(256), s, n, E->ps_nb, E->ps_part, E->gpair,
+               E->dEdcn, E->e_coul, E->e_disp, E->e_rep);
+    } else {
         const dim3 grid(cdiv(n, TILE), E->pair_nchunk);
         if (!E->mixed) {
             switch (E->nelem) {
@@ -4159,9 +4428,9 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
                 default: LAUNCH(KC_PAIRS, k_pairs_part32<0>, grid, dim3(TILE), s, pa, E->pair_cj, E->ppart); break;
             }
         }
+        LAUNCH(KC_PAIRS_AUX, k_pairs_reduce, dim3(cdiv(n, 256)), dim3(256), s, n, E->pair_nchunk, E->ppart, E->gpair,
+               E->dEdcn, E->e_coul, E->e_disp, E->e_rep);
     }
-    LAUNCH(KC_PAIRS_AUX, k_pairs_reduce, dim3(cdiv(n, 256)), dim3(256), s, n, E->pair_nchunk, E->ppart, E->gpair,
-           E->dEdcn, E->e_coul, E->e_disp, E->e_rep);
     LAUNCH(KC_NBREP, k_nbrep_fix, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->Z, E->metal, E->anb, E->repz,
                        E->toff, E->tidx, E->tdist, E->gpair, E->e_rep);
     // bonded lists
