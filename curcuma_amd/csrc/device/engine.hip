@@ -232,13 +232,14 @@ struct RedSm {
     double ra_[NB], ca_[NB], ura[RED_UMAX], uca[RED_UMAX];
     int sia[NB], sib[NB], slr[NB], slc[NB], sur[RED_UMAX], suc[RED_UMAX];
 };
-__global__ __launch_bounds__(256) void k_eeq_reduced(int nr, int np, const int* __restrict__ ra,
+constexpr int RED_T = 512;  // threads per tile of k_eeq_reduced
+__global__ __launch_bounds__(RED_T) void k_eeq_reduced(int nr, int np, const int* __restrict__ ra,
                                                      const int* __restrict__ rl, const int* __restrict__ lidx,
                                                      const int* __restrict__ ul, const int* __restrict__ nul,
                                                      const double* __restrict__ x, const double* __restrict__ alp,
                                                      const double* __restrict__ diag, double* __restrict__ A,
                                                      int full) {
-    // one 64x64 lower tile per workgroup of 256 threads; grid enumerates
+    // one 64x64 lower tile per workgroup of RED_T threads; grid enumerates
     // tiles (bi >= bj).  full == 0: lower triangle of the row-major np x np
     // matrix (Cholesky).  full == 1: also the mirrored upper tile.  full == 2:
     // packed symmetric storage for the iterative solver, tile t at A + t 64^2
@@ -280,28 +281,28 @@ __global__ __launch_bounds__(256) void k_eeq_reduced(int nr, int np, const int* 
     }
     __syncthreads();
     if (ded) {
-        for (int e = tid; e < NB * nuc; e += 256) {
+        for (int e = tid; e < NB * nuc; e += RED_T) {
             const int r = e / nuc, u = e - r * nuc;
             sm.c.Aiu[r][u] = (sm.sia[r] >= 0) ? eeq_a_v(sm.rp[r], sm.ra_[r], sm.ucp[u], sm.uca[u]) : 0.0;
         }
-        for (int e = tid; e < nur * NB; e += 256) {
+        for (int e = tid; e < nur * NB; e += RED_T) {
             const int u = e / NB, c = e - u * NB;
             sm.c.Aui[u][c] = (sm.sib[c] >= 0) ? eeq_a_v(sm.urp[u], sm.ura[u], sm.cp[c], sm.ca_[c]) : 0.0;
         }
-        for (int e = tid; e < nur * nuc; e += 256) {
+        for (int e = tid; e < nur * nuc; e += RED_T) {
             const int u = e / nuc, w = e - u * nuc;
             sm.c.Auu[u][w] = (sm.sur[u] == sm.suc[w]) ? diag[sm.sur[u]]
                                                       : eeq_a_v(sm.urp[u], sm.ura[u], sm.ucp[w], sm.uca[w]);
         }
         __syncthreads();
     }
-    // thread tid owns column cc = tid % 64 of rows rr = tid / 64 + 4 u
-    constexpr int PER = NB * NB / 256;
+    // thread tid owns column cc = tid % 64 of rows rr = tid / 64 + (RED_T / 64) u
+    constexpr int PER = NB * NB / RED_T, RS = RED_T / 64;
     const int cc = tid & 63, c = bj * NB + cc;
     double vals[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-        const int rr = (tid >> 6) + 4 * u, r = bi * NB + rr;
+        const int rr = (tid >> 6) + RS * u, r = bi * NB + rr;
         double v = 0.0;
         if (c <= r) {
             if (r >= nr || c >= nr) {
@@ -325,12 +326,12 @@ __global__ __launch_bounds__(256) void k_eeq_reduced(int nr, int np, const int* 
     if (!full || (full == 2 && bi != bj)) return;
     __syncthreads();  // the caches are no longer read: Tl reuses their memory
 #pragma unroll
-    for (int u = 0; u < PER; ++u) sm.Tl[(tid >> 6) + 4 * u][cc] = vals[u];
+    for (int u = 0; u < PER; ++u) sm.Tl[(tid >> 6) + RS * u][cc] = vals[u];
     __syncthreads();
     // upper counterpart: A(bj*64 + i, bi*64 + j) = T(j, i), row-contiguous stores
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-        const int i = (tid >> 6) + 4 * u, j = cc;
+        const int i = (tid >> 6) + RS * u, j = cc;
         if (bi == bj && j <= i) continue;
         if (full == 2)
             A[(size_t)t * NB * NB + i * NB + j] = sm.Tl[j][i];
@@ -4339,7 +4340,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     // EEQ
     const int np = E->np, nbk = np / NB;
     if (E->nmol == 1)
-        LAUNCH(KC_EEQ_BUILD, k_eeq_reduced, dim3(nbk * (nbk + 1) / 2), dim3(256), s, E->nr, np, E->red_atom,
+        LAUNCH(KC_EEQ_BUILD, k_eeq_reduced, dim3(nbk * (nbk + 1) / 2), dim3(RED_T), s, E->nr, np, E->red_atom,
                E->red_last, E->red_lidx, E->red_ul, E->red_nul, E->x, E->alp, E->diag, E->Ared, E->use_cg ? 2 : 0);
     else
         LAUNCH(KC_EEQ_BUILD, k_eeq_reduced_batch, dim3(E->nbtiles), dim3(256), s, E->btiles, E->mnr, E->mnp, E->mmo,
