@@ -79,6 +79,32 @@ __device__ inline void st3(double* s, int slot, double3 v) {
 // position of every hit, so each neighbour list is ascending in j with the
 // strict d^2 < 36 compare of the reference's builder (cn_calculator.cpp:
 // 367-428).  The raw CN is summed per lane and combined by a fixed butterfly.
+// exact bounding boxes of the 256-atom tiles of the atom order (k_cn's
+// partner tiles): bb[6 t] = (min x, min y, min z, max x, max y, max z)
+__global__ __launch_bounds__(256) void k_tile_box(int n, const double* __restrict__ x, double* __restrict__ bb) {
+    __shared__ double red[6][256];
+    const int t = blockIdx.x, tid = threadIdx.x, j = min(t * 256 + tid, n - 1);
+    const double3 p = ld3(x, j);
+    red[0][tid] = p.x;
+    red[1][tid] = p.y;
+    red[2][tid] = p.z;
+    red[3][tid] = p.x;
+    red[4][tid] = p.y;
+    red[5][tid] = p.z;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                red[d][tid] = fmin(red[d][tid], red[d][tid + w]);
+                red[3 + d][tid] = fmax(red[3 + d][tid], red[3 + d][tid + w]);
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < 6) bb[6 * t + tid] = red[tid][0];
+}
+
 __global__ __launch_bounds__(256) void k_cn(int n, const double* __restrict__ x, const double* __restrict__ rcov,
                                             const int* __restrict__ elem, const int* __restrict__ nref,
                                             const double* __restrict__ refcn, const double* __restrict__ chi_base,
@@ -86,7 +112,8 @@ __global__ __launch_bounds__(256) void k_cn(int n, const double* __restrict__ x,
                                             double* __restrict__ dlog, double* __restrict__ gw,
                                             double* __restrict__ dgw, double* __restrict__ rhs,
                                             int* __restrict__ nbn, int* __restrict__ nbl, int* __restrict__ flags,
-                                            const int* __restrict__ mol_of, const int* __restrict__ mol_off) {
+                                            const int* __restrict__ mol_of, const int* __restrict__ mol_off,
+                                            const double* __restrict__ tbox) {
     __shared__ double sx[256], sy[256], sz[256], sr[256];
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -107,7 +134,18 @@ __global__ __launch_bounds__(256) void k_cn(int n, const double* __restrict__ x,
         }
     }
     for (int t0 = jlo; t0 < jhi; t0 += 256) {
-        __syncthreads();
+        // single molecule: skip a partner tile whose bounding box (tbox, the
+        // 256-atom tiles of the atom order) is beyond the cut-off for all four
+        // atoms; the margin covers the rounding of both distance tests
+        bool near = true;
+        if (!mol_of) {
+            const double* bx = tbox + 6 * (t0 >> 8);
+            const double ex = fmax(0.0, fmax(bx[0] - xi.x, xi.x - bx[3]));
+            const double ey = fmax(0.0, fmax(bx[1] - xi.y, xi.y - bx[4]));
+            const double ez = fmax(0.0, fmax(bx[2] - xi.z, xi.z - bx[5]));
+            near = act && ex * ex + ey * ey + ez * ez <= 36.0 * (1.0 + 1e-9);
+        }
+        if (!__syncthreads_or(near)) continue;
         int jl = t0 + threadIdx.x;
         if (jl < jhi) {
             sx[threadIdx.x] = x[3 * jl];
@@ -3517,6 +3555,7 @@ struct DeviceEngine {
     int nslots = 0;
     int pair_cj = 0, pair_nchunk = 0;   // j-chunk size and count of the pair kernel
     int ps_nb = 0;                      // 64-atom blocks of the symmetric pair kernel (single molecules)
+    double* tbox = nullptr;             // bounding boxes of the 256-atom tiles (CN partner pruning)
     double* ps_part = nullptr;          // its per-tile row and column partials
     bool mixed = false;                 // FP32 pair terms + FP32 factorisation with FP64 refinement
     Lookahead la;                       // side stream of the factorisation
@@ -4203,6 +4242,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         if (E->nmol == 1) {
             // each unordered pair once: 64 x 64 tiles of the lower triangle
             E->ps_nb = cdiv(n, PS_B);
+            E->tbox = E->alloc<double>(6 * (size_t)cdiv(n, 256));
             E->ps_part = E->alloc<double>((size_t)E->ps_nb * (E->ps_nb + 1) / 2 * 2 * PART_K * PS_B);
         }
         E->e_bond = E->alloc<double>(std::max(E->nbond, 1));
@@ -4334,9 +4374,10 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     const int n = E->n;
     hipLaunchKernelGGL(k_copy, dim3(cdiv(3 * n, 256)), dim3(256), 0, s, 3 * n, xin, E->x);
     hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, s, 1, E->flags, 0);
+    if (!E->mol_of) LAUNCH(KC_CN, k_tile_box, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->tbox);
     LAUNCH(KC_CN, k_cn, dim3(cdiv(n, 4)), dim3(256), s, n, E->x, E->rcov_cn, E->elem, E->nref, E->refcn,
                        E->chi_base, E->cnf, E->cn, E->dlog, E->gw, E->dgw, E->rhs, E->nbn, E->nbl, E->flags, E->mol_of,
-           E->mol_off);
+           E->mol_off, E->tbox);
     // EEQ
     const int np = E->np, nbk = np / NB;
     if (E->nmol == 1)
