@@ -949,72 +949,45 @@ __device__ inline void st_cu(unsigned long long* p, unsigned long long v) {
 __device__ inline void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // Grid-wide all-reduce of two values per workgroup for a persistent kernel
-// whose G <= 256 workgroups are all resident.  Mailboxes are 128-byte lines
-// of 64-bit words holding MB_EMPTY (a NaN pattern no arithmetic produces)
-// until written: workgroup g != 0 writes its values to line (g, j & 1) of
-// barrier j, workgroup 0's thread t polls line t, adds the values in a fixed
-// order and writes the sums to the broadcast line j & 1, which the other
-// workgroups poll.  A line is cleared by its writer one barrier later
-// (workgroup 0 has read it by then), and the clear completes before the
-// writer's next value, so a poller never sees a stale value.  Every wait is
-// bounded (FLAG_SOLVE on a timeout).  Ordering without cache maintenance:
-// the data a workgroup shares is written with device-coherent stores that
-// complete (vmcnt) before its mailbox value is written, and read with
-// device-coherent loads issued after the broadcast was seen, so every shared
-// write before the barrier is visible to every workgroup after it.
-// Mailbox layout: lines [0, 512) producers (2 per workgroup), 512 and 513
-// broadcast; MB_WORDS words in all.
+// whose G <= 256 workgroups of 256 threads are all resident.  Mailboxes are
+// 128-byte lines of 64-bit words holding MB_EMPTY (a NaN pattern no
+// arithmetic produces) until written: at barrier j workgroup g writes its
+// values to line (g, j mod 3), and thread t of every workgroup polls line
+// (t, j mod 3) and reads producer t's values; every workgroup then adds the
+// G values in the same fixed order, so all hold identical sums.  A writer
+// clears its line (g, (j + 1) mod 3), last written at barrier j - 2, before
+// writing barrier j: having passed barrier j - 1 it has seen every
+// workgroup's barrier j - 1 value, each written after its writer finished
+// reading barrier j - 2, so no reader still needs the cleared line, and the
+// clear completes before the next value is written.  Every wait is bounded
+// (FLAG_SOLVE on a timeout).  Ordering without cache maintenance: the data a
+// workgroup shares is written with device-coherent stores that complete
+// (vmcnt) before its mailbox value is written, and read with device-coherent
+// loads issued after all values were seen, so every shared write before the
+// barrier is visible to every workgroup after it.
 constexpr unsigned long long MB_EMPTY = 0x7ff0deadbeef0001ull;
 constexpr int MB_LINE = 16;                  // 64-bit words per 128-byte line
-constexpr int MB_WORDS = 514 * MB_LINE;
+constexpr int MB_WORDS = 3 * 256 * MB_LINE;  // lines (g, k), k < 3, at (3 g + k) MB_LINE
 
 __device__ bool grid_allreduce2(double v0, double v1, double* sums, unsigned long long* mb, int j, int* flags,
                                 double* red) {
-    __shared__ double sv[2];
     __shared__ int bad_s;
-    const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, par = j & 1;
-    unsigned long long* const bc = mb + 512 * MB_LINE;
+    const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, k = j % 3;
+    if (tid == 0) bad_s = 0;  // every thread read the previous value before the last call's final barrier
     wait_vm();
     __syncthreads();
-    if (tid == 0) bad_s = 0;
-    if (g != 0) {
-        if (tid == 0) {
-            unsigned long long* const own = mb + (2 * g + par) * MB_LINE;
-            unsigned long long* const old = mb + (2 * g + (par ^ 1)) * MB_LINE;
-            st_cu(old, MB_EMPTY);
-            st_cu(old + 1, MB_EMPTY);
-            wait_vm();
-            st_cu(own, __double_as_longlong(v0));
-            st_cu(own + 1, __double_as_longlong(v1));
-            const unsigned long long* const b = bc + par * MB_LINE;
-            unsigned long long a0 = MB_EMPTY, a1 = MB_EMPTY;
-            for (int it = 0;; ++it) {
-                a0 = ld_cu(b);
-                a1 = ld_cu(b + 1);
-                if (a0 != MB_EMPTY && a1 != MB_EMPTY) break;
-                if (it > SPIN_MAX || (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_SOLVE)) {
-                    bad_s = 1;
-                    atomicOr(flags, FLAG_SOLVE);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-            sv[0] = __longlong_as_double(a0);
-            sv[1] = __longlong_as_double(a1);
-        }
-        __syncthreads();
-        sums[0] = sv[0];
-        sums[1] = sv[1];
-        return bad_s == 0;
-    }
-    // workgroup 0: gather, fixed-order sums, broadcast
-    __syncthreads();
-    double u0 = 0.0, u1 = 0.0;
     if (tid == 0) {
-        u0 = v0;
-        u1 = v1;
-    } else if (tid < G) {
-        const unsigned long long* const in = mb + (2 * tid + par) * MB_LINE;
+        unsigned long long* const own = mb + (3 * g + k) * MB_LINE;
+        unsigned long long* const old = mb + (3 * g + (k + 1) % 3) * MB_LINE;
+        st_cu(old, MB_EMPTY);
+        st_cu(old + 1, MB_EMPTY);
+        wait_vm();
+        st_cu(own, __double_as_longlong(v0));
+        st_cu(own + 1, __double_as_longlong(v1));
+    }
+    double u0 = 0.0, u1 = 0.0;
+    if (tid < G) {
+        const unsigned long long* const in = mb + (3 * tid + k) * MB_LINE;
         unsigned long long a0 = MB_EMPTY, a1 = MB_EMPTY;
         for (int it = 0;; ++it) {
             a0 = ld_cu(in);
@@ -1038,20 +1011,11 @@ __device__ bool grid_allreduce2(double v0, double v1, double* sums, unsigned lon
         red[4 + (tid >> 6)] = u1;
     }
     __syncthreads();
-    const double s0 = (red[0] + red[1]) + (red[2] + red[3]), s1 = (red[4] + red[5]) + (red[6] + red[7]);
-    if (tid == 0 && bad_s == 0) {
-        unsigned long long* const old = bc + (par ^ 1) * MB_LINE;
-        st_cu(old, MB_EMPTY);
-        st_cu(old + 1, MB_EMPTY);
-        wait_vm();
-        unsigned long long* const out = bc + par * MB_LINE;
-        st_cu(out, __double_as_longlong(s0));
-        st_cu(out + 1, __double_as_longlong(s1));
-    }
-    sums[0] = s0;
-    sums[1] = s1;
+    sums[0] = (red[0] + red[1]) + (red[2] + red[3]);
+    sums[1] = (red[4] + red[5]) + (red[6] + red[7]);
+    const bool ok = bad_s == 0;
     __syncthreads();
-    return bad_s == 0;
+    return ok;
 }
 
 // z_b = Linv_b^T (Linv_b r_b) for the 64-block b (r_b and the result in LDS;
