@@ -3520,6 +3520,8 @@ struct DeviceEngine {
     int pair_cj = 0, pair_nchunk = 0;   // j-chunk size and count of the pair kernel
     int ps_nb = 0;                      // 64-atom blocks of the symmetric pair kernel (single molecules)
     double* tbox = nullptr;             // bounding boxes of the 256-atom tiles (CN partner pruning)
+    hipStream_t side = nullptr;         // builds the reduced EEQ matrix alongside the evaluation stream
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     double* ps_part = nullptr;          // its per-tile row and column partials
     bool mixed = false;                 // FP32 pair terms + FP32 factorisation with FP64 refinement
     Lookahead la;                       // side stream of the factorisation
@@ -3631,6 +3633,12 @@ struct DeviceEngine {
         }
         if (la.fork) (void)hipEventDestroy(la.fork);
         if (la.join) (void)hipEventDestroy(la.join);
+        if (side) {
+            (void)hipStreamSynchronize(side);
+            (void)hipStreamDestroy(side);
+        }
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
         if (own) {
             (void)hipStreamSynchronize(own);
             (void)hipStreamDestroy(own);
@@ -4228,7 +4236,10 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->flags = E->alloc<int>(1);
         if (hipStreamCreate(&E->own) != hipSuccess || hipStreamCreate(&E->la.side) != hipSuccess ||
             hipEventCreateWithFlags(&E->la.fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&E->la.join, hipEventDisableTiming) != hipSuccess)
+            hipEventCreateWithFlags(&E->la.join, hipEventDisableTiming) != hipSuccess ||
+            hipStreamCreateWithFlags(&E->side, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&E->ev_fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&E->ev_join, hipEventDisableTiming) != hipSuccess)
             throw std::runtime_error("stream");
         hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, E->own, 1, E->flags, 0);
         hipLaunchKernelGGL(k_eeq_diag, dim3(cdiv(n, 256)), dim3(256), 0, E->own, n, E->gam, E->alp, E->diag);
@@ -4338,18 +4349,26 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     const int n = E->n;
     hipLaunchKernelGGL(k_copy, dim3(cdiv(3 * n, 256)), dim3(256), 0, s, 3 * n, xin, E->x);
     hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, s, 1, E->flags, 0);
+    // The reduced EEQ matrix needs only the coordinates: it is built on the
+    // side stream while this stream computes CN, the EEQ right-hand side and
+    // every term that does not depend on the EEQ charges; the solve waits
+    // for both (and runs alone, as the persistent solver requires).
+    HIP_OK(hipEventRecord(E->ev_fork, s));
+    HIP_OK(hipStreamWaitEvent(E->side, E->ev_fork, 0));
+    // EEQ
+    const int np = E->np, nbk = np / NB;
+    if (E->nmol == 1)
+        LAUNCH(KC_EEQ_BUILD, k_eeq_reduced, dim3(nbk * (nbk + 1) / 2), dim3(RED_T), E->side, E->nr, np,
+               E->red_atom, E->red_last, E->red_lidx, E->red_ul, E->red_nul, E->x, E->alp, E->diag, E->Ared,
+               E->use_cg ? 2 : 0);
+    else
+        LAUNCH(KC_EEQ_BUILD, k_eeq_reduced_batch, dim3(E->nbtiles), dim3(256), E->side, E->btiles, E->mnr, E->mnp,
+               E->mmo, E->mvo, E->red_atom, E->red_last, E->x, E->alp, E->diag, E->Ared);
+    HIP_OK(hipEventRecord(E->ev_join, E->side));
     if (!E->mol_of) LAUNCH(KC_CN, k_tile_box, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->tbox);
     LAUNCH(KC_CN, k_cn, dim3(cdiv(n, 4)), dim3(256), s, n, E->x, E->rcov_cn, E->elem, E->nref, E->refcn,
                        E->chi_base, E->cnf, E->cn, E->dlog, E->gw, E->dgw, E->rhs, E->nbn, E->nbl, E->flags, E->mol_of,
            E->mol_off, E->tbox);
-    // EEQ
-    const int np = E->np, nbk = np / NB;
-    if (E->nmol == 1)
-        LAUNCH(KC_EEQ_BUILD, k_eeq_reduced, dim3(nbk * (nbk + 1) / 2), dim3(RED_T), s, E->nr, np, E->red_atom,
-               E->red_last, E->red_lidx, E->red_ul, E->red_nul, E->x, E->alp, E->diag, E->Ared, E->use_cg ? 2 : 0);
-    else
-        LAUNCH(KC_EEQ_BUILD, k_eeq_reduced_batch, dim3(E->nbtiles), dim3(256), s, E->btiles, E->mnr, E->mnp, E->mmo,
-               E->mvo, E->red_atom, E->red_last, E->x, E->alp, E->diag, E->Ared);
     const double* rsrc = E->rhs;
     if (E->charged) {
         LAUNCH(KC_EEQ_BUILD, k_eeq_rhs_qp, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->alp, E->diag, E->qp,
@@ -4358,6 +4377,62 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     }
     LAUNCH(KC_EEQ_BUILD, k_eeq_rhs_reduce, dim3(cdiv(np, 256)), dim3(256), s, E->nr, np, E->red_atom, E->red_last,
                        rsrc, E->b);
+    // bonded lists
+    if (E->nbond)
+        LAUNCH(KC_BONDS, k_bonds, dim3(cdiv(E->nbond, 4)), dim3(256), s, E->nbond, E->bond_ij, E->bond_par,
+                           E->bond_hboff, E->bond_hbB, E->bond_slot, E->x, E->cn, E->rcov_d3, E->bond_H, E->slots,
+                           E->dcn_slot, E->e_bond);
+    if (E->nangle)
+        LAUNCH(KC_ANGLES, k_angles, dim3(cdiv(E->nangle, 256)), dim3(256), s, E->nangle, E->ang_ijk, E->ang_par,
+                           E->ang_slot, E->x, E->rcov_d3, E->slots, E->e_ang);
+    if (E->ntors)
+        LAUNCH(KC_TORS, k_tors, dim3(cdiv(E->ntors, 256)), dim3(256), s, E->ntors, E->tor_ijkl, E->tor_par,
+                           E->tor_slot, E->x, E->rcov_d3, E->slots, E->e_tor);
+    if (E->ninv)
+        LAUNCH(KC_TORS, k_inv, dim3(cdiv(E->ninv, 256)), dim3(256), s, E->ninv, E->inv_ijkl, E->inv_par, E->inv_slot,
+               E->x, E->rcov_d3, E->slots, E->e_inv);
+    if (E->nstors)
+        LAUNCH(KC_TORS, k_stors, dim3(cdiv(E->nstors, 256)), dim3(256), s, E->nstors, E->st_ijkl, E->st_par,
+               E->st_slot, E->x, E->slots, E->e_st);
+    if (E->nxb)
+        LAUNCH(KC_HB, k_xb, dim3(cdiv(E->nxb, 256)), dim3(256), s, E->nxb, E->xb_axb, E->xb_par, E->xb_slot, E->x,
+               E->hb_rad, E->slots, E->e_xb);
+    if (E->nbrep)
+        LAUNCH(KC_BREP, k_brep, dim3(cdiv(E->nbrep, 256)), dim3(256), s, E->nbrep, E->brep_ij, E->brep_par,
+                           E->brep_slot, E->x, E->slots, E->e_brep);
+    if (E->nhb) {
+        HBArgs ha{E->hb_cb, E->hb_ca, E->hb_Qh, E->hb_Qab, E->hb_nboff, E->hb_nbidx, E->hb_parc,
+                  E->hb_nbcoff, E->hb_nbcidx, E->hb_rad, E->repz, E->Z};
+        HBRoles hr{E->hb_ent, E->hb_goff, E->hb_gatom, E->hb_ng0, E->hb_ng1, E->hb_ngt};
+        // case-1 role kernels: FP32 terms on the mixed path, FP64 otherwise
+        const int gr0[3] = {0, E->hb_ng0, E->hb_ng1}, gr1[3] = {E->hb_ng0, E->hb_ng1, E->hb_ngt};
+        for (int role = 0; role < 3; ++role) {
+            const dim3 grid(cdiv(gr1[role] - gr0[role], 4));
+            if (gr1[role] <= gr0[role]) continue;
+            if (E->mixed) {
+                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+            } else {
+                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+            }
+        }
+        if (E->nhb2)
+            LAUNCH(KC_HB, k_hb_full, dim3(cdiv(E->nhb2, 256)), dim3(256), s, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
+                   E->hb_xslots, E->e_hb2);
+    }
+    LAUNCH(KC_HB, k_hb_gather, dim3(cdiv(n, 4)), dim3(256), s, n, E->hb_crange, E->hb_gout, E->hb_xoff, E->hb_xidx,
+           E->hb_xslots, E->ghb);
+    if (E->natm)
+        LAUNCH(KC_ATM, k_atm, dim3(cdiv(E->natm, 256)), dim3(256), s, E->natm, E->atm_ijk, E->atm_sc,
+                           E->atm_slot, E->x, E->rcov_d3, E->nelem, E->elem, E->nref, E->c6ref, E->gw, E->slots,
+                           E->e_atm);
+    if (E->nbatm)
+        LAUNCH(KC_BATM, k_batm, dim3(cdiv(E->nbatm, 256)), dim3(256), s, E->nbatm, E->batm_ijk, E->batm_zb,
+                           E->batm_slot, E->x, E->qa, E->slots, E->e_batm);
+    HIP_OK(hipStreamWaitEvent(s, E->ev_join, 0));
     if (E->nmol > 1) {
         // batch: every molecule's system factored and solved by one workgroup (FP64)
         LAUNCH(KC_CHOL_DIAG, k_eeq_batch_solve, dim3(E->nmol), dim3(256), s, E->mnp, E->mmo, E->mvo, E->Ared,
@@ -4439,61 +4514,6 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     }
     LAUNCH(KC_NBREP, k_nbrep_fix, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->Z, E->metal, E->anb, E->repz,
                        E->toff, E->tidx, E->tdist, E->gpair, E->e_rep);
-    // bonded lists
-    if (E->nbond)
-        LAUNCH(KC_BONDS, k_bonds, dim3(cdiv(E->nbond, 4)), dim3(256), s, E->nbond, E->bond_ij, E->bond_par,
-                           E->bond_hboff, E->bond_hbB, E->bond_slot, E->x, E->cn, E->rcov_d3, E->bond_H, E->slots,
-                           E->dcn_slot, E->e_bond);
-    if (E->nangle)
-        LAUNCH(KC_ANGLES, k_angles, dim3(cdiv(E->nangle, 256)), dim3(256), s, E->nangle, E->ang_ijk, E->ang_par,
-                           E->ang_slot, E->x, E->rcov_d3, E->slots, E->e_ang);
-    if (E->ntors)
-        LAUNCH(KC_TORS, k_tors, dim3(cdiv(E->ntors, 256)), dim3(256), s, E->ntors, E->tor_ijkl, E->tor_par,
-                           E->tor_slot, E->x, E->rcov_d3, E->slots, E->e_tor);
-    if (E->ninv)
-        LAUNCH(KC_TORS, k_inv, dim3(cdiv(E->ninv, 256)), dim3(256), s, E->ninv, E->inv_ijkl, E->inv_par, E->inv_slot,
-               E->x, E->rcov_d3, E->slots, E->e_inv);
-    if (E->nstors)
-        LAUNCH(KC_TORS, k_stors, dim3(cdiv(E->nstors, 256)), dim3(256), s, E->nstors, E->st_ijkl, E->st_par,
-               E->st_slot, E->x, E->slots, E->e_st);
-    if (E->nxb)
-        LAUNCH(KC_HB, k_xb, dim3(cdiv(E->nxb, 256)), dim3(256), s, E->nxb, E->xb_axb, E->xb_par, E->xb_slot, E->x,
-               E->hb_rad, E->slots, E->e_xb);
-    if (E->nbrep)
-        LAUNCH(KC_BREP, k_brep, dim3(cdiv(E->nbrep, 256)), dim3(256), s, E->nbrep, E->brep_ij, E->brep_par,
-                           E->brep_slot, E->x, E->slots, E->e_brep);
-    if (E->nhb) {
-        HBArgs ha{E->hb_cb, E->hb_ca, E->hb_Qh, E->hb_Qab, E->hb_nboff, E->hb_nbidx, E->hb_parc,
-                  E->hb_nbcoff, E->hb_nbcidx, E->hb_rad, E->repz, E->Z};
-        HBRoles hr{E->hb_ent, E->hb_goff, E->hb_gatom, E->hb_ng0, E->hb_ng1, E->hb_ngt};
-        // case-1 role kernels: FP32 terms on the mixed path, FP64 otherwise
-        const int gr0[3] = {0, E->hb_ng0, E->hb_ng1}, gr1[3] = {E->hb_ng0, E->hb_ng1, E->hb_ngt};
-        for (int role = 0; role < 3; ++role) {
-            const dim3 grid(cdiv(gr1[role] - gr0[role], 4));
-            if (gr1[role] <= gr0[role]) continue;
-            if (E->mixed) {
-                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
-            } else {
-                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
-            }
-        }
-        if (E->nhb2)
-            LAUNCH(KC_HB, k_hb_full, dim3(cdiv(E->nhb2, 256)), dim3(256), s, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
-                   E->hb_xslots, E->e_hb2);
-    }
-    LAUNCH(KC_HB, k_hb_gather, dim3(cdiv(n, 4)), dim3(256), s, n, E->hb_crange, E->hb_gout, E->hb_xoff, E->hb_xidx,
-           E->hb_xslots, E->ghb);
-    if (E->natm)
-        LAUNCH(KC_ATM, k_atm, dim3(cdiv(E->natm, 256)), dim3(256), s, E->natm, E->atm_ijk, E->atm_sc,
-                           E->atm_slot, E->x, E->rcov_d3, E->nelem, E->elem, E->nref, E->c6ref, E->gw, E->slots,
-                           E->e_atm);
-    if (E->nbatm)
-        LAUNCH(KC_BATM, k_batm, dim3(cdiv(E->nbatm, 256)), dim3(256), s, E->nbatm, E->batm_ijk, E->batm_zb,
-                           E->batm_slot, E->x, E->qa, E->slots, E->e_batm);
     LAUNCH(KC_CHAIN, k_dcn_total, dim3(cdiv(n, 256)), dim3(256), s, n, E->dcn_off, E->dcn_idx, E->dcn_slot,
                        E->dEdcn, E->qtmp, E->vchain);
     LAUNCH(KC_CHAIN, k_finish, dim3(cdiv(n, 4)), dim3(256), s, n, E->x, E->rcov_cn, E->dlog, E->vchain,
@@ -4693,9 +4713,9 @@ int device_profile_read(DeviceEngine* E, double* total_ms, int* launches, std::s
     KernelTimer& T = E->timer;
     *total_ms = 0.0;
     *launches = 0;
-    if (T.used >= 2) HIP_OK(hipEventSynchronize(T.ev[T.used - 1]));
     for (size_t i = 0; i + 1 < T.used; i += 2) {
         float ms = 0.0f;
+        HIP_OK(hipEventSynchronize(T.ev[i + 1]));  // launches of a class may sit on different streams
         HIP_OK(hipEventElapsedTime(&ms, T.ev[i], T.ev[i + 1]));
         *total_ms += ms;
         ++*launches;

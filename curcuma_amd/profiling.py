@@ -28,6 +28,9 @@ NB = 64                # Cholesky block size of the engine
 # every kernel class the engine can time
 CLASSES = ["cn", "eeq_build", "chol_diag", "chol_panel", "chol_update", "trsv", "charges", "pairs", "pairs_aux", "nbrep", "refine",
            "bonds", "angles", "torsions", "brep", "hb", "atm", "batm", "chain", "reduce", "pcg"]
+# classes whose kernels run on the engine's side stream, overlapped with the
+# evaluation stream (the reduced EEQ matrix build)
+SIDE_STREAM_CLASSES = {"eeq_build"}
 
 
 def read_counts(path: Path = HEADER) -> dict:
