@@ -2948,7 +2948,8 @@ __global__ __launch_bounds__(256) void k_eeq_reduced_batch(const int4* __restric
 // the blocked right-looking Cholesky (diag / panel / MFMA tile update, the
 // routines of the single-molecule path) and both substitutions, all inside
 // the workgroup.  b, x: concatenated padded vectors.
-__global__ __launch_bounds__(256) void k_eeq_batch_solve(const int* __restrict__ mnp, const long long* __restrict__ mo,
+__global__ __launch_bounds__(256) void k_eeq_batch_solve(const int* __restrict__ order,
+                                                         const int* __restrict__ mnp, const long long* __restrict__ mo,
                                                          const int* __restrict__ vo, double* __restrict__ A,
                                                          double* __restrict__ Linv, const double* __restrict__ b,
                                                          double* __restrict__ x, int* __restrict__ flags) {
@@ -2958,7 +2959,7 @@ __global__ __launch_bounds__(256) void k_eeq_batch_solve(const int* __restrict__
     __shared__ double rdiag[NB];
     __shared__ double vec[BATCH_MAX_NP];
     __shared__ double tmp[NB];
-    const int m = blockIdx.x, np = mnp[m], nb = np / NB, base = vo[m];
+    const int m = order[blockIdx.x], np = mnp[m], nb = np / NB, base = vo[m];  // largest systems first
     double* Am = A + mo[m];
     double* Lm = Linv + (size_t)base * NB;
     const int tid = threadIdx.x, t = tid >> 2, q = tid & 3;
@@ -3538,6 +3539,7 @@ struct DeviceEngine {
     int nmol = 1, nbtiles = 0;
     size_t mat_size = 0;                // doubles of the reduced EEQ matrix (or matrices)
     int *mol_of = nullptr, *mol_off = nullptr, *mol_seg = nullptr, *mnr = nullptr, *mnp = nullptr, *mvo = nullptr;
+    int* morder = nullptr;  // batch: molecules by decreasing reduced size (solve launch order)
     long long* mmo = nullptr;
     int4* btiles = nullptr;
     double* e_mol = nullptr;
@@ -3776,6 +3778,14 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
             E->btiles = E->upload(tiles);
             E->mnr = E->upload(mnr);
             E->mnp = E->upload(mnp);
+            {
+                // workgroup order of the per-molecule solves: largest reduced
+                // systems first, so the long factorizations start early
+                std::vector<int> order(P.nmol);
+                std::iota(order.begin(), order.end(), 0);
+                std::stable_sort(order.begin(), order.end(), [&](int a2, int b2) { return mnp[a2] > mnp[b2]; });
+                E->morder = E->upload(order);
+            }
             E->mvo = E->upload(vo);
             E->mmo = E->upload(mo);
             E->mol_off = E->upload(P.mol_off);
@@ -4435,8 +4445,8 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     HIP_OK(hipStreamWaitEvent(s, E->ev_join, 0));
     if (E->nmol > 1) {
         // batch: every molecule's system factored and solved by one workgroup (FP64)
-        LAUNCH(KC_CHOL_DIAG, k_eeq_batch_solve, dim3(E->nmol), dim3(256), s, E->mnp, E->mmo, E->mvo, E->Ared,
-               E->Linv, E->b, E->xbuf, E->flags);
+        LAUNCH(KC_CHOL_DIAG, k_eeq_batch_solve, dim3(E->nmol), dim3(256), s, E->morder, E->mnp, E->mmo, E->mvo,
+               E->Ared, E->Linv, E->b, E->xbuf, E->flags);
     } else if (E->use_cg) {
         // block-Jacobi factors of R's diagonal blocks, then the persistent PCG
         LAUNCH(KC_CHOL_DIAG, k_bjacobi_setup, dim3(nbk), dim3(256), s, E->Ared, E->Linv, E->flags);
