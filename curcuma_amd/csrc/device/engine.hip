@@ -79,6 +79,12 @@ __device__ inline void st3(double* s, int slot, double3 v) {
 // position of every hit, so each neighbour list is ascending in j with the
 // strict d^2 < 36 compare of the reference's builder (cn_calculator.cpp:
 // 367-428).  The raw CN is summed per lane and combined by a fixed butterfly.
+__device__ void cn_atom_finish(int i, double raw, int cnt, const int* __restrict__ elem, const int* __restrict__ nref,
+                               const double* __restrict__ refcn, const double* __restrict__ chi_base,
+                               const double* __restrict__ cnf, double* __restrict__ cn, double* __restrict__ dlog,
+                               double* __restrict__ gw, double* __restrict__ dgw, double* __restrict__ rhs,
+                               int* __restrict__ nbn, int* __restrict__ flags);
+
 // exact bounding boxes of the 256-atom tiles of the atom order (k_cn's
 // partner tiles): bb[6 t] = (min x, min y, min z, max x, max y, max z)
 __global__ __launch_bounds__(256) void k_tile_box(int n, const double* __restrict__ x, double* __restrict__ bb) {
@@ -183,6 +189,16 @@ __global__ __launch_bounds__(256) void k_cn(int n, const double* __restrict__ x,
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) raw += __shfl_xor(raw, off);
     if (!act || lane != 0) return;
+    cn_atom_finish(i, raw, cnt, elem, nref, refcn, chi_base, cnf, cn, dlog, gw, dgw, rhs, nbn, flags);
+}
+
+// Per-atom part of the CN kernels once the neighbour scan is done: list
+// length, log-capped CN and its derivative, D4 weights, EEQ right-hand side.
+__device__ void cn_atom_finish(int i, double raw, int cnt, const int* __restrict__ elem, const int* __restrict__ nref,
+                               const double* __restrict__ refcn, const double* __restrict__ chi_base,
+                               const double* __restrict__ cnf, double* __restrict__ cn, double* __restrict__ dlog,
+                               double* __restrict__ gw, double* __restrict__ dgw, double* __restrict__ rhs,
+                               int* __restrict__ nbn, int* __restrict__ flags) {
     if (cnt > MAXNB_CN) atomicOr(flags, FLAG_CNOVF);
     nbn[i] = min(cnt, MAXNB_CN);
     double c = log(1.0 + exp(CNMAX)) - log(1.0 + exp(CNMAX - raw));
@@ -217,6 +233,63 @@ __global__ __launch_bounds__(256) void k_cn(int n, const double* __restrict__ x,
         dgw[(size_t)i * 7 + r] = dg;
     }
     rhs[i] = chi_base[i] + cnf[i] * sqrt(c);
+}
+
+// CN for a batch of molecules of at most CNMOL_MAX atoms each: one
+// workgroup per molecule stages the molecule's coordinates and radii once;
+// wave w scans atoms w, w + 4, ... against every atom of the molecule with
+// the ballot-ordered list construction of k_cn (ascending partners).
+constexpr int CNMOL_MAX = 1024;
+__global__ __launch_bounds__(256) void k_cn_mol(const int* __restrict__ mol_off, const double* __restrict__ x,
+                                                const double* __restrict__ rcov, const int* __restrict__ elem,
+                                                const int* __restrict__ nref, const double* __restrict__ refcn,
+                                                const double* __restrict__ chi_base, const double* __restrict__ cnf,
+                                                double* __restrict__ cn, double* __restrict__ dlog,
+                                                double* __restrict__ gw, double* __restrict__ dgw,
+                                                double* __restrict__ rhs, int* __restrict__ nbn,
+                                                int* __restrict__ nbl, int* __restrict__ flags) {
+    __shared__ double sx[CNMOL_MAX], sy[CNMOL_MAX], sz[CNMOL_MAX], sr[CNMOL_MAX];
+    const int m = blockIdx.x, a0 = mol_off[m], nm = mol_off[m + 1] - a0;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int k = tid; k < nm; k += 256) {
+        sx[k] = x[3 * (a0 + k)];
+        sy[k] = x[3 * (a0 + k) + 1];
+        sz[k] = x[3 * (a0 + k) + 2];
+        sr[k] = rcov[a0 + k];
+    }
+    __syncthreads();
+    for (int ii = w; ii < nm; ii += 4) {
+        const int i = a0 + ii;
+        const double xix = sx[ii], xiy = sy[ii], xiz = sz[ii], ri = sr[ii];
+        double raw = 0.0;
+        int cnt = 0;
+        for (int s0 = 0; s0 < nm; s0 += 64) {
+            const int jj = s0 + lane;
+            bool hit = false;
+            double contrib = 0.0;
+            if (jj < nm && jj != ii) {
+                const double dx = xix - sx[jj], dy = xiy - sy[jj], dz = xiz - sz[jj];
+                // unfused products and sums, as in k_cn
+                const double d2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+                if (d2 < 36.0) {
+                    hit = true;
+                    const double r = sqrt(d2);
+                    const double rij = ri + sr[jj];
+                    contrib = 0.5 * (1.0 + erf(KN * (r - rij) / rij));
+                }
+            }
+            const unsigned long long mask = __ballot(hit);
+            if (hit) {
+                const int pos = cnt + __popcll(mask & ((1ull << lane) - 1ull));
+                if (pos < MAXNB_CN) nbl[(size_t)i * MAXNB_CN + pos] = a0 + jj;
+                raw += contrib;
+            }
+            cnt += __popcll(mask);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) raw += __shfl_xor(raw, off);
+        if (lane == 0) cn_atom_finish(i, raw, cnt, elem, nref, refcn, chi_base, cnf, cn, dlog, gw, dgw, rhs, nbn, flags);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -3540,6 +3613,7 @@ struct DeviceEngine {
     size_t mat_size = 0;                // doubles of the reduced EEQ matrix (or matrices)
     int *mol_of = nullptr, *mol_off = nullptr, *mol_seg = nullptr, *mnr = nullptr, *mnp = nullptr, *mvo = nullptr;
     int* morder = nullptr;  // batch: molecules by decreasing reduced size (solve launch order)
+    bool cn_by_mol = false;  // batch of molecules of at most CNMOL_MAX atoms: k_cn_mol
     long long* mmo = nullptr;
     int4* btiles = nullptr;
     double* e_mol = nullptr;
@@ -3786,6 +3860,9 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
                 std::stable_sort(order.begin(), order.end(), [&](int a2, int b2) { return mnp[a2] > mnp[b2]; });
                 E->morder = E->upload(order);
             }
+            int maxat = 0;
+            for (int m2 = 0; m2 < P.nmol; ++m2) maxat = std::max(maxat, P.mol_off[m2 + 1] - P.mol_off[m2]);
+            E->cn_by_mol = maxat <= CNMOL_MAX;
             E->mvo = E->upload(vo);
             E->mmo = E->upload(mo);
             E->mol_off = E->upload(P.mol_off);
@@ -4376,9 +4453,13 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
                E->mmo, E->mvo, E->red_atom, E->red_last, E->x, E->alp, E->diag, E->Ared);
     HIP_OK(hipEventRecord(E->ev_join, E->side));
     if (!E->mol_of) LAUNCH(KC_CN, k_tile_box, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->tbox);
-    LAUNCH(KC_CN, k_cn, dim3(cdiv(n, 4)), dim3(256), s, n, E->x, E->rcov_cn, E->elem, E->nref, E->refcn,
-                       E->chi_base, E->cnf, E->cn, E->dlog, E->gw, E->dgw, E->rhs, E->nbn, E->nbl, E->flags, E->mol_of,
-           E->mol_off, E->tbox);
+    if (E->cn_by_mol)
+        LAUNCH(KC_CN, k_cn_mol, dim3(E->nmol), dim3(256), s, E->mol_off, E->x, E->rcov_cn, E->elem, E->nref,
+               E->refcn, E->chi_base, E->cnf, E->cn, E->dlog, E->gw, E->dgw, E->rhs, E->nbn, E->nbl, E->flags);
+    else
+        LAUNCH(KC_CN, k_cn, dim3(cdiv(n, 4)), dim3(256), s, n, E->x, E->rcov_cn, E->elem, E->nref, E->refcn,
+               E->chi_base, E->cnf, E->cn, E->dlog, E->gw, E->dgw, E->rhs, E->nbn, E->nbl, E->flags, E->mol_of,
+               E->mol_off, E->tbox);
     const double* rsrc = E->rhs;
     if (E->charged) {
         LAUNCH(KC_EEQ_BUILD, k_eeq_rhs_qp, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->alp, E->diag, E->qp,
