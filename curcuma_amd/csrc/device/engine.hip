@@ -4451,7 +4451,6 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     else
         LAUNCH(KC_EEQ_BUILD, k_eeq_reduced_batch, dim3(E->nbtiles), dim3(256), E->side, E->btiles, E->mnr, E->mnp,
                E->mmo, E->mvo, E->red_atom, E->red_last, E->x, E->alp, E->diag, E->Ared);
-    HIP_OK(hipEventRecord(E->ev_join, E->side));
     if (!E->mol_of) LAUNCH(KC_CN, k_tile_box, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->tbox);
     if (E->cn_by_mol)
         LAUNCH(KC_CN, k_cn_mol, dim3(E->nmol), dim3(256), s, E->mol_off, E->x, E->rcov_cn, E->elem, E->nref,
@@ -4495,27 +4494,27 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         HBArgs ha{E->hb_cb, E->hb_ca, E->hb_Qh, E->hb_Qab, E->hb_nboff, E->hb_nbidx, E->hb_parc,
                   E->hb_nbcoff, E->hb_nbcidx, E->hb_rad, E->repz, E->Z};
         HBRoles hr{E->hb_ent, E->hb_goff, E->hb_gatom, E->hb_ng0, E->hb_ng1, E->hb_ngt};
-        // case-1 role kernels: FP32 terms on the mixed path, FP64 otherwise
+        // case-1 role kernels (FP32 terms on the mixed path, FP64 otherwise)
+        // on the side stream after the EEQ matrix build, to balance the two
+        // streams; the gather follows the join
         const int gr0[3] = {0, E->hb_ng0, E->hb_ng1}, gr1[3] = {E->hb_ng0, E->hb_ng1, E->hb_ngt};
         for (int role = 0; role < 3; ++role) {
             const dim3 grid(cdiv(gr1[role] - gr0[role], 4));
             if (gr1[role] <= gr0[role]) continue;
             if (E->mixed) {
-                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, float>), grid, dim3(256), E->side, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, float>), grid, dim3(256), E->side, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, float>), grid, dim3(256), E->side, hr, ha, E->x, E->hb_gout, E->e_hb);
             } else {
-                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, double>), grid, dim3(256), E->side, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, double>), grid, dim3(256), E->side, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, double>), grid, dim3(256), E->side, hr, ha, E->x, E->hb_gout, E->e_hb);
             }
         }
         if (E->nhb2)
             LAUNCH(KC_HB, k_hb_full, dim3(cdiv(E->nhb2, 256)), dim3(256), s, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
                    E->hb_xslots, E->e_hb2);
     }
-    LAUNCH(KC_HB, k_hb_gather, dim3(cdiv(n, 4)), dim3(256), s, n, E->hb_crange, E->hb_gout, E->hb_xoff, E->hb_xidx,
-           E->hb_xslots, E->ghb);
     if (E->natm)
         LAUNCH(KC_ATM, k_atm, dim3(cdiv(E->natm, 256)), dim3(256), s, E->natm, E->atm_ijk, E->atm_sc,
                            E->atm_slot, E->x, E->rcov_d3, E->nelem, E->elem, E->nref, E->c6ref, E->gw, E->slots,
@@ -4523,7 +4522,10 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     if (E->nbatm)
         LAUNCH(KC_BATM, k_batm, dim3(cdiv(E->nbatm, 256)), dim3(256), s, E->nbatm, E->batm_ijk, E->batm_zb,
                            E->batm_slot, E->x, E->qa, E->slots, E->e_batm);
+    HIP_OK(hipEventRecord(E->ev_join, E->side));
     HIP_OK(hipStreamWaitEvent(s, E->ev_join, 0));
+    LAUNCH(KC_HB, k_hb_gather, dim3(cdiv(n, 4)), dim3(256), s, n, E->hb_crange, E->hb_gout, E->hb_xoff, E->hb_xidx,
+           E->hb_xslots, E->ghb);
     if (E->nmol > 1) {
         // batch: every molecule's system factored and solved by one workgroup (FP64)
         LAUNCH(KC_CHOL_DIAG, k_eeq_batch_solve, dim3(E->nmol), dim3(256), s, E->morder, E->mnp, E->mmo, E->mvo,
