@@ -299,13 +299,30 @@ __global__ __launch_bounds__(256) void k_cn_mol(const int* __restrict__ mol_off,
 // with A the Coulomb matrix of eeq_solver.cpp:3571-3650.  Lower triangle of
 // the padded (np x np) matrix; the padding is the identity.
 // ---------------------------------------------------------------------------
+// 1/sqrt(v) in FP64: the hardware estimate refined by two Newton steps
+// (within about one ulp; each step doubles the correct bits)
+__device__ inline double rsqrt_nr2(double v) {
+    double y = __builtin_amdgcn_rsq(v);
+    const double h = 0.5 * v;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
+}
+
+// A(i, j) = erf(g r) / r with g = 1 / sqrt(alp_i + alp_j), evaluated as
+// erf(g r2 / r) / r from one reciprocal square root of r2 (no divide or
+// square root sequence on the dependency chain)
+__device__ inline double eeq_a_r2(double r2, double ai, double aj) {
+    const double rinv = rsqrt_nr2(r2);
+    const double g = rsqrt_nr2(ai + aj);
+    return erf(g * (r2 * rinv)) * rinv;
+}
+
 __device__ inline double eeq_a(int i, int j, const double* __restrict__ x, const double* __restrict__ alp,
                                const double* __restrict__ diag) {
     if (i == j) return diag[i];
     double3 d = ld3(x, i) - ld3(x, j);
-    double r = nrm(d);
-    double g = 1.0 / sqrt(alp[i] + alp[j]);
-    return erf(g * r) / r;
+    return eeq_a_r2(dot(d, d), alp[i], alp[j]);
 }
 
 __global__ __launch_bounds__(256) void k_eeq_diag(int n, const double* __restrict__ gam,
@@ -328,9 +345,7 @@ constexpr int RED_UMAX = 32;
 // operations as eeq_a, so the same bits)
 __device__ inline double eeq_a_v(double3 pi, double ai, double3 pj, double aj) {
     double3 d = pi - pj;
-    double r = nrm(d);
-    double g = 1.0 / sqrt(ai + aj);
-    return erf(g * r) / r;
+    return eeq_a_r2(dot(d, d), ai, aj);
 }
 struct RedSm {
     union {
