@@ -4459,10 +4459,18 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     HIP_OK(hipStreamWaitEvent(E->side, E->ev_fork, 0));
     // EEQ
     const int np = E->np, nbk = np / NB;
-    if (E->nmol == 1)
+    if (E->nmol == 1) {
         LAUNCH(KC_EEQ_BUILD, k_eeq_reduced, dim3(nbk * (nbk + 1) / 2), dim3(RED_T), E->side, E->nr, np,
                E->red_atom, E->red_last, E->red_lidx, E->red_ul, E->red_nul, E->x, E->alp, E->diag, E->Ared,
                E->use_cg ? 2 : 0);
+        if (E->use_cg) {
+            // block-Jacobi factors of R's diagonal blocks and the solver's
+            // mailboxes, also off the evaluation stream
+            LAUNCH(KC_CHOL_DIAG, k_bjacobi_setup, dim3(nbk), dim3(256), E->side, E->Ared, E->Linv, E->flags);
+            hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(MB_WORDS, 256)), dim3(256), 0, E->side, MB_WORDS, E->cg_mb,
+                               MB_EMPTY);
+        }
+    }
     else
         LAUNCH(KC_EEQ_BUILD, k_eeq_reduced_batch, dim3(E->nbtiles), dim3(256), E->side, E->btiles, E->mnr, E->mnp,
                E->mmo, E->mvo, E->red_atom, E->red_last, E->x, E->alp, E->diag, E->Ared);
@@ -4546,9 +4554,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         LAUNCH(KC_CHOL_DIAG, k_eeq_batch_solve, dim3(E->nmol), dim3(256), s, E->morder, E->mnp, E->mmo, E->mvo,
                E->Ared, E->Linv, E->b, E->xbuf, E->flags);
     } else if (E->use_cg) {
-        // block-Jacobi factors of R's diagonal blocks, then the persistent PCG
-        LAUNCH(KC_CHOL_DIAG, k_bjacobi_setup, dim3(nbk), dim3(256), s, E->Ared, E->Linv, E->flags);
-        hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(MB_WORDS, 256)), dim3(256), 0, s, MB_WORDS, E->cg_mb, MB_EMPTY);
+        // the persistent PCG (block-Jacobi factors and mailboxes were set up on the side stream)
         LAUNCH(KC_PCG, k_eeq_pcg, dim3(E->cg_groups), dim3(PCG_THREADS), s, np, E->Ared, E->Linv, E->b, E->xbuf,
                E->cg_z, E->cg_p, E->cg_part, E->cg_mb, E->cg_iters, E->cg_tol2, PCG_MAXIT, E->flags);
     } else if (!E->mixed) {
