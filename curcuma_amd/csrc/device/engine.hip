@@ -1842,8 +1842,9 @@ __global__ __launch_bounds__(TILE) void k_pairs_part32(PairArgs a, int cj, doubl
 // row-side and column-side sums (PART_K quantities x 64 atoms) to partials of
 // its own, which k_pairs_sym_reduce adds per atom in a fixed order.  The pair
 // arithmetic is that of k_pairs_part (T = double) or k_pairs_part32
-// (T = float, FP64 coordinate differences and FP64 sums); the partner's
-// dC6/dCN uses sum_b h_i[e_j][b] dgw_j[b], the transposed half contraction.
+// (T = float, FP64 coordinate differences and FP64 sums; the column-side
+// sums of one tile, at most 64 terms each, in FP32); the partner's dC6/dCN
+// uses sum_b h_i[e_j][b] dgw_j[b], the transposed half contraction.
 constexpr int PS_B = 64;
 __device__ inline float ps_rsqrt(float v) { return __builtin_amdgcn_rsqf(v); }
 __device__ inline double ps_rsqrt(double v) { return 1.0 / sqrt(v); }
@@ -1861,8 +1862,8 @@ __global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restri
     __shared__ double cx[PS_B], cy[PS_B], cz[PS_B];
     __shared__ T cq[PS_B], cal[PS_B], czt[PS_B], csz[PS_B], can[PS_B], crz[PS_B];
     __shared__ T cgw[7][PS_B], cdgw[7][PS_B];
-    __shared__ int cZ[PS_B], cel[PS_B], cmt[PS_B];
-    __shared__ double acc[PART_K][PS_B];
+    __shared__ int czem[PS_B];        // Z | element << 8 | metal << 16
+    __shared__ T acc[PART_K][PS_B];   // column-side sums of this tile (at most 64 terms each)
     const int t = blockIdx.x, l = threadIdx.x;
     int R = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
     while ((R + 1) * (R + 2) / 2 <= t) ++R;
@@ -1883,16 +1884,14 @@ __global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restri
         csz[l] = (T)a.szr[js];
         can[l] = (T)a.anb[js];
         crz[l] = (T)a.repz[js];
-        cZ[l] = a.Z[js];
-        cel[l] = a.elem[js];
-        cmt[l] = a.metal[js];
+        czem[l] = a.Z[js] | (a.elem[js] << 8) | (a.metal[js] << 16);
 #pragma unroll
         for (int r = 0; r < 7; ++r) {
             cgw[r][l] = (T)a.gw[(size_t)js * 7 + r];
             cdgw[r][l] = (T)a.dgw[(size_t)js * 7 + r];
         }
 #pragma unroll
-        for (int k = 0; k < PART_K; ++k) acc[k][l] = 0.0;
+        for (int k = 0; k < PART_K; ++k) acc[k][l] = T(0);
     }
     __syncthreads();
     const int is = acti ? i : 0;
@@ -1947,7 +1946,7 @@ __global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restri
             }
         }
         if (disp) {
-            const int ej = cel[jj];
+            const int ej = (czem[jj] >> 8) & 0xff;
             T c6 = 0;
             if (NE > 0) {
 #pragma unroll
@@ -1993,7 +1992,7 @@ __global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restri
             dcj = dcj * sum * zc;
         }
         if (rep) {
-            const T ff = (T)rep_ff(zi, cZ[jj], mi, cmt[jj]);
+            const T ff = (T)rep_ff(zi, czem[jj] & 0xff, mi, czem[jj] >> 16);
             const T al = ps_sqrt(ani * can[jj]) * ff;
             const T ab = rzi * crz[jj] * T(0.4270);
             const T sr = ps_sqrt(r);
@@ -2016,13 +2015,13 @@ __global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restri
         ed += (double)hed;
         er += (double)her;
         dcn -= (double)dci;
-        acc[0][jj] -= fx;
-        acc[1][jj] -= fy;
-        acc[2][jj] -= fz;
-        if (disp) acc[3][jj] -= (double)dcj;
-        acc[4][jj] += (double)hec;
-        if (disp) acc[5][jj] += (double)hed;
-        if (rep) acc[6][jj] += (double)her;
+        acc[0][jj] -= (T)fx;
+        acc[1][jj] -= (T)fy;
+        acc[2][jj] -= (T)fz;
+        if (disp) acc[3][jj] -= dcj;
+        acc[4][jj] += hec;
+        if (disp) acc[5][jj] += hed;
+        if (rep) acc[6][jj] += her;
     }
     __syncthreads();
     double* pr = part + (size_t)t * 2 * PART_K * PS_B;
@@ -2030,7 +2029,7 @@ __global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restri
 #pragma unroll
     for (int k = 0; k < PART_K; ++k) {
         pr[k * PS_B + l] = iv[k];
-        pr[(PART_K + k) * PS_B + l] = acc[k][l];
+        pr[(PART_K + k) * PS_B + l] = (double)acc[k][l];
     }
 }
 
