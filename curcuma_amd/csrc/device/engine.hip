@@ -311,11 +311,37 @@ __device__ inline double rsqrt_nr2(double v) {
 
 // A(i, j) = erf(g r) / r with g = 1 / sqrt(alp_i + alp_j), evaluated as
 // erf(g r2 / r) / r from one reciprocal square root of r2 (no divide or
-// square root sequence on the dependency chain)
+// square root sequence on the dependency chain).  For g r >= 6, erfc(g r) <
+// 2.2e-17 lies below half an ulp of 1 and erf(g r) is 1: the branch is
+// uniform over a wavefront of far-apart atoms and skips the erf evaluation.
 __device__ inline double eeq_a_r2(double r2, double ai, double aj) {
     const double rinv = rsqrt_nr2(r2);
     const double g = rsqrt_nr2(ai + aj);
-    return erf(g * (r2 * rinv)) * rinv;
+    const double v = g * (r2 * rinv);
+    double ef = 1.0;
+    if (v < 6.0) ef = erf(v);
+    return ef * rinv;
+}
+
+// erf(v) and exp(-v^2) for v >= 0 (screened Coulomb pair term and its
+// derivative).  From v = 9 (FP64) or v = 5 (FP32) on, erf(v) rounds to 1 and
+// the derivative's (2 / sqrt(pi)) v exp(-v^2) lies below the type's
+// resolution relative to erf(v), so both are skipped.
+__device__ inline void erf_exp(double v, double& ef, double& eg) {
+    ef = 1.0;
+    eg = 0.0;
+    if (v < 9.0) {
+        ef = erf(v);
+        eg = exp(-v * v);
+    }
+}
+__device__ inline void erf_exp(float v, float& ef, float& eg) {
+    ef = 1.0f;
+    eg = 0.0f;
+    if (v < 5.0f) {
+        ef = erff(v);
+        eg = __expf(-v * v);
+    }
 }
 
 __device__ inline double eeq_a(int i, int j, const double* __restrict__ x, const double* __restrict__ alp,
@@ -1625,10 +1651,11 @@ __global__ __launch_bounds__(TILE) void k_pairs_part(PairArgs a, int cj, double*
             if (r <= 100.0 && r >= 1e-10) {
                 double gij = 1.0 / sqrt(ali + sal[jj]);
                 double gr = gij * r;
-                double ef = erf(gr);
+                double ef, eg;
+                erf_exp(gr, ef, eg);
                 double qq = qi * sq[jj];
                 ec += 0.5 * qq * ef / r;
-                dEdr += qq * (gij * exp(-gr * gr) * (2.0 / 1.772453850905516) / r - ef / r2);
+                dEdr += qq * (gij * eg * (2.0 / 1.772453850905516) / r - ef / r2);
             }
             // D4 dispersion
             if (r <= 50.0 && r >= 1e-10) {
@@ -1761,10 +1788,11 @@ __global__ __launch_bounds__(TILE) void k_pairs_part32(PairArgs a, int cj, doubl
                 // Coulomb (r <= 100)
                 const float gij = __builtin_amdgcn_rsqf(ali + sal[jj]);
                 const float gr = gij * r;
-                const float ef = erff(gr);
+                float ef, eg;
+                erf_exp(gr, ef, eg);
                 const float qq = qi * sq[jj];
                 ec += (double)(0.5f * qq * ef * rinv);
-                dEdr += qq * (gij * __expf(-gr * gr) * 1.1283791671f * rinv - ef * rinv * rinv);
+                dEdr += qq * (gij * eg * 1.1283791671f * rinv - ef * rinv * rinv);
             }
             if (r2d <= 2500.0 && r2d >= 1e-20) {
                 // D4 dispersion (r <= 50)
@@ -1854,8 +1882,6 @@ __device__ inline float ps_rcp(float v) { return __builtin_amdgcn_rcpf(v); }
 __device__ inline double ps_rcp(double v) { return 1.0 / v; }
 __device__ inline float ps_exp(float v) { return __expf(v); }
 __device__ inline double ps_exp(double v) { return exp(v); }
-__device__ inline float ps_erf(float v) { return erff(v); }
-__device__ inline double ps_erf(double v) { return erf(v); }
 
 template <typename T, int NE>
 __global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restrict__ part) {
@@ -1935,14 +1961,15 @@ __global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restri
         if (coul) {
             const T gij = ps_rsqrt(ali + cal[jj]);
             const T gr = gij * r;
-            const T ef = ps_erf(gr);
+            T ef, eg;
+            erf_exp(gr, ef, eg);
             const T qq = qi * cq[jj];
             if (sizeof(T) == 4) {
                 hec = T(0.5) * qq * ef * rinv;
-                dEdr += qq * (gij * ps_exp(-gr * gr) * T(1.1283791671) * rinv - ef * rinv * rinv);
+                dEdr += qq * (gij * eg * T(1.1283791671) * rinv - ef * rinv * rinv);
             } else {
                 hec = T(0.5) * qq * ef / r;
-                dEdr += qq * (gij * ps_exp(-gr * gr) * T(2.0 / 1.772453850905516) / r - ef / r2);
+                dEdr += qq * (gij * eg * T(2.0 / 1.772453850905516) / r - ef / r2);
             }
         }
         if (disp) {
@@ -2807,6 +2834,104 @@ __device__ double hb_core1(int A, int H, int B, const HBArgs& a, const double* _
     return E;
 }
 
+// Case-2 triplet (acceptor B with covalent neighbours, no lone-pair or
+// carbonyl factor): the cs = 2 path of hb_core in the arithmetic of
+// hb_core1 (T = float on the mixed path: FP64 coordinate differences, FP32
+// terms).  The quotients expo / (rAH + rHB - rAB) and ex_k / (s_k - rAB) are
+// taken in closed form, 49 / (radab rAB) and cnb / rAB.  xs[k]: gradient on
+// the k-th neighbour of B.
+template <typename T>
+__device__ double hb_core2(int A, int H, int B, const HBArgs& a, const double* __restrict__ x, double3& ga,
+                           double3& gh, double3& gb, double* __restrict__ xs) {
+    const double3 z3 = make_double3(0, 0, 0);
+    ga = z3;
+    gh = z3;
+    gb = z3;
+    const int nb0 = a.nb_off[B], nb1 = a.nb_off[B + 1];
+    const double3 pA = ld3(x, A), pH = ld3(x, H), pB = ld3(x, B);
+    const V3<T> drah = v3<T>(pA - pH), drbh = v3<T>(pB - pH), drab = v3<T>(pA - pB);
+    const T rah2 = drah.x * drah.x + drah.y * drah.y + drah.z * drah.z;
+    const T rbh2 = drbh.x * drbh.x + drbh.y * drbh.y + drbh.z * drbh.z;
+    const T rab2 = drab.x * drab.x + drab.y * drab.y + drab.z * drab.z;
+    const T rAH = hb_sqrt(rah2), rHB = hb_sqrt(rbh2), rAB = hb_sqrt(rab2);
+    double E = 0.0;
+    bool written = false;
+    if (rAB <= T(50)) {
+        const T QH = (T)a.Qh[H], QA = (T)a.Qab[A], QB = (T)a.Qab[B];
+        const T radab = (T)(a.hb_rad[A] + a.hb_rad[B]);
+        const T irab2 = hb_rcp(rab2), irAB = hb_rcp(rAB), irHB = hb_rcp(rHB), irbh2 = irHB * irHB;
+        const T t3 = T(22) * radab * irab2, t1 = rab2 * T(1.0 / 85.0);
+        const T ratio3 = (t3 * t3) * (t3 * t3) * (t3 * t3), ratio1 = (t1 * t1) * (t1 * t1) * (t1 * t1);
+        const T denv = hb_rcp(T(1) + ratio3) * hb_rcp(T(1) + ratio1);
+        const T irad = hb_rcp(radab), c49 = T(49) * irad;
+        const T expo0 = c49 * ((rAH + rHB) * irAB - T(1));
+        const T doutl = (expo0 > T(15)) ? T(0) : T(2) * hb_rcp(T(1) + hb_exp(expo0));
+        const T cnb = ((a.Z[B] == 7 && nb1 - nb0 == 1) ? T(2.0) : T(11.20)) * irad;
+        T outl_nb = T(1);
+        for (int k = nb0; k < nb1; ++k) {
+            const double3 pn = ld3(x, a.nb_idx[k]);
+            const V3<T> dan = v3<T>(pA - pn), dbn = v3<T>(pB - pn);
+            const T sm = hb_sqrt(dan.x * dan.x + dan.y * dan.y + dan.z * dan.z) +
+                         hb_sqrt(dbn.x * dbn.x + dbn.y * dbn.y + dbn.z * dbn.z);
+            const T ex = cnb * (sm * irAB - T(1));
+            outl_nb *= T(2) * hb_rcp(T(1) + hb_exp(-ex)) - T(1);
+        }
+        const T rbhdamp = denv * T(1.8) * irbh2 * irHB, rabdamp = denv * T(-0.8) * irab2 * irAB;
+        const T rdamp = rbhdamp + rabdamp;
+        const T cval = (T)a.ca[A] * (T)a.cb[B] * QA * QB * T(0.268);
+        const T qhoutl = QH * doutl * outl_nb;
+        E = (double)(-rdamp * qhoutl * cval);
+        const T rahprbh = rAH + rHB + T(1e-12);
+        const T ddamp = (T(-12) * ratio1 * hb_rcp(T(1) + ratio1)) + (T(12) * ratio3 * hb_rcp(T(1) + ratio3));
+        const T expo = c49 * (rahprbh * irAB - T(1));
+        if (!(expo > T(15))) {
+            written = true;
+            const T ratio2 = hb_exp(expo);
+            const T dterm = -qhoutl * cval, aterm = -rdamp * QH * outl_nb * cval;
+            const T nbterm = -rdamp * QH * doutl * cval;
+            const T ie2 = hb_rcp((T(1) + ratio2) * (T(1) + ratio2));
+            const T tmp1 = T(-2) * aterm * ratio2 * ie2 * c49 * irAB;
+            // coefficients of drab (A: +, B: -), drah (A: +, H: -), drbh (B: +, H: -)
+            T cab = ((rabdamp + rbhdamp) * ddamp - T(3) * rabdamp) * irab2 * dterm - tmp1 * rahprbh * irab2;
+            const T cah = tmp1 * hb_rcp(rAH);
+            const T cbh = T(-3) * rbhdamp * irbh2 * dterm + tmp1 * irHB;
+            T gax = T(0), gay = T(0), gaz = T(0), gbx = T(0), gby = T(0), gbz = T(0);
+            int kk = 0;
+            for (int k = nb0; k < nb1; ++k, ++kk) {
+                const double3 pn = ld3(x, a.nb_idx[k]);
+                const V3<T> dan = v3<T>(pA - pn), dbn = v3<T>(pB - pn);
+                const T ranb = hb_sqrt(dan.x * dan.x + dan.y * dan.y + dan.z * dan.z);
+                const T rbnb = hb_sqrt(dbn.x * dbn.x + dbn.y * dbn.y + dbn.z * dbn.z);
+                const T sm = ranb + rbnb + T(1e-12);
+                const T ex = cnb * (sm * irAB - T(1));
+                const T r2n = hb_exp(-ex), ir = hb_rcp(T(1) + r2n);
+                const T oi = T(2) * ir - T(1);
+                const T others = (fabs(oi) > T(1e-12)) ? outl_nb * hb_rcp(oi) : T(1);
+                const T tmp2 = T(2) * nbterm * others * r2n * ir * ir * cnb * irAB;
+                cab -= tmp2 * sm * irab2;
+                const T fa = tmp2 * hb_rcp(ranb), fb = tmp2 * hb_rcp(rbnb);
+                gax += fa * dan.x;
+                gay += fa * dan.y;
+                gaz += fa * dan.z;
+                gbx += fb * dbn.x;
+                gby += fb * dbn.y;
+                gbz += fb * dbn.z;
+                st3(xs, kk, make_double3(-(double)(fa * dan.x + fb * dbn.x), -(double)(fa * dan.y + fb * dbn.y),
+                                         -(double)(fa * dan.z + fb * dbn.z)));
+            }
+            ga = make_double3((double)(cah * drah.x + cab * drab.x + gax), (double)(cah * drah.y + cab * drab.y + gay),
+                              (double)(cah * drah.z + cab * drab.z + gaz));
+            gb = make_double3((double)(cbh * drbh.x - cab * drab.x + gbx), (double)(cbh * drbh.y - cab * drab.y + gby),
+                              (double)(cbh * drbh.z - cab * drab.z + gbz));
+            gh = make_double3(-(double)(cah * drah.x + cbh * drbh.x), -(double)(cah * drah.y + cbh * drbh.y),
+                              -(double)(cah * drah.z + cbh * drbh.z));
+        }
+    }
+    if (!written)
+        for (int kk = 0; kk < nb1 - nb0; ++kk) st3(xs, kk, z3);
+    return E;
+}
+
 // Case-1 hydrogen bonds by role: the case-1 triplets are kept three times,
 // grouped by the atom in role A, H or B (entries: the other two atoms), each
 // group cut into chunks of at most HB_CHUNK triplets.  One wavefront per
@@ -2861,7 +2986,10 @@ __global__ __launch_bounds__(256) void k_hb_role(HBRoles r, HBArgs a, const doub
 
 // Hydrogen bonds of cases 2-4, one thread each: energy, and the gradient in
 // the triplet's slots s0 (A), s0 + 1 (H), s0 + 2 (B), s0 + 3 + k (k-th
-// neighbour of B), then the carbonyl's D atoms.
+// neighbour of B), then the carbonyl's D atoms.  MODE 0: every entry by the
+// FP64 core; MODE 1: case-2 entries only, by hb_core2<T> (the mixed path's
+// FP32 terms); MODE 2: entries of cases 3 and 4 only, FP64.
+template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_hb_full(int nh, const int4* __restrict__ ahbc, const int* __restrict__ slot0,
                                                  HBArgs a, const double* __restrict__ x, double* __restrict__ slots,
                                                  double* __restrict__ e_out) {
@@ -2870,7 +2998,12 @@ __global__ __launch_bounds__(256) void k_hb_full(int nh, const int4* __restrict_
     const int4 t = ahbc[h];
     const int s0 = slot0[h];
     double3 ga, gh, gb;
-    e_out[h] = hb_core<true>(t.x, t.y, t.z, t.w, a, x, ga, gh, gb, slots + 3 * ((size_t)s0 + 3));
+    double* xs = slots + 3 * ((size_t)s0 + 3);
+    if ((MODE == 1 && t.w != 2) || (MODE == 2 && t.w == 2)) return;
+    if (MODE == 1)
+        e_out[h] = hb_core2<T>(t.x, t.y, t.z, a, x, ga, gh, gb, xs);
+    else
+        e_out[h] = hb_core<true>(t.x, t.y, t.z, t.w, a, x, ga, gh, gb, xs);
     st3(slots, s0, ga);
     st3(slots, s0 + 1, gh);
     st3(slots, s0 + 2, gb);
@@ -3659,7 +3792,7 @@ struct DeviceEngine {
     int *hb_goff, *hb_gatom, *hb_crange, *hb_s0, *hb_xoff, *hb_xidx;
     int2* hb_ent;
     int4* hb_ahbc;
-    int hb_ng0, hb_ng1, hb_ngt, nhb2;
+    int hb_ng0, hb_ng1, hb_ngt, nhb2, nhb2_c2;  // nhb2_c2: case-2 entries of the cases 2-4 list
     double *hb_cb, *hb_ca, *hb_Qh, *hb_Qab, *hb_gout, *hb_xslots, *e_hb2;
     int *atm_ijk, *atm_slot;
     double* atm_sc;
@@ -4116,6 +4249,8 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
                     throw std::runtime_error("hydrogen-bond slot count exceeds the index range");
             }
             E->nhb2 = (int)ahbc.size();
+            E->nhb2_c2 = 0;
+            for (const int4& t : ahbc) E->nhb2_c2 += (t.w == 2);
             std::vector<int> xoff(n + 1, 0), xidx(xatom.size());
             for (int at : xatom) xoff[at + 1]++;
             for (int i = 0; i < n; ++i) xoff[i + 1] += xoff[i];
@@ -4516,25 +4651,36 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         HBArgs ha{E->hb_cb, E->hb_ca, E->hb_Qh, E->hb_Qab, E->hb_nboff, E->hb_nbidx, E->hb_parc,
                   E->hb_nbcoff, E->hb_nbcidx, E->hb_rad, E->repz, E->Z};
         HBRoles hr{E->hb_ent, E->hb_goff, E->hb_gatom, E->hb_ng0, E->hb_ng1, E->hb_ngt};
-        // case-1 role kernels (FP32 terms on the mixed path, FP64 otherwise)
-        // on the side stream after the EEQ matrix build, to balance the two
-        // streams; the gather follows the join
+        // case-1 role kernels (FP32 terms on the mixed path, FP64 otherwise):
+        // roles A and H on this stream, role B on the side stream after the
+        // EEQ matrix build, which balances the two streams before the join;
+        // the gather follows the join
         const int gr0[3] = {0, E->hb_ng0, E->hb_ng1}, gr1[3] = {E->hb_ng0, E->hb_ng1, E->hb_ngt};
         for (int role = 0; role < 3; ++role) {
             const dim3 grid(cdiv(gr1[role] - gr0[role], 4));
             if (gr1[role] <= gr0[role]) continue;
+            hipStream_t rs = (role == 2) ? E->side : s;
             if (E->mixed) {
-                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, float>), grid, dim3(256), E->side, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, float>), grid, dim3(256), E->side, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, float>), grid, dim3(256), E->side, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, float>), grid, dim3(256), rs, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, float>), grid, dim3(256), rs, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, float>), grid, dim3(256), rs, hr, ha, E->x, E->hb_gout, E->e_hb);
             } else {
-                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, double>), grid, dim3(256), E->side, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, double>), grid, dim3(256), E->side, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, double>), grid, dim3(256), E->side, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, double>), grid, dim3(256), rs, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, double>), grid, dim3(256), rs, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, double>), grid, dim3(256), rs, hr, ha, E->x, E->hb_gout, E->e_hb);
             }
         }
-        if (E->nhb2)
-            LAUNCH(KC_HB, k_hb_full, dim3(cdiv(E->nhb2, 256)), dim3(256), s, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
+        // cases 2-4: on the mixed path the case-2 entries by the lean FP32
+        // kernel and those of cases 3 and 4 by the FP64 one
+        const dim3 g2(cdiv(E->nhb2, 256));
+        if (E->nhb2 && !E->mixed)
+            LAUNCH(KC_HB, (k_hb_full<double, 0>), g2, dim3(256), s, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
+                   E->hb_xslots, E->e_hb2);
+        if (E->mixed && E->nhb2_c2)
+            LAUNCH(KC_HB, (k_hb_full<float, 1>), g2, dim3(256), s, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
+                   E->hb_xslots, E->e_hb2);
+        if (E->mixed && E->nhb2 > E->nhb2_c2)
+            LAUNCH(KC_HB, (k_hb_full<double, 2>), g2, dim3(256), s, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
                    E->hb_xslots, E->e_hb2);
     }
     if (E->natm)
