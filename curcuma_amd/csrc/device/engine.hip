@@ -4651,23 +4651,20 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         HBArgs ha{E->hb_cb, E->hb_ca, E->hb_Qh, E->hb_Qab, E->hb_nboff, E->hb_nbidx, E->hb_parc,
                   E->hb_nbcoff, E->hb_nbcidx, E->hb_rad, E->repz, E->Z};
         HBRoles hr{E->hb_ent, E->hb_goff, E->hb_gatom, E->hb_ng0, E->hb_ng1, E->hb_ngt};
-        // case-1 role kernels (FP32 terms on the mixed path, FP64 otherwise):
-        // roles A and H on this stream, role B on the side stream after the
-        // EEQ matrix build, which balances the two streams before the join;
-        // the gather follows the join
+        // case-1 role kernels (FP32 terms on the mixed path, FP64 otherwise);
+        // the side stream meanwhile builds the EEQ matrix
         const int gr0[3] = {0, E->hb_ng0, E->hb_ng1}, gr1[3] = {E->hb_ng0, E->hb_ng1, E->hb_ngt};
         for (int role = 0; role < 3; ++role) {
             const dim3 grid(cdiv(gr1[role] - gr0[role], 4));
             if (gr1[role] <= gr0[role]) continue;
-            hipStream_t rs = (role == 2) ? E->side : s;
             if (E->mixed) {
-                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, float>), grid, dim3(256), rs, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, float>), grid, dim3(256), rs, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, float>), grid, dim3(256), rs, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
             } else {
-                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, double>), grid, dim3(256), rs, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, double>), grid, dim3(256), rs, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, double>), grid, dim3(256), rs, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
             }
         }
         // cases 2-4: on the mixed path the case-2 entries by the lean FP32
@@ -4690,10 +4687,10 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     if (E->nbatm)
         LAUNCH(KC_BATM, k_batm, dim3(cdiv(E->nbatm, 256)), dim3(256), s, E->nbatm, E->batm_ijk, E->batm_zb,
                            E->batm_slot, E->x, E->qa, E->slots, E->e_batm);
-    HIP_OK(hipEventRecord(E->ev_join, E->side));
-    HIP_OK(hipStreamWaitEvent(s, E->ev_join, 0));
     LAUNCH(KC_HB, k_hb_gather, dim3(cdiv(n, 4)), dim3(256), s, n, E->hb_crange, E->hb_gout, E->hb_xoff, E->hb_xidx,
            E->hb_xslots, E->ghb);
+    HIP_OK(hipEventRecord(E->ev_join, E->side));
+    HIP_OK(hipStreamWaitEvent(s, E->ev_join, 0));
     if (E->nmol > 1) {
         // batch: every molecule's system factored and solved by one workgroup (FP64)
         LAUNCH(KC_CHOL_DIAG, k_eeq_batch_solve, dim3(E->nmol), dim3(256), s, E->morder, E->mnp, E->mmo, E->mvo,
