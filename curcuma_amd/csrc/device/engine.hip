@@ -2496,6 +2496,8 @@ struct HBArgs {
     const double* hb_rad;
     const double* repz;
     const int* Z;
+    const float4* pk;      // FP32 per atom {basicity, acidity, Qab, hb_rad} (roles A, B of the FP32 cores)
+    const float* qh32;     // FP32 Qh (role H of the FP32 cores)
 };
 __device__ inline double pow6(double t) {
     const double t2 = t * t;
@@ -2773,6 +2775,32 @@ template <typename T>
 __device__ inline double3 d3(T s, V3<T> v) {
     return make_double3((double)(s * v.x), (double)(s * v.y), (double)(s * v.z));
 }
+// per-atom HB parameters of a triplet: FP64 arrays for the FP64 cores, the
+// packed FP32 copies (two 16-byte loads and one 4-byte load) for T = float
+template <typename T>
+__device__ inline void hb_params(int A, int H, int B, const HBArgs& a, T& basA, T& basB, T& aciA, T& aciB, T& QH,
+                                 T& QA, T& QB, T& radab) {
+    if constexpr (sizeof(T) == sizeof(float)) {
+        const float4 pa = a.pk[A], pb = a.pk[B];
+        basA = pa.x;
+        aciA = pa.y;
+        QA = pa.z;
+        basB = pb.x;
+        aciB = pb.y;
+        QB = pb.z;
+        radab = pa.w + pb.w;
+        QH = a.qh32[H];
+    } else {
+        basA = a.cb[A];
+        basB = a.cb[B];
+        aciA = a.ca[A];
+        aciB = a.ca[B];
+        QH = a.Qh[H];
+        QA = a.Qab[A];
+        QB = a.Qab[B];
+        radab = a.hb_rad[A] + a.hb_rad[B];
+    }
+}
 template <typename T>
 __device__ double hb_core1(int A, int H, int B, const HBArgs& a, const double* __restrict__ x, double3& ga,
                            double3& gh, double3& gb) {
@@ -2788,13 +2816,12 @@ __device__ double hb_core1(int A, int H, int B, const HBArgs& a, const double* _
     const T rab2 = drab.x * drab.x + drab.y * drab.y + drab.z * drab.z;
     const T rAH = hb_sqrt(rah2), rHB = hb_sqrt(rbh2), rAB = hb_sqrt(rab2);
     if (!(rAB <= T(50))) return 0.0;
-    const T basA = (T)a.cb[A], basB = (T)a.cb[B], aciA = (T)a.ca[A], aciB = (T)a.ca[B];
-    const T QH = (T)a.Qh[H], QA = (T)a.Qab[A], QB = (T)a.Qab[B];
+    T basA, basB, aciA, aciB, QH, QA, QB, radab;
+    hb_params<T>(A, H, B, a, basA, basB, aciA, aciB, QH, QA, QB, radab);
     const T rAH4 = rah2 * rah2, rHB4 = rbh2 * rbh2;
     const T den = hb_rcp(rAH4 + rHB4);
     const T bas = (QA * basA * rAH4 + QB * basB * rHB4) * den;
     const T aci = (aciB * rAH4 + aciA * rHB4) * den;
-    const T radab = (T)(a.hb_rad[A] + a.hb_rad[B]);
     const T irab2 = hb_rcp(rab2), irAB = hb_rcp(rAB);
     const T t3 = T(22) * radab * irab2, t1 = rab2 * T(1.0 / 85.0);
     const T ratio3 = (t3 * t3) * (t3 * t3) * (t3 * t3), ratio1 = (t1 * t1) * (t1 * t1) * (t1 * t1);
@@ -2857,8 +2884,8 @@ __device__ double hb_core2(int A, int H, int B, const HBArgs& a, const double* _
     double E = 0.0;
     bool written = false;
     if (rAB <= T(50)) {
-        const T QH = (T)a.Qh[H], QA = (T)a.Qab[A], QB = (T)a.Qab[B];
-        const T radab = (T)(a.hb_rad[A] + a.hb_rad[B]);
+        T basA, basB, aciA, aciB, QH, QA, QB, radab;
+        hb_params<T>(A, H, B, a, basA, basB, aciA, aciB, QH, QA, QB, radab);
         const T irab2 = hb_rcp(rab2), irAB = hb_rcp(rAB), irHB = hb_rcp(rHB), irbh2 = irHB * irHB;
         const T t3 = T(22) * radab * irab2, t1 = rab2 * T(1.0 / 85.0);
         const T ratio3 = (t3 * t3) * (t3 * t3) * (t3 * t3), ratio1 = (t1 * t1) * (t1 * t1) * (t1 * t1);
@@ -2878,7 +2905,7 @@ __device__ double hb_core2(int A, int H, int B, const HBArgs& a, const double* _
         }
         const T rbhdamp = denv * T(1.8) * irbh2 * irHB, rabdamp = denv * T(-0.8) * irab2 * irAB;
         const T rdamp = rbhdamp + rabdamp;
-        const T cval = (T)a.ca[A] * (T)a.cb[B] * QA * QB * T(0.268);
+        const T cval = aciA * basB * QA * QB * T(0.268);
         const T qhoutl = QH * doutl * outl_nb;
         E = (double)(-rdamp * qhoutl * cval);
         const T rahprbh = rAH + rHB + T(1e-12);
@@ -3794,6 +3821,8 @@ struct DeviceEngine {
     int4* hb_ahbc;
     int hb_ng0, hb_ng1, hb_ngt, nhb2, nhb2_c2;  // nhb2_c2: case-2 entries of the cases 2-4 list
     double *hb_cb, *hb_ca, *hb_Qh, *hb_Qab, *hb_gout, *hb_xslots, *e_hb2;
+    float4* hb_pk;
+    float* hb_qh32;
     int *atm_ijk, *atm_slot;
     double* atm_sc;
     int *batm_ijk, *batm_slot;
@@ -4262,6 +4291,16 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
             E->hb_ca = E->upload(ca);
             E->hb_Qh = E->upload(Qh);
             E->hb_Qab = E->upload(Qab);
+            {
+                std::vector<float4> pk(n);
+                std::vector<float> qh32(n);
+                for (int i = 0; i < n; ++i) {
+                    pk[i] = make_float4((float)cb[i], (float)ca[i], (float)Qab[i], (float)P.hb_rad[i]);
+                    qh32[i] = (float)Qh[i];
+                }
+                E->hb_pk = E->upload(pk);
+                E->hb_qh32 = E->upload(qh32);
+            }
             E->hb_nboff = E->upload(nboff);
             E->hb_nbidx = E->upload(nbidx);
             E->hb_parc = E->upload(bC);
@@ -4649,7 +4688,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
                            E->brep_slot, E->x, E->slots, E->e_brep);
     if (E->nhb) {
         HBArgs ha{E->hb_cb, E->hb_ca, E->hb_Qh, E->hb_Qab, E->hb_nboff, E->hb_nbidx, E->hb_parc,
-                  E->hb_nbcoff, E->hb_nbcidx, E->hb_rad, E->repz, E->Z};
+                  E->hb_nbcoff, E->hb_nbcidx, E->hb_rad, E->repz, E->Z, E->hb_pk, E->hb_qh32};
         HBRoles hr{E->hb_ent, E->hb_goff, E->hb_gatom, E->hb_ng0, E->hb_ng1, E->hb_ngt};
         // case-1 role kernels (FP32 terms on the mixed path, FP64 otherwise);
         // the side stream meanwhile builds the EEQ matrix
