@@ -10,7 +10,7 @@
 //   k_pairs_part    Coulomb + D4 dispersion + non-bonded repulsion, LDS-tiled,
 //                   one row atom per thread, full rows split in j chunks
 //   k_pairs_reduce  fixed-order sum of the chunk partials (no atomics)
-//   k_nbrep_fix     bonded / 1,3 / 1,4 H-H corrections of the repulsion pair sum
+//   k_atom_tail     bonded / 1,3 / 1,4 H-H corrections of the repulsion pair sum, dE/dCN totals
 //   k_bonds .. k_batm   bonded term lists writing per-atom gradient slots
 //   k_finish        slot gather + CN chain rule + gradient assembly
 //   k_reduce_terms  fixed-order energy reductions
@@ -1499,7 +1499,7 @@ __global__ __launch_bounds__(256) void k_charges(int n, const int* __restrict__ 
 //   Coulomb  (r <= 100)  ff_workspace_gfnff.cpp:850-898
 //   D4       (r <= 50)   :715-767 with C6 from the current Gaussian weights
 //   repulsion (r <= 20)  :814-842 with the element-pair factor; bonded and
-//                        1,3/1,4 H-H pairs are corrected by k_nbrep_fix
+//                        1,3/1,4 H-H pairs are corrected by k_atom_tail
 // Each pair is evaluated from both sides; a row keeps half of each pair
 // energy and its own full gradient and dE/dCN, so no atomics are needed.
 // ---------------------------------------------------------------------------
@@ -1695,7 +1695,7 @@ __global__ __launch_bounds__(TILE) void k_pairs_part(PairArgs a, int cj, double*
                 dcn -= dci * sum * zc;
             }
             // non-bonded repulsion (generic pair factor; bonded / 1,3 / 1,4
-            // corrections follow in k_nbrep_fix)
+            // corrections follow in k_atom_tail)
             if (r <= 20.0 && r >= 1e-8) {
                 double ff = rep_ff(zi, sZ[jj], mi, smt[jj]);
                 double al = sqrt(ani * san[jj]) * ff;
@@ -2123,13 +2123,11 @@ __global__ __launch_bounds__(256) void k_pairs_reduce(int n, int nchunk, const d
 // K7: repulsion corrections over the topological neighbourhood
 // (gfnff_method.cpp:9781-9872: bonded pairs excluded, H-H 1,3 / 1,4 factors)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_nbrep_fix(int n, const double* __restrict__ x, const int* __restrict__ Z,
-                                                   const int* __restrict__ metal, const double* __restrict__ anb,
-                                                   const double* __restrict__ repz, const int* __restrict__ toff,
-                                                   const int* __restrict__ tidx, const int* __restrict__ tdist,
-                                                   double* __restrict__ g, double* __restrict__ e_rep) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ inline void nbrep_fix_atom(int i, const double* __restrict__ x, const int* __restrict__ Z,
+                                      const int* __restrict__ metal, const double* __restrict__ anb,
+                                      const double* __restrict__ repz, const int* __restrict__ toff,
+                                      const int* __restrict__ tidx, const int* __restrict__ tdist,
+                                      double* __restrict__ g, double* __restrict__ e_rep) {
     double3 xi = ld3(x, i);
     double gx = 0, gy = 0, gz = 0, er = 0;
     for (int k = toff[i]; k < toff[i + 1]; ++k) {
@@ -3469,7 +3467,7 @@ __global__ __launch_bounds__(TILE) void k_adj_pairs(PairArgs a, const double* __
 }
 
 // Topological corrections of the non-bonded repulsion (bonded pairs removed,
-// H-H 1,3 / 1,4 factors), mirroring k_nbrep_fix; row-wise, appended as one
+// H-H 1,3 / 1,4 factors), mirroring nbrep_fix_atom; row-wise, appended as one
 // more chunk of partials.
 template <int NE>
 __global__ __launch_bounds__(256) void k_adj_nbrep_fix(int n, const double* __restrict__ x,
@@ -3577,15 +3575,35 @@ __global__ __launch_bounds__(64) void k_adj_reduce(int n, int nchunk, const doub
 // K9: gradient assembly: slot gather, bond dE/dCN gather, CN chain rule with
 // Coulomb term 1b (ff_workspace.cpp:459-507), fixed order per atom.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_dcn_total(int n, const int* __restrict__ doff, const int* __restrict__ didx,
-                                                   const double* __restrict__ dcn_slot, double* __restrict__ dEdcn,
-                                                   const double* __restrict__ qtmp, double* __restrict__ v) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ inline void dcn_total_atom(int i, const int* __restrict__ doff, const int* __restrict__ didx,
+                                      const double* __restrict__ dcn_slot, double* __restrict__ dEdcn,
+                                      const double* __restrict__ qtmp, double* __restrict__ v) {
     double s = dEdcn[i];
     for (int k = doff[i]; k < doff[i + 1]; ++k) s += dcn_slot[didx[k]];
     dEdcn[i] = s;
     v[i] = s - qtmp[i];
+}
+
+// Per-atom tail after the pair terms, one thread per atom: the repulsion
+// corrections of the atom's topological neighbourhood (its gradient and
+// e_rep) and its dE/dCN total with the chain vector, which touch disjoint data.
+struct AtomTailArgs {
+    const double* x;
+    const int *Z, *metal;
+    const double *anb, *repz;
+    const int *toff, *tidx, *tdist;
+    double *g, *e_rep;
+    const int *doff, *didx;
+    const double* dcn_slot;
+    double* dEdcn;
+    const double* qtmp;
+    double* v;
+};
+__global__ __launch_bounds__(256) void k_atom_tail(int n, AtomTailArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    nbrep_fix_atom(i, a.x, a.Z, a.metal, a.anb, a.repz, a.toff, a.tidx, a.tdist, a.g, a.e_rep);
+    dcn_total_atom(i, a.doff, a.didx, a.dcn_slot, a.dEdcn, a.qtmp, a.v);
 }
 
 // one wavefront per atom: lanes stride the atom's slot list and its CN
@@ -3658,34 +3676,38 @@ __global__ __launch_bounds__(256) void k_reduce1(Segs s, double* __restrict__ pa
     }
     if (threadIdx.x == 0) part[sg * RED_BLOCKS + b] = red[0];
 }
-// stage 2: one wavefront per segment, fixed tree over the block partials
-__global__ __launch_bounds__(64) void k_reduce2(Segs s, const double* __restrict__ part, double* __restrict__ out) {
-    int sg = blockIdx.x;
-    double v = part[sg * RED_BLOCKS + threadIdx.x];
+// stage 2 and the term table, one workgroup: wavefront sg adds segment sg's
+// block partials with a fixed tree; thread 0 then forms out[0] = total and
+// out[1..15] = terms (term index + 1) in a fixed summation order
+__global__ __launch_bounds__(NPARTS * 64) void k_reduce_final(Segs s, const double* __restrict__ part,
+                                                               double* __restrict__ parts, double* __restrict__ out) {
+    __shared__ double sp[NPARTS];
+    const int sg = threadIdx.x >> 6, l = threadIdx.x & 63;
+    double v = part[sg * RED_BLOCKS + l];
     for (int w = 32; w > 0; w >>= 1) v += __shfl_xor(v, w, 64);
-    if (threadIdx.x == 0) out[s.dst[sg]] = v;
-}
-
-// out[0] = total, out[1..15] = terms (term index + 1), fixed summation order
-__global__ void k_total(const double* __restrict__ parts, double* __restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (l == 0) {
+        sp[s.dst[sg]] = v;
+        parts[s.dst[sg]] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     double t[GFNFF_NTERMS];
     for (int k = 0; k < GFNFF_NTERMS; ++k) t[k] = 0.0;
     // parts: 0 bond,1 angle,2 tors,3 inv,4 disp(pairs),5 brep,6 nbrep(pairs incl fix),7 coul pairs,
     // 8 self,9 hb case 1,10 atm,11 batm,12 triple-bond torsions,13 xb,14 hb cases 2-4
-    t[GFNFF_E_BOND] = parts[0];
-    t[GFNFF_E_ANGLE] = parts[1];
-    t[GFNFF_E_TORSION] = parts[2];
-    t[GFNFF_E_INVERSION] = parts[3];
-    t[GFNFF_E_DISPERSION] = parts[4];
-    t[GFNFF_E_BONDED_REP] = parts[5];
-    t[GFNFF_E_NONBONDED_REP] = parts[6];
-    t[GFNFF_E_COULOMB] = parts[7] + parts[8];
-    t[GFNFF_E_HBOND] = parts[9] + parts[14];
-    t[GFNFF_E_ATM] = parts[10];
-    t[GFNFF_E_BATM] = parts[11];
-    t[GFNFF_E_STORS] = parts[12];
-    t[GFNFF_E_XBOND] = parts[13];
+    t[GFNFF_E_BOND] = sp[0];
+    t[GFNFF_E_ANGLE] = sp[1];
+    t[GFNFF_E_TORSION] = sp[2];
+    t[GFNFF_E_INVERSION] = sp[3];
+    t[GFNFF_E_DISPERSION] = sp[4];
+    t[GFNFF_E_BONDED_REP] = sp[5];
+    t[GFNFF_E_NONBONDED_REP] = sp[6];
+    t[GFNFF_E_COULOMB] = sp[7] + sp[8];
+    t[GFNFF_E_HBOND] = sp[9] + sp[14];
+    t[GFNFF_E_ATM] = sp[10];
+    t[GFNFF_E_BATM] = sp[11];
+    t[GFNFF_E_STORS] = sp[12];
+    t[GFNFF_E_XBOND] = sp[13];
     double tot = 0.0;
     for (int k = 0; k < GFNFF_NTERMS; ++k) tot += t[k];
     out[0] = tot;
@@ -4807,10 +4829,12 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         LAUNCH(KC_PAIRS_AUX, k_pairs_reduce, dim3(cdiv(n, 256)), dim3(256), s, n, E->pair_nchunk, E->ppart, E->gpair,
                E->dEdcn, E->e_coul, E->e_disp, E->e_rep);
     }
-    LAUNCH(KC_NBREP, k_nbrep_fix, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->Z, E->metal, E->anb, E->repz,
-                       E->toff, E->tidx, E->tdist, E->gpair, E->e_rep);
-    LAUNCH(KC_CHAIN, k_dcn_total, dim3(cdiv(n, 256)), dim3(256), s, n, E->dcn_off, E->dcn_idx, E->dcn_slot,
-                       E->dEdcn, E->qtmp, E->vchain);
+    {
+        const AtomTailArgs ta{E->x,    E->Z,       E->metal,   E->anb,      E->repz,  E->toff,
+                              E->tidx, E->tdist,   E->gpair,   E->e_rep,    E->dcn_off, E->dcn_idx,
+                              E->dcn_slot, E->dEdcn, E->qtmp, E->vchain};
+        LAUNCH(KC_CHAIN, k_atom_tail, dim3(cdiv(n, 256)), dim3(256), s, n, ta);
+    }
     LAUNCH(KC_CHAIN, k_finish, dim3(cdiv(n, 4)), dim3(256), s, n, E->x, E->rcov_cn, E->dlog, E->vchain,
                        E->nbn, E->nbl, E->slot_off, E->slot_idx, E->slots, E->gpair, E->ghb, E->grad);
     Segs sg;
@@ -4824,8 +4848,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         sg.dst[k] = k;
     }
     LAUNCH(KC_REDUCE, k_reduce1, dim3(RED_BLOCKS, NPARTS), dim3(256), s, sg, E->rpart);
-    LAUNCH(KC_REDUCE, k_reduce2, dim3(NPARTS), dim3(64), s, sg, E->rpart, E->parts);
-    LAUNCH(KC_REDUCE, k_total, dim3(1), dim3(64), s, E->parts, E->out);
+    LAUNCH(KC_REDUCE, k_reduce_final, dim3(1), dim3(NPARTS * 64), s, sg, E->rpart, E->parts, E->out);
     if (E->nmol > 1) {
         MolEnergyArgs ma;
         ma.nmol = E->nmol;
