@@ -45,6 +45,14 @@ constexpr int MAXNB_CN = 128;   // CN neighbour list capacity per atom
 constexpr int TILE = 256;       // pair kernel tile / workgroup size
 
 constexpr double KN = -7.5;
+// CN count 0.5 (1 + erf(v)); for v <= -6, erf(v) rounds to -1 and the count
+// is 0 (the branch skips erf for the many list members beyond the covalent
+// distance)
+__device__ inline double cn_count(double v) {
+    double c = 0.0;
+    if (v > -6.0) c = 0.5 * (1.0 + erf(v));
+    return c;
+}
 constexpr double CNMAX = 4.4;
 constexpr double SQRTPI_TRUNC = 1.77245385091;   // constant used by the reference's dCN
 constexpr double TSQRT2PI = 0.797884560802866;
@@ -174,7 +182,7 @@ __global__ __launch_bounds__(256) void k_cn(int n, const double* __restrict__ x,
                     hit = true;
                     double r = sqrt(d2);
                     double rij = ri + sr[jj];
-                    contrib = 0.5 * (1.0 + erf(KN * (r - rij) / rij));
+                    contrib = cn_count(KN * (r - rij) / rij);
                 }
             }
             const unsigned long long mask = __ballot(hit);
@@ -275,7 +283,7 @@ __global__ __launch_bounds__(256) void k_cn_mol(const int* __restrict__ mol_off,
                     hit = true;
                     const double r = sqrt(d2);
                     const double rij = ri + sr[jj];
-                    contrib = 0.5 * (1.0 + erf(KN * (r - rij) / rij));
+                    contrib = cn_count(KN * (r - rij) / rij);
                 }
             }
             const unsigned long long mask = __ballot(hit);
@@ -2174,19 +2182,21 @@ __device__ inline void nbrep_fix_atom(int i, const double* __restrict__ x, const
 // (gathered per atom in k_finish in a fixed order) and its energy to e_out.
 // ---------------------------------------------------------------------------
 // HB coordination of X-H hydrogens (ff_workspace_gfnff.cpp:46-97) and bonds (:104-180)
-// one wavefront per bond: lanes stride the bond's hydrogen-bond partner list
-// (HB coordination and its gradient), fixed butterflies; lane 0 writes the
-// bond's own slots
+// bonds with an X-H hydrogen (bsel: their indices): one wavefront per bond,
+// lanes stride the bond's hydrogen-bond partner list (HB coordination and
+// its gradient), fixed butterflies; lane 0 writes the bond's own slots
 
-__global__ __launch_bounds__(256) void k_bonds(int nb, const int* __restrict__ ij, const double* __restrict__ par,
+__global__ __launch_bounds__(256) void k_bonds(int nsel, const int* __restrict__ bsel, const int* __restrict__ ij,
+                                               const double* __restrict__ par,
                                                const int* __restrict__ hb_off, const int* __restrict__ hb_B,
                                                const int* __restrict__ slot0, const double* __restrict__ x,
                                                const double* __restrict__ cn, const double* __restrict__ rcov_d3,
                                                const int* __restrict__ bond_H, double* __restrict__ slots,
                                                double* __restrict__ dcn_slot, double* __restrict__ e_out) {
     const int lane = threadIdx.x & 63;
-    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (b >= nb) return;
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= nsel) return;
+    const int b = bsel[w];
     int i = ij[2 * b], j = ij[2 * b + 1];
     const double* p = par + 8 * b;
     double3 d = ld3(x, i) - ld3(x, j);
@@ -2246,6 +2256,34 @@ __global__ __launch_bounds__(256) void k_bonds(int nb, const int* __restrict__ i
         gH.z = wave_sum(gH.z);
         if (lane == 0) st3(slots, s + 2, gH);
     }
+}
+
+// bonds without an X-H hydrogen (no HB coordination; bsel: their indices),
+// one thread each: the arithmetic of k_bonds' lane 0
+__global__ __launch_bounds__(256) void k_bonds_plain(int nsel, const int* __restrict__ bsel,
+                                                     const int* __restrict__ ij, const double* __restrict__ par,
+                                                     const int* __restrict__ slot0, const double* __restrict__ x,
+                                                     const double* __restrict__ cn, double* __restrict__ slots,
+                                                     double* __restrict__ dcn_slot, double* __restrict__ e_out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nsel) return;
+    const int b = bsel[t];
+    const int i = ij[2 * b], j = ij[2 * b + 1];
+    const double* p = par + 8 * b;
+    const double3 d = ld3(x, i) - ld3(x, j);
+    const double r = nrm(d);
+    const double r0 = (p[0] + p[2] * cn[i] + p[1] + p[3] * cn[j] + p[4]) * p[5];
+    const double dr = r - r0;
+    const double al = p[6];
+    const double e = p[7] * exp(-al * dr * dr);
+    const double dEdr = -2.0 * al * dr * e;
+    const int s = slot0[b];
+    e_out[b] = e;
+    st3(slots, s, (dEdr / r) * d);
+    st3(slots, s + 1, (-dEdr / r) * d);
+    const double yy = -dEdr;
+    dcn_slot[2 * b] = yy * p[5] * p[2];
+    dcn_slot[2 * b + 1] = yy * p[5] * p[3];
 }
 
 // angles: ff_workspace_gfnff.cpp:187-275
@@ -3038,22 +3076,23 @@ __global__ __launch_bounds__(256) void k_hb_full(int nh, const int4* __restrict_
 // case-1 chunk results in roles A, H, B (chunks [crange[r (n + 1) + i],
 // crange[r (n + 1) + i + 1])) and its slots of cases 2-4; lanes stride the
 // lists, fixed butterfly
+template <int LPA>
 __global__ __launch_bounds__(256) void k_hb_gather(int n, const int* __restrict__ crange,
                                                    const double* __restrict__ gout, const int* __restrict__ xoff,
                                                    const int* __restrict__ xidx, const double* __restrict__ xslots,
                                                    double* __restrict__ g) {
-    const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & (LPA - 1);
+    const int i = blockIdx.x * (256 / LPA) + threadIdx.x / LPA;
     if (i >= n) return;
     double3 s = make_double3(0, 0, 0);
 #pragma unroll
     for (int role = 0; role < 3; ++role) {
         const int* cr = crange + role * (n + 1);
-        for (int k = cr[i] + lane; k < cr[i + 1]; k += 64) s = s + ld3(gout, k);
+        for (int k = cr[i] + lane; k < cr[i + 1]; k += LPA) s = s + ld3(gout, k);
     }
-    for (int k = xoff[i] + lane; k < xoff[i + 1]; k += 64) s = s + ld3(xslots, xidx[k]);
+    for (int k = xoff[i] + lane; k < xoff[i + 1]; k += LPA) s = s + ld3(xslots, xidx[k]);
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
+    for (int o = LPA / 2; o > 0; o >>= 1) {
         s.x += __shfl_xor(s.x, o);
         s.y += __shfl_xor(s.y, o);
         s.z += __shfl_xor(s.z, o);
@@ -3606,23 +3645,31 @@ __global__ __launch_bounds__(256) void k_atom_tail(int n, AtomTailArgs a) {
     dcn_total_atom(i, a.doff, a.didx, a.dcn_slot, a.dEdcn, a.qtmp, a.v);
 }
 
-// one wavefront per atom: lanes stride the atom's slot list and its CN
-// neighbour list (chain rule), fixed butterfly
+// LPA lanes per atom (64: one wavefront; 16 for short lists, four atoms per
+// wavefront): lanes stride the atom's slot list and its CN neighbour list
+// (chain rule), fixed butterfly over the atom's lanes
+template <int W>
+__device__ inline double group_sum(double v) {
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+template <int LPA>
 __global__ __launch_bounds__(256) void k_finish(int n, const double* __restrict__ x, const double* __restrict__ rcov,
                                                 const double* __restrict__ dlog, const double* __restrict__ v,
                                                 const int* __restrict__ nbn, const int* __restrict__ nbl,
                                                 const int* __restrict__ soff, const int* __restrict__ sidx,
                                                 const double* __restrict__ slots, const double* __restrict__ gpair,
                                                 const double* __restrict__ ghb, double* __restrict__ grad) {
-    const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & (LPA - 1);
+    const int i = blockIdx.x * (256 / LPA) + threadIdx.x / LPA;
     if (i >= n) return;
     double3 g = make_double3(0, 0, 0);
-    for (int k = soff[i] + lane; k < soff[i + 1]; k += 64) g = g + ld3(slots, sidx[k]);
+    for (int k = soff[i] + lane; k < soff[i + 1]; k += LPA) g = g + ld3(slots, sidx[k]);
     const double3 xi = ld3(x, i);
     const double ai = dlog[i] * v[i];
     const int nn = nbn[i];
-    for (int k = lane; k < nn; k += 64) {
+    for (int k = lane; k < nn; k += LPA) {
         int j = nbl[(size_t)i * MAXNB_CN + k];
         double3 d = ld3(x, j) - xi;
         double r = nrm(d);
@@ -3632,9 +3679,9 @@ __global__ __launch_bounds__(256) void k_finish(int n, const double* __restrict_
         double s = dd * (ai + dlog[j] * v[j]);
         g = g + (-s / r) * d;
     }
-    g.x = wave_sum(g.x);
-    g.y = wave_sum(g.y);
-    g.z = wave_sum(g.z);
+    g.x = group_sum<LPA>(g.x);
+    g.y = group_sum<LPA>(g.y);
+    g.z = group_sum<LPA>(g.z);
     if (lane == 0) {
         g = g + ld3(gpair, i) + ld3(ghb, i);
         st3(grad, i, g);
@@ -3826,6 +3873,8 @@ struct DeviceEngine {
     int *red_lidx, *red_ul, *red_nul;  // distinct last atoms per 64-block of reduced variables
     double* qp;
     int *bond_ij, *bond_H, *bond_hboff, *bond_hbB, *bond_slot;
+    int *bsel_hb, *bsel_plain, nb_hb, nb_plain;  // bond indices with / without an X-H hydrogen
+    int fin_lpa, hbg_lpa;                          // lanes per atom of k_finish and k_hb_gather
     double* bond_par;
     int *ang_ijk, *ang_slot;
     double* ang_par;
@@ -4131,6 +4180,14 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->bond_hboff = E->upload(bhoff);
         E->bond_hbB = E->upload(bhB);
         E->bond_slot = E->upload(bslot);
+        {
+            std::vector<int> bh, bp;
+            for (int b = 0; b < (int)P.bond_hbH.size(); ++b) (P.bond_hbH[b] >= 0 ? bh : bp).push_back(b);
+            E->nb_hb = (int)bh.size();
+            E->nb_plain = (int)bp.size();
+            E->bsel_hb = E->upload(bh);
+            E->bsel_plain = E->upload(bp);
+        }
         E->nangle = (int)(P.angle_par.size() / 2);
         std::vector<int> aslot(E->nangle);
         for (int t = 0; t < E->nangle; ++t) {
@@ -4336,6 +4393,8 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
             E->hb_s0 = E->upload(s0v);
             E->hb_xoff = E->upload(xoff);
             E->hb_xidx = E->upload(xidx);
+            // lanes per atom of the gather: 16 when the atoms' lists are short on average
+            E->hbg_lpa = ((long long)crange[(size_t)2 * (n + 1) + n] + xoff[n] < 192LL * n) ? 16 : 64;
             E->hb_gout = E->alloc<double>(3 * (size_t)std::max(E->hb_ngt, 1));
             E->hb_xslots = E->alloc<double>(3 * std::max<size_t>(xatom.size(), 1));
             E->e_hb2 = E->alloc<double>(std::max(E->nhb2, 1));
@@ -4385,6 +4444,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
             for (int s = 0; s < slot; ++s) sidx[fill[slot_atom[s]]++] = s;
         }
         E->slot_off = E->upload(soff);
+        E->fin_lpa = ((long long)soff[n] < 192LL * n) ? 16 : 64;  // lanes per atom of k_finish
         E->slot_idx = E->upload(sidx);
         std::vector<int> doff(n + 1, 0), didx(2 * E->nbond);
         for (int b = 0; b < E->nbond; ++b) {
@@ -4686,10 +4746,16 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     LAUNCH(KC_EEQ_BUILD, k_eeq_rhs_reduce, dim3(cdiv(np, 256)), dim3(256), s, E->nr, np, E->red_atom, E->red_last,
                        rsrc, E->b);
     // bonded lists
-    if (E->nbond)
-        LAUNCH(KC_BONDS, k_bonds, dim3(cdiv(E->nbond, 4)), dim3(256), s, E->nbond, E->bond_ij, E->bond_par,
-                           E->bond_hboff, E->bond_hbB, E->bond_slot, E->x, E->cn, E->rcov_d3, E->bond_H, E->slots,
-                           E->dcn_slot, E->e_bond);
+    if (E->nbond) {
+        if (E->nb_hb)
+            LAUNCH(KC_BONDS, k_bonds, dim3(cdiv(E->nb_hb, 4)), dim3(256), s, E->nb_hb, E->bsel_hb, E->bond_ij,
+                   E->bond_par, E->bond_hboff, E->bond_hbB, E->bond_slot, E->x, E->cn, E->rcov_d3, E->bond_H,
+                   E->slots, E->dcn_slot, E->e_bond);
+        if (E->nb_plain)
+            LAUNCH(KC_BONDS, k_bonds_plain, dim3(cdiv(E->nb_plain, 256)), dim3(256), s, E->nb_plain,
+                   E->bsel_plain, E->bond_ij, E->bond_par, E->bond_slot, E->x, E->cn, E->slots, E->dcn_slot,
+                   E->e_bond);
+    }
     if (E->nangle)
         LAUNCH(KC_ANGLES, k_angles, dim3(cdiv(E->nangle, 256)), dim3(256), s, E->nangle, E->ang_ijk, E->ang_par,
                            E->ang_slot, E->x, E->rcov_d3, E->slots, E->e_ang);
@@ -4748,8 +4814,12 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     if (E->nbatm)
         LAUNCH(KC_BATM, k_batm, dim3(cdiv(E->nbatm, 256)), dim3(256), s, E->nbatm, E->batm_ijk, E->batm_zb,
                            E->batm_slot, E->x, E->qa, E->slots, E->e_batm);
-    LAUNCH(KC_HB, k_hb_gather, dim3(cdiv(n, 4)), dim3(256), s, n, E->hb_crange, E->hb_gout, E->hb_xoff, E->hb_xidx,
-           E->hb_xslots, E->ghb);
+    if (E->hbg_lpa == 64)
+        LAUNCH(KC_HB, k_hb_gather<64>, dim3(cdiv(n, 4)), dim3(256), s, n, E->hb_crange, E->hb_gout, E->hb_xoff,
+               E->hb_xidx, E->hb_xslots, E->ghb);
+    else
+        LAUNCH(KC_HB, k_hb_gather<16>, dim3(cdiv(n, 16)), dim3(256), s, n, E->hb_crange, E->hb_gout, E->hb_xoff,
+               E->hb_xidx, E->hb_xslots, E->ghb);
     HIP_OK(hipEventRecord(E->ev_join, E->side));
     HIP_OK(hipStreamWaitEvent(s, E->ev_join, 0));
     if (E->nmol > 1) {
@@ -4835,7 +4905,11 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
                               E->dcn_slot, E->dEdcn, E->qtmp, E->vchain};
         LAUNCH(KC_CHAIN, k_atom_tail, dim3(cdiv(n, 256)), dim3(256), s, n, ta);
     }
-    LAUNCH(KC_CHAIN, k_finish, dim3(cdiv(n, 4)), dim3(256), s, n, E->x, E->rcov_cn, E->dlog, E->vchain,
+    if (E->fin_lpa == 64)
+        LAUNCH(KC_CHAIN, k_finish<64>, dim3(cdiv(n, 4)), dim3(256), s, n, E->x, E->rcov_cn, E->dlog, E->vchain,
+                       E->nbn, E->nbl, E->slot_off, E->slot_idx, E->slots, E->gpair, E->ghb, E->grad);
+    else
+        LAUNCH(KC_CHAIN, k_finish<16>, dim3(cdiv(n, 16)), dim3(256), s, n, E->x, E->rcov_cn, E->dlog, E->vchain,
                        E->nbn, E->nbl, E->slot_off, E->slot_idx, E->slots, E->gpair, E->ghb, E->grad);
     Segs sg;
     const double* ptrs[NPARTS] = {E->e_bond, E->e_ang,  E->e_tor, E->e_inv, E->e_disp, E->e_brep, E->e_rep, E->e_coul,
