@@ -47,7 +47,7 @@ constexpr int TILE = 256;       // pair kernel tile / workgroup size
 constexpr double KN = -7.5;
 // CN count 0.5 (1 + erf(v)); for v <= -6, erf(v) rounds to -1 and the count
 // is 0 (the branch skips erf for the many list members beyond the covalent
-// distance)
+// distance; also the hydrogen-bond coordination count of X-H bonds)
 __device__ inline double cn_count(double v) {
     double c = 0.0;
     if (v > -6.0) c = 0.5 * (1.0 + erf(v));
@@ -2182,7 +2182,8 @@ __device__ inline void nbrep_fix_atom(int i, const double* __restrict__ x, const
 // (gathered per atom in k_finish in a fixed order) and its energy to e_out.
 // ---------------------------------------------------------------------------
 // HB coordination of X-H hydrogens (ff_workspace_gfnff.cpp:46-97) and bonds (:104-180)
-// bonds with an X-H hydrogen (bsel: their indices): one wavefront per bond,
+// bonds with an X-H hydrogen and a non-empty hydrogen-bond partner list
+// (bsel: their indices): one wavefront per bond,
 // lanes stride the bond's hydrogen-bond partner list (HB coordination and
 // its gradient), fixed butterflies; lane 0 writes the bond's own slots
 
@@ -2216,7 +2217,7 @@ __global__ __launch_bounds__(256) void k_bonds(int nsel, const int* __restrict__
             if (rr2 > 900.0) continue;
             double rr = sqrt(rr2);
             double rc = 1.78 * (4.0 / 3.0) * (rcov_d3[H] + rcov_d3[B]);
-            hbcn += 0.5 * (1.0 + erf(-27.5 * (rr - rc) / rc));
+            hbcn += cn_count(-27.5 * (rr - rc) / rc);
         }
         hbcn = wave_sum(hbcn);
         al = (-(1.0 - 0.9) * hbcn + 1.0) * a0;
@@ -2245,7 +2246,9 @@ __global__ __launch_bounds__(256) void k_bonds(int nsel, const int* __restrict__
                 double rr = sqrt(rr2);
                 double rc = 1.78 * (4.0 / 3.0) * (rcov_d3[H] + rcov_d3[B]);
                 double arg = -27.5 * (rr - rc) / rc;
-                double dc = 0.5641895835477563 * (-27.5 / rc) * exp(-arg * arg) / rr;
+                // exp(-arg^2) underflows to 0 for |arg| > 27.5: skipped there
+                double dc = 0.0;
+                if (arg > -27.5) dc = 0.5641895835477563 * (-27.5 / rc) * exp(-arg * arg) / rr;
                 gH = gH + (-zz * dc) * v;
                 gB = (zz * dc) * v;
             }
@@ -2258,13 +2261,16 @@ __global__ __launch_bounds__(256) void k_bonds(int nsel, const int* __restrict__
     }
 }
 
-// bonds without an X-H hydrogen (no HB coordination; bsel: their indices),
-// one thread each: the arithmetic of k_bonds' lane 0
+// bonds without HB coordination (no X-H hydrogen, or an empty partner list,
+// where hbcn = 0 leaves alpha = a0 exactly; bsel: their indices), one thread
+// each: the arithmetic of k_bonds' lane 0; an X-H bond's hydrogen slot gets
+// its zero gradient
 __global__ __launch_bounds__(256) void k_bonds_plain(int nsel, const int* __restrict__ bsel,
                                                      const int* __restrict__ ij, const double* __restrict__ par,
-                                                     const int* __restrict__ slot0, const double* __restrict__ x,
-                                                     const double* __restrict__ cn, double* __restrict__ slots,
-                                                     double* __restrict__ dcn_slot, double* __restrict__ e_out) {
+                                                     const int* __restrict__ slot0, const int* __restrict__ bond_H,
+                                                     const double* __restrict__ x, const double* __restrict__ cn,
+                                                     double* __restrict__ slots, double* __restrict__ dcn_slot,
+                                                     double* __restrict__ e_out) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nsel) return;
     const int b = bsel[t];
@@ -2284,6 +2290,7 @@ __global__ __launch_bounds__(256) void k_bonds_plain(int nsel, const int* __rest
     const double yy = -dEdr;
     dcn_slot[2 * b] = yy * p[5] * p[2];
     dcn_slot[2 * b + 1] = yy * p[5] * p[3];
+    if (bond_H[b] >= 0) st3(slots, s + 2, make_double3(0, 0, 0));
 }
 
 // angles: ff_workspace_gfnff.cpp:187-275
@@ -3873,7 +3880,7 @@ struct DeviceEngine {
     int *red_lidx, *red_ul, *red_nul;  // distinct last atoms per 64-block of reduced variables
     double* qp;
     int *bond_ij, *bond_H, *bond_hboff, *bond_hbB, *bond_slot;
-    int *bsel_hb, *bsel_plain, nb_hb, nb_plain;  // bond indices with / without an X-H hydrogen
+    int *bsel_hb, *bsel_plain, nb_hb, nb_plain;  // bond indices with / without HB coordination
     int fin_lpa, hbg_lpa;                          // lanes per atom of k_finish and k_hb_gather
     double* bond_par;
     int *ang_ijk, *ang_slot;
@@ -4182,7 +4189,8 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->bond_slot = E->upload(bslot);
         {
             std::vector<int> bh, bp;
-            for (int b = 0; b < (int)P.bond_hbH.size(); ++b) (P.bond_hbH[b] >= 0 ? bh : bp).push_back(b);
+            for (int b = 0; b < (int)P.bond_hbH.size(); ++b)
+                (P.bond_hbH[b] >= 0 && bhoff[b + 1] > bhoff[b] ? bh : bp).push_back(b);
             E->nb_hb = (int)bh.size();
             E->nb_plain = (int)bp.size();
             E->bsel_hb = E->upload(bh);
@@ -4753,8 +4761,8 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
                    E->slots, E->dcn_slot, E->e_bond);
         if (E->nb_plain)
             LAUNCH(KC_BONDS, k_bonds_plain, dim3(cdiv(E->nb_plain, 256)), dim3(256), s, E->nb_plain,
-                   E->bsel_plain, E->bond_ij, E->bond_par, E->bond_slot, E->x, E->cn, E->slots, E->dcn_slot,
-                   E->e_bond);
+                   E->bsel_plain, E->bond_ij, E->bond_par, E->bond_slot, E->bond_H, E->x, E->cn, E->slots,
+                   E->dcn_slot, E->e_bond);
     }
     if (E->nangle)
         LAUNCH(KC_ANGLES, k_angles, dim3(cdiv(E->nangle, 256)), dim3(256), s, E->nangle, E->ang_ijk, E->ang_par,
@@ -4774,39 +4782,6 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     if (E->nbrep)
         LAUNCH(KC_BREP, k_brep, dim3(cdiv(E->nbrep, 256)), dim3(256), s, E->nbrep, E->brep_ij, E->brep_par,
                            E->brep_slot, E->x, E->slots, E->e_brep);
-    if (E->nhb) {
-        HBArgs ha{E->hb_cb, E->hb_ca, E->hb_Qh, E->hb_Qab, E->hb_nboff, E->hb_nbidx, E->hb_parc,
-                  E->hb_nbcoff, E->hb_nbcidx, E->hb_rad, E->repz, E->Z, E->hb_pk, E->hb_qh32};
-        HBRoles hr{E->hb_ent, E->hb_goff, E->hb_gatom, E->hb_ng0, E->hb_ng1, E->hb_ngt};
-        // case-1 role kernels (FP32 terms on the mixed path, FP64 otherwise);
-        // the side stream meanwhile builds the EEQ matrix
-        const int gr0[3] = {0, E->hb_ng0, E->hb_ng1}, gr1[3] = {E->hb_ng0, E->hb_ng1, E->hb_ngt};
-        for (int role = 0; role < 3; ++role) {
-            const dim3 grid(cdiv(gr1[role] - gr0[role], 4));
-            if (gr1[role] <= gr0[role]) continue;
-            if (E->mixed) {
-                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, float>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
-            } else {
-                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
-                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, double>), grid, dim3(256), s, hr, ha, E->x, E->hb_gout, E->e_hb);
-            }
-        }
-        // cases 2-4: on the mixed path the case-2 entries by the lean FP32
-        // kernel and those of cases 3 and 4 by the FP64 one
-        const dim3 g2(cdiv(E->nhb2, 256));
-        if (E->nhb2 && !E->mixed)
-            LAUNCH(KC_HB, (k_hb_full<double, 0>), g2, dim3(256), s, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
-                   E->hb_xslots, E->e_hb2);
-        if (E->mixed && E->nhb2_c2)
-            LAUNCH(KC_HB, (k_hb_full<float, 1>), g2, dim3(256), s, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
-                   E->hb_xslots, E->e_hb2);
-        if (E->mixed && E->nhb2 > E->nhb2_c2)
-            LAUNCH(KC_HB, (k_hb_full<double, 2>), g2, dim3(256), s, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
-                   E->hb_xslots, E->e_hb2);
-    }
     if (E->natm)
         LAUNCH(KC_ATM, k_atm, dim3(cdiv(E->natm, 256)), dim3(256), s, E->natm, E->atm_ijk, E->atm_sc,
                            E->atm_slot, E->x, E->rcov_d3, E->nelem, E->elem, E->nref, E->c6ref, E->gw, E->slots,
@@ -4814,11 +4789,49 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     if (E->nbatm)
         LAUNCH(KC_BATM, k_batm, dim3(cdiv(E->nbatm, 256)), dim3(256), s, E->nbatm, E->batm_ijk, E->batm_zb,
                            E->batm_slot, E->x, E->qa, E->slots, E->e_batm);
+    // Hydrogen bonds (all kernels and their gather) need only the coordinates
+    // and set-up data: a single molecule evaluates them on this stream while
+    // the side stream builds the EEQ matrix; a batch, whose evaluation stream
+    // carries the longer CN and bonded work, appends them to the side stream
+    // after its matrix build.
+    hipStream_t hs = (E->nmol > 1) ? E->side : s;
+    if (E->nhb) {
+        HBArgs ha{E->hb_cb, E->hb_ca, E->hb_Qh, E->hb_Qab, E->hb_nboff, E->hb_nbidx, E->hb_parc,
+                  E->hb_nbcoff, E->hb_nbcidx, E->hb_rad, E->repz, E->Z, E->hb_pk, E->hb_qh32};
+        HBRoles hr{E->hb_ent, E->hb_goff, E->hb_gatom, E->hb_ng0, E->hb_ng1, E->hb_ngt};
+        // case-1 role kernels (FP32 terms on the mixed path, FP64 otherwise)
+        const int gr0[3] = {0, E->hb_ng0, E->hb_ng1}, gr1[3] = {E->hb_ng0, E->hb_ng1, E->hb_ngt};
+        for (int role = 0; role < 3; ++role) {
+            const dim3 grid(cdiv(gr1[role] - gr0[role], 4));
+            if (gr1[role] <= gr0[role]) continue;
+            if (E->mixed) {
+                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, float>), grid, dim3(256), hs, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, float>), grid, dim3(256), hs, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, float>), grid, dim3(256), hs, hr, ha, E->x, E->hb_gout, E->e_hb);
+            } else {
+                if (role == 0) LAUNCH(KC_HB, (k_hb_role<0, double>), grid, dim3(256), hs, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 1) LAUNCH(KC_HB, (k_hb_role<1, double>), grid, dim3(256), hs, hr, ha, E->x, E->hb_gout, E->e_hb);
+                if (role == 2) LAUNCH(KC_HB, (k_hb_role<2, double>), grid, dim3(256), hs, hr, ha, E->x, E->hb_gout, E->e_hb);
+            }
+        }
+        // cases 2-4: on the mixed path the case-2 entries by the lean FP32
+        // kernel and those of cases 3 and 4 by the FP64 one
+        const dim3 g2(cdiv(E->nhb2, 256));
+        if (E->nhb2 && !E->mixed)
+            LAUNCH(KC_HB, (k_hb_full<double, 0>), g2, dim3(256), hs, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
+                   E->hb_xslots, E->e_hb2);
+        if (E->mixed && E->nhb2_c2)
+            LAUNCH(KC_HB, (k_hb_full<float, 1>), g2, dim3(256), hs, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
+                   E->hb_xslots, E->e_hb2);
+        if (E->mixed && E->nhb2 > E->nhb2_c2)
+            LAUNCH(KC_HB, (k_hb_full<double, 2>), g2, dim3(256), hs, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
+                   E->hb_xslots, E->e_hb2);
+    }
     if (E->hbg_lpa == 64)
-        LAUNCH(KC_HB, k_hb_gather<64>, dim3(cdiv(n, 4)), dim3(256), s, n, E->hb_crange, E->hb_gout, E->hb_xoff,
+        LAUNCH(KC_HB, k_hb_gather<64>, dim3(cdiv(n, 4)), dim3(256), hs, n, E->hb_crange, E->hb_gout, E->hb_xoff,
                E->hb_xidx, E->hb_xslots, E->ghb);
     else
-        LAUNCH(KC_HB, k_hb_gather<16>, dim3(cdiv(n, 16)), dim3(256), s, n, E->hb_crange, E->hb_gout, E->hb_xoff,
+        LAUNCH(KC_HB, k_hb_gather<16>, dim3(cdiv(n, 16)), dim3(256), hs, n, E->hb_crange, E->hb_gout, E->hb_xoff,
                E->hb_xidx, E->hb_xslots, E->ghb);
     HIP_OK(hipEventRecord(E->ev_join, E->side));
     HIP_OK(hipStreamWaitEvent(s, E->ev_join, 0));
