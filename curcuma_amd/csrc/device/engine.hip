@@ -3772,6 +3772,13 @@ __global__ void k_copy(int n, const double* __restrict__ a, double* __restrict__
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = a[i];
 }
+// coordinates of one evaluation into the engine's buffer; thread 0 also
+// clears the status flags (one launch at the head of the evaluation)
+__global__ void k_copy_start(int n, const double* __restrict__ a, double* __restrict__ b, int* __restrict__ flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) flags[0] = 0;
+    if (i < n) b[i] = a[i];
+}
 __global__ void k_fill_i(int n, int* __restrict__ a, int v) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
@@ -4712,8 +4719,7 @@ static void queue_solve(DeviceEngine* E, hipStream_t s, const T* A, const T* Lin
 
 static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::string& msg) {
     const int n = E->n;
-    hipLaunchKernelGGL(k_copy, dim3(cdiv(3 * n, 256)), dim3(256), 0, s, 3 * n, xin, E->x);
-    hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, s, 1, E->flags, 0);
+    hipLaunchKernelGGL(k_copy_start, dim3(cdiv(3 * n, 256)), dim3(256), 0, s, 3 * n, xin, E->x, E->flags);
     // The reduced EEQ matrix needs only the coordinates: it is built on the
     // side stream while this stream computes CN, the EEQ right-hand side and
     // every term that does not depend on the EEQ charges; the solve waits
