@@ -26,7 +26,8 @@ class ParamsView(C.Structure):
         ("Z", _i32p), ("elem", _i32p), ("frag", _i32p),
         ("chi_base", _f64p), ("gam_eff", _f64p), ("alpeeq", _f64p), ("cnf", _f64p), ("rcov_cn", _f64p),
         ("rcov_d3", _f64p), ("hb_rad", _f64p), ("metal", _i32p), ("qa", _f64p), ("zeta", _f64p),
-        ("sqrt_zr4r2", _f64p), ("rep_anb", _f64p), ("repz", _f64p), ("qfrag", _f64p),
+        ("sqrt_zr4r2", _f64p), ("rep_anb", _f64p), ("repz", _f64p), ("hb_lpd", _f64p),
+        ("hb_bas", _f64p), ("hb_aci", _f64p), ("qfrag", _f64p),
         ("present_Z", _i32p), ("nref", _i32p), ("refcn", _f64p), ("c6ref", _f64p),
         ("topo_off", _i32p), ("topo_idx", _i32p), ("topo_dist", _i32p),
         ("nbond", C.c_int32), ("bond_ij", _i32p), ("bond_par", _f64p), ("bond_hbH", _i32p),
@@ -66,6 +67,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
     lib.gfnff_batch_energies_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     lib.gfnff_energy_gradient.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p]
     lib.gfnff_energy_gradient_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.gfnff_device_status.argtypes = [C.c_void_p, _i32p, C.c_int32]
     lib.gfnff_charges.argtypes = [C.c_void_p, _f64p]
     lib.gfnff_cn.argtypes = [C.c_void_p, _f64p]
     lib.gfnff_set_param.argtypes = [C.c_void_p, C.c_char_p, C.c_double]

@@ -66,7 +66,7 @@ def build_engine(verbose: bool = False) -> Path:
 
 def build_oracle(verbose: bool = False) -> Path:
     ORACLE_LIB.parent.mkdir(parents=True, exist_ok=True)
-    _run(["gcc", "-O2", "-std=c11", "-ffp-contract=off", "-fPIC", "-shared", str(ORACLE_SRC), "-o",
+    _run(["gcc", "-O2", "-std=c11", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", str(ORACLE_SRC), "-o",
           str(ORACLE_LIB), "-lm"])
     if verbose:
         print("built", ORACLE_LIB)

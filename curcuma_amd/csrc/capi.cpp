@@ -260,6 +260,22 @@ int gfnff_energy_gradient_device(gfnff_handle* h, const double* xyz_bohr_dev, do
     }
 }
 
+int gfnff_device_status(gfnff_handle* h, int32_t* flags, int32_t clear) {
+    if (!h || !flags) return GFNFF_STATUS_ARGS;
+    *flags = 0;
+    if (!h->dev) return GFNFF_STATUS_OK;
+    try {
+        std::string msg;
+        int f = 0;
+        int st = gfnff::device_status(h->dev.get(), &f, clear != 0, msg);
+        if (st) return fail(h, st, "gfnff_device_status: " + msg);
+        *flags = f;
+        return GFNFF_STATUS_OK;
+    } catch (const std::exception& e) {
+        return fail(h, GFNFF_STATUS_DEVICE, std::string("gfnff_device_status: ") + e.what());
+    }
+}
+
 int gfnff_charges(gfnff_handle* h, double* q) {
     if (!h || !q || !h->have_molecule) return GFNFF_STATUS_ARGS;
     std::memcpy(q, h->last_q.data(), sizeof(double) * h->last_q.size());

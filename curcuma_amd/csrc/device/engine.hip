@@ -21,7 +21,9 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "engine.hpp"
@@ -1225,6 +1227,10 @@ __global__ __launch_bounds__(256) void k_eeq_pcg(int np, const double* __restric
     __shared__ double xo[PCG_MAXB][NB], ro[PCG_MAXB][NB];  // owned rows of x and r, resident for the solve
     __shared__ double Lc[NB * NB];                          // Linv of the owned block when there is one
     __shared__ __align__(16) double pw[4][2][NB];            // per wavefront: p of the block's rows and columns
+    // a diagonal block that is not positive definite (FLAG_CHOL, set before
+    // this launch, so every workgroup reads the same value) leaves the solve to
+    // the LU fallback that follows
+    if (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_CHOL) return;
     const int G = gridDim.x, g = blockIdx.x, nbk = np / NB;
     const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
     const int nown = (nbk - g + G - 1) / G;
@@ -1469,6 +1475,121 @@ __global__ __launch_bounds__(256) void k_to_f32(int n, const double* __restrict_
 // ---------------------------------------------------------------------------
 // K5: charges, self energy (ff_workspace.cpp:426-449) and qtmp (:459-507)
 // ---------------------------------------------------------------------------
+// Fallback solve of a reduced EEQ system that is not positive definite
+// (FLAG_CHOL raised by the Cholesky of a diagonal block or of the matrix): the
+// reference falls back to an LU with partial pivoting of its bordered system
+// when the Cholesky of A fails (eeq_solver.cpp:2143-2165, the augmented-system
+// PartialPivLU).  The reduced system R p = b is the bordered system with the
+// constraints eliminated, so Gaussian elimination with partial pivoting on it
+// gives the same charges.  One workgroup; does nothing unless the flag is set;
+// clears FLAG_CHOL (and FLAG_SOLVE of the failed solve) when it succeeds (a
+// zero pivot keeps it: singular system).
+// R: layout 0 = packed lower 64x64 blocks (block I >= J at (I(I+1)/2 + J) 64^2,
+// row-major inside), layout 1 = row-major np x np, lower triangle used.
+// W: nr x (nr + 1) work (the system augmented with b).  nr <= LU_MAX.
+constexpr int LU_MAX = 1024;
+constexpr int LU_T = 1024;
+__global__ __launch_bounds__(LU_T) void k_eeq_lu_fallback(int np, int nr, int layout, const double* __restrict__ R,
+                                                          const double* __restrict__ b, double* __restrict__ W,
+                                                          double* __restrict__ x, int* __restrict__ flags) {
+    __shared__ int s_flag, s_piv;
+    __shared__ double s_best[LU_T / 64];
+    __shared__ int s_idx[LU_T / 64];
+    __shared__ double xs[LU_MAX];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid == 0) s_flag = __hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (!(s_flag & FLAG_CHOL) || nr > LU_MAX) return;
+    const int m = nr, ld = nr + 1;
+    for (long e = tid; e < (long)m * m; e += LU_T) {
+        const int i = (int)(e / m), j = (int)(e % m);
+        const int r = max(i, j), c = min(i, j);
+        double v;
+        if (layout == 0) {
+            const long I = r / NB, J = c / NB;
+            v = R[(I * (I + 1) / 2 + J) * NB * NB + (r % NB) * NB + (c % NB)];
+        } else {
+            v = R[(long)r * np + c];
+        }
+        W[(long)i * ld + j] = v;
+    }
+    for (int i = tid; i < m; i += LU_T) W[(long)i * ld + m] = b[i];
+    __syncthreads();
+    int ok = 1;
+    for (int k = 0; k < m; ++k) {
+        // pivot row: largest |W(i, k)| for i >= k, the lowest index on ties
+        double best = -1.0;
+        int bi = m;
+        for (int i = k + tid; i < m; i += LU_T) {
+            const double a = fabs(W[(long)i * ld + k]);
+            if (a > best) {
+                best = a;
+                bi = i;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ob = __shfl_xor(best, o);
+            const int oi = __shfl_xor(bi, o);
+            if (ob > best || (ob == best && oi < bi)) {
+                best = ob;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            s_best[wv] = best;
+            s_idx[wv] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double bb = s_best[0];
+            int ii = s_idx[0];
+            for (int w = 1; w < LU_T / 64; ++w)
+                if (s_best[w] > bb || (s_best[w] == bb && s_idx[w] < ii)) {
+                    bb = s_best[w];
+                    ii = s_idx[w];
+                }
+            s_piv = (bb > 0.0 && ii < m) ? ii : -1;
+        }
+        __syncthreads();
+        const int p = s_piv;
+        if (p < 0) {
+            ok = 0;
+            break;
+        }
+        if (p != k)
+            for (int j = k + tid; j <= m; j += LU_T) {
+                const double t = W[(long)k * ld + j];
+                W[(long)k * ld + j] = W[(long)p * ld + j];
+                W[(long)p * ld + j] = t;
+            }
+        __syncthreads();
+        const double inv = 1.0 / W[(long)k * ld + k];
+        const int cols = m - k;  // columns k+1 .. m (the last is b)
+        for (long e = tid; e < (long)(m - k - 1) * cols; e += LU_T) {
+            const int i = k + 1 + (int)(e / cols), j = k + 1 + (int)(e % cols);
+            W[(long)i * ld + j] -= (W[(long)i * ld + k] * inv) * W[(long)k * ld + j];
+        }
+        __syncthreads();
+    }
+    if (!ok) return;
+    for (int i = m - 1; i >= 0; --i) {
+        double sacc = 0.0;
+        for (int j = i + 1 + tid; j < m; j += LU_T) sacc += W[(long)i * ld + j] * xs[j];
+        for (int o = 32; o > 0; o >>= 1) sacc += __shfl_xor(sacc, o);
+        if (lane == 0) s_best[wv] = sacc;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0.0;
+            for (int w = 0; w < LU_T / 64; ++w) t += s_best[w];
+            xs[i] = (W[(long)i * ld + m] - t) / W[(long)i * ld + i];
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < np; i += LU_T) x[i] = (i < m) ? xs[i] : 0.0;
+    // the LU result replaces whatever the failed solve left (its bounded waits included)
+    if (tid == 0) atomicAnd(flags, ~(FLAG_CHOL | FLAG_SOLVE));
+}
+
 __global__ __launch_bounds__(256) void k_charges(int n, const int* __restrict__ red_of,
                                                  const int* __restrict__ last_of_frag_first,
                                                  const int* __restrict__ frag_off,
@@ -2537,7 +2658,7 @@ struct HBArgs {
     const int* nbc_off;    // per atom: neighbours of parentC other than the acceptor (CSR)
     const int* nbc_idx;
     const double* hb_rad;
-    const double* repz;
+    const double* lpd;     // per atom: case-4 lone-pair distance (element table)
     const int* Z;
     const float4* pk;      // FP32 per atom {basicity, acidity, Qab, hb_rad} (roles A, B of the FP32 cores)
     const float* qh32;     // FP32 Qh (role H of the FP32 cores)
@@ -2589,7 +2710,7 @@ __device__ double hb_core(int A, int H, int B, int cs, const HBArgs& a, const do
         double3 lpv = z3, lp = pB;
         int nlp = 0;
         if (cs == 4) {
-            lpd = 0.50 - 0.018 * a.repz[B];
+            lpd = a.lpd[B];
             for (int k = nb0; k < nb1; ++k) lpv = lpv + (ld3(x, a.nb_idx[k]) - pB);
             double vn = nrm(lpv);
             if (vn > 1e-10 && nb1 > nb0) {
@@ -3734,7 +3855,8 @@ __global__ __launch_bounds__(256) void k_reduce1(Segs s, double* __restrict__ pa
 // block partials with a fixed tree; thread 0 then forms out[0] = total and
 // out[1..15] = terms (term index + 1) in a fixed summation order
 __global__ __launch_bounds__(NPARTS * 64) void k_reduce_final(Segs s, const double* __restrict__ part,
-                                                               double* __restrict__ parts, double* __restrict__ out) {
+                                                               double* __restrict__ parts, double* __restrict__ out,
+                                                               const int* __restrict__ flags, int* __restrict__ sticky) {
     __shared__ double sp[NPARTS];
     const int sg = threadIdx.x >> 6, l = threadIdx.x & 63;
     double v = part[sg * RED_BLOCKS + l];
@@ -3764,6 +3886,14 @@ __global__ __launch_bounds__(NPARTS * 64) void k_reduce_final(Segs s, const doub
     t[GFNFF_E_XBOND] = sp[13];
     double tot = 0.0;
     for (int k = 0; k < GFNFF_NTERMS; ++k) tot += t[k];
+    // a status flag raised in this evaluation (CN list overflow, EEQ pivot,
+    // solver timeout) makes the total NaN and is kept in the sticky word
+    // until the host reads and clears it
+    const int f = __hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (f) {
+        tot = __builtin_nan("");
+        atomicOr(sticky, f);
+    }
     out[0] = tot;
     for (int k = 0; k < GFNFF_NTERMS; ++k) out[1 + k] = t[k];
 }
@@ -3882,7 +4012,7 @@ struct DeviceEngine {
     std::vector<void*> bufs;
     // parameters (device)
     int *Z, *elem, *nref, *metal, *frag, *toff, *tidx, *tdist;
-    double *chi_base, *gam, *alp, *cnf, *rcov_cn, *rcov_d3, *hb_rad, *qa, *zeta, *szr, *anb, *repz, *refcn, *c6ref;
+    double *chi_base, *gam, *alp, *cnf, *rcov_cn, *rcov_d3, *hb_rad, *hb_lpd, *qa, *zeta, *szr, *anb, *repz, *refcn, *c6ref;
     int *red_atom, *red_last, *red_of, *frag_off, *frag_atoms;
     int *red_lidx, *red_ul, *red_nul;  // distinct last atoms per 64-block of reduced variables
     double* qp;
@@ -3918,6 +4048,8 @@ struct DeviceEngine {
         *slots, *dcn_slot, *grad, *e_coul, *e_disp, *e_rep, *e_bond, *e_ang, *e_tor, *e_brep, *e_hb, *e_atm,
         *e_batm, *e_inv, *e_st, *e_xb, *parts, *rpart, *out, *diag, *ghb, *ybuf, *xbuf, *h6, *dh6, *ppart;
     int *nbn, *nbl, *flags;
+    int* sticky = nullptr;  // status flags of every evaluation since the last device_status read
+    double *lu_w = nullptr, *lu_src = nullptr;  // LU fallback of a non-SPD reduced system (nr <= LU_MAX)
 
     PairArgs pair_args() const {
         PairArgs a;
@@ -4038,6 +4170,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->rcov_cn = E->upload(P.rcov_cn);
         E->rcov_d3 = E->upload(P.rcov_d3);
         E->hb_rad = E->upload(P.hb_rad);
+        E->hb_lpd = E->upload(P.hb_lpd);
         E->qa = E->upload(P.qa);
         E->zeta = E->upload(P.zeta);
         E->szr = E->upload(P.sqrt_zr4r2);
@@ -4508,6 +4641,11 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->ybuf = E->alloc<double>(E->np);
         E->xbuf = E->alloc<double>(E->np);
         E->ready = E->alloc<int>(E->np / NB);
+        if (E->nmol == 1 && E->nr <= LU_MAX) {
+            E->lu_w = E->alloc<double>((size_t)std::max(E->nr, 1) * (std::max(E->nr, 1) + 1));
+            // the FP64 Cholesky factors R in place: keep a copy for the fallback
+            if (!E->use_cg && !E->mixed) E->lu_src = E->alloc<double>(E->mat_size);
+        }
         if (E->use_cg) {
             E->cg_z = E->alloc<double>(E->np);
             E->cg_p = E->alloc<double>(2 * (size_t)E->np);
@@ -4606,6 +4744,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         E->nbn = E->alloc<int>(n);
         E->nbl = E->alloc<int>((size_t)n * MAXNB_CN);
         E->flags = E->alloc<int>(1);
+        E->sticky = E->alloc<int>(1);
         if (hipStreamCreate(&E->own) != hipSuccess || hipStreamCreate(&E->la.side) != hipSuccess ||
             hipEventCreateWithFlags(&E->la.fork, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&E->la.join, hipEventDisableTiming) != hipSuccess ||
@@ -4614,6 +4753,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
             hipEventCreateWithFlags(&E->ev_join, hipEventDisableTiming) != hipSuccess)
             throw std::runtime_error("stream");
         hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, E->own, 1, E->flags, 0);
+        hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, E->own, 1, E->sticky, 0);
         hipLaunchKernelGGL(k_eeq_diag, dim3(cdiv(n, 256)), dim3(256), 0, E->own, n, E->gam, E->alp, E->diag);
         if (hipStreamSynchronize(E->own) != hipSuccess) throw std::runtime_error("init sync");
     } catch (const std::bad_alloc&) {
@@ -4803,7 +4943,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     hipStream_t hs = (E->nmol > 1) ? E->side : s;
     if (E->nhb) {
         HBArgs ha{E->hb_cb, E->hb_ca, E->hb_Qh, E->hb_Qab, E->hb_nboff, E->hb_nbidx, E->hb_parc,
-                  E->hb_nbcoff, E->hb_nbcidx, E->hb_rad, E->repz, E->Z, E->hb_pk, E->hb_qh32};
+                  E->hb_nbcoff, E->hb_nbcidx, E->hb_rad, E->hb_lpd, E->Z, E->hb_pk, E->hb_qh32};
         HBRoles hr{E->hb_ent, E->hb_goff, E->hb_gatom, E->hb_ng0, E->hb_ng1, E->hb_ngt};
         // case-1 role kernels (FP32 terms on the mixed path, FP64 otherwise)
         const int gr0[3] = {0, E->hb_ng0, E->hb_ng1}, gr1[3] = {E->hb_ng0, E->hb_ng1, E->hb_ngt};
@@ -4850,6 +4990,9 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         LAUNCH(KC_PCG, k_eeq_pcg, dim3(E->cg_groups), dim3(PCG_THREADS), s, np, E->Ared, E->Linv, E->b, E->xbuf,
                E->cg_z, E->cg_p, E->cg_part, E->cg_mb, E->cg_iters, E->cg_tol2, PCG_MAXIT, E->flags);
     } else if (!E->mixed) {
+        if (E->lu_src)
+            LAUNCH(KC_CHOL_DIAG, k_copy, dim3(cdiv((long)E->mat_size, 256)), dim3(256), s, (int)E->mat_size, E->Ared,
+                   E->lu_src);
         queue_cholesky(&E->timer, s, np, E->Ared, E->Linv, E->flags, E->use_lookahead ? &E->la : nullptr);
         queue_solve<double>(E, s, E->Ared, E->Linv, E->b, E->ybuf, E->xbuf);
     } else {
@@ -4868,6 +5011,9 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
             }
         }
     }
+    if (E->lu_w)  // no work unless the system was found not positive definite
+        LAUNCH(KC_CHARGES, k_eeq_lu_fallback, dim3(1), dim3(LU_T), s, np, E->nr, E->use_cg ? 0 : 1,
+               E->use_cg ? E->Ared : (E->lu_src ? E->lu_src : E->Ared), E->b, E->lu_w, E->xbuf, E->flags);
     LAUNCH(KC_CHARGES, k_charges, dim3(cdiv(n, 256)), dim3(256), s, n, E->red_of, (const int*)nullptr,
                        E->frag_off, E->frag_atoms, E->frag, E->qp, E->xbuf, E->chi_base, E->cnf, E->gam, E->alp, E->cn,
                        E->q, E->eself, E->qtmp);
@@ -4941,7 +5087,8 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         sg.dst[k] = k;
     }
     LAUNCH(KC_REDUCE, k_reduce1, dim3(RED_BLOCKS, NPARTS), dim3(256), s, sg, E->rpart);
-    LAUNCH(KC_REDUCE, k_reduce_final, dim3(1), dim3(NPARTS * 64), s, sg, E->rpart, E->parts, E->out);
+    LAUNCH(KC_REDUCE, k_reduce_final, dim3(1), dim3(NPARTS * 64), s, sg, E->rpart, E->parts, E->out, E->flags,
+           E->sticky);
     if (E->nmol > 1) {
         MolEnergyArgs ma;
         ma.nmol = E->nmol;
@@ -5003,6 +5150,17 @@ int device_eval(DeviceEngine* E, const double* xyz_bohr_dev, double* energy_dev,
     if (hipGetLastError() != hipSuccess) {
         msg = "kernel launch failed";
         return kErrDevice;
+    }
+    return kOk;
+}
+
+int device_status(DeviceEngine* E, int* flags, bool clear, std::string& msg) {
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(flags, E->sticky, sizeof(int), hipMemcpyDeviceToHost));
+    if (clear) {
+        hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, E->own, 1, E->sticky, 0);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(E->own));
     }
     return kOk;
 }
@@ -5191,6 +5349,7 @@ static int dense_cholesky(int n, const T* a, T* l_out, std::string& msg) {
 int device_dense_cholesky(int n, const double* a, double* l_out, std::string& msg) {
     return dense_cholesky<double>(n, a, l_out, msg);
 }
+
 int device_dense_cholesky_f32(int n, const float* a, float* l_out, std::string& msg) {
     return dense_cholesky<float>(n, a, l_out, msg);
 }

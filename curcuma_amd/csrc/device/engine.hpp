@@ -31,6 +31,10 @@ int device_eval(DeviceEngine* e, const double* xyz_bohr_dev, double* energy_dev,
 
 int device_count();
 
+// Status flags (FLAG_* bits) raised by any evaluation since the last cleared
+// read; synchronises the device.  clear resets the word.
+int device_status(DeviceEngine* e, int* flags, bool clear, std::string& msg);
+
 // Iterations of the last iterative EEQ solve (0 when the Cholesky path runs,
 // -1 on a device error); synchronises.
 int device_solver_iterations(DeviceEngine* e);

@@ -1858,6 +1858,9 @@ int merge_batch(const std::vector<ParamSet>& mols, ParamSet& B, std::string& msg
         reserve(&ParamSet::rcov_d3);
         reserve(&ParamSet::rep_anb);
         reserve(&ParamSet::repz);
+        reserve(&ParamSet::hb_lpd);
+        reserve(&ParamSet::hb_bas);
+        reserve(&ParamSet::hb_aci);
         reserve(&ParamSet::sqrt_zr4r2);
         reserve(&ParamSet::stors_ijkl);
         reserve(&ParamSet::stors_par);
@@ -1905,6 +1908,9 @@ int merge_batch(const std::vector<ParamSet>& mols, ParamSet& B, std::string& msg
         append(B.sqrt_zr4r2, P.sqrt_zr4r2);
         append(B.rep_anb, P.rep_anb);
         append(B.repz, P.repz);
+        append(B.hb_lpd, P.hb_lpd);
+        append(B.hb_bas, P.hb_bas);
+        append(B.hb_aci, P.hb_aci);
         append(B.qfrag, P.qfrag);
         append(B.hyb, P.hyb);
         append(B.pi_frag, P.pi_frag);
@@ -2007,6 +2013,8 @@ void detect_hbxb(const Mol& m, const Topo& t, const Options& opt, ParamSet& P, b
                 if (Z[nb] == 6 && t.hyb[nb] == 3) ++c;
             if (c == 1) ca[nn] *= 0.80;
         }
+        P.hb_bas = cb;
+        P.hb_aci = ca;
         std::vector<int> hbh;
         std::vector<char> is_hbh(n, 0);
         for (int h = 0; h < n; ++h) {
@@ -2190,7 +2198,10 @@ void detect_hbxb(const Mol& m, const Topo& t, const Options& opt, ParamSet& P, b
             if (X < 0) continue;
             if (Z[X] == 16 && t.adj[X].size() > 2) continue;
             for (int B = 0; B < n; ++B) {
-                if (m.dist2(X, B) >= 100.0) continue;
+                // within 10 Bohr of X: strict below the cell-list size, inclusive
+                // from it on (forEachNeighbor's r2 <= cutoff^2, spatial_cell_list.h:171)
+                const double rxb2 = m.dist2(X, B);
+                if (n >= kCellListMinAtoms ? rxb2 > 100.0 : rxb2 >= 100.0) continue;
                 if (B == A || B == X || cb[B] < 1e-6) continue;
                 if (Z[B] == 6) {
                     if (t.pi[B] == 0 || q[B] >= 0.05) continue;
@@ -2223,6 +2234,25 @@ int build_params(int n, const int32_t* Z, const double* xyz, int charge, const i
             msg = "non-finite coordinates";
             return kErrGeometry;
         }
+    // Two atoms closer than 1e-10 Bohr: the reference's Phase-1 EEQ falls back to
+    // placeholder charges (eeq_solver.cpp:2843-2846) and setMolecule refuses the
+    // molecule (qm_methods/gfnff_method.cpp:43-51).  Sweep over x-sorted atoms.
+    {
+        std::vector<int> ord(n);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::sort(ord.begin(), ord.end(), [&](int a, int b) { return xyz[3 * a] < xyz[3 * b]; });
+        for (int u = 0; u < n; ++u)
+            for (int w = u + 1; w < n && xyz[3 * ord[w]] - xyz[3 * ord[u]] < 1e-10; ++w) {
+                const int i = ord[u], j = ord[w];
+                const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1],
+                             dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+                if (std::sqrt(dx * dx + dy * dy + dz * dz) < 1e-10) {
+                    msg = "atoms " + std::to_string(std::min(i, j)) + " and " + std::to_string(std::max(i, j)) +
+                          " coincide (zero distance): the EEQ charges would be placeholders";
+                    return kErrEEQ;
+                }
+            }
+    }
     Mol m{n, Z, xyz};
     Topo t;
     detect_bonds(m, forced, nforced, t);
@@ -2303,6 +2333,7 @@ int build_params(int n, const int32_t* Z, const double* xyz, int charge, const i
     P.sqrt_zr4r2.resize(n);
     P.rep_anb.resize(n);
     P.repz.resize(n);
+    P.hb_lpd.resize(n);
     for (int i = 0; i < n; ++i) {
         int z = Z[i];
         P.chi_base[i] = -chi[z - 1] + t.dxi[i];
@@ -2319,6 +2350,9 @@ int build_params(int n, const int32_t* Z, const double* xyz, int charge, const i
         double fn = 1.0 + (-0.1270) / (1.0 + nbc * nbc);
         P.rep_anb[i] = repan[z - 1] * (1.0 + t.qa[i] * 0.3480) * fn;
         P.repz[i] = repz[z - 1];
+        // case-4 lone-pair distance from the static element table
+        // (ff_workspace_gfnff.cpp:1011-1014), not from the per-atom repz
+        P.hb_lpd[i] = 0.50 - 0.018 * repz[z - 1];
     }
     // topological neighbourhood CSR
     P.topo_off.assign(n + 1, 0);
@@ -2602,6 +2636,8 @@ int refresh_hbxb(ParamSet& P, const double* xyz_bohr, const Options& opt, bool& 
               Q.hb_nbB_off != P.hb_nbB_off || Q.hb_nbB_idx != P.hb_nbB_idx || Q.hb_parentC != P.hb_parentC ||
               Q.hb_nbC_off != P.hb_nbC_off || Q.hb_nbC_idx != P.hb_nbC_idx || Q.xb_axb != P.xb_axb ||
               Q.xb_par != P.xb_par;
+    P.hb_bas.swap(Q.hb_bas);  // the carbene basicity depends on the geometry
+    P.hb_aci.swap(Q.hb_aci);
     if (changed) {
         P.hb_ahb.swap(Q.hb_ahb);
         P.hb_case.swap(Q.hb_case);
@@ -2640,6 +2676,9 @@ gfnff_params_view ParamSet::view() const {
     v.sqrt_zr4r2 = sqrt_zr4r2.data();
     v.rep_anb = rep_anb.data();
     v.repz = repz.data();
+    v.hb_lpd = hb_lpd.data();
+    v.hb_bas = hb_bas.data();
+    v.hb_aci = hb_aci.data();
     v.qfrag = qfrag.data();
     v.present_Z = present_Z.data();
     v.nref = nref.data();

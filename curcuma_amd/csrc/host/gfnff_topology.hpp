@@ -24,6 +24,11 @@ constexpr double kAngToBohrGeom = 1.0 / 0.529177210903;
 constexpr double kAngToBohrCN = 1.8897259886;
 constexpr double kAngToBohrTab = 1.0 / 0.52917726;
 
+// Atom count from which the reference builds its spatial cell list for the
+// hydrogen/halogen-bond candidates (nb_cell_list_min_atoms, gfnff_method.cpp:8078);
+// it changes the XB cut-off compare from strict to inclusive.
+constexpr int kCellListMinAtoms = 800;
+
 struct Options {
     double cn_cutoff_bohr = 6.0;    // per-evaluation CN neighbour cutoff
     double hb_accuracy = 0.1;       // HB list thresholds hbthr1/hbthr2
@@ -50,7 +55,7 @@ struct ParamSet {
     int n = 0, nfrag = 0, nelem = 0, charge = 0;
     std::vector<int32_t> Z, elem, frag, present_Z, nref, metal;
     std::vector<double> chi_base, gam_eff, alpeeq, cnf, rcov_cn, rcov_d3, hb_rad, qa, zeta, sqrt_zr4r2,
-        rep_anb, repz, qfrag, refcn, c6ref;
+        rep_anb, repz, hb_lpd, hb_bas, hb_aci, qfrag, refcn, c6ref;
     std::vector<int32_t> topo_off, topo_idx, topo_dist;
 
     std::vector<int32_t> bond_ij, bond_hbH, hbcn_pair;

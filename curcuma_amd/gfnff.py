@@ -19,6 +19,8 @@ import numpy as np
 from . import _lib
 from ._lib import NTERMS, TERM_NAMES, as_f64, as_i32, fptr, iptr
 
+GFNFF_E = {name: k for k, name in enumerate(TERM_NAMES)}
+
 ANGSTROM_TO_BOHR = 1.0 / 0.529177210903  # src/core/units.h:81-82
 
 
@@ -104,10 +106,56 @@ class GFNFFMethod:
         return cn
 
     def getEnergyDecomposition(self) -> dict:
-        d = {name: float(v) for name, v in zip(TERM_NAMES, self._terms) if not name.startswith("reserved")}
-        d["repulsion"] = d["bonded_rep"] + d["nonbonded_rep"]
-        d["total"] = self._energy
-        return d
+        """The reference's ten keys (qm_methods/gfnff_method.cpp:152-185).  As
+        there, Torsion holds the primary and extra torsions
+        (FFEnergyComponents::dihedral, ff_workspace_gfnff.cpp:348,449); the
+        repulsion (getRepulsionEnergy) and the triple-bond torsions
+        (getSTorsionEnergy) are not among the keys."""
+        t = self._terms
+        return {"Bond": float(t[GFNFF_E["bond"]]), "Angle": float(t[GFNFF_E["angle"]]),
+                "Torsion": float(t[GFNFF_E["torsion"]]), "Inversion": float(t[GFNFF_E["inversion"]]),
+                "Dispersion": float(t[GFNFF_E["dispersion"]]), "Coulomb": float(t[GFNFF_E["coulomb"]]),
+                "HBond": float(t[GFNFF_E["hbond"]]), "XBond": float(t[GFNFF_E["xbond"]]),
+                "ATM": float(t[GFNFF_E["atm"]]), "BATM": float(t[GFNFF_E["batm"]])}
+
+    # per-term getters (computational_method.h:295-337, GFNFF::*Energy gfnff_method.cpp:10496-10575)
+    def getBondEnergy(self) -> float:
+        return float(self._terms[GFNFF_E["bond"]])
+
+    def getAngleEnergy(self) -> float:
+        return float(self._terms[GFNFF_E["angle"]])
+
+    def getDihedralEnergy(self) -> float:
+        return float(self._terms[GFNFF_E["torsion"]])
+
+    def getInversionEnergy(self) -> float:
+        return float(self._terms[GFNFF_E["inversion"]])
+
+    def getRepulsionEnergy(self) -> float:
+        """Bonded plus non-bonded repulsion (GFNFF::RepulsionEnergy, gfnff_method.cpp:10521-10525)."""
+        return float(self._terms[GFNFF_E["bonded_rep"]] + self._terms[GFNFF_E["nonbonded_rep"]])
+
+    def getDispersionEnergy(self) -> float:
+        return float(self._terms[GFNFF_E["dispersion"]])
+
+    def getCoulombEnergy(self) -> float:
+        return float(self._terms[GFNFF_E["coulomb"]])
+
+    def getSTorsionEnergy(self) -> float:
+        """Triple-bond torsions (FFEnergyComponents::stors, ff_workspace_gfnff.cpp:633)."""
+        return float(self._terms[GFNFF_E["stors"]])
+
+    def getTerms(self) -> dict:
+        """All per-term energies of the last evaluation, by engine term name."""
+        return {name: float(v) for name, v in zip(TERM_NAMES, self._terms) if not name.startswith("reserved")}
+
+    def deviceStatus(self, clear: bool = True) -> int:
+        """GFNFF_FLAG_* bits raised by device-resident evaluations since the last clearing read."""
+        f = C.c_int32(0)
+        st = self._lib.gfnff_device_status(self._h, C.byref(f), 1 if clear else 0)
+        if st != 0:
+            raise RuntimeError(self.getErrorMessage())
+        return int(f.value)
 
     def hasError(self) -> bool:
         return bool(self._lib.gfnff_has_error(self._h))

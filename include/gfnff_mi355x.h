@@ -87,12 +87,26 @@ int gfnff_update_geometry(gfnff_handle* h, const double* xyz_ang);
  * energies (may be NULL).  Runs on the GPU. */
 int gfnff_energy_gradient(gfnff_handle* h, int32_t want_grad, double* energy, double* grad, double* terms);
 
-/* Device-resident variant: xyz_bohr_dev (natoms*3, Bohr), energy_dev (1 or
- * GFNFF_NTERMS+1 doubles: total followed by the terms), grad_dev (natoms*3)
- * are device pointers; work is queued on `stream` (a hipStream_t, NULL = the
- * default stream) and the call returns without synchronising. */
+/* Device-resident variant: xyz_bohr_dev (natoms*3, Bohr), energy_dev
+ * (GFNFF_NTERMS+1 doubles, always written: the total followed by the
+ * terms), grad_dev (natoms*3, may be NULL) are device pointers; work is queued
+ * on `stream` (a hipStream_t, NULL = the default stream) and the call returns
+ * without synchronising.  Failures detected on the device (see
+ * GFNFF_FLAG_*) make energy_dev[0] NaN and are accumulated in the handle's
+ * status word, read with gfnff_device_status. */
 int gfnff_energy_gradient_device(gfnff_handle* h, const double* xyz_bohr_dev, double* energy_dev, double* grad_dev,
                                  void* stream);
+
+/* Device-side failure flags of an evaluation (bit mask). */
+#define GFNFF_FLAG_EEQ_NOT_SPD 1   /* EEQ matrix not positive definite (Cholesky pivot) */
+#define GFNFF_FLAG_CN_OVERFLOW 2   /* more than 128 atoms within the 6 Bohr CN cut-off of one atom */
+#define GFNFF_FLAG_SOLVE 4         /* EEQ solve did not converge or a bounded wait timed out */
+
+/* Engine extension (no counterpart in the reference, whose CPU evaluation
+ * reports through hasError()): the GFNFF_FLAG_* bits raised by every
+ * evaluation of the handle since the last call with clear != 0.
+ * Synchronises the device.  0 in *flags means all evaluations were clean. */
+int gfnff_device_status(gfnff_handle* h, int32_t* flags, int32_t clear);
 
 /* ComputationalMethod::getCharges() / getCN() (computational_method.h:295-337):
  * EEQ charges and coordination numbers of the last evaluation. */

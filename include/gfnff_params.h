@@ -55,6 +55,11 @@ typedef struct gfnff_params_view {
     const double* sqrt_zr4r2;   /* D4 sqrt(0.5*r4r2*sqrt(Z)) */
     const double* rep_anb;      /* non-bonded repulsion atomic exponent factor */
     const double* repz;         /* repulsion effective charge */
+    const double* hb_lpd;       /* HB case-4 lone-pair distance 0.50 - 0.018 repz(Z), from the
+                                   element table (independent of fitted repz scales) */
+    const double* hb_bas;       /* HB/XB basicity per atom (element value with the carbonyl,
+                                   nitro and carbene overrides) */
+    const double* hb_aci;       /* HB acidity per atom (amide N scaled by 0.80) */
     const double* qfrag;        /* [nfrag] fragment charges */
 
     /* present elements [nelem] and D4 reference data */

@@ -15,7 +15,7 @@
  * Units: Bohr, Hartree.  Every function cites the reference lines it follows.
  *
  * Parity status per function (see tests/test_oracle_fixtures.py): pinned by the
- * reference's Fortran-derived fixtures (test_cases/reference_data/*.ref.json)
+ * reference's Fortran-derived fixtures (test_cases/reference_data/<name>.ref.json)
  * for total and per-term energies, charges and per-term gradients; gradients
  * additionally by central finite differences of this file's own energy.
  */
@@ -24,6 +24,13 @@
 #include <string.h>
 
 #include "../include/gfnff_params.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#define THREAD_ID() omp_get_thread_num()
+#else
+#define THREAD_ID() 0
+#endif
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -75,10 +82,28 @@ static inline void gadd(gacc_t* G, int comp, int atom, double x, double y, doubl
 /* cn_calculator.cpp:367-428 (strict d^2 < cut^2)                            */
 /* ------------------------------------------------------------------------- */
 static void oracle_cn(const gfnff_params_view* P, const double* x, double cut, double* cn_raw, double* cn,
-                      double* dlogdcn) {
+                      double* dlogdcn, int nthr) {
     int n = P->natoms;
     double cut2 = cut * cut;
     memset(cn_raw, 0, sizeof(double) * n);
+    if (nthr > 1) {
+        /* threaded: striped rows, each atom sums its own row in ascending j */
+#pragma omp parallel for schedule(static, 1) num_threads(nthr)
+        for (int i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int j = 0; j < n; ++j) {
+                if (j == i) continue;
+                double d[3];
+                sub3(x + 3 * i, x + 3 * j, d);
+                double d2 = dot3(d, d);
+                if (!(d2 < cut2)) continue;
+                double r = sqrt(d2);
+                double rij = P->rcov_cn[i] + P->rcov_cn[j];
+                s += 0.5 * (1.0 + erf(KN * (r - rij) / rij));
+            }
+            cn_raw[i] = s;
+        }
+    } else
     for (int i = 0; i < n; ++i)
         for (int j = i + 1; j < n; ++j) {
             double d[3];
@@ -151,7 +176,7 @@ static void oracle_c6(const gfnff_params_view* P, const double* gw, const double
 /* system solved by Cholesky of A and the Schur complement over fragments    */
 /* (solveWithSchurCholesky :2087-2240).                                       */
 /* ------------------------------------------------------------------------- */
-static int chol(int n, double* A) {
+static int chol(int n, double* A, int nthr) {
     for (int k = 0; k < n; ++k) {
         double* Ak = A + (size_t)k * n;
         double d = Ak[k];
@@ -159,6 +184,8 @@ static int chol(int n, double* A) {
         if (!(d > 0.0)) return 0;
         d = sqrt(d);
         Ak[k] = d;
+        /* the rows below the pivot are independent: threaded over rows */
+#pragma omp parallel for schedule(static) num_threads(nthr) if (nthr > 1 && n - k > 128)
         for (int i = k + 1; i < n; ++i) {
             double* Ai = A + (size_t)i * n;
             double s = Ai[k];
@@ -215,22 +242,29 @@ static int lu_solve(int m, double* M, double* b) {
     return 1;
 }
 
-static int oracle_eeq(const gfnff_params_view* P, const double* x, const double* cn, double* q) {
+static int oracle_eeq(const gfnff_params_view* P, const double* x, const double* cn, double* q, int nthr) {
     int n = P->natoms, nf = P->nfrag;
     double* A = (double*)malloc(sizeof(double) * (size_t)n * n);
     double* rhs = (double*)malloc(sizeof(double) * n);
     double* Z2 = (double*)calloc((size_t)nf * n, sizeof(double));
     double* S = (double*)calloc((size_t)nf * nf, sizeof(double));
     double* lam = (double*)calloc(nf, sizeof(double));
-    int ok = 0;
-    if (!A || !rhs || !Z2 || !S || !lam) goto out;
+    int* fo = (int*)calloc(nf + 1, sizeof(int));  /* fragment atom lists, ascending (CSR) */
+    int* fa = (int*)calloc(n > 0 ? n : 1, sizeof(int));
+    int ok = 0, coincident = 0;
+    if (!A || !rhs || !Z2 || !S || !lam || !fo || !fa) goto out;
+    /* matrix rows striped over threads */
+#pragma omp parallel for schedule(static, 1) num_threads(nthr) if (nthr > 1) reduction(| : coincident)
     for (int i = 0; i < n; ++i) {
         A[(size_t)i * n + i] = P->gam_eff[i] + TSQRT2PI / sqrt(P->alpeeq[i]);
         for (int j = 0; j < i; ++j) {
             double d[3];
             sub3(x + 3 * i, x + 3 * j, d);
             double r = norm3(d);
-            if (r < 1e-10) goto out; /* reference: zero distance -> fallback charges */
+            if (r < 1e-10) { /* reference: zero distance -> fallback charges */
+                coincident = 1;
+                continue;
+            }
             double g = 1.0 / sqrt(P->alpeeq[i] + P->alpeeq[j]);
             double v = erf(g * r) / r;
             A[(size_t)i * n + j] = v;
@@ -238,7 +272,16 @@ static int oracle_eeq(const gfnff_params_view* P, const double* x, const double*
         }
         rhs[i] = P->chi_base[i] + P->cnf[i] * sqrt(cn[i]);
     }
-    if (!chol(n, A)) {
+    if (coincident) goto out;
+    for (int i = 0; i < n; ++i) fo[P->frag[i] + 1]++;
+    for (int f = 0; f < nf; ++f) fo[f + 1] += fo[f];
+    {
+        int* fill = (int*)calloc(nf > 0 ? nf : 1, sizeof(int));
+        if (!fill) goto out;
+        for (int i = 0; i < n; ++i) fa[fo[P->frag[i]] + fill[P->frag[i]]++] = i;
+        free(fill);
+    }
+    if (!chol(n, A, nthr)) {
         /* not positive definite: LU on the bordered system, as the reference's
            dispatchSolve falls back to LU (eeq_solver.cpp:2143-2165) */
         int m = n + nf;
@@ -274,22 +317,24 @@ static int oracle_eeq(const gfnff_params_view* P, const double* x, const double*
         goto out;
     }
     chol_solve(n, A, rhs);
+    /* one solve per fragment constraint column, threaded over fragments */
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthr) if (nthr > 1)
     for (int f = 0; f < nf; ++f) {
         double* c = Z2 + (size_t)f * n;
         for (int i = 0; i < n; ++i) c[i] = (P->frag[i] == f) ? 1.0 : 0.0;
         chol_solve(n, A, c);
     }
+    /* Schur complement S(f, g) = sum over the atoms of f (ascending) of Z2(g) */
+#pragma omp parallel for schedule(static) num_threads(nthr) if (nthr > 1)
     for (int f = 0; f < nf; ++f) {
         for (int g = 0; g < nf; ++g) {
             double s = 0.0;
             const double* c = Z2 + (size_t)g * n;
-            for (int i = 0; i < n; ++i)
-                if (P->frag[i] == f) s += c[i];
+            for (int k = fo[f]; k < fo[f + 1]; ++k) s += c[fa[k]];
             S[(size_t)f * nf + g] = s;
         }
         double s = 0.0;
-        for (int i = 0; i < n; ++i)
-            if (P->frag[i] == f) s += rhs[i];
+        for (int k = fo[f]; k < fo[f + 1]; ++k) s += rhs[fa[k]];
         lam[f] = s - P->qfrag[f];
     }
     if (!lu_solve(nf, S, lam)) goto out;
@@ -305,6 +350,8 @@ out:
     free(Z2);
     free(S);
     free(lam);
+    free(fo);
+    free(fa);
     return ok;
 }
 
@@ -388,34 +435,109 @@ static double oop_angle(const double* ri, const double* rj, const double* rk, co
 /* ------------------------------------------------------------------------- */
 /* the evaluation                                                             */
 /* ------------------------------------------------------------------------- */
-int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad, double* E_total, double* terms_out,
-                      double* grad, double* q_out, double* cn_out, double* gcomp /* [NGC*3n] or NULL */) {
+/* D4 ATM over bonded triplets: calcATM / calcATMGradient
+ * ff_workspace_gfnff.cpp:1526-1658, with the C6 of the current D4 weights.
+ * The energy uses +c9 and the gradient is assembled from -c9 with the
+ * reference's -(dgij + dgik) orientation.  Worksharing over the triplets when
+ * called inside a parallel region (linear per-thread ranges). */
+static void atm_terms(const gfnff_params_view* P, const double* x, const double* gw, const double* dgw,
+                      int want_grad, terms_t* Tp, gacc_t* Gp) {
+#define T (*Tp)
+#define G (*Gp)
+#pragma omp for schedule(static) nowait
+    for (int t = 0; t < P->natm; ++t) {
+        int i = P->atm_ijk[3 * t], j = P->atm_ijk[3 * t + 1], k = P->atm_ijk[3 * t + 2];
+        double c6ij, c6ik, c6jk, d1, d2;
+        oracle_c6(P, gw, dgw, i, j, &c6ij, &d1, &d2);
+        oracle_c6(P, gw, dgw, i, k, &c6ik, &d1, &d2);
+        oracle_c6(P, gw, dgw, j, k, &c6jk, &d1, &d2);
+        double c9 = 1.0 * sqrt(fabs(c6ij * c6ik * c6jk));
+        double vij[3], vik[3], vjk[3];
+        sub3(x + 3 * j, x + 3 * i, vij);
+        sub3(x + 3 * k, x + 3 * i, vik);
+        sub3(x + 3 * k, x + 3 * j, vjk);
+        double rij = norm3(vij), rik = norm3(vik), rjk = norm3(vjk);
+        double r2ij = rij * rij, r2ik = rik * rik, r2jk = rjk * rjk;
+        double r0ij = 0.58 * sqrt(3.0 * P->rcov_d3[i] * P->rcov_d3[j]) + 4.80;
+        double r0ik = 0.58 * sqrt(3.0 * P->rcov_d3[i] * P->rcov_d3[k]) + 4.80;
+        double r0jk = 0.58 * sqrt(3.0 * P->rcov_d3[j] * P->rcov_d3[k]) + 4.80;
+        double rijk = rij * rik * rjk, r2ijk = r2ij * r2ik * r2jk, r3ijk = rijk * r2ijk, r5ijk = r2ijk * r3ijk;
+        double tmp = pow(r0ij * r0ik * r0jk / rijk, 14.0 / 3.0);
+        double fdmp = 1.0 / (1.0 + 6.0 * tmp);
+        double Aa = r2ij + r2jk - r2ik, Bb = r2ij + r2ik - r2jk, Cc = r2ik + r2jk - r2ij;
+        double ang = (0.375 * Aa * Bb * Cc / r2ijk + 1.0) / r3ijk;
+        double sc = P->atm_scale[t];
+        T.e[GFNFF_E_ATM] += ang * fdmp * c9 / 3.0 * sc;
+        if (want_grad) {
+            double dfdmp = -2.0 * 14.0 * tmp * fdmp * fdmp;
+            double dij = -0.375 * (pow(r2ij, 3) + pow(r2ij, 2) * (r2jk + r2ik) +
+                                   r2ij * (3.0 * pow(r2jk, 2) + 2.0 * r2jk * r2ik + 3.0 * pow(r2ik, 2)) -
+                                   5.0 * pow(r2jk - r2ik, 2) * (r2jk + r2ik)) / r5ijk;
+            double dik = -0.375 * (pow(r2ik, 3) + pow(r2ik, 2) * (r2jk + r2ij) +
+                                   r2ik * (3.0 * pow(r2jk, 2) + 2.0 * r2jk * r2ij + 3.0 * pow(r2ij, 2)) -
+                                   5.0 * pow(r2jk - r2ij, 2) * (r2jk + r2ij)) / r5ijk;
+            double djk = -0.375 * (pow(r2jk, 3) + pow(r2jk, 2) * (r2ik + r2ij) +
+                                   r2jk * (3.0 * pow(r2ik, 2) + 2.0 * r2ik * r2ij + 3.0 * pow(r2ij, 2)) -
+                                   5.0 * pow(r2ik - r2ij, 2) * (r2ik + r2ij)) / r5ijk;
+            double pre = -c9 * sc / 3.0;
+            double a1 = pre * (-dij * fdmp + ang * dfdmp) / r2ij, a2 = pre * (-dik * fdmp + ang * dfdmp) / r2ik,
+                   a3 = pre * (-djk * fdmp + ang * dfdmp) / r2jk;
+            double gi[3], gj[3], gk[3];
+            for (int c = 0; c < 3; ++c) {
+                gi[c] = -(a1 * vij[c] + a2 * vik[c]);
+                gj[c] = a1 * vij[c] - a3 * vjk[c];
+                gk[c] = a2 * vik[c] + a3 * vjk[c];
+            }
+            gadd(&G, GAT, i, gi[0], gi[1], gi[2]);
+            gadd(&G, GAT, j, gj[0], gj[1], gj[2]);
+            gadd(&G, GAT, k, gk[0], gk[1], gk[2]);
+        }
+    }
+#undef T
+#undef G
+}
+
+/* One evaluation.  nthr > 1 runs the FFWorkspace threading scheme (no
+ * per-component gradients then); q_in, when given, replaces the EEQ solve by
+ * the caller's charges (used to check terms at sizes whose dense solve the
+ * CPU cannot afford; the charges are then checked separately). */
+static int eval_impl(const gfnff_params_view* P, const double* x, int want_grad, int nthr, const double* q_in,
+                     double* E_total, double* terms_out, double* grad, double* q_out, double* cn_out,
+                     double* gcomp /* [NGC*3n] or NULL */) {
     const int n = P->natoms;
     int status = 0;
+    if (nthr > 1 && gcomp) return 1;
     double *cn_raw = calloc(n, 8), *cn = calloc(n, 8), *dlog = calloc(n, 8), *gw = calloc(7 * (size_t)n, 8),
-           *dgw = calloc(7 * (size_t)n, 8), *q = calloc(n, 8), *dEdcn = calloc(n, 8), *g = calloc(3 * (size_t)n, 8);
-    terms_t T;
-    memset(&T, 0, sizeof(T));
-    gacc_t G;
-    G.n = n;
-    G.g = g;
-    for (int c = 0; c < NGC; ++c) G.gc[c] = gcomp ? gcomp + (size_t)c * 3 * n : NULL;
+           *dgw = calloc(7 * (size_t)n, 8), *q = calloc(n, 8), *dEdcn_m = calloc(n, 8), *g = calloc(3 * (size_t)n, 8);
+    terms_t Tm;
+    memset(&Tm, 0, sizeof(Tm));
+    gacc_t Gm;
+    Gm.n = n;
+    Gm.g = g;
+    for (int c = 0; c < NGC; ++c) Gm.gc[c] = gcomp ? gcomp + (size_t)c * 3 * n : NULL;
     if (gcomp) memset(gcomp, 0, sizeof(double) * NGC * 3 * (size_t)n);
-    double* dEdcn_bond = calloc(n, 8);
-    if (!cn_raw || !cn || !dlog || !gw || !dgw || !q || !dEdcn || !g || !dEdcn_bond) {
+    double* dEdcn_bond_m = calloc(n, 8);
+    double* hbcn = NULL;
+    if (!cn_raw || !cn || !dlog || !gw || !dgw || !q || !dEdcn_m || !g || !dEdcn_bond_m) {
         status = 1;
         goto done;
     }
 
-    oracle_cn(P, x, 6.0, cn_raw, cn, dlog);
+    oracle_cn(P, x, 6.0, cn_raw, cn, dlog, nthr);
     oracle_gw(P, cn, gw, dgw);
-    if (!oracle_eeq(P, x, cn, q)) {
+    if (q_in) {
+        memcpy(q, q_in, sizeof(double) * n);
+    } else if (!oracle_eeq(P, x, cn, q, nthr)) {
         status = 3;
         goto done;
     }
 
     /* --- HB coordination numbers for X-H bonds: ff_workspace_gfnff.cpp:46-97 */
-    double* hbcn = calloc(n, 8);
+    hbcn = calloc(n, 8);
+    if (!hbcn) {
+        status = 1;
+        goto done;
+    }
     for (int e = 0; e < P->nhbcn; ++e) {
         int H = P->hbcn_pair[2 * e], B = P->hbcn_pair[2 * e + 1];
         double d[3];
@@ -428,7 +550,46 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
         hbcn[H] += 0.5 * (1.0 + erf(arg));
     }
 
+    /* Term section: FFWorkspace's scheme (ff_workspace.cpp:127-162, 369-413).
+       Every term list is cut into linear per-thread ranges (schedule(static)),
+       the O(N^2) pair loops into striped rows; each thread accumulates its own
+       energies, gradient and dE/dCN, and the accumulators are reduced in thread
+       order after the region.  With one thread the accumulators are the
+       outputs themselves. */
+    for (int h = 0; h < P->nhb; ++h)
+        if (P->hb_nbC_off[h + 1] - P->hb_nbC_off[h] > 16) {
+            status = 1;
+            goto done;
+        }
+    const int NT = nthr > 1 ? nthr : 1;
+    double* tl = NULL;
+    terms_t* tlT = NULL;
+    if (NT > 1) {
+        tl = calloc((size_t)NT * 5 * n, 8);
+        tlT = calloc(NT, sizeof(terms_t));
+        if (!tl || !tlT) {
+            free(tl);
+            free(tlT);
+            status = 1;
+            goto done;
+        }
+    }
+#pragma omp parallel num_threads(NT) if (NT > 1)
+    {
+    const int tid = THREAD_ID();
+    terms_t T;
+    memset(&T, 0, sizeof(T));
+    gacc_t G = Gm;
+    double* dEdcn = dEdcn_m;
+    double* dEdcn_bond = dEdcn_bond_m;
+    if (NT > 1) {
+        G.g = tl + (size_t)tid * 5 * n;
+        for (int c = 0; c < NGC; ++c) G.gc[c] = NULL;
+        dEdcn = G.g + 3 * (size_t)n;
+        dEdcn_bond = dEdcn + n;
+    }
     /* --- bonds: calcBonds ff_workspace_gfnff.cpp:104-180 */
+#pragma omp for schedule(static) nowait
     for (int b = 0; b < P->nbond; ++b) {
         int i = P->bond_ij[2 * b], j = P->bond_ij[2 * b + 1];
         const double* p = P->bond_par + 8 * b;
@@ -470,9 +631,9 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
             dEdcn_bond[j] += yy * p[5] * p[3];
         }
     }
-    free(hbcn);
 
     /* --- angles: calcAngles ff_workspace_gfnff.cpp:187-275 */
+#pragma omp for schedule(static) nowait
     for (int a = 0; a < P->nangle; ++a) {
         int i = P->angle_ijk[3 * a], j = P->angle_ijk[3 * a + 1], k = P->angle_ijk[3 * a + 2];
         double fc = P->angle_par[2 * a], th0 = P->angle_par[2 * a + 1];
@@ -521,6 +682,7 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
     }
 
     /* --- torsions (primary + extra): ff_workspace_gfnff.cpp:282-482 */
+#pragma omp for schedule(static) nowait
     for (int t = 0; t < P->ntors; ++t) {
         const int* id = P->tors_ijkl + 4 * t;
         const double* p = P->tors_par + 4 * t;
@@ -561,6 +723,7 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
     }
     /* --- inversions: calcInversions ff_workspace_gfnff.cpp:489-604 (damping
        over the three distances from the first neighbour j) */
+#pragma omp for schedule(static) nowait
     for (int v = 0; v < P->ninv; ++v) {
         const int* id = P->inv_ijkl + 4 * v;
         const double* p = P->inv_par + 3 * v;
@@ -608,6 +771,7 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
     }
 
     /* --- triple-bond torsions: calcSTorsions ff_workspace_gfnff.cpp:611-648 (undamped) */
+#pragma omp for schedule(static) nowait
     for (int t = 0; t < P->nstors; ++t) {
         const int* id = P->stors_ijkl + 4 * t;
         double eh = P->stors_par[t];
@@ -622,6 +786,7 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
     }
 
     /* --- D4 dispersion pairs: calcD4Dispersion ff_workspace_gfnff.cpp:715-767 */
+#pragma omp for schedule(static, 1) nowait
     for (int i = 0; i < n; ++i)
         for (int j = i + 1; j < n; ++j) {
             double d[3];
@@ -651,6 +816,7 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
     /* --- repulsion: bonded ff_workspace_gfnff.cpp:774-807, non-bonded :814-842
        (non-bonded pair set = all non-bonded pairs within 20 Bohr,
         gfnff_method.cpp:9781-9872) */
+#pragma omp for schedule(static) nowait
     for (int b = 0; b < P->nbrep; ++b) {
         int i = P->brep_ij[2 * b], j = P->brep_ij[2 * b + 1];
         double al = P->brep_par[2 * b], ab = P->brep_par[2 * b + 1];
@@ -666,6 +832,7 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
             gadd(&G, GR, j, -dEdr * d[0] / r, -dEdr * d[1] / r, -dEdr * d[2] / r);
         }
     }
+#pragma omp for schedule(static, 1) nowait
     for (int i = 0; i < n; ++i) {
         int zi = P->Z[i];
         for (int j = i + 1; j < n; ++j) {
@@ -708,6 +875,7 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
     }
 
     /* --- Coulomb pairs: calcCoulomb ff_workspace_gfnff.cpp:850-898 (r_cut 100) */
+#pragma omp for schedule(static, 1) nowait
     for (int i = 0; i < n; ++i)
         for (int j = i + 1; j < n; ++j) {
             double d[3];
@@ -725,7 +893,7 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
             }
         }
     /* self energy: ff_workspace.cpp:426-449 */
-    {
+    if (tid == 0) {
         double Een = 0.0, Es = 0.0;
         for (int i = 0; i < n; ++i) {
             double chi = P->chi_base[i] + P->cnf[i] * sqrt(fmax(cn[i], 0.0));
@@ -736,6 +904,7 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
     }
 
     /* --- hydrogen bonds: calcHydrogenBonds ff_workspace_gfnff.cpp:938-1404 */
+#pragma omp for schedule(static) nowait
     for (int h = 0; h < P->nhb; ++h) {
         int A = P->hb_ahb[3 * h], Hh = P->hb_ahb[3 * h + 1], B = P->hb_ahb[3 * h + 2];
         int cs = P->hb_case[h];
@@ -781,7 +950,7 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
         double outl_lp = 1.0, lpd = 0.0, lpv[3] = {0, 0, 0}, lp[3] = {pB[0], pB[1], pB[2]}, xlp = 0.0;
         int nlp = 0;
         if (cs == 4) {
-            lpd = 0.50 - 0.018 * P->repz[B];
+            lpd = P->hb_lpd[B];
             for (int k = nb0; k < nb1; ++k)
                 for (int c = 0; c < 3; ++c) lpv[c] += x[3 * P->hb_nbB_idx[k] + c] - pB[c];
             double vn = norm3(lpv);
@@ -823,10 +992,7 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
             }
             c0i = P->hb_nbC_off[h];
             c1i = P->hb_nbC_off[h + 1];
-            if (c1i - c0i > 16) {
-                status = 1;
-                goto done;
-            }
+            /* list lengths were checked (<= 16) before the term section */
             for (int k = c0i; k < c1i; ++k) {
                 int D = P->hb_nbC_idx[k];
                 int ok;
@@ -1000,6 +1166,7 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
        powered ratios); these are not the derivatives of f_s and f_l, so this
        gradient differs from finite differences of the energy along r_XB,
        exactly as the reference's does.  The out-of-line part is exact. */
+#pragma omp for schedule(static) nowait
     for (int b = 0; b < P->nxb; ++b) {
         int A = P->xb_axb[3 * b], X = P->xb_axb[3 * b + 1], B = P->xb_axb[3 * b + 2];
         const double* p = P->xb_par + 3 * b;
@@ -1035,59 +1202,11 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
         gadd(&G, GX, B, gxb * vxb[0] + gab * vab[0], gxb * vxb[1] + gab * vab[1], gxb * vxb[2] + gab * vab[2]);
     }
 
-    /* --- D4 ATM over bonded triplets: calcATM / calcATMGradient
-       ff_workspace_gfnff.cpp:1526-1658.  The energy uses +c9 and the gradient
-       -c9, as the reference does. */
-    for (int t = 0; t < P->natm; ++t) {
-        int i = P->atm_ijk[3 * t], j = P->atm_ijk[3 * t + 1], k = P->atm_ijk[3 * t + 2];
-        double c6ij, c6ik, c6jk, d1, d2;
-        oracle_c6(P, gw, dgw, i, j, &c6ij, &d1, &d2);
-        oracle_c6(P, gw, dgw, i, k, &c6ik, &d1, &d2);
-        oracle_c6(P, gw, dgw, j, k, &c6jk, &d1, &d2);
-        double c9 = 1.0 * sqrt(fabs(c6ij * c6ik * c6jk));
-        double vij[3], vik[3], vjk[3];
-        sub3(x + 3 * j, x + 3 * i, vij);
-        sub3(x + 3 * k, x + 3 * i, vik);
-        sub3(x + 3 * k, x + 3 * j, vjk);
-        double rij = norm3(vij), rik = norm3(vik), rjk = norm3(vjk);
-        double r2ij = rij * rij, r2ik = rik * rik, r2jk = rjk * rjk;
-        double r0ij = 0.58 * sqrt(3.0 * P->rcov_d3[i] * P->rcov_d3[j]) + 4.80;
-        double r0ik = 0.58 * sqrt(3.0 * P->rcov_d3[i] * P->rcov_d3[k]) + 4.80;
-        double r0jk = 0.58 * sqrt(3.0 * P->rcov_d3[j] * P->rcov_d3[k]) + 4.80;
-        double rijk = rij * rik * rjk, r2ijk = r2ij * r2ik * r2jk, r3ijk = rijk * r2ijk, r5ijk = r2ijk * r3ijk;
-        double tmp = pow(r0ij * r0ik * r0jk / rijk, 14.0 / 3.0);
-        double fdmp = 1.0 / (1.0 + 6.0 * tmp);
-        double Aa = r2ij + r2jk - r2ik, Bb = r2ij + r2ik - r2jk, Cc = r2ik + r2jk - r2ij;
-        double ang = (0.375 * Aa * Bb * Cc / r2ijk + 1.0) / r3ijk;
-        double sc = P->atm_scale[t];
-        T.e[GFNFF_E_ATM] += ang * fdmp * c9 / 3.0 * sc;
-        if (want_grad) {
-            double dfdmp = -2.0 * 14.0 * tmp * fdmp * fdmp;
-            double dij = -0.375 * (pow(r2ij, 3) + pow(r2ij, 2) * (r2jk + r2ik) +
-                                   r2ij * (3.0 * pow(r2jk, 2) + 2.0 * r2jk * r2ik + 3.0 * pow(r2ik, 2)) -
-                                   5.0 * pow(r2jk - r2ik, 2) * (r2jk + r2ik)) / r5ijk;
-            double dik = -0.375 * (pow(r2ik, 3) + pow(r2ik, 2) * (r2jk + r2ij) +
-                                   r2ik * (3.0 * pow(r2jk, 2) + 2.0 * r2jk * r2ij + 3.0 * pow(r2ij, 2)) -
-                                   5.0 * pow(r2jk - r2ij, 2) * (r2jk + r2ij)) / r5ijk;
-            double djk = -0.375 * (pow(r2jk, 3) + pow(r2jk, 2) * (r2ik + r2ij) +
-                                   r2jk * (3.0 * pow(r2ik, 2) + 2.0 * r2ik * r2ij + 3.0 * pow(r2ij, 2)) -
-                                   5.0 * pow(r2ik - r2ij, 2) * (r2ik + r2ij)) / r5ijk;
-            double pre = -c9 * sc / 3.0;
-            double a1 = pre * (-dij * fdmp + ang * dfdmp) / r2ij, a2 = pre * (-dik * fdmp + ang * dfdmp) / r2ik,
-                   a3 = pre * (-djk * fdmp + ang * dfdmp) / r2jk;
-            double gi[3], gj[3], gk[3];
-            for (int c = 0; c < 3; ++c) {
-                gi[c] = -(a1 * vij[c] + a2 * vik[c]);
-                gj[c] = a1 * vij[c] - a3 * vjk[c];
-                gk[c] = a2 * vik[c] + a3 * vjk[c];
-            }
-            gadd(&G, GAT, i, gi[0], gi[1], gi[2]);
-            gadd(&G, GAT, j, gj[0], gj[1], gj[2]);
-            gadd(&G, GAT, k, gk[0], gk[1], gk[2]);
-        }
-    }
+    /* --- D4 ATM over bonded triplets (atm_terms) */
+    atm_terms(P, x, gw, dgw, want_grad, &T, &G);
 
     /* --- bonded ATM: calcBATM ff_workspace_gfnff.cpp:1665-1739 */
+#pragma omp for schedule(static) nowait
     for (int t = 0; t < P->nbatm; ++t) {
         int i = P->batm_ijk[3 * t], j = P->batm_ijk[3 * t + 1], k = P->batm_ijk[3 * t + 2];
         const double* zb = P->batm_par + 3 * t;
@@ -1129,6 +1248,27 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
         }
     }
 
+    if (NT > 1) {
+        tlT[tid] = T;
+    } else {
+        Tm = T;
+    }
+    } /* end of the parallel term section */
+    if (NT > 1) {
+        /* fixed-order reduce over the thread accumulators */
+        for (int t = 0; t < NT; ++t) {
+            for (int k = 0; k < GFNFF_NTERMS; ++k) Tm.e[k] += tlT[t].e[k];
+            const double* a = tl + (size_t)t * 5 * n;
+            for (size_t k = 0; k < 3 * (size_t)n; ++k) g[k] += a[k];
+            for (int k = 0; k < n; ++k) {
+                dEdcn_m[k] += a[3 * (size_t)n + k];
+                dEdcn_bond_m[k] += a[4 * (size_t)n + k];
+            }
+        }
+        free(tl);
+        free(tlT);
+    }
+
     /* --- CN chain rule + Coulomb term 1b: ff_workspace.cpp:459-507 */
     if (want_grad) {
         double* v = calloc(n, 8);
@@ -1136,9 +1276,28 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
         for (int i = 0; i < n; ++i) {
             double c = fmax(cn[i], 0.0);
             vq[i] = q[i] * P->cnf[i] / (2.0 * sqrt(c) + 1e-16);
-            v[i] = dEdcn[i] - vq[i];
+            v[i] = dEdcn_m[i] - vq[i];
         }
         double cut2 = 36.0;
+        double* tg = (NT > 1) ? calloc((size_t)NT * 3 * n, 8) : NULL;
+        if (NT > 1 && !tg) {
+            free(v);
+            free(vq);
+            status = 1;
+            goto done;
+        }
+        /* pairs in striped rows, per-thread gradients reduced in thread order */
+#pragma omp parallel num_threads(NT) if (NT > 1)
+        {
+        const int tid = THREAD_ID();
+        gacc_t G = Gm;
+        const double* dEdcn = dEdcn_m;
+        const double* dEdcn_bond = dEdcn_bond_m;
+        if (NT > 1) {
+            G.g = tg + (size_t)tid * 3 * n;
+            for (int c = 0; c < NGC; ++c) G.gc[c] = NULL;
+        }
+#pragma omp for schedule(static, 1) nowait
         for (int i = 0; i < n; ++i)
             for (int j = 0; j < i; ++j) {
                 double d[3];
@@ -1170,15 +1329,21 @@ int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad
                     G.gc[GC][3 * j] += sq * ux, G.gc[GC][3 * j + 1] += sq * uy, G.gc[GC][3 * j + 2] += sq * uz;
                 }
             }
+        }
+        if (NT > 1) {
+            for (int t = 0; t < NT; ++t)
+                for (size_t k = 0; k < 3 * (size_t)n; ++k) g[k] += tg[(size_t)t * 3 * n + k];
+            free(tg);
+        }
         free(v);
         free(vq);
     }
 
     {
         double tot = 0.0;
-        for (int k = 0; k < GFNFF_NTERMS; ++k) tot += T.e[k];
+        for (int k = 0; k < GFNFF_NTERMS; ++k) tot += Tm.e[k];
         if (E_total) *E_total = tot;
-        if (terms_out) memcpy(terms_out, T.e, sizeof(T.e));
+        if (terms_out) memcpy(terms_out, Tm.e, sizeof(Tm.e));
         if (grad) memcpy(grad, g, sizeof(double) * 3 * (size_t)n);
         if (q_out) memcpy(q_out, q, sizeof(double) * n);
         if (cn_out) memcpy(cn_out, cn, sizeof(double) * n);
@@ -1190,8 +1355,78 @@ done:
     free(gw);
     free(dgw);
     free(q);
-    free(dEdcn);
-    free(dEdcn_bond);
+    free(dEdcn_m);
+    free(dEdcn_bond_m);
     free(g);
+    free(hbcn);
+    return status;
+}
+
+/* The reference's serial evaluation (one thread). */
+int gfnff_oracle_eval(const gfnff_params_view* P, const double* x, int want_grad, double* E_total, double* terms_out,
+                      double* grad, double* q_out, double* cn_out, double* gcomp /* [NGC*3n] or NULL */) {
+    return eval_impl(P, x, want_grad, 1, NULL, E_total, terms_out, grad, q_out, cn_out, gcomp);
+}
+
+/* Threaded evaluation (FFWorkspace scheme, nthr threads); q_in optional (see
+ * eval_impl).  The CPU baseline and large-size checks use it. */
+int gfnff_oracle_eval_mt(const gfnff_params_view* P, const double* x, int want_grad, int nthr, const double* q_in,
+                         double* E_total, double* terms_out, double* grad, double* q_out, double* cn_out) {
+    return eval_impl(P, x, want_grad, nthr, q_in, E_total, terms_out, grad, q_out, cn_out, NULL);
+}
+
+/* The ATM term alone at geometry x: with gw_in (natoms x 7 D4 weights) held
+ * fixed, or with the weights of the CN at x when gw_in is NULL.  E receives
+ * the ATM energy, grad (natoms x 3, may be NULL) the reference's analytic ATM
+ * gradient, which does not include the CN dependence of C6. */
+int gfnff_oracle_atm(const gfnff_params_view* P, const double* x, const double* gw_in, double* E, double* grad) {
+    const int n = P->natoms;
+    double *cn_raw = calloc(n, 8), *cn = calloc(n, 8), *dlog = calloc(n, 8), *gw = calloc(7 * (size_t)n, 8),
+           *dgw = calloc(7 * (size_t)n, 8), *g = calloc(3 * (size_t)n, 8);
+    int status = 0;
+    if (!cn_raw || !cn || !dlog || !gw || !dgw || !g) {
+        status = 1;
+    } else {
+        oracle_cn(P, x, 6.0, cn_raw, cn, dlog, 1);
+        oracle_gw(P, cn, gw, dgw);
+        if (gw_in) memcpy(gw, gw_in, sizeof(double) * 7 * (size_t)n);
+        terms_t T;
+        memset(&T, 0, sizeof(T));
+        gacc_t G;
+        G.n = n;
+        G.g = g;
+        for (int c = 0; c < NGC; ++c) G.gc[c] = NULL;
+        atm_terms(P, x, gw, dgw, grad != NULL, &T, &G);
+        *E = T.e[GFNFF_E_ATM];
+        if (grad) memcpy(grad, g, sizeof(double) * 3 * (size_t)n);
+    }
+    free(cn_raw);
+    free(cn);
+    free(dlog);
+    free(gw);
+    free(dgw);
+    free(g);
+    return status;
+}
+
+/* CN (log-capped) and D4 Gaussian weights (natoms x 7) at geometry x. */
+int gfnff_oracle_cn_gw(const gfnff_params_view* P, const double* x, double* cn_out, double* gw_out) {
+    const int n = P->natoms;
+    double *cn_raw = calloc(n, 8), *cn = calloc(n, 8), *dlog = calloc(n, 8), *gw = calloc(7 * (size_t)n, 8),
+           *dgw = calloc(7 * (size_t)n, 8);
+    int status = 0;
+    if (!cn_raw || !cn || !dlog || !gw || !dgw) {
+        status = 1;
+    } else {
+        oracle_cn(P, x, 6.0, cn_raw, cn, dlog, 1);
+        oracle_gw(P, cn, gw, dgw);
+        if (cn_out) memcpy(cn_out, cn, sizeof(double) * n);
+        if (gw_out) memcpy(gw_out, gw, sizeof(double) * 7 * (size_t)n);
+    }
+    free(cn_raw);
+    free(cn);
+    free(dlog);
+    free(gw);
+    free(dgw);
     return status;
 }

@@ -56,6 +56,26 @@ class Oracle:
             out["comp"] = {k: gc.reshape(len(GCOMP), n, 3)[i] for i, k in enumerate(GCOMP)}
         return out
 
+    def eval_mt(self, view: ParamsView, xyz_bohr: np.ndarray, nthreads: int, grad: bool = True, q_in=None):
+        """Threaded oracle (FFWorkspace scheme); q_in replaces the EEQ solve."""
+        lib = self.lib
+        lib.gfnff_oracle_eval_mt.argtypes = [C.POINTER(ParamsView), C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        n = view.natoms
+        x = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1)
+        qi = None if q_in is None else np.ascontiguousarray(q_in, dtype=np.float64)
+        E = C.c_double()
+        terms = np.zeros(NTERMS)
+        g = np.zeros(3 * n)
+        q = np.zeros(n)
+        cn = np.zeros(n)
+        st = lib.gfnff_oracle_eval_mt(C.byref(view), x.ctypes.data, 1 if grad else 0, int(nthreads),
+                                      qi.ctypes.data if qi is not None else None, C.byref(E), terms.ctypes.data,
+                                      g.ctypes.data, q.ctypes.data, cn.ctypes.data)
+        if st != 0:
+            raise RuntimeError(f"oracle status {st}")
+        return {"E": E.value, "terms": terms, "grad": g.reshape(n, 3), "q": q, "cn": cn}
+
 
 @pytest.fixture(scope="session")
 def oracle():
@@ -68,3 +88,38 @@ def gpu_available() -> bool:
         return load_library().gfnff_device_count() > 0
     except Exception:
         return False
+
+
+class HipRuntime:
+    """Minimal ctypes access to the HIP runtime the engine library links
+    (device buffers for the device-resident entry points in GPU tests)."""
+
+    def __init__(self):
+        from curcuma_amd._lib import load_library
+        load_library()  # the engine's runtime is the one resolved below
+        self.rt = C.CDLL("libamdhip64.so.7")
+        self.rt.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.rt.hipFree.argtypes = [C.c_void_p]
+        self.rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.rt.hipDeviceSynchronize.argtypes = []
+
+    def upload(self, a: np.ndarray) -> C.c_void_p:
+        a = np.ascontiguousarray(a)
+        p = C.c_void_p()
+        assert self.rt.hipMalloc(C.byref(p), max(a.nbytes, 8)) == 0
+        assert self.rt.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0  # hipMemcpyHostToDevice
+        return p
+
+    def alloc(self, nbytes: int) -> C.c_void_p:
+        p = C.c_void_p()
+        assert self.rt.hipMalloc(C.byref(p), max(nbytes, 8)) == 0
+        return p
+
+    def download(self, p: C.c_void_p, shape, dtype=np.float64) -> np.ndarray:
+        out = np.zeros(shape, dtype=dtype)
+        assert self.rt.hipDeviceSynchronize() == 0
+        assert self.rt.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0  # hipMemcpyDeviceToHost
+        return out
+
+    def free(self, p: C.c_void_p) -> None:
+        self.rt.hipFree(p)

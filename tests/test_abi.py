@@ -85,3 +85,93 @@ def test_topology_of_water_and_alkane():
     assert m.setMolecule(Mol(Z, x))
     hyb, pi, bonds = m.topology()
     assert len(bonds) == len(Z) - 1 and m.params_view().nfrag == 1
+
+
+def test_energy_decomposition_keys_and_sum(oracle):
+    """getEnergyDecomposition returns exactly the reference's ten keys
+    (qm_methods/gfnff_method.cpp:152-185).  As in the reference, the ten keys
+    plus getRepulsionEnergy (bonded + non-bonded, gfnff_method.cpp:10521-10525)
+    and the triple-bond torsions (FFEnergyComponents::stors) make the total
+    (FFEnergyComponents::total, ff_workspace.h:84-89).  The per-term energies
+    are the oracle's (no GPU here); the mapping is the host logic under test."""
+    from conftest import BOHR, load_fixture
+    from curcuma_amd._lib import NTERMS
+    fx = load_fixture("complex")
+    Z = np.array(fx["Z"])
+    m = GFNFFMethod()
+    assert m.setMolecule(Mol(Z, np.array(fx["xyz_bohr"]) * BOHR))
+    r = oracle.eval(m.params_view(), np.array(fx["xyz_bohr"]), grad=False)
+    m._terms = np.array(r["terms"], dtype=np.float64)
+    m._energy = r["E"]
+    assert len(m._terms) == NTERMS
+    d = m.getEnergyDecomposition()
+    assert list(d) == ["Bond", "Angle", "Torsion", "Inversion", "Dispersion", "Coulomb", "HBond", "XBond", "ATM",
+                       "BATM"]
+    total = sum(d.values()) + m.getRepulsionEnergy() + m.getSTorsionEnergy()
+    assert abs(total - r["E"]) <= 1e-12 * abs(r["E"])
+    assert d["Inversion"] != 0.0 and d["HBond"] != 0.0 and d["ATM"] != 0.0
+    assert m.getDihedralEnergy() == d["Torsion"] and m.getBondEnergy() == d["Bond"]
+    assert m.getCoulombEnergy() == d["Coulomb"] and m.getDispersionEnergy() == d["Dispersion"]
+
+
+def test_threaded_oracle_agrees_with_serial(oracle):
+    """The FFWorkspace-threaded oracle (per-thread accumulators, fixed-order
+    reduce) used as the CPU baseline gives the serial oracle's results."""
+    import ctypes as C
+    from conftest import BOHR, load_fixture
+    fx = load_fixture("polymer")
+    Z = np.array(fx["Z"])
+    x = np.array(fx["xyz_bohr"])
+    m = GFNFFMethod()
+    assert m.setMolecule(Mol(Z, x * BOHR))
+    v = m.params_view()
+    x = (x * BOHR) * (1.0 / BOHR)
+    ref = oracle.eval(v, x)
+    mt = oracle.eval_mt(v, x, nthreads=4)
+    assert abs(mt["E"] - ref["E"]) <= 1e-11
+    assert np.abs(mt["grad"] - ref["grad"]).max() <= 1e-12
+    assert np.abs(mt["q"] - ref["q"]).max() <= 1e-12
+    assert np.abs(mt["terms"] - ref["terms"]).max() <= 1e-11
+    # the same with the charges handed in (the path for sizes whose dense solve is skipped)
+    mq = oracle.eval_mt(v, x, nthreads=4, q_in=ref["q"])
+    assert abs(mq["E"] - mt["E"]) <= 1e-11 and np.abs(mq["grad"] - mt["grad"]).max() <= 1e-12
+
+
+def test_coincident_atoms_are_refused_at_setup():
+    """Zero distance between two atoms: the reference's Phase-1 EEQ falls back
+    to placeholder charges (eeq_solver.cpp:2843-2846) and setMolecule refuses
+    the molecule with an error (qm_methods/gfnff_method.cpp:43-51)."""
+    from curcuma_amd.workloads import water_cluster
+    Z, x = water_cluster(4, seed=1)
+    for a, b in ((5, 8), (3, 0)):
+        x2 = x.copy()
+        x2[a] = x2[b]
+        m = GFNFFMethod()
+        assert not m.setMolecule(Mol(Z, x2))
+        assert m.hasError() and "coincide" in m.getErrorMessage()
+    m = GFNFFMethod()
+    assert m.setMolecule(Mol(Z, x)), m.getErrorMessage()
+
+
+def test_eeq_not_positive_definite_falls_back_in_the_oracle(oracle):
+    """H2 at its fixture geometry: the EEQ matrix A is indefinite (eigenvalues
+    -0.064 and 1.895), so the reference's Cholesky fails and its augmented LU
+    gives the charges (eeq_solver.cpp:2143-2165); by symmetry they are zero.
+    The oracle takes the same fallback; the engine's LU fallback is checked
+    against it on the GPU."""
+    from math import erf, pi, sqrt
+    from conftest import BOHR, load_fixture
+    fx = load_fixture("HH")
+    xb = np.array(fx["xyz_bohr"])
+    m = GFNFFMethod()
+    assert m.setMolecule(Mol(np.array(fx["Z"]), xb * BOHR))
+    v = m.params_view()
+    gam = np.ctypeslib.as_array(v.gam_eff, (2,))
+    alp = np.ctypeslib.as_array(v.alpeeq, (2,))
+    r = np.linalg.norm(xb[0] - xb[1])
+    a01 = erf(r / sqrt(alp[0] + alp[1])) / r
+    A = np.array([[gam[0] + sqrt(2 / pi) / sqrt(alp[0]), a01], [a01, gam[1] + sqrt(2 / pi) / sqrt(alp[1])]])
+    assert np.linalg.eigvalsh(A)[0] < 0.0
+    res = oracle.eval(v, xb)
+    assert np.abs(res["q"]).max() < 1e-14
+    assert abs(sum(fx["energy_components"].values()) - res["E"] + res["terms"][10]) <= 1e-6

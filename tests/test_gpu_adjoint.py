@@ -25,10 +25,16 @@ def grad_at(m, theta):
     return m.getGradient().copy()
 
 
-@pytest.mark.parametrize("case", ["alkane30", "water27"])
+@pytest.mark.parametrize("case", ["alkane30", "water27", "complex"])
 def test_adjoint_matches_finite_differences(case):
+    """complex (H, C, N, O; 231 atoms) holds case-4 hydrogen bonds, whose
+    lone-pair distance must not follow the fitted repz scales."""
     if case == "alkane30":
         Z, xyz = alkane(30, sigma=0.05, seed=5)
+    elif case == "complex":
+        from conftest import BOHR, load_fixture
+        fx = load_fixture("complex")
+        Z, xyz = np.array(fx["Z"]), np.array(fx["xyz_bohr"]) * BOHR
     else:
         Z, xyz = water_cluster(27, seed=7)
     m = GFNFFMethod()

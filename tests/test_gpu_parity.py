@@ -44,8 +44,8 @@ def check(m, e, ref, n, etol=1e-9, gtol=1e-9):
     assert np.abs(m.getCN() - ref["cn"]).max() <= 1e-12
 
 
-@pytest.mark.parametrize("name", ["H2O", "CH4", "CH3OH", "CH3OCH3", "HCl", "triose", "HCN", "O3", "C6H6",
-                                  "acetic_acid_dimer", "caffeine", "complex"])
+@pytest.mark.parametrize("name", ["HH", "OH", "H2O", "water", "CH4", "methane", "CH3OH", "CH3OCH3", "HCl", "triose",
+                                  "HCN", "O3", "C6H6", "acetic_acid_dimer", "caffeine", "complex", "polymer"])
 def test_fixture_molecules(name, oracle):
     """Includes the pi fixtures: inversions (planar and saturated N) and
     hydrogen bonds of cases 3 (carbonyl acceptor) and 4 (sp2 N lone pair)."""
@@ -142,13 +142,22 @@ def brute_cn_lists(xb):
     return lists
 
 
-@pytest.mark.parametrize("case", ["water64", "alkane40"])
+@pytest.mark.parametrize("case", ["water64", "alkane40", "water3000", "bundle16"])
 def test_cn_neighbour_lists_bit_exact(case):
+    """water3000 (9000 atoms, 36 partner tiles) and a 16-chain alkane bundle
+    (3872 atoms) run the tile-box pruning of the CN kernel; the two small
+    cases fit one partner tile."""
     from curcuma_amd._lib import as_i32, iptr, load_library
+    from curcuma_amd.workloads import alkane_bundle
     if case == "water64":
         Z, xyz = water_cluster(64, seed=11)
+    elif case == "water3000":
+        Z, xyz = water_cluster(3000, seed=2)
+    elif case == "bundle16":
+        Z, xyz = alkane_bundle(16, 80, seed=4)
     else:
         Z, xyz = alkane(40, sigma=0.02, seed=1)
+    print(f"{case}: {len(Z)} atoms", flush=True)
     m = GFNFFMethod()
     assert m.setMolecule(Mol(Z, xyz)), m.getErrorMessage()
     m.calculateEnergy(True)
@@ -234,3 +243,32 @@ def test_pcg_on_chain_molecule(oracle):
     m, e, ref = run_both(Z, xyz, oracle)
     check(m, e, ref, len(Z))
     assert 0 < m.solverIterations() < 400
+
+
+def test_batch_cn_neighbour_lists_bit_exact():
+    """The per-molecule CN kernel of batches (k_cn_mol): every molecule's lists
+    equal the reference's builder run on that molecule alone (global atom
+    indices)."""
+    from curcuma_amd import GFNFFBatch
+    from curcuma_amd._lib import as_i32, iptr, load_library
+    from curcuma_amd.workloads import mixed_batch
+    off, Z, xyz = mixed_batch(48, 20, 400, sigma=0.03, seed=3)
+    m = GFNFFBatch()
+    assert m.setBatch(off, Z, xyz), m.getErrorMessage()
+    m.calculateEnergy(True)
+    assert not m.hasError(), m.getErrorMessage()
+    n = len(Z)
+    maxnb = 128
+    counts = as_i32(np.zeros(n))
+    idx = as_i32(np.zeros(n * maxnb))
+    assert load_library().gfnff_cn_neighbours(m.handle, iptr(counts), iptr(idx), maxnb) == 0
+    xb = np.asarray(xyz) * (1.0 / BOHR)
+    total = 0
+    for k in range(len(off) - 1):
+        a0, a1 = int(off[k]), int(off[k + 1])
+        ref = brute_cn_lists(xb[a0:a1])
+        for i in range(a0, a1):
+            got = list(idx[i * maxnb:i * maxnb + counts[i]])
+            assert got == [a0 + j for j in ref[i - a0]], (k, i)
+            total += len(got)
+    print(f"batch of {len(off) - 1} molecules, {n} atoms, {total // 2} pairs, lists identical")
