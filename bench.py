@@ -1,14 +1,20 @@
 """Benchmark: GFN-FF energy + gradient throughput on one MI355X.
 
-Workload (one "step" = one full energy+gradient evaluation): the non-periodic
-cluster of 3000 H2O (9000 atoms) on a 3.1 A cubic lattice with seeded random
-rigid orientations and +-0.05 A jitter, FP64.  Coordinates, outputs and all
-parameters are resident in HBM before the timed region starts; the host only
-enqueues kernels.
+Headline workload (one "step" = one full energy+gradient evaluation of every
+molecule): mixed_batch_4096, 4096 independent molecules (all-trans n-alkanes
+and extended glycine oligopeptides, 20-400 atoms each, sigma 0.03 A, seed 3)
+on the batched mixed-precision path.  Beside it, in the same run:
+  * the same batch on the FP64 path;
+  * the plain FP64 single-molecule path on the 3000-H2O water cluster
+    (9000 atoms), the denominator of the mixed-batched / plain-FP64 ratio;
+  * accuracy of the headline against the FP64 CPU oracle on a sample of the
+    batch's molecules, and against the FP64 batch on every atom.
+Coordinates, outputs and all parameters are resident in HBM before the timed
+region starts; the host only enqueues kernels.
 
 Prints one JSON line (see the repository's bench contract).  Extra objects:
   roofline      dominant kernel, algorithmic work per launch / measured launch time
-  cpu_baseline  the CPU oracle timed on this host on a bounded sample
+  cpu_baseline  the threaded CPU oracle timed on this host on a bounded sample
 """
 from __future__ import annotations
 
@@ -16,6 +22,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import statistics
 import sys
 import time
 from pathlib import Path
@@ -26,6 +33,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "GFN-FF E+grad atom·evals/s, mixed vs FP64 on 1 GPU at |ΔE|≤1e-6 Eh/atom"
+BOHR = 0.529177210903
 
 
 def parse():
@@ -33,47 +41,23 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="water_3000")
+    p.add_argument("--workload", default="mixed_batch_4096")
     p.add_argument("--precision", default="mixed", choices=["fp64", "mixed"],
                    help="path of the headline number; the FP64 path is always timed beside it")
+    p.add_argument("--plain-workload", default="water_3000",
+                   help="single-molecule workload timed on the plain FP64 path ('' to skip)")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-sample", default="water_400")
+    p.add_argument("--cpu-sample-mols", type=int, default=1024,
+                   help="molecules of a batch workload timed by the CPU oracle")
+    p.add_argument("--cpu-sample", default="water_400", help="CPU sample of a single-molecule workload")
+    p.add_argument("--oracle-check-mols", type=int, default=12,
+                   help="batch molecules checked against the FP64 oracle (0 to skip)")
     p.add_argument("--lookahead", type=int, default=0, help="engine option cholesky_lookahead (0/1)")
     p.add_argument("--solver", type=int, default=0, help="engine option eeq_solver (0 PCG, 1 Cholesky)")
     p.add_argument("--roofline-kernel", default="auto", help="kernel class timed for the roofline object")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="run one device copy of this size first (HBM counter calibration under rocprofv3 --pmc)")
     return p.parse_args()
-
-
-def cpu_baseline(sample: str):
-    """Times the CPU oracle (test infrastructure, single thread) on a bounded
-    sample of the same workload family and reports atom*evals/s."""
-    from curcuma_amd import GFNFFMethod, Mol
-    from curcuma_amd.workloads import water_cluster
-    sys.path.insert(0, str(ROOT / "tests"))
-    from conftest import Oracle  # noqa: E402
-
-    nmol = int(sample.split("_")[1])
-    Z, xyz = water_cluster(nmol, seed=2)
-    m = GFNFFMethod()
-    if not m.setMolecule(Mol(Z, xyz)):
-        return None
-    orc = Oracle()
-    v = m.params_view()
-    x = xyz * (1.0 / 0.529177210903)
-    t0 = time.perf_counter()
-    reps = 0
-    while True:
-        orc.eval(v, x, grad=True)
-        reps += 1
-        if time.perf_counter() - t0 > 10.0 and reps >= 3:
-            break
-    dt = (time.perf_counter() - t0) / reps
-    return {"value": len(Z) / dt, "unit": "atom*evals/s", "cores": 1, "kind": "port",
-            "sample": f"water cluster of {nmol} H2O ({len(Z)} atoms) from the same generator, "
-                      f"{reps} FP64 energy+gradient evaluation(s) of the single-threaded CPU oracle, "
-                      f"{dt:.3f} s each"}
 
 
 WORKLOAD_TEXT = {
@@ -86,6 +70,85 @@ WORKLOAD_TEXT = {
                         "uniform in [20, 400] rounded down to the family size (3n+2 / 7n+3), sigma 0.03 A, "
                         "seed 3, one launch per kernel",
 }
+
+
+def batch_workloads():
+    from curcuma_amd.workloads import alkane_batch, mixed_batch
+    return {"alkane_batch_4096": lambda: alkane_batch(4096, 20, 400, sigma=0.03, seed=3),
+            "mixed_batch_4096": lambda: mixed_batch(4096, 20, 400, sigma=0.03, seed=3)}
+
+
+def load_workload(name):
+    """(atom_off or None, Z, xyz[A]) of a named workload."""
+    from curcuma_amd.workloads import WORKLOADS
+    bw = batch_workloads()
+    if name in bw:
+        return bw[name]()
+    Z, xyz = WORKLOADS[name]()
+    return None, Z, xyz
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    threads = min(avail, int(env)) if env and env.isdigit() and int(env) > 0 else avail
+    return model, avail, threads
+
+
+def cpu_baseline(args, atom_off, Z, xyz):
+    """Times the threaded CPU oracle (test infrastructure; FFWorkspace-style
+    partitions, per-thread accumulators, fixed-order reduce) on a bounded
+    sample of the headline workload: 1 warm-up, then the median of 5
+    repetitions.  Returns atom*evals/s."""
+    from curcuma_amd import GFNFFMethod, Mol
+    sys.path.insert(0, str(ROOT / "tests"))
+    from conftest import Oracle  # noqa: E402
+
+    model, avail, nthr = cpu_info()
+    orc = Oracle()
+    jobs = []
+    if atom_off is not None:
+        nm = min(args.cpu_sample_mols, len(atom_off) - 1)
+        for k in range(nm):
+            a0, a1 = int(atom_off[k]), int(atom_off[k + 1])
+            m = GFNFFMethod()
+            if not m.setMolecule(Mol(Z[a0:a1], xyz[a0:a1])):
+                return None
+            jobs.append((m, m.params_view(), xyz[a0:a1] * (1.0 / BOHR)))
+        sample = f"the first {nm} molecules of the same batch ({sum(len(j[2]) for j in jobs)} atoms)"
+    else:
+        from curcuma_amd.workloads import water_cluster
+        nmol = int(args.cpu_sample.split("_")[1])
+        Zs, xs = water_cluster(nmol, seed=2)
+        m = GFNFFMethod()
+        if not m.setMolecule(Mol(Zs, xs)):
+            return None
+        jobs.append((m, m.params_view(), xs * (1.0 / BOHR)))
+        sample = f"water cluster of {nmol} H2O ({len(Zs)} atoms) from the same generator"
+    natoms = sum(len(j[2]) for j in jobs)
+
+    def one_pass():
+        t0 = time.perf_counter()
+        for _, v, x in jobs:
+            orc.eval_mt(v, x, nthr, grad=True)
+        return time.perf_counter() - t0
+
+    one_pass()  # warm-up
+    reps = [one_pass() for _ in range(5)]
+    med = statistics.median(reps)
+    return {"value": natoms / med, "unit": "atom*evals/s", "cores": nthr, "kind": "port",
+            "sample": f"{sample}, FP64 energy+gradient of the threaded CPU oracle with {nthr} threads "
+                      f"(one molecule at a time, threads inside the evaluation); 1 warm-up + median of 5 "
+                      f"passes, {med:.3f} s per pass",
+            "cpu_model": model, "cpus_available": avail}
 
 
 def pmc_traffic(kernel: str, workload: str):
@@ -101,69 +164,47 @@ def pmc_traffic(kernel: str, workload: str):
     return None
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the engine has no multi-GPU path in this build", file=sys.stderr)
-        sys.exit(2)
-    import torch
-    from curcuma_amd import GFNFFBatch, GFNFFMethod, Mol
-    from curcuma_amd._lib import NTERMS
-    from curcuma_amd.workloads import WORKLOADS, alkane_batch, mixed_batch
+class Run:
+    """One engine instance on one workload and precision, with its
+    device-resident evaluation step."""
 
-    BATCH_WORKLOADS = {"alkane_batch_4096": lambda: alkane_batch(4096, 20, 400, sigma=0.03, seed=3),
-                       "mixed_batch_4096": lambda: mixed_batch(4096, 20, 400, sigma=0.03, seed=3)}
-
-    if args.copy_calibration_mib > 0:
-        nel = args.copy_calibration_mib * (1 << 20) // 8
-        a = torch.ones(nel, dtype=torch.float64, device="cuda:0")
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
-
-    batch = args.workload in BATCH_WORKLOADS
-    if batch:
-        atom_off, Z, xyz = BATCH_WORKLOADS[args.workload]()
-    else:
-        Z, xyz = WORKLOADS[args.workload]()
-        atom_off = None
-    n = len(Z)
-    dev = torch.device("cuda:0")
-    x_dev = torch.tensor(xyz * (1.0 / 0.529177210903), dtype=torch.float64, device=dev).contiguous()
-    stream = torch.cuda.current_stream()
-    sp = C.c_void_p(stream.cuda_stream)
-
-    def make(precision):
-        cls = GFNFFBatch if batch else GFNFFMethod
-        m = cls(precision=precision, parameters={"cholesky_lookahead": args.lookahead, "eeq_solver": args.solver})
+    def __init__(self, args, torch, atom_off, Z, xyz, precision):
+        from curcuma_amd import GFNFFBatch, GFNFFMethod, Mol
+        from curcuma_amd._lib import NTERMS
+        self.batch = atom_off is not None
+        self.n = len(Z)
+        dev = torch.device("cuda:0")
+        self.x_dev = torch.tensor(xyz * (1.0 / BOHR), dtype=torch.float64, device=dev).contiguous()
+        self.sp = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        cls = GFNFFBatch if self.batch else GFNFFMethod
+        self.m = cls(precision=precision,
+                     parameters={"cholesky_lookahead": args.lookahead, "eeq_solver": args.solver})
         t0 = time.perf_counter()
-        ok = m.setBatch(atom_off, Z, xyz) if batch else m.setMolecule(Mol(Z, xyz))
+        ok = self.m.setBatch(atom_off, Z, xyz) if self.batch else self.m.setMolecule(Mol(Z, xyz))
         if not ok:
-            print("set-up failed: " + m.getErrorMessage(), file=sys.stderr)
+            print("set-up failed: " + self.m.getErrorMessage(), file=sys.stderr)
             sys.exit(1)
-        e_dev = torch.zeros(NTERMS + 1, dtype=torch.float64, device=dev)
-        g_dev = torch.zeros((n, 3), dtype=torch.float64, device=dev)
-        lib = m._lib
+        self.setup_s = time.perf_counter() - t0
+        self.e_dev = torch.zeros(NTERMS + 1, dtype=torch.float64, device=dev)
+        self.g_dev = torch.zeros((self.n, 3), dtype=torch.float64, device=dev)
 
-        def step():
-            st = lib.gfnff_energy_gradient_device(m.handle, C.c_void_p(x_dev.data_ptr()),
-                                                  C.c_void_p(e_dev.data_ptr()), C.c_void_p(g_dev.data_ptr()), sp)
-            if st != 0:
-                raise RuntimeError(lib.gfnff_last_error(m.handle).decode())
-        return m, step, e_dev, g_dev, time.perf_counter() - t0
+    def step(self):
+        lib = self.m._lib
+        st = lib.gfnff_energy_gradient_device(self.m.handle, C.c_void_p(self.x_dev.data_ptr()),
+                                              C.c_void_p(self.e_dev.data_ptr()), C.c_void_p(self.g_dev.data_ptr()),
+                                              self.sp)
+        if st != 0:
+            raise RuntimeError(lib.gfnff_last_error(self.m.handle).decode())
 
-    from curcuma_amd.profiling import SIDE_STREAM_CLASSES, KernelTimer, roofline, survey
-
-    def timed(m, step, kernel):
+    def timed(self, args, torch, kernel):
         """Warm-up, per-class survey, then the timed region with HIP events on
         the evaluation stream around every launch of `kernel`."""
+        from curcuma_amd.profiling import SIDE_STREAM_CLASSES, KernelTimer, survey
         for _ in range(args.warmup):
-            step()
+            self.step()
         torch.cuda.synchronize()
-        timer = KernelTimer(m.handle, m._lib)
-        per_class = survey(timer, step)
+        timer = KernelTimer(self.m.handle, self.m._lib)
+        per_class = survey(timer, self.step)
         # the dominant class of the critical path: classes that run on the
         # engine's side stream, overlapped with the evaluation stream, are not
         # candidates (their event times include the sharing)
@@ -173,80 +214,144 @@ def main():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            step()
+            self.step()
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         k_ms, k_launches = timer.read()
         timer.select(None)
-        return dt, k, k_ms, k_launches, per_class
+        self.dt = dt
+        self.value = self.n * args.steps / dt
+        self.ms = dt / args.steps * 1e3
+        self.kernel, self.k_ms, self.k_launches, self.per_class = k, k_ms, k_launches, per_class
+        return self
 
-    # FP64 path (the reference's arithmetic) and the mixed-precision path on the same input
-    m64, step64, e64_dev, g64_dev, setup64 = make("fp64")
-    dt64, _, _, _, classes64 = timed(m64, step64, "pairs")
-    headline = args.precision
-    if headline == "mixed":
-        mh, steph, eh_dev, gh_dev, setup_h = make("mixed")
-        dt, kernel, k_ms, k_launches, per_class = timed(mh, steph, args.roofline_kernel)
-    else:
-        mh, steph, eh_dev, gh_dev, setup_h = m64, step64, e64_dev, g64_dev, setup64
-        dt, kernel, k_ms, k_launches, per_class = timed(mh, steph, args.roofline_kernel)
-    ms = dt / args.steps * 1e3
-    value = n * args.steps / dt
-    value64 = n * args.steps / dt64
+    def energy(self):
+        return float(self.e_dev[0].item())
 
-    e64 = float(e64_dev[0].item())
-    eh = float(eh_dev[0].item())
-    g64 = g64_dev.cpu().numpy()
-    gh = gh_dev.cpu().numpy()
-    accuracy = {"vs": "FP64 GPU path (parity-pinned to the FP64 oracle)",
-                "dE_per_atom_Eh": abs(eh - e64) / n,
-                "grad_rms_Eh_per_bohr": float(np.sqrt(np.mean((gh - g64) ** 2))),
+    def grad(self):
+        return self.g_dev.cpu().numpy()
+
+    def classes(self):
+        return {c: round(v[0], 4) for c, v in sorted(self.per_class.items(), key=lambda t: -t[1][0])}
+
+
+def oracle_check(head, atom_off, Z, xyz, nmol):
+    """Headline batch against the FP64 CPU oracle on `nmol` molecules spread
+    over the batch (per-molecule energies and gradient blocks)."""
+    from curcuma_amd import GFNFFMethod, Mol
+    sys.path.insert(0, str(ROOT / "tests"))
+    from conftest import Oracle  # noqa: E402
+    _, _, nthr = cpu_info()
+    orc = Oracle()
+    e_mol = head.m.getEnergies()
+    g = head.grad()
+    total = len(atom_off) - 1
+    pick = sorted(set(np.linspace(0, total - 1, nmol).round().astype(int).tolist()))
+    de, rms = [], []
+    for k in pick:
+        a0, a1 = int(atom_off[k]), int(atom_off[k + 1])
+        m = GFNFFMethod()
+        if not m.setMolecule(Mol(Z[a0:a1], xyz[a0:a1])):
+            raise RuntimeError(m.getErrorMessage())
+        ref = orc.eval_mt(m.params_view(), xyz[a0:a1] * (1.0 / BOHR), nthr)
+        de.append(abs(e_mol[k] - ref["E"]) / (a1 - a0))
+        rms.append(float(np.sqrt(np.mean((g[a0:a1] - ref["grad"]) ** 2))))
+    return {"vs": f"FP64 CPU oracle, {len(pick)} molecules spread over the batch (indices {pick[0]}..{pick[-1]})",
+            "dE_per_atom_Eh_max": max(de), "grad_rms_Eh_per_bohr_max": max(rms)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the engine has no multi-GPU path in this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from curcuma_amd.profiling import roofline
+
+    if args.copy_calibration_mib > 0:
+        nel = args.copy_calibration_mib * (1 << 20) // 8
+        a = torch.ones(nel, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
+    atom_off, Z, xyz = load_workload(args.workload)
+    batch = atom_off is not None
+    n = len(Z)
+
+    # FP64 path (the reference's arithmetic) and the headline path on the same input
+    r64 = Run(args, torch, atom_off, Z, xyz, "fp64")
+    r64.timed(args, torch, args.roofline_kernel if args.precision == "fp64" else "pairs")
+    head = r64
+    if args.precision == "mixed":
+        head = Run(args, torch, atom_off, Z, xyz, "mixed").timed(args, torch, args.roofline_kernel)
+
+    g64, gh = r64.grad(), head.grad()
+    accuracy = {"mixed_vs_fp64_gpu": {"dE_per_atom_Eh": abs(head.energy() - r64.energy()) / n,
+                                      "grad_rms_Eh_per_bohr": float(np.sqrt(np.mean((gh - g64) ** 2)))},
                 "bar": "|dE| <= 1e-6 Eh/atom, grad RMS <= 1e-5 Eh/Bohr"}
+    if batch and args.oracle_check_mols > 0:
+        accuracy["oracle"] = oracle_check(head, atom_off, Z, xyz, args.oracle_check_mols)
 
-    pv = mh.params_view()
+    pv = head.m.params_view()
     nr = pv.natoms - pv.nfrag
-    info = {"xyz_bohr": xyz * (1.0 / 0.529177210903), "np_pad": max(64, -(-nr // 64) * 64), "nhb": pv.nhb,
+    info = {"xyz_bohr": xyz * (1.0 / BOHR), "np_pad": max(64, -(-nr // 64) * 64), "nhb": pv.nhb,
             "nbond": pv.nbond}
-    pcg_iters = mh.solverIterations() if not batch else 0
+    pcg_iters = head.m.solverIterations() if not batch else 0
     info["pcg_iters"] = pcg_iters
     if batch:
         frag = np.ctypeslib.as_array(pv.frag, shape=(n,))
         info["mol_off"] = atom_off
         info["mol_nr"] = [int(atom_off[k + 1] - atom_off[k] - len(np.unique(frag[atom_off[k]:atom_off[k + 1]])))
                           for k in range(len(atom_off) - 1)]
-    roof = roofline(kernel, k_ms, k_launches, args.steps, info, traffic=pmc_traffic(kernel, args.workload),
-                    precision=headline)
-    roof["kernel_ms_per_eval"] = {c: round(v[0], 4) for c, v in sorted(per_class.items(), key=lambda t: -t[1][0])}
-    t_setup = setup_h
-    e_last = eh
+    roof = roofline(head.kernel, head.k_ms, head.k_launches, args.steps, info,
+                    traffic=pmc_traffic(head.kernel, args.workload), precision=args.precision)
+    roof["kernel_ms_per_eval"] = head.classes()
+
+    plain = None
+    if batch and args.plain_workload:
+        p_off, pZ, pxyz = load_workload(args.plain_workload)
+        rp = Run(args, torch, p_off, pZ, pxyz, "fp64").timed(args, torch, "pairs")
+        plain = {"workload": WORKLOAD_TEXT.get(args.plain_workload, args.plain_workload), "atoms": len(pZ),
+                 "precision": "fp64", "eeq_solver": "PCG" if args.solver == 0 else "blocked Cholesky",
+                 "value": rp.value, "ms_per_step": rp.ms, "energy_Eh": rp.energy(),
+                 "kernel_ms_per_eval": rp.classes()}
+        del rp
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(args.cpu_sample)
+            cpu = cpu_baseline(args, atom_off, Z, xyz)
         except Exception as ex:
             print(f"cpu baseline failed: {ex}", file=sys.stderr)
+
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head.value,
         "unit": "atom*evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": head.ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if headline == "fp64" else "f32+f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": WORKLOAD_TEXT.get(args.workload, args.workload),
-                   "atoms": n, "precision": headline, "setup_s": round(t_setup, 3), "energy_Eh": e_last,
+                   "molecules": (len(atom_off) - 1) if batch else 1,
+                   "atoms": n, "precision": args.precision, "setup_s": round(head.setup_s, 3),
+                   "energy_Eh": head.energy(),
                    "eeq_solver": ("per-molecule Cholesky" if batch else
                                   ("PCG" if args.solver == 0 else "blocked Cholesky")),
                    "pcg_iterations": pcg_iters},
-        "fp64": {"value": value64, "ms_per_step": dt64 / args.steps * 1e3, "energy_Eh": e64,
-                 "kernel_ms_per_eval": {c: round(v[0], 4) for c, v in
-                                        sorted(classes64.items(), key=lambda t: -t[1][0])}},
-        "mixed_over_fp64": value / value64,
+        "fp64": {"value": r64.value, "ms_per_step": r64.ms, "energy_Eh": r64.energy(),
+                 "kernel_ms_per_eval": r64.classes()},
+        "mixed_over_fp64": head.value / r64.value,
+        "plain_fp64": plain,
+        "mixed_batched_over_plain_fp64": (head.value / plain["value"]) if plain else None,
         "accuracy": accuracy,
         "roofline": roof,
         "cpu_baseline": cpu,

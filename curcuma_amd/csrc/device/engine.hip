@@ -3930,11 +3930,12 @@ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 enum KernelClass {
     KC_CN, KC_EEQ_BUILD, KC_CHOL_DIAG, KC_CHOL_PANEL, KC_CHOL_UPDATE, KC_TRSV, KC_CHARGES, KC_PAIRS, KC_PAIRS_AUX,
     KC_NBREP, KC_REFINE, KC_BONDS, KC_ANGLES, KC_TORS, KC_BREP, KC_HB, KC_ATM, KC_BATM, KC_CHAIN, KC_REDUCE, KC_PCG,
-    KC_COUNT
+    KC_BATCH_SOLVE, KC_COUNT
 };
 const char* const kKernelClassNames[KC_COUNT] = {
     "cn",    "eeq_build", "chol_diag", "chol_panel", "chol_update", "trsv", "charges", "pairs", "pairs_aux", "nbrep", "refine",
-    "bonds", "angles",    "torsions",  "brep",       "hb",          "atm",  "batm",    "chain", "reduce", "pcg"};
+    "bonds", "angles",    "torsions",  "brep",       "hb",          "atm",  "batm",    "chain", "reduce", "pcg",
+    "batch_solve"};
 
 // Start/stop event pairs around every launch of the selected class, recorded
 // on the stream the kernels run on.
@@ -4983,7 +4984,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     HIP_OK(hipStreamWaitEvent(s, E->ev_join, 0));
     if (E->nmol > 1) {
         // batch: every molecule's system factored and solved by one workgroup (FP64)
-        LAUNCH(KC_CHOL_DIAG, k_eeq_batch_solve, dim3(E->nmol), dim3(256), s, E->morder, E->mnp, E->mmo, E->mvo,
+        LAUNCH(KC_BATCH_SOLVE, k_eeq_batch_solve, dim3(E->nmol), dim3(256), s, E->morder, E->mnp, E->mmo, E->mvo,
                E->Ared, E->Linv, E->b, E->xbuf, E->flags);
     } else if (E->use_cg) {
         // the persistent PCG (block-Jacobi factors and mailboxes were set up on the side stream)

@@ -27,7 +27,7 @@ NB = 64                # Cholesky block size of the engine
 
 # every kernel class the engine can time
 CLASSES = ["cn", "eeq_build", "chol_diag", "chol_panel", "chol_update", "trsv", "charges", "pairs", "pairs_aux", "nbrep", "refine",
-           "bonds", "angles", "torsions", "brep", "hb", "atm", "batm", "chain", "reduce", "pcg"]
+           "bonds", "angles", "torsions", "brep", "hb", "atm", "batm", "chain", "reduce", "pcg", "batch_solve"]
 # classes whose kernels run on the engine's side stream, overlapped with the
 # evaluation stream (the reduced EEQ matrix build)
 SIDE_STREAM_CLASSES = {"eeq_build"}
@@ -116,7 +116,7 @@ def work_per_eval(cls: str, info: dict, rc: dict) -> tuple[float, str]:
                 tot = [a + b for a, b in zip(tot, c)]
             nc, nd, nr = tot
             return (nc * rc["COULOMB_PAIR_FLOPS"] + nd * rc["DISP_PAIR_FLOPS"] + nr * rc["REP_PAIR_FLOPS"]), "flop"
-        if cls == "chol_diag":  # k_eeq_batch_solve: factorisation + both substitutions per molecule
+        if cls == "batch_solve":  # k_eeq_batch_solve: factorisation + both substitutions per molecule
             return sum(m ** 3 / rc["CHOL_DIV"] + rc["TRSV_FACTOR"] * m * m for m in info["mol_nr"]), "flop"
     if cls == "pairs":
         nc, nd, nr = info.get("pairs") or pair_counts(info["xyz_bohr"])
@@ -147,8 +147,11 @@ def work_per_eval(cls: str, info: dict, rc: dict) -> tuple[float, str]:
     raise KeyError(f"no work convention for kernel class {cls}")
 
 
-# kernel classes that run in FP32 on the mixed-precision path
+# kernel classes that run in FP32 on the mixed-precision path (the batched
+# per-molecule solve, class batch_solve, stays FP64 on both paths)
 FP32_CLASSES_MIXED = {"pairs", "chol_update", "chol_diag", "chol_panel", "trsv"}
+# classes whose flops run on the matrix cores
+MFMA_CLASSES = {"chol_update", "chol_diag", "chol_panel", "batch_solve"}
 
 
 def roofline(cls: str, ms_total: float, launches: int, evals: int, info: dict, traffic=None,
@@ -161,7 +164,7 @@ def roofline(cls: str, ms_total: float, launches: int, evals: int, info: dict, t
     achieved = per_launch / avg_s
     if kind == "flop":
         peak = PEAK_FP32 if (precision == "mixed" and cls in FP32_CLASSES_MIXED) else PEAK_FP64
-        bound = "mfma" if cls.startswith("chol") else "valu"
+        bound = "mfma" if cls in MFMA_CLASSES else "valu"
         unit = "TFLOP/s"
         scale = 1e12
     else:

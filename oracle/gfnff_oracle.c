@@ -1430,3 +1430,73 @@ int gfnff_oracle_cn_gw(const gfnff_params_view* P, const double* x, double* cn_o
     free(dgw);
     return status;
 }
+
+/* Relative residual of the constrained EEQ system at given charges q (a check
+ * of the engine's solves, not part of an evaluation).  The bordered system
+ * A q + C lambda = rhs, C^T q = qfrag (calculateFinalCharges,
+ * eeq_solver.cpp:3571-3710) restricted to the null space of C^T, spanned by
+ * e_a - e_last for every atom a of a fragment but its last (highest-index)
+ * atom, has the residual r_a = v_a - v_last with v = rhs - A q.  Returns
+ * |r|_2 / |b|_2 with b the same projection of rhs - A q_p, q_p_i =
+ * qfrag_f / m_f (the reduced right-hand side), and the largest
+ * |sum_{i in f} q_i - qfrag_f|.  cn: the log-capped coordination numbers of
+ * the evaluation; rows of A are striped over nthr threads. */
+int gfnff_oracle_eeq_residual(const gfnff_params_view* P, const double* x, const double* cn, const double* q,
+                              int nthr, double* rel, double* qsum_err) {
+    const int n = P->natoms, nf = P->nfrag;
+    double* v = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
+    double* b = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
+    double* qp = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
+    int* cnt = (int*)calloc(nf > 0 ? nf : 1, sizeof(int));
+    int* last = (int*)malloc(sizeof(int) * (nf > 0 ? nf : 1));
+    double* qs = (double*)calloc(nf > 0 ? nf : 1, sizeof(double));
+    int status = 1;
+    if (!v || !b || !qp || !cnt || !last || !qs) goto out;
+    for (int i = 0; i < n; ++i) {
+        cnt[P->frag[i]]++;
+        last[P->frag[i]] = i;
+        qs[P->frag[i]] += q[i];
+    }
+    for (int i = 0; i < n; ++i) qp[i] = P->qfrag[P->frag[i]] / cnt[P->frag[i]];
+#pragma omp parallel for schedule(static, 16) num_threads(nthr) if (nthr > 1)
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0, sp = 0.0;
+        for (int j = 0; j < n; ++j) {
+            double a;
+            if (j == i) {
+                a = P->gam_eff[i] + TSQRT2PI / sqrt(P->alpeeq[i]);
+            } else {
+                double d[3];
+                sub3(x + 3 * i, x + 3 * j, d);
+                double r = norm3(d);
+                a = erf(r / sqrt(P->alpeeq[i] + P->alpeeq[j])) / r;
+            }
+            s += a * q[j];
+            sp += a * qp[j];
+        }
+        double rhs = P->chi_base[i] + P->cnf[i] * sqrt(cn[i]);
+        v[i] = rhs - s;
+        b[i] = rhs - sp;
+    }
+    {
+        double rr = 0.0, bb = 0.0, qe = 0.0;
+        for (int i = 0; i < n; ++i) {
+            int l = last[P->frag[i]];
+            if (i == l) continue;
+            rr += (v[i] - v[l]) * (v[i] - v[l]);
+            bb += (b[i] - b[l]) * (b[i] - b[l]);
+        }
+        for (int f = 0; f < nf; ++f) qe = fmax(qe, fabs(qs[f] - P->qfrag[f]));
+        *rel = bb > 0.0 ? sqrt(rr / bb) : sqrt(rr);
+        *qsum_err = qe;
+    }
+    status = 0;
+out:
+    free(v);
+    free(b);
+    free(qp);
+    free(cnt);
+    free(last);
+    free(qs);
+    return status;
+}

@@ -77,6 +77,23 @@ class Oracle:
         return {"E": E.value, "terms": terms, "grad": g.reshape(n, 3), "q": q, "cn": cn}
 
 
+    def eeq_residual(self, view: ParamsView, xyz_bohr: np.ndarray, cn: np.ndarray, q: np.ndarray, nthreads: int = 1):
+        """(|r|/|b| of the constrained EEQ system at charges q, max fragment
+        charge-sum error); cn: the evaluation's log-capped CN."""
+        lib = self.lib
+        lib.gfnff_oracle_eeq_residual.argtypes = [C.POINTER(ParamsView), C.c_void_p, C.c_void_p, C.c_void_p,
+                                                  C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        x = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1)
+        cn = np.ascontiguousarray(cn, dtype=np.float64)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        rel, qe = C.c_double(), C.c_double()
+        st = lib.gfnff_oracle_eeq_residual(C.byref(view), x.ctypes.data, cn.ctypes.data, q.ctypes.data,
+                                           int(nthreads), C.byref(rel), C.byref(qe))
+        if st != 0:
+            raise RuntimeError(f"oracle status {st}")
+        return rel.value, qe.value
+
+
 @pytest.fixture(scope="session")
 def oracle():
     return Oracle()
