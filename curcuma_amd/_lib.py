@@ -81,6 +81,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
     lib.gfnff_solver_iterations.argtypes = [C.c_void_p]
     lib.gfnff_dense_cholesky.argtypes = [C.c_int32, _f64p, _f64p]
     lib.gfnff_dense_cholesky_f32.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
+    lib.gfnff_dense_cholesky_noalloc.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.gfnff_param_count.argtypes = [C.c_void_p]
     lib.gfnff_param_scales.argtypes = [C.c_void_p, _f64p, C.c_int32]
     lib.gfnff_param_adjoint.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int32]

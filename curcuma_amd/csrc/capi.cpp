@@ -445,6 +445,16 @@ int gfnff_dense_cholesky(int32_t n, const double* a, double* l_out) {
     }
 }
 
+int gfnff_dense_cholesky_noalloc(int32_t n, const double* a_dev, double* l_dev, int32_t* status_dev, void* stream) {
+    if (n <= 0 || n > GFNFF_NOALLOC_MAX_N || !a_dev || !l_dev || !status_dev) return GFNFF_STATUS_ARGS;
+    std::string msg;
+    try {
+        return gfnff::device_dense_cholesky_noalloc(n, a_dev, l_dev, status_dev, stream, msg);
+    } catch (...) {
+        return GFNFF_STATUS_DEVICE;
+    }
+}
+
 int gfnff_dense_cholesky_f32(int32_t n, const float* a, float* l_out) {
     if (n <= 0 || !a || !l_out) return GFNFF_STATUS_ARGS;
     std::string msg;

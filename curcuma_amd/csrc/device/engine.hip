@@ -5355,4 +5355,87 @@ int device_dense_cholesky_f32(int n, const float* a, float* l_out, std::string& 
     return dense_cholesky<float>(n, a, l_out, msg);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free variant of the dense Cholesky: the padded working matrix,
+// the inverses of its diagonal blocks and the status word sit in one
+// file-scope device array (8.5 MB on every device the library runs on, held
+// until the process ends).  Host code never uses the array's name as an
+// address: the device address comes from hipGetSymbolAddress, taken once per
+// device at the first call there, and the kernels receive that address.
+// Calls that overlap on one device share the array and race.
+// ---------------------------------------------------------------------------
+constexpr int NOALLOC_MAX_N = 1024;
+constexpr size_t NOALLOC_MAT = (size_t)NOALLOC_MAX_N * NOALLOC_MAX_N;  // padded np x np matrix
+constexpr size_t NOALLOC_LINV = (size_t)NOALLOC_MAX_N * NB;           // np / 64 inverses of 64 x 64 blocks
+constexpr size_t NOALLOC_DOUBLES = NOALLOC_MAT + NOALLOC_LINV + 32;    // + status word (256-byte line)
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+// A = a padded to np x np with the identity; clears the status word (a store,
+// not a memset, so that the sequence can be captured in a graph).
+__global__ __launch_bounds__(256) void k_noalloc_pad(int n, int np, const double* __restrict__ a,
+                                                     double* __restrict__ A, int* __restrict__ flag) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e == 0) flag[0] = 0;
+    if (e >= (size_t)np * np) return;
+    const int i = (int)(e / np), j = (int)(e % np);
+    A[e] = (i < n && j < n) ? a[(size_t)i * n + j] : (i == j ? 1.0 : 0.0);
+}
+
+// l = lower triangle of the factored A (zeros above the diagonal); status
+// GFNFF_STATUS_EEQ (3) when a pivot was not positive, else 0.
+__global__ __launch_bounds__(256) void k_noalloc_out(int n, int np, const double* __restrict__ A,
+                                                     double* __restrict__ l, const int* __restrict__ flag,
+                                                     int* __restrict__ status) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e == 0) status[0] = (flag[0] & FLAG_CHOL) ? kErrEEQ : kOk;
+    if (e >= (size_t)n * n) return;
+    const int i = (int)(e / n), j = (int)(e % n);
+    l[e] = (j <= i) ? A[(size_t)i * np + j] : 0.0;
+}
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_base[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+int device_dense_cholesky_noalloc(int n, const double* a_dev, double* l_dev, int* status_dev, void* stream,
+                                  std::string& msg) {
+    if (n <= 0 || n > NOALLOC_MAX_N) {
+        msg = "n must be in 1.." + std::to_string(NOALLOC_MAX_N);
+        return kErrArgs;
+    }
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+        msg = "device index out of range";
+        return kErrArgs;
+    }
+    double* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        if (!g_noalloc_base[dev]) {
+            void* p = nullptr;
+            HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            if (!p) {
+                msg = "no device address for the scratch array";
+                return kErrDevice;
+            }
+            g_noalloc_base[dev] = static_cast<double*>(p);
+        }
+        base = g_noalloc_base[dev];
+    }
+    const int np = cdiv(n, NB) * NB;
+    double* A = base;
+    double* Linv = base + NOALLOC_MAT;
+    int* flag = reinterpret_cast<int*>(base + NOALLOC_MAT + NOALLOC_LINV);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_noalloc_pad, dim3(cdiv((long)np * np, 256)), dim3(256), 0, s, n, np, a_dev, A, flag);
+    queue_cholesky<double>(nullptr, s, np, A, Linv, flag);
+    hipLaunchKernelGGL(k_noalloc_out, dim3(cdiv((long)n * n, 256)), dim3(256), 0, s, n, np, A, l_dev, flag,
+                       status_dev);
+    HIP_OK(hipGetLastError());
+    return kOk;
+}
+
 }  // namespace gfnff

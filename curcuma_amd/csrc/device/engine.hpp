@@ -59,6 +59,11 @@ int device_profile_read(DeviceEngine* e, double* total_ms, int* launches, std::s
 // Dense SPD factorisation with the EEQ Cholesky kernels (host buffers).
 int device_dense_cholesky(int n, const double* a, double* l_out, std::string& msg);
 int device_dense_cholesky_f32(int n, const float* a, float* l_out, std::string& msg);
+// The same on device buffers without any allocation (scratch in a static device
+// array, n <= 1024); asynchronous on `stream`, status_dev receives 0 or kErrEEQ.
+int device_dense_cholesky_noalloc(int n, const double* a_dev, double* l_dev, int* status_dev, void* stream,
+                                  std::string& msg);
+
 
 }  // namespace gfnff
 

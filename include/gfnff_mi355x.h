@@ -202,6 +202,24 @@ int gfnff_dense_cholesky(int32_t n, const double* a, double* l_out);
 /* The same with the FP32 kernels of the mixed-precision path (FP32 MFMA). */
 int gfnff_dense_cholesky_f32(int32_t n, const float* a, float* l_out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): gfnff_dense_cholesky on device buffers, allocating
+ * nothing.  a_dev: n x n SPD matrix (row-major, read only); l_dev: n x n,
+ * receives L (zeros above the diagonal); status_dev: one int32, receives 0 or
+ * GFNFF_STATUS_EEQ (a pivot was not positive) when the stream reaches it.
+ * Work is queued on `stream` (hipStream_t, NULL = default) without
+ * synchronising.  The padded working matrix, the inverses of its diagonal
+ * blocks and the status word live in one static device array of
+ * GFNFF_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends), which
+ * every call on that GPU shares: calls that overlap on one GPU race.
+ * n > GFNFF_NOALLOC_MAX_N, n <= 0 or a NULL pointer return GFNFF_STATUS_ARGS
+ * and queue nothing. */
+#define GFNFF_NOALLOC_MAX_N 1024
+#define GFNFF_NOALLOC_SCRATCH_BYTES (8 * (1024 * 1024 + 1024 * 64 + 32))
+int gfnff_dense_cholesky_noalloc(int32_t n, const double* a_dev, double* l_dev, int32_t* status_dev, void* stream);
+
+
 #ifdef __cplusplus
 }
 #endif
