@@ -54,6 +54,11 @@ def parse():
                    help="batch molecules checked against the FP64 oracle (0 to skip)")
     p.add_argument("--lookahead", type=int, default=0, help="engine option cholesky_lookahead (0/1)")
     p.add_argument("--solver", type=int, default=0, help="engine option eeq_solver (0 PCG, 1 Cholesky)")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="skip the launch-shape autotuner (curcuma_amd/autotune.py) and its JSON cache")
+    p.add_argument("--retune", action="store_true", help="tune again even when the cache holds this workload")
+    p.add_argument("--no-graph", action="store_true",
+                   help="time eager launches for the headline instead of replays of the captured HIP graph")
     p.add_argument("--roofline-kernel", default="auto", help="kernel class timed for the roofline object")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="run one device copy of this size first (HBM counter calibration under rocprofv3 --pmc)")
@@ -86,6 +91,11 @@ def load_workload(name):
         return bw[name]()
     Z, xyz = WORKLOADS[name]()
     return None, Z, xyz
+
+
+def progress(msg: str) -> None:
+    """One progress line on stderr per phase (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_info():
@@ -168,17 +178,19 @@ class Run:
     """One engine instance on one workload and precision, with its
     device-resident evaluation step."""
 
-    def __init__(self, args, torch, atom_off, Z, xyz, precision):
+    def __init__(self, args, torch, atom_off, Z, xyz, precision, params=None):
         from curcuma_amd import GFNFFBatch, GFNFFMethod, Mol
         from curcuma_amd._lib import NTERMS
         self.batch = atom_off is not None
         self.n = len(Z)
         dev = torch.device("cuda:0")
         self.x_dev = torch.tensor(xyz * (1.0 / BOHR), dtype=torch.float64, device=dev).contiguous()
-        self.sp = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        self.stream = torch.cuda.Stream()  # a created stream: graph capture needs one
+        self.sp = C.c_void_p(self.stream.cuda_stream)
         cls = GFNFFBatch if self.batch else GFNFFMethod
+        self.params = dict(params or {})
         self.m = cls(precision=precision,
-                     parameters={"cholesky_lookahead": args.lookahead, "eeq_solver": args.solver})
+                     parameters={"cholesky_lookahead": args.lookahead, "eeq_solver": args.solver, **self.params})
         t0 = time.perf_counter()
         ok = self.m.setBatch(atom_off, Z, xyz) if self.batch else self.m.setMolecule(Mol(Z, xyz))
         if not ok:
@@ -195,6 +207,27 @@ class Run:
                                               self.sp)
         if st != 0:
             raise RuntimeError(lib.gfnff_last_error(self.m.handle).decode())
+
+    def set_params(self, params):
+        self.params = dict(params)
+        self.m.setParameters(self.params)
+
+    def timed_graph(self, args, torch):
+        """Captures one evaluation into a HIP graph and times `steps` replays."""
+        self.step()
+        torch.cuda.synchronize()
+        self.m.captureGraph(self.x_dev.data_ptr(), self.e_dev.data_ptr(), self.g_dev.data_ptr(), self.stream.cuda_stream)
+        for _ in range(args.warmup):
+            self.m.launchGraph(self.stream.cuda_stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            self.m.launchGraph(self.stream.cuda_stream)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        self.graph_value = self.n * args.steps / dt
+        self.graph_ms = dt / args.steps * 1e3
+        return self
 
     def timed(self, args, torch, kernel):
         """Warm-up, per-class survey, then the timed region with HIP events on
@@ -233,6 +266,36 @@ class Run:
 
     def classes(self):
         return {c: round(v[0], 4) for c, v in sorted(self.per_class.items(), key=lambda t: -t[1][0])}
+
+
+def autotune_run(args, torch, atom_off, Z, xyz):
+    """Launch-shape keys of the headline path: from the JSON cache, or tuned
+    now (outside every timed region) and stored there."""
+    from curcuma_amd import autotune
+    device = torch.cuda.get_device_name(0)
+    sig = autotune.signature(Z, atom_off, args.precision, args.solver)
+    space = autotune.SPACE_BATCH if atom_off is not None else autotune.SPACE_SINGLE
+    path = autotune.default_cache_path()
+    if not args.retune:
+        hit = autotune.lookup(device, sig, path)
+        if hit is not None:
+            return {"params": hit["params"], "from_cache": True, "cache": str(path), "timings_ms": hit.get("timings_ms")}
+    run = Run(args, torch, atom_off, Z, xyz, args.precision)
+    defaults = {k: 0 for k in space}
+
+    def make(p):
+        run.set_params({**defaults, **p})
+        return run.m
+
+    best, timings = autotune.tune(torch, make, space, run.x_dev, run.e_dev, run.g_dev, run.stream)
+    cache = autotune.load_cache(path)
+    cache[f"{device}|{sig}"] = {"params": best, "timings_ms": timings}
+    try:
+        autotune.save_cache(cache, path)
+    except OSError as ex:
+        print(f"autotune cache not written: {ex}", file=sys.stderr)
+    del run
+    return {"params": best, "from_cache": False, "cache": str(path), "timings_ms": timings}
 
 
 def oracle_check(head, atom_off, Z, xyz, nmol):
@@ -282,17 +345,31 @@ def main():
     n = len(Z)
 
     # FP64 path (the reference's arithmetic) and the headline path on the same input
-    r64 = Run(args, torch, atom_off, Z, xyz, "fp64")
+    progress(f"workload {args.workload}: {n} atoms")
+    tuned = None
+    if not args.no_autotune:
+        progress("autotune")
+        tuned = autotune_run(args, torch, atom_off, Z, xyz)
+    params = tuned["params"] if tuned else {}
+    progress("FP64 path")
+    r64 = Run(args, torch, atom_off, Z, xyz, "fp64", params if args.precision == "fp64" else None)
     r64.timed(args, torch, args.roofline_kernel if args.precision == "fp64" else "pairs")
     head = r64
     if args.precision == "mixed":
-        head = Run(args, torch, atom_off, Z, xyz, "mixed").timed(args, torch, args.roofline_kernel)
+        progress("mixed path")
+        head = Run(args, torch, atom_off, Z, xyz, "mixed", params).timed(args, torch, args.roofline_kernel)
+    eager_value, eager_ms = head.value, head.ms
+    if not args.no_graph:
+        progress("graph replay")
+        head.timed_graph(args, torch)
+        head.value, head.ms = head.graph_value, head.graph_ms
 
     g64, gh = r64.grad(), head.grad()
     accuracy = {"mixed_vs_fp64_gpu": {"dE_per_atom_Eh": abs(head.energy() - r64.energy()) / n,
                                       "grad_rms_Eh_per_bohr": float(np.sqrt(np.mean((gh - g64) ** 2)))},
                 "bar": "|dE| <= 1e-6 Eh/atom, grad RMS <= 1e-5 Eh/Bohr"}
     if batch and args.oracle_check_mols > 0:
+        progress("oracle check")
         accuracy["oracle"] = oracle_check(head, atom_off, Z, xyz, args.oracle_check_mols)
 
     pv = head.m.params_view()
@@ -312,6 +389,7 @@ def main():
 
     plain = None
     if batch and args.plain_workload:
+        progress(f"plain FP64 path on {args.plain_workload}")
         p_off, pZ, pxyz = load_workload(args.plain_workload)
         rp = Run(args, torch, p_off, pZ, pxyz, "fp64").timed(args, torch, "pairs")
         plain = {"workload": WORKLOAD_TEXT.get(args.plain_workload, args.plain_workload), "atoms": len(pZ),
@@ -323,6 +401,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         try:
+            progress("CPU baseline")
             cpu = cpu_baseline(args, atom_off, Z, xyz)
         except Exception as ex:
             print(f"cpu baseline failed: {ex}", file=sys.stderr)
@@ -347,6 +426,10 @@ def main():
                    "eeq_solver": ("per-molecule Cholesky" if batch else
                                   ("PCG" if args.solver == 0 else "blocked Cholesky")),
                    "pcg_iterations": pcg_iters},
+        "launch": {"mode": "eager" if args.no_graph else "HIP graph replay",
+                   "graph_ms_per_step": None if args.no_graph else head.graph_ms,
+                   "eager_ms_per_step": eager_ms, "eager_value": eager_value,
+                   "engine_params": head.params, "autotune": tuned},
         "fp64": {"value": r64.value, "ms_per_step": r64.ms, "energy_Eh": r64.energy(),
                  "kernel_ms_per_eval": r64.classes()},
         "mixed_over_fp64": head.value / r64.value,

@@ -68,6 +68,10 @@ def load_library(path: Path | None = None) -> C.CDLL:
     lib.gfnff_energy_gradient.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p]
     lib.gfnff_energy_gradient_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.gfnff_device_status.argtypes = [C.c_void_p, _i32p, C.c_int32]
+    lib.gfnff_graph_capture.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.gfnff_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
+    lib.gfnff_md_steps.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _f64p, C.c_double, C.c_double,
+                                   C.c_double, C.c_int32, C.c_void_p, C.c_void_p]
     lib.gfnff_charges.argtypes = [C.c_void_p, _f64p]
     lib.gfnff_cn.argtypes = [C.c_void_p, _f64p]
     lib.gfnff_set_param.argtypes = [C.c_void_p, C.c_char_p, C.c_double]

@@ -1,0 +1,114 @@
+"""Launch-shape autotuner for the GFN-FF engine, with a JSON cache.
+
+The engine keys tuned here change only how the work is cut into launches, so
+every candidate computes the same result (the batch-solve variants and the
+PCG workgroup counts reorder no sum; the gathers add in the same fixed
+order):
+  batch_solver   0 / 1    per-molecule Cholesky of batches: tile phases shared
+                          out over wavefronts, or one tile step per workgroup
+  gather_lanes   16 / 64  lanes per atom in the slot and hydrogen-bond gathers
+  pcg_groups     0 / 128 / 192
+                          workgroups of the persistent PCG solve (single
+                          molecules; 0 = all that fit, at most 256)
+A configuration is timed on the device-resident evaluation: warm-up
+evaluations, then the median over repetitions of a few back-to-back
+evaluations between two HIP events on one stream.  The search is coordinate
+descent from the engine's defaults (one key at a time, the best value kept).
+The cache maps "device name | workload signature" to the chosen keys; the
+signature holds the atom and molecule counts, the element composition, the
+precision and the EEQ solver, so a cache entry is reused only for the same
+kind of workload on the same kind of device.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import statistics
+from pathlib import Path
+
+import numpy as np
+
+SPACE_BATCH = {"batch_solver": [0, 1], "gather_lanes": [16, 64]}
+SPACE_SINGLE = {"pcg_groups": [0, 128, 192], "gather_lanes": [16, 64]}
+
+
+def default_cache_path() -> Path:
+    env = os.environ.get("CURCUMA_AMD_AUTOTUNE_CACHE")
+    if env:
+        return Path(env)
+    base = os.environ.get("XDG_CACHE_HOME") or str(Path.home() / ".cache")
+    return Path(base) / "curcuma_amd" / "autotune.json"
+
+
+def signature(Z, atom_off, precision: str, solver: int) -> str:
+    z = np.asarray(Z, dtype=np.int64)
+    comp = np.bincount(z, minlength=87)
+    nmol = 1 if atom_off is None else len(atom_off) - 1
+    h = hashlib.sha1(comp.tobytes()).hexdigest()[:12]
+    return f"n={len(z)};nmol={nmol};elements={h};precision={precision};eeq_solver={solver}"
+
+
+def load_cache(path: Path | None = None) -> dict:
+    p = Path(path) if path else default_cache_path()
+    try:
+        return json.loads(p.read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def save_cache(cache: dict, path: Path | None = None) -> Path:
+    p = Path(path) if path else default_cache_path()
+    p.parent.mkdir(parents=True, exist_ok=True)
+    tmp = p.with_suffix(".tmp")
+    tmp.write_text(json.dumps(cache, indent=1, sort_keys=True))
+    os.replace(tmp, p)
+    return p
+
+
+def lookup(device: str, sig: str, path: Path | None = None) -> dict | None:
+    return load_cache(path).get(f"{device}|{sig}")
+
+
+def time_config(torch, make, params: dict, x_dev, e_dev, g_dev, stream, warmup: int = 2, reps: int = 5,
+                per_rep: int = 3) -> float:
+    """Median ms per evaluation of a fresh engine with `params`."""
+    m = make(params)
+    sp = stream.cuda_stream
+    for _ in range(warmup):
+        m.energyGradientDevice(x_dev.data_ptr(), e_dev.data_ptr(), g_dev.data_ptr(), sp)
+    times = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(per_rep):
+            m.energyGradientDevice(x_dev.data_ptr(), e_dev.data_ptr(), g_dev.data_ptr(), sp)
+        b.record(stream)
+        b.synchronize()
+        times.append(a.elapsed_time(b) / per_rep)
+    del m
+    return statistics.median(times)
+
+
+def tune(torch, make, space: dict, x_dev, e_dev, g_dev, stream, base: dict | None = None, **kw):
+    """Coordinate descent over `space` from `base`; returns (best params,
+    {config label: ms per evaluation})."""
+    best = dict(base or {})
+    timings = {}
+
+    def label(p):
+        return ",".join(f"{k}={p[k]}" for k in sorted(p)) or "defaults"
+
+    best_ms = time_config(torch, make, best, x_dev, e_dev, g_dev, stream, **kw)
+    timings[label(best)] = best_ms
+    for key, values in space.items():
+        for v in values:
+            if best.get(key) == v:
+                continue
+            cand = dict(best)
+            cand[key] = v
+            ms = time_config(torch, make, cand, x_dev, e_dev, g_dev, stream, **kw)
+            timings[label(cand)] = ms
+            if ms < best_ms:
+                best, best_ms = cand, ms
+    return best, timings

@@ -203,8 +203,7 @@ static int ensure_device(gfnff_handle* h) {
     if (h->dev) return GFNFF_STATUS_OK;
     std::string msg;
     int st = GFNFF_STATUS_OK;
-    h->dev.reset(gfnff::device_create(h->params, h->precision, h->opt.cholesky_lookahead, h->opt.eeq_solver,
-                                      h->opt.pcg_tol, st, msg));
+    h->dev.reset(gfnff::device_create(h->params, h->precision, h->opt, st, msg));
     if (!h->dev) return fail(h, st ? st : GFNFF_STATUS_DEVICE, "device set-up: " + msg);
     return GFNFF_STATUS_OK;
 }
@@ -260,6 +259,52 @@ int gfnff_energy_gradient_device(gfnff_handle* h, const double* xyz_bohr_dev, do
     }
 }
 
+int gfnff_graph_capture(gfnff_handle* h, const double* xyz_bohr_dev, double* energy_dev, double* grad_dev,
+                        void* stream) {
+    if (!h) return GFNFF_STATUS_ARGS;
+    if (!h->have_molecule || !xyz_bohr_dev || !energy_dev || !stream)
+        return fail(h, GFNFF_STATUS_ARGS, "gfnff_graph_capture: invalid arguments");
+    try {
+        int st = ensure_device(h);
+        if (st) return st;
+        std::string msg;
+        st = gfnff::device_graph_capture(h->dev.get(), xyz_bohr_dev, energy_dev, grad_dev, stream, msg);
+        if (st) return fail(h, st, "gfnff_graph_capture: " + msg);
+        return GFNFF_STATUS_OK;
+    } catch (const std::exception& e) {
+        return fail(h, GFNFF_STATUS_DEVICE, std::string("gfnff_graph_capture: ") + e.what());
+    }
+}
+
+int gfnff_md_steps(gfnff_handle* h, double* xyz_bohr_dev, double* vel_dev, double* grad_dev, const double* mass_amu,
+                   double dt_fs, double T0, double tau_fs, int32_t nsteps, double* record_dev, void* stream) {
+    if (!h) return GFNFF_STATUS_ARGS;
+    if (!h->have_molecule || !xyz_bohr_dev || !vel_dev || !grad_dev || !mass_amu || !record_dev || nsteps < 0 ||
+        !(dt_fs > 0.0) || (T0 > 0.0 && !(tau_fs > 0.0)))
+        return fail(h, GFNFF_STATUS_ARGS, "gfnff_md_steps: invalid arguments");
+    try {
+        int st = ensure_device(h);
+        if (st) return st;
+        // the coordinates of the last hydrogen/halogen-bond detection (set-up or update_geometry)
+        const std::vector<double>& ref = h->hb_ref.size() == h->xyz_bohr.size() ? h->hb_ref : h->xyz_bohr;
+        std::string msg;
+        st = gfnff::device_md_steps(h->dev.get(), xyz_bohr_dev, vel_dev, grad_dev, mass_amu, ref.data(), dt_fs, T0,
+                                    tau_fs, h->opt.hb_update_rmsd_bohr, nsteps, record_dev, stream, msg);
+        if (st) return fail(h, st, "gfnff_md_steps: " + msg);
+        return GFNFF_STATUS_OK;
+    } catch (const std::exception& e) {
+        return fail(h, GFNFF_STATUS_DEVICE, std::string("gfnff_md_steps: ") + e.what());
+    }
+}
+
+int gfnff_graph_launch(gfnff_handle* h, void* stream) {
+    if (!h) return GFNFF_STATUS_ARGS;
+    if (!h->dev) return fail(h, GFNFF_STATUS_ARGS, "gfnff_graph_launch: no captured evaluation");
+    std::string msg;
+    const int st = gfnff::device_graph_launch(h->dev.get(), stream, msg);
+    return st ? fail(h, st, "gfnff_graph_launch: " + msg) : GFNFF_STATUS_OK;
+}
+
 int gfnff_device_status(gfnff_handle* h, int32_t* flags, int32_t clear) {
     if (!h || !flags) return GFNFF_STATUS_ARGS;
     *flags = 0;
@@ -312,6 +357,19 @@ int gfnff_set_param(gfnff_handle* h, const char* key, double value) {
         h->dev.reset();
     } else if (k == "cholesky_lookahead") {
         h->opt.cholesky_lookahead = value != 0.0;
+        h->dev.reset();
+    } else if (k == "batch_solver") {
+        if (value != 0.0 && value != 1.0) return fail(h, GFNFF_STATUS_ARGS, "batch_solver: 0 or 1");
+        h->opt.batch_solver = (int)value;
+        h->dev.reset();
+    } else if (k == "gather_lanes") {
+        if (value != 0.0 && value != 16.0 && value != 64.0) return fail(h, GFNFF_STATUS_ARGS, "gather_lanes: 0, 16 or 64");
+        h->opt.gather_lanes = (int)value;
+        h->dev.reset();
+    } else if (k == "pcg_groups") {
+        if (!(value >= 0.0 && value <= 256.0) || value != std::floor(value))
+            return fail(h, GFNFF_STATUS_ARGS, "pcg_groups: 0 (all that fit) or 1..256");
+        h->opt.pcg_groups = (int)value;
         h->dev.reset();
     } else {
         return fail(h, GFNFF_STATUS_UNSUPPORTED, "unknown parameter: " + k);

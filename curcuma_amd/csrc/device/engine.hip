@@ -248,7 +248,9 @@ __device__ void cn_atom_finish(int i, double raw, int cnt, const int* __restrict
 // CN for a batch of molecules of at most CNMOL_MAX atoms each: one
 // workgroup per molecule stages the molecule's coordinates and radii once;
 // wave w scans atoms w, w + 4, ... against every atom of the molecule with
-// the ballot-ordered list construction of k_cn (ascending partners).
+// the ballot-ordered list construction of k_cn (ascending partners) and
+// leaves the raw CN and list length in LDS; the per-atom epilogue (log cap,
+// D4 weights, EEQ right-hand side) then runs one thread per atom.
 constexpr int CNMOL_MAX = 1024;
 __global__ __launch_bounds__(256) void k_cn_mol(const int* __restrict__ mol_off, const double* __restrict__ x,
                                                 const double* __restrict__ rcov, const int* __restrict__ elem,
@@ -258,7 +260,8 @@ __global__ __launch_bounds__(256) void k_cn_mol(const int* __restrict__ mol_off,
                                                 double* __restrict__ gw, double* __restrict__ dgw,
                                                 double* __restrict__ rhs, int* __restrict__ nbn,
                                                 int* __restrict__ nbl, int* __restrict__ flags) {
-    __shared__ double sx[CNMOL_MAX], sy[CNMOL_MAX], sz[CNMOL_MAX], sr[CNMOL_MAX];
+    __shared__ double sx[CNMOL_MAX], sy[CNMOL_MAX], sz[CNMOL_MAX], sr[CNMOL_MAX], sraw[CNMOL_MAX];
+    __shared__ int scnt[CNMOL_MAX];
     const int m = blockIdx.x, a0 = mol_off[m], nm = mol_off[m + 1] - a0;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     for (int k = tid; k < nm; k += 256) {
@@ -298,8 +301,14 @@ __global__ __launch_bounds__(256) void k_cn_mol(const int* __restrict__ mol_off,
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) raw += __shfl_xor(raw, off);
-        if (lane == 0) cn_atom_finish(i, raw, cnt, elem, nref, refcn, chi_base, cnf, cn, dlog, gw, dgw, rhs, nbn, flags);
+        if (lane == 0) {  // scan results; the per-atom epilogue runs below, one thread per atom
+            sraw[ii] = raw;
+            scnt[ii] = cnt;
+        }
     }
+    __syncthreads();
+    for (int k = tid; k < nm; k += 256)
+        cn_atom_finish(a0 + k, sraw[k], scnt[k], elem, nref, refcn, chi_base, cnf, cn, dlog, gw, dgw, rhs, nbn, flags);
 }
 
 // ---------------------------------------------------------------------------
@@ -3329,7 +3338,13 @@ __global__ __launch_bounds__(256) void k_batm(int nt, const int* __restrict__ ij
 // ---------------------------------------------------------------------------
 constexpr int BATCH_MAX_NP = 512;  // largest padded reduced system of a batched molecule
 
-// one 64 x 64 lower tile (m, bi, bj) of a molecule's reduced matrix per workgroup
+// One 64 x 64 lower tile (m, bi, bj) of a molecule's reduced matrix per
+// workgroup.  When the tile's rows share one last atom la and its columns
+// one last atom lb (always within one fragment, so for every tile of a
+// one-fragment molecule), the entries A(ia, lb), A(la, ib) and A(la, lb) are
+// evaluated once per row, column and tile into LDS and every element needs
+// one fresh A(ia, ib); the sum keeps the order of the direct four-term form,
+// so R is the same bit for bit.  Other tiles use the direct form.
 __global__ __launch_bounds__(256) void k_eeq_reduced_batch(const int4* __restrict__ tiles, const int* __restrict__ mnr,
                                                            const int* __restrict__ mnp,
                                                            const long long* __restrict__ mo,
@@ -3337,16 +3352,40 @@ __global__ __launch_bounds__(256) void k_eeq_reduced_batch(const int4* __restric
                                                            const int* __restrict__ rl, const double* __restrict__ x,
                                                            const double* __restrict__ alp,
                                                            const double* __restrict__ diag, double* __restrict__ A) {
+    __shared__ double urow[NB], wcol[NB];
+    __shared__ double sll;
     const int4 tt = tiles[blockIdx.x];
     const int m = tt.x, bi = tt.y, bj = tt.z;
     const int nr = mnr[m], np = mnp[m], base = vo[m];
+    const int tid = threadIdx.x;
     double* Am = A + mo[m];
-    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
-        const int r = bi * NB + e / NB, c = bj * NB + e % NB;
+    // shared last atoms of the tile's valid rows and columns (-1 when none is valid)
+    const int r0 = bi * NB, c0 = bj * NB;
+    const int la0 = r0 < nr ? rl[base + r0] : -1, lb0 = c0 < nr ? rl[base + c0] : -1;
+    bool same = true;
+    if (tid < NB) same = !(r0 + tid < nr) || rl[base + r0 + tid] == la0;
+    else if (tid < 2 * NB) same = !(c0 + tid - NB < nr) || rl[base + c0 + tid - NB] == lb0;
+    const bool uni = __syncthreads_and(same) && la0 >= 0 && lb0 >= 0;
+    if (uni) {
+        if (tid < NB) {
+            const int r = r0 + tid;
+            urow[tid] = r < nr ? eeq_a(ra[base + r], lb0, x, alp, diag) : 0.0;
+        } else if (tid < 2 * NB) {
+            const int c = c0 + tid - NB;
+            wcol[tid - NB] = c < nr ? eeq_a(la0, ra[base + c], x, alp, diag) : 0.0;
+        } else if (tid == 2 * NB) {
+            sll = eeq_a(la0, lb0, x, alp, diag);
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < NB * NB; e += blockDim.x) {
+        const int r = r0 + e / NB, c = c0 + e % NB;
         if (c > r) continue;
         double v;
         if (r >= nr || c >= nr) {
             v = (r == c) ? 1.0 : 0.0;
+        } else if (uni) {
+            v = eeq_a(ra[base + r], ra[base + c], x, alp, diag) - urow[e / NB] - wcol[e % NB] + sll;
         } else {
             const int ia = ra[base + r], la = rl[base + r], ib = ra[base + c], lb = rl[base + c];
             v = eeq_a(ia, ib, x, alp, diag) - eeq_a(ia, lb, x, alp, diag) - eeq_a(la, ib, x, alp, diag) +
@@ -3356,11 +3395,172 @@ __global__ __launch_bounds__(256) void k_eeq_reduced_batch(const int4* __restric
     }
 }
 
-// Factorisation and solve of one molecule's reduced system per workgroup:
-// the blocked right-looking Cholesky (diag / panel / MFMA tile update, the
-// routines of the single-molecule path) and both substitutions, all inside
-// the workgroup.  b, x: concatenated padded vectors.
-__global__ __launch_bounds__(256) void k_eeq_batch_solve(const int* __restrict__ order,
+// ---- per-molecule factorisation: tile phases shared out over wavefronts ----
+// One 64 x 64 output tile per wavefront on FP64 MFMA (v_mfma_f64_16x16x4f64),
+// in two 32-row halves; operands are read from memory (L2) one 4-column
+// step ahead of the MFMAs.  Lane layout: A(i = lane%16, k = lane/16),
+// B(k = lane/16, j = lane%16); result element v of a 16 x 16 block at row
+// 4 v + lane/16, column lane%16.
+//   SUB = true : A(ri.., rj..) -= P(ri.., 0..63) Q(rj.., 0..63)^T (trailing
+//                update; P = Q = the panel, leading dimension ldq = np;
+//                diagonal tiles keep the lower triangle)
+//   SUB = false: A(ri.., rj..)  = P(ri.., 0..63) Q(0.., 0..63)^T (panel
+//                solve, Q = the inverse of the diagonal block, ldq = 64)
+template <bool SUB>
+__device__ __forceinline__ void batch_tile_mfma(double* __restrict__ Am, int np, int ri, int rj,
+                                                const double* __restrict__ P, const double* __restrict__ Q,
+                                                int ldq) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const bool lower_only = SUB && ri == rj;
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+        const double* Pi = P + (size_t)(32 * h) * np;
+        dbl4 acc[2][4];
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) acc[x][y] = dbl4{0.0, 0.0, 0.0, 0.0};
+        double fa[2], fb[4];
+#pragma unroll
+        for (int x = 0; x < 2; ++x) fa[x] = Pi[(size_t)(16 * x + li) * np + lk];
+#pragma unroll
+        for (int y = 0; y < 4; ++y) fb[y] = Q[(size_t)(16 * y + li) * ldq + lk];
+#pragma unroll 2
+        for (int kk = 0; kk < NB; kk += 4) {
+            double na[2], nb[4];
+            if (kk + 4 < NB) {
+#pragma unroll
+                for (int x = 0; x < 2; ++x) na[x] = Pi[(size_t)(16 * x + li) * np + kk + 4 + lk];
+#pragma unroll
+                for (int y = 0; y < 4; ++y) nb[y] = Q[(size_t)(16 * y + li) * ldq + kk + 4 + lk];
+            }
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y)
+                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[x], fb[y], acc[x][y], 0, 0, 0);
+            if (kk + 4 < NB) {
+#pragma unroll
+                for (int x = 0; x < 2; ++x) fa[x] = na[x];
+#pragma unroll
+                for (int y = 0; y < 4; ++y) fb[y] = nb[y];
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = 32 * h + 16 * x + 4 * v + lk, c = 16 * y + li;
+                    double* p = Am + (size_t)(ri + r) * np + rj + c;
+                    if (!SUB) *p = acc[x][y][v];
+                    else if (!lower_only || c <= r) *p = *p - acc[x][y][v];
+                }
+    }
+}
+
+// Factorisation and solve of one molecule's reduced system per workgroup of
+// four wavefronts, right-looking by 64-column blocks.  Step k: the workgroup
+// factors the diagonal block and forms its inverse (chol_diag_block); the
+// panel tiles L(i, k) = A(i, k) L(k, k)^-T and then the trailing tiles
+// A(i, j) -= L(i, k) L(j, k)^T are computed one tile per wavefront on MFMA,
+// dealt round-robin, with one barrier after each phase.  The substitutions
+// use the stored inverses of the diagonal blocks.  Registers and LDS are
+// sized for four workgroups per CU, so that the diagonal-block phases of
+// one molecule overlap the MFMA phases of others.  b, x: concatenated padded
+// vectors.
+constexpr int BSOLVE_T = 256;
+// The diagonal-block routine as a call of its own, so that its register
+// footprint is not added to the state of the surrounding phases.
+__device__ __noinline__ void batch_diag_block(int np, int k0, double* __restrict__ A, double* __restrict__ Linv,
+                                              int* __restrict__ flags, double (*S0)[NB + 1], double (*col)[NB],
+                                              double* rdiag) {
+    chol_diag_block<double>(np, k0, A, Linv, flags, S0, col, rdiag);
+}
+__global__ __launch_bounds__(BSOLVE_T, 3) void k_eeq_batch_solve(const int* __restrict__ order,
+                                                                 const int* __restrict__ mnp,
+                                                                 const long long* __restrict__ mo,
+                                                                 const int* __restrict__ vo, double* __restrict__ A,
+                                                                 double* __restrict__ Linv,
+                                                                 const double* __restrict__ b,
+                                                                 double* __restrict__ x, int* __restrict__ flags) {
+    __shared__ double S0[NB][NB + 1];
+    __shared__ double col[2][NB];
+    __shared__ double rdiag[NB];
+    __shared__ double vec[BATCH_MAX_NP];
+    __shared__ double tmp[NB];
+    constexpr int NW = BSOLVE_T / 64;
+    const int m = order[blockIdx.x], np = mnp[m], nb = np / NB, base = vo[m];  // largest systems first
+    double* Am = A + mo[m];
+    double* Lm = Linv + (size_t)base * NB;
+    const int tid = threadIdx.x, wave = tid >> 6, t = tid >> 2, q = tid & 3;
+    for (int kb = 0; kb < nb; ++kb) {
+        const int k0 = kb * NB;
+        batch_diag_block(np, k0, Am, Lm, flags, S0, col, rdiag);
+        __syncthreads();
+        for (int bi = kb + 1 + wave; bi < nb; bi += NW)
+            batch_tile_mfma<false>(Am, np, bi * NB, k0, Am + (size_t)bi * NB * np + k0, Lm + (size_t)kb * NB * NB,
+                                   NB);
+        __syncthreads();
+        int tile = 0;
+        for (int bi = kb + 1; bi < nb; ++bi)
+            for (int bj = kb + 1; bj <= bi; ++bj, ++tile)
+                if (tile % NW == wave)
+                    batch_tile_mfma<true>(Am, np, bi * NB, bj * NB, Am + (size_t)bi * NB * np + k0,
+                                          Am + (size_t)bj * NB * np + k0, np);
+        __syncthreads();
+    }
+    for (int r = tid; r < np; r += BSOLVE_T) vec[r] = b[base + r];
+    __syncthreads();
+    // forward: y_kb = Linv_kb (b_kb - sum_{k < kb} L(kb, k) y_k), y kept in vec
+    for (int kb = 0; kb < nb; ++kb) {
+        double acc = (q == 0) ? vec[kb * NB + t] : 0.0;
+        for (int k = 0; k < kb; ++k)
+#pragma unroll
+            for (int u = 0; u < NB / 4; ++u)
+                acc -= Am[(size_t)(kb * NB + t) * np + k * NB + 4 * u + q] * vec[k * NB + 4 * u + q];
+        acc += __shfl_xor(acc, 1);
+        acc += __shfl_xor(acc, 2);
+        if (q == 0) tmp[t] = acc;
+        __syncthreads();
+        const double* Li = Lm + (size_t)kb * NB * NB;
+        double s = 0.0;
+#pragma unroll
+        for (int u = 0; u < NB / 4; ++u) s += Li[t * NB + 4 * u + q] * tmp[4 * u + q];
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        if (q == 0) vec[kb * NB + t] = s;
+        __syncthreads();
+    }
+    // backward: x_kb = Linv_kb^T (y_kb - sum_{k > kb} L(k, kb)^T x_k)
+    for (int kb = nb - 1; kb >= 0; --kb) {
+        double acc = (q == 0) ? vec[kb * NB + t] : 0.0;
+        for (int k = nb - 1; k > kb; --k)
+#pragma unroll
+            for (int u = 0; u < NB / 4; ++u)
+                acc -= Am[(size_t)(k * NB + 4 * u + q) * np + kb * NB + t] * vec[k * NB + 4 * u + q];
+        acc += __shfl_xor(acc, 1);
+        acc += __shfl_xor(acc, 2);
+        if (q == 0) tmp[t] = acc;
+        __syncthreads();
+        const double* Li = Lm + (size_t)kb * NB * NB;
+        double s = 0.0;
+#pragma unroll
+        for (int u = 0; u < NB / 4; ++u) s += Li[(4 * u + q) * NB + t] * tmp[4 * u + q];
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        if (q == 0) vec[kb * NB + t] = s;
+        __syncthreads();
+    }
+    for (int r = tid; r < np; r += BSOLVE_T) x[base + r] = vec[r];
+}
+
+// Previous per-molecule solve (whole workgroup per tile step), kept for
+// comparison: the blocked right-looking Cholesky (diag / panel / MFMA tile
+// update, the routines of the single-molecule path) and both substitutions,
+// all inside the workgroup.  b, x: concatenated padded vectors.
+__global__ __launch_bounds__(256) void k_eeq_batch_solve_tiles(const int* __restrict__ order,
                                                          const int* __restrict__ mnp, const long long* __restrict__ mo,
                                                          const int* __restrict__ vo, double* __restrict__ A,
                                                          double* __restrict__ Linv, const double* __restrict__ b,
@@ -3919,6 +4119,91 @@ __global__ void k_fill_u64(int n, unsigned long long* __restrict__ a, unsigned l
     if (i < n) a[i] = v;
 }
 
+// ---------------------------------------------------------------------------
+// K12: device-resident molecular dynamics.  Velocity Verlet as in
+// SimpleMD::Verlet (simplemd.cpp:2580-2673) with the optional Berendsen bath
+// of SimpleMD::Berendson (:3967-3975), in atomic units (Bohr, electron
+// masses, atomic time units; gradient in Eh/Bohr).  One step:
+//   k_md_drift   x += dt v - dt^2/2 g/m,  v -= dt/2 g/m   (g of the old x)
+//   evaluation   g, Epot at the new x
+//   k_md_kick    v -= dt/2 g/m; per-block partial sums of m v^2 and of the
+//                displacement |x - x_ref|^2 since the last HB/XB detection
+//   k_md_finish  Ekin, T = 2 Ekin / (kB dof), Berendsen factor, the step's
+//                record (Epot, Ekin, T, Epot + Ekin), and GFNFF flag 8 when
+//                sqrt(sum dx^2) / N exceeds the re-detection threshold
+//                (gfnff_method.cpp:2243-2251)
+//   k_md_scale   v *= lambda (bath only)
+// Sums are formed in a fixed order; nothing is copied to the host.
+// ---------------------------------------------------------------------------
+constexpr int FLAG_HB_STALE = 8;
+constexpr int MD_T = 256;
+
+__global__ __launch_bounds__(MD_T) void k_md_drift(int n3, double dt, const double* __restrict__ im3,
+                                                   const double* __restrict__ g, double* __restrict__ x,
+                                                   double* __restrict__ v) {
+    const int k = blockIdx.x * MD_T + threadIdx.x;
+    if (k >= n3) return;
+    const double a = g[k] * im3[k];
+    x[k] = x[k] + dt * v[k] - 0.5 * a * dt * dt;
+    v[k] = v[k] - 0.5 * dt * a;
+}
+
+__global__ __launch_bounds__(MD_T) void k_md_kick(int n3, double dt, const double* __restrict__ im3,
+                                                  const double* __restrict__ m3, const double* __restrict__ g,
+                                                  const double* __restrict__ x, const double* __restrict__ xref,
+                                                  double* __restrict__ v, double* __restrict__ part) {
+    __shared__ double sk[MD_T / 64], sd[MD_T / 64];
+    const int k = blockIdx.x * MD_T + threadIdx.x;
+    double ek = 0.0, dd = 0.0;
+    if (k < n3) {
+        const double vk = v[k] - 0.5 * dt * g[k] * im3[k];
+        v[k] = vk;
+        ek = m3[k] * vk * vk;
+        const double dx = x[k] - xref[k];
+        dd = dx * dx;
+    }
+    ek = wave_sum(ek);
+    dd = wave_sum(dd);
+    if ((threadIdx.x & 63) == 0) {
+        sk[threadIdx.x >> 6] = ek;
+        sd[threadIdx.x >> 6] = dd;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = (sk[0] + sk[1]) + (sk[2] + sk[3]);
+        part[2 * blockIdx.x + 1] = (sd[0] + sd[1]) + (sd[2] + sd[3]);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_md_finish(int nb, const double* __restrict__ part,
+                                                  const double* __restrict__ epot, double kb_dof, double dt, double T0,
+                                                  double tau, double disp2_max, double* __restrict__ lambda,
+                                                  double* __restrict__ rec, int* __restrict__ sticky) {
+    const int lane = threadIdx.x;
+    double ek = 0.0, dd = 0.0;
+    for (int b = lane; b < nb; b += 64) {
+        ek += part[2 * b];
+        dd += part[2 * b + 1];
+    }
+    ek = 0.5 * wave_sum(ek);
+    dd = wave_sum(dd);
+    if (lane != 0) return;
+    const double T = 2.0 * ek / kb_dof;
+    double lam = 1.0;
+    if (T0 > 0.0 && T > 0.0) lam = sqrt(1.0 + (dt / 2.0 * (T0 - T)) / (T * tau));
+    *lambda = lam;
+    rec[0] = epot[0];
+    rec[1] = ek;
+    rec[2] = T;
+    rec[3] = epot[0] + ek;
+    if (dd > disp2_max) atomicOr(sticky, FLAG_HB_STALE);
+}
+
+__global__ __launch_bounds__(MD_T) void k_md_scale(int n3, const double* __restrict__ lambda, double* __restrict__ v) {
+    const int k = blockIdx.x * MD_T + threadIdx.x;
+    if (k < n3) v[k] *= *lambda;
+}
+
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
 }  // namespace
@@ -3989,6 +4274,7 @@ struct DeviceEngine {
     Lookahead la;                       // side stream of the factorisation
     bool use_lookahead = false;
     bool use_cg = false;                // iterative EEQ solve (single molecules)
+    int batch_solver = 0;               // batches: 0 k_eeq_batch_solve, 1 k_eeq_batch_solve_tiles
     int cg_groups = 0;                  // workgroups of the persistent PCG kernel
     double cg_tol2 = 0.0;               // squared relative residual at which the PCG solve stops
     double *cg_z = nullptr, *cg_p = nullptr;
@@ -4096,7 +4382,18 @@ struct DeviceEngine {
             throw std::runtime_error("upload failed");
         return d;
     }
+    double *md_im3 = nullptr, *md_m3 = nullptr, *md_xref = nullptr, *md_part = nullptr, *md_e = nullptr,
+           *md_lambda = nullptr;        // molecular dynamics state (device_md_steps)
+    hipGraph_t graph = nullptr;         // captured device-resident evaluation (gfnff_graph_capture)
+    hipGraphExec_t gexec = nullptr;
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+    }
     ~DeviceEngine() {
+        drop_graph();
         if (la.side) {
             (void)hipStreamSynchronize(la.side);
             (void)hipStreamDestroy(la.side);
@@ -4117,8 +4414,10 @@ struct DeviceEngine {
     }
 };
 
-DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, int solver, double pcg_tol,
-                            int& status, std::string& msg) {
+DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt, int& status, std::string& msg) {
+    const bool lookahead = opt.cholesky_lookahead;
+    const int solver = opt.eeq_solver;
+    const double pcg_tol = opt.pcg_tol;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         status = kErrDevice;
@@ -4141,6 +4440,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
     E->mixed = precision == 1;
     E->use_lookahead = lookahead;
     E->use_cg = solver == 0 && P.nmol == 1;
+    E->batch_solver = opt.batch_solver;
     {
         const double tol = pcg_tol > 0.0 ? pcg_tol : (E->mixed ? PCG_TOL_MIXED : PCG_TOL_FP64);
         E->cg_tol2 = tol * tol;
@@ -4544,6 +4844,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
             E->hb_xidx = E->upload(xidx);
             // lanes per atom of the gather: 16 when the atoms' lists are short on average
             E->hbg_lpa = ((long long)crange[(size_t)2 * (n + 1) + n] + xoff[n] < 192LL * n) ? 16 : 64;
+            if (opt.gather_lanes) E->hbg_lpa = opt.gather_lanes;
             E->hb_gout = E->alloc<double>(3 * (size_t)std::max(E->hb_ngt, 1));
             E->hb_xslots = E->alloc<double>(3 * std::max<size_t>(xatom.size(), 1));
             E->e_hb2 = E->alloc<double>(std::max(E->nhb2, 1));
@@ -4594,6 +4895,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
         }
         E->slot_off = E->upload(soff);
         E->fin_lpa = ((long long)soff[n] < 192LL * n) ? 16 : 64;  // lanes per atom of k_finish
+        if (opt.gather_lanes) E->fin_lpa = opt.gather_lanes;
         E->slot_idx = E->upload(sidx);
         std::vector<int> doff(n + 1, 0), didx(2 * E->nbond);
         for (int b = 0; b < E->nbond; ++b) {
@@ -4633,6 +4935,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, in
             }
             const long T = (long)nbk * (nbk + 1) / 2;
             E->cg_groups = (int)std::min<long>({(long)ncu * per_cu, 256L, T});
+            if (opt.pcg_groups > 0) E->cg_groups = std::min(E->cg_groups, opt.pcg_groups);
             if (E->cg_groups < 1 || cdiv(nbk, E->cg_groups) > PCG_MAXB) E->use_cg = false;
             if (E->use_cg) E->mat_size = (size_t)T * NB * NB;  // packed symmetric storage
         }
@@ -4984,8 +5287,12 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     HIP_OK(hipStreamWaitEvent(s, E->ev_join, 0));
     if (E->nmol > 1) {
         // batch: every molecule's system factored and solved by one workgroup (FP64)
-        LAUNCH(KC_BATCH_SOLVE, k_eeq_batch_solve, dim3(E->nmol), dim3(256), s, E->morder, E->mnp, E->mmo, E->mvo,
-               E->Ared, E->Linv, E->b, E->xbuf, E->flags);
+        if (E->batch_solver == 1)
+            LAUNCH(KC_BATCH_SOLVE, k_eeq_batch_solve_tiles, dim3(E->nmol), dim3(256), s, E->morder, E->mnp, E->mmo,
+                   E->mvo, E->Ared, E->Linv, E->b, E->xbuf, E->flags);
+        else
+            LAUNCH(KC_BATCH_SOLVE, k_eeq_batch_solve, dim3(E->nmol), dim3(BSOLVE_T), s, E->morder, E->mnp, E->mmo,
+                   E->mvo, E->Ared, E->Linv, E->b, E->xbuf, E->flags);
     } else if (E->use_cg) {
         // the persistent PCG (block-Jacobi factors and mailboxes were set up on the side stream)
         LAUNCH(KC_PCG, k_eeq_pcg, dim3(E->cg_groups), dim3(PCG_THREADS), s, np, E->Ared, E->Linv, E->b, E->xbuf,
@@ -5152,6 +5459,107 @@ int device_eval(DeviceEngine* E, const double* xyz_bohr_dev, double* energy_dev,
         msg = "kernel launch failed";
         return kErrDevice;
     }
+    return kOk;
+}
+
+// Records one device-resident evaluation with fixed buffers into a HIP graph
+// (both streams: the fork and join events become graph dependencies) and
+// instantiates it; nothing runs during the capture.  The evaluation queues
+// no memset or host synchronisation, and the kernel-class timer is off
+// while recording.
+int device_graph_capture(DeviceEngine* E, const double* xyz_bohr_dev, double* energy_dev, double* grad_dev,
+                         void* stream, std::string& msg) {
+    hipStream_t s = (hipStream_t)stream;
+    if (!s) {
+        msg = "graph capture needs a created stream (not the default stream)";
+        return kErrArgs;
+    }
+    E->drop_graph();
+    const int cls = E->timer.cls;
+    E->timer.cls = -1;
+    HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int st = device_eval(E, xyz_bohr_dev, energy_dev, grad_dev, stream, msg);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(s, &g);
+    E->timer.cls = cls;
+    if (st != kOk || ce != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        if (st == kOk) msg = std::string("hipStreamEndCapture: ") + hipGetErrorString(ce);
+        return st != kOk ? st : kErrDevice;
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        msg = std::string("hipGraphInstantiate: ") + hipGetErrorString(ie);
+        return kErrDevice;
+    }
+    E->graph = g;
+    E->gexec = ex;
+    return kOk;
+}
+
+int device_graph_launch(DeviceEngine* E, void* stream, std::string& msg) {
+    if (!E->gexec) {
+        msg = "no captured evaluation (the engine was rebuilt, or gfnff_graph_capture was not called)";
+        return kErrArgs;
+    }
+    HIP_OK(hipGraphLaunch(E->gexec, (hipStream_t)stream));
+    return kOk;
+}
+
+int device_md_steps(DeviceEngine* E, double* x_dev, double* v_dev, double* grad_dev, const double* mass_amu,
+                    const double* xref_bohr, double dt_fs, double T0, double tau_fs, double disp_thresh, int nsteps,
+                    double* rec_dev, void* stream, std::string& msg) {
+    const int n = E->n, n3 = 3 * n;
+    hipStream_t s = (hipStream_t)stream;
+    constexpr double kAmuToMe = 1822.8884850;    // electron masses per amu (global.h:46)
+    constexpr double kFsToAu = 41.34137314;      // atomic time units per fs (global.h:50)
+    constexpr double kBoltzEh = 3.166811e-6;     // Hartree per Kelvin (global.h:47)
+    if (!E->md_im3) {
+        try {
+            E->md_im3 = E->alloc<double>(n3);
+            E->md_m3 = E->alloc<double>(n3);
+            E->md_xref = E->alloc<double>(n3);
+            E->md_part = E->alloc<double>(2 * (size_t)cdiv(n3, MD_T));
+            E->md_e = E->alloc<double>(GFNFF_NTERMS + 1);
+            E->md_lambda = E->alloc<double>(1);
+        } catch (const std::bad_alloc&) {
+            msg = "device allocation failed";
+            return kErrDevice;
+        }
+    }
+    std::vector<double> m3(n3), im3(n3);
+    for (int i = 0; i < n; ++i) {
+        if (!(mass_amu[i] > 0.0)) {
+            msg = "masses must be positive";
+            return kErrArgs;
+        }
+        for (int c = 0; c < 3; ++c) {
+            m3[3 * i + c] = mass_amu[i] * kAmuToMe;
+            im3[3 * i + c] = 1.0 / (mass_amu[i] * kAmuToMe);
+        }
+    }
+    // once per call (not per step): masses and the coordinates of the last HB/XB detection
+    HIP_OK(hipStreamSynchronize(s));
+    HIP_OK(hipMemcpy(E->md_m3, m3.data(), sizeof(double) * n3, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(E->md_im3, im3.data(), sizeof(double) * n3, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(E->md_xref, xref_bohr, sizeof(double) * n3, hipMemcpyHostToDevice));
+    const double dt = dt_fs * kFsToAu, tau = tau_fs * kFsToAu;
+    const int dof = n > 2 ? 3 * n - 6 : (n == 2 ? 1 : 3);
+    const double disp2_max = (disp_thresh * n) * (disp_thresh * n);
+    const int nb = cdiv(n3, MD_T);
+    for (int step = 0; step < nsteps; ++step) {
+        hipLaunchKernelGGL(k_md_drift, dim3(nb), dim3(MD_T), 0, s, n3, dt, E->md_im3, grad_dev, x_dev, v_dev);
+        const int st = device_eval(E, x_dev, E->md_e, grad_dev, stream, msg);
+        if (st) return st;
+        hipLaunchKernelGGL(k_md_kick, dim3(nb), dim3(MD_T), 0, s, n3, dt, E->md_im3, E->md_m3, grad_dev, x_dev,
+                           E->md_xref, v_dev, E->md_part);
+        hipLaunchKernelGGL(k_md_finish, dim3(1), dim3(64), 0, s, nb, E->md_part, E->md_e, kBoltzEh * dof, dt, T0, tau,
+                           disp2_max, E->md_lambda, rec_dev + 4 * (size_t)step, E->sticky);
+        if (T0 > 0.0) hipLaunchKernelGGL(k_md_scale, dim3(nb), dim3(MD_T), 0, s, n3, E->md_lambda, v_dev);
+    }
+    HIP_OK(hipGetLastError());
     return kOk;
 }
 

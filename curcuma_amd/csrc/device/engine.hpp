@@ -12,12 +12,12 @@ namespace gfnff {
 struct DeviceEngine;
 
 // Uploads the parameter set; returns nullptr and sets status/msg on failure.
-// solver: 0 = preconditioned CG on the reduced EEQ system (single molecules;
-// batches always use the per-molecule Cholesky), 1 = blocked Cholesky.
-// pcg_tol: relative residual of the PCG solve (0: 1e-13 for FP64, 1e-10 for
-// the mixed path, whose accuracy bar is 1e-6 Eh/atom).
-DeviceEngine* device_create(const ParamSet& P, int precision, bool lookahead, int solver, double pcg_tol,
-                            int& status, std::string& msg);
+// opt.eeq_solver: 0 = preconditioned CG on the reduced EEQ system (single
+// molecules; batches always use the per-molecule Cholesky), 1 = blocked
+// Cholesky.  opt.pcg_tol: relative residual of the PCG solve (0: 1e-13 for
+// FP64, 1e-10 for the mixed path, whose accuracy bar is 1e-6 Eh/atom).
+// opt.cholesky_lookahead, opt.batch_solver: see Options.
+DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt, int& status, std::string& msg);
 void device_destroy(DeviceEngine* e);
 
 // Host-buffer evaluation: uploads xyz (Bohr), evaluates, downloads
@@ -28,6 +28,20 @@ int device_eval_host(DeviceEngine* e, const double* xyz_bohr, bool want_grad, do
 // Device-resident evaluation, asynchronous on `stream`.
 int device_eval(DeviceEngine* e, const double* xyz_bohr_dev, double* energy_dev, double* grad_dev, void* stream,
                 std::string& msg);
+
+// Graph capture of one device-resident evaluation (fixed buffers, non-default
+// stream) and its replay; the graph lives as long as the engine.
+int device_graph_capture(DeviceEngine* e, const double* xyz_bohr_dev, double* energy_dev, double* grad_dev,
+                         void* stream, std::string& msg);
+int device_graph_launch(DeviceEngine* e, void* stream, std::string& msg);
+
+// Device-resident velocity-Verlet MD: nsteps steps queued on `stream`, x/v/grad
+// device arrays updated in place (grad must hold the gradient at x on entry),
+// rec_dev receives (Epot, Ekin, T, Etot) per step; flag 8 in the status word
+// when sqrt(sum |x - xref|^2) / N exceeds disp_thresh.  Units: see engine.hip K12.
+int device_md_steps(DeviceEngine* e, double* x_dev, double* v_dev, double* grad_dev, const double* mass_amu,
+                    const double* xref_bohr, double dt_fs, double T0, double tau_fs, double disp_thresh, int nsteps,
+                    double* rec_dev, void* stream, std::string& msg);
 
 int device_count();
 
@@ -63,7 +77,6 @@ int device_dense_cholesky_f32(int n, const float* a, float* l_out, std::string& 
 // array, n <= 1024); asynchronous on `stream`, status_dev receives 0 or kErrEEQ.
 int device_dense_cholesky_noalloc(int n, const double* a_dev, double* l_dev, int* status_dev, void* stream,
                                   std::string& msg);
-
 
 }  // namespace gfnff
 

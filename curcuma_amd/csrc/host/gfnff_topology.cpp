@@ -2216,6 +2216,31 @@ void detect_hbxb(const Mol& m, const Topo& t, const Options& opt, ParamSet& P, b
     }
 }
 
+// EEQ electrostatic energy of charges q with the element parameters
+// (GFNFF::calculateEEQEnergy, gfnff_method.cpp:7505-7553):
+//   sum_{i<j} q_i q_j erf(g r) / r,  g = 1 / sqrt(a_i + a_j)
+//   + sum_i [-q_i (chi_i + cnf_i sqrt(cn_i)) + q_i^2 (gam_i + sqrt(2/pi) / sqrt(a_i)) / 2],
+// a = alpha^2; used to place the charge of a two-fragment system.
+double eeq_energy(const Mol& m, const std::vector<double>& q, const std::vector<double>& cn) {
+    const double TSQRT2PI = 0.79788456080287;
+    double e = 0.0;
+    for (int i = 0; i < m.n; ++i) {
+        const int zi = m.Z[i];
+        const double ai = alp[zi - 1] * alp[zi - 1];
+        for (int j = 0; j < i; ++j) {
+            const int zj = m.Z[j];
+            const double aj = alp[zj - 1] * alp[zj - 1];
+            const double r = std::sqrt(m.dist2(i, j));
+            const double g = 1.0 / std::sqrt(ai + aj);
+            e += q[i] * q[j] * (std::erf(g * r) / r);
+        }
+        const double self = gam[zi - 1] + TSQRT2PI / std::sqrt(ai);
+        const double en = chi[zi - 1] + cnf[zi - 1] * std::sqrt(cn[i]);
+        e += -q[i] * en + 0.5 * q[i] * q[i] * self;
+    }
+    return e;
+}
+
 }  // namespace
 
 int build_params(int n, const int32_t* Z, const double* xyz, int charge, const int32_t* forced, int nforced,
@@ -2257,15 +2282,11 @@ int build_params(int n, const int32_t* Z, const double* xyz, int charge, const i
     Topo t;
     detect_bonds(m, forced, nforced, t);
     detect_fragments(n, t);
+    // molecular charge over the fragments (gfnff_method.cpp:9062-9101): one
+    // fragment takes it; more than two put it on fragment 0; two fragments try
+    // both placements below and keep the lower EEQ energy
     t.qfrag.assign(t.nfrag, 0.0);
-    if (t.nfrag == 1) t.qfrag[0] = charge;
-    else if (charge != 0) {
-        if (t.nfrag == 2) {
-            msg = "charged two-fragment systems are not supported yet";
-            return kErrUnsupported;
-        }
-        t.qfrag[0] = charge;
-    }
+    if (t.nfrag == 1 || charge != 0) t.qfrag[0] = charge;
     topo_cn(m, t);
     hybridization(m, t);
     pi_systems(m, t);
@@ -2276,6 +2297,17 @@ int build_params(int n, const int32_t* Z, const double* xyz, int charge, const i
         t.is_metal[i] = metal_type[z - 1] > 0 || (z >= 21 && z <= 30) || (z >= 39 && z <= 48) || (z >= 57 && z <= 80);
     }
     if (!phase1_charges(m, t, opt.threads, msg)) return kErrEEQ;
+    if (t.nfrag == 2 && charge != 0) {
+        const double ea = eeq_energy(m, t.qa, t.cn);
+        const std::vector<double> qa_a = t.qa;
+        t.qfrag = {0.0, (double)charge};
+        if (!phase1_charges(m, t, opt.threads, msg)) return kErrEEQ;
+        const double eb = eeq_energy(m, t.qa, t.cn);
+        if (ea <= eb) {
+            t.qfrag = {(double)charge, 0.0};
+            t.qa = qa_a;
+        }
+    }
     t.dxi = calc_dxi(m, t);
     t.alpeeq = calc_alpeeq(m, t);
     t.dgam = calc_dgam(m, t, t.qa);

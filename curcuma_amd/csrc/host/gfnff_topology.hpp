@@ -37,6 +37,9 @@ struct Options {
     double hb_update_rmsd_bohr = 0.3;  // HB/XB re-detection threshold on sqrt(sum dx^2)/N
     int eeq_solver = 0;               // 0: preconditioned CG (single molecules), 1: blocked Cholesky
     double pcg_tol = 0.0;             // PCG relative residual |r| / |b|; 0: 1e-13 (FP64), 1e-10 (mixed)
+    int batch_solver = 0;             // batches: 0 wavefront-parallel per-molecule Cholesky, 1 tile-step variant
+    int gather_lanes = 0;             // lanes per atom of the slot and HB gathers: 0 by list length, 16 or 64
+    int pcg_groups = 0;               // workgroups of the persistent PCG: 0 all that fit (at most 256), else at most this
 };
 
 // Error codes shared with the C-ABI.

@@ -149,6 +149,29 @@ class GFNFFMethod:
         """All per-term energies of the last evaluation, by engine term name."""
         return {name: float(v) for name, v in zip(TERM_NAMES, self._terms) if not name.startswith("reserved")}
 
+    def energyGradientDevice(self, xyz_bohr_dev: int, energy_dev: int, grad_dev: int | None, stream: int | None) -> None:
+        """Device-resident evaluation (gfnff_energy_gradient_device); pointers and
+        the hipStream_t as integers, queued without synchronising."""
+        st = self._lib.gfnff_energy_gradient_device(self._h, C.c_void_p(xyz_bohr_dev), C.c_void_p(energy_dev),
+                                                    C.c_void_p(grad_dev) if grad_dev else None,
+                                                    C.c_void_p(stream) if stream else None)
+        if st != 0:
+            raise RuntimeError(self.getErrorMessage())
+
+    def captureGraph(self, xyz_bohr_dev: int, energy_dev: int, grad_dev: int | None, stream: int) -> None:
+        """Records one device-resident evaluation with these buffers into a HIP
+        graph (gfnff_graph_capture); `stream` must be a created stream."""
+        st = self._lib.gfnff_graph_capture(self._h, C.c_void_p(xyz_bohr_dev), C.c_void_p(energy_dev),
+                                           C.c_void_p(grad_dev) if grad_dev else None, C.c_void_p(stream))
+        if st != 0:
+            raise RuntimeError(self.getErrorMessage())
+
+    def launchGraph(self, stream: int | None) -> None:
+        """Replays the captured evaluation (gfnff_graph_launch) on `stream`."""
+        st = self._lib.gfnff_graph_launch(self._h, C.c_void_p(stream) if stream else None)
+        if st != 0:
+            raise RuntimeError(self.getErrorMessage())
+
     def deviceStatus(self, clear: bool = True) -> int:
         """GFNFF_FLAG_* bits raised by device-resident evaluations since the last clearing read."""
         f = C.c_int32(0)

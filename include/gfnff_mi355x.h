@@ -97,10 +97,47 @@ int gfnff_energy_gradient(gfnff_handle* h, int32_t want_grad, double* energy, do
 int gfnff_energy_gradient_device(gfnff_handle* h, const double* xyz_bohr_dev, double* energy_dev, double* grad_dev,
                                  void* stream);
 
+/* Engine extension (no counterpart in the reference, whose CPU evaluation
+ * has no launches to replay): records one device-resident evaluation with
+ * these fixed buffers (as gfnff_energy_gradient_device) into a HIP graph,
+ * both streams of the engine included, and instantiates it.  Nothing runs
+ * during the capture.  `stream` must be a created hipStream_t (capture is not
+ * possible on the default stream).  gfnff_graph_launch replays it on any
+ * stream: every replay evaluates the coordinates then in xyz_bohr_dev and
+ * writes energy_dev and grad_dev.  The graph is dropped when the engine is
+ * rebuilt (new molecule, parameters, options, or a hydrogen-bond list change
+ * found by gfnff_update_geometry); gfnff_graph_launch then returns
+ * GFNFF_STATUS_ARGS and a new capture is needed. */
+int gfnff_graph_capture(gfnff_handle* h, const double* xyz_bohr_dev, double* energy_dev, double* grad_dev,
+                        void* stream);
+int gfnff_graph_launch(gfnff_handle* h, void* stream);
+
+/* Device-resident molecular dynamics (replaces the per-step energy calls of
+ * SimpleMD: simplemd.cpp:3866-3890 FastEnergy inside SimpleMD::Verlet
+ * :2580-2673).  Queues nsteps velocity-Verlet steps on `stream`, each one
+ * device-resident evaluation of the handle's molecule plus a few update
+ * kernels, with no host transfer per step.  xyz_bohr_dev, vel_dev (Bohr per
+ * atomic time unit) and grad_dev (Eh/Bohr) are natoms x 3 device arrays
+ * updated in place; grad_dev must hold the gradient at xyz_bohr_dev on entry
+ * (one gfnff_energy_gradient_device call) and holds it on return.
+ * mass_amu: natoms host masses in amu; dt_fs: time step in fs.  T0 > 0
+ * couples to a Berendsen bath of time constant tau_fs (SimpleMD::Berendson,
+ * simplemd.cpp:3967-3975; the bath scales after the second half-kick, as
+ * there), T0 <= 0 runs NVE.  record_dev (device, nsteps x 4 doubles)
+ * receives Epot, Ekin, T (K, 3N - 6 degrees of freedom) and Epot + Ekin per
+ * step.  When the displacement since the last hydrogen/halogen-bond
+ * detection, sqrt(sum dx^2) / N, exceeds hb_update_rmsd_bohr
+ * (gfnff_method.cpp:2243-2251), GFNFF_FLAG_HB_STALE is raised in the status
+ * word: the caller then reads the coordinates back, calls
+ * gfnff_update_geometry (which re-detects the lists) and continues. */
+int gfnff_md_steps(gfnff_handle* h, double* xyz_bohr_dev, double* vel_dev, double* grad_dev, const double* mass_amu,
+                   double dt_fs, double T0, double tau_fs, int32_t nsteps, double* record_dev, void* stream);
+
 /* Device-side failure flags of an evaluation (bit mask). */
 #define GFNFF_FLAG_EEQ_NOT_SPD 1   /* EEQ matrix not positive definite (Cholesky pivot) */
 #define GFNFF_FLAG_CN_OVERFLOW 2   /* more than 128 atoms within the 6 Bohr CN cut-off of one atom */
 #define GFNFF_FLAG_SOLVE 4         /* EEQ solve did not converge or a bounded wait timed out */
+#define GFNFF_FLAG_HB_STALE 8      /* gfnff_md_steps: moved past the HB/XB re-detection threshold (not a failure) */
 
 /* Engine extension (no counterpart in the reference, whose CPU evaluation
  * reports through hasError()): the GFNFF_FLAG_* bits raised by every
@@ -122,7 +159,13 @@ int gfnff_cn(gfnff_handle* h, double* cn);
  * |r| / |b| at which the CG solve stops (0 = default: 1e-13 for FP64, 1e-10
  * for the mixed path, at most 1e-6); "cholesky_lookahead" 0/1 overlaps the
  * trailing update with the next super-panel on a second stream (off by
- * default). */
+ * default).  Launch-shape keys (results unchanged, chosen by the autotuner,
+ * curcuma_amd/autotune.py): "batch_solver" 0 = per-molecule Cholesky with
+ * the tile phases shared out over wavefronts (default), 1 = one tile step
+ * at a time per workgroup; "gather_lanes" 0 (by mean list length), 16 or 64
+ * lanes per atom in the slot and hydrogen-bond gathers; "pcg_groups" 0 (all
+ * workgroups that fit, at most 256) or an upper bound on the persistent
+ * PCG's workgroups. */
 int gfnff_set_param(gfnff_handle* h, const char* key, double value);
 
 /* GFNFF_PREC_FP64 (default, the reference's arithmetic) or GFNFF_PREC_MIXED:
@@ -218,7 +261,6 @@ int gfnff_dense_cholesky_f32(int32_t n, const float* a, float* l_out);
 #define GFNFF_NOALLOC_MAX_N 1024
 #define GFNFF_NOALLOC_SCRATCH_BYTES (8 * (1024 * 1024 + 1024 * 64 + 32))
 int gfnff_dense_cholesky_noalloc(int32_t n, const double* a_dev, double* l_dev, int32_t* status_dev, void* stream);
-
 
 #ifdef __cplusplus
 }

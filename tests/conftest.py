@@ -6,6 +6,11 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+try:  # loaded before the engine library, so that tests handing torch device memory to it share one HIP runtime
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = Path(__file__).resolve().parent.parent
 sys.path.insert(0, str(ROOT))
 
@@ -119,6 +124,14 @@ class HipRuntime:
         self.rt.hipFree.argtypes = [C.c_void_p]
         self.rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         self.rt.hipDeviceSynchronize.argtypes = []
+        self.rt.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.rt.hipStreamDestroy.argtypes = [C.c_void_p]
+        self.rt.hipStreamSynchronize.argtypes = [C.c_void_p]
+
+    def stream(self) -> C.c_void_p:
+        s = C.c_void_p()
+        assert self.rt.hipStreamCreate(C.byref(s)) == 0
+        return s
 
     def upload(self, a: np.ndarray) -> C.c_void_p:
         a = np.ascontiguousarray(a)

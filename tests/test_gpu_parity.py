@@ -272,3 +272,18 @@ def test_batch_cn_neighbour_lists_bit_exact():
             assert got == [a0 + j for j in ref[i - a0]], (k, i)
             total += len(got)
     print(f"batch of {len(off) - 1} molecules, {n} atoms, {total // 2} pairs, lists identical")
+
+
+@pytest.mark.parametrize("ion,charge", [("H3O+", 1), ("OH-", -1)])
+def test_charged_two_fragments(ion, charge, oracle):
+    """A charged two-fragment system (charge placed by the lower EEQ energy
+    of both placements) on the GPU against the oracle on the same parameters."""
+    from test_abi import _ion_water
+    Z, xyz, _ = _ion_water(ion, True)
+    m = GFNFFMethod()
+    assert m.setMolecule(Mol(Z, xyz, charge=charge)), m.getErrorMessage()
+    e = m.calculateEnergy(True)
+    assert not m.hasError(), m.getErrorMessage()
+    ref = oracle.eval(m.params_view(), np.asarray(xyz) * (1.0 / BOHR))
+    check(m, e, ref, len(Z))
+    assert abs(m.getCharges().sum() - charge) < 1e-10
