@@ -60,6 +60,8 @@ def parse():
     p.add_argument("--no-graph", action="store_true",
                    help="time eager launches for the headline instead of replays of the captured HIP graph")
     p.add_argument("--roofline-kernel", default="auto", help="kernel class timed for the roofline object")
+    p.add_argument("--param", action="append", default=[], metavar="KEY=VALUE",
+                   help="engine launch-shape key applied over the tuned (or default) ones; repeatable")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="run one device copy of this size first (HBM counter calibration under rocprofv3 --pmc)")
     return p.parse_args()
@@ -281,13 +283,13 @@ def autotune_run(args, torch, atom_off, Z, xyz):
         if hit is not None:
             return {"params": hit["params"], "from_cache": True, "cache": str(path), "timings_ms": hit.get("timings_ms")}
     run = Run(args, torch, atom_off, Z, xyz, args.precision)
-    defaults = {k: 0 for k in space}
+    defaults = {k: autotune.ENGINE_DEFAULTS[k] for k in space}
 
     def make(p):
         run.set_params({**defaults, **p})
         return run.m
 
-    best, timings = autotune.tune(torch, make, space, run.x_dev, run.e_dev, run.g_dev, run.stream)
+    best, timings = autotune.tune(torch, make, space, run.x_dev, run.e_dev, run.g_dev, run.stream, base=defaults)
     cache = autotune.load_cache(path)
     cache[f"{device}|{sig}"] = {"params": best, "timings_ms": timings}
     try:
@@ -350,9 +352,14 @@ def main():
     if not args.no_autotune:
         progress("autotune")
         tuned = autotune_run(args, torch, atom_off, Z, xyz)
-    params = tuned["params"] if tuned else {}
+    params = dict(tuned["params"]) if tuned else {}
+    overrides = {}
+    for kv in args.param:
+        k, _, v = kv.partition("=")
+        overrides[k.strip()] = float(v) if "." in v else int(v)
+    params.update(overrides)
     progress("FP64 path")
-    r64 = Run(args, torch, atom_off, Z, xyz, "fp64", params if args.precision == "fp64" else None)
+    r64 = Run(args, torch, atom_off, Z, xyz, "fp64", params if args.precision == "fp64" else (overrides or None))
     r64.timed(args, torch, args.roofline_kernel if args.precision == "fp64" else "pairs")
     head = r64
     if args.precision == "mixed":
@@ -380,6 +387,10 @@ def main():
     info["pcg_iters"] = pcg_iters
     if batch:
         frag = np.ctypeslib.as_array(pv.frag, shape=(n,))
+        fp32_solve = (args.precision == "mixed" and head.params.get("batch_fp32", 0) == 1
+                      and head.params.get("batch_solver", 0) != 2)
+        info["batch_fp32"] = fp32_solve
+        info["batch_refine_steps"] = 2 if fp32_solve else 0  # BSOLVE_NREF of the engine
         info["mol_off"] = atom_off
         info["mol_nr"] = [int(atom_off[k + 1] - atom_off[k] - len(np.unique(frag[atom_off[k]:atom_off[k + 1]])))
                           for k in range(len(atom_off) - 1)]

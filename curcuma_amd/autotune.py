@@ -1,11 +1,23 @@
 """Launch-shape autotuner for the GFN-FF engine, with a JSON cache.
 
-The engine keys tuned here change only how the work is cut into launches, so
-every candidate computes the same result (the batch-solve variants and the
-PCG workgroup counts reorder no sum; the gathers add in the same fixed
-order):
-  batch_solver   0 / 1    per-molecule Cholesky of batches: tile phases shared
-                          out over wavefronts, or one tile step per workgroup
+The engine keys tuned here change only how the work is cut into launches,
+except batch_fp32, which trades the FP64 factor for an FP32 one refined in
+FP64.  The batch-solve variants, the PCG workgroup counts and the gather
+widths reorder no sum, so their results are bitwise the same; the two batched
+pair decompositions add each atom's pair terms in a different order, so
+theirs agree to rounding:
+  batch_solver   0..5     per-molecule Cholesky of batches: right-looking with
+                          tile phases shared out over wavefronts in two
+                          occupancy / operand-prefetch variants, one tile step
+                          per workgroup, or one column block at a time over
+                          the whole batch; left-looking with the matrix formed
+                          inside the factorisation, per molecule or one
+                          column block at a time over the batch
+  batch_fp32     0 / 1    mixed path: factor in FP64, or in FP32 on MFMA with
+                          two FP64 refinement steps (charges agree with the
+                          FP64 factor's to about 1e-12 e)
+  batch_pairs    0 / 1    pair terms of batches: each pair once in 64 x 64
+                          tiles per molecule, or one thread per row atom
   gather_lanes   16 / 64  lanes per atom in the slot and hydrogen-bond gathers
   pcg_groups     0 / 128 / 192
                           workgroups of the persistent PCG solve (single
@@ -29,7 +41,9 @@ from pathlib import Path
 
 import numpy as np
 
-SPACE_BATCH = {"batch_solver": [0, 1], "gather_lanes": [16, 64]}
+SPACE_BATCH = {"batch_solver": [0, 1, 2, 3, 4, 5], "batch_fp32": [0, 1], "batch_pairs": [0, 1], "gather_lanes": [16, 64]}
+# the engine's values when a key is not set (gather_lanes 0: by mean list length)
+ENGINE_DEFAULTS = {"batch_solver": 5, "batch_fp32": 0, "batch_pairs": 0, "gather_lanes": 0, "pcg_groups": 0}
 SPACE_SINGLE = {"pcg_groups": [0, 128, 192], "gather_lanes": [16, 64]}
 
 
@@ -90,16 +104,16 @@ def time_config(torch, make, params: dict, x_dev, e_dev, g_dev, stream, warmup: 
     return statistics.median(times)
 
 
-def tune(torch, make, space: dict, x_dev, e_dev, g_dev, stream, base: dict | None = None, **kw):
-    """Coordinate descent over `space` from `base`; returns (best params,
-    {config label: ms per evaluation})."""
+def descend(space: dict, measure, base: dict | None = None):
+    """Coordinate descent over `space` from `base` with measure(params) -> ms;
+    returns (best params, {config label: ms})."""
     best = dict(base or {})
     timings = {}
 
     def label(p):
         return ",".join(f"{k}={p[k]}" for k in sorted(p)) or "defaults"
 
-    best_ms = time_config(torch, make, best, x_dev, e_dev, g_dev, stream, **kw)
+    best_ms = measure(best)
     timings[label(best)] = best_ms
     for key, values in space.items():
         for v in values:
@@ -107,8 +121,13 @@ def tune(torch, make, space: dict, x_dev, e_dev, g_dev, stream, base: dict | Non
                 continue
             cand = dict(best)
             cand[key] = v
-            ms = time_config(torch, make, cand, x_dev, e_dev, g_dev, stream, **kw)
+            ms = measure(cand)
             timings[label(cand)] = ms
             if ms < best_ms:
                 best, best_ms = cand, ms
     return best, timings
+
+
+def tune(torch, make, space: dict, x_dev, e_dev, g_dev, stream, base: dict | None = None, **kw):
+    """Times engine configurations on the device and descends over `space`."""
+    return descend(space, lambda p: time_config(torch, make, p, x_dev, e_dev, g_dev, stream, **kw), base)

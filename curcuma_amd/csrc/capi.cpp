@@ -359,8 +359,17 @@ int gfnff_set_param(gfnff_handle* h, const char* key, double value) {
         h->opt.cholesky_lookahead = value != 0.0;
         h->dev.reset();
     } else if (k == "batch_solver") {
-        if (value != 0.0 && value != 1.0) return fail(h, GFNFF_STATUS_ARGS, "batch_solver: 0 or 1");
+        if (!(value >= 0.0 && value <= 5.0) || value != std::floor(value))
+            return fail(h, GFNFF_STATUS_ARGS, "batch_solver: 0 .. 5");
         h->opt.batch_solver = (int)value;
+        h->dev.reset();
+    } else if (k == "batch_fp32") {
+        if (value != 0.0 && value != 1.0) return fail(h, GFNFF_STATUS_ARGS, "batch_fp32: 0 or 1");
+        h->opt.batch_fp32 = (int)value;
+        h->dev.reset();
+    } else if (k == "batch_pairs") {
+        if (value != 0.0 && value != 1.0) return fail(h, GFNFF_STATUS_ARGS, "batch_pairs: 0 or 1");
+        h->opt.batch_pairs = (int)value;
         h->dev.reset();
     } else if (k == "gather_lanes") {
         if (value != 0.0 && value != 16.0 && value != 64.0) return fail(h, GFNFF_STATUS_ARGS, "gather_lanes: 0, 16 or 64");

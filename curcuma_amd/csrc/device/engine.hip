@@ -66,6 +66,8 @@ enum Flag { FLAG_CHOL = 1, FLAG_CNOVF = 2, FLAG_SOLVE = 4 };
 // small helpers
 // ---------------------------------------------------------------------------
 __device__ inline double3 ld3(const double* x, int i) { return make_double3(x[3 * i], x[3 * i + 1], x[3 * i + 2]); }
+__device__ inline double sq(double v) { return v * v; }
+__device__ inline double cube(double v) { return v * v * v; }
 __device__ inline double3 operator-(double3 a, double3 b) { return make_double3(a.x - b.x, a.y - b.y, a.z - b.z); }
 __device__ inline double3 operator+(double3 a, double3 b) { return make_double3(a.x + b.x, a.y + b.y, a.z + b.z); }
 __device__ inline double3 operator*(double s, double3 a) { return make_double3(s * a.x, s * a.y, s * a.z); }
@@ -250,9 +252,13 @@ __device__ void cn_atom_finish(int i, double raw, int cnt, const int* __restrict
 // wave w scans atoms w, w + 4, ... against every atom of the molecule with
 // the ballot-ordered list construction of k_cn (ascending partners) and
 // leaves the raw CN and list length in LDS; the per-atom epilogue (log cap,
-// D4 weights, EEQ right-hand side) then runs one thread per atom.
+// D4 weights, EEQ right-hand side) then runs one thread per atom.  The LDS
+// arrays are sized at launch for the batch's largest molecule (cap atoms,
+// 44 bytes each), so that workgroups of small-molecule batches share a CU
+// many at a time and hide the latency of each wavefront's scan.
 constexpr int CNMOL_MAX = 1024;
-__global__ __launch_bounds__(256) void k_cn_mol(const int* __restrict__ mol_off, const double* __restrict__ x,
+constexpr int CNMOL_LDS_BYTES = 5 * 8 + 4;  // per atom of the cap
+__global__ __launch_bounds__(256, 6) void k_cn_mol(int cap, const int* __restrict__ mol_off, const double* __restrict__ x,
                                                 const double* __restrict__ rcov, const int* __restrict__ elem,
                                                 const int* __restrict__ nref, const double* __restrict__ refcn,
                                                 const double* __restrict__ chi_base, const double* __restrict__ cnf,
@@ -260,8 +266,9 @@ __global__ __launch_bounds__(256) void k_cn_mol(const int* __restrict__ mol_off,
                                                 double* __restrict__ gw, double* __restrict__ dgw,
                                                 double* __restrict__ rhs, int* __restrict__ nbn,
                                                 int* __restrict__ nbl, int* __restrict__ flags) {
-    __shared__ double sx[CNMOL_MAX], sy[CNMOL_MAX], sz[CNMOL_MAX], sr[CNMOL_MAX], sraw[CNMOL_MAX];
-    __shared__ int scnt[CNMOL_MAX];
+    extern __shared__ double cnm_lds[];
+    double *sx = cnm_lds, *sy = sx + cap, *sz = sy + cap, *sr = sz + cap, *sraw = sr + cap;
+    int* scnt = (int*)(sraw + cap);
     const int m = blockIdx.x, a0 = mol_off[m], nm = mol_off[m + 1] - a0;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     for (int k = tid; k < nm; k += 256) {
@@ -685,6 +692,7 @@ __global__ __launch_bounds__(256) void k_chol_panel(int np, int k0, T* __restric
 // A(i = lane%16, k = lane/16), B(k = lane/16, j = lane%16); result element v
 // of a block sits at (row 4*v + lane/16, column lane%16) (measured on gfx950).
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef double dbl2 __attribute__((ext_vector_type(2)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
 // Per-type 16x16x4 MFMA and its result layout: element v of a lane is at row
@@ -2008,8 +2016,8 @@ __global__ __launch_bounds__(TILE) void k_pairs_part32(PairArgs a, int cj, doubl
 // row-side and column-side sums (PART_K quantities x 64 atoms) to partials of
 // its own, which k_pairs_sym_reduce adds per atom in a fixed order.  The pair
 // arithmetic is that of k_pairs_part (T = double) or k_pairs_part32
-// (T = float, FP64 coordinate differences and FP64 sums; the column-side
-// sums of one tile, at most 64 terms each, in FP32); the partner's dC6/dCN
+// (T = float: FP64 coordinate differences, FP32 terms, every sum in FP64,
+// row side and column side); the partner's dC6/dCN
 // uses sum_b h_i[e_j][b] dgw_j[b], the transposed half contraction.
 constexpr int PS_B = 64;
 __device__ inline float ps_rsqrt(float v) { return __builtin_amdgcn_rsqf(v); }
@@ -2022,23 +2030,34 @@ __device__ inline float ps_exp(float v) { return __expf(v); }
 __device__ inline double ps_exp(double v) { return exp(v); }
 
 template <typename T, int NE>
-__global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restrict__ part) {
+__global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restrict__ part,
+                                                    const int4* __restrict__ tiles) {
     __shared__ double cx[PS_B], cy[PS_B], cz[PS_B];
     __shared__ T cq[PS_B], cal[PS_B], czt[PS_B], csz[PS_B], can[PS_B], crz[PS_B];
     __shared__ T cgw[7][PS_B], cdgw[7][PS_B];
     __shared__ int czem[PS_B];        // Z | element << 8 | metal << 16
-    __shared__ T acc[PART_K][PS_B];   // column-side sums of this tile (at most 64 terms each)
+    __shared__ double acc[PART_K][PS_B];  // column-side sums of this tile, in FP64 on both paths
     const int t = blockIdx.x, l = threadIdx.x;
-    int R = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while ((R + 1) * (R + 2) / 2 <= t) ++R;
-    while (R * (R + 1) / 2 > t) --R;
-    const int C = t - R * (R + 1) / 2;
-    const int n = a.n, ne = a.nelem;
-    const int i = R * PS_B + l, jl = C * PS_B + l;
-    const bool acti = i < n;
+    // the tile's molecule (atoms a0 .. a0 + n - 1) and its row and column blocks
+    int a0 = 0, n = a.n, R, C;
+    if (tiles) {
+        const int4 tt = tiles[t];
+        a0 = tt.x;
+        n = tt.y;
+        R = tt.z;
+        C = tt.w;
+    } else {
+        R = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while ((R + 1) * (R + 2) / 2 <= t) ++R;
+        while (R * (R + 1) / 2 > t) --R;
+        C = t - R * (R + 1) / 2;
+    }
+    const int ne = a.nelem;
+    const int i = a0 + R * PS_B + l, jl = a0 + C * PS_B + l;
+    const bool acti = R * PS_B + l < n;
     {
-        const bool actj = jl < n;
-        const int js = actj ? jl : 0;
+        const bool actj = C * PS_B + l < n;
+        const int js = actj ? jl : a0;
         cx[l] = a.x[3 * js];
         cy[l] = a.x[3 * js + 1];
         cz[l] = a.x[3 * js + 2];
@@ -2055,10 +2074,10 @@ __global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restri
             cdgw[r][l] = (T)a.dgw[(size_t)js * 7 + r];
         }
 #pragma unroll
-        for (int k = 0; k < PART_K; ++k) acc[k][l] = T(0);
+        for (int k = 0; k < PART_K; ++k) acc[k][l] = 0.0;
     }
     __syncthreads();
-    const int is = acti ? i : 0;
+    const int is = acti ? i : a0;
     const double3 xi = ld3(a.x, is);
     const T qi = (T)a.q[is], ali = (T)a.alp[is], zti = (T)a.zeta[is], szi = (T)a.szr[is];
     const T ani = (T)a.anb[is], rzi = (T)a.repz[is];
@@ -2180,13 +2199,13 @@ __global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restri
         ed += (double)hed;
         er += (double)her;
         dcn -= (double)dci;
-        acc[0][jj] -= (T)fx;
-        acc[1][jj] -= (T)fy;
-        acc[2][jj] -= (T)fz;
-        if (disp) acc[3][jj] -= dcj;
-        acc[4][jj] += hec;
-        if (disp) acc[5][jj] += hed;
-        if (rep) acc[6][jj] += her;
+        acc[0][jj] -= fx;
+        acc[1][jj] -= fy;
+        acc[2][jj] -= fz;
+        if (disp) acc[3][jj] -= (double)dcj;
+        acc[4][jj] += (double)hec;
+        if (disp) acc[5][jj] += (double)hed;
+        if (rep) acc[6][jj] += (double)her;
     }
     __syncthreads();
     double* pr = part + (size_t)t * 2 * PART_K * PS_B;
@@ -2194,7 +2213,7 @@ __global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restri
 #pragma unroll
     for (int k = 0; k < PART_K; ++k) {
         pr[k * PS_B + l] = iv[k];
-        pr[(PART_K + k) * PS_B + l] = (double)acc[k][l];
+        pr[(PART_K + k) * PS_B + l] = acc[k][l];
     }
 }
 
@@ -2202,18 +2221,29 @@ __global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restri
 // then the column-side partials of tiles (R, A), R = A .. nb - 1, in that
 // order; one wavefront per block per quarter of the tiles, the four quarters
 // added in order.
+// Batches (blocks != nullptr): one workgroup per 64-atom block {a0, n, A, base}
+// of a molecule (atoms a0 .. a0 + n - 1, its tiles numbered from base).
 __global__ __launch_bounds__(256) void k_pairs_sym_reduce(int n, int nb, const double* __restrict__ part,
                                                           double* __restrict__ g, double* __restrict__ dEdcn,
                                                           double* __restrict__ e_coul, double* __restrict__ e_disp,
-                                                          double* __restrict__ e_rep) {
+                                                          double* __restrict__ e_rep, const int4* __restrict__ blocks) {
     __shared__ double qs[4][PART_K][PS_B];
-    const int A = blockIdx.x, l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int A = blockIdx.x, a0 = 0, base = 0;
+    if (blocks) {
+        const int4 bb = blocks[blockIdx.x];
+        a0 = bb.x;
+        n = bb.y;
+        A = bb.z;
+        base = bb.w;
+        nb = (n + PS_B - 1) / PS_B;
+    }
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ntile = nb + 1;  // A + 1 row-side tiles and nb - A column-side tiles
     double s[PART_K] = {0, 0, 0, 0, 0, 0, 0};
     for (int u = w; u < ntile; u += 4) {
         const bool rowside = u <= A;
         const int Rr = u - 1;  // column side: tile (R, A), R = u - 1 = A .. nb - 1
-        const int t = rowside ? A * (A + 1) / 2 + u : Rr * (Rr + 1) / 2 + A;
+        const int t = base + (rowside ? A * (A + 1) / 2 + u : Rr * (Rr + 1) / 2 + A);
         const double* p = part + ((size_t)t * 2 + (rowside ? 0 : 1)) * PART_K * PS_B + l;
 #pragma unroll
         for (int k = 0; k < PART_K; ++k) s[k] += p[k * PS_B];
@@ -2221,8 +2251,8 @@ __global__ __launch_bounds__(256) void k_pairs_sym_reduce(int n, int nb, const d
 #pragma unroll
     for (int k = 0; k < PART_K; ++k) qs[w][k][l] = s[k];
     __syncthreads();
-    const int i = A * PS_B + threadIdx.x;
-    if (threadIdx.x >= PS_B || i >= n) return;
+    if (threadIdx.x >= PS_B || A * PS_B + (int)threadIdx.x >= n) return;
+    const int i = a0 + A * PS_B + threadIdx.x;
     double v[PART_K];
 #pragma unroll
     for (int k = 0; k < PART_K; ++k) v[k] = (qs[0][k][l] + qs[1][k][l]) + (qs[2][k][l] + qs[3][k][l]);
@@ -3274,15 +3304,15 @@ __global__ __launch_bounds__(256) void k_atm(int nt, const int* __restrict__ ijk
     double s = sc[t];
     e_out[t] = ang * fdmp * c9 / 3.0 * s;
     double dfdmp = -2.0 * 14.0 * tmp * fdmp * fdmp;
-    double dij = -0.375 * (pow(r2ij, 3) + pow(r2ij, 2) * (r2jk + r2ik) +
-                           r2ij * (3.0 * pow(r2jk, 2) + 2.0 * r2jk * r2ik + 3.0 * pow(r2ik, 2)) -
-                           5.0 * pow(r2jk - r2ik, 2) * (r2jk + r2ik)) / r5ijk;
-    double dik = -0.375 * (pow(r2ik, 3) + pow(r2ik, 2) * (r2jk + r2ij) +
-                           r2ik * (3.0 * pow(r2jk, 2) + 2.0 * r2jk * r2ij + 3.0 * pow(r2ij, 2)) -
-                           5.0 * pow(r2jk - r2ij, 2) * (r2jk + r2ij)) / r5ijk;
-    double djk = -0.375 * (pow(r2jk, 3) + pow(r2jk, 2) * (r2ik + r2ij) +
-                           r2jk * (3.0 * pow(r2ik, 2) + 2.0 * r2ik * r2ij + 3.0 * pow(r2ij, 2)) -
-                           5.0 * pow(r2ik - r2ij, 2) * (r2ik + r2ij)) / r5ijk;
+    double dij = -0.375 * (cube(r2ij) + sq(r2ij) * (r2jk + r2ik) +
+                           r2ij * (3.0 * sq(r2jk) + 2.0 * r2jk * r2ik + 3.0 * sq(r2ik)) -
+                           5.0 * sq(r2jk - r2ik) * (r2jk + r2ik)) / r5ijk;
+    double dik = -0.375 * (cube(r2ik) + sq(r2ik) * (r2jk + r2ij) +
+                           r2ik * (3.0 * sq(r2jk) + 2.0 * r2jk * r2ij + 3.0 * sq(r2ij)) -
+                           5.0 * sq(r2jk - r2ij) * (r2jk + r2ij)) / r5ijk;
+    double djk = -0.375 * (cube(r2jk) + sq(r2jk) * (r2ik + r2ij) +
+                           r2jk * (3.0 * sq(r2ik) + 2.0 * r2ik * r2ij + 3.0 * sq(r2ij)) -
+                           5.0 * sq(r2ik - r2ij) * (r2ik + r2ij)) / r5ijk;
     double pre = -c9 * s / 3.0;
     double a1 = pre * (-dij * fdmp + ang * dfdmp) / r2ij, a2 = pre * (-dik * fdmp + ang * dfdmp) / r2ik,
            a3 = pre * (-djk * fdmp + ang * dfdmp) / r2jk;
@@ -3305,22 +3335,22 @@ __global__ __launch_bounds__(256) void k_batm(int nt, const int* __restrict__ ij
     double rijk3 = r2ij * r2jk * r2ik;
     double mijk = -r2ij + r2jk + r2ik, imjk = r2ij - r2jk + r2ik, ijmk = r2ij + r2jk - r2ik;
     double ang = 0.375 * ijmk * imjk * mijk / rijk3;
-    double rav3 = pow(rijk3, 1.5);
+    double rav3 = rijk3 * sqrt(rijk3);
     double angr9 = (ang + 1.0) / rav3;
     double fi = fmin(fmax(1.0 - 3.0 * qa[i], -4.0), 4.0);
     double fj = fmin(fmax(1.0 - 3.0 * qa[j], -4.0), 4.0);
     double fk = fmin(fmax(1.0 - 3.0 * qa[k], -4.0), 4.0);
     double c9 = fi * fj * fk * zb[3 * t] * zb[3 * t + 1] * zb[3 * t + 2];
     e_out[t] = c9 * angr9;
-    double dij = -0.375 * (pow(r2ij, 3) + pow(r2ij, 2) * (r2jk + r2ik) +
-                           r2ij * (3.0 * pow(r2jk, 2) + 2.0 * r2jk * r2ik + 3.0 * pow(r2ik, 2)) -
-                           5.0 * pow(r2jk - r2ik, 2) * (r2jk + r2ik)) / (rij * rijk3 * rav3);
-    double djk = -0.375 * (pow(r2jk, 3) + pow(r2jk, 2) * (r2ik + r2ij) +
-                           r2jk * (3.0 * pow(r2ik, 2) + 2.0 * r2ik * r2ij + 3.0 * pow(r2ij, 2)) -
-                           5.0 * pow(r2ik - r2ij, 2) * (r2ik + r2ij)) / (rjk * rijk3 * rav3);
-    double dik = -0.375 * (pow(r2ik, 3) + pow(r2ik, 2) * (r2jk + r2ij) +
-                           r2ik * (3.0 * pow(r2jk, 2) + 2.0 * r2jk * r2ij + 3.0 * pow(r2ij, 2)) -
-                           5.0 * pow(r2jk - r2ij, 2) * (r2jk + r2ij)) / (rik * rijk3 * rav3);
+    double dij = -0.375 * (cube(r2ij) + sq(r2ij) * (r2jk + r2ik) +
+                           r2ij * (3.0 * sq(r2jk) + 2.0 * r2jk * r2ik + 3.0 * sq(r2ik)) -
+                           5.0 * sq(r2jk - r2ik) * (r2jk + r2ik)) / (rij * rijk3 * rav3);
+    double djk = -0.375 * (cube(r2jk) + sq(r2jk) * (r2ik + r2ij) +
+                           r2jk * (3.0 * sq(r2ik) + 2.0 * r2ik * r2ij + 3.0 * sq(r2ij)) -
+                           5.0 * sq(r2ik - r2ij) * (r2ik + r2ij)) / (rjk * rijk3 * rav3);
+    double dik = -0.375 * (cube(r2ik) + sq(r2ik) * (r2jk + r2ij) +
+                           r2ik * (3.0 * sq(r2jk) + 2.0 * r2jk * r2ij + 3.0 * sq(r2ij)) -
+                           5.0 * sq(r2jk - r2ij) * (r2jk + r2ij)) / (rik * rijk3 * rav3);
     double3 gij = (-dij * c9 / rij) * vij, gjk = (-djk * c9 / rjk) * vjk, gik = (-dik * c9 / rik) * vik;
     int q = slot0[t];
     st3(slots, q, gij + gik);                 // i
@@ -3344,14 +3374,16 @@ constexpr int BATCH_MAX_NP = 512;  // largest padded reduced system of a batched
 // one-fragment molecule), the entries A(ia, lb), A(la, ib) and A(la, lb) are
 // evaluated once per row, column and tile into LDS and every element needs
 // one fresh A(ia, ib); the sum keeps the order of the direct four-term form,
-// so R is the same bit for bit.  Other tiles use the direct form.
+// so R is the same bit for bit.  Other tiles use the direct form.  A32 (the
+// mixed path) receives the same entries rounded to FP32.
 __global__ __launch_bounds__(256) void k_eeq_reduced_batch(const int4* __restrict__ tiles, const int* __restrict__ mnr,
                                                            const int* __restrict__ mnp,
                                                            const long long* __restrict__ mo,
                                                            const int* __restrict__ vo, const int* __restrict__ ra,
                                                            const int* __restrict__ rl, const double* __restrict__ x,
                                                            const double* __restrict__ alp,
-                                                           const double* __restrict__ diag, double* __restrict__ A) {
+                                                           const double* __restrict__ diag, double* __restrict__ A,
+                                                           float* __restrict__ A32) {
     __shared__ double urow[NB], wcol[NB];
     __shared__ double sll;
     const int4 tt = tiles[blockIdx.x];
@@ -3392,58 +3424,91 @@ __global__ __launch_bounds__(256) void k_eeq_reduced_batch(const int4* __restric
                 eeq_a(la, lb, x, alp, diag);
         }
         Am[(size_t)r * np + c] = v;
+        if (A32) A32[mo[m] + (size_t)r * np + c] = (float)v;  // the mixed path factors this copy
     }
 }
 
 // ---- per-molecule factorisation: tile phases shared out over wavefronts ----
-// One 64 x 64 output tile per wavefront on FP64 MFMA (v_mfma_f64_16x16x4f64),
-// in two 32-row halves; operands are read from memory (L2) one 4-column
-// step ahead of the MFMAs.  Lane layout: A(i = lane%16, k = lane/16),
-// B(k = lane/16, j = lane%16); result element v of a 16 x 16 block at row
-// 4 v + lane/16, column lane%16.
+// One 64 x 64 output tile per wavefront on MFMA (v_mfma_f64_16x16x4f64 or
+// v_mfma_f32_16x16x4f32), in two 32-row halves, operands read from memory
+// (L2).  Lane layout: A(i = lane%16, k = lane/16), B(k = lane/16,
+// j = lane%16); result element v of a 16 x 16 block at row 4 v + lane/16
+// (FP64) or 4 (lane/16) + v (FP32), column lane%16.  Operands are read DEPTH 4-column steps
+// ahead (the L2 latency exceeds one step's eight MFMAs); DEPTH 1 keeps the
+// register footprint of three workgroups per CU.
 //   SUB = true : A(ri.., rj..) -= P(ri.., 0..63) Q(rj.., 0..63)^T (trailing
 //                update; P = Q = the panel, leading dimension ldq = np;
 //                diagonal tiles keep the lower triangle)
 //   SUB = false: A(ri.., rj..)  = P(ri.., 0..63) Q(0.., 0..63)^T (panel
 //                solve, Q = the inverse of the diagonal block, ldq = 64)
-template <bool SUB>
-__device__ __forceinline__ void batch_tile_mfma(double* __restrict__ Am, int np, int ri, int rj,
-                                                const double* __restrict__ P, const double* __restrict__ Q,
-                                                int ldq) {
+template <typename T, bool SUB, int DEPTH>
+__device__ __forceinline__ void batch_tile_mfma(T* __restrict__ Am, int np, int ri, int rj,
+                                                const T* __restrict__ P, const T* __restrict__ Q, int ldq) {
+    using M = MfmaT<T>;
+    using V = typename M::V;
     const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
     const bool lower_only = SUB && ri == rj;
 #pragma unroll 1
     for (int h = 0; h < 2; ++h) {
-        const double* Pi = P + (size_t)(32 * h) * np;
-        dbl4 acc[2][4];
+        const T* Pi = P + (size_t)(32 * h) * np;
+        V acc[2][4];
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
-            for (int y = 0; y < 4; ++y) acc[x][y] = dbl4{0.0, 0.0, 0.0, 0.0};
-        double fa[2], fb[4];
+            for (int y = 0; y < 4; ++y) acc[x][y] = V{0, 0, 0, 0};
+        if constexpr (DEPTH == 1) {
+            // the next step's operands are loaded before the current step's MFMAs
+            T fa[2], fb[4];
 #pragma unroll
-        for (int x = 0; x < 2; ++x) fa[x] = Pi[(size_t)(16 * x + li) * np + lk];
+            for (int x = 0; x < 2; ++x) fa[x] = Pi[(size_t)(16 * x + li) * np + lk];
 #pragma unroll
-        for (int y = 0; y < 4; ++y) fb[y] = Q[(size_t)(16 * y + li) * ldq + lk];
+            for (int y = 0; y < 4; ++y) fb[y] = Q[(size_t)(16 * y + li) * ldq + lk];
 #pragma unroll 2
-        for (int kk = 0; kk < NB; kk += 4) {
-            double na[2], nb[4];
-            if (kk + 4 < NB) {
+            for (int kk = 0; kk < NB; kk += 4) {
+                T na[2], nb[4];
+                if (kk + 4 < NB) {
 #pragma unroll
-                for (int x = 0; x < 2; ++x) na[x] = Pi[(size_t)(16 * x + li) * np + kk + 4 + lk];
+                    for (int x = 0; x < 2; ++x) na[x] = Pi[(size_t)(16 * x + li) * np + kk + 4 + lk];
 #pragma unroll
-                for (int y = 0; y < 4; ++y) nb[y] = Q[(size_t)(16 * y + li) * ldq + kk + 4 + lk];
+                    for (int y = 0; y < 4; ++y) nb[y] = Q[(size_t)(16 * y + li) * ldq + kk + 4 + lk];
+                }
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y = 0; y < 4; ++y)
+                        acc[x][y] = M::op(fa[x], fb[y], acc[x][y]);
+                if (kk + 4 < NB) {
+#pragma unroll
+                    for (int x = 0; x < 2; ++x) fa[x] = na[x];
+#pragma unroll
+                    for (int y = 0; y < 4; ++y) fb[y] = nb[y];
+                }
+            }
+        } else {
+            // operands of DEPTH 4-column steps in flight (ring of registers)
+            T fa[DEPTH][2], fb[DEPTH][4];
+#pragma unroll
+            for (int d = 0; d < DEPTH; ++d) {
+#pragma unroll
+                for (int x = 0; x < 2; ++x) fa[d][x] = Pi[(size_t)(16 * x + li) * np + 4 * d + lk];
+#pragma unroll
+                for (int y = 0; y < 4; ++y) fb[d][y] = Q[(size_t)(16 * y + li) * ldq + 4 * d + lk];
             }
 #pragma unroll
-            for (int x = 0; x < 2; ++x)
+            for (int st = 0; st < NB / 4; ++st) {
+                const int sl = st % DEPTH;
 #pragma unroll
-                for (int y = 0; y < 4; ++y)
-                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[x], fb[y], acc[x][y], 0, 0, 0);
-            if (kk + 4 < NB) {
+                for (int x = 0; x < 2; ++x)
 #pragma unroll
-                for (int x = 0; x < 2; ++x) fa[x] = na[x];
+                    for (int y = 0; y < 4; ++y)
+                        acc[x][y] = M::op(fa[sl][x], fb[sl][y], acc[x][y]);
+                if (st + DEPTH < NB / 4) {
+                    const int kk = 4 * (st + DEPTH);
 #pragma unroll
-                for (int y = 0; y < 4; ++y) fb[y] = nb[y];
+                    for (int x = 0; x < 2; ++x) fa[sl][x] = Pi[(size_t)(16 * x + li) * np + kk + lk];
+#pragma unroll
+                    for (int y = 0; y < 4; ++y) fb[sl][y] = Q[(size_t)(16 * y + li) * ldq + kk + lk];
+                }
             }
         }
 #pragma unroll
@@ -3452,70 +3517,31 @@ __device__ __forceinline__ void batch_tile_mfma(double* __restrict__ Am, int np,
             for (int y = 0; y < 4; ++y)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int r = 32 * h + 16 * x + 4 * v + lk, c = 16 * y + li;
-                    double* p = Am + (size_t)(ri + r) * np + rj + c;
+                    const int r = 32 * h + 16 * x + (M::kRowFromReg ? 4 * v + lk : 4 * lk + v), c = 16 * y + li;
+                    T* p = Am + (size_t)(ri + r) * np + rj + c;
                     if (!SUB) *p = acc[x][y][v];
                     else if (!lower_only || c <= r) *p = *p - acc[x][y][v];
                 }
     }
 }
 
-// Factorisation and solve of one molecule's reduced system per workgroup of
-// four wavefronts, right-looking by 64-column blocks.  Step k: the workgroup
-// factors the diagonal block and forms its inverse (chol_diag_block); the
-// panel tiles L(i, k) = A(i, k) L(k, k)^-T and then the trailing tiles
-// A(i, j) -= L(i, k) L(j, k)^T are computed one tile per wavefront on MFMA,
-// dealt round-robin, with one barrier after each phase.  The substitutions
-// use the stored inverses of the diagonal blocks.  Registers and LDS are
-// sized for four workgroups per CU, so that the diagonal-block phases of
-// one molecule overlap the MFMA phases of others.  b, x: concatenated padded
-// vectors.
-constexpr int BSOLVE_T = 256;
 // The diagonal-block routine as a call of its own, so that its register
 // footprint is not added to the state of the surrounding phases.
-__device__ __noinline__ void batch_diag_block(int np, int k0, double* __restrict__ A, double* __restrict__ Linv,
-                                              int* __restrict__ flags, double (*S0)[NB + 1], double (*col)[NB],
-                                              double* rdiag) {
-    chol_diag_block<double>(np, k0, A, Linv, flags, S0, col, rdiag);
+template <typename T>
+__device__ __noinline__ void batch_diag_block(int np, int k0, T* __restrict__ A, T* __restrict__ Linv,
+                                              int* __restrict__ flags, T (*S0)[NB + 1], T (*col)[NB], T* rdiag) {
+    chol_diag_block<T>(np, k0, A, Linv, flags, S0, col, rdiag);
 }
-__global__ __launch_bounds__(BSOLVE_T, 3) void k_eeq_batch_solve(const int* __restrict__ order,
-                                                                 const int* __restrict__ mnp,
-                                                                 const long long* __restrict__ mo,
-                                                                 const int* __restrict__ vo, double* __restrict__ A,
-                                                                 double* __restrict__ Linv,
-                                                                 const double* __restrict__ b,
-                                                                 double* __restrict__ x, int* __restrict__ flags) {
-    __shared__ double S0[NB][NB + 1];
-    __shared__ double col[2][NB];
-    __shared__ double rdiag[NB];
-    __shared__ double vec[BATCH_MAX_NP];
-    __shared__ double tmp[NB];
-    constexpr int NW = BSOLVE_T / 64;
-    const int m = order[blockIdx.x], np = mnp[m], nb = np / NB, base = vo[m];  // largest systems first
-    double* Am = A + mo[m];
-    double* Lm = Linv + (size_t)base * NB;
-    const int tid = threadIdx.x, wave = tid >> 6, t = tid >> 2, q = tid & 3;
+
+// z = L^-T L^-1 v for one molecule (blocked substitutions with the stored
+// inverses of the diagonal blocks): v in vec (LDS, type T), result in vec.
+template <typename T>
+__device__ __forceinline__ void batch_substitute(int np, const T* __restrict__ Am, const T* __restrict__ Lm, T* vec,
+                                                 T* tmp) {
+    const int tid = threadIdx.x, t = tid >> 2, q = tid & 3, nb = np / NB;
+    // forward: y_kb = Linv_kb (v_kb - sum_{k < kb} L(kb, k) y_k)
     for (int kb = 0; kb < nb; ++kb) {
-        const int k0 = kb * NB;
-        batch_diag_block(np, k0, Am, Lm, flags, S0, col, rdiag);
-        __syncthreads();
-        for (int bi = kb + 1 + wave; bi < nb; bi += NW)
-            batch_tile_mfma<false>(Am, np, bi * NB, k0, Am + (size_t)bi * NB * np + k0, Lm + (size_t)kb * NB * NB,
-                                   NB);
-        __syncthreads();
-        int tile = 0;
-        for (int bi = kb + 1; bi < nb; ++bi)
-            for (int bj = kb + 1; bj <= bi; ++bj, ++tile)
-                if (tile % NW == wave)
-                    batch_tile_mfma<true>(Am, np, bi * NB, bj * NB, Am + (size_t)bi * NB * np + k0,
-                                          Am + (size_t)bj * NB * np + k0, np);
-        __syncthreads();
-    }
-    for (int r = tid; r < np; r += BSOLVE_T) vec[r] = b[base + r];
-    __syncthreads();
-    // forward: y_kb = Linv_kb (b_kb - sum_{k < kb} L(kb, k) y_k), y kept in vec
-    for (int kb = 0; kb < nb; ++kb) {
-        double acc = (q == 0) ? vec[kb * NB + t] : 0.0;
+        T acc = (q == 0) ? vec[kb * NB + t] : T(0);
         for (int k = 0; k < kb; ++k)
 #pragma unroll
             for (int u = 0; u < NB / 4; ++u)
@@ -3524,8 +3550,8 @@ __global__ __launch_bounds__(BSOLVE_T, 3) void k_eeq_batch_solve(const int* __re
         acc += __shfl_xor(acc, 2);
         if (q == 0) tmp[t] = acc;
         __syncthreads();
-        const double* Li = Lm + (size_t)kb * NB * NB;
-        double s = 0.0;
+        const T* Li = Lm + (size_t)kb * NB * NB;
+        T s = 0;
 #pragma unroll
         for (int u = 0; u < NB / 4; ++u) s += Li[t * NB + 4 * u + q] * tmp[4 * u + q];
         s += __shfl_xor(s, 1);
@@ -3535,7 +3561,7 @@ __global__ __launch_bounds__(BSOLVE_T, 3) void k_eeq_batch_solve(const int* __re
     }
     // backward: x_kb = Linv_kb^T (y_kb - sum_{k > kb} L(k, kb)^T x_k)
     for (int kb = nb - 1; kb >= 0; --kb) {
-        double acc = (q == 0) ? vec[kb * NB + t] : 0.0;
+        T acc = (q == 0) ? vec[kb * NB + t] : T(0);
         for (int k = nb - 1; k > kb; --k)
 #pragma unroll
             for (int u = 0; u < NB / 4; ++u)
@@ -3544,8 +3570,8 @@ __global__ __launch_bounds__(BSOLVE_T, 3) void k_eeq_batch_solve(const int* __re
         acc += __shfl_xor(acc, 2);
         if (q == 0) tmp[t] = acc;
         __syncthreads();
-        const double* Li = Lm + (size_t)kb * NB * NB;
-        double s = 0.0;
+        const T* Li = Lm + (size_t)kb * NB * NB;
+        T s = 0;
 #pragma unroll
         for (int u = 0; u < NB / 4; ++u) s += Li[(4 * u + q) * NB + t] * tmp[4 * u + q];
         s += __shfl_xor(s, 1);
@@ -3553,7 +3579,359 @@ __global__ __launch_bounds__(BSOLVE_T, 3) void k_eeq_batch_solve(const int* __re
         if (q == 0) vec[kb * NB + t] = s;
         __syncthreads();
     }
-    for (int r = tid; r < np; r += BSOLVE_T) x[base + r] = vec[r];
+}
+
+// Factorisation of one molecule's reduced system per workgroup of four
+// wavefronts, right-looking by 64-column blocks.  Step k: the workgroup
+// factors the diagonal block and forms its inverse (chol_diag_block, a call
+// of its own so that its registers do not add to the other phases'); the
+// panel tiles L(i, k) = A(i, k) L(k, k)^-T and then the trailing tiles
+// A(i, j) -= L(i, k) L(j, k)^T are computed one tile per wavefront on MFMA,
+// dealt round-robin, with one barrier after each phase.  T = double factors
+// the FP64 matrix in place; T = float (mixed path, batch_fp32 = 1) factors
+// the FP32 copy.  The factorisation needs only the matrix, so it runs on the
+// side stream right after the matrix build, alongside CN and the bonded
+// terms; the substitutions (k_bsolve_subst) follow once the right-hand side
+// is known.  Registers and LDS let MINB workgroups share a CU, so the
+// diagonal-block phases of one molecule overlap the MFMA phases of others.
+constexpr int BSOLVE_T = 256;
+constexpr int BSOLVE_NREF = 2;  // refinement steps of the FP32 factor
+template <typename T, int MINB, int DEPTH>
+__global__ __launch_bounds__(BSOLVE_T, MINB) void k_eeq_batch_factor(const int* __restrict__ order,
+                                                                  const int* __restrict__ mnp,
+                                                                  const long long* __restrict__ mo,
+                                                                  const int* __restrict__ vo, T* __restrict__ A,
+                                                                  T* __restrict__ Linv, int* __restrict__ flags) {
+    __shared__ T S0[NB][NB + 1];
+    __shared__ T col[2][NB];
+    __shared__ T rdiag[NB];
+    constexpr int NW = BSOLVE_T / 64;
+    const int m = order[blockIdx.x], np = mnp[m], nb = np / NB, base = vo[m];  // largest systems first
+    T* Am = A + mo[m];
+    T* Lm = Linv + (size_t)base * NB;
+    const int wave = threadIdx.x >> 6;
+    for (int kb = 0; kb < nb; ++kb) {
+        const int k0 = kb * NB;
+        batch_diag_block<T>(np, k0, Am, Lm, flags, S0, col, rdiag);
+        __syncthreads();
+        for (int bi = kb + 1 + wave; bi < nb; bi += NW)
+            batch_tile_mfma<T, false, DEPTH>(Am, np, bi * NB, k0, Am + (size_t)bi * NB * np + k0,
+                                             Lm + (size_t)kb * NB * NB, NB);
+        __syncthreads();
+        int tile = 0;
+        for (int bi = kb + 1; bi < nb; ++bi)
+            for (int bj = kb + 1; bj <= bi; ++bj, ++tile)
+                if (tile % NW == wave)
+                    batch_tile_mfma<T, true, DEPTH>(Am, np, bi * NB, bj * NB, Am + (size_t)bi * NB * np + k0,
+                                                    Am + (size_t)bj * NB * np + k0, np);
+        __syncthreads();
+    }
+}
+
+// Substitutions of one molecule's factored system (workgroup of BSOLVE_T
+// threads): x = L^-T L^-1 b, and for T = float BSOLVE_NREF FP64 refinement
+// steps against Rm (FP64, lower triangle).
+template <typename T>
+__device__ __forceinline__ void batch_solve_tail(int np, const T* __restrict__ Am, const T* __restrict__ Lm,
+                                                 const double* __restrict__ Rm, const double* __restrict__ b,
+                                                 double* __restrict__ x, T* vec, T* tmp, double* xs) {
+    const int tid = threadIdx.x, t = tid >> 2, q = tid & 3;
+    for (int r = tid; r < np; r += BSOLVE_T) vec[r] = (T)b[r];
+    __syncthreads();
+    batch_substitute<T>(np, Am, Lm, vec, tmp);
+    if constexpr (sizeof(T) == 4) {
+        for (int r = tid; r < np; r += BSOLVE_T) xs[r] = (double)vec[r];
+        __syncthreads();
+        for (int it = 0; it < BSOLVE_NREF; ++it) {
+            // r_i = b_i - (sum_{j <= i} R(i, j) x_j + sum_{j > i} R(j, i) x_j), four lanes per row
+            for (int i0 = 0; i0 < np; i0 += NB) {
+                const int i = i0 + t;
+                double acc = 0.0;
+                for (int j = q; j <= i; j += 4) acc += Rm[(size_t)i * np + j] * xs[j];
+                for (int j = i + 1 + q; j < np; j += 4) acc += Rm[(size_t)j * np + i] * xs[j];
+                acc += __shfl_xor(acc, 1);
+                acc += __shfl_xor(acc, 2);
+                if (q == 0) vec[i] = (T)(b[i] - acc);
+            }
+            __syncthreads();
+            batch_substitute<T>(np, Am, Lm, vec, tmp);
+            for (int r = tid; r < np; r += BSOLVE_T) xs[r] += (double)vec[r];
+            __syncthreads();
+        }
+        for (int r = tid; r < np; r += BSOLVE_T) x[r] = xs[r];
+    } else {
+        for (int r = tid; r < np; r += BSOLVE_T) x[r] = (double)vec[r];
+    }
+}
+
+// ---- per-molecule factorisation by steps over the whole batch (batch_solver 3)
+// Step k is three launches over every molecule with more than k column
+// blocks: the diagonal blocks (one workgroup each), the panel tiles and the
+// trailing tiles (one wavefront per tile, four per workgroup, any molecule).
+// Every tile of a phase is independent, so a launch holds thousands of
+// independent tiles and needs no barrier inside a molecule.  The
+// substitutions follow in one launch (k_bsolve_subst) on the evaluation
+// stream.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bstep_diag(const int* __restrict__ mols, int k, const int* __restrict__ mnp,
+                                                    const long long* __restrict__ mo, const int* __restrict__ vo,
+                                                    T* __restrict__ A, T* __restrict__ Linv, int* __restrict__ flags) {
+    __shared__ T S0[NB][NB + 1];
+    __shared__ T col[2][NB];
+    __shared__ T rdiag[NB];
+    const int m = mols[blockIdx.x];
+    chol_diag_block<T>(mnp[m], k * NB, A + mo[m], Linv + (size_t)vo[m] * NB, flags, S0, col, rdiag);
+}
+
+// tiles {m, bi, bj}: panel tiles (SUB = false, bj unused) or trailing tiles of step k
+template <typename T, bool SUB>
+__global__ __launch_bounds__(256) void k_bstep_tiles(const int4* __restrict__ tl, int ntl, int k,
+                                                     const int* __restrict__ mnp, const long long* __restrict__ mo,
+                                                     const int* __restrict__ vo, T* __restrict__ A,
+                                                     const T* __restrict__ Linv) {
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= ntl) return;  // wavefront-local work: no barrier follows
+    const int4 tt = tl[w];
+    const int m = tt.x, bi = tt.y, bj = tt.z, np = mnp[m], k0 = k * NB;
+    T* Am = A + mo[m];
+    if (!SUB)
+        batch_tile_mfma<T, false, 4>(Am, np, bi * NB, k0, Am + (size_t)bi * NB * np + k0,
+                                     Linv + (size_t)vo[m] * NB + (size_t)k * NB * NB, NB);
+    else
+        batch_tile_mfma<T, true, 4>(Am, np, bi * NB, bj * NB, Am + (size_t)bi * NB * np + k0,
+                                    Am + (size_t)bj * NB * np + k0, np);
+}
+
+template <typename T>
+__global__ __launch_bounds__(BSOLVE_T) void k_bsolve_subst(const int* __restrict__ order, const int* __restrict__ mnp,
+                                                           const long long* __restrict__ mo,
+                                                           const int* __restrict__ vo, const T* __restrict__ A,
+                                                           const T* __restrict__ Linv,
+                                                           const double* __restrict__ A64,
+                                                           const double* __restrict__ b, double* __restrict__ x) {
+    __shared__ T vec[BATCH_MAX_NP];
+    __shared__ T tmp[NB];
+    __shared__ double xs[sizeof(T) == 4 ? BATCH_MAX_NP : 1];
+    const int m = order[blockIdx.x], base = vo[m];
+    batch_solve_tail<T>(mnp[m], A + mo[m], Linv + (size_t)base * NB, A64 ? A64 + mo[m] : nullptr, b + base, x + base,
+                        vec, tmp, xs);
+}
+
+// ---- per-molecule left-looking factorisation with the matrix formed in place (batch_solver 4)
+// Tile (i, j), i >= j, of column block j on one wavefront:
+//   C = R(i, j) - L(i, 0:K) L(j, 0:K)^T,  K = 64 j,
+// stored at the tile, in two 32-row halves.  First the half's entries of R
+// are evaluated from the coordinates with the operations of
+// k_eeq_reduced_batch, lane = column, one row at a time: one-fragment
+// molecules (uni) as A(ia, ib) - u_a - u_b + A(L, L) with u_r = A(ia_r, L),
+// the row's data (coordinates, alpha, u) read from the lane of the half that
+// loaded it (readlane, no dependent gather per row); other molecules in the
+// direct four-term form; padded rows and columns as the identity.  They are
+// stored at the tile (the lines stay in L2) and read back in the MFMA
+// accumulator layout as -R; the product adds L(i, :) L(j, :)^T over the whole
+// K on FP64 MFMA, and C = -acc is stored.  The operands are read eight
+// columns at a time as 16-byte vectors (lane group g holds columns 2 g and
+// 2 g + 1 of the eight, MFMA step s takes column 2 g + s: a permutation of
+// the columns that both operands share), the next eight columns in flight
+// while the current ones are multiplied.
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ void fused_tile(double* __restrict__ Am, int np, int nr, int i, int j,
+                                           const int* __restrict__ rab, const int* __restrict__ rlb,
+                                           const double* __restrict__ x, const double* __restrict__ alp,
+                                           const double* __restrict__ diag, const double* su, double sll, bool uni) {
+    using M = MfmaT<double>;
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int K = j * NB;
+    const double* Qj = Am + (size_t)(j * NB) * np;
+    // this lane's column of the entry pass
+    const int Cc = j * NB + lane;
+    const bool cin = Cc < nr;
+    const int ib = cin ? rab[Cc] : 0, lb = (cin && !uni) ? rlb[Cc] : 0;
+    const double3 cx = (cin && uni) ? ld3(x, ib) : make_double3(0.0, 0.0, 0.0);
+    const double ca = (cin && uni) ? alp[ib] : 0.0, cu = (cin && uni) ? su[Cc] : 0.0;
+    const double cdg = (cin && uni) ? diag[ib] : 0.0;
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+        const int r0 = i * NB + 32 * h;
+        if (uni) {
+            // row r0 + (lane & 31) loaded by this lane
+            const int Rl = r0 + (lane & 31);
+            const bool rinl = Rl < nr;
+            const int ial = rinl ? rab[Rl] : 0;
+            const double3 rxl = rinl ? ld3(x, ial) : make_double3(0.0, 0.0, 0.0);
+            const double ral = rinl ? alp[ial] : 0.0, rul = rinl ? su[Rl] : 0.0;
+#pragma unroll 4
+            for (int rr = 0; rr < 32; ++rr) {
+                const int R = r0 + rr;
+                const double3 rx = make_double3(readlane_d(rxl.x, rr), readlane_d(rxl.y, rr), readlane_d(rxl.z, rr));
+                const double ra = readlane_d(ral, rr), ru = readlane_d(rul, rr);
+                double a;
+                if (R >= nr || !cin) {
+                    a = (R == Cc) ? 1.0 : 0.0;
+                } else {
+                    const double3 d = rx - cx;
+                    const double aij = (R == Cc) ? cdg : eeq_a_r2(dot(d, d), ra, ca);
+                    a = aij - ru - cu + sll;
+                }
+                Am[(size_t)R * np + Cc] = a;
+            }
+        } else {
+#pragma unroll 2
+            for (int rr = 0; rr < 32; ++rr) {
+                const int R = r0 + rr;
+                double a;
+                if (R >= nr || !cin) {
+                    a = (R == Cc) ? 1.0 : 0.0;
+                } else {
+                    const int ia = rab[R], la = rlb[R];
+                    a = eeq_a(ia, ib, x, alp, diag) - eeq_a(ia, lb, x, alp, diag) - eeq_a(la, ib, x, alp, diag) +
+                        eeq_a(la, lb, x, alp, diag);
+                }
+                Am[(size_t)R * np + Cc] = a;
+            }
+        }
+        // element v of block (xx, y): row r0 + 16 xx + 4 v + lk, column j 64 + 16 y + li,
+        // written above by lane 16 y + li and read after the wavefront's own stores
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        double* pt = Am + (size_t)(r0 + lk) * np + j * NB + li;  // this lane's element (0, 0, 0)
+        dbl4 acc[2][4];
+#pragma unroll
+        for (int xx = 0; xx < 2; ++xx)
+#pragma unroll
+            for (int y = 0; y < 4; ++y)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[xx][y][v] = -pt[(size_t)(16 * xx + 4 * v) * np + 16 * y];
+        if (K > 0) {
+            const double* Pi = Am + (size_t)r0 * np;
+            dbl2 fa[2], fb[4];
+#pragma unroll
+            for (int xx = 0; xx < 2; ++xx) fa[xx] = *(const dbl2*)(Pi + (size_t)(16 * xx + li) * np + 2 * lk);
+#pragma unroll
+            for (int y = 0; y < 4; ++y) fb[y] = *(const dbl2*)(Qj + (size_t)(16 * y + li) * np + 2 * lk);
+#pragma unroll 1
+            for (int kc = 0; kc < K; kc += 8) {
+                const bool more = kc + 8 < K;
+                dbl2 na[2], nb[4];
+                if (more) {
+#pragma unroll
+                    for (int xx = 0; xx < 2; ++xx)
+                        na[xx] = *(const dbl2*)(Pi + (size_t)(16 * xx + li) * np + kc + 8 + 2 * lk);
+#pragma unroll
+                    for (int y = 0; y < 4; ++y) nb[y] = *(const dbl2*)(Qj + (size_t)(16 * y + li) * np + kc + 8 + 2 * lk);
+                }
+#pragma unroll
+                for (int st = 0; st < 2; ++st)
+#pragma unroll
+                    for (int xx = 0; xx < 2; ++xx)
+#pragma unroll
+                        for (int y = 0; y < 4; ++y) acc[xx][y] = M::op(fa[xx][st], fb[y][st], acc[xx][y]);
+                if (more) {
+#pragma unroll
+                    for (int xx = 0; xx < 2; ++xx) fa[xx] = na[xx];
+#pragma unroll
+                    for (int y = 0; y < 4; ++y) fb[y] = nb[y];
+                }
+            }
+        }
+        // the element addresses are formed again here rather than kept in
+        // registers across the product (an empty asm makes pt opaque)
+        asm volatile("" : "+v"(pt));
+#pragma unroll
+        for (int xx = 0; xx < 2; ++xx)
+#pragma unroll
+            for (int y = 0; y < 4; ++y)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) pt[(size_t)(16 * xx + 4 * v) * np + 16 * y] = -acc[xx][y][v];
+    }
+}
+
+// One workgroup of four wavefronts per molecule (largest systems first),
+// left-looking by 64-column blocks.  Column block j: the tiles (i, j), i >= j,
+// are formed (fused_tile, one per wavefront, dealt round-robin); the diagonal
+// block is factored and inverted (chol_diag_block); the tiles below it become
+// L(i, j) = C(i, j) L(j, j)^-T (MFMA, one per wavefront).  Against the
+// right-looking form every tile is written once, the factored panels are read
+// once per column block with the product accumulated in registers over the
+// whole K, and the matrix is never built as a separate pass (no R in HBM):
+// the only matrix traffic is L, so the kernel reads and writes about a third
+// of the bytes.  The substitutions follow in k_bsolve_subst<double>.
+template <int MINB>
+__global__ __launch_bounds__(BSOLVE_T, MINB) void k_eeq_batch_fused(
+    const int* __restrict__ order, const int* __restrict__ mnr, const int* __restrict__ mnp,
+    const long long* __restrict__ mo, const int* __restrict__ vo, const int* __restrict__ ra,
+    const int* __restrict__ rl, const double* __restrict__ x, const double* __restrict__ alp,
+    const double* __restrict__ diag, double* __restrict__ A, double* __restrict__ Linv, int* __restrict__ flags) {
+    __shared__ double S0[NB][NB + 1];
+    __shared__ double col[2][NB];
+    __shared__ double rdiag[NB];
+    __shared__ double su[BATCH_MAX_NP];
+    constexpr int NW = BSOLVE_T / 64;
+    const int m = order[blockIdx.x], nr = mnr[m], np = mnp[m], nb = np / NB, base = vo[m];
+    const int tid = threadIdx.x, wave = tid >> 6;
+    double* Am = A + mo[m];
+    double* Lm = Linv + (size_t)base * NB;
+    const int* rab = ra + base;
+    const int* rlb = rl + base;
+    // one fragment: every row's last atom is row 0's
+    const int L0 = nr > 0 ? rlb[0] : 0;
+    bool same = true;
+    for (int r = tid; r < nr; r += BSOLVE_T) same = same && rlb[r] == L0;
+    const bool uni = __syncthreads_and(same) && nr > 0;
+    if (uni)
+        for (int r = tid; r < nr; r += BSOLVE_T) su[r] = eeq_a(rab[r], L0, x, alp, diag);
+    const double sll = uni ? eeq_a(L0, L0, x, alp, diag) : 0.0;
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        for (int i = j + wave; i < nb; i += NW) fused_tile(Am, np, nr, i, j, rab, rlb, x, alp, diag, su, sll, uni);
+        __syncthreads();
+        batch_diag_block<double>(np, j * NB, Am, Lm, flags, S0, col, rdiag);
+        __syncthreads();
+        for (int i = j + 1 + wave; i < nb; i += NW)
+            batch_tile_mfma<double, false, 1>(Am, np, i * NB, j * NB, Am + (size_t)i * NB * np + j * NB,
+                                              Lm + (size_t)j * NB * NB, NB);
+        __syncthreads();
+    }
+}
+
+// ---- left-looking factorisation by column blocks over the whole batch (batch_solver 5)
+// Column block j is three launches over every molecule with more than j
+// column blocks: the tiles (i, j), i >= j, formed by fused_tile (one
+// wavefront per tile, any molecule), the diagonal blocks (k_bstep_diag) and
+// the panel tiles L(i, j) = C(i, j) L(j, j)^-T (k_bstep_tiles<double,
+// false>).  The per-molecule form's parallelism (four wavefronts) becomes
+// thousands of independent tiles per launch, and the entries of R are still
+// evaluated once, inside the tile that needs them.
+// u_v = A(ia_v, last atom of v's fragment) for every reduced row (0 on padding)
+__global__ __launch_bounds__(256) void k_red_u(int nv, const int* __restrict__ ra, const int* __restrict__ rl,
+                                               const double* __restrict__ x, const double* __restrict__ alp,
+                                               const double* __restrict__ diag, double* __restrict__ u) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    u[v] = (ra[v] >= 0) ? eeq_a(ra[v], rl[v], x, alp, diag) : 0.0;
+}
+
+// tiles {m, i, one-fragment flag, 0} of column block j
+__global__ __launch_bounds__(256, 3) void k_lstep_tiles(const int4* __restrict__ tl, int ntl, int j,
+                                                     const int* __restrict__ mnr, const int* __restrict__ mnp,
+                                                     const long long* __restrict__ mo, const int* __restrict__ vo,
+                                                     const int* __restrict__ ra, const int* __restrict__ rl,
+                                                     const double* __restrict__ x, const double* __restrict__ alp,
+                                                     const double* __restrict__ diag, const double* __restrict__ u,
+                                                     double* __restrict__ A) {
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= ntl) return;  // wavefront-local work: no barrier follows
+    const int4 tt = tl[w];
+    const int m = tt.x, base = vo[m];
+    const bool uni = tt.z != 0;
+    const double sll = uni ? eeq_a(rl[base], rl[base], x, alp, diag) : 0.0;
+    fused_tile(A + mo[m], mnp[m], mnr[m], tt.y, j, ra + base, rl + base, x, alp, diag, u + base, sll, uni);
 }
 
 // Previous per-molecule solve (whole workgroup per tile step), kept for
@@ -4257,6 +4635,12 @@ struct Lookahead {
         hipLaunchKernelGGL(kern, grid, block, 0, stream, __VA_ARGS__);        \
         E->timer.mark(cls, stream);                                           \
     } while (0)
+#define LAUNCH_SHM(cls, kern, grid, block, shm, stream, ...)                  \
+    do {                                                                      \
+        E->timer.mark(cls, stream);                                           \
+        hipLaunchKernelGGL(kern, grid, block, shm, stream, __VA_ARGS__);      \
+        E->timer.mark(cls, stream);                                           \
+    } while (0)
 
 struct DeviceEngine {
     KernelTimer timer;
@@ -4266,15 +4650,22 @@ struct DeviceEngine {
     int nslots = 0;
     int pair_cj = 0, pair_nchunk = 0;   // j-chunk size and count of the pair kernel
     int ps_nb = 0;                      // 64-atom blocks of the symmetric pair kernel (single molecules)
+    int ps_ntiles = 0, ps_nblocks = 0;  // its tiles and 64-atom blocks (0 tiles: batches on the row kernel)
+    const int4* ps_tiles = nullptr;     // batches: tiles {a0, n, R, C} of every molecule
+    const int4* ps_blocks = nullptr;    // batches: blocks {a0, n, A, first tile of the molecule}
     double* tbox = nullptr;             // bounding boxes of the 256-atom tiles (CN partner pruning)
     hipStream_t side = nullptr;         // builds the reduced EEQ matrix alongside the evaluation stream
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fact = nullptr;       // batches: the factorisations on the side stream are done
     double* ps_part = nullptr;          // its per-tile row and column partials
     bool mixed = false;                 // FP32 pair terms + FP32 factorisation with FP64 refinement
     Lookahead la;                       // side stream of the factorisation
     bool use_lookahead = false;
     bool use_cg = false;                // iterative EEQ solve (single molecules)
-    int batch_solver = 0;               // batches: 0 k_eeq_batch_solve, 1 k_eeq_batch_solve_tiles
+    int batch_solver = 5;               // batches: 0 / 1 variants of k_eeq_batch_factor, 2 k_eeq_batch_solve_tiles,
+                                        // 3 factorisation by steps over the batch, 4 k_eeq_batch_fused,
+                                        // 5 left-looking by column blocks over the batch
+    bool batch_fp32 = false;            // mixed batches: FP32 factor with FP64 refinement
     int cg_groups = 0;                  // workgroups of the persistent PCG kernel
     double cg_tol2 = 0.0;               // squared relative residual at which the PCG solve stops
     double *cg_z = nullptr, *cg_p = nullptr;
@@ -4288,8 +4679,18 @@ struct DeviceEngine {
     int *mol_of = nullptr, *mol_off = nullptr, *mol_seg = nullptr, *mnr = nullptr, *mnp = nullptr, *mvo = nullptr;
     int* morder = nullptr;  // batch: molecules by decreasing reduced size (solve launch order)
     bool cn_by_mol = false;  // batch of molecules of at most CNMOL_MAX atoms: k_cn_mol
+    int cn_cap = 0;          // its LDS capacity in atoms (the largest molecule, rounded up to 64)
     long long* mmo = nullptr;
     int4* btiles = nullptr;
+    // batch_solver 3: work of step k over the batch, at [off[k], off[k + 1]) of each list
+    std::vector<int> bs_doff, bs_poff, bs_uoff;
+    int* bs_dmols = nullptr;            // molecules with more than k column blocks
+    int4* bs_ptiles = nullptr;          // panel tiles {m, bi, 0, 0}
+    int4* bs_utiles = nullptr;          // trailing tiles {m, bi, bj, 0}
+    // batch_solver 5: tiles {m, i, one-fragment flag, 0} of column block j at [ls_off[j], ls_off[j + 1])
+    std::vector<int> ls_off;
+    int4* ls_tiles = nullptr;
+    double* ured = nullptr;             // u of every reduced row (k_red_u)
     double* e_mol = nullptr;
     double *adj_w = nullptr, *adj_u = nullptr, *adj_part = nullptr, *adj_bpart = nullptr, *adj_out = nullptr;
     float *Ared32 = nullptr, *Linv32 = nullptr, *b32 = nullptr, *y32 = nullptr, *d32 = nullptr;
@@ -4406,6 +4807,7 @@ struct DeviceEngine {
         }
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        if (ev_fact) (void)hipEventDestroy(ev_fact);
         if (own) {
             (void)hipStreamSynchronize(own);
             (void)hipStreamDestroy(own);
@@ -4441,6 +4843,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt
     E->use_lookahead = lookahead;
     E->use_cg = solver == 0 && P.nmol == 1;
     E->batch_solver = opt.batch_solver;
+    E->batch_fp32 = opt.batch_fp32 != 0;
     {
         const double tol = pcg_tol > 0.0 ? pcg_tol : (E->mixed ? PCG_TOL_MIXED : PCG_TOL_FP64);
         E->cg_tol2 = tol * tol;
@@ -4545,6 +4948,52 @@ DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt
                     for (int bj = 0; bj <= bi; ++bj) tiles.push_back(make_int4(m, bi, bj, 0));
             E->nbtiles = (int)tiles.size();
             E->btiles = E->upload(tiles);
+            {
+                int maxnb = 0;
+                for (int m = 0; m < P.nmol; ++m) maxnb = std::max(maxnb, mnp[m] / NB);
+                std::vector<std::vector<int>> dm(maxnb);
+                std::vector<std::vector<int4>> pt(maxnb), ut(maxnb);
+                for (int m = 0; m < P.nmol; ++m) {
+                    const int nbm = mnp[m] / NB;
+                    for (int k = 0; k < nbm; ++k) {
+                        dm[k].push_back(m);
+                        for (int bi = k + 1; bi < nbm; ++bi) {
+                            pt[k].push_back(make_int4(m, bi, 0, 0));
+                            for (int bj = k + 1; bj <= bi; ++bj) ut[k].push_back(make_int4(m, bi, bj, 0));
+                        }
+                    }
+                }
+                std::vector<int> dflat;
+                std::vector<int4> pflat, uflat;
+                E->bs_doff.assign(1, 0);
+                E->bs_poff.assign(1, 0);
+                E->bs_uoff.assign(1, 0);
+                for (int k = 0; k < maxnb; ++k) {
+                    dflat.insert(dflat.end(), dm[k].begin(), dm[k].end());
+                    pflat.insert(pflat.end(), pt[k].begin(), pt[k].end());
+                    uflat.insert(uflat.end(), ut[k].begin(), ut[k].end());
+                    E->bs_doff.push_back((int)dflat.size());
+                    E->bs_poff.push_back((int)pflat.size());
+                    E->bs_uoff.push_back((int)uflat.size());
+                }
+                E->bs_dmols = E->upload(dflat);
+                if (!pflat.empty()) E->bs_ptiles = E->upload(pflat);
+                if (!uflat.empty()) E->bs_utiles = E->upload(uflat);
+                // left-looking column tiles; a molecule is one-fragment when
+                // all its reduced rows share the last atom
+                std::vector<int4> lflat;
+                E->ls_off.assign(1, 0);
+                std::vector<int> uni(P.nmol, 1);
+                for (int m = 0; m < P.nmol; ++m)
+                    for (int r = 1; r < mnr[m]; ++r)
+                        if (red_last[vo[m] + r] != red_last[vo[m]]) uni[m] = 0;
+                for (int j = 0; j < maxnb; ++j) {
+                    for (int m = 0; m < P.nmol; ++m)
+                        for (int i = j; i < mnp[m] / NB; ++i) lflat.push_back(make_int4(m, i, uni[m] && mnr[m] > 0, 0));
+                    E->ls_off.push_back((int)lflat.size());
+                }
+                E->ls_tiles = E->upload(lflat);
+            }
             E->mnr = E->upload(mnr);
             E->mnp = E->upload(mnp);
             {
@@ -4558,6 +5007,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt
             int maxat = 0;
             for (int m2 = 0; m2 < P.nmol; ++m2) maxat = std::max(maxat, P.mol_off[m2 + 1] - P.mol_off[m2]);
             E->cn_by_mol = maxat <= CNMOL_MAX;
+            E->cn_cap = std::max(64, (maxat + 63) / 64 * 64);
             E->mvo = E->upload(vo);
             E->mmo = E->upload(mo);
             E->mol_off = E->upload(P.mol_off);
@@ -4945,6 +5395,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt
         E->ybuf = E->alloc<double>(E->np);
         E->xbuf = E->alloc<double>(E->np);
         E->ready = E->alloc<int>(E->np / NB);
+        if (E->nmol > 1) E->ured = E->alloc<double>(E->np);
         if (E->nmol == 1 && E->nr <= LU_MAX) {
             E->lu_w = E->alloc<double>((size_t)std::max(E->nr, 1) * (std::max(E->nr, 1) + 1));
             // the FP64 Cholesky factors R in place: keep a copy for the fallback
@@ -4956,6 +5407,9 @@ DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt
             E->cg_part = E->alloc<double>((size_t)(E->np / NB) * E->np);
             E->cg_mb = E->alloc<unsigned long long>(MB_WORDS);
             E->cg_iters = E->alloc<int>(1);
+        } else if (E->mixed && E->nmol > 1 && E->batch_fp32 && E->batch_solver != 2 && E->batch_solver < 4) {
+            E->Ared32 = E->alloc<float>(E->mat_size);  // FP32 copies of the molecules' matrices and factors
+            E->Linv32 = E->alloc<float>((size_t)E->np * NB);
         } else if (E->mixed && E->nmol == 1) {
             E->Ared32 = E->alloc<float>((size_t)E->np * E->np);
             E->Linv32 = E->alloc<float>((size_t)E->np * NB);
@@ -5028,8 +5482,26 @@ DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt
         if (E->nmol == 1) {
             // each unordered pair once: 64 x 64 tiles of the lower triangle
             E->ps_nb = cdiv(n, PS_B);
+            E->ps_ntiles = E->ps_nb * (E->ps_nb + 1) / 2;
+            E->ps_nblocks = E->ps_nb;
             E->tbox = E->alloc<double>(6 * (size_t)cdiv(n, 256));
-            E->ps_part = E->alloc<double>((size_t)E->ps_nb * (E->ps_nb + 1) / 2 * 2 * PART_K * PS_B);
+            E->ps_part = E->alloc<double>((size_t)E->ps_ntiles * 2 * PART_K * PS_B);
+        } else if (opt.batch_pairs == 0) {
+            // batches: the same tiles per molecule, listed {a0, n, R, C}, and the
+            // molecules' 64-atom blocks {a0, n, A, first tile} for the reduction
+            std::vector<int4> tl, bl;
+            for (int m = 0; m < P.nmol; ++m) {
+                const int a0 = P.mol_off[m], nm = P.mol_off[m + 1] - a0, nbm = cdiv(nm, PS_B);
+                const int base = (int)tl.size();
+                for (int R = 0; R < nbm; ++R)
+                    for (int C = 0; C <= R; ++C) tl.push_back(make_int4(a0, nm, R, C));
+                for (int A = 0; A < nbm; ++A) bl.push_back(make_int4(a0, nm, A, base));
+            }
+            E->ps_ntiles = (int)tl.size();
+            E->ps_nblocks = (int)bl.size();
+            E->ps_tiles = E->upload(tl);
+            E->ps_blocks = E->upload(bl);
+            E->ps_part = E->alloc<double>((size_t)E->ps_ntiles * 2 * PART_K * PS_B);
         }
         E->e_bond = E->alloc<double>(std::max(E->nbond, 1));
         E->e_ang = E->alloc<double>(std::max(E->nangle, 1));
@@ -5054,7 +5526,8 @@ DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt
             hipEventCreateWithFlags(&E->la.join, hipEventDisableTiming) != hipSuccess ||
             hipStreamCreateWithFlags(&E->side, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&E->ev_fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&E->ev_join, hipEventDisableTiming) != hipSuccess)
+            hipEventCreateWithFlags(&E->ev_join, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&E->ev_fact, hipEventDisableTiming) != hipSuccess)
             throw std::runtime_error("stream");
         hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, E->own, 1, E->flags, 0);
         hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, E->own, 1, E->sticky, 0);
@@ -5161,6 +5634,71 @@ static void queue_solve(DeviceEngine* E, hipStream_t s, const T* A, const T* Lin
     LAUNCH(KC_TRSV, k_trsv_bwd_sweep<T>, dim3(nbk), dim3(256), s, np, A, Linv, y, x, E->ready, E->flags);
 }
 
+// batch_solver 3: the per-molecule factorisations advanced one column block at
+// a time over the whole batch (three launches per step), then the
+// substitutions (and, for FP32 factors, the FP64 refinement) in one launch.
+template <typename T>
+static void queue_batch_steps(DeviceEngine* E, hipStream_t s, T* A, T* Linv) {
+    const int nsteps = (int)E->bs_doff.size() - 1;
+    for (int k = 0; k < nsteps; ++k) {
+        const int nd = E->bs_doff[k + 1] - E->bs_doff[k];
+        const int npn = E->bs_poff[k + 1] - E->bs_poff[k];
+        const int nu = E->bs_uoff[k + 1] - E->bs_uoff[k];
+        LAUNCH(KC_BATCH_SOLVE, k_bstep_diag<T>, dim3(nd), dim3(256), s, E->bs_dmols + E->bs_doff[k], k, E->mnp, E->mmo,
+               E->mvo, A, Linv, E->flags);
+        if (npn)
+            LAUNCH(KC_BATCH_SOLVE, (k_bstep_tiles<T, false>), dim3(cdiv(npn, 4)), dim3(256), s,
+                   E->bs_ptiles + E->bs_poff[k], npn, k, E->mnp, E->mmo, E->mvo, A, Linv);
+        if (nu)
+            LAUNCH(KC_BATCH_SOLVE, (k_bstep_tiles<T, true>), dim3(cdiv(nu, 4)), dim3(256), s,
+                   E->bs_utiles + E->bs_uoff[k], nu, k, E->mnp, E->mmo, E->mvo, A, Linv);
+    }
+}
+
+// The batch factorisation (every variant but the whole-workgroup solve, which
+// includes its substitutions) on stream s.
+static void queue_batch_factor(DeviceEngine* E, hipStream_t s) {
+    const dim3 g(E->nmol), b(BSOLVE_T);
+    if (E->batch_solver == 4) {  // left-looking, matrix formed in place (FP64 on both paths)
+        LAUNCH(KC_BATCH_SOLVE, k_eeq_batch_fused<3>, g, b, s, E->morder, E->mnr, E->mnp, E->mmo, E->mvo,
+               E->red_atom, E->red_last, E->x, E->alp, E->diag, E->Ared, E->Linv, E->flags);
+    } else if (E->batch_solver == 5) {  // left-looking by column blocks over the batch (FP64 on both paths)
+        LAUNCH(KC_BATCH_SOLVE, k_red_u, dim3(cdiv(E->np, 256)), dim3(256), s, E->np, E->red_atom, E->red_last, E->x,
+               E->alp, E->diag, E->ured);
+        const int ncol = (int)E->ls_off.size() - 1;
+        for (int j = 0; j < ncol; ++j) {
+            const int nt = E->ls_off[j + 1] - E->ls_off[j];
+            const int nd = E->bs_doff[j + 1] - E->bs_doff[j];
+            const int npn = E->bs_poff[j + 1] - E->bs_poff[j];
+            LAUNCH(KC_BATCH_SOLVE, k_lstep_tiles, dim3(cdiv(nt, 4)), dim3(256), s, E->ls_tiles + E->ls_off[j], nt, j,
+                   E->mnr, E->mnp, E->mmo, E->mvo, E->red_atom, E->red_last, E->x, E->alp, E->diag, E->ured, E->Ared);
+            LAUNCH(KC_BATCH_SOLVE, k_bstep_diag<double>, dim3(nd), dim3(256), s, E->bs_dmols + E->bs_doff[j], j,
+                   E->mnp, E->mmo, E->mvo, E->Ared, E->Linv, E->flags);
+            if (npn)
+                LAUNCH(KC_BATCH_SOLVE, (k_bstep_tiles<double, false>), dim3(cdiv(npn, 4)), dim3(256), s,
+                       E->bs_ptiles + E->bs_poff[j], npn, j, E->mnp, E->mmo, E->mvo, E->Ared, E->Linv);
+        }
+    } else if (E->mixed && E->batch_fp32) {
+        if (E->batch_solver == 3)
+            queue_batch_steps<float>(E, s, E->Ared32, E->Linv32);
+        else if (E->batch_solver == 1)  // two workgroups per CU, operands four steps ahead
+            LAUNCH(KC_BATCH_SOLVE, (k_eeq_batch_factor<float, 2, 4>), g, b, s, E->morder, E->mnp, E->mmo, E->mvo,
+                   E->Ared32, E->Linv32, E->flags);
+        else  // three workgroups per CU, operands one step ahead
+            LAUNCH(KC_BATCH_SOLVE, (k_eeq_batch_factor<float, 3, 1>), g, b, s, E->morder, E->mnp, E->mmo, E->mvo,
+                   E->Ared32, E->Linv32, E->flags);
+    } else {
+        if (E->batch_solver == 3)
+            queue_batch_steps<double>(E, s, E->Ared, E->Linv);
+        else if (E->batch_solver == 1)
+            LAUNCH(KC_BATCH_SOLVE, (k_eeq_batch_factor<double, 2, 4>), g, b, s, E->morder, E->mnp, E->mmo, E->mvo,
+                   E->Ared, E->Linv, E->flags);
+        else
+            LAUNCH(KC_BATCH_SOLVE, (k_eeq_batch_factor<double, 3, 1>), g, b, s, E->morder, E->mnp, E->mmo, E->mvo,
+                   E->Ared, E->Linv, E->flags);
+    }
+}
+
 static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::string& msg) {
     const int n = E->n;
     hipLaunchKernelGGL(k_copy_start, dim3(cdiv(3 * n, 256)), dim3(256), 0, s, 3 * n, xin, E->x, E->flags);
@@ -5184,12 +5722,19 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
                                MB_EMPTY);
         }
     }
-    else
+    else if (E->batch_solver < 4)  // batch_solver 4 and 5 form the matrix inside their factorisations
         LAUNCH(KC_EEQ_BUILD, k_eeq_reduced_batch, dim3(E->nbtiles), dim3(256), E->side, E->btiles, E->mnr, E->mnp,
-               E->mmo, E->mvo, E->red_atom, E->red_last, E->x, E->alp, E->diag, E->Ared);
+               E->mmo, E->mvo, E->red_atom, E->red_last, E->x, E->alp, E->diag, E->Ared,
+               E->Ared32);  // the FP32 copy exists only when the mixed path factors in FP32
+    const bool split = E->nmol > 1 && E->batch_solver != 2;
+    if (split) {
+        queue_batch_factor(E, E->side);
+        HIP_OK(hipEventRecord(E->ev_fact, E->side));
+    }
     if (!E->mol_of) LAUNCH(KC_CN, k_tile_box, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->tbox);
     if (E->cn_by_mol)
-        LAUNCH(KC_CN, k_cn_mol, dim3(E->nmol), dim3(256), s, E->mol_off, E->x, E->rcov_cn, E->elem, E->nref,
+        LAUNCH_SHM(KC_CN, k_cn_mol, dim3(E->nmol), dim3(256), (size_t)E->cn_cap * CNMOL_LDS_BYTES, s, E->cn_cap,
+                   E->mol_off, E->x, E->rcov_cn, E->elem, E->nref,
                E->refcn, E->chi_base, E->cnf, E->cn, E->dlog, E->gw, E->dgw, E->rhs, E->nbn, E->nbl, E->flags);
     else
         LAUNCH(KC_CN, k_cn, dim3(cdiv(n, 4)), dim3(256), s, n, E->x, E->rcov_cn, E->elem, E->nref, E->refcn,
@@ -5284,15 +5829,21 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         LAUNCH(KC_HB, k_hb_gather<16>, dim3(cdiv(n, 16)), dim3(256), hs, n, E->hb_crange, E->hb_gout, E->hb_xoff,
                E->hb_xidx, E->hb_xslots, E->ghb);
     HIP_OK(hipEventRecord(E->ev_join, E->side));
-    HIP_OK(hipStreamWaitEvent(s, E->ev_join, 0));
+    // batches wait here for the factorisations only; the hydrogen-bond work
+    // still queued behind them on the side stream joins before k_finish
+    HIP_OK(hipStreamWaitEvent(s, split ? E->ev_fact : E->ev_join, 0));
     if (E->nmol > 1) {
-        // batch: every molecule's system factored and solved by one workgroup (FP64)
-        if (E->batch_solver == 1)
+        // batch: every molecule's system factored (side stream, above) and
+        // solved by one workgroup per molecule
+        if (E->batch_solver == 2)
             LAUNCH(KC_BATCH_SOLVE, k_eeq_batch_solve_tiles, dim3(E->nmol), dim3(256), s, E->morder, E->mnp, E->mmo,
                    E->mvo, E->Ared, E->Linv, E->b, E->xbuf, E->flags);
+        else if (E->mixed && E->batch_fp32 && E->batch_solver < 4)
+            LAUNCH(KC_BATCH_SOLVE, k_bsolve_subst<float>, dim3(E->nmol), dim3(BSOLVE_T), s, E->morder, E->mnp,
+                   E->mmo, E->mvo, E->Ared32, E->Linv32, E->Ared, E->b, E->xbuf);
         else
-            LAUNCH(KC_BATCH_SOLVE, k_eeq_batch_solve, dim3(E->nmol), dim3(BSOLVE_T), s, E->morder, E->mnp, E->mmo,
-                   E->mvo, E->Ared, E->Linv, E->b, E->xbuf, E->flags);
+            LAUNCH(KC_BATCH_SOLVE, k_bsolve_subst<double>, dim3(E->nmol), dim3(BSOLVE_T), s, E->morder, E->mnp,
+                   E->mmo, E->mvo, E->Ared, E->Linv, (const double*)nullptr, E->b, E->xbuf);
     } else if (E->use_cg) {
         // the persistent PCG (block-Jacobi factors and mailboxes were set up on the side stream)
         LAUNCH(KC_PCG, k_eeq_pcg, dim3(E->cg_groups), dim3(PCG_THREADS), s, np, E->Ared, E->Linv, E->b, E->xbuf,
@@ -5329,27 +5880,27 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     PairArgs pa = E->pair_args();
     LAUNCH(KC_PAIRS_AUX, k_d4_half, dim3(cdiv(n * E->nelem, 256)), dim3(256), s, n, E->nelem, E->elem, E->nref,
            E->c6ref, E->gw, E->dgw, E->h6, E->dh6);
-    if (E->nmol == 1) {
-        const dim3 grid(E->ps_nb * (E->ps_nb + 1) / 2);
+    if (E->ps_ntiles > 0) {  // each unordered pair once (single molecules; batches unless batch_pairs = 1)
+        const dim3 grid(E->ps_ntiles);
         if (!E->mixed) {
             switch (E->nelem) {
-                case 1: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 1>), grid, dim3(PS_B), s, pa, E->ps_part); break;
-                case 2: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 2>), grid, dim3(PS_B), s, pa, E->ps_part); break;
-                case 3: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 3>), grid, dim3(PS_B), s, pa, E->ps_part); break;
-                case 4: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 4>), grid, dim3(PS_B), s, pa, E->ps_part); break;
-                default: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 0>), grid, dim3(PS_B), s, pa, E->ps_part); break;
+                case 1: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 1>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
+                case 2: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 2>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
+                case 3: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 3>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
+                case 4: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 4>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
+                default: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 0>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
             }
         } else {
             switch (E->nelem) {
-                case 1: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 1>), grid, dim3(PS_B), s, pa, E->ps_part); break;
-                case 2: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 2>), grid, dim3(PS_B), s, pa, E->ps_part); break;
-                case 3: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 3>), grid, dim3(PS_B), s, pa, E->ps_part); break;
-                case 4: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 4>), grid, dim3(PS_B), s, pa, E->ps_part); break;
-                default: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 0>), grid, dim3(PS_B), s, pa, E->ps_part); break;
+                case 1: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 1>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
+                case 2: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 2>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
+                case 3: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 3>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
+                case 4: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 4>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
+                default: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 0>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
             }
         }
-        LAUNCH(KC_PAIRS_AUX, k_pairs_sym_reduce, dim3(E->ps_nb), dim3(256), s, n, E->ps_nb, E->ps_part, E->gpair,
-               E->dEdcn, E->e_coul, E->e_disp, E->e_rep);
+        LAUNCH(KC_PAIRS_AUX, k_pairs_sym_reduce, dim3(E->ps_nblocks), dim3(256), s, n, E->ps_nb, E->ps_part,
+               E->gpair, E->dEdcn, E->e_coul, E->e_disp, E->e_rep, E->ps_blocks);
     } else {
         const dim3 grid(cdiv(n, TILE), E->pair_nchunk);
         if (!E->mixed) {
@@ -5378,6 +5929,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
                               E->dcn_slot, E->dEdcn, E->qtmp, E->vchain};
         LAUNCH(KC_CHAIN, k_atom_tail, dim3(cdiv(n, 256)), dim3(256), s, n, ta);
     }
+    if (split) HIP_OK(hipStreamWaitEvent(s, E->ev_join, 0));
     if (E->fin_lpa == 64)
         LAUNCH(KC_CHAIN, k_finish<64>, dim3(cdiv(n, 4)), dim3(256), s, n, E->x, E->rcov_cn, E->dlog, E->vchain,
                        E->nbn, E->nbl, E->slot_off, E->slot_idx, E->slots, E->gpair, E->ghb, E->grad);

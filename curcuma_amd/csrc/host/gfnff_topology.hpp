@@ -37,8 +37,15 @@ struct Options {
     double hb_update_rmsd_bohr = 0.3;  // HB/XB re-detection threshold on sqrt(sum dx^2)/N
     int eeq_solver = 0;               // 0: preconditioned CG (single molecules), 1: blocked Cholesky
     double pcg_tol = 0.0;             // PCG relative residual |r| / |b|; 0: 1e-13 (FP64), 1e-10 (mixed)
-    int batch_solver = 0;             // batches: 0 wavefront-parallel per-molecule Cholesky, 1 tile-step variant
+    int batch_solver = 5;             // batches: right-looking per-molecule Cholesky with tile phases per
+                                      // wavefront in two occupancy / prefetch variants 0, 1; 2 one FP64 tile
+                                      // step at a time per workgroup; 3 one column block at a time over the
+                                      // whole batch (three launches per block); 4 left-looking per molecule
+                                      // with the matrix formed in place (FP64); 5 (default) the same by column
+                                      // blocks over the whole batch
+    int batch_fp32 = 0;               // mixed batches: 1 factors in FP32 (MFMA) with FP64 refinement
     int gather_lanes = 0;             // lanes per atom of the slot and HB gathers: 0 by list length, 16 or 64
+    int batch_pairs = 0;              // batches: 0 each pair once in 64x64 tiles per molecule, 1 row kernel
     int pcg_groups = 0;               // workgroups of the persistent PCG: 0 all that fit (at most 256), else at most this
 };
 

@@ -116,8 +116,10 @@ def work_per_eval(cls: str, info: dict, rc: dict) -> tuple[float, str]:
                 tot = [a + b for a, b in zip(tot, c)]
             nc, nd, nr = tot
             return (nc * rc["COULOMB_PAIR_FLOPS"] + nd * rc["DISP_PAIR_FLOPS"] + nr * rc["REP_PAIR_FLOPS"]), "flop"
-        if cls == "batch_solve":  # k_eeq_batch_solve: factorisation + both substitutions per molecule
-            return sum(m ** 3 / rc["CHOL_DIV"] + rc["TRSV_FACTOR"] * m * m for m in info["mol_nr"]), "flop"
+        if cls == "batch_solve":  # factorisation + both substitutions per molecule (+ FP64 refinement steps)
+            nref = info.get("batch_refine_steps", 0)
+            return sum(m ** 3 / rc["CHOL_DIV"] + rc["TRSV_FACTOR"] * m * m + nref * 4.0 * m * m
+                       for m in info["mol_nr"]), "flop"
     if cls == "pairs":
         nc, nd, nr = info.get("pairs") or pair_counts(info["xyz_bohr"])
         return (nc * rc["COULOMB_PAIR_FLOPS"] + nd * rc["DISP_PAIR_FLOPS"] + nr * rc["REP_PAIR_FLOPS"]), "flop"
@@ -148,7 +150,8 @@ def work_per_eval(cls: str, info: dict, rc: dict) -> tuple[float, str]:
 
 
 # kernel classes that run in FP32 on the mixed-precision path (the batched
-# per-molecule solve, class batch_solve, stays FP64 on both paths)
+# per-molecule solve, class batch_solve, factors in FP32 only with the engine
+# key batch_fp32 = 1: info["batch_fp32"])
 FP32_CLASSES_MIXED = {"pairs", "chol_update", "chol_diag", "chol_panel", "trsv"}
 # classes whose flops run on the matrix cores
 MFMA_CLASSES = {"chol_update", "chol_diag", "chol_panel", "batch_solve"}
@@ -163,7 +166,8 @@ def roofline(cls: str, ms_total: float, launches: int, evals: int, info: dict, t
     avg_s = ms_total * 1e-3 / max(launches, 1)
     achieved = per_launch / avg_s
     if kind == "flop":
-        peak = PEAK_FP32 if (precision == "mixed" and cls in FP32_CLASSES_MIXED) else PEAK_FP64
+        fp32 = precision == "mixed" and (cls in FP32_CLASSES_MIXED or (cls == "batch_solve" and info.get("batch_fp32")))
+        peak = PEAK_FP32 if fp32 else PEAK_FP64
         bound = "mfma" if cls in MFMA_CLASSES else "valu"
         unit = "TFLOP/s"
         scale = 1e12

@@ -160,9 +160,22 @@ int gfnff_cn(gfnff_handle* h, double* cn);
  * for the mixed path, at most 1e-6); "cholesky_lookahead" 0/1 overlaps the
  * trailing update with the next super-panel on a second stream (off by
  * default).  Launch-shape keys (results unchanged, chosen by the autotuner,
- * curcuma_amd/autotune.py): "batch_solver" 0 = per-molecule Cholesky with
- * the tile phases shared out over wavefronts (default), 1 = one tile step
- * at a time per workgroup; "gather_lanes" 0 (by mean list length), 16 or 64
+ * curcuma_amd/autotune.py): "batch_solver" 0 / 1 = per-molecule Cholesky
+ * with the tile phases shared out over wavefronts in two occupancy /
+ * operand-prefetch variants, 2 = one FP64 tile step at a time per
+ * workgroup (factor and substitutions in one launch), 3 = the
+ * factorisations advanced one column block at a time over the whole batch
+ * (three launches per block: diagonal blocks, panel tiles, trailing tiles),
+ * 4 = left-looking per molecule with the matrix entries formed inside the
+ * factorisation (no separate matrix build; FP64 on both paths), 5 = the
+ * same left-looking form advanced one column block at a time over the whole
+ * batch (three launches per block: column tiles, diagonal blocks, panels;
+ * the default);
+ * "batch_fp32" 1 = on the mixed path, batches factor in FP32 (FP32 MFMA)
+ * and refine in FP64 (charges agree with the FP64 factor to about 1e-12 e;
+ * batch_solver 0, 1 and 3); "batch_pairs" 0 = pair terms of batches with each
+ * pair once in 64 x 64 tiles per molecule (default), 1 = one thread per row
+ * atom over its molecule; "gather_lanes" 0 (by mean list length), 16 or 64
  * lanes per atom in the slot and hydrogen-bond gathers; "pcg_groups" 0 (all
  * workgroups that fit, at most 256) or an upper bound on the persistent
  * PCG's workgroups. */
