@@ -16,6 +16,9 @@ theirs agree to rounding:
   batch_fp32     0 / 1    mixed path: factor in FP64, or in FP32 on MFMA with
                           two FP64 refinement steps (charges agree with the
                           FP64 factor's to about 1e-12 e)
+  pair_split     0 / 1    pair terms in one pass after the EEQ solve, or the
+                          charge-independent dispersion and repulsion before
+                          it (beside the solve) and the Coulomb term after
   batch_pairs    0 / 1    pair terms of batches: each pair once in 64 x 64
                           tiles per molecule, or one thread per row atom
   gather_lanes   16 / 64  lanes per atom in the slot and hydrogen-bond gathers
@@ -41,10 +44,12 @@ from pathlib import Path
 
 import numpy as np
 
-SPACE_BATCH = {"batch_solver": [0, 1, 2, 3, 4, 5], "batch_fp32": [0, 1], "batch_pairs": [0, 1], "gather_lanes": [16, 64]}
+SPACE_BATCH = {"batch_solver": [0, 1, 2, 3, 4, 5], "batch_fp32": [0, 1], "batch_pairs": [0, 1], "pair_split": [0, 1],
+               "gather_lanes": [16, 64]}
 # the engine's values when a key is not set (gather_lanes 0: by mean list length)
-ENGINE_DEFAULTS = {"batch_solver": 5, "batch_fp32": 0, "batch_pairs": 0, "gather_lanes": 0, "pcg_groups": 0}
-SPACE_SINGLE = {"pcg_groups": [0, 128, 192], "gather_lanes": [16, 64]}
+ENGINE_DEFAULTS = {"batch_solver": 5, "batch_fp32": 0, "batch_pairs": 0, "pair_split": 0, "gather_lanes": 0,
+                   "pcg_groups": 0}
+SPACE_SINGLE = {"pcg_groups": [0, 128, 192], "pair_split": [0, 1], "gather_lanes": [16, 64]}
 
 
 def default_cache_path() -> Path:

@@ -1,19 +1,25 @@
-// GFN-FF energy + gradient on one MI355X (gfx950), FP64 path.
+// GFN-FF energy + gradient on one MI355X (gfx950): FP64 path and mixed path,
+// single molecules and batches of independent molecules.
 //
-// One evaluation is a fixed sequence of kernels on one stream:
-//   k_cn            CN (6 Bohr), log cap, dlogdcn, D4 weights, EEQ rhs, CN neighbour lists
-//   k_eeq_reduced   reduced EEQ matrix (fragment constraints eliminated)
-//   k_chol_*        blocked Cholesky (diag / panel / trailing update)
-//   k_trsv_*        forward/backward substitution
-//   k_charges       charges from the reduced solution, self energy, qtmp
-//   k_d4_half       per-atom D4 half contraction of C6 and dC6/dCN
-//   k_pairs_part    Coulomb + D4 dispersion + non-bonded repulsion, LDS-tiled,
-//                   one row atom per thread, full rows split in j chunks
-//   k_pairs_reduce  fixed-order sum of the chunk partials (no atomics)
-//   k_atom_tail     bonded / 1,3 / 1,4 H-H corrections of the repulsion pair sum, dE/dCN totals
+// One evaluation is a fixed sequence of kernels on two streams (launch_eval):
+//   second stream: the reduced EEQ matrix (k_eeq_reduced / k_eeq_reduced_batch)
+//     and, for batches, the per-molecule factorisation (k_lstep_tiles,
+//     k_bstep_diag, k_bstep_tiles or the per-molecule forms), then the
+//     hydrogen-bond kernels of batches;
+//   evaluation stream:
+//   k_cn / k_cn_mol     CN (6 Bohr), log cap, dlogdcn, D4 weights, EEQ rhs, CN neighbour lists
+//   k_d4_half           per-atom D4 half contraction of C6 and dC6/dCN
 //   k_bonds .. k_batm   bonded term lists writing per-atom gradient slots
-//   k_finish        slot gather + CN chain rule + gradient assembly
-//   k_reduce_terms  fixed-order energy reductions
+//   k_slot_gather       per-atom sum of the slots
+//   k_hb_*              hydrogen bonds (single molecules on this stream)
+//   -- join --
+//   EEQ solve           k_eeq_pcg / k_chol_* + k_trsv_* (single), k_bsolve_subst (batches)
+//   k_charges           charges from the reduced solution, self energy, qtmp
+//   k_pairs_sym         Coulomb + D4 dispersion + repulsion, each pair once (64 x 64 tiles)
+//   k_pairs_sym_reduce  fixed-order sum of the tile partials (no atomics)
+//   k_atom_tail         bonded / 1,3 / 1,4 H-H corrections of the repulsion pair sum, dE/dCN totals
+//   k_finish            CN chain rule + gradient assembly
+//   k_reduce1/_final    fixed-order energy reductions
 // Reference semantics: see oracle/gfnff_oracle.c (same formulas, same cut-offs).
 #include <hip/hip_runtime.h>
 
@@ -2029,9 +2035,13 @@ __device__ inline double ps_rcp(double v) { return 1.0 / v; }
 __device__ inline float ps_exp(float v) { return __expf(v); }
 __device__ inline double ps_exp(double v) { return exp(v); }
 
+// terms: PT_COUL | PT_DISP | PT_REP select the pair terms evaluated (all three
+// in one pass, or the charge-independent dispersion and repulsion before the
+// EEQ solve and the Coulomb term after it, into separate partials).
+constexpr int PT_COUL = 1, PT_DISP = 2, PT_REP = 4, PT_ALL = 7;
 template <typename T, int NE>
 __global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restrict__ part,
-                                                    const int4* __restrict__ tiles) {
+                                                    const int4* __restrict__ tiles, int terms) {
     __shared__ double cx[PS_B], cy[PS_B], cz[PS_B];
     __shared__ T cq[PS_B], cal[PS_B], czt[PS_B], csz[PS_B], can[PS_B], crz[PS_B];
     __shared__ T cgw[7][PS_B], cdgw[7][PS_B];
@@ -2101,7 +2111,7 @@ __global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restri
         if (!acti || jj >= jmax) continue;
         const double dxd = xi.x - cx[jj], dyd = xi.y - cy[jj], dzd = xi.z - cz[jj];
         const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
-        if (r2d > 10000.0) continue;  // beyond every pair cut-off (Coulomb 100 Bohr)
+        if (r2d > ((terms & PT_COUL) ? 10000.0 : 2500.0)) continue;  // beyond every pair cut-off (Coulomb 100 Bohr)
         T dEdr = 0, hec = 0, hed = 0, her = 0, dci = 0, dcj = 0;
         const T r2 = (T)r2d;
         T r, rinv;
@@ -2112,9 +2122,10 @@ __global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restri
             r = ps_sqrt(r2);
             rinv = T(1) / r;
         }
-        const bool coul = (sizeof(T) == 4) ? (r2d >= 1e-20) : (r <= T(100) && r >= T(1e-10));
-        const bool disp = (sizeof(T) == 4) ? (r2d <= 2500.0 && r2d >= 1e-20) : (r <= T(50) && r >= T(1e-10));
-        const bool rep = (sizeof(T) == 4) ? (r2d <= 400.0 && r2d >= 1e-16) : (r <= T(20) && r >= T(1e-8));
+        const bool coul = (terms & PT_COUL) && ((sizeof(T) == 4) ? (r2d >= 1e-20) : (r <= T(100) && r >= T(1e-10)));
+        const bool disp = (terms & PT_DISP) &&
+                          ((sizeof(T) == 4) ? (r2d <= 2500.0 && r2d >= 1e-20) : (r <= T(50) && r >= T(1e-10)));
+        const bool rep = (terms & PT_REP) && ((sizeof(T) == 4) ? (r2d <= 400.0 && r2d >= 1e-16) : (r <= T(20) && r >= T(1e-8)));
         if (coul) {
             const T gij = ps_rsqrt(ali + cal[jj]);
             const T gr = gij * r;
@@ -2223,7 +2234,10 @@ __global__ __launch_bounds__(PS_B) void k_pairs_sym(PairArgs a, double* __restri
 // added in order.
 // Batches (blocks != nullptr): one workgroup per 64-atom block {a0, n, A, base}
 // of a molecule (atoms a0 .. a0 + n - 1, its tiles numbered from base).
+// part2 (nullable): a second set of tile partials (the split pair terms),
+// added tile by tile after the first.
 __global__ __launch_bounds__(256) void k_pairs_sym_reduce(int n, int nb, const double* __restrict__ part,
+                                                          const double* __restrict__ part2,
                                                           double* __restrict__ g, double* __restrict__ dEdcn,
                                                           double* __restrict__ e_coul, double* __restrict__ e_disp,
                                                           double* __restrict__ e_rep, const int4* __restrict__ blocks) {
@@ -2244,9 +2258,14 @@ __global__ __launch_bounds__(256) void k_pairs_sym_reduce(int n, int nb, const d
         const bool rowside = u <= A;
         const int Rr = u - 1;  // column side: tile (R, A), R = u - 1 = A .. nb - 1
         const int t = base + (rowside ? A * (A + 1) / 2 + u : Rr * (Rr + 1) / 2 + A);
-        const double* p = part + ((size_t)t * 2 + (rowside ? 0 : 1)) * PART_K * PS_B + l;
+        const size_t o = ((size_t)t * 2 + (rowside ? 0 : 1)) * PART_K * PS_B + l;
+        const double* p = part + o;
 #pragma unroll
         for (int k = 0; k < PART_K; ++k) s[k] += p[k * PS_B];
+        if (part2) {
+#pragma unroll
+            for (int k = 0; k < PART_K; ++k) s[k] += part2[o + k * PS_B];
+        }
     }
 #pragma unroll
     for (int k = 0; k < PART_K; ++k) qs[w][k][l] = s[k];
@@ -4360,18 +4379,34 @@ __device__ inline double group_sum(double v) {
     for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
+// Per-atom sum of the bonded-list gradient slots (every slot writer runs
+// before the EEQ solve, so this gather runs there too, off the solve's path).
 template <int LPA>
-__global__ __launch_bounds__(256) void k_finish(int n, const double* __restrict__ x, const double* __restrict__ rcov,
-                                                const double* __restrict__ dlog, const double* __restrict__ v,
-                                                const int* __restrict__ nbn, const int* __restrict__ nbl,
-                                                const int* __restrict__ soff, const int* __restrict__ sidx,
-                                                const double* __restrict__ slots, const double* __restrict__ gpair,
-                                                const double* __restrict__ ghb, double* __restrict__ grad) {
+__global__ __launch_bounds__(256) void k_slot_gather(int n, const int* __restrict__ soff, const int* __restrict__ sidx,
+                                                     const double* __restrict__ slots, double* __restrict__ gslot) {
     const int lane = threadIdx.x & (LPA - 1);
     const int i = blockIdx.x * (256 / LPA) + threadIdx.x / LPA;
     if (i >= n) return;
     double3 g = make_double3(0, 0, 0);
     for (int k = soff[i] + lane; k < soff[i + 1]; k += LPA) g = g + ld3(slots, sidx[k]);
+    g.x = group_sum<LPA>(g.x);
+    g.y = group_sum<LPA>(g.y);
+    g.z = group_sum<LPA>(g.z);
+    if (lane == 0) st3(gslot, i, g);
+}
+
+// CN chain rule over the atom's neighbour list plus the gathered slot, pair
+// and hydrogen-bond gradients
+template <int LPA>
+__global__ __launch_bounds__(256) void k_finish(int n, const double* __restrict__ x, const double* __restrict__ rcov,
+                                                const double* __restrict__ dlog, const double* __restrict__ v,
+                                                const int* __restrict__ nbn, const int* __restrict__ nbl,
+                                                const double* __restrict__ gslot, const double* __restrict__ gpair,
+                                                const double* __restrict__ ghb, double* __restrict__ grad) {
+    const int lane = threadIdx.x & (LPA - 1);
+    const int i = blockIdx.x * (256 / LPA) + threadIdx.x / LPA;
+    if (i >= n) return;
+    double3 g = make_double3(0, 0, 0);
     const double3 xi = ld3(x, i);
     const double ai = dlog[i] * v[i];
     const int nn = nbn[i];
@@ -4389,7 +4424,7 @@ __global__ __launch_bounds__(256) void k_finish(int n, const double* __restrict_
     g.y = group_sum<LPA>(g.y);
     g.z = group_sum<LPA>(g.z);
     if (lane == 0) {
-        g = g + ld3(gpair, i) + ld3(ghb, i);
+        g = g + ld3(gslot, i) + ld3(gpair, i) + ld3(ghb, i);
         st3(grad, i, g);
     }
 }
@@ -4658,6 +4693,8 @@ struct DeviceEngine {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev_fact = nullptr;       // batches: the factorisations on the side stream are done
     double* ps_part = nullptr;          // its per-tile row and column partials
+    double* ps_part2 = nullptr;         // pair_split: the partials of dispersion and repulsion
+    bool pair_split = false;            // dispersion and repulsion before the EEQ solve, Coulomb after
     bool mixed = false;                 // FP32 pair terms + FP32 factorisation with FP64 refinement
     Lookahead la;                       // side stream of the factorisation
     bool use_lookahead = false;
@@ -4734,7 +4771,7 @@ struct DeviceEngine {
     // work
     double *x, *cn, *dlog, *gw, *dgw, *rhs, *Ared, *Linv, *b, *rres, *q, *eself, *qtmp, *gpair, *dEdcn, *vchain,
         *slots, *dcn_slot, *grad, *e_coul, *e_disp, *e_rep, *e_bond, *e_ang, *e_tor, *e_brep, *e_hb, *e_atm,
-        *e_batm, *e_inv, *e_st, *e_xb, *parts, *rpart, *out, *diag, *ghb, *ybuf, *xbuf, *h6, *dh6, *ppart;
+        *e_batm, *e_inv, *e_st, *e_xb, *parts, *rpart, *out, *diag, *ghb, *gslot, *ybuf, *xbuf, *h6, *dh6, *ppart;
     int *nbn, *nbl, *flags;
     int* sticky = nullptr;  // status flags of every evaluation since the last device_status read
     double *lu_w = nullptr, *lu_src = nullptr;  // LU fallback of a non-SPD reduced system (nr <= LU_MAX)
@@ -5503,6 +5540,8 @@ DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt
             E->ps_blocks = E->upload(bl);
             E->ps_part = E->alloc<double>((size_t)E->ps_ntiles * 2 * PART_K * PS_B);
         }
+        E->pair_split = opt.pair_split != 0 && E->ps_ntiles > 0;
+        if (E->pair_split) E->ps_part2 = E->alloc<double>((size_t)E->ps_ntiles * 2 * PART_K * PS_B);
         E->e_bond = E->alloc<double>(std::max(E->nbond, 1));
         E->e_ang = E->alloc<double>(std::max(E->nangle, 1));
         E->e_tor = E->alloc<double>(std::max(E->ntors, 1));
@@ -5516,6 +5555,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt
         E->parts = E->alloc<double>(16);
         E->rpart = E->alloc<double>(16 * RED_BLOCKS);
         E->ghb = E->alloc<double>(3 * (size_t)n);
+        E->gslot = E->alloc<double>(3 * (size_t)n);
         E->out = E->alloc<double>(GFNFF_NTERMS + 1);
         E->nbn = E->alloc<int>(n);
         E->nbl = E->alloc<int>((size_t)n * MAXNB_CN);
@@ -5699,6 +5739,28 @@ static void queue_batch_factor(DeviceEngine* E, hipStream_t s) {
     }
 }
 
+// k_pairs_sym over the engine's tiles for the selected pair terms
+static void queue_pairs_sym(DeviceEngine* E, hipStream_t s, const PairArgs& pa, double* part, int terms) {
+    const dim3 grid(E->ps_ntiles);
+    if (!E->mixed) {
+        switch (E->nelem) {
+            case 1: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 1>), grid, dim3(PS_B), s, pa, part, E->ps_tiles, terms); break;
+            case 2: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 2>), grid, dim3(PS_B), s, pa, part, E->ps_tiles, terms); break;
+            case 3: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 3>), grid, dim3(PS_B), s, pa, part, E->ps_tiles, terms); break;
+            case 4: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 4>), grid, dim3(PS_B), s, pa, part, E->ps_tiles, terms); break;
+            default: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 0>), grid, dim3(PS_B), s, pa, part, E->ps_tiles, terms); break;
+        }
+    } else {
+        switch (E->nelem) {
+            case 1: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 1>), grid, dim3(PS_B), s, pa, part, E->ps_tiles, terms); break;
+            case 2: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 2>), grid, dim3(PS_B), s, pa, part, E->ps_tiles, terms); break;
+            case 3: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 3>), grid, dim3(PS_B), s, pa, part, E->ps_tiles, terms); break;
+            case 4: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 4>), grid, dim3(PS_B), s, pa, part, E->ps_tiles, terms); break;
+            default: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 0>), grid, dim3(PS_B), s, pa, part, E->ps_tiles, terms); break;
+        }
+    }
+}
+
 static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::string& msg) {
     const int n = E->n;
     hipLaunchKernelGGL(k_copy_start, dim3(cdiv(3 * n, 256)), dim3(256), 0, s, 3 * n, xin, E->x, E->flags);
@@ -5740,6 +5802,12 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         LAUNCH(KC_CN, k_cn, dim3(cdiv(n, 4)), dim3(256), s, n, E->x, E->rcov_cn, E->elem, E->nref, E->refcn,
                E->chi_base, E->cnf, E->cn, E->dlog, E->gw, E->dgw, E->rhs, E->nbn, E->nbl, E->flags, E->mol_of,
                E->mol_off, E->tbox);
+    // D4 half contractions need only CN: evaluated before the solve, and with
+    // pair_split the charge-independent pair terms (dispersion, repulsion) too
+    LAUNCH(KC_PAIRS_AUX, k_d4_half, dim3(cdiv(n * E->nelem, 256)), dim3(256), s, n, E->nelem, E->elem, E->nref,
+           E->c6ref, E->gw, E->dgw, E->h6, E->dh6);
+    const PairArgs pa = E->pair_args();
+    if (E->pair_split) queue_pairs_sym(E, s, pa, E->ps_part2, PT_DISP | PT_REP);
     const double* rsrc = E->rhs;
     if (E->charged) {
         LAUNCH(KC_EEQ_BUILD, k_eeq_rhs_qp, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->alp, E->diag, E->qp,
@@ -5784,11 +5852,18 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     if (E->nbatm)
         LAUNCH(KC_BATM, k_batm, dim3(cdiv(E->nbatm, 256)), dim3(256), s, E->nbatm, E->batm_ijk, E->batm_zb,
                            E->batm_slot, E->x, E->qa, E->slots, E->e_batm);
+    if (E->fin_lpa == 64)
+        LAUNCH(KC_CHAIN, k_slot_gather<64>, dim3(cdiv(n, 4)), dim3(256), s, n, E->slot_off, E->slot_idx, E->slots,
+               E->gslot);
+    else
+        LAUNCH(KC_CHAIN, k_slot_gather<16>, dim3(cdiv(n, 16)), dim3(256), s, n, E->slot_off, E->slot_idx, E->slots,
+               E->gslot);
     // Hydrogen bonds (all kernels and their gather) need only the coordinates
     // and set-up data: a single molecule evaluates them on this stream while
-    // the side stream builds the EEQ matrix; a batch, whose evaluation stream
-    // carries the longer CN and bonded work, appends them to the side stream
-    // after its matrix build.
+    // the side stream builds the EEQ matrix; a batch appends them to the side
+    // stream behind its matrix build and factorisation (run beside the
+    // factorisation, they slowed it, and it is the longer chain), and they
+    // join this stream before k_finish.
     hipStream_t hs = (E->nmol > 1) ? E->side : s;
     if (E->nhb) {
         HBArgs ha{E->hb_cb, E->hb_ca, E->hb_Qh, E->hb_Qab, E->hb_nboff, E->hb_nbidx, E->hb_parc,
@@ -5829,8 +5904,8 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         LAUNCH(KC_HB, k_hb_gather<16>, dim3(cdiv(n, 16)), dim3(256), hs, n, E->hb_crange, E->hb_gout, E->hb_xoff,
                E->hb_xidx, E->hb_xslots, E->ghb);
     HIP_OK(hipEventRecord(E->ev_join, E->side));
-    // batches wait here for the factorisations only; the hydrogen-bond work
-    // still queued behind them on the side stream joins before k_finish
+    // batches wait here for the factorisation only; the hydrogen-bond work
+    // queued behind it on the side stream joins before k_finish
     HIP_OK(hipStreamWaitEvent(s, split ? E->ev_fact : E->ev_join, 0));
     if (E->nmol > 1) {
         // batch: every molecule's system factored (side stream, above) and
@@ -5877,29 +5952,10 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
                        E->frag_off, E->frag_atoms, E->frag, E->qp, E->xbuf, E->chi_base, E->cnf, E->gam, E->alp, E->cn,
                        E->q, E->eself, E->qtmp);
     // pair terms
-    PairArgs pa = E->pair_args();
-    LAUNCH(KC_PAIRS_AUX, k_d4_half, dim3(cdiv(n * E->nelem, 256)), dim3(256), s, n, E->nelem, E->elem, E->nref,
-           E->c6ref, E->gw, E->dgw, E->h6, E->dh6);
     if (E->ps_ntiles > 0) {  // each unordered pair once (single molecules; batches unless batch_pairs = 1)
-        const dim3 grid(E->ps_ntiles);
-        if (!E->mixed) {
-            switch (E->nelem) {
-                case 1: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 1>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
-                case 2: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 2>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
-                case 3: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 3>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
-                case 4: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 4>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
-                default: LAUNCH(KC_PAIRS, (k_pairs_sym<double, 0>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
-            }
-        } else {
-            switch (E->nelem) {
-                case 1: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 1>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
-                case 2: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 2>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
-                case 3: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 3>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
-                case 4: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 4>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
-                default: LAUNCH(KC_PAIRS, (k_pairs_sym<float, 0>), grid, dim3(PS_B), s, pa, E->ps_part, E->ps_tiles); break;
-            }
-        }
+        queue_pairs_sym(E, s, pa, E->ps_part, E->pair_split ? PT_COUL : PT_ALL);
         LAUNCH(KC_PAIRS_AUX, k_pairs_sym_reduce, dim3(E->ps_nblocks), dim3(256), s, n, E->ps_nb, E->ps_part,
+               E->pair_split ? (const double*)E->ps_part2 : nullptr,
                E->gpair, E->dEdcn, E->e_coul, E->e_disp, E->e_rep, E->ps_blocks);
     } else {
         const dim3 grid(cdiv(n, TILE), E->pair_nchunk);
@@ -5932,10 +5988,10 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     if (split) HIP_OK(hipStreamWaitEvent(s, E->ev_join, 0));
     if (E->fin_lpa == 64)
         LAUNCH(KC_CHAIN, k_finish<64>, dim3(cdiv(n, 4)), dim3(256), s, n, E->x, E->rcov_cn, E->dlog, E->vchain,
-                       E->nbn, E->nbl, E->slot_off, E->slot_idx, E->slots, E->gpair, E->ghb, E->grad);
+                       E->nbn, E->nbl, E->gslot, E->gpair, E->ghb, E->grad);
     else
         LAUNCH(KC_CHAIN, k_finish<16>, dim3(cdiv(n, 16)), dim3(256), s, n, E->x, E->rcov_cn, E->dlog, E->vchain,
-                       E->nbn, E->nbl, E->slot_off, E->slot_idx, E->slots, E->gpair, E->ghb, E->grad);
+                       E->nbn, E->nbl, E->gslot, E->gpair, E->ghb, E->grad);
     Segs sg;
     const double* ptrs[NPARTS] = {E->e_bond, E->e_ang,  E->e_tor, E->e_inv, E->e_disp, E->e_brep, E->e_rep, E->e_coul,
                                   E->eself,  E->e_hb,   E->e_atm, E->e_batm, E->e_st,  E->e_xb,   E->e_hb2};

@@ -46,6 +46,7 @@ struct Options {
     int batch_fp32 = 0;               // mixed batches: 1 factors in FP32 (MFMA) with FP64 refinement
     int gather_lanes = 0;             // lanes per atom of the slot and HB gathers: 0 by list length, 16 or 64
     int batch_pairs = 0;              // batches: 0 each pair once in 64x64 tiles per molecule, 1 row kernel
+    int pair_split = 0;               // 1: dispersion + repulsion before the EEQ solve, Coulomb after (tiles)
     int pcg_groups = 0;               // workgroups of the persistent PCG: 0 all that fit (at most 256), else at most this
 };
 

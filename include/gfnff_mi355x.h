@@ -173,7 +173,10 @@ int gfnff_cn(gfnff_handle* h, double* cn);
  * the default);
  * "batch_fp32" 1 = on the mixed path, batches factor in FP32 (FP32 MFMA)
  * and refine in FP64 (charges agree with the FP64 factor to about 1e-12 e;
- * batch_solver 0, 1 and 3); "batch_pairs" 0 = pair terms of batches with each
+ * batch_solver 0, 1 and 3); "pair_split" 1 = the charge-independent pair
+ * terms (dispersion, repulsion) evaluated before the EEQ solve, beside it,
+ * and the Coulomb term after it (pair tiles only; 0 = one pass after the
+ * solve, the default); "batch_pairs" 0 = pair terms of batches with each
  * pair once in 64 x 64 tiles per molecule (default), 1 = one thread per row
  * atom over its molecule; "gather_lanes" 0 (by mean list length), 16 or 64
  * lanes per atom in the slot and hydrogen-bond gathers; "pcg_groups" 0 (all
