@@ -563,49 +563,96 @@ __device__ __forceinline__ float rsqrt_nr(float d) {
 }
 
 // Factorisation L L^T of one NB x NB diagonal block and the inverse of L by
-// one workgroup of 256 threads.  Thread (t, q) = (tid / 4, tid % 4) keeps the
-// entries j = 4u + q of row t in registers.  At step k the owners of column k
-// publish it in LDS (double-buffered, one barrier per step) and every thread
-// updates A(t, j) -= A(t, k) A(j, k) / A(k, k) for j > k from broadcast reads.
-// The inverse X = L^{-1} is formed row by row: thread (c, q) keeps
-// X(4u + q, c) in registers, sums its share of L(r, :) X(:, c), and the four
-// shares are combined with quad shuffles (no barrier).  Both step loops are
-// unrolled so that every register index is static and the LDS reads of a step
-// issue back to back.
+// one workgroup of 256 threads, in steps of four columns, with the working
+// matrix resident in LDS.  Thread (t, q) = (tid / 4, tid % 4) owns the
+// entries (t, 4u + q).  Step b (columns k = 4b .. 4b + 3):
+//  1. every thread reads the 4 x 4 pivot block and factors it redundantly
+//     (four reciprocal square roots), reads its row's four entries and solves
+//     them against the pivot factor: L(t, k .. k + 3);
+//  2. the rows below the block store their L(t, k + q) (only their own quad
+//     read those entries); barrier; the pivot rows store theirs;
+//  3. every thread updates its entries A(t, j) -= sum_m L(t, k + m) L(j, k + m),
+//     k + 3 < j <= t, reading the published columns; barrier.
+// Two barriers per four columns (32 for the block), a rolled step loop (no
+// register arrays, compact code).  The inverse X = L^{-1} is formed four
+// rows at a time: thread (c, q) keeps X(4u + q, c) in registers; for rows
+// 4b .. 4b + 3 the four partial sums L(r, :4b) X(:4b, c) are taken together
+// (independent chains), combined over the quad by shuffles, and the 4 x 4
+// lower-triangular system with the pivot factor is solved (no barrier).
 // The block routine operates on caller-provided LDS so that one workgroup can
-// also factor a whole small matrix (batched path).
+// also factor a whole small matrix (batched path); `col` is not used.
 template <typename T>
 __device__ __forceinline__ void chol_diag_block(int np, int k0, T* __restrict__ A, T* __restrict__ Linv, int* __restrict__ flags,
                                 T (*sL)[NB + 1], T (*col)[NB], T* rdiag, bool write_L = true) {
     const int tid = threadIdx.x;
-    const int t = tid >> 2, q = tid & 3;
+    const int t = tid >> 2, q = tid & 3, qb = tid & ~3 & 63;  // qb: lane of the quad's q = 0
     _Pragma("unroll") for (int u_ = 0, e = tid; u_ < NB * NB / 256; ++u_, e += 256) sL[e / NB][e % NB] = A[(size_t)(k0 + e / NB) * np + k0 + e % NB];
     __syncthreads();
-    T a[NB / 4];
-#pragma unroll
-    for (int u = 0; u < NB / 4; ++u) a[u] = sL[t][4 * u + q];
+    // the working matrix stays in LDS: thread (t, q) owns the entries (t, 4u + q)
     int bad = 0;
+#pragma unroll 1
+    for (int b = 0; b < NB / 4; ++b) {
+        const int k = 4 * b;
+        // pivot factor (lower 4 x 4), redundantly in every thread
+        T p[4][4];
 #pragma unroll
-    for (int k = 0; k < NB; ++k) {
-        const int buf = k & 1;
-        if (q == (k & 3)) col[buf][t] = a[k >> 2];
-        __syncthreads();
-        T d = col[buf][k];
-        if (!(d > T(0))) {
-            bad = 1;
-            d = T(1);
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int jj = 0; jj <= i; ++jj) p[i][jj] = sL[k + i][k + jj];
+        T l[4][4] = {}, il[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+            for (int jj = 0; jj < i; ++jj) {
+                T s = p[i][jj];
+#pragma unroll
+                for (int m = 0; m < jj; ++m) s -= l[i][m] * l[jj][m];
+                l[i][jj] = s * il[jj];
+            }
+            T d = p[i][i];
+#pragma unroll
+            for (int m = 0; m < i; ++m) d -= l[i][m] * l[i][m];
+            if (!(d > T(0))) {
+                bad = 1;
+                d = T(1);
+            }
+            il[i] = rsqrt_nr(d);
+            l[i][i] = d * il[i];
         }
-        const T isd = rsqrt_nr(d);  // 1 / sqrt(d) without a divide on the step's critical path
-        const T sd = d * isd;
-        const T ct = col[buf][t];
-        const T f = ct * (isd * isd);
+        // this row's L(t, k .. k + 3): solved against the pivot factor below the block,
+        // the pivot factor's row inside it
+        T y[4];
 #pragma unroll
-        for (int u = k >> 2; u < NB / 4; ++u)
-            if (4 * u + q > k) a[u] -= f * col[buf][4 * u + q];
-        if (q == (k & 3)) a[k >> 2] = (t == k) ? sd : (t > k ? ct * isd : a[k >> 2]);
+        for (int m = 0; m < 4; ++m) {
+            T sv = sL[t][k + m];
+#pragma unroll
+            for (int jj = 0; jj < m; ++jj) sv -= l[m][jj] * y[jj];
+            y[m] = sv * il[m];
+        }
+        const bool piv_row = t >= k && t < k + 4;
+        if (piv_row) {
+            const int i = t - k;  // static register indices: select among the pivot factor's rows
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const T lm = (i == 0) ? l[0][m] : (i == 1) ? l[1][m] : (i == 2) ? l[2][m] : l[3][m];
+                y[m] = (m <= i) ? lm : T(0);
+            }
+        }
+        // rows below the pivot block publish now (only their own quad read these entries);
+        // the pivot rows after the barrier, once every thread has read the pivot block
+        if (t >= k + 4) sL[t][k + q] = y[q];
+        __syncthreads();
+        if (piv_row) sL[t][k + q] = y[q];
+        // A(t, j) -= sum_m L(t, k + m) L(j, k + m) for this thread's columns j > k + 3
+        if (t >= k + 4) {
+            for (int j = k + 4 + q; j <= t; j += 4)
+                sL[t][j] -= y[0] * sL[j][k] + y[1] * sL[j][k + 1] + y[2] * sL[j][k + 2] + y[3] * sL[j][k + 3];
+        }
+        __syncthreads();
     }
 #pragma unroll
-    for (int u = 0; u < NB / 4; ++u) sL[t][4 * u + q] = (4 * u + q <= t) ? a[u] : T(0);
+    for (int u = 0; u < NB / 4; ++u)
+        if (4 * u + q > t) sL[t][4 * u + q] = T(0);  // the factor's upper triangle
     __syncthreads();
     if (q == 0) rdiag[t] = T(1) / sL[t][t];
     if (write_L) {
@@ -620,14 +667,28 @@ __device__ __forceinline__ void chol_diag_block(int np, int k0, T* __restrict__ 
 #pragma unroll
     for (int u = 0; u < NB / 4; ++u) xr[u] = T(0);
 #pragma unroll
-    for (int r = 0; r < NB; ++r) {
-        T part = 0.0;
+    for (int b = 0; b < NB / 4; ++b) {
+        const int r = 4 * b;
+        T part[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-        for (int u = 0; u <= (r - 1) / 4 && r > 0; ++u) part += sL[r][4 * u + q] * xr[u];
-        part += __shfl_xor(part, 1);
-        part += __shfl_xor(part, 2);
-        const T x = (((r == c) ? T(1) : T(0)) - part) * rdiag[r];
-        if (q == (r & 3)) xr[r >> 2] = x;
+        for (int u = 0; u < b; ++u)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) part[m] += sL[r + m][4 * u + q] * xr[u];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            part[m] += __shfl_xor(part[m], 1);
+            part[m] += __shfl_xor(part[m], 2);
+        }
+        T x[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            T s = ((r + m == c) ? T(1) : T(0)) - part[m];
+#pragma unroll
+            for (int jj = 0; jj < m; ++jj) s -= sL[r + m][r + jj] * x[jj];
+            x[m] = s * rdiag[r + m];
+        }
+        xr[b] = x[q];
+        __builtin_amdgcn_sched_barrier(0);
     }
     T* Li = Linv + (size_t)(k0 / NB) * NB * NB;
 #pragma unroll
@@ -636,7 +697,7 @@ __device__ __forceinline__ void chol_diag_block(int np, int k0, T* __restrict__ 
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_chol_diag(int np, int k0, T* __restrict__ A, T* __restrict__ Linv,
+__global__ __launch_bounds__(256, 3) void k_chol_diag(int np, int k0, T* __restrict__ A, T* __restrict__ Linv,
                                                    int* __restrict__ flags) {
     __shared__ T sL[NB][NB + 1];
     __shared__ T col[2][NB];
@@ -1050,7 +1111,7 @@ __global__ __launch_bounds__(256) void k_trsv_bwd_sweep(int np, const T* __restr
 // of R (one workgroup per block; R is left intact).
 // block-Jacobi factors: L^-1 of every diagonal block of the packed matrix
 // (block I at Rp + (I (I + 1) / 2 + I) 64^2)
-__global__ __launch_bounds__(256) void k_bjacobi_setup(double* __restrict__ Rp, double* __restrict__ Lj,
+__global__ __launch_bounds__(256, 3) void k_bjacobi_setup(double* __restrict__ Rp, double* __restrict__ Lj,
                                                        int* __restrict__ flags) {
     __shared__ double sL[NB][NB + 1];
     __shared__ double col[2][NB];
@@ -3692,7 +3753,7 @@ __device__ __forceinline__ void batch_solve_tail(int np, const T* __restrict__ A
 // substitutions follow in one launch (k_bsolve_subst) on the evaluation
 // stream.
 template <typename T>
-__global__ __launch_bounds__(256) void k_bstep_diag(const int* __restrict__ mols, int k, const int* __restrict__ mnp,
+__global__ __launch_bounds__(256, 3) void k_bstep_diag(const int* __restrict__ mols, int k, const int* __restrict__ mnp,
                                                     const long long* __restrict__ mo, const int* __restrict__ vo,
                                                     T* __restrict__ A, T* __restrict__ Linv, int* __restrict__ flags) {
     __shared__ T S0[NB][NB + 1];
