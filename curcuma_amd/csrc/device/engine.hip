@@ -1674,10 +1674,25 @@ __global__ __launch_bounds__(LU_T) void k_eeq_lu_fallback(int np, int nr, int la
     if (tid == 0) atomicAnd(flags, ~(FLAG_CHOL | FLAG_SOLVE));
 }
 
+// Per fragment f, the sum of the reduced solution over its atoms (the last
+// atom's charge is qp minus it): one wavefront per fragment, lanes striding
+// the fragment's atom list, fixed butterfly.
+__global__ __launch_bounds__(256) void k_frag_sum(int nfrag, const int* __restrict__ frag_off,
+                                                  const int* __restrict__ frag_atoms, const int* __restrict__ red_of,
+                                                  const double* __restrict__ p, double* __restrict__ fsum) {
+    const int f = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (f >= nfrag) return;
+    double s = 0.0;
+    for (int k = frag_off[f] + lane; k < frag_off[f + 1]; k += 64) {
+        const int ra = red_of[frag_atoms[k]];
+        if (ra >= 0) s += p[ra];
+    }
+    s = wave_sum(s);
+    if (lane == 0) fsum[f] = s;
+}
+
 __global__ __launch_bounds__(256) void k_charges(int n, const int* __restrict__ red_of,
-                                                 const int* __restrict__ last_of_frag_first,
-                                                 const int* __restrict__ frag_off,
-                                                 const int* __restrict__ frag_atoms, const int* __restrict__ frag,
+                                                 const double* __restrict__ fsum, const int* __restrict__ frag,
                                                  const double* __restrict__ qp, const double* __restrict__ p,
                                                  const double* __restrict__ chi_base, const double* __restrict__ cnf,
                                                  const double* __restrict__ gam, const double* __restrict__ alp,
@@ -1685,21 +1700,8 @@ __global__ __launch_bounds__(256) void k_charges(int n, const int* __restrict__ 
                                                  double* __restrict__ e_self, double* __restrict__ qtmp) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double qi;
-    int ri = red_of[i];
-    if (ri >= 0) {
-        qi = qp[i] + p[ri];
-    } else {  // last atom of its fragment
-        int f = frag[i];
-        double s = 0.0;
-        for (int k = frag_off[f]; k < frag_off[f + 1]; ++k) {
-            int a = frag_atoms[k];
-            int ra = red_of[a];
-            if (ra >= 0) s += p[ra];
-        }
-        qi = qp[i] - s;
-    }
-    (void)last_of_frag_first;
+    const int ri = red_of[i];
+    const double qi = (ri >= 0) ? qp[i] + p[ri] : qp[i] - fsum[frag[i]];  // ri < 0: last atom of its fragment
     q[i] = qi;
     double c = cn[i];
     double chi = chi_base[i] + cnf[i] * sqrt(fmax(c, 0.0));
@@ -4800,6 +4802,7 @@ struct DeviceEngine {
     int *Z, *elem, *nref, *metal, *frag, *toff, *tidx, *tdist;
     double *chi_base, *gam, *alp, *cnf, *rcov_cn, *rcov_d3, *hb_rad, *hb_lpd, *qa, *zeta, *szr, *anb, *repz, *refcn, *c6ref;
     int *red_atom, *red_last, *red_of, *frag_off, *frag_atoms;
+    double* fsum = nullptr;  // per fragment: the reduced solution summed over its atoms
     int *red_lidx, *red_ul, *red_nul;  // distinct last atoms per 64-block of reduced variables
     double* qp;
     int *bond_ij, *bond_H, *bond_hboff, *bond_hbB, *bond_slot;
@@ -5142,6 +5145,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt
         E->frag_off = E->upload(foff);
         E->frag_atoms = E->upload(fatoms);
         E->qp = E->upload(qp);
+        E->fsum = E->alloc<double>(std::max(P.nfrag, 1));
 
         // term lists and gradient slots
         int slot = 0;
@@ -6009,9 +6013,10 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     if (E->lu_w)  // no work unless the system was found not positive definite
         LAUNCH(KC_CHARGES, k_eeq_lu_fallback, dim3(1), dim3(LU_T), s, np, E->nr, E->use_cg ? 0 : 1,
                E->use_cg ? E->Ared : (E->lu_src ? E->lu_src : E->Ared), E->b, E->lu_w, E->xbuf, E->flags);
-    LAUNCH(KC_CHARGES, k_charges, dim3(cdiv(n, 256)), dim3(256), s, n, E->red_of, (const int*)nullptr,
-                       E->frag_off, E->frag_atoms, E->frag, E->qp, E->xbuf, E->chi_base, E->cnf, E->gam, E->alp, E->cn,
-                       E->q, E->eself, E->qtmp);
+    LAUNCH(KC_CHARGES, k_frag_sum, dim3(cdiv(E->nfrag, 4)), dim3(256), s, E->nfrag, E->frag_off, E->frag_atoms,
+           E->red_of, E->xbuf, E->fsum);
+    LAUNCH(KC_CHARGES, k_charges, dim3(cdiv(n, 256)), dim3(256), s, n, E->red_of, E->fsum, E->frag, E->qp, E->xbuf,
+           E->chi_base, E->cnf, E->gam, E->alp, E->cn, E->q, E->eself, E->qtmp);
     // pair terms
     if (E->ps_ntiles > 0) {  // each unordered pair once (single molecules; batches unless batch_pairs = 1)
         queue_pairs_sym(E, s, pa, E->ps_part, E->pair_split ? PT_COUL : PT_ALL);
