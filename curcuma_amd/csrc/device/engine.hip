@@ -5867,12 +5867,6 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         LAUNCH(KC_CN, k_cn, dim3(cdiv(n, 4)), dim3(256), s, n, E->x, E->rcov_cn, E->elem, E->nref, E->refcn,
                E->chi_base, E->cnf, E->cn, E->dlog, E->gw, E->dgw, E->rhs, E->nbn, E->nbl, E->flags, E->mol_of,
                E->mol_off, E->tbox);
-    // D4 half contractions need only CN: evaluated before the solve, and with
-    // pair_split the charge-independent pair terms (dispersion, repulsion) too
-    LAUNCH(KC_PAIRS_AUX, k_d4_half, dim3(cdiv(n * E->nelem, 256)), dim3(256), s, n, E->nelem, E->elem, E->nref,
-           E->c6ref, E->gw, E->dgw, E->h6, E->dh6);
-    const PairArgs pa = E->pair_args();
-    if (E->pair_split) queue_pairs_sym(E, s, pa, E->ps_part2, PT_DISP | PT_REP);
     const double* rsrc = E->rhs;
     if (E->charged) {
         LAUNCH(KC_EEQ_BUILD, k_eeq_rhs_qp, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->alp, E->diag, E->qp,
@@ -5881,6 +5875,10 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     }
     LAUNCH(KC_EEQ_BUILD, k_eeq_rhs_reduce, dim3(cdiv(np, 256)), dim3(256), s, E->nr, np, E->red_atom, E->red_last,
                        rsrc, E->b);
+    // D4 half contractions need only CN: evaluated before the solve
+    LAUNCH(KC_PAIRS_AUX, k_d4_half, dim3(cdiv(n * E->nelem, 256)), dim3(256), s, n, E->nelem, E->elem, E->nref,
+           E->c6ref, E->gw, E->dgw, E->h6, E->dh6);
+    const PairArgs pa = E->pair_args();
     // bonded lists
     if (E->nbond) {
         if (E->nb_hb)
@@ -5923,6 +5921,9 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     else
         LAUNCH(KC_CHAIN, k_slot_gather<16>, dim3(cdiv(n, 16)), dim3(256), s, n, E->slot_off, E->slot_idx, E->slots,
                E->gslot);
+    // pair_split: the charge-independent pair terms (dispersion, repulsion)
+    // last before the solve, beside the factorisation's smaller column blocks
+    if (E->pair_split) queue_pairs_sym(E, s, pa, E->ps_part2, PT_DISP | PT_REP);
     // Hydrogen bonds (all kernels and their gather) need only the coordinates
     // and set-up data: a single molecule evaluates them on this stream while
     // the side stream builds the EEQ matrix; a batch appends them to the side
