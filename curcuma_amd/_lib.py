@@ -10,7 +10,7 @@ LIB_PATH = Path(__file__).resolve().parent / "lib" / "libgfnff_mi355x.so"
 
 NTERMS = 15
 TERM_NAMES = ["bond", "angle", "torsion", "inversion", "dispersion", "bonded_rep", "nonbonded_rep", "coulomb",
-              "hbond", "xbond", "atm", "batm", "stors", "reserved13", "reserved14"]
+              "hbond", "xbond", "atm", "batm", "stors", "solvation", "reserved14"]
 
 STATUS = {0: "ok", 1: "args", 2: "unsupported", 3: "eeq", 4: "geometry", 5: "device", 6: "nan"}
 
@@ -49,6 +49,15 @@ class ParamsView(C.Structure):
 _LIB = None
 
 
+class AlpbView(C.Structure):
+    """Mirror of gfnff_alpb_view (include/gfnff_params.h)."""
+
+    _fields_ = [("natoms", C.c_int32), ("hbond", C.c_int32), ("nang", C.c_int32), ("keps", C.c_double),
+                ("alpbet", C.c_double), ("born_scale", C.c_double), ("gshift", C.c_double), ("lrcut", C.c_double),
+                ("srcut", C.c_double)] + [(k, _f64p) for k in ("vdwr", "rho", "svdw", "vdwsa", "wrp", "trj_lo",
+                                                                 "trj_hi", "gamsasa", "hbmag", "grid")]
+
+
 def load_library(path: Path | None = None) -> C.CDLL:
     """Load the engine library; raises if it has not been built."""
     global _LIB
@@ -80,6 +89,9 @@ def load_library(path: Path | None = None) -> C.CDLL:
     lib.gfnff_last_error.argtypes = [C.c_void_p]
     lib.gfnff_last_error.restype = C.c_char_p
     lib.gfnff_get_params.argtypes = [C.c_void_p, C.POINTER(ParamsView)]
+    lib.gfnff_set_solvent.argtypes = [C.c_void_p, C.c_char_p]
+    lib.gfnff_solvation_energy.argtypes = [C.c_void_p, _f64p]
+    lib.gfnff_get_solvation.argtypes = [C.c_void_p, C.POINTER(AlpbView)]
     lib.gfnff_get_topology.argtypes = [C.c_void_p, _i32p, _i32p, _i32p, C.c_int32, _i32p]
     lib.gfnff_device_count.restype = C.c_int
     lib.gfnff_solver_iterations.argtypes = [C.c_void_p]

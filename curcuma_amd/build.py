@@ -31,7 +31,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-munsafe-fp-atomics"]
 
-HOST_SOURCES = ["capi.cpp", "host/gfnff_topology.cpp"]
+HOST_SOURCES = ["capi.cpp", "host/gfnff_topology.cpp", "host/alpb.cpp"]
 DEVICE_SOURCES = ["device/engine.hip"]
 
 

@@ -35,6 +35,9 @@ struct gfnff_handle {
     // geometry (Bohr) at which the hydrogen/halogen-bond lists were last
     // detected during evaluation; empty until the first evaluation
     std::vector<double> hb_ref;
+    // ALPB solvent (canonical name; empty: gas phase), applied to every
+    // molecule set afterwards
+    std::string solvent;
 };
 
 namespace {
@@ -74,6 +77,10 @@ int gfnff_set_molecule(gfnff_handle* h, int32_t natoms, const int32_t* Z, const 
         int st = gfnff::build_params(natoms, Z, h->xyz_bohr.data(), charge, forced_bonds, nbonds, h->opt, h->params,
                                      msg);
         if (st != gfnff::kOk) return fail(h, st, "gfnff_set_molecule: " + msg);
+        if (!h->solvent.empty()) {
+            st = gfnff::alpb_setup(h->solvent.c_str(), natoms, Z, h->params.solv, msg);
+            if (st != gfnff::kOk) return fail(h, st, "gfnff_set_molecule: " + msg);
+        }
         h->have_molecule = true;
         h->dev_geom_dirty = true;
         h->hb_ref.clear();
@@ -99,6 +106,8 @@ int gfnff_set_batch(gfnff_handle* h, int32_t nmol, const int32_t* atom_off, cons
         return fail(h, GFNFF_STATUS_ARGS, "gfnff_set_batch: invalid arguments");
     for (int m = 0; m < nmol; ++m)
         if (atom_off[m + 1] <= atom_off[m]) return fail(h, GFNFF_STATUS_ARGS, "gfnff_set_batch: empty molecule");
+    if (!h->solvent.empty())
+        return fail(h, GFNFF_STATUS_UNSUPPORTED, "gfnff_set_batch: ALPB solvation is built for single molecules");
     try {
         h->failed = false;
         h->err.clear();
@@ -398,6 +407,53 @@ int gfnff_set_precision(gfnff_handle* h, int32_t mode) {
         h->dev.reset();
     }
     return GFNFF_STATUS_OK;
+}
+
+int gfnff_set_solvent(gfnff_handle* h, const char* solvent) {
+    if (!h) return GFNFF_STATUS_ARGS;
+    try {
+        const std::string req = solvent ? solvent : "";
+        std::string name;
+        if (!req.empty() && req != "none") {
+            name = gfnff::alpb_solvent_name(solvent);
+            if (name.empty()) {
+                // the reference continues in the gas phase after an unknown
+                // solvent (gfnff_method.cpp:826-829); so does the handle
+                h->solvent.clear();
+                h->params.solv = gfnff::AlpbSet();
+                h->dev.reset();
+                return fail(h, GFNFF_STATUS_ARGS, "gfnff_set_solvent: unknown solvent '" + req + "'");
+            }
+        }
+        if (!name.empty() && h->have_molecule && h->params.nmol != 1)
+            return fail(h, GFNFF_STATUS_UNSUPPORTED, "gfnff_set_solvent: ALPB solvation is built for single molecules");
+        h->solvent = name;
+        h->params.solv = gfnff::AlpbSet();
+        if (!name.empty() && h->have_molecule) {
+            std::string msg;
+            int st = gfnff::alpb_setup(name.c_str(), h->params.n, h->params.Z.data(), h->params.solv, msg);
+            if (st != gfnff::kOk) return fail(h, st, "gfnff_set_solvent: " + msg);
+        }
+        h->dev.reset();
+        return GFNFF_STATUS_OK;
+    } catch (const std::exception& e) {
+        return fail(h, GFNFF_STATUS_ARGS, std::string("gfnff_set_solvent: ") + e.what());
+    }
+}
+
+int gfnff_get_solvation(const gfnff_handle* h, gfnff_alpb_view* out) {
+    if (!h || !out || !h->have_molecule) return GFNFF_STATUS_ARGS;
+    *out = h->params.solv.view();
+    return GFNFF_STATUS_OK;
+}
+
+int gfnff_solvation_energy(gfnff_handle* h, double* parts) {
+    if (!h || !parts) return GFNFF_STATUS_ARGS;
+    if (!h->have_molecule || !h->dev || h->params.solv.n == 0)
+        return fail(h, GFNFF_STATUS_ARGS, "gfnff_solvation_energy: no solvated evaluation");
+    std::string msg;
+    int st = gfnff::device_solvation_parts(h->dev.get(), parts, msg);
+    return st ? fail(h, st, "gfnff_solvation_energy: " + msg) : GFNFF_STATUS_OK;
 }
 
 int gfnff_has_error(const gfnff_handle* h) { return h ? (h->failed ? 1 : 0) : 1; }

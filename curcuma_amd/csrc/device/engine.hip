@@ -4500,7 +4500,7 @@ struct Segs {
     int dst[16];
 };
 constexpr int RED_BLOCKS = 64;
-constexpr int NPARTS = 15;  // energy segments reduced per evaluation
+constexpr int NPARTS = 16;  // energy segments reduced per evaluation
 // stage 1: segment sg, block b reduces the fixed chunk b of the segment
 __global__ __launch_bounds__(256) void k_reduce1(Segs s, double* __restrict__ part) {
     __shared__ double red[256];
@@ -4546,7 +4546,7 @@ __global__ __launch_bounds__(NPARTS * 64) void k_reduce_final(Segs s, const doub
     double t[GFNFF_NTERMS];
     for (int k = 0; k < GFNFF_NTERMS; ++k) t[k] = 0.0;
     // parts: 0 bond,1 angle,2 tors,3 inv,4 disp(pairs),5 brep,6 nbrep(pairs incl fix),7 coul pairs,
-    // 8 self,9 hb case 1,10 atm,11 batm,12 triple-bond torsions,13 xb,14 hb cases 2-4
+    // 8 self,9 hb case 1,10 atm,11 batm,12 triple-bond torsions,13 xb,14 hb cases 2-4,15 solvation
     t[GFNFF_E_BOND] = sp[0];
     t[GFNFF_E_ANGLE] = sp[1];
     t[GFNFF_E_TORSION] = sp[2];
@@ -4560,6 +4560,7 @@ __global__ __launch_bounds__(NPARTS * 64) void k_reduce_final(Segs s, const doub
     t[GFNFF_E_BATM] = sp[11];
     t[GFNFF_E_STORS] = sp[12];
     t[GFNFF_E_XBOND] = sp[13];
+    t[GFNFF_E_SOLVATION] = sp[15];
     double tot = 0.0;
     for (int k = 0; k < GFNFF_NTERMS; ++k) tot += t[k];
     // a status flag raised in this evaluation (CN list overflow, EEQ pivot,
@@ -4680,6 +4681,437 @@ __global__ __launch_bounds__(MD_T) void k_md_scale(int n3, const double* __restr
     if (k < n3) v[k] *= *lambda;
 }
 
+// ---------------------------------------------------------------------------
+// K13: ALPB implicit solvation of one molecule, FP64 on both precision
+// paths.  Born radii from the OBC-II transform of the pairwise descreening
+// sum (alpb_solvation.cpp:502-690), surface areas on the 230-point angular
+// grid with the smooth neighbour switching (:696-801), the H-bond diagonal
+// (:873-881) and the shape descriptor (:908-938).  The Born matrix
+// (P16 kernel, :810-867) is never stored: its entries are formed where they
+// are used, added to the reduced EEQ matrix and right-hand side (the
+// reference's A + B, eeq_solver.cpp:3774-3784), and contracted with the
+// solved charges for the energy and the gradient at fixed charges
+// (:1016-1041, :1139-1201, :1265-1300).  Every sum runs in a fixed order.
+// ---------------------------------------------------------------------------
+constexpr int SOLV_NB = 512;     // surface neighbours per atom (LDS list)
+constexpr int SOLV_NANG = 230;   // angular grid points per atom
+constexpr int FLAG_SOLV_NB = 16; // more than SOLV_NB surface neighbours
+constexpr double ALPB_W = 0.3 / 0.52917726;
+constexpr double ALPB_AH0 = 0.5;
+constexpr double ALPB_AH1 = 3.0 / (4.0 * ALPB_W);
+constexpr double ALPB_AH3 = -1.0 / (4.0 * ALPB_W * ALPB_W * ALPB_W);
+constexpr double ALPB_ZETA = 1.028;
+constexpr double ALPB_TOLSESP = 1.0e-6;
+
+struct SolvArgs {
+    int n;
+    double keps, alpbet, c1, gshift, lr2, sr2;
+    const double* x;
+    const double *vdwr, *rho, *svdw, *vdwsa, *wrp, *trjlo, *trjhi, *gam, *hbmag, *grid;
+    double *brad, *dbr, *sasa, *hbw;  // Born radius, d radius / d psi, surface area, H-bond diagonal
+    double* sv;                       // [n][SOLV_NB][3] d sasa_i / d x_i per neighbour slot
+    int *sj, *sn;                     // [n][SOLV_NB] neighbour of each slot (ascending), [n] count
+    double* shape;                    // adet, totrad3, centre[3], inertia[9], gradient scale, cofactors[9]
+    double *w, *c, *eat, *gs;         // radius weight, surface weight, atom energy, direct gradient [3n]
+    double *e_solv, *parts;           // total, {gborn, ghb, gsasa, gshift}
+    int* flags;
+};
+
+// One descreening contribution to psi of atom s from atom o at distance r
+// (rho_o: descreening radius of o, rv: van der Waals radius of s), the
+// non-overlapping or overlapping integral; dg = (d g / d r) / r.
+__device__ inline double alpb_psi(double r, double rho_o, double rv, double& dg) {
+    if (r >= rv + rho_o) {
+        const double r1 = 1.0 / r, ap = r + rho_o, am = r - rho_o, ab = ap * am, rhab = rho_o / ab;
+        const double lnab = 0.5 * log(am / ap) * r1;
+        dg = -2.0 * rhab / ab + (rhab - lnab) * r1 * r1;
+        return rhab + lnab;
+    }
+    if (r + rho_o <= rv) {
+        dg = 0.0;
+        return 0.0;
+    }
+    const double r1 = 1.0 / r, r12 = 0.5 * r1, r24 = r12 * r12, ap = r + rho_o, am = r - rho_o;
+    const double rh1 = 1.0 / rv, rhr1 = 1.0 / ap, aprh1 = ap * rh1, lnab = log(aprh1);
+    dg = (rhr1 * rhr1 * (1.0 - 0.25 * am * r1 * (1.0 + aprh1 * aprh1)) + rho_o * r24 * (rhr1 - rh1 * aprh1) +
+          r12 * (r1 * lnab - rhr1)) *
+         r1;
+    return rh1 - rhr1 + r12 * (0.5 * am * (rhr1 - rh1 * aprh1) - lnab);
+}
+
+// P16 interaction a16 = (ab / (ab + zeta r / 16))^16; returns a1 = the base
+__device__ inline double p16_pow(double ab, double r, double& a16) {
+    const double a1 = ab / (ab + (ALPB_ZETA / 16.0) * r);
+    double t = a1 * a1;
+    t = t * t;
+    t = t * t;
+    a16 = t * t;
+    return a1;
+}
+
+// Born matrix entry without the shape constant (which cancels in the
+// reduced EEQ system: every reduced row and column sums entries with total
+// weight zero)
+__device__ inline double born_b(const SolvArgs& a, int i, int j) {
+    if (i == j) return a.keps / a.brad[i] + 2.0 * a.hbw[i];
+    const double3 d = ld3(a.x, i) - ld3(a.x, j);
+    const double r = sqrt(dot(d, d)), ab = sqrt(a.brad[i] * a.brad[j]);
+    double a16;
+    (void)p16_pow(ab, r, a16);
+    return a.keps / (r + ab * a16);
+}
+
+// Born radii: one wavefront per atom over all partners within lrcut
+__global__ __launch_bounds__(256) void k_solv_born(SolvArgs a) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+    if (i >= a.n) return;  // wavefront-uniform
+    const double3 xi = ld3(a.x, i);
+    const double rvi = a.vdwr[i];
+    double psi = 0.0;
+    for (int j = l; j < a.n; j += 64) {
+        if (j == i) continue;
+        const double3 d = xi - ld3(a.x, j);
+        const double r2 = dot(d, d);
+        if (r2 >= a.lr2) continue;
+        double dg;
+        psi += alpb_psi(sqrt(r2), a.rho[j], rvi, dg);
+    }
+    psi = wave_sum(psi);
+    if (l != 0) return;
+    // OBC-II: R = c1 / (1/svdw - tanh(arg) / vdwr), arg a cubic in psi svdw / 2
+    const double s1 = 1.0 / a.svdw[i], v1 = 1.0 / rvi, s2 = 0.5 * a.svdw[i];
+    const double br = psi * s2;
+    double arg2 = br * (4.85 * br - 0.8);
+    const double arg = br * (1.0 + arg2);
+    arg2 = 2.0 * arg2 + 1.0 + 4.85 * br * br;
+    const double th = tanh(arg), ch = cosh(arg);
+    a.brad[i] = a.c1 / (s1 - v1 * th);
+    const double dp = ch * (s1 - v1 * th);
+    a.dbr[i] = a.c1 * s2 * v1 * arg2 / (dp * dp);
+}
+
+// Surface area of atom i (one workgroup): the neighbours within srcut are
+// compacted into LDS in ascending order; one thread per grid point forms
+// the switched product; one thread per neighbour then sums its derivative
+// over the grid points (no atomics).
+__global__ __launch_bounds__(256) void k_solv_sasa(SolvArgs a) {
+    __shared__ double3 nx[SOLV_NB];
+    __shared__ double nlo[SOLV_NB], nhi[SOLV_NB], nsa[SOLV_NB];
+    __shared__ int nj[SOLV_NB];
+    __shared__ double3 px[256];
+    __shared__ double pw[256];
+    __shared__ int wcnt[4];
+    __shared__ double red[4];
+    const int i = blockIdx.x, tid = threadIdx.x, l = tid & 63, wv = tid >> 6;
+    const double3 xi = ld3(a.x, i);
+    int cnt = 0;
+    for (int j0 = 0; j0 < a.n; j0 += 256) {
+        const int j = j0 + tid;
+        bool in = false;
+        if (j < a.n && j != i) {
+            const double3 d = xi - ld3(a.x, j);
+            const double r2 = dot(d, d);
+            in = r2 < a.lr2 && r2 < a.sr2;
+        }
+        const unsigned long long b = __ballot(in);
+        const int pre = __popcll(b & ((1ull << l) - 1ull));
+        if (l == 0) wcnt[wv] = __popcll(b);
+        __syncthreads();
+        int off = cnt;
+        for (int w = 0; w < wv; ++w) off += wcnt[w];
+        const int tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        if (in && off + pre < SOLV_NB) {
+            const int s = off + pre;
+            nx[s] = ld3(a.x, j);
+            nlo[s] = a.trjlo[j];
+            nhi[s] = a.trjhi[j];
+            nsa[s] = a.vdwsa[j];
+            nj[s] = j;
+        }
+        cnt += tot;
+        __syncthreads();
+    }
+    if (cnt > SOLV_NB) {
+        if (tid == 0) atomicOr(a.flags, FLAG_SOLV_NB);
+        cnt = SOLV_NB;
+    }
+    const double rsa = a.vdwsa[i];
+    double wsa = 0.0;
+    double3 xp = xi;
+    if (tid < SOLV_NANG) {
+        const double* g = a.grid + 4 * tid;
+        xp = make_double3(xi.x + rsa * g[0], xi.y + rsa * g[1], xi.z + rsa * g[2]);
+        double sp = 1.0;
+        for (int s = 0; s < cnt; ++s) {
+            const double3 t = xp - nx[s];
+            const double t2 = dot(t, t);
+            if (t2 < nhi[s]) {
+                if (t2 <= nlo[s]) {
+                    sp = 0.0;
+                    break;
+                }
+                const double u = sqrt(t2) - nsa[s], ah3u2 = ALPB_AH3 * u * u;
+                sp *= ALPB_AH0 + (ALPB_AH1 + ah3u2) * u;
+            }
+        }
+        if (sp > ALPB_TOLSESP) wsa = g[3] * a.wrp[i] * sp;
+    }
+    px[tid] = xp;
+    pw[tid] = wsa;
+    const double sasa = wg_sum256(wsa, red);  // its barriers also publish px and pw
+    for (int s = tid; s < cnt; s += 256) {
+        const double3 xj = nx[s];
+        const double hi = nhi[s], sa = nsa[s];
+        double vx = 0.0, vy = 0.0, vz = 0.0;
+        for (int p = 0; p < SOLV_NANG; ++p) {
+            const double w = pw[p];
+            if (w == 0.0) continue;
+            const double3 t = px[p] - xj;
+            const double t2 = dot(t, t);
+            if (t2 < hi) {
+                const double sq = sqrt(t2), u = sq - sa, ah3u2 = ALPB_AH3 * u * u;
+                const double f = ALPB_AH0 + (ALPB_AH1 + ah3u2) * u;
+                const double df = w * (ALPB_AH1 + 3.0 * ah3u2) / (f * sq);
+                vx += df * t.x;
+                vy += df * t.y;
+                vz += df * t.z;
+            }
+        }
+        const size_t o = (size_t)i * SOLV_NB + s;
+        a.sv[3 * o] = vx;
+        a.sv[3 * o + 1] = vy;
+        a.sv[3 * o + 2] = vz;
+        a.sj[o] = nj[s];
+    }
+    if (tid == 0) {
+        a.sasa[i] = sasa;
+        a.hbw[i] = a.hbmag[i] * sasa / (rsa * rsa);
+        a.sn[i] = cnt;
+    }
+}
+
+// Shape descriptor (one workgroup): the inertia tensor of spheres with the
+// van der Waals radii about their volume-weighted centre, and
+// adet = sqrt(det^(1/3) / (2/5 sum r^3)).
+__global__ __launch_bounds__(256) void k_solv_shape(SolvArgs a) {
+    __shared__ double red[4];
+    const int tid = threadIdx.x;
+    double tr = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+    for (int i = tid; i < a.n; i += 256) {
+        const double r3 = a.vdwr[i] * a.vdwr[i] * a.vdwr[i];
+        tr += r3;
+        cx += a.x[3 * i] * r3;
+        cy += a.x[3 * i + 1] * r3;
+        cz += a.x[3 * i + 2] * r3;
+    }
+    tr = wg_sum256(tr, red);
+    cx = wg_sum256(cx, red) / tr;
+    cy = wg_sum256(cy, red) / tr;
+    cz = wg_sum256(cz, red) / tr;
+    double I[6] = {0, 0, 0, 0, 0, 0};  // xx, yy, zz, xy, xz, yz
+    for (int i = tid; i < a.n; i += 256) {
+        const double r2v = a.vdwr[i] * a.vdwr[i], r3 = r2v * a.vdwr[i];
+        const double vx = a.x[3 * i] - cx, vy = a.x[3 * i + 1] - cy, vz = a.x[3 * i + 2] - cz;
+        const double d = vx * vx + vy * vy + vz * vz + 0.4 * r2v;
+        I[0] += r3 * (d - vx * vx);
+        I[1] += r3 * (d - vy * vy);
+        I[2] += r3 * (d - vz * vz);
+        I[3] -= r3 * vx * vy;
+        I[4] -= r3 * vx * vz;
+        I[5] -= r3 * vy * vz;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) I[k] = wg_sum256(I[k], red);
+    if (tid != 0) return;
+    const double xx = I[0], yy = I[1], zz = I[2], xy = I[3], xz = I[4], yz = I[5];
+    const double det = xx * (yy * zz - yz * yz) - xy * (xy * zz - yz * xz) + xz * (xy * yz - yy * xz);
+    double* s = a.shape;
+    s[0] = sqrt(cbrt(det) / (0.4 * tr));
+    s[1] = tr;
+    s[2] = cx;
+    s[3] = cy;
+    s[4] = cz;
+    const double full[9] = {xx, xy, xz, xy, yy, yz, xz, yz, zz};
+    for (int k = 0; k < 9; ++k) s[5 + k] = full[k];
+}
+
+// Reduced Born matrix added to the reduced EEQ matrix, in the storage of
+// k_eeq_reduced (full == 0: lower triangle of the row-major matrix, 1: also
+// the upper triangle, 2: packed tiles with full diagonal tiles)
+__global__ __launch_bounds__(256) void k_solv_red_add(int nr, int np, const int* __restrict__ ra,
+                                                      const int* __restrict__ rl, SolvArgs a, double* __restrict__ A,
+                                                      int full) {
+    const int t = blockIdx.x;
+    int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+    while (bi * (bi + 1) / 2 > t) --bi;
+    const int bj = t - bi * (bi + 1) / 2;
+    for (int e = threadIdx.x; e < NB * NB; e += 256) {
+        const int rr = e >> 6, cc = e & 63, r = bi * NB + rr, c = bj * NB + cc;
+        if (c > r || r >= nr || c >= nr) continue;
+        const int ia = ra[r], la = rl[r], ib = ra[c], lb = rl[c];
+        const double v = born_b(a, ia, ib) - born_b(a, ia, lb) - born_b(a, la, ib) + born_b(a, la, lb);
+        if (full == 2) {
+            A[(size_t)t * NB * NB + rr * NB + cc] += v;
+            if (bi == bj && cc != rr) A[(size_t)t * NB * NB + cc * NB + rr] += v;
+        } else {
+            A[(size_t)r * np + c] += v;
+            if (full == 1 && c != r) A[(size_t)c * np + r] += v;
+        }
+    }
+}
+
+// Charged fragments: the Born part of A qp leaves the right-hand side
+__global__ __launch_bounds__(256) void k_solv_rhs(SolvArgs a, const double* __restrict__ qp, double* __restrict__ r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double s = 0.0;
+    for (int j = 0; j < a.n; ++j)
+        if (qp[j] != 0.0) s += born_b(a, i, j) * qp[j];
+    r[i] -= s;
+}
+
+// Born pair terms at the solved charges (one wavefront per atom i over all
+// partners): atom energy, direct gradient, and d E / d R_i folded with
+// d R_i / d psi_i; the surface weight gamma_i + d(H-bond)/d sasa_i.
+__global__ __launch_bounds__(256) void k_solv_pairs(SolvArgs a, const double* __restrict__ q) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+    if (i >= a.n) return;
+    const double3 xi = ld3(a.x, i);
+    const double Ri = a.brad[i], qi = q[i];
+    double e = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, dr = 0.0;
+    for (int j = l; j < a.n; j += 64) {
+        if (j == i) continue;
+        const double3 d = xi - ld3(a.x, j);
+        const double r = sqrt(dot(d, d)), Rj = a.brad[j], ab = sqrt(Ri * Rj), qj = q[j];
+        double a16;
+        const double a1 = p16_pow(ab, r, a16);
+        const double df = 1.0 / (r + ab * a16), df2 = df * df;
+        e += qj * df;
+        const double ap = (1.0 - ALPB_ZETA * a1 * a16) * df2 * qj / r;
+        gx -= ap * d.x;
+        gy -= ap * d.y;
+        gz -= ap * d.z;
+        dr += Rj * (-0.5 * (r * ALPB_ZETA / ab * a1 + 1.0) / ab * a16 * df2) * qj;
+    }
+    e = wave_sum(e);
+    gx = wave_sum(gx);
+    gy = wave_sum(gy);
+    gz = wave_sum(gz);
+    dr = wave_sum(dr);
+    if (l != 0) return;
+    const double kq = a.keps * qi;
+    a.eat[i] = 0.5 * kq * e + 0.5 * qi * qi * (a.keps / Ri + 2.0 * a.hbw[i]);
+    a.gs[3 * i] = kq * gx;
+    a.gs[3 * i + 1] = kq * gy;
+    a.gs[3 * i + 2] = kq * gz;
+    a.w[i] = (kq * dr - 0.5 * kq * qi / (Ri * Ri)) * a.dbr[i];
+    a.c[i] = a.gam[i] + a.hbmag[i] / (a.vdwsa[i] * a.vdwsa[i]) * qi * qi;
+}
+
+// Solvation energy (one workgroup, fixed order) and the shape gradient's scale
+__global__ __launch_bounds__(256) void k_solv_energy(SolvArgs a, const double* __restrict__ q) {
+    __shared__ double red[4];
+    double eb = 0.0, ehb = 0.0, es = 0.0, qt = 0.0;
+    for (int i = threadIdx.x; i < a.n; i += 256) {
+        eb += a.eat[i];
+        ehb += a.hbw[i] * q[i] * q[i];
+        es += a.gam[i] * a.sasa[i];
+        qt += q[i];
+    }
+    eb = wg_sum256(eb, red);
+    ehb = wg_sum256(ehb, red);
+    es = wg_sum256(es, red);
+    qt = wg_sum256(qt, red);
+    if (threadIdx.x != 0) return;
+    double* s = a.shape;
+    const double ad = s[0], tr = s[1];
+    double esh = 0.0;
+    s[14] = 0.0;
+    if (a.alpbet > 0.0) {
+        esh = 0.5 * a.keps * a.alpbet * qt * qt / ad;
+        s[14] = (250.0 / (48.0 * tr * tr * tr * pow(ad, 5.0))) * (-0.5 * a.keps * a.alpbet * qt * qt / (ad * ad));
+    }
+    // cofactor-type matrix of the inertia tensor (gbsa.f90:769-779)
+    const double* I = s + 5;
+    double* D = s + 15;
+    D[0] = I[0] * (I[4] + I[8]) - I[1] * I[1] - I[2] * I[2];
+    D[1] = I[1] * I[8] - I[2] * I[5];
+    D[2] = I[2] * I[4] - I[1] * I[7];
+    D[3] = D[1];
+    D[4] = I[4] * (I[0] + I[8]) - I[1] * I[1] - I[5] * I[5];
+    D[5] = I[0] * I[5] - I[1] * I[2];
+    D[6] = D[2];
+    D[7] = D[5];
+    D[8] = I[8] * (I[0] + I[4]) - I[2] * I[2] - I[5] * I[5];
+    a.parts[0] = eb - ehb + esh;
+    a.parts[1] = ehb;
+    a.parts[2] = es;
+    a.parts[3] = a.gshift;
+    a.e_solv[0] = ((eb + esh) + es) + a.gshift;
+}
+
+// Solvation gradient of atom k (one wavefront): the Born-radius chain rule
+// over all partners within lrcut, the surface terms of k's own sphere and of
+// every neighbour's sphere (k's slot in the neighbour's ascending list by
+// bisection), the direct pair gradient and the shape term; added to grad.
+__global__ __launch_bounds__(256) void k_solv_grad(SolvArgs a, double* __restrict__ grad) {
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+    if (k >= a.n) return;
+    const double3 xk = ld3(a.x, k);
+    const double wk = a.w[k], rvk = a.vdwr[k], rhok = a.rho[k], ck = a.c[k];
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    for (int j = l; j < a.n; j += 64) {
+        if (j == k) continue;
+        const double3 d = xk - ld3(a.x, j);
+        const double r2 = dot(d, d);
+        if (r2 >= a.lr2) continue;
+        const double r = sqrt(r2);
+        double d1, d2;
+        (void)alpb_psi(r, a.rho[j], rvk, d1);
+        (void)alpb_psi(r, rhok, a.vdwr[j], d2);
+        const double f = wk * d1 + a.w[j] * d2;
+        gx += f * d.x;
+        gy += f * d.y;
+        gz += f * d.z;
+    }
+    const int nk = a.sn[k];
+    for (int s = l; s < nk; s += 64) {
+        const size_t o = (size_t)k * SOLV_NB + s;
+        gx += ck * a.sv[3 * o];
+        gy += ck * a.sv[3 * o + 1];
+        gz += ck * a.sv[3 * o + 2];
+        const int j = a.sj[o];
+        const int* lj = a.sj + (size_t)j * SOLV_NB;
+        int lo = 0, hi = a.sn[j];
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (lj[mid] < k)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        if (lo < a.sn[j] && lj[lo] == k) {
+            const size_t oj = (size_t)j * SOLV_NB + lo;
+            const double cj = a.c[j];
+            gx -= cj * a.sv[3 * oj];
+            gy -= cj * a.sv[3 * oj + 1];
+            gz -= cj * a.sv[3 * oj + 2];
+        }
+    }
+    gx = wave_sum(gx);
+    gy = wave_sum(gy);
+    gz = wave_sum(gz);
+    if (l != 0) return;
+    const double* s = a.shape;
+    const double r3 = rvk * rvk * rvk * s[14];
+    const double vx = xk.x - s[2], vy = xk.y - s[3], vz = xk.z - s[4];
+    const double* D = s + 15;
+    grad[3 * k] += (gx + a.gs[3 * k]) + r3 * (D[0] * vx + D[1] * vy + D[2] * vz);
+    grad[3 * k + 1] += (gy + a.gs[3 * k + 1]) + r3 * (D[3] * vx + D[4] * vy + D[5] * vz);
+    grad[3 * k + 2] += (gz + a.gs[3 * k + 2]) + r3 * (D[6] * vx + D[7] * vy + D[8] * vz);
+}
+
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
 }  // namespace
@@ -4691,12 +5123,12 @@ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 enum KernelClass {
     KC_CN, KC_EEQ_BUILD, KC_CHOL_DIAG, KC_CHOL_PANEL, KC_CHOL_UPDATE, KC_TRSV, KC_CHARGES, KC_PAIRS, KC_PAIRS_AUX,
     KC_NBREP, KC_REFINE, KC_BONDS, KC_ANGLES, KC_TORS, KC_BREP, KC_HB, KC_ATM, KC_BATM, KC_CHAIN, KC_REDUCE, KC_PCG,
-    KC_BATCH_SOLVE, KC_COUNT
+    KC_BATCH_SOLVE, KC_SOLV, KC_COUNT
 };
 const char* const kKernelClassNames[KC_COUNT] = {
     "cn",    "eeq_build", "chol_diag", "chol_panel", "chol_update", "trsv", "charges", "pairs", "pairs_aux", "nbrep", "refine",
     "bonds", "angles",    "torsions",  "brep",       "hb",          "atm",  "batm",    "chain", "reduce", "pcg",
-    "batch_solve"};
+    "batch_solve", "solvation"};
 
 // Start/stop event pairs around every launch of the selected class, recorded
 // on the stream the kernels run on.
@@ -4766,6 +5198,8 @@ struct DeviceEngine {
                                         // 3 factorisation by steps over the batch, 4 k_eeq_batch_fused,
                                         // 5 left-looking by column blocks over the batch
     bool batch_fp32 = false;            // mixed batches: FP32 factor with FP64 refinement
+    bool solv = false;                  // ALPB solvation (single molecules)
+    SolvArgs sa{};                      // its parameters and buffers
     int cg_groups = 0;                  // workgroups of the persistent PCG kernel
     double cg_tol2 = 0.0;               // squared relative residual at which the PCG solve stops
     double *cg_z = nullptr, *cg_p = nullptr;
@@ -5618,6 +6052,50 @@ DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt
         E->e_atm = E->alloc<double>(std::max(E->natm, 1));
         E->e_batm = E->alloc<double>(std::max(E->nbatm, 1));
         E->parts = E->alloc<double>(16);
+        if (P.solv.n > 0) {
+            if (P.nmol != 1 || P.solv.n != n) {
+                status = kErrUnsupported;
+                msg = "ALPB solvation is built for single molecules";
+                delete E;
+                return nullptr;
+            }
+            const AlpbSet& A = P.solv;
+            SolvArgs& a = E->sa;
+            E->solv = true;
+            a.n = n;
+            a.keps = A.keps;
+            a.alpbet = A.alpbet;
+            a.c1 = A.born_scale;
+            a.gshift = A.gshift;
+            a.lr2 = A.lrcut * A.lrcut;
+            a.sr2 = A.srcut * A.srcut;
+            a.x = E->x;
+            a.vdwr = E->upload(A.vdwr);
+            a.rho = E->upload(A.rho);
+            a.svdw = E->upload(A.svdw);
+            a.vdwsa = E->upload(A.vdwsa);
+            a.wrp = E->upload(A.wrp);
+            a.trjlo = E->upload(A.trj_lo);
+            a.trjhi = E->upload(A.trj_hi);
+            a.gam = E->upload(A.gamsasa);
+            a.hbmag = E->upload(A.hbmag);
+            if (A.nang != SOLV_NANG) throw std::runtime_error("ALPB grid size");
+            a.grid = E->upload(A.grid);
+            a.brad = E->alloc<double>(n);
+            a.dbr = E->alloc<double>(n);
+            a.sasa = E->alloc<double>(n);
+            a.hbw = E->alloc<double>(n);
+            a.sv = E->alloc<double>((size_t)n * SOLV_NB * 3);
+            a.sj = E->alloc<int>((size_t)n * SOLV_NB);
+            a.sn = E->alloc<int>(n);
+            a.shape = E->alloc<double>(32);
+            a.w = E->alloc<double>(n);
+            a.c = E->alloc<double>(n);
+            a.eat = E->alloc<double>(n);
+            a.gs = E->alloc<double>(3 * (size_t)n);
+            a.e_solv = E->alloc<double>(1);
+            a.parts = E->alloc<double>(4);
+        }
         E->rpart = E->alloc<double>(16 * RED_BLOCKS);
         E->ghb = E->alloc<double>(3 * (size_t)n);
         E->gslot = E->alloc<double>(3 * (size_t)n);
@@ -5626,6 +6104,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt
         E->nbl = E->alloc<int>((size_t)n * MAXNB_CN);
         E->flags = E->alloc<int>(1);
         E->sticky = E->alloc<int>(1);
+        E->sa.flags = E->flags;
         if (hipStreamCreate(&E->own) != hipSuccess || hipStreamCreate(&E->la.side) != hipSuccess ||
             hipEventCreateWithFlags(&E->la.fork, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&E->la.join, hipEventDisableTiming) != hipSuccess ||
@@ -5833,6 +6312,13 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     // side stream while this stream computes CN, the EEQ right-hand side and
     // every term that does not depend on the EEQ charges; the solve waits
     // for both (and runs alone, as the persistent solver requires).
+    if (E->solv) {
+        // Born radii, surface areas and the shape descriptor: geometry only,
+        // ahead of the fork so that the reduced matrix can take the Born part
+        LAUNCH(KC_SOLV, k_solv_born, dim3(cdiv(n, 4)), dim3(256), s, E->sa);
+        LAUNCH(KC_SOLV, k_solv_sasa, dim3(n), dim3(256), s, E->sa);
+        LAUNCH(KC_SOLV, k_solv_shape, dim3(1), dim3(256), s, E->sa);
+    }
     HIP_OK(hipEventRecord(E->ev_fork, s));
     HIP_OK(hipStreamWaitEvent(E->side, E->ev_fork, 0));
     // EEQ
@@ -5841,6 +6327,9 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         LAUNCH(KC_EEQ_BUILD, k_eeq_reduced, dim3(nbk * (nbk + 1) / 2), dim3(RED_T), E->side, E->nr, np,
                E->red_atom, E->red_last, E->red_lidx, E->red_ul, E->red_nul, E->x, E->alp, E->diag, E->Ared,
                E->use_cg ? 2 : 0);
+        if (E->solv)
+            LAUNCH(KC_SOLV, k_solv_red_add, dim3(nbk * (nbk + 1) / 2), dim3(256), E->side, E->nr, np, E->red_atom,
+                   E->red_last, E->sa, E->Ared, E->use_cg ? 2 : 0);
         if (E->use_cg) {
             // block-Jacobi factors of R's diagonal blocks and the solver's
             // mailboxes, also off the evaluation stream
@@ -5871,6 +6360,7 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     if (E->charged) {
         LAUNCH(KC_EEQ_BUILD, k_eeq_rhs_qp, dim3(cdiv(n, 256)), dim3(256), s, n, E->x, E->alp, E->diag, E->qp,
                            E->rhs, E->rres, E->mol_of, E->mol_off);
+        if (E->solv) LAUNCH(KC_SOLV, k_solv_rhs, dim3(cdiv(n, 256)), dim3(256), s, E->sa, E->qp, E->rres);
         rsrc = E->rres;
     }
     LAUNCH(KC_EEQ_BUILD, k_eeq_rhs_reduce, dim3(cdiv(np, 256)), dim3(256), s, E->nr, np, E->red_atom, E->red_last,
@@ -6059,11 +6549,17 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
     else
         LAUNCH(KC_CHAIN, k_finish<16>, dim3(cdiv(n, 16)), dim3(256), s, n, E->x, E->rcov_cn, E->dlog, E->vchain,
                        E->nbn, E->nbl, E->gslot, E->gpair, E->ghb, E->grad);
+    if (E->solv) {
+        LAUNCH(KC_SOLV, k_solv_pairs, dim3(cdiv(n, 4)), dim3(256), s, E->sa, E->q);
+        LAUNCH(KC_SOLV, k_solv_energy, dim3(1), dim3(256), s, E->sa, E->q);
+        LAUNCH(KC_SOLV, k_solv_grad, dim3(cdiv(n, 4)), dim3(256), s, E->sa, E->grad);
+    }
     Segs sg;
-    const double* ptrs[NPARTS] = {E->e_bond, E->e_ang,  E->e_tor, E->e_inv, E->e_disp, E->e_brep, E->e_rep, E->e_coul,
-                                  E->eself,  E->e_hb,   E->e_atm, E->e_batm, E->e_st,  E->e_xb,   E->e_hb2};
+    const double* ptrs[NPARTS] = {E->e_bond, E->e_ang,  E->e_tor, E->e_inv, E->e_disp, E->e_brep,
+                                  E->e_rep,  E->e_coul, E->eself, E->e_hb,  E->e_atm,  E->e_batm,
+                                  E->e_st,   E->e_xb,   E->e_hb2, E->solv ? E->sa.e_solv : E->e_bond};
     int cnts[NPARTS] = {E->nbond, E->nangle, E->ntors, E->ninv, n, E->nbrep, n, n, n, E->hb_ng0, E->natm, E->nbatm,
-                        E->nstors, E->nxb, E->nhb2};
+                        E->nstors, E->nxb, E->nhb2, E->solv ? 1 : 0};
     for (int k = 0; k < NPARTS; ++k) {
         sg.p[k] = ptrs[k];
         sg.n[k] = cnts[k];
@@ -6374,6 +6870,18 @@ int device_profile_read(DeviceEngine* E, double* total_ms, int* launches, std::s
         ++*launches;
     }
     T.used = 0;
+    return kOk;
+}
+
+int device_solvation_parts(DeviceEngine* E, double* parts, std::string& msg) {
+    if (!E->solv) {
+        msg = "no solvent";
+        return kErrArgs;
+    }
+    if (hipMemcpy(parts, E->sa.parts, 4 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+        msg = "download failed";
+        return kErrDevice;
+    }
     return kOk;
 }
 

@@ -45,6 +45,10 @@ int device_md_steps(DeviceEngine* e, double* x_dev, double* v_dev, double* grad_
 
 int device_count();
 
+// ALPB solvation energy parts of the last evaluation {gborn, ghb, gsasa,
+// gshift} (synchronises); kErrArgs when the engine has no solvent.
+int device_solvation_parts(DeviceEngine* e, double* parts, std::string& msg);
+
 // Status flags (FLAG_* bits) raised by any evaluation since the last cleared
 // read; synchronises the device.  clear resets the word.
 int device_status(DeviceEngine* e, int* flags, bool clear, std::string& msg);

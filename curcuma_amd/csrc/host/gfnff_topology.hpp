@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "alpb.hpp"
 #include "gfnff_params.h"
 
 namespace gfnff {
@@ -97,6 +98,9 @@ struct ParamSet {
     // batches of independent molecules (nmol > 1): atom offsets and, per term
     // list (bond, angle, torsion, bonded repulsion, HB, ATM, BATM, inversion,
     // triple-bond torsion, XB), the entry offsets of every molecule ([10][nmol + 1])
+    // ALPB solvation of a single molecule (n == 0: gas phase)
+    AlpbSet solv;
+
     int nmol = 1;
     std::vector<int32_t> mol_off, mol_seg;
     int mol_of_atom(int i) const;

@@ -187,8 +187,33 @@ class GFNFFMethod:
         return self._lib.gfnff_last_error(self._h).decode()
 
     def setParameters(self, parameters: dict) -> None:
+        """Engine keys (numbers) and the reference's "solvent" / "solvent_model"
+        (gfnff_method.cpp:811-835; GFN-FF always uses ALPB, so solvent_model
+        "gbsa" or "alpb" select the same model)."""
         for k, v in parameters.items():
-            self._lib.gfnff_set_param(self._h, k.encode(), float(v))
+            if k == "solvent":
+                self.setSolvent(v)
+            elif k == "solvent_model":
+                if str(v).lower() not in ("alpb", "gbsa", "gb", "2", "2.0"):
+                    raise ValueError(f"unknown solvent_model {v!r}")
+            else:
+                self._lib.gfnff_set_param(self._h, k.encode(), float(v))
+
+    def setSolvent(self, solvent: str | None) -> None:
+        """ALPB implicit solvation (one of the reference's GFN-FF solvents, or
+        None / "none" for the gas phase); raises on an unknown name, after
+        which the method is in the gas phase, as the reference continues."""
+        st = self._lib.gfnff_set_solvent(self._h, (solvent or "").encode())
+        if st != 0:
+            raise ValueError(self.getErrorMessage())
+
+    def solvationEnergy(self) -> dict:
+        """gborn, ghb, gsasa, gshift of the last evaluation (Hartree)."""
+        p = np.zeros(4)
+        st = self._lib.gfnff_solvation_energy(self._h, fptr(p))
+        if st != 0:
+            raise RuntimeError(self.getErrorMessage())
+        return {"gborn": float(p[0]), "ghb": float(p[1]), "gsasa": float(p[2]), "gshift": float(p[3])}
 
     def solverIterations(self) -> int:
         """Iterations of the last iterative EEQ solve (0 on the Cholesky path)."""
@@ -205,6 +230,13 @@ class GFNFFMethod:
     def params_view(self) -> _lib.ParamsView:
         v = _lib.ParamsView()
         st = self._lib.gfnff_get_params(self._h, C.byref(v))
+        if st != 0:
+            raise RuntimeError("no molecule set")
+        return v
+
+    def solvation_view(self) -> _lib.AlpbView:
+        v = _lib.AlpbView()
+        st = self._lib.gfnff_get_solvation(self._h, C.byref(v))
         if st != 0:
             raise RuntimeError("no molecule set")
         return v

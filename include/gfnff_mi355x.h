@@ -191,6 +191,28 @@ int gfnff_set_param(gfnff_handle* h, const char* key, double value);
  * refinement against the FP64 matrix; everything else FP64. */
 int gfnff_set_precision(gfnff_handle* h, int32_t mode);
 
+/* GFNFF's "solvent" / "solvent_model" parameters (gfnff_method.cpp:811-835,
+ * -gfnff.solvent NAME): ALPB implicit solvation, which GFN-FF always uses
+ * (P16 Born kernel; a request for GBSA means ALPB there too).  NAME is one of
+ * the reference's 25 GFN-FF solvents or their aliases (alpb_parameters.h:
+ * 1856-1891; case-insensitive); NULL, "" or "none" select the gas phase.
+ * Applies to the current molecule and to every molecule set afterwards; the
+ * energy then includes the solvation free energy (gsolv state) in
+ * GFNFF_E_SOLVATION, and the EEQ charges feel the reaction field
+ * (gfnff_method.cpp:1129-1140).  An unknown name returns GFNFF_STATUS_ARGS
+ * and leaves the handle in the gas phase, as the reference continues
+ * gas-phase after a failed initialisation (gfnff_method.cpp:826-829).
+ * Single molecules only: GFNFF_STATUS_UNSUPPORTED for batches. */
+int gfnff_set_solvent(gfnff_handle* h, const char* solvent);
+
+/* ALPBSolvation::getEnergyParts (alpb_solvation.cpp:1016-1041) of the last
+ * evaluation: parts[4] = {gborn, ghb, gsasa, gshift} in Hartree. */
+int gfnff_solvation_energy(gfnff_handle* h, double* parts);
+
+/* Engine extension: read-only view of the solvation parameters of the
+ * current molecule (natoms 0 in the gas phase). */
+int gfnff_get_solvation(const gfnff_handle* h, gfnff_alpb_view* out);
+
 /* ComputationalMethod::hasError() / getErrorMessage() (computational_method.h:190,201). */
 int gfnff_has_error(const gfnff_handle* h);
 const char* gfnff_last_error(const gfnff_handle* h);

@@ -125,6 +125,30 @@ typedef struct gfnff_params_view {
     const double* batm_par;     /* [3*nbatm] */
 } gfnff_params_view;
 
+/* ALPB implicit solvation of one molecule (units Bohr, Hartree): the
+   per-atom parameters of alpb_solvation.cpp:362-396 and the angular grid. */
+typedef struct gfnff_alpb_view {
+    int32_t natoms;             /* 0: gas phase */
+    int32_t hbond;              /* 1 when any atom has a H-bond strength */
+    int32_t nang;               /* angular grid points */
+    double keps;                /* (1/eps - 1) / (1 + alpbet) */
+    double alpbet;              /* 0.571412 / eps */
+    double born_scale;          /* c1 */
+    double gshift;              /* free-energy shift */
+    double lrcut;               /* Born-radius pair cutoff */
+    double srcut;               /* surface neighbour cutoff */
+    const double* vdwr;         /* D3 van der Waals radius */
+    const double* rho;          /* descreening radius vdwr * sx */
+    const double* svdw;         /* vdwr - Born offset */
+    const double* vdwsa;        /* vdwr + probe radius */
+    const double* wrp;          /* radial weight of the surface shell */
+    const double* trj_lo;       /* (vdwsa - w)^2 */
+    const double* trj_hi;       /* (vdwsa + w)^2 */
+    const double* gamsasa;      /* surface tension */
+    const double* hbmag;        /* H-bond strength (<= 0) */
+    const double* grid;         /* [4*nang] x, y, z, weight */
+} gfnff_alpb_view;
+
 /* Per-term energy slots, in the order of the reference's FFEnergyComponents
    (ff_methods/ff_workspace.h:49-95) restricted to GFN-FF terms. */
 enum gfnff_term {
@@ -141,7 +165,7 @@ enum gfnff_term {
     GFNFF_E_ATM,
     GFNFF_E_BATM,
     GFNFF_E_STORS,
-    GFNFF_E_RESERVED13,
+    GFNFF_E_SOLVATION,          /* ALPB solvation free energy (0 in the gas phase) */
     GFNFF_E_RESERVED14,
     GFNFF_NTERMS
 };

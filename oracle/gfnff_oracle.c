@@ -242,7 +242,10 @@ static int lu_solve(int m, double* M, double* b) {
     return 1;
 }
 
-static int oracle_eeq(const gfnff_params_view* P, const double* x, const double* cn, double* q, int nthr) {
+/* B: optional n x n reaction-field matrix added to A before the solve
+ * (eeq_solver.cpp:3774-3784); NULL in the gas phase. */
+static int oracle_eeq(const gfnff_params_view* P, const double* x, const double* cn, double* q, int nthr,
+                      const double* B) {
     int n = P->natoms, nf = P->nfrag;
     double* A = (double*)malloc(sizeof(double) * (size_t)n * n);
     double* rhs = (double*)malloc(sizeof(double) * n);
@@ -273,6 +276,8 @@ static int oracle_eeq(const gfnff_params_view* P, const double* x, const double*
         rhs[i] = P->chi_base[i] + P->cnf[i] * sqrt(cn[i]);
     }
     if (coincident) goto out;
+    if (B)
+        for (size_t k = 0; k < (size_t)n * n; ++k) A[k] += B[k];
     for (int i = 0; i < n; ++i) fo[P->frag[i] + 1]++;
     for (int f = 0; f < nf; ++f) fo[f + 1] += fo[f];
     {
@@ -303,6 +308,11 @@ static int oracle_eeq(const gfnff_params_view* P, const double* x, const double*
                 M[(size_t)i * m + j] = v;
                 M[(size_t)j * m + i] = v;
             }
+            if (B)
+                for (int j = 0; j <= i; ++j) {
+                    M[(size_t)i * m + j] += B[(size_t)i * n + j];
+                    if (j < i) M[(size_t)j * m + i] += B[(size_t)j * n + i];
+                }
             M[(size_t)i * m + n + P->frag[i]] = 1.0;
             M[(size_t)(n + P->frag[i]) * m + i] = 1.0;
             b[i] = P->chi_base[i] + P->cnf[i] * sqrt(cn[i]);
@@ -527,7 +537,7 @@ static int eval_impl(const gfnff_params_view* P, const double* x, int want_grad,
     oracle_gw(P, cn, gw, dgw);
     if (q_in) {
         memcpy(q, q_in, sizeof(double) * n);
-    } else if (!oracle_eeq(P, x, cn, q, nthr)) {
+    } else if (!oracle_eeq(P, x, cn, q, nthr, NULL)) {
         status = 3;
         goto done;
     }
@@ -1498,5 +1508,328 @@ out:
     free(cnt);
     free(last);
     free(qs);
+    return status;
+}
+
+/* ------------------------------------------------------------------------- */
+/* ALPB implicit solvation (alpb_solvation.cpp), serial FP64                  */
+/* ------------------------------------------------------------------------- */
+/* Constants of alpb_parameters.h:30-54. */
+#define ALPB_W (0.3 / 0.52917726)
+#define ALPB_AH0 0.5
+#define ALPB_AH1 (3.0 / (4.0 * ALPB_W))
+#define ALPB_AH3 (-1.0 / (4.0 * ALPB_W * ALPB_W * ALPB_W))
+#define ALPB_ZETA 1.028
+#define ALPB_TOLSESP 1.0e-6
+#define OBC_ALPHA 1.0
+#define OBC_BETA 0.8
+#define OBC_GAMMA 4.85
+
+typedef struct {
+    int n;
+    double* brad;  /* Born radii */
+    double* brdr;  /* [n][3n]: d brad_i / d x_k at brdr[i*3n + 3k + c] */
+    double* sasa;  /* surface area per atom (the grid-weighted switching) */
+    double* dsdr;  /* [n][3n]: d sasa_i / d x_k */
+    double* hbw;   /* H-bond diagonal hbmag sasa / vdwsa^2 */
+    double* B;     /* [n][n] Born matrix incl. self, H-bond and shape terms */
+    double adet, totrad3, center[3], inertia[9];
+} alpb_state;
+
+static void alpb_free(alpb_state* st) {
+    free(st->brad);
+    free(st->brdr);
+    free(st->sasa);
+    free(st->dsdr);
+    free(st->hbw);
+    free(st->B);
+}
+
+/* One descreening contribution to psi of atom s from atom o at distance r:
+ * the non-overlapping or overlapping integral of born.f90 as restated in
+ * alpb_solvation.cpp:530-589 (rho_o: descreening radius of o, rv: van der
+ * Waals radius of s).  *dg: (d g / d r) / r. */
+static double alpb_psi_term(double r, double rho_o, double rv, double* dg) {
+    if (r >= rv + rho_o) {
+        const double r1 = 1.0 / r, ap = r + rho_o, am = r - rho_o, ab = ap * am, rhab = rho_o / ab;
+        const double lnab = 0.5 * log(am / ap) * r1;
+        *dg = -2.0 * rhab / ab + (rhab - lnab) * r1 * r1;
+        return rhab + lnab;
+    }
+    if (r + rho_o <= rv) { /* o lies inside s */
+        *dg = 0.0;
+        return 0.0;
+    }
+    const double r1 = 1.0 / r, r12 = 0.5 * r1, r24 = r12 * r12, ap = r + rho_o, am = r - rho_o;
+    const double rh1 = 1.0 / rv, rhr1 = 1.0 / ap, aprh1 = ap * rh1, lnab = log(aprh1);
+    double d = rhr1 * rhr1 * (1.0 - 0.25 * am * r1 * (1.0 + aprh1 * aprh1)) + rho_o * r24 * (rhr1 - rh1 * aprh1) +
+               r12 * (r1 * lnab - rhr1);
+    *dg = d * r1;
+    return rh1 - rhr1 + r12 * (0.5 * am * (rhr1 - rh1 * aprh1) - lnab);
+}
+
+/* Born radii, surface areas, H-bond weights, shape descriptor and the Born
+ * matrix at geometry x: ALPBSolvation::update (alpb_solvation.cpp:410-448)
+ * with computePsi / computeBornRadii (:502-690), computeSASA (:696-801),
+ * computeHBCorrection (:873-881), computeADet (:908-938) and
+ * buildBornMatrix (:810-867, P16 kernel). */
+static int alpb_update(const gfnff_alpb_view* S, const double* x, alpb_state* st) {
+    const int n = S->natoms, n3 = 3 * n;
+    memset(st, 0, sizeof(*st));
+    st->n = n;
+    st->brad = calloc(n, 8);
+    st->brdr = calloc((size_t)n * n3, 8);
+    st->sasa = calloc(n, 8);
+    st->dsdr = calloc((size_t)n * n3, 8);
+    st->hbw = calloc(n, 8);
+    st->B = calloc((size_t)n * n, 8);
+    if (!st->brad || !st->brdr || !st->sasa || !st->dsdr || !st->hbw || !st->B) return 0;
+    const double lr2 = S->lrcut * S->lrcut, sr2 = S->srcut * S->srcut;
+    /* psi and its derivative, pairs i > j within lrcut */
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j) {
+            double d[3];
+            sub3(x + 3 * i, x + 3 * j, d);
+            const double r2 = dot3(d, d);
+            if (r2 >= lr2) continue;
+            const double r = sqrt(r2);
+            double dgi, dgj;
+            st->brad[i] += alpb_psi_term(r, S->rho[j], S->vdwr[i], &dgi);
+            st->brad[j] += alpb_psi_term(r, S->rho[i], S->vdwr[j], &dgj);
+            for (int c = 0; c < 3; ++c) {
+                /* s = i: v = dg (x_i - x_j); s = j: v = dg (x_j - x_i) */
+                st->brdr[(size_t)i * n3 + 3 * i + c] += dgi * d[c];
+                st->brdr[(size_t)i * n3 + 3 * j + c] -= dgi * d[c];
+                st->brdr[(size_t)j * n3 + 3 * j + c] -= dgj * d[c];
+                st->brdr[(size_t)j * n3 + 3 * i + c] += dgj * d[c];
+            }
+        }
+    /* OBC-II transform psi -> Born radius (born.f90:79-112) */
+    for (int i = 0; i < n; ++i) {
+        const double s1 = 1.0 / S->svdw[i], v1 = 1.0 / S->vdwr[i], s2 = 0.5 * S->svdw[i];
+        const double br = st->brad[i] * s2;
+        double arg2 = br * (OBC_GAMMA * br - OBC_BETA);
+        const double arg = br * (OBC_ALPHA + arg2);
+        arg2 = 2.0 * arg2 + OBC_ALPHA + OBC_GAMMA * br * br;
+        const double th = tanh(arg), ch = cosh(arg);
+        st->brad[i] = S->born_scale / (s1 - v1 * th);
+        double dp = ch * (s1 - v1 * th);
+        dp = S->born_scale * s2 * v1 * arg2 / (dp * dp);
+        for (int k = 0; k < n3; ++k) st->brdr[(size_t)i * n3 + k] *= dp;
+    }
+    /* surface areas: grid points on each atom's probe sphere, switched by
+       every neighbour within srcut */
+    int* idx = (int*)malloc(sizeof(int) * (n > 0 ? n : 1));
+    double* gv = (double*)malloc(sizeof(double) * 3 * (n > 0 ? n : 1));
+    if (!idx || !gv) {
+        free(idx);
+        free(gv);
+        return 0;
+    }
+    for (int i = 0; i < n; ++i) {
+        for (int p = 0; p < S->nang; ++p) {
+            const double* gp = S->grid + 4 * p;
+            double xp[3];
+            for (int c = 0; c < 3; ++c) xp[c] = x[3 * i + c] + S->vdwsa[i] * gp[c];
+            double sasap = 1.0;
+            int nni = 0;
+            for (int j = 0; j < n && sasap != 0.0; ++j) {
+                if (j == i) continue;
+                double dij[3];
+                sub3(x + 3 * i, x + 3 * j, dij);
+                if (!(dot3(dij, dij) < lr2 && dot3(dij, dij) < sr2)) continue;
+                double t[3];
+                sub3(xp, x + 3 * j, t);
+                const double t2 = dot3(t, t);
+                if (t2 >= S->trj_hi[j]) continue;
+                if (t2 <= S->trj_lo[j]) {
+                    sasap = 0.0;
+                    break;
+                }
+                const double sq = sqrt(t2), uj = sq - S->vdwsa[j], ah3u2 = ALPB_AH3 * uj * uj;
+                const double f = ALPB_AH0 + (ALPB_AH1 + ah3u2) * uj;
+                const double df = (ALPB_AH1 + 3.0 * ah3u2) / (f * sq);
+                sasap *= f;
+                idx[nni] = j;
+                for (int c = 0; c < 3; ++c) gv[3 * nni + c] = df * t[c];
+                nni++;
+            }
+            if (sasap > ALPB_TOLSESP) {
+                const double wsa = gp[3] * S->wrp[i] * sasap;
+                st->sasa[i] += wsa;
+                for (int k = 0; k < nni; ++k)
+                    for (int c = 0; c < 3; ++c) {
+                        st->dsdr[(size_t)i * n3 + 3 * i + c] += wsa * gv[3 * k + c];
+                        st->dsdr[(size_t)i * n3 + 3 * idx[k] + c] -= wsa * gv[3 * k + c];
+                    }
+            }
+        }
+        if (S->hbond) st->hbw[i] = S->hbmag[i] * st->sasa[i] / (S->vdwsa[i] * S->vdwsa[i]);
+    }
+    free(idx);
+    free(gv);
+    /* shape descriptor: inertia of spheres of the van der Waals radii */
+    {
+        const double tof = 0.4;
+        double c[3] = {0, 0, 0}, tr = 0.0, I[9] = {0};
+        for (int i = 0; i < n; ++i) {
+            const double r3 = S->vdwr[i] * S->vdwr[i] * S->vdwr[i];
+            tr += r3;
+            for (int k = 0; k < 3; ++k) c[k] += x[3 * i + k] * r3;
+        }
+        for (int k = 0; k < 3; ++k) c[k] /= tr;
+        for (int i = 0; i < n; ++i) {
+            const double r2v = S->vdwr[i] * S->vdwr[i], r3 = r2v * S->vdwr[i];
+            double v[3];
+            sub3(x + 3 * i, c, v);
+            const double vv = dot3(v, v);
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) I[3 * a + b] += r3 * ((a == b ? vv + tof * r2v : 0.0) - v[a] * v[b]);
+        }
+        const double det = I[0] * (I[4] * I[8] - I[5] * I[7]) - I[1] * (I[3] * I[8] - I[5] * I[6]) +
+                           I[2] * (I[3] * I[7] - I[4] * I[6]);
+        st->adet = sqrt(cbrt(det) / (tof * tr));
+        st->totrad3 = tr;
+        memcpy(st->center, c, sizeof(c));
+        memcpy(st->inertia, I, sizeof(I));
+    }
+    /* Born matrix, P16 kernel over all pairs */
+    const double shape = S->alpbet > 0.0 ? S->keps * S->alpbet / st->adet : 0.0;
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < i; ++j) {
+            double d[3];
+            sub3(x + 3 * i, x + 3 * j, d);
+            const double r = norm3(d), ab = sqrt(st->brad[i] * st->brad[j]);
+            double a = ab / (ab + ALPB_ZETA / 16.0 * r);
+            a = a * a;
+            a = a * a;
+            a = a * a;
+            a = a * a;
+            const double v = S->keps / (r + ab * a) + shape;
+            st->B[(size_t)i * n + j] = v;
+            st->B[(size_t)j * n + i] = v;
+        }
+        st->B[(size_t)i * n + i] = S->keps / st->brad[i] + 2.0 * st->hbw[i] + shape;
+    }
+    return 1;
+}
+
+/* Solvation energy at charges q (ALPBSolvation::getEnergyParts,
+ * alpb_solvation.cpp:1016-1041): parts = {gborn, ghb, gsasa, gshift}; and,
+ * when grad != NULL, its gradient at fixed q added to grad
+ * (ALPBSolvation::addGradient, :1139-1201 and :1265-1300). */
+static double alpb_energy_grad(const gfnff_alpb_view* S, const double* x, const alpb_state* st, const double* q,
+                               double* parts, double* grad) {
+    const int n = S->natoms, n3 = 3 * n;
+    double eb = 0.0, ehb = 0.0, es = 0.0, qt = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) s += st->B[(size_t)i * n + j] * q[j];
+        eb += 0.5 * q[i] * s;
+        ehb += st->hbw[i] * q[i] * q[i];
+        es += S->gamsasa[i] * st->sasa[i];
+        qt += q[i];
+    }
+    if (parts) {
+        parts[0] = eb - ehb;
+        parts[1] = ehb;
+        parts[2] = es;
+        parts[3] = S->gshift;
+    }
+    if (grad) {
+        double* dEdbr = calloc(n, 8);
+        if (!dEdbr) return NAN;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j) {
+                double d[3];
+                sub3(x + 3 * i, x + 3 * j, d);
+                const double r = norm3(d), ab = sqrt(st->brad[i] * st->brad[j]), qq = q[i] * q[j];
+                const double a1 = ab / (ab + ALPB_ZETA / 16.0 * r);
+                double a16 = a1 * a1;
+                a16 = a16 * a16;
+                a16 = a16 * a16;
+                a16 = a16 * a16;
+                const double fgb = r + ab * a16, df = 1.0 / fgb, df2 = df * df;
+                const double ap = (1.0 - ALPB_ZETA * a1 * a16) * df2;
+                for (int c = 0; c < 3; ++c) {
+                    const double g = ap * d[c] * S->keps / r * qq;
+                    grad[3 * i + c] -= g;
+                    grad[3 * j + c] += g;
+                }
+                const double bp = -0.5 * (r * ALPB_ZETA / ab * a1 + 1.0) / ab * a16 * df2;
+                dEdbr[i] += st->brad[j] * bp * S->keps * qq;
+                dEdbr[j] += st->brad[i] * bp * S->keps * qq;
+            }
+        for (int i = 0; i < n; ++i) dEdbr[i] -= 0.5 * S->keps * q[i] * q[i] / (st->brad[i] * st->brad[i]);
+        for (int i = 0; i < n; ++i) {
+            /* surface tension and the H-bond term's surface dependence */
+            const double w = S->gamsasa[i] + (S->hbond ? S->hbmag[i] / (S->vdwsa[i] * S->vdwsa[i]) * q[i] * q[i] : 0.0);
+            for (int k = 0; k < n3; ++k)
+                grad[k] += st->brdr[(size_t)i * n3 + k] * dEdbr[i] + st->dsdr[(size_t)i * n3 + k] * w;
+        }
+        free(dEdbr);
+        if (S->alpbet > 0.0) {
+            /* shape term 0.5 keps alpbet qt^2 / adet (gbsa.f90:722-789) */
+            const double* I = st->inertia;
+            double D[9];
+            D[0] = I[0] * (I[4] + I[8]) - I[1] * I[1] - I[2] * I[2];
+            D[1] = I[1] * I[8] - I[2] * I[5];
+            D[2] = I[2] * I[4] - I[1] * I[7];
+            D[3] = D[1];
+            D[4] = I[4] * (I[0] + I[8]) - I[1] * I[1] - I[5] * I[5];
+            D[5] = I[0] * I[5] - I[1] * I[2];
+            D[6] = D[2];
+            D[7] = D[5];
+            D[8] = I[8] * (I[0] + I[4]) - I[2] * I[2] - I[5] * I[5];
+            const double tr = st->totrad3, ad = st->adet;
+            const double sc = (250.0 / (48.0 * tr * tr * tr * pow(ad, 5.0))) *
+                              (-0.5 * S->keps * S->alpbet * qt * qt / (ad * ad));
+            for (int i = 0; i < n; ++i) {
+                const double r3 = S->vdwr[i] * S->vdwr[i] * S->vdwr[i];
+                double v[3];
+                sub3(x + 3 * i, st->center, v);
+                for (int a = 0; a < 3; ++a) grad[3 * i + a] += r3 * sc * (D[3 * a] * v[0] + D[3 * a + 1] * v[1] + D[3 * a + 2] * v[2]);
+            }
+        }
+    }
+    return eb + es + S->gshift;
+}
+
+/* GFN-FF with ALPB solvation at geometry x (gfnff_method.cpp:1129-1140 and
+ * 1805-1854): the Phase-2 EEQ solved with the Born matrix added to its
+ * Coulomb matrix, the gas-phase terms at those charges, plus the solvation
+ * energy; the gradient is the gas-phase gradient at those charges plus the
+ * solvation gradient at fixed charges.  terms_out[GFNFF_E_SOLVATION] holds
+ * the solvation energy; parts (4, may be NULL) = {gborn, ghb, gsasa, gshift}. */
+int gfnff_oracle_eval_alpb(const gfnff_params_view* P, const gfnff_alpb_view* S, const double* x, int want_grad,
+                           int nthr, double* E_total, double* terms_out, double* grad, double* q_out,
+                           double* parts) {
+    const int n = P->natoms;
+    if (S->natoms != n) return 1;
+    alpb_state st;
+    memset(&st, 0, sizeof(st));
+    double *cn_raw = calloc(n, 8), *cn = calloc(n, 8), *dlog = calloc(n, 8), *q = calloc(n, 8);
+    double terms[GFNFF_NTERMS];
+    int status = 1;
+    if (!cn_raw || !cn || !dlog || !q || !alpb_update(S, x, &st)) goto out;
+    oracle_cn(P, x, 6.0, cn_raw, cn, dlog, nthr);
+    if (!oracle_eeq(P, x, cn, q, nthr, st.B)) {
+        status = 3;
+        goto out;
+    }
+    status = eval_impl(P, x, want_grad, nthr, q, E_total, terms, grad, q_out, NULL, NULL);
+    if (status) goto out;
+    {
+        const double es = alpb_energy_grad(S, x, &st, q, parts, (want_grad && grad) ? grad : NULL);
+        terms[GFNFF_E_SOLVATION] = es;
+        *E_total += es;
+    }
+    if (terms_out) memcpy(terms_out, terms, sizeof(terms));
+out:
+    alpb_free(&st);
+    free(cn_raw);
+    free(cn);
+    free(dlog);
+    free(q);
     return status;
 }

@@ -82,6 +82,26 @@ class Oracle:
         return {"E": E.value, "terms": terms, "grad": g.reshape(n, 3), "q": q, "cn": cn}
 
 
+    def eval_alpb(self, view: ParamsView, sview, xyz_bohr: np.ndarray, grad: bool = True, nthreads: int = 1):
+        """GFN-FF with ALPB solvation (sview: the engine's AlpbView)."""
+        from curcuma_amd._lib import AlpbView
+        lib = self.lib
+        lib.gfnff_oracle_eval_alpb.argtypes = [C.POINTER(ParamsView), C.POINTER(AlpbView), C.c_void_p, C.c_int,
+                                               C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        n = view.natoms
+        x = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1)
+        E = C.c_double()
+        terms = np.zeros(NTERMS)
+        g = np.zeros(3 * n)
+        q = np.zeros(n)
+        parts = np.zeros(4)
+        st = lib.gfnff_oracle_eval_alpb(C.byref(view), C.byref(sview), x.ctypes.data, 1 if grad else 0,
+                                        int(nthreads), C.byref(E), terms.ctypes.data, g.ctypes.data, q.ctypes.data,
+                                        parts.ctypes.data)
+        if st != 0:
+            raise RuntimeError(f"oracle status {st}")
+        return {"E": E.value, "terms": terms, "grad": g.reshape(n, 3), "q": q, "parts": parts}
+
     def eeq_residual(self, view: ParamsView, xyz_bohr: np.ndarray, cn: np.ndarray, q: np.ndarray, nthreads: int = 1):
         """(|r|/|b| of the constrained EEQ system at charges q, max fragment
         charge-sum error); cn: the evaluation's log-capped CN."""
