@@ -12,12 +12,16 @@
  *   EEQ Phase 2      eeq_solver.cpp:3201-4006 (+ Schur solve :2087-2240)
  *   term kernels     ff_workspace_gfnff.cpp:46-1739
  *   self energy and CN chain rule   ff_workspace.cpp:415-536
+ *   ALPB solvation   alpb_solvation.cpp:410-1300, eeq_solver.cpp:3774-3784
  * Units: Bohr, Hartree.  Every function cites the reference lines it follows.
  *
  * Parity status per function (see tests/test_oracle_fixtures.py): pinned by the
  * reference's Fortran-derived fixtures (test_cases/reference_data/<name>.ref.json)
  * for total and per-term energies, charges and per-term gradients; gradients
  * additionally by central finite differences of this file's own energy.
+ * ALPB: total energies pinned by the reference's committed xtb GFN-FF+ALPB
+ * energies (tests/test_alpb.py), the angular grid by exact sphere moments,
+ * the solvation gradient by central finite differences.
  */
 #include <math.h>
 #include <stdlib.h>
