@@ -3300,13 +3300,16 @@ __global__ __launch_bounds__(256) void k_hb_role(HBRoles r, HBArgs a, const doub
 // the triplet's slots s0 (A), s0 + 1 (H), s0 + 2 (B), s0 + 3 + k (k-th
 // neighbour of B), then the carbonyl's D atoms.  MODE 0: every entry by the
 // FP64 core; MODE 1: case-2 entries only, by hb_core2<T> (the mixed path's
-// FP32 terms); MODE 2: entries of cases 3 and 4 only, FP64.
+// FP32 terms); MODE 2: entries of cases 3 and 4 only, FP64.  ord (MODE 1
+// and 2): the entries of the mode, case 2 first, so that no lane idles on
+// an entry of the other mode; nh counts the mode's entries.
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_hb_full(int nh, const int4* __restrict__ ahbc, const int* __restrict__ slot0,
-                                                 HBArgs a, const double* __restrict__ x, double* __restrict__ slots,
-                                                 double* __restrict__ e_out) {
-    const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= nh) return;
+__global__ __launch_bounds__(256, 2) void k_hb_full(int nh, const int* __restrict__ ord, const int4* __restrict__ ahbc,
+                                                 const int* __restrict__ slot0, HBArgs a, const double* __restrict__ x,
+                                                 double* __restrict__ slots, double* __restrict__ e_out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nh) return;
+    const int h = MODE == 0 ? k : ord[k];
     const int4 t = ahbc[h];
     const int s0 = slot0[h];
     double3 ga, gh, gb;
@@ -4000,7 +4003,7 @@ __global__ __launch_bounds__(256) void k_red_u(int nv, const int* __restrict__ r
 }
 
 // tiles {m, i, one-fragment flag, 0} of column block j
-__global__ __launch_bounds__(256, 3) void k_lstep_tiles(const int4* __restrict__ tl, int ntl, int j,
+__global__ __launch_bounds__(256, 2) void k_lstep_tiles(const int4* __restrict__ tl, int ntl, int j,
                                                      const int* __restrict__ mnr, const int* __restrict__ mnp,
                                                      const long long* __restrict__ mo, const int* __restrict__ vo,
                                                      const int* __restrict__ ra, const int* __restrict__ rl,
@@ -5258,6 +5261,7 @@ struct DeviceEngine {
     int2* hb_ent;
     int4* hb_ahbc;
     int hb_ng0, hb_ng1, hb_ngt, nhb2, nhb2_c2;  // nhb2_c2: case-2 entries of the cases 2-4 list
+    int* hb_ord = nullptr;                      // entries of the cases 2-4 list, case 2 first
     double *hb_cb, *hb_ca, *hb_Qh, *hb_Qab, *hb_gout, *hb_xslots, *e_hb2;
     float4* hb_pk;
     float* hb_qh32;
@@ -5793,7 +5797,13 @@ DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt
             }
             E->nhb2 = (int)ahbc.size();
             E->nhb2_c2 = 0;
-            for (const int4& t : ahbc) E->nhb2_c2 += (t.w == 2);
+            std::vector<int> ord;
+            for (size_t h = 0; h < ahbc.size(); ++h)
+                if (ahbc[h].w == 2) ord.push_back((int)h);
+            E->nhb2_c2 = (int)ord.size();
+            for (size_t h = 0; h < ahbc.size(); ++h)
+                if (ahbc[h].w != 2) ord.push_back((int)h);
+            E->hb_ord = E->upload(ord);
             std::vector<int> xoff(n + 1, 0), xidx(xatom.size());
             for (int at : xatom) xoff[at + 1]++;
             for (int i = 0; i < n; ++i) xoff[i + 1] += xoff[i];
@@ -6442,16 +6452,16 @@ static int launch_eval(DeviceEngine* E, const double* xin, hipStream_t s, std::s
         }
         // cases 2-4: on the mixed path the case-2 entries by the lean FP32
         // kernel and those of cases 3 and 4 by the FP64 one
-        const dim3 g2(cdiv(E->nhb2, 256));
+        const int n34 = E->nhb2 - E->nhb2_c2;
         if (E->nhb2 && !E->mixed)
-            LAUNCH(KC_HB, (k_hb_full<double, 0>), g2, dim3(256), hs, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
-                   E->hb_xslots, E->e_hb2);
+            LAUNCH(KC_HB, (k_hb_full<double, 0>), dim3(cdiv(E->nhb2, 256)), dim3(256), hs, E->nhb2, E->hb_ord,
+                   E->hb_ahbc, E->hb_s0, ha, E->x, E->hb_xslots, E->e_hb2);
         if (E->mixed && E->nhb2_c2)
-            LAUNCH(KC_HB, (k_hb_full<float, 1>), g2, dim3(256), hs, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
-                   E->hb_xslots, E->e_hb2);
-        if (E->mixed && E->nhb2 > E->nhb2_c2)
-            LAUNCH(KC_HB, (k_hb_full<double, 2>), g2, dim3(256), hs, E->nhb2, E->hb_ahbc, E->hb_s0, ha, E->x,
-                   E->hb_xslots, E->e_hb2);
+            LAUNCH(KC_HB, (k_hb_full<float, 1>), dim3(cdiv(E->nhb2_c2, 256)), dim3(256), hs, E->nhb2_c2, E->hb_ord,
+                   E->hb_ahbc, E->hb_s0, ha, E->x, E->hb_xslots, E->e_hb2);
+        if (E->mixed && n34 > 0)
+            LAUNCH(KC_HB, (k_hb_full<double, 2>), dim3(cdiv(n34, 256)), dim3(256), hs, n34, E->hb_ord + E->nhb2_c2,
+                   E->hb_ahbc, E->hb_s0, ha, E->x, E->hb_xslots, E->e_hb2);
     }
     if (E->hbg_lpa == 64)
         LAUNCH(KC_HB, k_hb_gather<64>, dim3(cdiv(n, 4)), dim3(256), hs, n, E->hb_crange, E->hb_gout, E->hb_xoff,
