@@ -3826,14 +3826,75 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-__device__ __forceinline__ void fused_tile(double* __restrict__ Am, int np, int nr, int i, int j,
-                                           const int* __restrict__ rab, const int* __restrict__ rlb,
-                                           const double* __restrict__ x, const double* __restrict__ alp,
-                                           const double* __restrict__ diag, const double* su, double sll, bool uni) {
+// C = -(R - L(r0:r0+16XX, 0:K) L(j, 0:K)^T) over 16 XX rows from r0: element v
+// of block (xx, y) at row r0 + 16 xx + 4 v + lk, column 64 j + 16 y + li, read
+// in the MFMA accumulator layout after the wavefront's own entry stores;
+// the operands eight columns at a time, the next eight in flight.
+template <int XX>
+__device__ __forceinline__ void fused_product(double* __restrict__ Am, int np, int r0, int j) {
     using M = MfmaT<double>;
     const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
     const int K = j * NB;
     const double* Qj = Am + (size_t)(j * NB) * np;
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    double* pt = Am + (size_t)(r0 + lk) * np + j * NB + li;  // this lane's element (0, 0, 0)
+    dbl4 acc[XX][4];
+#pragma unroll
+    for (int xx = 0; xx < XX; ++xx)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[xx][y][v] = -pt[(size_t)(16 * xx + 4 * v) * np + 16 * y];
+    if (K > 0) {
+        const double* Pi = Am + (size_t)r0 * np;
+        dbl2 fa[XX], fb[4];
+#pragma unroll
+        for (int xx = 0; xx < XX; ++xx) fa[xx] = *(const dbl2*)(Pi + (size_t)(16 * xx + li) * np + 2 * lk);
+#pragma unroll
+        for (int y = 0; y < 4; ++y) fb[y] = *(const dbl2*)(Qj + (size_t)(16 * y + li) * np + 2 * lk);
+#pragma unroll 1
+        for (int kc = 0; kc < K; kc += 8) {
+            const bool more = kc + 8 < K;
+            dbl2 na[XX], nb[4];
+            if (more) {
+#pragma unroll
+                for (int xx = 0; xx < XX; ++xx)
+                    na[xx] = *(const dbl2*)(Pi + (size_t)(16 * xx + li) * np + kc + 8 + 2 * lk);
+#pragma unroll
+                for (int y = 0; y < 4; ++y) nb[y] = *(const dbl2*)(Qj + (size_t)(16 * y + li) * np + kc + 8 + 2 * lk);
+            }
+#pragma unroll
+            for (int st = 0; st < 2; ++st)
+#pragma unroll
+                for (int xx = 0; xx < XX; ++xx)
+#pragma unroll
+                    for (int y = 0; y < 4; ++y) acc[xx][y] = M::op(fa[xx][st], fb[y][st], acc[xx][y]);
+            if (more) {
+#pragma unroll
+                for (int xx = 0; xx < XX; ++xx) fa[xx] = na[xx];
+#pragma unroll
+                for (int y = 0; y < 4; ++y) fb[y] = nb[y];
+            }
+        }
+    }
+    // the element addresses are formed again here rather than kept in
+    // registers across the product (an empty asm makes pt opaque)
+    asm volatile("" : "+v"(pt));
+#pragma unroll
+    for (int xx = 0; xx < XX; ++xx)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) pt[(size_t)(16 * xx + 4 * v) * np + 16 * y] = -acc[xx][y][v];
+}
+
+template <bool WHOLE>
+__device__ __forceinline__ void fused_tile(double* __restrict__ Am, int np, int nr, int i, int j,
+                                           const int* __restrict__ rab, const int* __restrict__ rlb,
+                                           const double* __restrict__ x, const double* __restrict__ alp,
+                                           const double* __restrict__ diag, const double* su, double sll, bool uni) {
+    const int lane = threadIdx.x & 63;
     // this lane's column of the entry pass
     const int Cc = j * NB + lane;
     const bool cin = Cc < nr;
@@ -3881,60 +3942,12 @@ __device__ __forceinline__ void fused_tile(double* __restrict__ Am, int np, int 
                 Am[(size_t)R * np + Cc] = a;
             }
         }
-        // element v of block (xx, y): row r0 + 16 xx + 4 v + lk, column j 64 + 16 y + li,
-        // written above by lane 16 y + li and read after the wavefront's own stores
-        __builtin_amdgcn_s_waitcnt(0);
-        __builtin_amdgcn_wave_barrier();
-        double* pt = Am + (size_t)(r0 + lk) * np + j * NB + li;  // this lane's element (0, 0, 0)
-        dbl4 acc[2][4];
-#pragma unroll
-        for (int xx = 0; xx < 2; ++xx)
-#pragma unroll
-            for (int y = 0; y < 4; ++y)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) acc[xx][y][v] = -pt[(size_t)(16 * xx + 4 * v) * np + 16 * y];
-        if (K > 0) {
-            const double* Pi = Am + (size_t)r0 * np;
-            dbl2 fa[2], fb[4];
-#pragma unroll
-            for (int xx = 0; xx < 2; ++xx) fa[xx] = *(const dbl2*)(Pi + (size_t)(16 * xx + li) * np + 2 * lk);
-#pragma unroll
-            for (int y = 0; y < 4; ++y) fb[y] = *(const dbl2*)(Qj + (size_t)(16 * y + li) * np + 2 * lk);
-#pragma unroll 1
-            for (int kc = 0; kc < K; kc += 8) {
-                const bool more = kc + 8 < K;
-                dbl2 na[2], nb[4];
-                if (more) {
-#pragma unroll
-                    for (int xx = 0; xx < 2; ++xx)
-                        na[xx] = *(const dbl2*)(Pi + (size_t)(16 * xx + li) * np + kc + 8 + 2 * lk);
-#pragma unroll
-                    for (int y = 0; y < 4; ++y) nb[y] = *(const dbl2*)(Qj + (size_t)(16 * y + li) * np + kc + 8 + 2 * lk);
-                }
-#pragma unroll
-                for (int st = 0; st < 2; ++st)
-#pragma unroll
-                    for (int xx = 0; xx < 2; ++xx)
-#pragma unroll
-                        for (int y = 0; y < 4; ++y) acc[xx][y] = M::op(fa[xx][st], fb[y][st], acc[xx][y]);
-                if (more) {
-#pragma unroll
-                    for (int xx = 0; xx < 2; ++xx) fa[xx] = na[xx];
-#pragma unroll
-                    for (int y = 0; y < 4; ++y) fb[y] = nb[y];
-                }
-            }
-        }
-        // the element addresses are formed again here rather than kept in
-        // registers across the product (an empty asm makes pt opaque)
-        asm volatile("" : "+v"(pt));
-#pragma unroll
-        for (int xx = 0; xx < 2; ++xx)
-#pragma unroll
-            for (int y = 0; y < 4; ++y)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) pt[(size_t)(16 * xx + 4 * v) * np + 16 * y] = -acc[xx][y][v];
+        if (!WHOLE) fused_product<2>(Am, np, r0, j);
     }
+    // WHOLE: one product over the 64-row tile, so that the column's own row
+    // panel L(j, 0:K) is read once per tile (needs the registers of two
+    // wavefronts per SIMD)
+    if (WHOLE) fused_product<4>(Am, np, i * NB, j);
 }
 
 // One workgroup of four wavefronts per molecule (largest systems first),
@@ -3974,7 +3987,7 @@ __global__ __launch_bounds__(BSOLVE_T, MINB) void k_eeq_batch_fused(
     const double sll = uni ? eeq_a(L0, L0, x, alp, diag) : 0.0;
     __syncthreads();
     for (int j = 0; j < nb; ++j) {
-        for (int i = j + wave; i < nb; i += NW) fused_tile(Am, np, nr, i, j, rab, rlb, x, alp, diag, su, sll, uni);
+        for (int i = j + wave; i < nb; i += NW) fused_tile<false>(Am, np, nr, i, j, rab, rlb, x, alp, diag, su, sll, uni);
         __syncthreads();
         batch_diag_block<double>(np, j * NB, Am, Lm, flags, S0, col, rdiag);
         __syncthreads();
@@ -4016,7 +4029,7 @@ __global__ __launch_bounds__(256, 2) void k_lstep_tiles(const int4* __restrict__
     const int m = tt.x, base = vo[m];
     const bool uni = tt.z != 0;
     const double sll = uni ? eeq_a(rl[base], rl[base], x, alp, diag) : 0.0;
-    fused_tile(A + mo[m], mnp[m], mnr[m], tt.y, j, ra + base, rl + base, x, alp, diag, u + base, sll, uni);
+    fused_tile<true>(A + mo[m], mnp[m], mnr[m], tt.y, j, ra + base, rl + base, x, alp, diag, u + base, sll, uni);
 }
 
 // Previous per-molecule solve (whole workgroup per tile step), kept for
