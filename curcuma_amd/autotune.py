@@ -6,13 +6,14 @@ FP64.  The batch-solve variants, the PCG workgroup counts and the gather
 widths reorder no sum, so their results are bitwise the same; the two batched
 pair decompositions add each atom's pair terms in a different order, so
 theirs agree to rounding:
-  batch_solver   0..5     per-molecule Cholesky of batches: right-looking with
+  batch_solver   0..6     per-molecule Cholesky of batches: right-looking with
                           tile phases shared out over wavefronts in two
                           occupancy / operand-prefetch variants, one tile step
                           per workgroup, or one column block at a time over
                           the whole batch; left-looking with the matrix formed
                           inside the factorisation, per molecule or one
-                          column block at a time over the batch
+                          column block at a time over the batch (6: with the
+                          panel product inside the tiles below the diagonal)
   batch_fp32     0 / 1    mixed path: factor in FP64, or in FP32 on MFMA with
                           two FP64 refinement steps (charges agree with the
                           FP64 factor's to about 1e-12 e)
@@ -44,7 +45,7 @@ from pathlib import Path
 
 import numpy as np
 
-SPACE_BATCH = {"batch_solver": [0, 1, 2, 3, 4, 5], "batch_fp32": [0, 1], "batch_pairs": [0, 1], "pair_split": [0, 1],
+SPACE_BATCH = {"batch_solver": [0, 1, 2, 3, 4, 5, 6], "batch_fp32": [0, 1], "batch_pairs": [0, 1], "pair_split": [0, 1],
                "gather_lanes": [16, 64]}
 # the engine's values when a key is not set (gather_lanes 0: by mean list length)
 ENGINE_DEFAULTS = {"batch_solver": 5, "batch_fp32": 0, "batch_pairs": 0, "pair_split": 0, "gather_lanes": 0,

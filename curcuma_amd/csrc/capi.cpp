@@ -368,8 +368,8 @@ int gfnff_set_param(gfnff_handle* h, const char* key, double value) {
         h->opt.cholesky_lookahead = value != 0.0;
         h->dev.reset();
     } else if (k == "batch_solver") {
-        if (!(value >= 0.0 && value <= 5.0) || value != std::floor(value))
-            return fail(h, GFNFF_STATUS_ARGS, "batch_solver: 0 .. 5");
+        if (!(value >= 0.0 && value <= 6.0) || value != std::floor(value))
+            return fail(h, GFNFF_STATUS_ARGS, "batch_solver: 0 .. 6");
         h->opt.batch_solver = (int)value;
         h->dev.reset();
     } else if (k == "batch_fp32") {

@@ -3889,11 +3889,95 @@ __device__ __forceinline__ void fused_product(double* __restrict__ Am, int np, i
             for (int v = 0; v < 4; ++v) pt[(size_t)(16 * xx + 4 * v) * np + 16 * y] = -acc[xx][y][v];
 }
 
-template <bool WHOLE>
+// MODE 2 of fused_tile: the tile is formed transposed, D = C^T (rows the
+// column block's 64 rows cj, columns the tile's rows ri), so that the
+// accumulator is directly the B operand of the panel product
+//   L(i, j)^T = Linv(j) C^T,   i.e.  L(i, j) = C(i, j) Linv(j)^T,
+// the product of the panel launch of the other column orders, without the
+// round trip of C through memory.  Lj: the inverse of the diagonal block,
+// 64 x 64 row-major, zero above the diagonal.
+__device__ __forceinline__ void fused_product_panel(double* __restrict__ Am, int np, int r0, int j,
+                                                    const double* __restrict__ Lj) {
+    using M = MfmaT<double>;
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int K = j * NB;
+    const double* Qj = Am + (size_t)(j * NB) * np;
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    // element v of block (yy, xx): D(16 yy + 4 v + lk, 16 xx + li) = C(r0 + 16 xx + li, 64 j + 16 yy + 4 v + lk)
+    double* pr = Am + (size_t)(r0 + li) * np + j * NB + lk;
+    dbl4 acc[4][4];
+#pragma unroll
+    for (int yy = 0; yy < 4; ++yy)
+#pragma unroll
+        for (int xx = 0; xx < 4; ++xx)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[yy][xx][v] = -pr[(size_t)(16 * xx) * np + 16 * yy + 4 * v];
+    if (K > 0) {
+        const double* Pi = Am + (size_t)r0 * np;
+        dbl2 fa[4], fb[4];
+#pragma unroll
+        for (int xx = 0; xx < 4; ++xx) fa[xx] = *(const dbl2*)(Pi + (size_t)(16 * xx + li) * np + 2 * lk);
+#pragma unroll
+        for (int y = 0; y < 4; ++y) fb[y] = *(const dbl2*)(Qj + (size_t)(16 * y + li) * np + 2 * lk);
+#pragma unroll 1
+        for (int kc = 0; kc < K; kc += 8) {
+            const bool more = kc + 8 < K;
+            dbl2 na[4], nb[4];
+            if (more) {
+#pragma unroll
+                for (int xx = 0; xx < 4; ++xx)
+                    na[xx] = *(const dbl2*)(Pi + (size_t)(16 * xx + li) * np + kc + 8 + 2 * lk);
+#pragma unroll
+                for (int y = 0; y < 4; ++y) nb[y] = *(const dbl2*)(Qj + (size_t)(16 * y + li) * np + kc + 8 + 2 * lk);
+            }
+#pragma unroll
+            for (int st = 0; st < 2; ++st)
+#pragma unroll
+                for (int yy = 0; yy < 4; ++yy)
+#pragma unroll
+                    for (int xx = 0; xx < 4; ++xx) acc[yy][xx] = M::op(fb[yy][st], fa[xx][st], acc[yy][xx]);
+            if (more) {
+#pragma unroll
+                for (int xx = 0; xx < 4; ++xx) fa[xx] = na[xx];
+#pragma unroll
+                for (int y = 0; y < 4; ++y) fb[y] = nb[y];
+            }
+        }
+    }
+    // acc = -C^T.  Per column block xx of the tile's rows: out(mm) = sum over
+    // the 16 four-row chunks (yy, v) of Linv(16 mm + li, 16 yy + 4 v + lk)
+    // times chunk (yy, v) of acc, = -L(i, j)^T block (mm, xx)
+    asm volatile("" : "+v"(pr));
+#pragma unroll
+    for (int xx = 0; xx < 4; ++xx) {
+        dbl4 out[4];
+#pragma unroll
+        for (int mm = 0; mm < 4; ++mm) out[mm] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int yy = 0; yy < 4; ++yy)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int mm = 0; mm < 4; ++mm)
+                    out[mm] = M::op(Lj[(16 * mm + li) * NB + 16 * yy + 4 * v + lk], acc[yy][xx][v], out[mm]);
+#pragma unroll
+        for (int mm = 0; mm < 4; ++mm)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) pr[(size_t)(16 * xx) * np + 16 * mm + 4 * v] = -out[mm][v];
+    }
+}
+
+// MODE 0: product per 32-row half; 1: one product over the 64-row tile, so
+// that the column's own row panel L(j, 0:K) is read once per tile (needs the
+// registers of two wavefronts per SIMD); 2: as 1, transposed, followed by
+// the panel product with Lj (fused_product_panel)
+template <int MODE>
 __device__ __forceinline__ void fused_tile(double* __restrict__ Am, int np, int nr, int i, int j,
                                            const int* __restrict__ rab, const int* __restrict__ rlb,
                                            const double* __restrict__ x, const double* __restrict__ alp,
-                                           const double* __restrict__ diag, const double* su, double sll, bool uni) {
+                                           const double* __restrict__ diag, const double* su, double sll, bool uni,
+                                           const double* __restrict__ Lj = nullptr) {
     const int lane = threadIdx.x & 63;
     // this lane's column of the entry pass
     const int Cc = j * NB + lane;
@@ -3942,12 +4026,10 @@ __device__ __forceinline__ void fused_tile(double* __restrict__ Am, int np, int 
                 Am[(size_t)R * np + Cc] = a;
             }
         }
-        if (!WHOLE) fused_product<2>(Am, np, r0, j);
+        if (MODE == 0) fused_product<2>(Am, np, r0, j);
     }
-    // WHOLE: one product over the 64-row tile, so that the column's own row
-    // panel L(j, 0:K) is read once per tile (needs the registers of two
-    // wavefronts per SIMD)
-    if (WHOLE) fused_product<4>(Am, np, i * NB, j);
+    if (MODE == 1) fused_product<4>(Am, np, i * NB, j);
+    if (MODE == 2) fused_product_panel(Am, np, i * NB, j, Lj);
 }
 
 // One workgroup of four wavefronts per molecule (largest systems first),
@@ -3987,7 +4069,7 @@ __global__ __launch_bounds__(BSOLVE_T, MINB) void k_eeq_batch_fused(
     const double sll = uni ? eeq_a(L0, L0, x, alp, diag) : 0.0;
     __syncthreads();
     for (int j = 0; j < nb; ++j) {
-        for (int i = j + wave; i < nb; i += NW) fused_tile<false>(Am, np, nr, i, j, rab, rlb, x, alp, diag, su, sll, uni);
+        for (int i = j + wave; i < nb; i += NW) fused_tile<0>(Am, np, nr, i, j, rab, rlb, x, alp, diag, su, sll, uni);
         __syncthreads();
         batch_diag_block<double>(np, j * NB, Am, Lm, flags, S0, col, rdiag);
         __syncthreads();
@@ -4029,7 +4111,27 @@ __global__ __launch_bounds__(256, 2) void k_lstep_tiles(const int4* __restrict__
     const int m = tt.x, base = vo[m];
     const bool uni = tt.z != 0;
     const double sll = uni ? eeq_a(rl[base], rl[base], x, alp, diag) : 0.0;
-    fused_tile<true>(A + mo[m], mnp[m], mnr[m], tt.y, j, ra + base, rl + base, x, alp, diag, u + base, sll, uni);
+    fused_tile<1>(A + mo[m], mnp[m], mnr[m], tt.y, j, ra + base, rl + base, x, alp, diag, u + base, sll, uni);
+}
+
+// batch_solver 6: the tiles below the diagonal of column block j after its
+// diagonal block is factored: formed and multiplied by Linv(j) in one pass,
+// L(i, j) stored directly (fused_tile MODE 2)
+__global__ __launch_bounds__(256, 2) void k_lpanel_tiles(const int4* __restrict__ tl, int ntl, int j,
+                                                      const int* __restrict__ mnr, const int* __restrict__ mnp,
+                                                      const long long* __restrict__ mo, const int* __restrict__ vo,
+                                                      const int* __restrict__ ra, const int* __restrict__ rl,
+                                                      const double* __restrict__ x, const double* __restrict__ alp,
+                                                      const double* __restrict__ diag, const double* __restrict__ u,
+                                                      double* __restrict__ A, const double* __restrict__ Linv) {
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= ntl) return;  // wavefront-local work: no barrier follows
+    const int4 tt = tl[w];
+    const int m = tt.x, base = vo[m];
+    const bool uni = tt.z != 0;
+    const double sll = uni ? eeq_a(rl[base], rl[base], x, alp, diag) : 0.0;
+    fused_tile<2>(A + mo[m], mnp[m], mnr[m], tt.y, j, ra + base, rl + base, x, alp, diag, u + base, sll, uni,
+                  Linv + (size_t)base * NB + (size_t)j * NB * NB);
 }
 
 // Previous per-molecule solve (whole workgroup per tile step), kept for
@@ -5240,6 +5342,9 @@ struct DeviceEngine {
     // batch_solver 5: tiles {m, i, one-fragment flag, 0} of column block j at [ls_off[j], ls_off[j + 1])
     std::vector<int> ls_off;
     int4* ls_tiles = nullptr;
+    // batch_solver 6: diagonal tiles and the tiles below them, per column block
+    std::vector<int> l6d_off, l6o_off;
+    int4 *l6d_tiles = nullptr, *l6o_tiles = nullptr;
     double* ured = nullptr;             // u of every reduced row (k_red_u)
     double* e_mol = nullptr;
     double *adj_w = nullptr, *adj_u = nullptr, *adj_part = nullptr, *adj_bpart = nullptr, *adj_out = nullptr;
@@ -5545,6 +5650,17 @@ DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt
                     E->ls_off.push_back((int)lflat.size());
                 }
                 E->ls_tiles = E->upload(lflat);
+                // batch_solver 6: per column block the diagonal tiles, then the tiles below them
+                std::vector<int4> dl, ol;
+                E->l6d_off.assign(1, 0);
+                E->l6o_off.assign(1, 0);
+                for (int j = 0; j < maxnb; ++j) {
+                    for (int k = E->ls_off[j]; k < E->ls_off[j + 1]; ++k) (lflat[k].y == j ? dl : ol).push_back(lflat[k]);
+                    E->l6d_off.push_back((int)dl.size());
+                    E->l6o_off.push_back((int)ol.size());
+                }
+                E->l6d_tiles = E->upload(dl);
+                if (!ol.empty()) E->l6o_tiles = E->upload(ol);
             }
             E->mnr = E->upload(mnr);
             E->mnp = E->upload(mnp);
@@ -6284,6 +6400,24 @@ static void queue_batch_factor(DeviceEngine* E, hipStream_t s) {
             if (npn)
                 LAUNCH(KC_BATCH_SOLVE, (k_bstep_tiles<double, false>), dim3(cdiv(npn, 4)), dim3(256), s,
                        E->bs_ptiles + E->bs_poff[j], npn, j, E->mnp, E->mmo, E->mvo, E->Ared, E->Linv);
+        }
+    } else if (E->batch_solver == 6) {  // as 5, the panel product inside the tiles below the diagonal
+        LAUNCH(KC_BATCH_SOLVE, k_red_u, dim3(cdiv(E->np, 256)), dim3(256), s, E->np, E->red_atom, E->red_last, E->x,
+               E->alp, E->diag, E->ured);
+        const int ncol = (int)E->l6d_off.size() - 1;
+        for (int j = 0; j < ncol; ++j) {
+            const int ndt = E->l6d_off[j + 1] - E->l6d_off[j];
+            const int not_ = E->l6o_off[j + 1] - E->l6o_off[j];
+            const int nd = E->bs_doff[j + 1] - E->bs_doff[j];
+            LAUNCH(KC_BATCH_SOLVE, k_lstep_tiles, dim3(cdiv(ndt, 4)), dim3(256), s, E->l6d_tiles + E->l6d_off[j], ndt,
+                   j, E->mnr, E->mnp, E->mmo, E->mvo, E->red_atom, E->red_last, E->x, E->alp, E->diag, E->ured,
+                   E->Ared);
+            LAUNCH(KC_BATCH_SOLVE, k_bstep_diag<double>, dim3(nd), dim3(256), s, E->bs_dmols + E->bs_doff[j], j,
+                   E->mnp, E->mmo, E->mvo, E->Ared, E->Linv, E->flags);
+            if (not_)
+                LAUNCH(KC_BATCH_SOLVE, k_lpanel_tiles, dim3(cdiv(not_, 4)), dim3(256), s,
+                       E->l6o_tiles + E->l6o_off[j], not_, j, E->mnr, E->mnp, E->mmo, E->mvo, E->red_atom,
+                       E->red_last, E->x, E->alp, E->diag, E->ured, E->Ared, E->Linv);
         }
     } else if (E->mixed && E->batch_fp32) {
         if (E->batch_solver == 3)

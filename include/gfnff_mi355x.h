@@ -170,7 +170,9 @@ int gfnff_cn(gfnff_handle* h, double* cn);
  * factorisation (no separate matrix build; FP64 on both paths), 5 = the
  * same left-looking form advanced one column block at a time over the whole
  * batch (three launches per block: column tiles, diagonal blocks, panels;
- * the default);
+ * the default), 6 = as 5 with the panel product done inside the tiles below
+ * the diagonal, formed after the diagonal block (three launches per block:
+ * diagonal tiles, diagonal blocks, tiles below);
  * "batch_fp32" 1 = on the mixed path, batches factor in FP32 (FP32 MFMA)
  * and refine in FP64 (charges agree with the FP64 factor to about 1e-12 e;
  * batch_solver 0, 1 and 3); "pair_split" 1 = the charge-independent pair

@@ -50,5 +50,5 @@ def test_tuned_keys_accepted_by_the_library():
         for k, vals in space.items():
             for v in list(vals) + [autotune.ENGINE_DEFAULTS[k]]:
                 assert lib.gfnff_set_param(h, k.encode(), float(v)) == 0, (k, v)
-    assert lib.gfnff_set_param(h, b"batch_solver", 6.0) != 0
+    assert lib.gfnff_set_param(h, b"batch_solver", 7.0) != 0
     assert lib.gfnff_set_param(h, b"batch_fp32", 2.0) != 0

@@ -92,7 +92,7 @@ def test_mixed_chemistry_batch():
 
 
 @pytest.mark.parametrize("batch_fp32", [0, 1])
-@pytest.mark.parametrize("batch_solver", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("batch_solver", [0, 1, 2, 3, 4, 5, 6])
 def test_mixed_batch_charges_refined(batch_solver, batch_fp32):
     """With batch_fp32 = 1 the mixed path factors each molecule's reduced EEQ
     system in FP32 on MFMA and refines in FP64 against the FP64 matrix
@@ -125,7 +125,7 @@ def test_batch_solvers_agree(oracle):
     xyz = np.vstack([xyz, xw + np.array([200.0, 0.0, 0.0])])
     off = np.concatenate([off, [len(Z)]])
     res = {}
-    for bs in (0, 1, 2, 3, 4, 5):
+    for bs in (0, 1, 2, 3, 4, 5, 6):
         b = GFNFFBatch(precision="fp64", parameters={"batch_solver": bs})
         assert b.setBatch(off, Z, xyz), b.getErrorMessage()
         b.calculateEnergy(True)
@@ -138,7 +138,7 @@ def test_batch_solvers_agree(oracle):
             assert abs(res[5][1][0] - ref["E"]) <= 1e-9 * max(1.0, abs(ref["E"]))
             assert np.abs(res[5][0][a0:a1] - ref["q"]).max() <= 1e-10
     q0, e0, g0 = res[0]
-    for bs in (1, 2, 3, 4, 5):
+    for bs in (1, 2, 3, 4, 5, 6):
         q, e, g = res[bs]
         dq, de, dg = np.abs(q - q0).max(), np.abs(e - e0).max(), np.abs(g - g0).max()
         print(f"batch_solver {bs}: max|dq|={dq:.2e} max|dE|={de:.2e} max|dg|={dg:.2e}")
