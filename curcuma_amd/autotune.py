@@ -48,8 +48,8 @@ import numpy as np
 SPACE_BATCH = {"batch_solver": [0, 1, 2, 3, 4, 5, 6], "batch_fp32": [0, 1], "batch_pairs": [0, 1], "pair_split": [0, 1],
                "gather_lanes": [16, 64]}
 # the engine's values when a key is not set (gather_lanes 0: by mean list length)
-ENGINE_DEFAULTS = {"batch_solver": 5, "batch_fp32": 0, "batch_pairs": 0, "pair_split": 0, "gather_lanes": 0,
-                   "pcg_groups": 0}
+ENGINE_DEFAULTS = {"batch_solver": 5, "batch_fp32": 0, "batch_pairs": 0, "pair_split": 0, "batch_streams": 1,
+                   "gather_lanes": 0, "pcg_groups": 0}
 SPACE_SINGLE = {"pcg_groups": [0, 128, 192], "pair_split": [0, 1], "gather_lanes": [16, 64]}
 
 

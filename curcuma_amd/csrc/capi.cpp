@@ -380,6 +380,10 @@ int gfnff_set_param(gfnff_handle* h, const char* key, double value) {
         if (value != 0.0 && value != 1.0) return fail(h, GFNFF_STATUS_ARGS, "pair_split: 0 or 1");
         h->opt.pair_split = (int)value;
         h->dev.reset();
+    } else if (k == "batch_streams") {
+        if (value != 1.0 && value != 2.0) return fail(h, GFNFF_STATUS_ARGS, "batch_streams: 1 or 2");
+        h->opt.batch_streams = (int)value;
+        h->dev.reset();
     } else if (k == "batch_pairs") {
         if (value != 0.0 && value != 1.0) return fail(h, GFNFF_STATUS_ARGS, "batch_pairs: 0 or 1");
         h->opt.batch_pairs = (int)value;

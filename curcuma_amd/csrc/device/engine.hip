@@ -5305,6 +5305,14 @@ struct DeviceEngine {
     hipStream_t side = nullptr;         // builds the reduced EEQ matrix alongside the evaluation stream
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev_fact = nullptr;       // batches: the factorisations on the side stream are done
+    // batch_streams 2: the second half of the batch's factorisation on a third stream, forked from
+    // and joined back to the side stream; per half h the column lists of batch_solver 5
+    hipStream_t side2 = nullptr;
+    hipEvent_t ev_h2f = nullptr, ev_h2j = nullptr;
+    int batch_streams = 1;
+    std::vector<int> h2_doff[2], h2_poff[2], h2_loff[2];
+    int* h2_dmols[2] = {nullptr, nullptr};
+    int4 *h2_ptiles[2] = {nullptr, nullptr}, *h2_ltiles[2] = {nullptr, nullptr};
     double* ps_part = nullptr;          // its per-tile row and column partials
     double* ps_part2 = nullptr;         // pair_split: the partials of dispersion and repulsion
     bool pair_split = false;            // dispersion and repulsion before the EEQ solve, Coulomb after
@@ -5465,6 +5473,12 @@ struct DeviceEngine {
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (ev_fact) (void)hipEventDestroy(ev_fact);
+        if (side2) {
+            (void)hipStreamSynchronize(side2);
+            (void)hipStreamDestroy(side2);
+        }
+        if (ev_h2f) (void)hipEventDestroy(ev_h2f);
+        if (ev_h2j) (void)hipEventDestroy(ev_h2j);
         if (own) {
             (void)hipStreamSynchronize(own);
             (void)hipStreamDestroy(own);
@@ -5500,6 +5514,7 @@ DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt
     E->use_lookahead = lookahead;
     E->use_cg = solver == 0 && P.nmol == 1;
     E->batch_solver = opt.batch_solver;
+    E->batch_streams = (opt.batch_streams == 2 && opt.batch_solver == 5 && P.nmol > 1) ? 2 : 1;
     E->batch_fp32 = opt.batch_fp32 != 0;
     {
         const double tol = pcg_tol > 0.0 ? pcg_tol : (E->mixed ? PCG_TOL_MIXED : PCG_TOL_FP64);
@@ -5661,6 +5676,34 @@ DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt
                 }
                 E->l6d_tiles = E->upload(dl);
                 if (!ol.empty()) E->l6o_tiles = E->upload(ol);
+                // batch_streams 2: molecules dealt alternately, by decreasing size, to two halves
+                if (E->batch_streams == 2) {
+                    std::vector<int> ordm(P.nmol), half(P.nmol);
+                    for (int m = 0; m < P.nmol; ++m) ordm[m] = m;
+                    std::stable_sort(ordm.begin(), ordm.end(), [&](int a, int b) { return mnp[a] > mnp[b]; });
+                    for (int k = 0; k < P.nmol; ++k) half[ordm[k]] = k & 1;
+                    for (int h = 0; h < 2; ++h) {
+                        std::vector<int> d2;
+                        std::vector<int4> p2, l2;
+                        E->h2_doff[h].assign(1, 0);
+                        E->h2_poff[h].assign(1, 0);
+                        E->h2_loff[h].assign(1, 0);
+                        for (int j = 0; j < maxnb; ++j) {
+                            for (int m : dm[j])
+                                if (half[m] == h) d2.push_back(m);
+                            for (const int4& t : pt[j])
+                                if (half[t.x] == h) p2.push_back(t);
+                            for (int k = E->ls_off[j]; k < E->ls_off[j + 1]; ++k)
+                                if (half[lflat[k].x] == h) l2.push_back(lflat[k]);
+                            E->h2_doff[h].push_back((int)d2.size());
+                            E->h2_poff[h].push_back((int)p2.size());
+                            E->h2_loff[h].push_back((int)l2.size());
+                        }
+                        if (!d2.empty()) E->h2_dmols[h] = E->upload(d2);
+                        if (!p2.empty()) E->h2_ptiles[h] = E->upload(p2);
+                        if (!l2.empty()) E->h2_ltiles[h] = E->upload(l2);
+                    }
+                }
             }
             E->mnr = E->upload(mnr);
             E->mnp = E->upload(mnp);
@@ -6252,6 +6295,11 @@ DeviceEngine* device_create(const ParamSet& P, int precision, const Options& opt
             hipEventCreateWithFlags(&E->ev_join, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&E->ev_fact, hipEventDisableTiming) != hipSuccess)
             throw std::runtime_error("stream");
+        if (E->batch_streams == 2 &&
+            (hipStreamCreateWithFlags(&E->side2, hipStreamNonBlocking) != hipSuccess ||
+             hipEventCreateWithFlags(&E->ev_h2f, hipEventDisableTiming) != hipSuccess ||
+             hipEventCreateWithFlags(&E->ev_h2j, hipEventDisableTiming) != hipSuccess))
+            throw std::runtime_error("stream");
         hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, E->own, 1, E->flags, 0);
         hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, E->own, 1, E->sticky, 0);
         hipLaunchKernelGGL(k_eeq_diag, dim3(cdiv(n, 256)), dim3(256), 0, E->own, n, E->gam, E->alp, E->diag);
@@ -6385,6 +6433,31 @@ static void queue_batch_factor(DeviceEngine* E, hipStream_t s) {
     if (E->batch_solver == 4) {  // left-looking, matrix formed in place (FP64 on both paths)
         LAUNCH(KC_BATCH_SOLVE, k_eeq_batch_fused<3>, g, b, s, E->morder, E->mnr, E->mnp, E->mmo, E->mvo,
                E->red_atom, E->red_last, E->x, E->alp, E->diag, E->Ared, E->Linv, E->flags);
+    } else if (E->batch_solver == 5 && E->batch_streams == 2) {  // two halves of the batch on two streams
+        LAUNCH(KC_BATCH_SOLVE, k_red_u, dim3(cdiv(E->np, 256)), dim3(256), s, E->np, E->red_atom, E->red_last, E->x,
+               E->alp, E->diag, E->ured);
+        (void)hipEventRecord(E->ev_h2f, s);
+        (void)hipStreamWaitEvent(E->side2, E->ev_h2f, 0);
+        const int ncol = (int)E->h2_doff[0].size() - 1;
+        for (int j = 0; j < ncol; ++j)
+            for (int h = 0; h < 2; ++h) {
+                hipStream_t st = h ? E->side2 : s;
+                const int nt = E->h2_loff[h][j + 1] - E->h2_loff[h][j];
+                const int nd = E->h2_doff[h][j + 1] - E->h2_doff[h][j];
+                const int npn = E->h2_poff[h][j + 1] - E->h2_poff[h][j];
+                if (nt)
+                    LAUNCH(KC_BATCH_SOLVE, k_lstep_tiles, dim3(cdiv(nt, 4)), dim3(256), st,
+                           E->h2_ltiles[h] + E->h2_loff[h][j], nt, j, E->mnr, E->mnp, E->mmo, E->mvo, E->red_atom,
+                           E->red_last, E->x, E->alp, E->diag, E->ured, E->Ared);
+                if (nd)
+                    LAUNCH(KC_BATCH_SOLVE, k_bstep_diag<double>, dim3(nd), dim3(256), st,
+                           E->h2_dmols[h] + E->h2_doff[h][j], j, E->mnp, E->mmo, E->mvo, E->Ared, E->Linv, E->flags);
+                if (npn)
+                    LAUNCH(KC_BATCH_SOLVE, (k_bstep_tiles<double, false>), dim3(cdiv(npn, 4)), dim3(256), st,
+                           E->h2_ptiles[h] + E->h2_poff[h][j], npn, j, E->mnp, E->mmo, E->mvo, E->Ared, E->Linv);
+            }
+        (void)hipEventRecord(E->ev_h2j, E->side2);
+        (void)hipStreamWaitEvent(s, E->ev_h2j, 0);
     } else if (E->batch_solver == 5) {  // left-looking by column blocks over the batch (FP64 on both paths)
         LAUNCH(KC_BATCH_SOLVE, k_red_u, dim3(cdiv(E->np, 256)), dim3(256), s, E->np, E->red_atom, E->red_last, E->x,
                E->alp, E->diag, E->ured);
@@ -6800,6 +6873,12 @@ int device_graph_capture(DeviceEngine* E, const double* xyz_bohr_dev, double* en
     hipStream_t s = (hipStream_t)stream;
     if (!s) {
         msg = "graph capture needs a created stream (not the default stream)";
+        return kErrArgs;
+    }
+    if (E->batch_streams == 2) {
+        // its capture ended in a crash inside the runtime whose cause is not
+        // established: the two-stream factorisation runs eagerly only
+        msg = "graph capture is not available with batch_streams 2";
         return kErrArgs;
     }
     E->drop_graph();

@@ -48,6 +48,7 @@ struct Options {
     int gather_lanes = 0;             // lanes per atom of the slot and HB gathers: 0 by list length, 16 or 64
     int batch_pairs = 0;              // batches: 0 each pair once in 64x64 tiles per molecule, 1 row kernel
     int pair_split = 0;               // 1: dispersion + repulsion before the EEQ solve, Coulomb after (tiles)
+    int batch_streams = 1;            // batch_solver 5: 2 = the batch's factorisation in two halves on two streams
     int pcg_groups = 0;               // workgroups of the persistent PCG: 0 all that fit (at most 256), else at most this
 };
 

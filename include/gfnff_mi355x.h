@@ -173,6 +173,10 @@ int gfnff_cn(gfnff_handle* h, double* cn);
  * the default), 6 = as 5 with the panel product done inside the tiles below
  * the diagonal, formed after the diagonal block (three launches per block:
  * diagonal tiles, diagonal blocks, tiles below);
+ * "batch_streams" 2 = with batch_solver 5, the batch's factorisation in
+ * two halves (molecules dealt alternately by size) on two streams (1, one
+ * stream, is the default; 2 runs eagerly only: gfnff_graph_capture refuses
+ * it, and it is not faster on the mixed path);
  * "batch_fp32" 1 = on the mixed path, batches factor in FP32 (FP32 MFMA)
  * and refine in FP64 (charges agree with the FP64 factor to about 1e-12 e;
  * batch_solver 0, 1 and 3); "pair_split" 1 = the charge-independent pair

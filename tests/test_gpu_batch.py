@@ -174,3 +174,57 @@ def test_pair_split_agrees(precision):
         out[split] = (m.calculateEnergy(True), m.getGradient())
         assert not m.hasError(), m.getErrorMessage()
     assert abs(out[1][0] - out[0][0]) <= 1e-10 and np.abs(out[1][1] - out[0][1]).max() <= gtol
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_streams_identical(precision):
+    """batch_streams 2 (the factorisation in two halves on two streams) runs
+    the same arithmetic per molecule: charges, energies and gradient equal bit
+    for bit to the one-stream run (eager launches; graph capture is refused
+    for it)."""
+    off, Z, xyz = mixed_batch(nmol=31, nmin=20, nmax=400, seed=5)
+    res = {}
+    for bs in (1, 2):
+        b = GFNFFBatch(precision=precision, parameters={"batch_streams": bs})
+        assert b.setBatch(off, Z, xyz), b.getErrorMessage()
+        b.calculateEnergy(True)
+        assert not b.hasError(), b.getErrorMessage()
+        res[bs] = (b.getCharges(), b.getEnergies(), b.getGradient())
+    for k in range(3):
+        assert np.array_equal(res[1][k], res[2][k]), k
+
+
+@pytest.mark.skip(reason="graph capture with batch_streams 2 crashed inside the HIP runtime (host segfault in "
+                         "gfnff_graph_capture); the cause is not established, so the engine refuses that capture")
+def test_batch_streams_graph():
+    """Graph replays of the two-stream factorisation equal its eager run."""
+    import torch
+    off, Z, xyz = mixed_batch(nmol=31, nmin=20, nmax=400, seed=5)
+    b = GFNFFBatch(precision="mixed", parameters={"batch_streams": 2})
+    assert b.setBatch(off, Z, xyz), b.getErrorMessage()
+    b.calculateEnergy(True)
+    g0 = b.getGradient()
+    x = torch.tensor(np.asarray(xyz) / BOHR, dtype=torch.float64, device="cuda").reshape(-1)
+    out = torch.zeros(16, dtype=torch.float64, device="cuda")
+    g = torch.zeros_like(x)
+    st = torch.cuda.Stream()
+    b.captureGraph(x.data_ptr(), out.data_ptr(), g.data_ptr(), st.cuda_stream)
+    for _ in range(3):
+        b.launchGraph(st.cuda_stream)
+        st.synchronize()
+        assert np.array_equal(g.cpu().numpy().reshape(-1, 3), g0)
+        torch.cuda.synchronize()
+
+
+def test_batch_streams_capture_refused():
+    """The engine refuses the capture that crashed, with a message."""
+    import torch
+    off, Z, xyz = mixed_batch(nmol=4, nmin=20, nmax=100, seed=5)
+    b = GFNFFBatch(precision="mixed", parameters={"batch_streams": 2})
+    assert b.setBatch(off, Z, xyz), b.getErrorMessage()
+    x = torch.tensor(np.asarray(xyz) / BOHR, dtype=torch.float64, device="cuda").reshape(-1)
+    out = torch.zeros(16, dtype=torch.float64, device="cuda")
+    g = torch.zeros_like(x)
+    st = torch.cuda.Stream()
+    with pytest.raises(RuntimeError, match="batch_streams 2"):
+        b.captureGraph(x.data_ptr(), out.data_ptr(), g.data_ptr(), st.cuda_stream)
