@@ -387,8 +387,9 @@ def main():
     info["pcg_iters"] = pcg_iters
     if batch:
         frag = np.ctypeslib.as_array(pv.frag, shape=(n,))
+        # the engine factors in FP32 only in the forms that build the matrix first (0, 1, 3)
         fp32_solve = (args.precision == "mixed" and head.params.get("batch_fp32", 0) == 1
-                      and head.params.get("batch_solver", 0) != 2)
+                      and head.params.get("batch_solver", 5) in (0, 1, 3))
         info["batch_fp32"] = fp32_solve
         info["batch_refine_steps"] = 2 if fp32_solve else 0  # BSOLVE_NREF of the engine
         info["mol_off"] = atom_off
