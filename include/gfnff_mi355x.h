@@ -138,6 +138,7 @@ int gfnff_md_steps(gfnff_handle* h, double* xyz_bohr_dev, double* vel_dev, doubl
 #define GFNFF_FLAG_CN_OVERFLOW 2   /* more than 128 atoms within the 6 Bohr CN cut-off of one atom */
 #define GFNFF_FLAG_SOLVE 4         /* EEQ solve did not converge or a bounded wait timed out */
 #define GFNFF_FLAG_HB_STALE 8      /* gfnff_md_steps: moved past the HB/XB re-detection threshold (not a failure) */
+#define GFNFF_FLAG_SOLV_NEIGHBOURS 16 /* ALPB: more than 512 atoms within the surface cut-off of one atom */
 
 /* Engine extension (no counterpart in the reference, whose CPU evaluation
  * reports through hasError()): the GFNFF_FLAG_* bits raised by every
