@@ -3835,6 +3835,7 @@ __device__ __forceinline__ void fused_product(double* __restrict__ Am, int np, i
     using M = MfmaT<double>;
     const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
     const int K = j * NB;
+    if (K == 0) return;  // column block 0: C = R, already stored by the entry pass
     const double* Qj = Am + (size_t)(j * NB) * np;
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
