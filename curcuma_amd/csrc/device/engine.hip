@@ -3768,9 +3768,59 @@ __global__ __launch_bounds__(256, 3) void k_bstep_diag(const int* __restrict__ m
     chol_diag_block<T>(mnp[m], k * NB, A + mo[m], Linv + (size_t)vo[m] * NB, flags, S0, col, rdiag);
 }
 
+// FP64 panel tile over the whole 64-row tile in one product (the column
+// panel of the inverse read once per tile instead of once per half):
+// A(ri.., rj..) = P(ri.., 0..63) Q(0..63, 0..63)^T, P the tile itself (leading
+// dimension np), Q the inverse of the diagonal block (64 x 64, zero above the
+// diagonal); operands eight columns at a time, the next eight in flight.
+__device__ __forceinline__ void panel_tile_whole(double* __restrict__ Am, int np, int ri, int rj,
+                                                 const double* __restrict__ P, const double* __restrict__ Q) {
+    using M = MfmaT<double>;
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    dbl4 acc[4][4];
+#pragma unroll
+    for (int xx = 0; xx < 4; ++xx)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[xx][y] = dbl4{0.0, 0.0, 0.0, 0.0};
+    dbl2 fa[4], fb[4];
+#pragma unroll
+    for (int xx = 0; xx < 4; ++xx) fa[xx] = *(const dbl2*)(P + (size_t)(16 * xx + li) * np + 2 * lk);
+#pragma unroll
+    for (int y = 0; y < 4; ++y) fb[y] = *(const dbl2*)(Q + (16 * y + li) * NB + 2 * lk);
+#pragma unroll 1
+    for (int kc = 0; kc < NB; kc += 8) {
+        const bool more = kc + 8 < NB;
+        dbl2 na[4], nb[4];
+        if (more) {
+#pragma unroll
+            for (int xx = 0; xx < 4; ++xx) na[xx] = *(const dbl2*)(P + (size_t)(16 * xx + li) * np + kc + 8 + 2 * lk);
+#pragma unroll
+            for (int y = 0; y < 4; ++y) nb[y] = *(const dbl2*)(Q + (16 * y + li) * NB + kc + 8 + 2 * lk);
+        }
+#pragma unroll
+        for (int st = 0; st < 2; ++st)
+#pragma unroll
+            for (int xx = 0; xx < 4; ++xx)
+#pragma unroll
+                for (int y = 0; y < 4; ++y) acc[xx][y] = M::op(fa[xx][st], fb[y][st], acc[xx][y]);
+        if (more) {
+#pragma unroll
+            for (int xx = 0; xx < 4; ++xx) fa[xx] = na[xx];
+#pragma unroll
+            for (int y = 0; y < 4; ++y) fb[y] = nb[y];
+        }
+    }
+#pragma unroll
+    for (int xx = 0; xx < 4; ++xx)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Am[(size_t)(ri + 16 * xx + 4 * v + lk) * np + rj + 16 * y + li] = acc[xx][y][v];
+}
+
 // tiles {m, bi, bj}: panel tiles (SUB = false, bj unused) or trailing tiles of step k
 template <typename T, bool SUB>
-__global__ __launch_bounds__(256) void k_bstep_tiles(const int4* __restrict__ tl, int ntl, int k,
+__global__ __launch_bounds__(256, 2) void k_bstep_tiles(const int4* __restrict__ tl, int ntl, int k,
                                                      const int* __restrict__ mnp, const long long* __restrict__ mo,
                                                      const int* __restrict__ vo, T* __restrict__ A,
                                                      const T* __restrict__ Linv) {
@@ -3779,7 +3829,10 @@ __global__ __launch_bounds__(256) void k_bstep_tiles(const int4* __restrict__ tl
     const int4 tt = tl[w];
     const int m = tt.x, bi = tt.y, bj = tt.z, np = mnp[m], k0 = k * NB;
     T* Am = A + mo[m];
-    if (!SUB)
+    if constexpr (!SUB && sizeof(T) == 8)
+        panel_tile_whole(Am, np, bi * NB, k0, Am + (size_t)bi * NB * np + k0,
+                         Linv + (size_t)vo[m] * NB + (size_t)k * NB * NB);
+    else if (!SUB)
         batch_tile_mfma<T, false, 4>(Am, np, bi * NB, k0, Am + (size_t)bi * NB * np + k0,
                                      Linv + (size_t)vo[m] * NB + (size_t)k * NB * NB, NB);
     else
